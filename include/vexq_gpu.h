@@ -1,0 +1,268 @@
+/*
+ * vexq_gpu.h — C ABI of the MI355X (gfx950) execution library for vexq's
+ * data-parallel hot path: selection-vector Filter, zone-map-pruned TableScan,
+ * dictionary-code GROUP BY (HashAggregate / ParallelHashAggregate) and the
+ * radix-partitioned hash join feeding an aggregate (ParallelHashJoinAggregate).
+ *
+ * Plain C: no C++ types, no HIP types, no torch types cross this boundary.
+ * Every entry point returns an int status (VQ_OK == 0); on failure the
+ * calling thread's vq_last_error() names the cause.  Host pointers passed in
+ * are never retained after the call returns (the cgo pointer rule).
+ *
+ * Ownership: the library owns every device allocation and every handle it
+ * returns; the caller owns host output buffers and frees handles with the
+ * matching *_destroy call.  A vq_table is immutable after upload and may be
+ * shared by concurrent calls.  A vq_ctx may be used from several host threads;
+ * each call enqueues on the context's stream and returns when its result is on
+ * the host.
+ *
+ * Reference interfaces each group replaces are cited file:line in the vexq
+ * checkout (Go).
+ */
+#ifndef VEXQ_GPU_H
+#define VEXQ_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (wrapped Go-side as fmt.Errorf("exec: gpu ...: %w")) ---- */
+#define VQ_OK 0
+#define VQ_E_ARG 1         /* invalid argument (mirrors the exec: ... errors)   */
+#define VQ_E_DEVICE 2      /* HIP runtime / device failure                      */
+#define VQ_E_BUDGET 3      /* query exceeds the context's device-memory budget  */
+#define VQ_E_UNSUPPORTED 4 /* shape the GPU operators do not cover: fall back   */
+#define VQ_E_NOMEM 5       /* host allocation failure                           */
+
+/* Column physical types as they sit in HBM.  Mirrors storage type codes
+ * (storage/format.go:20-26) after the decode in exec/scan.go:294-409:
+ * dictionary STRING columns become global codes (u8/u16), DATE is int32 days
+ * since 1970-01-01, INT64 and FLOAT64 keep their width. */
+#define VQ_U8 1
+#define VQ_U16 2
+#define VQ_DATE 3
+#define VQ_I64 4
+#define VQ_F64 5
+
+/* Rows per row group and per batch (storage/format.go:8-10). */
+#define VQ_ROW_GROUP_ROWS 65536
+#define VQ_BLOCK_ROWS 1024
+
+/* Per-row-group, per-column zone map, raw bits exactly as storage.ZoneMap
+ * (storage/schema.go:26-35) and storage/writer.go:205-348 compute them. */
+typedef struct vq_zone {
+  uint64_t min_bits;
+  uint64_t max_bits;
+  int64_t null_count;
+  int32_t has_minmax;
+  int32_t pad_;
+} vq_zone;
+
+/* One column of a bulk upload.  data points at nrows host values of the
+ * type's width; dict_size is the number of global dictionary codes for
+ * VQ_U8/VQ_U16 dictionary columns (0 for a plain integer column). */
+typedef struct vq_col_desc {
+  int32_t type;
+  int32_t dict_size;
+  const void* data;
+} vq_col_desc;
+
+/* ---- predicates: a conjunction of `column OP literal` terms -------------
+ * Replaces Filter(BinOp compare, AndExpr, BetweenExpr) — exec/filter.go:49-78,
+ * exec/expr.go:320-896,1030-1067,1319-1336.  BETWEEN lo AND hi is passed as
+ * the two terms GE lo, LE hi (the reference's own rewrite, expr.go:1319-1336).
+ * ival is the literal for VQ_U8/VQ_U16/VQ_DATE/VQ_I64 columns, fval for
+ * VQ_F64 columns (an integer literal next to a FLOAT64 column is coerced to
+ * float by the caller, planner/physical.go:773-818). */
+#define VQ_EQ 0
+#define VQ_NE 1
+#define VQ_LT 2
+#define VQ_LE 3
+#define VQ_GT 4
+#define VQ_GE 5
+
+#define VQ_MAX_TERMS 8
+typedef struct vq_term {
+  int32_t col;
+  int32_t op;
+  int64_t ival;
+  double fval;
+} vq_term;
+
+typedef struct vq_pred {
+  int32_t nterms; /* 0 = every row passes */
+  int32_t pad_;
+  vq_term terms[VQ_MAX_TERMS];
+} vq_pred;
+
+/* ---- row-local arithmetic: a left fold of up to four factors -------------
+ *   value = ((f0 c1 f1) c2 f2) c3 f3
+ * where each factor is a column, a literal, (literal op column) or
+ * (column op literal), and c_k / op are + - * /.  Every operation is rounded
+ * to float64 on its own (no fused multiply-add), as Project materialises each
+ * BinOp result before the next (exec/project.go:64-110, expr.go:956-1023).
+ * is_float = 1 evaluates in float64 (integer columns cast, CastIntToFloatExpr
+ * expr.go:194-222; x/0 = NaN); is_float = 0 evaluates in int64 (x/0 = 0). */
+#define VQ_F_COL 0
+#define VQ_F_LIT 1
+#define VQ_F_LIT_OP_COL 2
+#define VQ_F_COL_OP_LIT 3
+
+#define VQ_ADD 0
+#define VQ_SUB 1
+#define VQ_MUL 2
+#define VQ_DIV 3
+
+#define VQ_MAX_FACTORS 4
+typedef struct vq_factor {
+  int32_t kind;
+  int32_t col;
+  int32_t op;      /* op inside the factor (LIT_OP_COL / COL_OP_LIT)      */
+  int32_t combine; /* how this factor folds into the running value (k>0) */
+  double fval;     /* literal when is_float                              */
+  int64_t ival;    /* literal when !is_float                             */
+} vq_factor;
+
+typedef struct vq_expr {
+  int32_t nfactors; /* 0 = no argument (COUNT(*)) */
+  int32_t is_float;
+  vq_factor f[VQ_MAX_FACTORS];
+} vq_expr;
+
+/* Aggregates: exec.AggExpr (exec/aggregate.go:25-37).  accum is the
+ * AccumType rule of AccumTypeFor (aggregate.go:51-72): VQ_F64 for AVG and for
+ * SUM/MIN/MAX over a float argument, VQ_I64 otherwise. */
+#define VQ_AGG_COUNT 0
+#define VQ_AGG_SUM 1
+#define VQ_AGG_AVG 2
+#define VQ_AGG_MIN 3
+#define VQ_AGG_MAX 4
+
+#define VQ_MAX_AGGS 8
+typedef struct vq_agg {
+  int32_t kind;
+  int32_t accum;
+  vq_expr arg;
+} vq_agg;
+
+/* Aggregate output, one row per group.  The caller sizes the arrays for
+ * `capacity` groups; ngroups is set even when it exceeds capacity (then the
+ * call returns VQ_E_ARG and the caller retries with more room — the two-call
+ * size query).  Groups come out in first-occurrence order (the order the
+ * serial HashAggregate inserts keys, exec/aggregate.go:83-129), or for a
+ * top-k join in rank order.  Per aggregate j of group g:
+ *   ivals[g*naggs+j] for VQ_I64 accumulators, fvals[g*naggs+j] for VQ_F64,
+ *   valid[g*naggs+j] = 0 where the SQL result is NULL (buildOutputBatch,
+ *   aggregate.go:780-876).  keys: nkeys global codes per group (dictionary
+ *   GROUP BY) or the int64 join key (join aggregate). */
+typedef struct vq_agg_result {
+  int64_t capacity;
+  int64_t ngroups;
+  int64_t* keys;      /* [capacity * nkeys] (codes widened to int64) */
+  int64_t* counts;    /* [capacity] input rows folded into the group  */
+  int64_t* first_row; /* [capacity] lowest contributing row index     */
+  int64_t* ivals;     /* [capacity * naggs]                           */
+  double* fvals;      /* [capacity * naggs]                           */
+  uint8_t* valid;     /* [capacity * naggs]                           */
+} vq_agg_result;
+
+/* Per-call measurements, filled when a non-NULL pointer is passed. */
+typedef struct vq_stats {
+  double device_ms;      /* HIP-event time of the call's kernel sequence  */
+  double main_kernel_ms; /* HIP-event time of its dominant kernel         */
+  int64_t rows_scanned;  /* rows in row groups that survived zone pruning */
+  int64_t rg_total;
+  int64_t rg_scanned;
+  int64_t bytes_alg;     /* algorithmic bytes: referenced columns once     */
+  int64_t build_rows;    /* join: build rows inserted (JoinBuildStats.Rows) */
+  int64_t build_keys;    /* join: distinct build keys   (JoinBuildStats.Keys) */
+  int32_t partitions;    /* join: radix partitions (JoinBuildStats.Partitions) */
+  int32_t radix_bits;
+} vq_stats;
+
+typedef struct vq_ctx vq_ctx;
+typedef struct vq_table vq_table;
+typedef struct vq_selvec vq_selvec;
+
+/* ---- context --------------------------------------------------------------
+ * budget_bytes bounds the device memory this context may hold for tables and
+ * query arenas (0 = 90% of the device's free memory).  No reference
+ * counterpart: vexq's operators allocate from the Go heap. */
+int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out);
+int vq_ctx_destroy(vq_ctx* ctx);
+/* Thread-local text of the last error on this thread ("" if none). */
+const char* vq_last_error(void);
+/* Device bytes currently held by the context (tables + arenas). */
+uint64_t vq_ctx_bytes_in_use(const vq_ctx* ctx);
+
+/* ---- tables: the one bulk upload from storage.Reader ----------------------
+ * Replaces the per-batch decode of TableScan (exec/scan.go:254-409) with one
+ * upload; rg_rows gives the row count of each row group (storage.RowGroupMeta
+ * .NumRows, storage/schema.go:47-52).  zones may be NULL, in which case the
+ * library computes them the way storage/writer.go:205-348 does. */
+int vq_table_upload(vq_ctx* ctx, int64_t nrows, int32_t ncols, const vq_col_desc* cols,
+                    int32_t nrg, const int64_t* rg_rows, const vq_zone* zones,
+                    vq_table** out);
+int vq_table_destroy(vq_table* t);
+int64_t vq_table_rows(const vq_table* t);
+/* Copy the zone maps (nrg * ncols entries, row-group-major) into out. */
+int vq_table_zones(const vq_table* t, vq_zone* out, int64_t cap);
+
+/* Zone-map pruning alone: writes 1/0 per row group into keep (nrg entries).
+ * Restates TableScan's zonePred check (exec/scan.go:211-214) with the
+ * predicate evaluation of planner/physical.go:624-750. */
+int vq_zone_prune(const vq_table* t, const vq_pred* pred, uint8_t* keep);
+
+/* ---- Filter -> selection vector -------------------------------------------
+ * Filter.Next + BoolToSelVecInto (exec/filter.go:49-78, expr.go:1545-1562)
+ * over the zone-pruned TableScan: every row that passes, as ascending global
+ * row indices (u32), left on the device in *out.  The same semantics as the
+ * AVX2 kernel filter_gt_i64_avx2 (bench/simd_filter/cpp/filter_kernel.h). */
+int vq_filter(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, vq_selvec** out,
+              vq_stats* stats);
+int64_t vq_selvec_count(const vq_selvec* sv);
+/* Copies min(count, cap) row indices to host memory. */
+int vq_selvec_copy(const vq_selvec* sv, uint32_t* host_rows, int64_t cap);
+int vq_selvec_destroy(vq_selvec* sv);
+
+/* ---- dictionary-code GROUP BY ---------------------------------------------
+ * ParallelHashAggregate over Scan -> Filter -> Project -> HashAggregate with
+ * the int-key fast path (exec/parallel.go:78-86,124-314, exec/agg_intkey.go:
+ * 206-361, exec/aggregate.go:361-876).  key_cols name 0..4 dictionary columns
+ * (VQ_U8/VQ_U16 with dict_size > 0); nkeys = 0 is the global aggregate
+ * (accumulateDirect, aggregate.go:567-745; one row COUNT 0 / NULLs on empty
+ * input, aggregate.go:961-982). */
+int vq_agg_dict(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t nkeys,
+                const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
+                vq_agg_result* out, vq_stats* stats);
+
+/* ---- radix-partitioned hash join feeding an aggregate ---------------------
+ * ParallelHashJoinAggregate (exec/parallel_join.go:665-758,780-988) over
+ * BuildSharedHashTableParallel (parallel_join.go:389-552) and the flat table
+ * of exec/join_table.go:400-569: inner equi-join of build.build_key with
+ * probe.probe_key (int64 columns), each side filtered by its own predicate,
+ * GROUP BY the join key, aggregates over probe-side expressions, each join
+ * output row (a probe row times every build row carrying its key) folded
+ * once.  radix_bits < 0 lets the library choose.  topk > 0 keeps the topk
+ * groups by aggregate 0 descending (ExternalSort + Limit peeled above the
+ * merge, planner/parallel_join.go:479-506; ties in first-occurrence order,
+ * the stable-sort order of the serial plan). */
+int vq_join_agg(vq_ctx* ctx, const vq_table* build, int32_t build_key, const vq_pred* build_pred,
+                const vq_table* probe, int32_t probe_key, const vq_pred* probe_pred,
+                int32_t naggs, const vq_agg* aggs, int32_t radix_bits, int32_t topk,
+                vq_agg_result* out, vq_stats* stats);
+
+/* RadixBitsFor (exec/parallel_join.go:73-82), radixHash (91-99) and
+ * joinTableCapacity (exec/join_table.go:461-476), exported so host planners
+ * and tests share the library's definitions. */
+int32_t vq_radix_bits_for(int64_t est_rows);
+uint64_t vq_radix_hash(int64_t key);
+int64_t vq_join_table_capacity(int64_t keys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VEXQ_GPU_H */

@@ -1,0 +1,1158 @@
+// vq_api.hip — the C ABI (include/vexq_gpu.h): contexts, resident tables,
+// zone-map pruning, query planning onto the kernels and result assembly.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "vq_internal.h"
+
+using namespace vq;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(VQ_E_DEVICE, "exec: gpu: %s: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+int width_of(int type) {
+  switch (type) {
+    case VQ_U8: return 1;
+    case VQ_U16: return 2;
+    case VQ_DATE: return 4;
+    case VQ_I64:
+    case VQ_F64: return 8;
+    default: return 0;
+  }
+}
+
+// Grow-only device scratch owned by a context.
+struct Arena {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct vq_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  std::atomic<uint64_t> in_use{0};
+  std::mutex mu;  // one query at a time per context (arena + stream)
+  Arena arena;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  hipEvent_t ev[6] = {};
+  // cached selection-vector buffer
+  void* sv_cache = nullptr;
+  size_t sv_cache_bytes = 0;
+};
+
+struct ColDev {
+  int32_t type;
+  int32_t dict_size;
+  void* d;
+};
+
+struct vq_table {
+  vq_ctx* ctx;
+  int64_t nrows;
+  int32_t ncols;
+  std::vector<ColDev> cols;
+  int32_t nrg;
+  std::vector<int64_t> rg_rows, rg_start;
+  std::vector<vq_zone> zones;  // nrg * ncols
+  uint64_t bytes;
+};
+
+struct vq_selvec {
+  vq_ctx* ctx;
+  uint32_t* rows;
+  size_t bytes;
+  int64_t count;
+};
+
+namespace {
+
+bool reserve(vq_ctx* c, uint64_t bytes) {
+  uint64_t cur = c->in_use.load();
+  while (true) {
+    if (cur + bytes > c->budget) return false;
+    if (c->in_use.compare_exchange_weak(cur, cur + bytes)) return true;
+  }
+}
+
+void release(vq_ctx* c, uint64_t bytes) { c->in_use.fetch_sub(bytes); }
+
+int arena_get(vq_ctx* c, size_t bytes, void** out) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (c->arena.bytes < bytes) {
+    size_t want = std::max(bytes, c->arena.bytes * 3 / 2);
+    if (c->arena.p) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipFree(c->arena.p));
+      release(c, c->arena.bytes);
+      c->arena.p = nullptr;
+      c->arena.bytes = 0;
+    }
+    if (!reserve(c, want))
+      return fail(VQ_E_BUDGET, "exec: gpu: query arena of %zu bytes exceeds the device budget", want);
+    hipError_t e = hipMalloc(&c->arena.p, want);
+    if (e != hipSuccess) {
+      release(c, want);
+      return fail(VQ_E_DEVICE, "exec: gpu: hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    }
+    c->arena.bytes = want;
+  }
+  *out = c->arena.p;
+  return VQ_OK;
+}
+
+int pinned_get(vq_ctx* c, size_t bytes, void** out) {
+  if (c->pinned_bytes < bytes) {
+    if (c->pinned) HIP_TRY(hipHostFree(c->pinned));
+    c->pinned = nullptr;
+    size_t want = std::max(bytes, (size_t)1 << 16);
+    HIP_TRY(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+    c->pinned_bytes = want;
+  }
+  *out = c->pinned;
+  return VQ_OK;
+}
+
+// Bump allocator over the arena.
+struct Carve {
+  size_t off = 0;
+  template <class T>
+  T* take(char* base, size_t n) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = (T*)(base + off);
+    off += n * sizeof(T);
+    return p;
+  }
+};
+
+// ---- zone maps: storage/writer.go:205-348 -----------------------------------
+void compute_zones(const vq_col_desc* cols, int32_t ncols, int32_t nrg,
+                   const std::vector<int64_t>& rg_start, const std::vector<int64_t>& rg_rows,
+                   std::vector<vq_zone>& out) {
+  out.assign((size_t)nrg * ncols, vq_zone{});
+  for (int32_t g = 0; g < nrg; g++) {
+    const int64_t b = rg_start[g], n = rg_rows[g];
+    for (int32_t c = 0; c < ncols; c++) {
+      vq_zone& z = out[(size_t)g * ncols + c];
+      if (n <= 0) continue;
+      z.has_minmax = 1;
+      const vq_col_desc& cd = cols[c];
+      switch (cd.type) {
+        case VQ_U8:
+        case VQ_U16: {
+          if (cd.dict_size > 0) {
+            z.min_bits = 0;
+            z.max_bits = (uint64_t)(cd.dict_size - 1);
+          } else {
+            int64_t mn = INT64_MAX, mx = INT64_MIN;
+            for (int64_t r = b; r < b + n; r++) {
+              int64_t v = cd.type == VQ_U8 ? ((const uint8_t*)cd.data)[r] : ((const uint16_t*)cd.data)[r];
+              mn = std::min(mn, v);
+              mx = std::max(mx, v);
+            }
+            z.min_bits = (uint64_t)mn;
+            z.max_bits = (uint64_t)mx;
+          }
+          break;
+        }
+        case VQ_DATE: {
+          const int32_t* d = (const int32_t*)cd.data;
+          int32_t mn = d[b], mx = d[b];
+          for (int64_t r = b + 1; r < b + n; r++) {
+            if (d[r] < mn) mn = d[r];
+            if (d[r] > mx) mx = d[r];
+          }
+          z.min_bits = (uint64_t)(uint32_t)mn;
+          z.max_bits = (uint64_t)(uint32_t)mx;
+          break;
+        }
+        case VQ_I64: {
+          const int64_t* d = (const int64_t*)cd.data;
+          int64_t mn = d[b], mx = d[b];
+          for (int64_t r = b + 1; r < b + n; r++) {
+            if (d[r] < mn) mn = d[r];
+            if (d[r] > mx) mx = d[r];
+          }
+          z.min_bits = (uint64_t)mn;
+          z.max_bits = (uint64_t)mx;
+          break;
+        }
+        default: {
+          const double* d = (const double*)cd.data;
+          double mn = d[b], mx = d[b];
+          for (int64_t r = b + 1; r < b + n; r++) {
+            if (d[r] < mn) mn = d[r];
+            if (d[r] > mx) mx = d[r];
+          }
+          memcpy(&z.min_bits, &mn, 8);
+          memcpy(&z.max_bits, &mx, 8);
+        }
+      }
+    }
+  }
+}
+
+// zoneRangePred (planner/physical.go:667-750) on one conjunct.
+bool zone_term_keep(const ColDev& c, const vq_zone& z, const vq_term& t) {
+  if (!z.has_minmax) return true;
+  if ((c.type == VQ_U8 || c.type == VQ_U16) && c.dict_size > 0) return true;
+  int64_t v;
+  if (c.type == VQ_F64) memcpy(&v, &t.fval, 8);
+  else v = t.ival;
+  const int64_t mn = (int64_t)z.min_bits, mx = (int64_t)z.max_bits;
+  switch (t.op) {
+    case VQ_EQ: return mn <= v && v <= mx;
+    case VQ_NE: return true;
+    case VQ_LT: return mn < v;
+    case VQ_LE: return mn <= v;
+    case VQ_GT: return mx > v;
+    default: return mx >= v;
+  }
+}
+
+void prune(const vq_table* t, const vq_pred* pred, std::vector<uint8_t>& keep) {
+  keep.assign(t->nrg, 1);
+  if (!pred) return;
+  for (int32_t g = 0; g < t->nrg; g++)
+    for (int k = 0; k < pred->nterms && keep[g]; k++) {
+      const vq_term& tm = pred->terms[k];
+      keep[g] = zone_term_keep(t->cols[tm.col], t->zones[(size_t)g * t->ncols + tm.col], tm);
+    }
+}
+
+// Surviving row groups -> contiguous spans -> tiles.  Returns rows scanned.
+int make_spans(const vq_table* t, const std::vector<uint8_t>& keep, int32_t tile_rows,
+               QSpan* spans, int32_t* nspans, int64_t* ntiles, int64_t* rows, int64_t* rg_kept) {
+  int ns = 0;
+  int64_t tiles = 0, nrows = 0, kept = 0;
+  for (int32_t g = 0; g < t->nrg; g++) {
+    if (!keep[g] || t->rg_rows[g] == 0) continue;
+    kept++;
+    const int64_t st = t->rg_start[g], n = t->rg_rows[g];
+    if (ns > 0 && spans[ns - 1].start + spans[ns - 1].rows == st) {
+      spans[ns - 1].rows += n;
+    } else {
+      if (ns == kMaxSpans)
+        return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d disjoint row-group runs survive pruning", kMaxSpans);
+      spans[ns].start = st;
+      spans[ns].rows = n;
+      ns++;
+    }
+    nrows += n;
+  }
+  for (int i = 0; i < ns; i++) {
+    spans[i].tile0 = tiles;
+    tiles += (spans[i].rows + tile_rows - 1) / tile_rows;
+  }
+  *nspans = ns;
+  *ntiles = tiles;
+  *rows = nrows;
+  *rg_kept = kept;
+  return VQ_OK;
+}
+
+// Maps table columns to the query's slots (first use order).
+struct SlotMap {
+  int32_t col_of[kMaxSlots];
+  int32_t n = 0;
+  int get(int32_t col) {
+    for (int i = 0; i < n; i++)
+      if (col_of[i] == col) return i;
+    if (n == kMaxSlots) return -1;
+    col_of[n] = col;
+    return n++;
+  }
+};
+
+int check_col(const vq_table* t, int32_t c) {
+  if (c < 0 || c >= t->ncols) return fail(VQ_E_ARG, "exec: gpu: column %d out of range", c);
+  return VQ_OK;
+}
+
+int add_pred(const vq_table* t, const vq_pred* pred, SlotMap& sm, QSlot* slots) {
+  if (!pred) return VQ_OK;
+  if (pred->nterms < 0 || pred->nterms > VQ_MAX_TERMS)
+    return fail(VQ_E_ARG, "exec: gpu: predicate has %d terms", pred->nterms);
+  for (int k = 0; k < pred->nterms; k++) {
+    const vq_term& tm = pred->terms[k];
+    if (int rc = check_col(t, tm.col)) return rc;
+    if (tm.op < VQ_EQ || tm.op > VQ_GE) return fail(VQ_E_ARG, "exec: gpu: bad comparison op %d", tm.op);
+    int s = sm.get(tm.col);
+    if (s < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+    QSlot& sl = slots[s];
+    if (sl.nterms == kMaxSlotTerms)
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d terms on one column", kMaxSlotTerms);
+    QTerm& q = sl.terms[sl.nterms++];
+    q.op = tm.op;
+    q.ival = tm.ival;
+    q.fval = tm.fval;
+  }
+  return VQ_OK;
+}
+
+int add_expr(const vq_table* t, const vq_expr& e, SlotMap& sm, QExpr& out) {
+  memset(&out, 0, sizeof out);
+  if (e.nfactors < 0 || e.nfactors > VQ_MAX_FACTORS)
+    return fail(VQ_E_ARG, "exec: gpu: expression has %d factors", e.nfactors);
+  out.nfactors = e.nfactors;
+  out.is_float = e.is_float ? 1 : 0;
+  for (int k = 0; k < e.nfactors; k++) {
+    const vq_factor& f = e.f[k];
+    QFactor& q = out.f[k];
+    if (f.kind < VQ_F_COL || f.kind > VQ_F_COL_OP_LIT) return fail(VQ_E_ARG, "exec: gpu: bad factor kind");
+    if (f.op < VQ_ADD || f.op > VQ_DIV || f.combine < VQ_ADD || f.combine > VQ_DIV)
+      return fail(VQ_E_ARG, "exec: gpu: bad arithmetic op");
+    q.kind = f.kind;
+    q.op = f.op;
+    q.combine = f.combine;
+    q.fval = f.fval;
+    q.ival = f.ival;
+    if (f.kind != VQ_F_LIT) {
+      if (int rc = check_col(t, f.col)) return rc;
+      if (!e.is_float && t->cols[f.col].type == VQ_F64)
+        return fail(VQ_E_ARG, "exec: gpu: float column in an integer expression");
+      int s = sm.get(f.col);
+      if (s < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+      q.slot = s;
+    }
+  }
+  return VQ_OK;
+}
+
+void fill_slot_cols(const vq_table* t, const SlotMap& sm, QSlot* slots) {
+  for (int i = 0; i < sm.n; i++) {
+    slots[i].p = t->cols[sm.col_of[i]].d;
+    slots[i].type = t->cols[sm.col_of[i]].type;
+  }
+}
+
+int64_t slot_bytes_per_row(const vq_table* t, const SlotMap& sm) {
+  int64_t w = 0;
+  for (int i = 0; i < sm.n; i++) w += width_of(t->cols[sm.col_of[i]].type);
+  return w;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+  return ms;
+}
+
+// AccumTypeFor (exec/aggregate.go:51-72) agreement check.
+int check_agg(const vq_agg& a) {
+  if (a.kind < VQ_AGG_COUNT || a.kind > VQ_AGG_MAX) return fail(VQ_E_ARG, "exec: gpu: bad aggregate kind %d", a.kind);
+  if (a.kind == VQ_AGG_COUNT) return VQ_OK;
+  if (a.arg.nfactors <= 0) return fail(VQ_E_ARG, "exec: gpu: aggregate %d needs an argument", a.kind);
+  const int want = a.kind == VQ_AGG_AVG ? VQ_F64 : (a.arg.is_float ? VQ_F64 : VQ_I64);
+  if (a.accum != want)
+    return fail(VQ_E_ARG, "exec: gpu: accumulator type %d disagrees with AccumTypeFor (%d)", a.accum, want);
+  return VQ_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// context
+// ===========================================================================
+extern "C" const char* vq_last_error(void) { return g_err.c_str(); }
+
+extern "C" int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out) {
+  if (!out) return fail(VQ_E_ARG, "exec: gpu: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(VQ_E_ARG, "exec: gpu: device %d of %d", device, n);
+  HIP_TRY(hipSetDevice(device));
+  vq_ctx* c = new (std::nothrow) vq_ctx();
+  if (!c) return fail(VQ_E_NOMEM, "exec: gpu: context allocation");
+  c->device = device;
+  size_t fr = 0, tot = 0;
+  hipError_t e = hipMemGetInfo(&fr, &tot);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(VQ_E_DEVICE, "exec: gpu: hipMemGetInfo: %s", hipGetErrorString(e));
+  }
+  c->budget = budget_bytes ? budget_bytes : (uint64_t)(fr * 0.9);
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(VQ_E_DEVICE, "exec: gpu: hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  for (auto& ev : c->ev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(VQ_E_DEVICE, "exec: gpu: hipEventCreate: %s", hipGetErrorString(e));
+    }
+  }
+  *out = c;
+  return VQ_OK;
+}
+
+extern "C" int vq_ctx_destroy(vq_ctx* c) {
+  if (!c) return VQ_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  if (c->arena.p) hipFree(c->arena.p);
+  if (c->sv_cache) hipFree(c->sv_cache);
+  if (c->pinned) hipHostFree(c->pinned);
+  for (auto& ev : c->ev)
+    if (ev) hipEventDestroy(ev);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return VQ_OK;
+}
+
+extern "C" uint64_t vq_ctx_bytes_in_use(const vq_ctx* c) { return c ? c->in_use.load() : 0; }
+
+// ===========================================================================
+// tables
+// ===========================================================================
+extern "C" int vq_table_upload(vq_ctx* c, int64_t nrows, int32_t ncols, const vq_col_desc* cols,
+                               int32_t nrg, const int64_t* rg_rows, const vq_zone* zones,
+                               vq_table** out) {
+  if (!c || !out || (ncols > 0 && !cols) || ncols < 0 || nrows < 0 || nrows > 0xFFFFFFFELL)
+    return fail(VQ_E_ARG, "exec: gpu: bad table upload arguments");
+  *out = nullptr;
+  if (nrg < 0 || (nrg > 0 && !rg_rows)) return fail(VQ_E_ARG, "exec: gpu: bad row-group list");
+  int64_t tot = 0;
+  for (int32_t g = 0; g < nrg; g++) {
+    if (rg_rows[g] < 0) return fail(VQ_E_ARG, "exec: gpu: negative row-group size");
+    tot += rg_rows[g];
+  }
+  if (tot != nrows) return fail(VQ_E_ARG, "exec: gpu: row groups hold %lld rows, table %lld",
+                                (long long)tot, (long long)nrows);
+  uint64_t bytes = 0;
+  for (int32_t i = 0; i < ncols; i++) {
+    int w = width_of(cols[i].type);
+    if (!w) return fail(VQ_E_ARG, "exec: gpu: column %d has unknown type %d", i, cols[i].type);
+    if (nrows > 0 && !cols[i].data) return fail(VQ_E_ARG, "exec: gpu: column %d has no data", i);
+    if (cols[i].dict_size < 0 || (cols[i].type == VQ_U8 && cols[i].dict_size > 256) ||
+        (cols[i].type == VQ_U16 && cols[i].dict_size > 65536) ||
+        (cols[i].dict_size > 0 && cols[i].type != VQ_U8 && cols[i].type != VQ_U16))
+      return fail(VQ_E_ARG, "exec: gpu: column %d: bad dictionary size %d", i, cols[i].dict_size);
+    bytes += ((uint64_t)nrows * w + 255) & ~255ULL;
+  }
+  // dictionary codes must index their dictionary
+  for (int32_t i = 0; i < ncols; i++) {
+    if (cols[i].dict_size <= 0) continue;
+    for (int64_t r = 0; r < nrows; r++) {
+      int v = cols[i].type == VQ_U8 ? ((const uint8_t*)cols[i].data)[r] : ((const uint16_t*)cols[i].data)[r];
+      if (v >= cols[i].dict_size)
+        return fail(VQ_E_ARG, "exec: gpu: column %d row %lld: code %d outside dictionary of %d", i,
+                    (long long)r, v, cols[i].dict_size);
+    }
+  }
+  if (!reserve(c, bytes)) return fail(VQ_E_BUDGET, "exec: gpu: table of %llu bytes exceeds the device budget",
+                                      (unsigned long long)bytes);
+  vq_table* t = new (std::nothrow) vq_table();
+  if (!t) {
+    release(c, bytes);
+    return fail(VQ_E_NOMEM, "exec: gpu: table allocation");
+  }
+  t->ctx = c;
+  t->nrows = nrows;
+  t->ncols = ncols;
+  t->nrg = nrg;
+  t->bytes = bytes;
+  t->rg_rows.assign(rg_rows, rg_rows + nrg);
+  t->rg_start.resize(nrg);
+  int64_t st = 0;
+  for (int32_t g = 0; g < nrg; g++) {
+    t->rg_start[g] = st;
+    st += rg_rows[g];
+  }
+  hipSetDevice(c->device);
+  for (int32_t i = 0; i < ncols; i++) {
+    ColDev cd{cols[i].type, cols[i].dict_size, nullptr};
+    size_t nb = (size_t)nrows * width_of(cols[i].type);
+    hipError_t e = hipMalloc(&cd.d, nb ? nb : 256);
+    if (e == hipSuccess && nb) e = hipMemcpy(cd.d, cols[i].data, nb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (cd.d) hipFree(cd.d);
+      for (auto& x : t->cols) hipFree(x.d);
+      release(c, bytes);
+      delete t;
+      return fail(VQ_E_DEVICE, "exec: gpu: upload of column %d: %s", i, hipGetErrorString(e));
+    }
+    t->cols.push_back(cd);
+  }
+  if (zones) t->zones.assign(zones, zones + (size_t)nrg * ncols);
+  else compute_zones(cols, ncols, nrg, t->rg_start, t->rg_rows, t->zones);
+  *out = t;
+  return VQ_OK;
+}
+
+extern "C" int vq_table_destroy(vq_table* t) {
+  if (!t) return VQ_OK;
+  hipSetDevice(t->ctx->device);
+  hipStreamSynchronize(t->ctx->stream);
+  for (auto& x : t->cols) hipFree(x.d);
+  release(t->ctx, t->bytes);
+  delete t;
+  return VQ_OK;
+}
+
+extern "C" int64_t vq_table_rows(const vq_table* t) { return t ? t->nrows : -1; }
+
+extern "C" int vq_table_zones(const vq_table* t, vq_zone* out, int64_t cap) {
+  if (!t || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (cap < (int64_t)t->zones.size()) return fail(VQ_E_ARG, "exec: gpu: zone buffer too small");
+  memcpy(out, t->zones.data(), t->zones.size() * sizeof(vq_zone));
+  return VQ_OK;
+}
+
+extern "C" int vq_zone_prune(const vq_table* t, const vq_pred* pred, uint8_t* keep) {
+  if (!t || !keep) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (pred) {
+    if (pred->nterms < 0 || pred->nterms > VQ_MAX_TERMS) return fail(VQ_E_ARG, "exec: gpu: bad predicate");
+    for (int k = 0; k < pred->nterms; k++)
+      if (int rc = check_col(t, pred->terms[k].col)) return rc;
+  }
+  std::vector<uint8_t> k;
+  prune(t, pred, k);
+  memcpy(keep, k.data(), k.size());
+  return VQ_OK;
+}
+
+extern "C" int32_t vq_radix_bits_for(int64_t est) {
+  if (est < 65536) return 0;
+  int32_t bits = 4;
+  while (bits < 6 && ((int64_t)1 << bits) * 8192 < est) bits++;
+  return bits;
+}
+
+extern "C" uint64_t vq_radix_hash(int64_t key) { return radix_hash((uint64_t)key); }
+
+extern "C" int64_t vq_join_table_capacity(int64_t keys) {
+  int64_t slots = 8;
+  if (keys <= 0) return slots;
+  int64_t need = (keys * 10 + 6) / 7;
+  while (slots < need) slots *= 2;
+  return slots;
+}
+
+// ===========================================================================
+// dictionary-code GROUP BY
+// ===========================================================================
+namespace {
+
+constexpr int32_t kDenseTileRows = 64 * 64;  // rows per wave64 workgroup
+constexpr size_t kDenseLdsMax = 64 * 1024;
+
+struct AccMap {
+  int32_t acc[VQ_MAX_AGGS];  // accumulator index per aggregate (-1 = COUNT)
+};
+
+int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int32_t* key_cols,
+               int32_t naggs, const vq_agg* aggs, QDesc& d, AccMap& am, SlotMap& sm,
+               int32_t card[4]) {
+  memset(&d, 0, sizeof d);
+  if (nkeys < 0 || nkeys > 4 || (nkeys > 0 && !key_cols))
+    return fail(VQ_E_ARG, "exec: gpu: %d group-by columns", nkeys);
+  if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs))
+    return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
+  if (naggs == 0 && nkeys == 0)
+    return fail(VQ_E_ARG, "exec: hash aggregate: no group-by columns or aggregate expressions");
+  if (int rc = add_pred(t, pred, sm, d.slot)) return rc;
+  int64_t G = 1;
+  for (int i = 0; i < nkeys; i++) {
+    if (int rc = check_col(t, key_cols[i])) return rc;
+    const ColDev& c = t->cols[key_cols[i]];
+    if ((c.type != VQ_U8 && c.type != VQ_U16) || c.dict_size <= 0)
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: group-by column %d is not a dictionary code", key_cols[i]);
+    card[i] = c.dict_size;
+    G *= c.dict_size;
+  }
+  if (G > 4096) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld dense group slots", (long long)G);
+  int64_t stride = 1;
+  for (int i = nkeys - 1; i >= 0; i--) {
+    int s = sm.get(key_cols[i]);
+    if (s < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+    d.slot[s].key_stride += (int32_t)stride;
+    stride *= card[i];
+  }
+  d.ngroups = (int32_t)G;
+  // accumulators, de-duplicated (SUM(price) and AVG(price) share one)
+  for (int j = 0; j < naggs; j++) {
+    if (int rc = check_agg(aggs[j])) return rc;
+    am.acc[j] = -1;
+    if (aggs[j].kind == VQ_AGG_COUNT) continue;
+    QAcc qa;
+    memset(&qa, 0, sizeof qa);
+    if (int rc = add_expr(t, aggs[j].arg, sm, qa.e)) return rc;
+    const bool f = aggs[j].arg.is_float;
+    switch (aggs[j].kind) {
+      case VQ_AGG_SUM:
+      case VQ_AGG_AVG: qa.op = f ? ACC_SUM_F : ACC_SUM_I; break;
+      case VQ_AGG_MIN: qa.op = f ? ACC_MIN_F : ACC_MIN_I; break;
+      default: qa.op = f ? ACC_MAX_F : ACC_MAX_I; break;
+    }
+    int found = -1;
+    for (int k = 0; k < d.nacc; k++)
+      if (memcmp(&d.acc[k], &qa, sizeof qa) == 0) found = k;
+    if (found < 0) {
+      if (d.nacc == kMaxAcc) return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d accumulators", kMaxAcc);
+      d.acc[d.nacc] = qa;
+      found = d.nacc++;
+    }
+    am.acc[j] = found;
+  }
+  fill_slot_cols(t, sm, d.slot);
+  d.nslots = sm.n;
+  if (d.nslots == 0) {  // COUNT(*) with no predicate still needs a row source
+    if (t->ncols == 0) return fail(VQ_E_ARG, "exec: gpu: table has no columns");
+    int s = sm.get(0);
+    d.slot[s].p = t->cols[0].d;
+    d.slot[s].type = t->cols[0].type;
+    d.nslots = 1;
+  }
+  if (G > 1) {
+    size_t lds = (size_t)G * d.nacc * 64 * 8 + (size_t)G * 64 * 8;
+    if (lds > kDenseLdsMax)
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld groups x %d accumulators exceed LDS", (long long)G, d.nacc);
+  }
+  d.tile_rows = kDenseTileRows;
+  return VQ_OK;
+}
+
+}  // namespace
+
+extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
+                           const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
+                           vq_agg_result* out, vq_stats* stats) {
+  if (!c || !t || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (t->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  QDesc d;
+  AccMap am;
+  SlotMap sm;
+  int32_t card[4] = {1, 1, 1, 1};
+  if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card)) return rc;
+  std::vector<uint8_t> keep;
+  prune(t, pred, keep);
+  int64_t rows = 0, kept = 0;
+  if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
+
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  const int NI = d.nacc + 2;
+  const size_t part_words = (size_t)std::max<int64_t>(d.ntiles, 1) * d.ngroups * NI;
+  const size_t out_words = (size_t)d.ngroups * NI;
+  void* ar;
+  if (int rc = arena_get(c, (part_words + out_words) * 8 + 1024, &ar)) return rc;
+  Carve cv;
+  uint64_t* part = cv.take<uint64_t>((char*)ar, part_words);
+  uint64_t* fin = cv.take<uint64_t>((char*)ar, out_words);
+  void* hp;
+  if (int rc = pinned_get(c, out_words * 8, &hp)) return rc;
+  uint64_t* hfin = (uint64_t*)hp;
+
+  if (d.ntiles > 0) {
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    HIP_TRY(launch_agg_dense(d, part, c->stream));
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(launch_agg_final(d, part, fin, c->stream));
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(hipMemcpyAsync(hfin, fin, out_words * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    for (int g = 0; g < d.ngroups; g++)
+      for (int i = 0; i < NI; i++) hfin[g * NI + i] = (i == d.nacc + 1) ? 0xFFFFFFFFULL : 0;
+  }
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    if (d.ntiles > 0) {
+      stats->device_ms = elapsed(c->ev[0], c->ev[2]);
+      stats->main_kernel_ms = elapsed(c->ev[0], c->ev[1]);
+    }
+    stats->rows_scanned = rows;
+    stats->rg_total = t->nrg;
+    stats->rg_scanned = kept;
+    stats->bytes_alg = rows * slot_bytes_per_row(t, sm);
+  }
+
+  // groups in first-occurrence order (the serial HashAggregate's key order)
+  std::vector<std::pair<uint64_t, int>> order;
+  for (int g = 0; g < d.ngroups; g++) {
+    const uint64_t cnt = hfin[g * NI + d.nacc];
+    if (cnt) order.push_back({hfin[g * NI + d.nacc + 1], g});
+  }
+  std::sort(order.begin(), order.end());
+  const bool empty_global = order.empty() && nkeys == 0;
+  const int64_t ng = empty_global ? 1 : (int64_t)order.size();
+  out->ngroups = ng;
+  if (ng > out->capacity)
+    return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)ng,
+                (long long)out->capacity);
+  if (empty_global) {  // buildEmptyGlobalResult (exec/aggregate.go:961-982)
+    out->counts[0] = 0;
+    out->first_row[0] = -1;
+    for (int j = 0; j < naggs; j++) {
+      out->ivals[j] = 0;
+      out->fvals[j] = 0;
+      out->valid[j] = aggs[j].kind == VQ_AGG_COUNT;
+    }
+    return VQ_OK;
+  }
+  for (int64_t i = 0; i < ng; i++) {
+    const int g = order[i].second;
+    const uint64_t* r = &hfin[g * NI];
+    const int64_t cnt = (int64_t)r[d.nacc];
+    int64_t rem = g;
+    for (int k = nkeys - 1; k >= 0; k--) {
+      out->keys[i * nkeys + k] = rem % card[k];
+      rem /= card[k];
+    }
+    out->counts[i] = cnt;
+    out->first_row[i] = (int64_t)r[d.nacc + 1];
+    for (int j = 0; j < naggs; j++) {
+      const int64_t o = i * naggs + j;
+      out->ivals[o] = 0;
+      out->fvals[o] = 0;
+      out->valid[o] = 1;
+      const vq_agg& a = aggs[j];
+      if (a.kind == VQ_AGG_COUNT) {
+        out->ivals[o] = cnt;
+        continue;
+      }
+      const uint64_t w = r[am.acc[j]];
+      if (a.kind == VQ_AGG_AVG) {
+        const double s = a.arg.is_float ? bits_to_f64((int64_t)w) : (double)(int64_t)w;
+        out->fvals[o] = s / (double)cnt;
+      } else if (a.accum == VQ_F64) {
+        out->fvals[o] = bits_to_f64((int64_t)w);
+      } else {
+        out->ivals[o] = (int64_t)w;
+      }
+    }
+  }
+  return VQ_OK;
+}
+
+// ===========================================================================
+// Filter -> selection vector
+// ===========================================================================
+extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_selvec** out,
+                         vq_stats* stats) {
+  if (!c || !t || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (t->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  *out = nullptr;
+  QDesc d;
+  memset(&d, 0, sizeof d);
+  SlotMap sm;
+  if (int rc = add_pred(t, pred, sm, d.slot)) return rc;
+  fill_slot_cols(t, sm, d.slot);
+  d.nslots = sm.n;
+  if (d.nslots == 0 && t->ncols > 0) {
+    int s = sm.get(0);
+    d.slot[s].p = t->cols[0].d;
+    d.slot[s].type = t->cols[0].type;
+    d.nslots = 1;
+  }
+  d.ngroups = 1;
+  d.tile_rows = kDenseTileRows;
+  std::vector<uint8_t> keep;
+  prune(t, pred, keep);
+  int64_t rows = 0, kept = 0;
+  if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
+
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  const size_t nwords = (size_t)(t->nrows / 64 + 2);
+  const size_t nt = (size_t)std::max<int64_t>(d.ntiles, 1);
+  void* ar;
+  if (int rc = arena_get(c, nwords * 8 + nt * 8 + 64 + 2048, &ar)) return rc;
+  Carve cv;
+  uint64_t* bits = cv.take<uint64_t>((char*)ar, nwords);
+  uint32_t* tcount = cv.take<uint32_t>((char*)ar, nt);
+  uint32_t* toff = cv.take<uint32_t>((char*)ar, nt);
+  uint64_t* dcount = cv.take<uint64_t>((char*)ar, 1);
+  // selection-vector storage: reuse the cached buffer when it is large enough
+  const size_t need = (size_t)std::max<int64_t>(rows, 1) * 4;
+  vq_selvec* sv = new (std::nothrow) vq_selvec();
+  if (!sv) return fail(VQ_E_NOMEM, "exec: gpu: selvec allocation");
+  sv->ctx = c;
+  if (c->sv_cache && c->sv_cache_bytes >= need) {
+    sv->rows = (uint32_t*)c->sv_cache;
+    sv->bytes = c->sv_cache_bytes;
+    c->sv_cache = nullptr;
+    c->sv_cache_bytes = 0;
+  } else {
+    if (!reserve(c, need)) {
+      delete sv;
+      return fail(VQ_E_BUDGET, "exec: gpu: selection vector of %zu bytes exceeds the budget", need);
+    }
+    hipError_t e = hipMalloc((void**)&sv->rows, need);
+    if (e != hipSuccess) {
+      release(c, need);
+      delete sv;
+      return fail(VQ_E_DEVICE, "exec: gpu: hipMalloc: %s", hipGetErrorString(e));
+    }
+    sv->bytes = need;
+  }
+  void* hp;
+  if (int rc = pinned_get(c, 64, &hp)) {
+    vq_selvec_destroy(sv);
+    return rc;
+  }
+  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  HIP_TRY(launch_filter(d, bits, tcount, toff, sv->rows, dcount, c->stream, c->ev[1], c->ev[2]));
+  HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  HIP_TRY(hipMemcpyAsync(hp, dcount, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  sv->count = (int64_t)*(uint64_t*)hp;
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    stats->device_ms = elapsed(c->ev[0], c->ev[3]);
+    stats->main_kernel_ms = d.ntiles ? elapsed(c->ev[1], c->ev[2]) : 0;
+    stats->rows_scanned = rows;
+    stats->rg_total = t->nrg;
+    stats->rg_scanned = kept;
+    stats->bytes_alg = rows * slot_bytes_per_row(t, sm) + sv->count * 4;
+  }
+  *out = sv;
+  return VQ_OK;
+}
+
+extern "C" int64_t vq_selvec_count(const vq_selvec* sv) { return sv ? sv->count : -1; }
+
+extern "C" int vq_selvec_copy(const vq_selvec* sv, uint32_t* host_rows, int64_t cap) {
+  if (!sv || (!host_rows && cap > 0)) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  const int64_t n = std::min(cap, sv->count);
+  if (n <= 0) return VQ_OK;
+  hipSetDevice(sv->ctx->device);
+  HIP_TRY(hipMemcpy(host_rows, sv->rows, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return VQ_OK;
+}
+
+extern "C" int vq_selvec_destroy(vq_selvec* sv) {
+  if (!sv) return VQ_OK;
+  vq_ctx* c = sv->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->sv_cache) {
+    c->sv_cache = sv->rows;
+    c->sv_cache_bytes = sv->bytes;
+  } else if (c->sv_cache_bytes < sv->bytes) {
+    hipFree(c->sv_cache);
+    release(c, c->sv_cache_bytes);
+    c->sv_cache = sv->rows;
+    c->sv_cache_bytes = sv->bytes;
+  } else {
+    hipFree(sv->rows);
+    release(c, sv->bytes);
+  }
+  delete sv;
+  return VQ_OK;
+}
+
+// ===========================================================================
+// radix-partitioned join + aggregate
+// ===========================================================================
+namespace {
+
+struct Interval {
+  double lo, hi;
+  bool ok;
+};
+
+Interval col_interval(const vq_table* t, int32_t col) {
+  const ColDev& c = t->cols[col];
+  Interval iv{0, 0, true};
+  bool any = false;
+  for (int32_t g = 0; g < t->nrg; g++) {
+    const vq_zone& z = t->zones[(size_t)g * t->ncols + col];
+    if (!z.has_minmax) continue;
+    double lo, hi;
+    switch (c.type) {
+      case VQ_F64: lo = bits_to_f64((int64_t)z.min_bits); hi = bits_to_f64((int64_t)z.max_bits); break;
+      case VQ_DATE: lo = (double)(int32_t)(uint32_t)z.min_bits; hi = (double)(int32_t)(uint32_t)z.max_bits; break;
+      default: lo = (double)(int64_t)z.min_bits; hi = (double)(int64_t)z.max_bits; break;
+    }
+    if (!std::isfinite(lo) || !std::isfinite(hi)) iv.ok = false;
+    if (!any) {
+      iv.lo = lo;
+      iv.hi = hi;
+      any = true;
+    } else {
+      iv.lo = std::min(iv.lo, lo);
+      iv.hi = std::max(iv.hi, hi);
+    }
+  }
+  return iv;
+}
+
+Interval iv_op(int op, Interval a, Interval b) {
+  Interval r{0, 0, a.ok && b.ok};
+  switch (op) {
+    case VQ_ADD: r.lo = a.lo + b.lo; r.hi = a.hi + b.hi; break;
+    case VQ_SUB: r.lo = a.lo - b.hi; r.hi = a.hi - b.lo; break;
+    case VQ_MUL: {
+      double p[4] = {a.lo * b.lo, a.lo * b.hi, a.hi * b.lo, a.hi * b.hi};
+      r.lo = *std::min_element(p, p + 4);
+      r.hi = *std::max_element(p, p + 4);
+      break;
+    }
+    default:
+      if (b.lo <= 0 && b.hi >= 0) {
+        r.ok = false;
+      } else {
+        double p[4] = {a.lo / b.lo, a.lo / b.hi, a.hi / b.lo, a.hi / b.hi};
+        r.lo = *std::min_element(p, p + 4);
+        r.hi = *std::max_element(p, p + 4);
+      }
+  }
+  if (!std::isfinite(r.lo) || !std::isfinite(r.hi)) r.ok = false;
+  return r;
+}
+
+// Bound of |expr| over the table from its zone maps (interval arithmetic).
+Interval expr_interval(const vq_table* t, const vq_expr& e) {
+  Interval v{0, 0, true};
+  for (int k = 0; k < e.nfactors; k++) {
+    const vq_factor& f = e.f[k];
+    Interval x;
+    const Interval lit{f.fval, f.fval, std::isfinite(f.fval)};
+    switch (f.kind) {
+      case VQ_F_COL: x = col_interval(t, f.col); break;
+      case VQ_F_LIT: x = lit; break;
+      case VQ_F_LIT_OP_COL: x = iv_op(f.op, lit, col_interval(t, f.col)); break;
+      default: x = iv_op(f.op, col_interval(t, f.col), lit); break;
+    }
+    v = k == 0 ? x : iv_op(f.combine, v, x);
+  }
+  return v;
+}
+
+int ilog2_ceil(uint64_t x) {
+  int l = 0;
+  while (l < 64 && ((uint64_t)1 << l) < x) l++;
+  return l;
+}
+
+int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg* aggs,
+              int32_t naggs, JSide& sd, JDesc* jd, SlotMap& sm) {
+  memset(&sd, 0, sizeof sd);
+  if (int rc = check_col(t, key)) return rc;
+  if (t->cols[key].type == VQ_F64) return fail(VQ_E_UNSUPPORTED, "exec: gpu: float join key");
+  if (int rc = add_pred(t, pred, sm, sd.slot)) return rc;
+  int ks = sm.get(key);
+  if (ks < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+  sd.key_slot = ks;
+  if (jd) {
+    for (int j = 0; j < naggs; j++)
+      if (aggs[j].kind != VQ_AGG_COUNT)
+        if (int rc = add_expr(t, aggs[j].arg, sm, jd->agg[j].e)) return rc;
+  }
+  fill_slot_cols(t, sm, sd.slot);
+  sd.nslots = sm.n;
+  sd.tile_rows = 256 * 4 * 16;
+  return VQ_OK;
+}
+
+}  // namespace
+
+extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
+                           const vq_pred* build_pred, const vq_table* probe, int32_t probe_key,
+                           const vq_pred* probe_pred, int32_t naggs, const vq_agg* aggs,
+                           int32_t radix_bits, int32_t topk, vq_agg_result* out, vq_stats* stats) {
+  if (!c || !build || !probe || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (build->ctx != c || probe->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs)) return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
+  if (topk < 0 || topk > 64) return fail(VQ_E_UNSUPPORTED, "exec: gpu: top-k of %d", topk);
+  if (radix_bits > 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d radix bits", radix_bits);
+  JDesc jd;
+  memset(&jd, 0, sizeof jd);
+  jd.naggs = naggs;
+  JSide bs, ps;
+  SlotMap bsm, psm;
+  if (int rc = plan_side(build, build_key, build_pred, nullptr, 0, bs, nullptr, bsm)) return rc;
+  if (int rc = plan_side(probe, probe_key, probe_pred, aggs, naggs, ps, &jd, psm)) return rc;
+  // accumulator words and fixed-point scales
+  const int L = ilog2_ceil((uint64_t)probe->nrows + 1);
+  int nw = 0, nvals = 0;
+  for (int j = 0; j < naggs; j++) {
+    if (int rc = check_agg(aggs[j])) return rc;
+    JAgg& a = jd.agg[j];
+    a.kind = aggs[j].kind;
+    a.accum = aggs[j].accum;
+    a.word0 = nw;
+    if (a.kind == VQ_AGG_COUNT) {
+      a.nwords = 0;
+      continue;
+    }
+    nvals++;
+    if ((a.kind == VQ_AGG_SUM || a.kind == VQ_AGG_AVG) && aggs[j].arg.is_float) {
+      Interval iv = expr_interval(probe, aggs[j].arg);
+      if (!iv.ok)
+        return fail(VQ_E_UNSUPPORTED, "exec: gpu: aggregate %d has no finite bound for exact summation", j);
+      double m = std::max(std::fabs(iv.lo), std::fabs(iv.hi)) * (1 + 1e-9);
+      int E = 0;
+      if (m > 0) {
+        std::frexp(m, &E);  // m < 2^E
+      }
+      const int w = std::min(40, 62 - L);
+      if (w < 20) return fail(VQ_E_UNSUPPORTED, "exec: gpu: probe side too large for exact sums");
+      a.fs.w = w;
+      a.fs.lsb = E - 3 * w;
+      a.nwords = 3;
+    } else {
+      a.nwords = 1;
+    }
+    nw += a.nwords;
+  }
+  jd.nwords = nw;
+
+  // partitioning plan: partitions small enough for an LDS table
+  const int64_t est = build->nrows;  // footer row count, as buildSideRadixOptions uses
+  int bits = radix_bits >= 0 ? radix_bits : std::max(0, ilog2_ceil((uint64_t)(est + 511) / 512));
+  const size_t slot_bytes = 8 + 8 * (size_t)nw + 12;
+
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  for (int attempt = 0; attempt < 8; attempt++) {
+    JoinPlan plan;
+    memset(&plan, 0, sizeof plan);
+    plan.bits = bits;
+    plan.P = 1 << bits;
+    const double per = (double)est / plan.P;
+    int64_t cap = 8;
+    while (cap < (int64_t)std::ceil(per * 1.5 / 0.7) + 1) cap *= 2;
+    while (cap > 1024 || (size_t)(cap + 1) * slot_bytes > 64 * 1024) {
+      if (bits >= 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: build side needs more than 2^16 partitions");
+      bits++;
+      plan.bits = bits;
+      plan.P = 1 << bits;
+      cap = 8;
+      const double per2 = (double)est / plan.P;
+      while (cap < (int64_t)std::ceil(per2 * 1.5 / 0.7) + 1) cap *= 2;
+    }
+    plan.cap = (int32_t)cap;
+    plan.topk = topk;
+    plan.nvals = nvals;
+    plan.b_rows_total = build->nrows;
+    plan.p_rows_total = probe->nrows;
+    const int P = plan.P;
+
+    std::vector<uint8_t> bkeep, pkeep;
+    prune(build, build_pred, bkeep);
+    prune(probe, probe_pred, pkeep);
+    int64_t brows = 0, bkept = 0, prows = 0, pkept = 0;
+    if (int rc = make_spans(build, bkeep, bs.tile_rows, bs.span, &bs.nspans, &bs.ntiles, &brows, &bkept)) return rc;
+    if (int rc = make_spans(probe, pkeep, ps.tile_rows, ps.span, &ps.nspans, &ps.ntiles, &prows, &pkept)) return rc;
+
+    const int RW = 3 + naggs;
+    const int64_t full_cap = topk > 0 ? topk : std::max<int64_t>(1, std::min(brows, prows));
+    JoinBufs B;
+    memset(&B, 0, sizeof B);
+    B.cap_b = std::max<int64_t>(brows, 1);
+    B.cap_p = std::max<int64_t>(prows, 1);
+    const size_t nbt = (size_t)std::max<int64_t>(bs.ntiles, 1) * P;
+    const size_t npt = (size_t)std::max<int64_t>(ps.ntiles, 1) * P;
+    size_t bytes = 0;
+    bytes += B.cap_b * 8 + nbt * 8 + (2 * P + 2) * 4;
+    bytes += B.cap_p * 8 + B.cap_p * 4 + (size_t)std::max(nvals, 1) * B.cap_p * 8 + npt * 8 + (2 * P + 2) * 4;
+    bytes += (size_t)P * std::max(topk, 1) * RW * 8 + (size_t)full_cap * RW * 8 + 64;
+    bytes += 16 * 256;
+    void* ar;
+    if (int rc = arena_get(c, bytes, &ar)) return rc;
+    Carve cv;
+    char* base = (char*)ar;
+    B.b_keys = cv.take<int64_t>(base, B.cap_b);
+    B.b_counts = cv.take<uint32_t>(base, nbt);
+    B.b_offsets = cv.take<uint32_t>(base, nbt);
+    B.b_pstart = cv.take<uint32_t>(base, 2 * P + 2);
+    B.p_keys = cv.take<int64_t>(base, B.cap_p);
+    B.p_rows = cv.take<uint32_t>(base, B.cap_p);
+    B.p_vals = cv.take<uint64_t>(base, (size_t)std::max(nvals, 1) * B.cap_p);
+    B.p_counts = cv.take<uint32_t>(base, npt);
+    B.p_offsets = cv.take<uint32_t>(base, npt);
+    B.p_pstart = cv.take<uint32_t>(base, 2 * P + 2);
+    B.cand = cv.take<uint64_t>(base, (size_t)P * std::max(topk, 1) * RW);
+    B.out = cv.take<uint64_t>(base, (size_t)full_cap * RW);
+    B.flags = cv.take<uint32_t>(base, 16);
+
+    HIP_TRY(launch_join(bs, ps, jd, plan, B, c->stream, c->ev));
+    // read back flags, build total and the records
+    const size_t rec_bytes = (size_t)full_cap * RW * 8;
+    void* hp;
+    if (int rc = pinned_get(c, rec_bytes + 64, &hp)) return rc;
+    uint32_t* hflags = (uint32_t*)hp;
+    uint64_t* hrec = (uint64_t*)((char*)hp + 64);
+    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hflags + 4, B.b_pstart + P, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hflags[0]) {  // a partition table overflowed: repartition finer
+      if (bits >= 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: join partitions overflow at 2^16");
+      bits++;
+      continue;
+    }
+    const uint32_t nrec = hflags[1];
+    if (nrec > (uint64_t)full_cap) return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)full_cap);
+    if (nrec) HIP_TRY(hipMemcpy(hrec, B.out, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
+    if (stats) {
+      memset(stats, 0, sizeof *stats);
+      stats->device_ms = elapsed(c->ev[0], c->ev[3]);
+      stats->main_kernel_ms = elapsed(c->ev[0], c->ev[1]);
+      stats->rows_scanned = brows + prows;
+      stats->rg_total = build->nrg + probe->nrg;
+      stats->rg_scanned = bkept + pkept;
+      stats->bytes_alg = brows * slot_bytes_per_row(build, bsm) + prows * slot_bytes_per_row(probe, psm);
+      stats->build_rows = hflags[4];
+      stats->build_keys = -1;
+      stats->partitions = P;
+      stats->radix_bits = bits;
+    }
+    // order: rank order for top-k; first-occurrence order otherwise
+    std::vector<std::pair<uint64_t, uint32_t>> ord(nrec);
+    for (uint32_t i = 0; i < nrec; i++) ord[i] = {topk > 0 ? (uint64_t)i : hrec[(size_t)i * RW + 2], i};
+    if (topk <= 0) std::sort(ord.begin(), ord.end());
+    out->ngroups = nrec;
+    if ((int64_t)nrec > out->capacity)
+      return fail(VQ_E_ARG, "exec: gpu: %u groups exceed result capacity %lld", nrec, (long long)out->capacity);
+    for (uint32_t i = 0; i < nrec; i++) {
+      const uint64_t* r = &hrec[(size_t)ord[i].second * RW];
+      out->keys[i] = (int64_t)r[0];
+      out->counts[i] = (int64_t)r[1];
+      out->first_row[i] = (int64_t)r[2];
+      for (int j = 0; j < naggs; j++) {
+        const int64_t o = (int64_t)i * naggs + j;
+        out->valid[o] = 1;
+        out->ivals[o] = 0;
+        out->fvals[o] = 0;
+        const bool isf = aggs[j].kind == VQ_AGG_AVG || (aggs[j].kind != VQ_AGG_COUNT && aggs[j].accum == VQ_F64);
+        if (isf) out->fvals[o] = bits_to_f64((int64_t)r[3 + j]);
+        else out->ivals[o] = (int64_t)r[3 + j];
+      }
+    }
+    return VQ_OK;
+  }
+  return fail(VQ_E_UNSUPPORTED, "exec: gpu: join partitioning did not converge");
+}

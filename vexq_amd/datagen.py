@@ -1,0 +1,270 @@
+"""Seeded synthetic tables for the hot-path workloads.
+
+Every value is a pure function of (seed, stream, row index) through the
+splitmix64 finaliser, so a table is identical on every machine and any slice
+can be regenerated without the rest.  Shapes follow the workloads the
+reference benchmarks on (bench/parallel/bench_test.go:110-180 generates a
+6M-row lineitem-like table the same way, from seed 42):
+
+* ``lineitem``  — flagA u8 dictionary code (3 values, weights 1/4,1/4,1/2),
+  flagB u8 code (2 values), quantity int64 U{1..50},
+  price float64 = round_to_cents(quantity * U{900.00..104950.00} / 100),
+  discount float64 = U{0..10}/100, tax float64 = U{0..8}/100,
+  shipdate int32 days U{8035..10560} (2,526 days).
+* ``orders`` (join build) — key int64 = 1 + 4 i, orderdate int32 days
+  U{8035..10440} (2,406 days), priority u8 code (5 values).
+* ``lineitem_probe`` (join probe) — fk drawn so each build key gets 1..7 rows,
+  shipdate = orderdate(fk) + U{1..121}, quantity/price/discount as lineitem.
+
+Dictionary columns carry global codes in first-occurrence order, the order
+storage.DictBuilder assigns (storage/codec.go:55-63).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_STREAM = np.uint64(0xD1B54A32D192ED03)
+
+DATE_MIN = 8035            # 1992-01-01 in days since 1970-01-01
+SHIP_DAYS = 2526           # lineitem shipdate range
+ORDER_DAYS = 2406          # orders orderdate range
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def rand_u64(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
+    """n pseudo-random uint64 values for rows [offset, offset+n) of a stream."""
+    with np.errstate(over="ignore"):
+        base = _mix(np.array([seed], dtype=np.uint64) * _GOLDEN
+                    + np.array([stream], dtype=np.uint64) * _STREAM)[0]
+        i = np.arange(offset, offset + n, dtype=np.uint64) + np.uint64(1)
+        return _mix(base + i * _GOLDEN)
+
+
+def rand_int(seed, stream, n, lo, hi, offset=0) -> np.ndarray:
+    """Uniform integers in [lo, hi] (int64)."""
+    r = rand_u64(seed, stream, n, offset)
+    return (r % np.uint64(hi - lo + 1)).astype(np.int64) + lo
+
+
+def first_occurrence_codes(values: np.ndarray, names):
+    """Map value indices to global codes in first-occurrence order."""
+    uniq, first = np.unique(values, return_index=True)
+    order = uniq[np.argsort(first, kind="stable")]
+    remap = np.zeros(int(values.max()) + 1 if values.size else 1, dtype=np.int64)
+    remap[order] = np.arange(order.size)
+    codes = remap[values] if values.size else values
+    dictionary = [names[v] for v in order]
+    return codes, dictionary
+
+
+@dataclass
+class Column:
+    name: str
+    type: int
+    data: np.ndarray
+    dict_size: int = 0
+    dictionary: list = field(default_factory=list)
+
+
+@dataclass
+class Table:
+    columns: list
+    rg_rows: np.ndarray
+
+    @property
+    def nrows(self) -> int:
+        return int(self.columns[0].data.shape[0]) if self.columns else 0
+
+    def index(self, name: str) -> int:
+        for i, c in enumerate(self.columns):
+            if c.name == name:
+                return i
+        raise KeyError(name)
+
+    def col(self, name: str) -> np.ndarray:
+        return self.columns[self.index(name)].data
+
+    def take(self, order: np.ndarray) -> "Table":
+        cols = [Column(c.name, c.type, np.ascontiguousarray(c.data[order]), c.dict_size,
+                       list(c.dictionary)) for c in self.columns]
+        return Table(cols, row_groups(len(order)))
+
+
+def row_groups(nrows: int, rg: int = abi.ROW_GROUP_ROWS) -> np.ndarray:
+    n_full, rem = divmod(nrows, rg)
+    sizes = [rg] * n_full + ([rem] if rem else [])
+    return np.array(sizes, dtype=np.int64)
+
+
+_NP = {abi.VQ_U8: np.uint8, abi.VQ_U16: np.uint16, abi.VQ_DATE: np.int32,
+       abi.VQ_I64: np.int64, abi.VQ_F64: np.float64}
+
+
+def _col(name, typ, data, dictionary=None):
+    arr = np.ascontiguousarray(data, dtype=_NP[typ])
+    d = list(dictionary or [])
+    return Column(name, typ, arr, len(d), d)
+
+
+def _price(seed, stream_base, quantity, n, offset=0):
+    # quantity * uniform(900.00, 104950.00) / 100, rounded to cents
+    unit_cents = rand_int(seed, stream_base, n, 90000, 10495000, offset)
+    cents = (quantity * unit_cents + 50) // 100
+    return cents.astype(np.float64) / 100.0
+
+
+def lineitem(nrows: int, seed: int = 42, offset: int = 0) -> Table:
+    """The dict-coded GROUP BY table (8 columns, 38 referenced bytes/row)."""
+    n = nrows
+    a = rand_u64(seed, 1, n, offset) % np.uint64(4)
+    a = np.minimum(a, np.uint64(2)).astype(np.int64)        # 1/4, 1/4, 1/2
+    flag_a, dict_a = first_occurrence_codes(a, ["A", "N", "R"])
+    b = (rand_u64(seed, 2, n, offset) % np.uint64(2)).astype(np.int64)
+    flag_b, dict_b = first_occurrence_codes(b, ["F", "O"])
+    quantity = rand_int(seed, 3, n, 1, 50, offset)
+    price = _price(seed, 4, quantity, n, offset)
+    discount = rand_int(seed, 5, n, 0, 10, offset).astype(np.float64) / 100.0
+    tax = rand_int(seed, 6, n, 0, 8, offset).astype(np.float64) / 100.0
+    shipdate = rand_int(seed, 7, n, DATE_MIN, DATE_MIN + SHIP_DAYS - 1, offset)
+    cols = [
+        _col("flagA", abi.VQ_U8, flag_a, dict_a),
+        _col("flagB", abi.VQ_U8, flag_b, dict_b),
+        _col("quantity", abi.VQ_I64, quantity),
+        _col("price", abi.VQ_F64, price),
+        _col("discount", abi.VQ_F64, discount),
+        _col("tax", abi.VQ_F64, tax),
+        _col("shipdate", abi.VQ_DATE, shipdate),
+    ]
+    return Table(cols, row_groups(n))
+
+
+def lineitem_sorted(nrows: int, seed: int = 42) -> Table:
+    """lineitem written sorted by shipdate (zone maps then prune windows)."""
+    t = lineitem(nrows, seed)
+    order = np.argsort(t.col("shipdate"), kind="stable")
+    return t.take(order)
+
+
+def orders(nrows: int, seed: int = 42) -> Table:
+    """Join build side: unique keys 1 + 4 i."""
+    n = nrows
+    key = 1 + 4 * np.arange(n, dtype=np.int64)
+    orderdate = rand_int(seed, 11, n, DATE_MIN, DATE_MIN + ORDER_DAYS - 1)
+    pr = (rand_u64(seed, 12, n) % np.uint64(5)).astype(np.int64)
+    prio, dict_p = first_occurrence_codes(pr, ["1-URGENT", "2-HIGH", "3-MEDIUM",
+                                               "4-NOT SPECIFIED", "5-LOW"])
+    cols = [
+        _col("key", abi.VQ_I64, key),
+        _col("orderdate", abi.VQ_DATE, orderdate),
+        _col("priority", abi.VQ_U8, prio, dict_p),
+    ]
+    return Table(cols, row_groups(n))
+
+
+def probe_counts(nbuild: int, nprobe: int, seed: int = 42) -> np.ndarray:
+    """Rows per build key: U{1..7}, adjusted (in key order) to sum to nprobe
+    while staying inside [1, 7]."""
+    c = rand_int(seed, 13, nbuild, 1, 7)
+    diff = int(nprobe - c.sum())
+    if diff > 0:
+        room = 7 - c
+        cum = np.cumsum(room)
+        k = int(np.searchsorted(cum, diff))
+        if k >= nbuild:
+            raise ValueError("nprobe exceeds 7 rows per key")
+        add = room.copy()
+        add[k + 1:] = 0
+        add[k] = diff - (cum[k - 1] if k else 0)
+        c = c + add
+    elif diff < 0:
+        room = c - 1
+        cum = np.cumsum(room)
+        k = int(np.searchsorted(cum, -diff))
+        if k >= nbuild:
+            raise ValueError("nprobe below 1 row per key")
+        sub = room.copy()
+        sub[k + 1:] = 0
+        sub[k] = -diff - (cum[k - 1] if k else 0)
+        c = c - sub
+    return c
+
+
+def lineitem_probe(build: Table, nrows: int, seed: int = 42) -> Table:
+    """Join probe side: each build key appears 1..7 times, shuffled."""
+    nb = build.nrows
+    counts = probe_counts(nb, nrows, seed)
+    brow = np.repeat(np.arange(nb, dtype=np.int64), counts)
+    perm = np.argsort(rand_u64(seed, 14, nrows), kind="stable")
+    brow = brow[perm]
+    fk = build.col("key")[brow]
+    shipdate = build.col("orderdate")[brow].astype(np.int64) + rand_int(seed, 15, nrows, 1, 121)
+    quantity = rand_int(seed, 16, nrows, 1, 50)
+    price = _price(seed, 17, quantity, nrows)
+    discount = rand_int(seed, 18, nrows, 0, 10).astype(np.float64) / 100.0
+    cols = [
+        _col("fk", abi.VQ_I64, fk),
+        _col("shipdate", abi.VQ_DATE, shipdate),
+        _col("quantity", abi.VQ_I64, quantity),
+        _col("price", abi.VQ_F64, price),
+        _col("discount", abi.VQ_F64, discount),
+    ]
+    return Table(cols, row_groups(nrows))
+
+
+# ---- the workload queries ---------------------------------------------------
+
+def q_groupby(t: Table):
+    """Filtered dict-coded GROUP BY (flagA, flagB) with eight aggregates:
+    SUM(quantity), SUM(price), SUM(price*(1-discount)),
+    SUM(price*(1-discount)*(1+tax)), AVG(quantity), AVG(price), AVG(discount),
+    COUNT(*); predicate shipdate <= max - 90."""
+    X = abi.X
+    pred = abi.make_pred(t, [("shipdate", "<=", DATE_MIN + SHIP_DAYS - 1 - 90)])
+    disc_price = X.col("price") * (1 - X.col("discount"))
+    charge = X.col("price") * (1 - X.col("discount")) * (1 + X.col("tax"))
+    aggs = [
+        abi.agg_sum("quantity", t), abi.agg_sum("price", t), abi.agg_sum(disc_price, t),
+        abi.agg_sum(charge, t), abi.agg_avg("quantity", t), abi.agg_avg("price", t),
+        abi.agg_avg("discount", t), abi.count_star(t),
+    ]
+    keys = [t.index("flagA"), t.index("flagB")]
+    return pred, keys, aggs
+
+
+def q_scan_sum(t: Table, seed: int = 42, which: int = 0):
+    """Range-predicate scan: shipdate in a random 365-day window, discount
+    BETWEEN d-0.01 AND d+0.01 (d = k/100, k in 2..9), quantity < 24;
+    SUM(price*discount)."""
+    r = rand_u64(seed, 100 + which, 2)
+    lo = DATE_MIN + int(r[0] % np.uint64(SHIP_DAYS - 365))
+    d = int(2 + r[1] % np.uint64(8)) / 100.0
+    X = abi.X
+    pred = abi.make_pred(t, [("shipdate", ">=", lo), ("shipdate", "<", lo + 365),
+                             ("discount", ">=", d - 0.01), ("discount", "<=", d + 0.01),
+                             ("quantity", "<", 24)])
+    aggs = [abi.agg_sum(X.col("price") * X.col("discount"), t)]
+    return pred, [], aggs
+
+
+JOIN_CUTOFF = DATE_MIN + 1155      # orderdate < cutoff keeps ~48% of orders
+
+
+def q_join(build: Table, probe: Table, cutoff: int = JOIN_CUTOFF):
+    """orders JOIN lineitem ON key = fk WHERE orderdate < c AND shipdate > c
+    GROUP BY key, SUM(price*(1-discount)), ORDER BY 2 DESC LIMIT 10."""
+    X = abi.X
+    bpred = abi.make_pred(build, [("orderdate", "<", cutoff)])
+    ppred = abi.make_pred(probe, [("shipdate", ">", cutoff)])
+    aggs = [abi.agg_sum(X.col("price") * (1 - X.col("discount")), probe)]
+    return bpred, ppred, aggs

@@ -1,0 +1,173 @@
+"""Host binding of libvexqgpu.so (the gfx950 operators) through its C ABI.
+
+The library must have been built (``python -m vexq_amd.build``); loading it is
+mandatory and a missing build raises at import of this module's first use.
+There is no CPU fallback here: every call runs the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libvexqgpu.so")
+_lib = None
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{abi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded C ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"{_LIB_PATH} is not built: run `python -m vexq_amd.build`")
+    L = C.CDLL(_LIB_PATH)
+    P = C.POINTER
+    L.vq_last_error.restype = C.c_char_p
+    L.vq_ctx_create.argtypes = [C.c_int, C.c_uint64, P(C.c_void_p)]
+    L.vq_ctx_destroy.argtypes = [C.c_void_p]
+    L.vq_ctx_bytes_in_use.argtypes = [C.c_void_p]
+    L.vq_ctx_bytes_in_use.restype = C.c_uint64
+    L.vq_table_upload.argtypes = [C.c_void_p, C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32,
+                                  P(C.c_int64), P(abi.Zone), P(C.c_void_p)]
+    L.vq_table_destroy.argtypes = [C.c_void_p]
+    L.vq_table_rows.argtypes = [C.c_void_p]
+    L.vq_table_rows.restype = C.c_int64
+    L.vq_table_zones.argtypes = [C.c_void_p, P(abi.Zone), C.c_int64]
+    L.vq_zone_prune.argtypes = [C.c_void_p, P(abi.Pred), P(C.c_uint8)]
+    L.vq_filter.argtypes = [C.c_void_p, C.c_void_p, P(abi.Pred), P(C.c_void_p), P(abi.Stats)]
+    L.vq_selvec_count.argtypes = [C.c_void_p]
+    L.vq_selvec_count.restype = C.c_int64
+    L.vq_selvec_copy.argtypes = [C.c_void_p, P(C.c_uint32), C.c_int64]
+    L.vq_selvec_destroy.argtypes = [C.c_void_p]
+    L.vq_agg_dict.argtypes = [C.c_void_p, C.c_void_p, P(abi.Pred), C.c_int32, P(C.c_int32),
+                              C.c_int32, P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
+    L.vq_join_agg.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_void_p,
+                              C.c_int32, P(abi.Pred), C.c_int32, P(abi.Agg), C.c_int32,
+                              C.c_int32, P(abi.AggResult), P(abi.Stats)]
+    L.vq_radix_bits_for.argtypes = [C.c_int64]
+    L.vq_radix_bits_for.restype = C.c_int32
+    L.vq_radix_hash.argtypes = [C.c_int64]
+    L.vq_radix_hash.restype = C.c_uint64
+    L.vq_join_table_capacity.argtypes = [C.c_int64]
+    L.vq_join_table_capacity.restype = C.c_int64
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != abi.VQ_OK:
+        raise GpuError(rc, lib().vq_last_error().decode())
+
+
+def _pp(x):
+    return C.byref(x) if x is not None else None
+
+
+class Context:
+    """vq_ctx: one device, one stream, a device-memory budget."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0):
+        h = C.c_void_p()
+        _check(lib().vq_ctx_create(device, budget_bytes, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().vq_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def bytes_in_use(self) -> int:
+        return int(lib().vq_ctx_bytes_in_use(self.h))
+
+    def upload(self, table, zones=None) -> "DeviceTable":
+        return DeviceTable(self, table, zones)
+
+    # -- operators ----------------------------------------------------------
+    def filter(self, dt: "DeviceTable", pred) -> np.ndarray:
+        sv = C.c_void_p()
+        st = abi.Stats()
+        _check(lib().vq_filter(self.h, dt.h, _pp(pred), C.byref(sv), C.byref(st)))
+        try:
+            n = lib().vq_selvec_count(sv)
+            rows = np.empty(max(n, 0), dtype=np.uint32)
+            _check(lib().vq_selvec_copy(sv, rows.ctypes.data_as(C.POINTER(C.c_uint32)), n))
+        finally:
+            lib().vq_selvec_destroy(sv)
+        self.last_stats = st
+        return rows
+
+    def agg_dict(self, dt: "DeviceTable", pred, key_cols, aggs, capacity: int = 4096) -> abi.AggOut:
+        keys = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
+        arr = abi.agg_array(aggs)
+        out = abi.AggOut(capacity, len(key_cols), len(aggs))
+        st = abi.Stats()
+        _check(lib().vq_agg_dict(self.h, dt.h, _pp(pred), len(key_cols), keys, len(aggs), arr,
+                                 C.byref(out.c), C.byref(st)))
+        out.stats = st
+        return out
+
+    def join_agg(self, build: "DeviceTable", build_key: int, build_pred, probe: "DeviceTable",
+                 probe_key: int, probe_pred, aggs, radix_bits: int = -1, topk: int = 0,
+                 capacity: int | None = None) -> abi.AggOut:
+        arr = abi.agg_array(aggs)
+        if capacity is None:
+            capacity = topk if topk > 0 else max(1, min(build.nrows, probe.nrows))
+        out = abi.AggOut(capacity, 1, len(aggs))
+        st = abi.Stats()
+        _check(lib().vq_join_agg(self.h, build.h, build_key, _pp(build_pred), probe.h, probe_key,
+                                 _pp(probe_pred), len(aggs), arr, radix_bits, topk,
+                                 C.byref(out.c), C.byref(st)))
+        out.stats = st
+        return out
+
+
+class DeviceTable:
+    """vq_table: columns resident in HBM after one bulk upload."""
+
+    def __init__(self, ctx: Context, table, zones=None):
+        self.ctx = ctx
+        self.table = table
+        self.nrows = table.nrows
+        self.nrg = len(table.rg_rows)
+        descs = abi.col_descs(table)
+        rg, rgp = abi.rg_array(table)
+        zp = None
+        if zones is not None:
+            zp = zones.ctypes.data_as(C.POINTER(abi.Zone)) if hasattr(zones, "ctypes") else zones
+        h = C.c_void_p()
+        _check(lib().vq_table_upload(ctx.h, table.nrows, len(table.columns), descs, len(rg), rgp,
+                                     zp, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().vq_table_destroy(self.h)
+            self.h = None
+
+    def zones(self):
+        n = self.nrg * len(self.table.columns)
+        z = (abi.Zone * max(n, 1))()
+        _check(lib().vq_table_zones(self.h, z, n))
+        return z
+
+    def prune(self, pred) -> np.ndarray:
+        keep = np.zeros(max(self.nrg, 1), dtype=np.uint8)
+        _check(lib().vq_zone_prune(self.h, _pp(pred), keep.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return keep[: self.nrg]
