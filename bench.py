@@ -179,7 +179,7 @@ def main():
                 pass
 
             def filt():
-                ctx.filter(dts, pred)
+                ctx.filter_count(dts, pred)
                 o = _R()
                 o.stats = ctx.last_stats
                 return o

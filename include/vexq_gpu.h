@@ -181,6 +181,8 @@ typedef struct vq_stats {
   int64_t build_keys;    /* join: distinct build keys   (JoinBuildStats.Keys) */
   int32_t partitions;    /* join: radix partitions (JoinBuildStats.Partitions) */
   int32_t radix_bits;
+  int32_t grid;          /* workgroups of the dominant kernel */
+  int32_t tile_rows;     /* rows per tile of the scan */
 } vq_stats;
 
 typedef struct vq_ctx vq_ctx;

@@ -15,6 +15,17 @@ pytestmark = pytest.mark.gpu
 REL_TOL = 1e-12
 
 
+@pytest.fixture(params=["jit", "generic"])
+def kernel_path(request, monkeypatch):
+    """Runs a test through the shape-specialized (run-time compiled) kernels and
+    through the generic descriptor-driven ones."""
+    if request.param == "generic":
+        monkeypatch.setenv("VQ_NO_JIT", "1")
+    else:
+        monkeypatch.delenv("VQ_NO_JIT", raising=False)
+    return request.param
+
+
 def _print_case(name, **kw):
     print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
 
@@ -46,8 +57,8 @@ def same_bits(a, b):
 # Filter -> selection vector
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("nrows", [1, 7, 1023, 1025, 65536, 100_003, 400_000])
-def test_filter_matches_oracle(gpu_ctx, nrows):
-    _print_case("filter", nrows=nrows)
+def test_filter_matches_oracle(gpu_ctx, nrows, kernel_path):
+    _print_case("filter", nrows=nrows, path=kernel_path)
     t = datagen.lineitem(nrows, seed=7)
     dt = gpu_ctx.upload(t)
     try:
@@ -105,8 +116,8 @@ def test_filter_matches_reference_avx2_kernel(gpu_ctx, n):
 # Dictionary-code GROUP BY
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("nrows", [1, 1000, 65537, 1_000_000])
-def test_groupby_matches_oracle(gpu_ctx, nrows):
-    _print_case("groupby", nrows=nrows)
+def test_groupby_matches_oracle(gpu_ctx, nrows, kernel_path):
+    _print_case("groupby", nrows=nrows, path=kernel_path)
     t = datagen.lineitem(nrows)
     pred, keys, aggs = datagen.q_groupby(t)
     dt = gpu_ctx.upload(t)
@@ -117,8 +128,10 @@ def test_groupby_matches_oracle(gpu_ctx, nrows):
         dt.close()
     ref = O.agg_dict(t, pred, keys, aggs, exact=True)
     assert_agg_equal(got, ref, aggs)
-    # the reference's own serial row-order float sums agree within the bar too
-    assert_agg_equal(got, O.agg_dict(t, pred, keys, aggs, exact=False), aggs)
+    # The serial plan's row-order left fold drifts from the exact sum by more
+    # than 1e-12 at 1M rows; against it the reference's own float tolerance
+    # applies (FloatEpsilon 1e-9, internal/goldentest/reference.go:994).
+    assert_agg_equal(got, O.agg_dict(t, pred, keys, aggs, exact=False), aggs, rel=1e-9)
     assert same_bits(got, again)
 
 
@@ -136,7 +149,7 @@ def test_groupby_6m_bitwise_stable(gpu_ctx):
     assert same_bits(runs[0], runs[1]) and same_bits(runs[0], runs[2])
 
 
-def test_groupby_variants(gpu_ctx):
+def test_groupby_variants(gpu_ctx, kernel_path):
     """Single key, three keys, MIN/MAX, integer arithmetic, empty results."""
     t = datagen.lineitem(300_000, seed=11)
     X = abi.X
@@ -165,7 +178,7 @@ def test_groupby_variants(gpu_ctx):
 # ---------------------------------------------------------------------------
 # Zone-pruned scan + SUM(expr)
 # ---------------------------------------------------------------------------
-def test_scan_sum_pruned(gpu_ctx):
+def test_scan_sum_pruned(gpu_ctx, kernel_path):
     t = datagen.lineitem_sorted(6_000_000)
     dt = gpu_ctx.upload(t)
     try:

@@ -73,7 +73,8 @@ class Stats(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("main_kernel_ms", C.c_double),
                 ("rows_scanned", C.c_int64), ("rg_total", C.c_int64), ("rg_scanned", C.c_int64),
                 ("bytes_alg", C.c_int64), ("build_rows", C.c_int64), ("build_keys", C.c_int64),
-                ("partitions", C.c_int32), ("radix_bits", C.c_int32)]
+                ("partitions", C.c_int32), ("radix_bits", C.c_int32), ("grid", C.c_int32),
+                ("tile_rows", C.c_int32)]
 
 
 # ---- predicate builder -----------------------------------------------------

@@ -17,7 +17,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvexqgpu.so")
-SOURCES = ["vq_api.hip", "k_scan.hip", "k_join.hip"]
+SOURCES = ["vq_api.hip", "k_scan.hip", "k_join.hip", "vq_jit.hip"]
+JIT_TEMPLATE = "jit_template.hip"
+JIT_HEADER = "jit_template_src.h"  # generated: the template as a string literal
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VQ_OFFLOAD_ARCH", "gfx950")
@@ -30,10 +32,37 @@ FLAGS = [
 ]
 
 
+def embed_template() -> None:
+    """Writes csrc/jit_template_src.h: jit_template.hip as a C++ raw string."""
+    with open(os.path.join(CSRC, JIT_TEMPLATE)) as f:
+        text = f.read()
+    assert ")VQJIT\"" not in text
+    out = ("// generated by vexq_amd/build.py from jit_template.hip; do not edit\n"
+           "#pragma once\n"
+           f"static const char kJitTemplateSrc[] = R\"VQJIT({text})VQJIT\";\n")
+    path = os.path.join(CSRC, JIT_HEADER)
+    if not os.path.exists(path) or open(path).read() != out:
+        with open(path, "w") as f:
+            f.write(out)
+
+
+def check_template(verbose: bool = False) -> None:
+    """Compiles the run-time template once with its built-in default shape, so
+    a syntax error surfaces at build time rather than at the first query."""
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O1", "-std=c++17", "-c", "--cuda-device-only",
+           "-ffp-contract=off", "-o", os.devnull, os.path.join(CSRC, JIT_TEMPLATE)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("run-time kernel template does not compile")
+
+
 def build(verbose: bool = False) -> str:
+    embed_template()
+    check_template(verbose)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs, "-lhiprtc"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,0 +1,457 @@
+// jit_template.hip — the row program of one query shape, compiled at run
+// time with hipRTC for gfx950 and cached by shape (vq_jit.cpp).
+//
+// A generated prologue fixes the shape as compile-time constants: the type of
+// every referenced column, each predicate term's column and comparison, the
+// group-key strides, and every aggregate's left fold of factors.  Literal
+// values stay kernel arguments, so queries that differ only in constants share
+// one compiled kernel.  With the shape known, every loop below unrolls and
+// every switch folds: a row costs its loads, its compares and its arithmetic,
+// with no descriptor interpretation on the scalar unit.
+//
+// The kernels follow the same contracts as the static ones in k_scan.hip:
+//   vq_dense       per-wave partial aggregates (persistent grid, prefetch)
+//   vq_filter_bits pass bits + per-tile counts for the selection vector
+//   vq_part_count / vq_part_scatter   radix partitioning of a join side
+#ifndef VQ_JIT_PROLOGUE
+// Stand-alone build check (hipcc -c): the 8-aggregate GROUP BY shape.
+#define VQ_JIT_PROLOGUE 1
+#define VQ_NS 7
+#define VQ_U 4
+#define VQ_G 6
+#define VQ_NA 5
+#define VQ_NT 1
+#define VQ_KEYSLOT 0
+#define VQ_NV 0
+static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
+static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kKS[8] = {2, 1, 0, 0, 0, 0, 0, 0};
+static constexpr int kAOP[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kAISF[8] = {0, 1, 1, 1, 1, 0, 0, 0};
+static constexpr int kANF[8] = {1, 1, 2, 3, 1, 0, 0, 0};
+static constexpr int kFK[8][4] = {{0}, {0}, {0, 2}, {0, 2, 2}, {0}};
+static constexpr int kFS[8][4] = {{2}, {3}, {3, 4}, {3, 4, 5}, {4}};
+static constexpr int kFO[8][4] = {{0}, {0}, {0, 1}, {0, 1, 0}, {0}};
+static constexpr int kFC[8][4] = {{0}, {0}, {0, 2}, {0, 2, 2}, {0}};
+#include <hip/hip_runtime.h>
+#endif
+
+// No headers: hipRTC compiles this with its built-in runtime declarations only.
+typedef long long i64;
+typedef unsigned long long u64;
+typedef int i32;
+typedef unsigned int u32;
+typedef unsigned short u16;
+typedef unsigned char u8;
+constexpr i64 kI64Min = -0x7fffffffffffffffLL - 1;
+constexpr i64 kI64Max = 0x7fffffffffffffffLL;
+
+#define VQ_U8 1
+#define VQ_U16 2
+#define VQ_DATE 3
+#define VQ_I64 4
+#define VQ_F64 5
+
+namespace vqj {
+
+constexpr int kWave = 64;
+constexpr int kMaxSpans = 24;
+
+// compile-time index sequences (no <utility> under hipRTC)
+template <int... I>
+struct Seq {};
+template <int N, int... I>
+struct MakeSeq : MakeSeq<N - 1, N - 1, I...> {};
+template <int... I>
+struct MakeSeq<0, I...> {
+  using type = Seq<I...>;
+};
+template <int N>
+using seq = typename MakeSeq<N>::type;
+
+struct Args {
+  const void* col[8];
+  i64 ti[8];
+  double tf[8];
+  i64 fi[8][4];
+  double ff[8][4];
+  i64 sp_start[kMaxSpans];
+  i64 sp_rows[kMaxSpans];
+  i64 sp_tile0[kMaxSpans];
+  i32 nspans;
+  i32 tile_rows;
+  i64 ntiles;
+  u64* part;          // vq_dense: partials [grid][G][NA+2]
+  u64* bits;          // vq_filter_bits: pass words
+  u32* tile_counts;   // vq_filter_bits / vq_part_*: per-tile counts
+  // join partitioning
+  u32* offsets;       // [P][ntiles] exclusive offsets within a partition
+  const u32* pstart;  // [P + 1] partition starts
+  i64* keys;
+  u32* rows;
+  u64* vals;          // [NV][vstride]
+  i64 vstride;
+  i32 P;
+  i32 pad_;
+};
+
+// ---- column reads: the element type is a compile-time constant -------------
+template <int T>
+__device__ __forceinline__ i64 ld(const void* p, i64 r) {
+  if constexpr (T == VQ_U8) return ((const u8*)p)[r];
+  else if constexpr (T == VQ_U16) return ((const u16*)p)[r];
+  else if constexpr (T == VQ_DATE) return ((const i32*)p)[r];
+  else return ((const i64*)p)[r];
+}
+
+__device__ __forceinline__ double as_f64(i64 b) { return __longlong_as_double(b); }
+__device__ __forceinline__ i64 f64_bits(double d) { return __double_as_longlong(d); }
+
+template <int OP, class V>
+__device__ __forceinline__ bool cmp(V a, V b) {
+  if constexpr (OP == 0) return a == b;
+  else if constexpr (OP == 1) return a != b;
+  else if constexpr (OP == 2) return a < b;
+  else if constexpr (OP == 3) return a <= b;
+  else if constexpr (OP == 4) return a > b;
+  else return a >= b;
+}
+
+// applyArithFloat64 / applyArithInt64 (exec/expr.go:990-1020).
+template <int OP>
+__device__ __forceinline__ double arf(double a, double b) {
+  if constexpr (OP == 0) return a + b;
+  else if constexpr (OP == 1) return a - b;
+  else if constexpr (OP == 2) return a * b;
+  else return b == 0.0 ? __longlong_as_double(0x7ff8000000000000LL) : a / b;
+}
+
+template <int OP>
+__device__ __forceinline__ i64 ari(i64 a, i64 b) {
+  if constexpr (OP == 0) return (i64)((u64)a + (u64)b);
+  else if constexpr (OP == 1) return (i64)((u64)a - (u64)b);
+  else if constexpr (OP == 2) return (i64)((u64)a * (u64)b);
+  else return b == 0 ? 0 : ((a == kI64Min && b == -1) ? kI64Min : a / b);
+}
+
+template <int S>
+__device__ __forceinline__ double colf(i64 raw) {
+  if constexpr (kST[S] == VQ_F64) return as_f64(raw);
+  else return (double)raw;
+}
+
+// ---- the row program -----------------------------------------------------------
+template <int U, int... C>
+__device__ __forceinline__ void load_one(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u,
+                                         Seq<C...>) {
+  ((raw[C][u] = ld<kST[C]>(a.col[C], r)), ...);
+}
+
+template <int U>
+__device__ __forceinline__ void load_rows(const Args& a, i64 base, i64 rend, int lane,
+                                          i64 (&raw)[VQ_NS][U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    i64 r = base + u * kWave + lane;
+    r = r < rend ? r : rend - 1;
+    load_one(a, r, raw, u, seq<VQ_NS>{});
+  }
+}
+
+template <int T>
+__device__ __forceinline__ bool term(const Args& a, const i64* rawrow) {
+  constexpr int S = kTS[T];
+  if constexpr (kST[S] == VQ_F64) return cmp<kTO[T]>(as_f64(rawrow[S]), a.tf[T]);
+  else return cmp<kTO[T]>(rawrow[S], a.ti[T]);
+}
+
+template <int... T>
+__device__ __forceinline__ bool pass_all(const Args& a, const i64* rawrow,
+                                         Seq<T...>) {
+  return (true && ... && term<T>(a, rawrow));
+}
+
+template <int... C>
+__device__ __forceinline__ int group_of(const i64* rawrow, Seq<C...>) {
+  return (0 + ... + (kKS[C] * (int)rawrow[C]));
+}
+
+// One factor of aggregate J (float fold).
+template <int J, int K>
+__device__ __forceinline__ double factor_f(const Args& a, const i64* rawrow) {
+  constexpr int kind = kFK[J][K];
+  if constexpr (kind == 1) {
+    return a.ff[J][K];
+  } else {
+    const double c = colf<kFS[J][K]>(rawrow[kFS[J][K]]);
+    if constexpr (kind == 0) return c;
+    else if constexpr (kind == 2) return arf<kFO[J][K]>(a.ff[J][K], c);
+    else return arf<kFO[J][K]>(c, a.ff[J][K]);
+  }
+}
+
+template <int J, int K>
+__device__ __forceinline__ i64 factor_i(const Args& a, const i64* rawrow) {
+  constexpr int kind = kFK[J][K];
+  if constexpr (kind == 1) {
+    return a.fi[J][K];
+  } else {
+    const i64 c = rawrow[kFS[J][K]];
+    if constexpr (kind == 0) return c;
+    else if constexpr (kind == 2) return ari<kFO[J][K]>(a.fi[J][K], c);
+    else return ari<kFO[J][K]>(c, a.fi[J][K]);
+  }
+}
+
+// The left fold ((f0 c1 f1) c2 f2) c3 f3, one rounding per operation.
+template <int J, int K>
+__device__ __forceinline__ double fold_f(const Args& a, const i64* rawrow, double v) {
+  if constexpr (K >= kANF[J]) {
+    return v;
+  } else {
+    const double x = factor_f<J, K>(a, rawrow);
+    return fold_f<J, K + 1>(a, rawrow, K == 0 ? x : arf<kFC[J][K]>(v, x));
+  }
+}
+
+template <int J, int K>
+__device__ __forceinline__ i64 fold_i(const Args& a, const i64* rawrow, i64 v) {
+  if constexpr (K >= kANF[J]) {
+    return v;
+  } else {
+    const i64 x = factor_i<J, K>(a, rawrow);
+    return fold_i<J, K + 1>(a, rawrow, K == 0 ? x : ari<kFC[J][K]>(v, x));
+  }
+}
+
+// ---- accumulators ------------------------------------------------------------
+// ops: 0 SUM_F, 1 SUM_I, 2 MIN_F, 3 MAX_F, 4 MIN_I, 5 MAX_I
+__device__ __forceinline__ i64 f64_ord(double d) {
+  const i64 b = f64_bits(d);
+  return b >= 0 ? b : (i64)((u64)b ^ 0x7FFFFFFFFFFFFFFFULL);
+}
+
+template <int OP>
+__device__ __forceinline__ u64 acc_init() {
+  if constexpr (OP == 2) return (u64)f64_ord(1.7976931348623157e308);
+  else if constexpr (OP == 3) return (u64)f64_ord(-1.7976931348623157e308);
+  else if constexpr (OP == 4) return (u64)kI64Max;
+  else if constexpr (OP == 5) return (u64)kI64Min;
+  else return 0;
+}
+
+// Fold one row's value of aggregate J into register accumulator r.
+template <int J>
+__device__ __forceinline__ void fold_reg(u64& r, const Args& a, const i64* rawrow) {
+  constexpr int OP = kAOP[J];
+  if constexpr (kAISF[J]) {
+    const double v = fold_f<J, 0>(a, rawrow, 0.0);
+    if constexpr (OP == 0) {
+      r = (u64)f64_bits(as_f64((i64)r) + v);
+    } else if (v == v) {
+      const i64 o = f64_ord(v);
+      if constexpr (OP == 2) r = o < (i64)r ? (u64)o : r;
+      else r = o > (i64)r ? (u64)o : r;
+    }
+  } else {
+    const i64 v = fold_i<J, 0>(a, rawrow, 0);
+    if constexpr (OP == 1) r = (u64)((i64)r + v);
+    else if constexpr (OP == 4) r = v < (i64)r ? (u64)v : r;
+    else r = v > (i64)r ? (u64)v : r;
+  }
+}
+
+// Same, into this lane's private LDS word (no-return LDS atomic).
+template <int J>
+__device__ __forceinline__ void fold_lds(u64* w, const Args& a, const i64* rawrow) {
+  constexpr int OP = kAOP[J];
+  if constexpr (kAISF[J]) {
+    const double v = fold_f<J, 0>(a, rawrow, 0.0);
+    if constexpr (OP == 0) {
+      atomicAdd((double*)w, v);
+    } else if (v == v) {
+      if constexpr (OP == 2) atomicMin((long long*)w, (long long)f64_ord(v));
+      else atomicMax((long long*)w, (long long)f64_ord(v));
+    }
+  } else {
+    const i64 v = fold_i<J, 0>(a, rawrow, 0);
+    if constexpr (OP == 1) atomicAdd((unsigned long long*)w, (unsigned long long)v);
+    else if constexpr (OP == 4) atomicMin((long long*)w, (long long)v);
+    else atomicMax((long long*)w, (long long)v);
+  }
+}
+
+// ---- tile iteration (persistent grid) ---------------------------------------
+struct TileIt {
+  i64 t, base, rend;
+  bool have;
+};
+
+__device__ __forceinline__ void tile_begin(const Args& a, i64 t, TileIt& it) {
+  it.t = t;
+  it.have = t < a.ntiles;
+  if (!it.have) return;
+  int s = 0;
+  while (s + 1 < a.nspans && t >= a.sp_tile0[s + 1]) s++;
+  it.base = a.sp_start[s] + (t - a.sp_tile0[s]) * (i64)a.tile_rows;
+  const i64 send = a.sp_start[s] + a.sp_rows[s];
+  it.rend = it.base + a.tile_rows < send ? it.base + a.tile_rows : send;
+}
+
+template <int U>
+__device__ __forceinline__ void tile_advance(const Args& a, TileIt& it) {
+  it.base += kWave * U;
+  if (it.base >= it.rend) tile_begin(a, it.t + gridDim.x, it);
+}
+
+__device__ __forceinline__ u64 combine(int op, u64 a, u64 b) {
+  switch (op) {
+    case 0: return (u64)f64_bits(as_f64((i64)a) + as_f64((i64)b));
+    case 1:
+    case 100: return a + b;
+    case 2:
+    case 4: return (i64)b < (i64)a ? b : a;
+    case 3:
+    case 5: return (i64)b > (i64)a ? b : a;
+    default: return b < a ? b : a;  // 101: first row (unsigned min)
+  }
+}
+
+__device__ __forceinline__ u64 wave_reduce(int op, u64 v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const u64 o = (u64)__shfl_xor((long long)v, m, kWave);
+    const u64 lo = (threadIdx.x & m) ? o : v;
+    const u64 hi = (threadIdx.x & m) ? v : o;
+    v = combine(op, lo, hi);
+  }
+  return v;
+}
+
+template <int... J>
+__device__ __forceinline__ void init_regs(u64* r, Seq<J...>) {
+  ((r[J] = acc_init<kAOP[J]>()), ...);
+}
+
+template <int... J>
+__device__ __forceinline__ void fold_regs(u64* r, const Args& a, const i64* row, Seq<J...>) {
+  (fold_reg<J>(r[J], a, row), ...);
+}
+
+// w points at this lane's word of accumulator 0 of the row's group; the
+// accumulator J word is kWave * J further.
+template <int... J>
+__device__ __forceinline__ void fold_ldss(u64* w, const Args& a, const i64* row, Seq<J...>) {
+  (fold_lds<J>(w + J * kWave, a, row), ...);
+}
+
+}  // namespace vqj
+
+using namespace vqj;
+
+// ---------------------------------------------------------------------------
+// Dense GROUP BY (see k_agg_dense in k_scan.hip for the contract).
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
+  extern __shared__ u64 lds[];
+  constexpr int U = VQ_U, G = VQ_G, NA = VQ_NA;
+  const int lane = threadIdx.x;
+  u32* cnt = (u32*)(lds + (u64)G * NA * kWave);
+  u32* first = cnt + G * kWave;
+  u64 racc[NA > 0 ? NA : 1];
+  u32 rcnt = 0, rfirst = 0xFFFFFFFFu;
+  init_regs(racc, seq<NA>{});
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int j = 0; j < NA; j++)
+      for (int gi = 0; gi < G; gi++) lds[(gi * NA + j) * kWave + lane] = racc[j];
+    for (int gi = 0; gi < G; gi++) {
+      cnt[gi * kWave + lane] = 0;
+      first[gi * kWave + lane] = 0xFFFFFFFFu;
+    }
+  }
+  TileIt it;
+  tile_begin(a, blockIdx.x, it);
+  i64 nraw[VQ_NS][U];
+  if (it.have) load_rows<U>(a, it.base, it.rend, lane, nraw);
+  while (it.have) {
+    i64 raw[VQ_NS][U];
+#pragma unroll
+    for (int c = 0; c < VQ_NS; c++)
+#pragma unroll
+      for (int u = 0; u < U; u++) raw[c][u] = nraw[c][u];
+    const i64 base = it.base, rend = it.rend;
+    tile_advance<U>(a, it);
+    if (it.have) load_rows<U>(a, it.base, it.rend, lane, nraw);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      i64 row[VQ_NS];
+#pragma unroll
+      for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
+      const i64 r = base + u * kWave + lane;
+      if (!(r < rend && pass_all(a, row, seq<VQ_NT>{}))) continue;
+      if constexpr (G == 1) {
+        rfirst = rcnt == 0 ? (u32)r : rfirst;
+        rcnt++;
+        fold_regs(racc, a, row, seq<NA>{});
+      } else {
+        const int g = group_of(row, seq<VQ_NS>{});
+        atomicAdd(&cnt[g * kWave + lane], 1u);
+        atomicMin(&first[g * kWave + lane], (u32)r);
+        fold_ldss(&lds[g * NA * kWave + lane], a, row, seq<NA>{});
+      }
+    }
+  }
+  if constexpr (G > 1) __syncthreads();
+  constexpr int NI = NA + 2;
+  u64* out = a.part + (u64)blockIdx.x * G * NI;
+  for (int gi = 0; gi < G; gi++) {
+#pragma unroll
+    for (int item = 0; item < NI; item++) {
+      u64 v;
+      int op;
+      if (item < NA) {
+        v = G == 1 ? racc[item] : lds[(gi * NA + item) * kWave + lane];
+        op = kAOP[item];
+      } else if (item == NA) {
+        v = G == 1 ? rcnt : cnt[gi * kWave + lane];
+        op = 100;
+      } else {
+        v = G == 1 ? rfirst : first[gi * kWave + lane];
+        op = 101;
+      }
+      v = wave_reduce(op, v);
+      if (lane == 0) out[gi * NI + item] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Filter pass bits (see k_filter_bits in k_scan.hip): one tile per wave.
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
+  constexpr int U = VQ_U;
+  const int lane = threadIdx.x;
+  TileIt it;
+  tile_begin(a, blockIdx.x, it);
+  if (!it.have) return;
+  u32 count = 0;
+  for (i64 base = it.base; base < it.rend; base += kWave * U) {
+    i64 raw[VQ_NS][U];
+    load_rows<U>(a, base, it.rend, lane, raw);
+    u64 mine = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      i64 row[VQ_NS];
+#pragma unroll
+      for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
+      const bool p = base + u * kWave + lane < it.rend && pass_all(a, row, seq<VQ_NT>{});
+      const u64 m = __ballot(p);
+      count += (u32)__popcll(m);
+      if (lane == u) mine = m;
+    }
+    if (lane < U && base + lane * kWave < it.rend) a.bits[(base >> 6) + lane] = mine;
+  }
+  if (lane == 0) a.tile_counts[it.t] = count;
+}

@@ -11,6 +11,8 @@
 // fixed function of its row index, lanes combine in a fixed butterfly, and the
 // cross-tile reduction runs in tile order with double-double compensation.
 // Reruns of one query on one table therefore give identical bits.
+#include <cstdlib>
+
 #include "vq_internal.h"
 
 namespace vq {
@@ -18,13 +20,34 @@ namespace vq {
 constexpr int kWave = 64;
 constexpr int kU = 4;  // rows per lane in flight per step
 
-__device__ __forceinline__ int64_t load_slot(const QSlot& s, int64_t r) {
-  switch (s.type) {
-    case VQ_U8: return ((const uint8_t*)s.p)[r];
-    case VQ_U16: return ((const uint16_t*)s.p)[r];
-    case VQ_DATE: return ((const int32_t*)s.p)[r];
-    default: return ((const int64_t*)s.p)[r];
-  }
+// Branch-free column read: every column type is fetched as the aligned
+// 8-byte word holding the element, then shifted, masked and (for DATE)
+// sign-extended.  One load instruction shape for all columns keeps all of a
+// chunk's loads in flight together (a per-type switch around the loads makes
+// the compiler wait at every join).  Column buffers are padded to 256 bytes,
+// so the aligned word of the last element is inside the allocation.
+struct SlotLd {
+  const uint8_t* p;
+  uint64_t mask;
+  int lw;     // log2(element width)
+  bool sext;  // sign-extend a 32-bit value (DATE)
+};
+
+__device__ __forceinline__ SlotLd slot_ld(const QSlot& s) {
+  SlotLd l;
+  l.p = (const uint8_t*)s.p;
+  l.lw = s.type == VQ_U8 ? 0 : (s.type == VQ_U16 ? 1 : (s.type == VQ_DATE ? 2 : 3));
+  l.mask = l.lw == 3 ? ~0ULL : ((1ULL << (8 << l.lw)) - 1);
+  l.sext = s.type == VQ_DATE;
+  return l;
+}
+
+__device__ __forceinline__ int64_t ld64(const SlotLd& l, int64_t r) {
+  const int64_t byte = r << l.lw;
+  const uint64_t w = *(const uint64_t*)(l.p + (byte & ~(int64_t)7));
+  const uint64_t x = (w >> ((byte & 7) * 8)) & l.mask;
+  const int64_t sx = (int64_t)(int32_t)(uint32_t)x;
+  return l.sext ? sx : (int64_t)x;
 }
 
 __device__ __forceinline__ bool cmp_i(int op, int64_t a, int64_t b) {
@@ -74,7 +97,7 @@ __device__ __forceinline__ int64_t arith_i(int op, int64_t a, int64_t b) {
 // Loads every referenced column for U rows of this lane.  Rows past the tile
 // end re-read the tile's last row (in bounds) and are masked out afterwards.
 template <int U>
-__device__ __forceinline__ void load_rows(const QSlot* slot, int nslots, int64_t base,
+__device__ __forceinline__ void load_rows(const SlotLd* ld, int nslots, int64_t base,
                                           int64_t rend, int lane, int64_t (&raw)[kMaxSlots][U]) {
 #pragma unroll
   for (int c = 0; c < kMaxSlots; c++) {
@@ -83,7 +106,7 @@ __device__ __forceinline__ void load_rows(const QSlot* slot, int nslots, int64_t
       for (int u = 0; u < U; u++) {
         int64_t r = base + u * kWave + lane;
         r = r < rend ? r : rend - 1;
-        raw[c][u] = load_slot(slot[c], r);
+        raw[c][u] = ld64(ld[c], r);
       }
     }
   }
@@ -219,45 +242,22 @@ __device__ __forceinline__ void eval_i(const QExpr& e, const int64_t (&raw)[kMax
   }
 }
 
+// float64 <-> order-preserving int64 (MIN/MAX of floats run as integer
+// MIN/MAX; NaN rows are skipped, as foldRow's `v < cur` never takes a NaN).
+__device__ __forceinline__ int64_t f64_ord(double d) {
+  const int64_t b = f64_to_bits(d);
+  return b >= 0 ? b : (int64_t)((uint64_t)b ^ 0x7FFFFFFFFFFFFFFFULL);
+}
+
+// Accumulator seeds: newAccums (exec/aggregate.go:255-279) puts +-MaxFloat64 /
+// +-MaxInt64 sentinels in MIN/MAX slots.
 __device__ __forceinline__ uint64_t acc_init(int op) {
   switch (op) {
-    case ACC_MIN_F: return (uint64_t)f64_to_bits(1.7976931348623157e308);
-    case ACC_MAX_F: return (uint64_t)f64_to_bits(-1.7976931348623157e308);
+    case ACC_MIN_F: return (uint64_t)f64_ord(1.7976931348623157e308);
+    case ACC_MAX_F: return (uint64_t)f64_ord(-1.7976931348623157e308);
     case ACC_MIN_I: return (uint64_t)INT64_MAX;
     case ACC_MAX_I: return (uint64_t)INT64_MIN;
     default: return 0;
-  }
-}
-
-// One fold step of foldRow (exec/agg_intkey.go:303-361) on the raw 64-bit
-// accumulator word (float64 kept as IEEE bits, like HashAggregate's []int64).
-__device__ __forceinline__ uint64_t fold_f(int op, uint64_t acc, double v) {
-  double cur = bits_to_f64((int64_t)acc);
-  switch (op) {
-    case ACC_SUM_F: return (uint64_t)f64_to_bits(cur + v);
-    case ACC_MIN_F: return v < cur ? (uint64_t)f64_to_bits(v) : acc;
-    default: return v > cur ? (uint64_t)f64_to_bits(v) : acc;
-  }
-}
-
-__device__ __forceinline__ uint64_t fold_i(int op, uint64_t acc, int64_t v) {
-  int64_t cur = (int64_t)acc;
-  switch (op) {
-    case ACC_SUM_I: return (uint64_t)cur + (uint64_t)v;
-    case ACC_MIN_I: return v < cur ? (uint64_t)v : acc;
-    default: return v > cur ? (uint64_t)v : acc;
-  }
-}
-
-// Combine two partial accumulators (mergePartialAgg, exec/parallel.go:237-314).
-__device__ __forceinline__ uint64_t combine(int op, uint64_t a, uint64_t b) {
-  switch (op) {
-    case ACC_SUM_F: return (uint64_t)f64_to_bits(bits_to_f64((int64_t)a) + bits_to_f64((int64_t)b));
-    case ACC_SUM_I: return a + b;
-    case ACC_MIN_F: return fold_f(ACC_MIN_F, a, bits_to_f64((int64_t)b));
-    case ACC_MAX_F: return fold_f(ACC_MAX_F, a, bits_to_f64((int64_t)b));
-    case ACC_MIN_I: return fold_i(ACC_MIN_I, a, (int64_t)b);
-    default: return fold_i(ACC_MAX_I, a, (int64_t)b);
   }
 }
 
@@ -268,51 +268,81 @@ __device__ __forceinline__ int item_op(const QDesc& d, int item) {
   return item < d.nacc ? d.acc[item].op : (item == d.nacc ? ITEM_COUNT : ITEM_FIRST);
 }
 
+// Combine two partials (mergePartialAgg, exec/parallel.go:237-314).  Float
+// sums are combined in double-double by the final kernel, not here.
 __device__ __forceinline__ uint64_t combine_item(int op, uint64_t a, uint64_t b) {
-  if (op == ITEM_COUNT) return a + b;
-  if (op == ITEM_FIRST) return a < b ? a : b;
-  return combine(op, a, b);
+  switch (op) {
+    case ACC_SUM_F: return (uint64_t)f64_to_bits(bits_to_f64((int64_t)a) + bits_to_f64((int64_t)b));
+    case ACC_SUM_I:
+    case ITEM_COUNT: return a + b;
+    case ACC_MIN_F:
+    case ACC_MIN_I: return (int64_t)b < (int64_t)a ? b : a;
+    case ACC_MAX_F:
+    case ACC_MAX_I: return (int64_t)b > (int64_t)a ? b : a;
+    default: return b < a ? b : a;  // ITEM_FIRST (unsigned)
+  }
 }
 
 // Fixed-order butterfly over the wave; every lane ends with the same value.
 __device__ __forceinline__ uint64_t wave_reduce(int op, uint64_t v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
-    uint64_t o = (uint64_t)__shfl_xor((long long)v, m, kWave);
-    // order the pair by lane so both partners compute the same value
-    uint64_t lo = (threadIdx.x & m) ? o : v;
-    uint64_t hi = (threadIdx.x & m) ? v : o;
+    const uint64_t o = (uint64_t)__shfl_xor((long long)v, m, kWave);
+    const uint64_t lo = (threadIdx.x & m) ? o : v;
+    const uint64_t hi = (threadIdx.x & m) ? v : o;
     v = combine_item(op, lo, hi);
   }
   return v;
 }
 
-__device__ __forceinline__ int tile_span(const QDesc& d, int64_t t) {
+struct TileIt {
+  int64_t t, base, rend;
+  bool have;
+};
+
+__device__ __forceinline__ void tile_begin(const QDesc& d, int64_t t, TileIt& it) {
+  it.t = t;
+  it.have = t < d.ntiles;
+  if (!it.have) return;
   int s = 0;
   while (s + 1 < d.nspans && t >= d.span[s + 1].tile0) s++;
-  return s;
+  it.base = d.span[s].start + (t - d.span[s].tile0) * (int64_t)d.tile_rows;
+  const int64_t send = d.span[s].start + d.span[s].rows;
+  it.rend = it.base + d.tile_rows < send ? it.base + d.tile_rows : send;
+}
+
+// Next chunk of kWave*U rows: the rest of this tile, then the wave's next tile
+// (tiles t, t + gridDim.x, ... — a fixed assignment, so reruns fold every row
+// into the same lane in the same order).
+template <int U>
+__device__ __forceinline__ void tile_advance(const QDesc& d, TileIt& it) {
+  it.base += kWave * U;
+  if (it.base >= it.rend) tile_begin(d, it.t + gridDim.x, it);
 }
 
 // ---------------------------------------------------------------------------
 // Dense GROUP BY: filter -> arithmetic -> per-lane accumulators.
+//
+// A persistent grid of single-wave workgroups; wave w folds the tiles w,
+// w + grid, ... .  Loads for the next chunk are issued before the current
+// chunk is folded.
 // SINGLE (no GROUP BY, accumulateDirect): accumulators live in registers.
-// Otherwise each lane owns a private column of LDS accumulators
-// lds[(g*nacc + j)*64 + lane], so a row updates only its own group with one
-// read-modify-write per accumulator, conflict free, whatever the group count.
-// Output: per tile and group, nacc accumulators + row count + first row.
+// Otherwise lane l owns LDS column l of every accumulator,
+// lds[(g*nacc + j)*64 + l], updated with no-return LDS atomics: conflict free
+// (one lane per address), in program order (one lane's updates to an address
+// execute in issue order), and with no read-modify-write latency chain.
+// Output: per wave and group, nacc accumulators + row count + first row.
 // ---------------------------------------------------------------------------
 template <bool SINGLE, int U>
 __global__ __launch_bounds__(kWave) void k_agg_dense(const QDesc d, uint64_t* __restrict__ part) {
   extern __shared__ uint64_t lds[];
   const int lane = threadIdx.x;
-  const int64_t t = blockIdx.x;
-  const int s = tile_span(d, t);
-  const int64_t row0 = d.span[s].start + (t - d.span[s].tile0) * (int64_t)d.tile_rows;
-  const int64_t send = d.span[s].start + d.span[s].rows;
-  const int64_t rend = row0 + d.tile_rows < send ? row0 + d.tile_rows : send;
-  const int G = d.ngroups, NA = d.nacc;
+  const int G = d.ngroups, NA = d.nacc, NS = d.nslots;
   uint32_t* cnt = (uint32_t*)(lds + (size_t)G * NA * kWave);
   uint32_t* first = cnt + G * kWave;
+  SlotLd ld[kMaxSlots];
+#pragma unroll
+  for (int c = 0; c < kMaxSlots; c++) ld[c] = slot_ld(d.slot[c < NS ? c : 0]);
 
   uint64_t racc[kMaxAcc];
   uint32_t rcnt = 0, rfirst = 0xFFFFFFFFu;
@@ -326,71 +356,92 @@ __global__ __launch_bounds__(kWave) void k_agg_dense(const QDesc d, uint64_t* __
     }
   }
 
-  for (int64_t base = row0; base < rend; base += kWave * U) {
+  TileIt it;
+  tile_begin(d, blockIdx.x, it);
+  int64_t nraw[kMaxSlots][U];
+  if (it.have) load_rows<U>(ld, NS, it.base, it.rend, lane, nraw);
+  while (it.have) {
     int64_t raw[kMaxSlots][U];
-    load_rows<U>(d.slot, d.nslots, base, rend, lane, raw);
+#pragma unroll
+    for (int c = 0; c < kMaxSlots; c++)
+#pragma unroll
+      for (int u = 0; u < U; u++) raw[c][u] = nraw[c][u];
+    const int64_t base = it.base, rend = it.rend;
+    tile_advance<U>(d, it);
+    if (it.have) load_rows<U>(ld, NS, it.base, it.rend, lane, nraw);
+
     bool pass[U];
     int g[U];
-    eval_pred<U>(d.slot, d.nslots, base, rend, lane, raw, pass, g);
-    if (SINGLE) {
+    eval_pred<U>(d.slot, NS, base, rend, lane, raw, pass, g);
 #pragma unroll
-      for (int u = 0; u < U; u++)
-        if (pass[u]) {
-          if (rcnt == 0) rfirst = (uint32_t)(base + u * kWave + lane);
-          rcnt++;
-        }
-    } else {
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        if (pass[u]) {
-          const int ci = g[u] * kWave + lane;
-          const uint32_t c = cnt[ci];
-          if (c == 0) first[ci] = (uint32_t)(base + u * kWave + lane);
-          cnt[ci] = c + 1;
-        }
+    for (int u = 0; u < U; u++) {
+      if (!pass[u]) continue;
+      const uint32_t row = (uint32_t)(base + u * kWave + lane);
+      if (SINGLE) {
+        rfirst = rcnt == 0 ? row : rfirst;
+        rcnt++;
+      } else {
+        atomicAdd(&cnt[g[u] * kWave + lane], 1u);
+        atomicMin(&first[g[u] * kWave + lane], row);
+      }
     }
 #pragma unroll
     for (int j = 0; j < kMaxAcc; j++) {
-      if (j < NA) {
-        const QAcc& a = d.acc[j];
-        const int op = a.op;
-        if (op == ACC_SUM_F || op == ACC_MIN_F || op == ACC_MAX_F) {
-          double v[U];
-          eval_f<U>(a.e, d.slot, raw, v);
+      if (j >= NA) continue;
+      const QAcc& a = d.acc[j];
+      const int op = a.op;
+      if (op == ACC_SUM_F || op == ACC_MIN_F || op == ACC_MAX_F) {
+        double v[U];
+        eval_f<U>(a.e, d.slot, raw, v);
 #pragma unroll
-          for (int u = 0; u < U; u++) {
+        for (int u = 0; u < U; u++) {
+          if (!pass[u]) continue;
+          if (op == ACC_SUM_F) {
+            if (SINGLE) racc[j] = (uint64_t)f64_to_bits(bits_to_f64((int64_t)racc[j]) + v[u]);
+            else atomicAdd((double*)&lds[(g[u] * NA + j) * kWave + lane], v[u]);
+          } else if (v[u] == v[u]) {  // skip NaN
+            const int64_t o = f64_ord(v[u]);
             if (SINGLE) {
-              if (pass[u]) racc[j] = fold_f(op, racc[j], v[u]);
-            } else if (pass[u]) {
-              uint64_t* p = &lds[(g[u] * NA + j) * kWave + lane];
-              *p = fold_f(op, *p, v[u]);
+              racc[j] = (uint64_t)(op == ACC_MIN_F ? (o < (int64_t)racc[j] ? o : (int64_t)racc[j])
+                                                   : (o > (int64_t)racc[j] ? o : (int64_t)racc[j]));
+            } else {
+              long long* p = (long long*)&lds[(g[u] * NA + j) * kWave + lane];
+              if (op == ACC_MIN_F) atomicMin(p, (long long)o);
+              else atomicMax(p, (long long)o);
             }
           }
-        } else {
-          int64_t v[U];
-          eval_i<U>(a.e, raw, v);
+        }
+      } else {
+        int64_t v[U];
+        eval_i<U>(a.e, raw, v);
 #pragma unroll
-          for (int u = 0; u < U; u++) {
-            if (SINGLE) {
-              if (pass[u]) racc[j] = fold_i(op, racc[j], v[u]);
-            } else if (pass[u]) {
-              uint64_t* p = &lds[(g[u] * NA + j) * kWave + lane];
-              *p = fold_i(op, *p, v[u]);
-            }
+        for (int u = 0; u < U; u++) {
+          if (!pass[u]) continue;
+          if (SINGLE) {
+            const int64_t cur = (int64_t)racc[j];
+            racc[j] = op == ACC_SUM_I ? (uint64_t)cur + (uint64_t)v[u]
+                                      : (uint64_t)(op == ACC_MIN_I ? (v[u] < cur ? v[u] : cur)
+                                                                   : (v[u] > cur ? v[u] : cur));
+          } else {
+            unsigned long long* p = (unsigned long long*)&lds[(g[u] * NA + j) * kWave + lane];
+            if (op == ACC_SUM_I) atomicAdd(p, (unsigned long long)v[u]);
+            else if (op == ACC_MIN_I) atomicMin((long long*)p, (long long)v[u]);
+            else atomicMax((long long*)p, (long long)v[u]);
           }
         }
       }
     }
   }
+  if (!SINGLE) __syncthreads();
 
-  // Lane combine (fixed butterfly), lane 0 writes the tile partials.
+  // Lane combine (fixed butterfly), lane 0 writes the wave's partials.
   const int NI = NA + 2;
-  uint64_t* out = part + (size_t)t * G * NI;
+  uint64_t* out = part + (size_t)blockIdx.x * G * NI;
   for (int gi = 0; gi < G; gi++) {
     for (int item = 0; item < NI; item++) {
       uint64_t v;
       if (SINGLE) {
-        v = item < NA ? racc[0] : 0;
+        v = racc[0];
 #pragma unroll
         for (int j = 1; j < kMaxAcc; j++)
           if (item == j) v = racc[j];
@@ -401,8 +452,7 @@ __global__ __launch_bounds__(kWave) void k_agg_dense(const QDesc d, uint64_t* __
         else if (item == NA) v = cnt[gi * kWave + lane];
         else v = first[gi * kWave + lane];
       }
-      const int op = item_op(d, item);
-      v = wave_reduce(op, v);
+      v = wave_reduce(item_op(d, item), v);
       if (lane == 0) out[gi * NI + item] = v;
     }
   }
@@ -414,22 +464,23 @@ struct DD {
 };
 
 __device__ __forceinline__ DD two_sum(double a, double b) {
-  double s = a + b;
-  double bp = s - a;
-  double ap = s - bp;
+  const double s = a + b;
+  const double bp = s - a;
+  const double ap = s - bp;
   return {s, (a - ap) + (b - bp)};
 }
 
 __device__ __forceinline__ DD dd_add(DD a, DD b) {
-  DD s = two_sum(a.hi, b.hi);
-  double e = s.lo + (a.lo + b.lo);
+  const DD s = two_sum(a.hi, b.hi);
+  const double e = s.lo + (a.lo + b.lo);
   return two_sum(s.hi, e);
 }
 
-// Cross-tile reduction: block (group, item) folds the tiles in index order;
-// float sums are carried in double-double so the result is the correctly
-// ordered compensated sum of the tile partials.
-__global__ __launch_bounds__(256) void k_agg_final(const QDesc d, const uint64_t* __restrict__ part,
+// Cross-wave reduction: block (group, item) folds the wave partials in a
+// fixed order; float sums are carried in double-double, so the result is the
+// compensated sum of the partials.
+__global__ __launch_bounds__(256) void k_agg_final(const QDesc d, int32_t nparts,
+                                                   const uint64_t* __restrict__ part,
                                                    uint64_t* __restrict__ outv) {
   __shared__ uint64_t s_hi[256];
   __shared__ double s_lo[256];
@@ -440,8 +491,8 @@ __global__ __launch_bounds__(256) void k_agg_final(const QDesc d, const uint64_t
   const int G = d.ngroups;
   uint64_t v = op == ITEM_COUNT ? 0 : (op == ITEM_FIRST ? 0xFFFFFFFFu : acc_init(op));
   DD dd = {0.0, 0.0};
-  for (int64_t w = tid; w < d.ntiles; w += 256) {
-    uint64_t x = part[((size_t)w * G + gi) * NI + item];
+  for (int w = tid; w < nparts; w += 256) {
+    const uint64_t x = part[((size_t)w * G + gi) * NI + item];
     if (op == ACC_SUM_F) dd = dd_add(dd, {bits_to_f64((int64_t)x), 0.0});
     else v = combine_item(op, v, x);
   }
@@ -455,9 +506,9 @@ __global__ __launch_bounds__(256) void k_agg_final(const QDesc d, const uint64_t
   for (int stride = 128; stride >= 1; stride >>= 1) {
     if (tid < stride) {
       if (op == ACC_SUM_F) {
-        DD a = {bits_to_f64((int64_t)s_hi[tid]), s_lo[tid]};
-        DD b = {bits_to_f64((int64_t)s_hi[tid + stride]), s_lo[tid + stride]};
-        DD c = dd_add(a, b);
+        const DD a = {bits_to_f64((int64_t)s_hi[tid]), s_lo[tid]};
+        const DD b = {bits_to_f64((int64_t)s_hi[tid + stride]), s_lo[tid + stride]};
+        const DD c = dd_add(a, b);
         s_hi[tid] = (uint64_t)f64_to_bits(c.hi);
         s_lo[tid] = c.lo;
       } else {
@@ -467,12 +518,8 @@ __global__ __launch_bounds__(256) void k_agg_final(const QDesc d, const uint64_t
     __syncthreads();
   }
   if (tid == 0) {
-    if (op == ACC_SUM_F) {
-      double r = bits_to_f64((int64_t)s_hi[0]) + s_lo[0];
-      outv[gi * NI + item] = (uint64_t)f64_to_bits(r);
-    } else {
-      outv[gi * NI + item] = s_hi[0];
-    }
+    if (op == ACC_SUM_F) outv[gi * NI + item] = (uint64_t)f64_to_bits(bits_to_f64((int64_t)s_hi[0]) + s_lo[0]);
+    else outv[gi * NI + item] = s_hi[0];
   }
 }
 
@@ -487,29 +534,33 @@ template <int U>
 __global__ __launch_bounds__(kWave) void k_filter_bits(const QDesc d, uint64_t* __restrict__ bits,
                                                        uint32_t* __restrict__ tile_counts) {
   const int lane = threadIdx.x;
-  const int64_t t = blockIdx.x;
-  const int s = tile_span(d, t);
-  const int64_t row0 = d.span[s].start + (t - d.span[s].tile0) * (int64_t)d.tile_rows;
-  const int64_t send = d.span[s].start + d.span[s].rows;
-  const int64_t rend = row0 + d.tile_rows < send ? row0 + d.tile_rows : send;
+  const int NS = d.nslots;
+  SlotLd ld[kMaxSlots];
+#pragma unroll
+  for (int c = 0; c < kMaxSlots; c++) ld[c] = slot_ld(d.slot[c < NS ? c : 0]);
+  TileIt it;
+  tile_begin(d, blockIdx.x, it);
+  const int64_t t0 = it.t;
   uint32_t count = 0;
-  for (int64_t base = row0; base < rend; base += kWave * U) {
+  // this kernel runs one tile per wave (grid = ntiles)
+  while (it.have && it.t == t0) {
     int64_t raw[kMaxSlots][U];
-    load_rows<U>(d.slot, d.nslots, base, rend, lane, raw);
+    load_rows<U>(ld, NS, it.base, it.rend, lane, raw);
     bool pass[U];
     int g[U];
-    eval_pred<U>(d.slot, d.nslots, base, rend, lane, raw, pass, g);
+    eval_pred<U>(d.slot, NS, it.base, it.rend, lane, raw, pass, g);
+    uint64_t mine = 0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int64_t wb = base + u * kWave;
       const uint64_t m = __ballot(pass[u]);
-      if (wb < rend) {
-        if (lane == 0) bits[wb >> 6] = m;
-        count += (uint32_t)__popcll(m);
-      }
+      count += (uint32_t)__popcll(m);
+      if (lane == u) mine = m;
     }
+    if (lane < U && it.base + lane * kWave < it.rend) bits[(it.base >> 6) + lane] = mine;
+    it.base += kWave * U;
+    if (it.base >= it.rend) it.have = false;
   }
-  if (lane == 0) tile_counts[t] = count;
+  if (lane == 0) tile_counts[t0] = count;
 }
 
 // Exclusive scan of the tile counts by one block; total -> *out_count.
@@ -525,7 +576,7 @@ __global__ __launch_bounds__(1024) void k_filter_scan(const uint32_t* __restrict
   s[tid] = sum;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    uint64_t x = tid >= off ? s[tid - off] : 0;
+    const uint64_t x = tid >= off ? s[tid - off] : 0;
     __syncthreads();
     s[tid] += x;
     __syncthreads();
@@ -542,14 +593,12 @@ __global__ __launch_bounds__(kWave) void k_filter_write(const QDesc d, const uin
                                                         const uint32_t* __restrict__ offsets,
                                                         uint32_t* __restrict__ out_rows) {
   const int lane = threadIdx.x;
-  const int64_t t = blockIdx.x;
-  const int s = tile_span(d, t);
-  const int64_t row0 = d.span[s].start + (t - d.span[s].tile0) * (int64_t)d.tile_rows;
-  const int64_t send = d.span[s].start + d.span[s].rows;
-  const int64_t rend = row0 + d.tile_rows < send ? row0 + d.tile_rows : send;
-  uint32_t pos = offsets[t];
+  TileIt it;
+  tile_begin(d, blockIdx.x, it);
+  if (!it.have) return;
+  uint32_t pos = offsets[it.t];
   const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
-  for (int64_t wb = row0; wb < rend; wb += kWave) {
+  for (int64_t wb = it.base; wb < it.rend; wb += kWave) {
     const uint64_t m = bits[wb >> 6];
     if ((m >> lane) & 1ULL) out_rows[pos + (uint32_t)__popcll(m & below)] = (uint32_t)(wb + lane);
     pos += (uint32_t)__popcll(m);
@@ -559,34 +608,67 @@ __global__ __launch_bounds__(kWave) void k_filter_write(const QDesc d, const uin
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static size_t dense_lds_bytes(const QDesc& d) {
+size_t dense_lds_bytes(const QDesc& d) {
+  if (d.ngroups == 1) return 0;
   return (size_t)d.ngroups * d.nacc * kWave * 8 + (size_t)d.ngroups * kWave * 4 * 2;
 }
 
-hipError_t launch_agg_dense(const QDesc& d, uint64_t* part, hipStream_t s) {
-  if (d.ntiles == 0) return hipSuccess;
+// Waves the persistent dense kernel runs with: what fits the device at once.
+int32_t dense_grid(const QDesc& d) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+  }
+  int per_cu = 0;
+  if (d.ngroups == 1)
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_agg_dense<true, kU>, kWave, 0);
+  else
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_agg_dense<false, kU>, kWave,
+                                                 dense_lds_bytes(d));
+  if (per_cu <= 0) per_cu = 1;
+  // VQ_WAVES_PER_CU overrides the occupancy-derived wave count (tuning sweeps)
+  if (const char* e = getenv("VQ_WAVES_PER_CU")) {
+    const int v = atoi(e);
+    if (v > 0) per_cu = v;
+  }
+  const int64_t g = (int64_t)cus * per_cu;
+  return (int32_t)(d.ntiles < g ? d.ntiles : g);
+}
+
+hipError_t launch_agg_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s) {
+  if (d.ntiles == 0 || grid == 0) return hipSuccess;
   if (d.ngroups == 1) {
-    hipLaunchKernelGGL((k_agg_dense<true, kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, part);
+    hipLaunchKernelGGL((k_agg_dense<true, kU>), dim3((unsigned)grid), dim3(kWave), 0, s, d, part);
   } else {
-    hipLaunchKernelGGL((k_agg_dense<false, kU>), dim3((unsigned)d.ntiles), dim3(kWave),
+    hipLaunchKernelGGL((k_agg_dense<false, kU>), dim3((unsigned)grid), dim3(kWave),
                        dense_lds_bytes(d), s, d, part);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_agg_final(const QDesc& d, const uint64_t* part, uint64_t* out, hipStream_t s) {
+hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part, uint64_t* out,
+                            hipStream_t s) {
   const int NI = d.nacc + 2;
-  hipLaunchKernelGGL(k_agg_final, dim3((unsigned)(d.ngroups * NI)), dim3(256), 0, s, d, part, out);
+  hipLaunchKernelGGL(k_agg_final, dim3((unsigned)(d.ngroups * NI)), dim3(256), 0, s, d, nparts,
+                     part, out);
   return hipGetLastError();
 }
 
-hipError_t launch_filter(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,
+hipError_t launch_filter(const QDesc& d, bool jit, uint64_t* bits, uint32_t* tile_counts,
                          uint32_t* tile_offsets, uint32_t* out_rows, uint64_t* out_count,
                          hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (d.ntiles == 0) return hipMemsetAsync(out_count, 0, sizeof(uint64_t), s);
   if (ev0) hipEventRecord(ev0, s);
-  hipLaunchKernelGGL((k_filter_bits<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, bits,
-                     tile_counts);
+  if (jit) {
+    hipError_t e = jit_launch_filter_bits(d, bits, tile_counts, s);
+    if (e != hipSuccess) return e;
+  } else {
+    hipLaunchKernelGGL((k_filter_bits<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, bits,
+                       tile_counts);
+  }
   if (ev1) hipEventRecord(ev1, s);
   hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(1024), 0, s, tile_counts, d.ntiles,
                      tile_offsets, out_count);

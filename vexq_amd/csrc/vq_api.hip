@@ -459,7 +459,7 @@ extern "C" int vq_table_upload(vq_ctx* c, int64_t nrows, int32_t ncols, const vq
         (cols[i].type == VQ_U16 && cols[i].dict_size > 65536) ||
         (cols[i].dict_size > 0 && cols[i].type != VQ_U8 && cols[i].type != VQ_U16))
       return fail(VQ_E_ARG, "exec: gpu: column %d: bad dictionary size %d", i, cols[i].dict_size);
-    bytes += ((uint64_t)nrows * w + 255) & ~255ULL;
+    bytes += (((uint64_t)nrows * w + 255) & ~255ULL) + 256;
   }
   // dictionary codes must index their dictionary
   for (int32_t i = 0; i < ncols; i++) {
@@ -494,7 +494,8 @@ extern "C" int vq_table_upload(vq_ctx* c, int64_t nrows, int32_t ncols, const vq
   for (int32_t i = 0; i < ncols; i++) {
     ColDev cd{cols[i].type, cols[i].dict_size, nullptr};
     size_t nb = (size_t)nrows * width_of(cols[i].type);
-    hipError_t e = hipMalloc(&cd.d, nb ? nb : 256);
+    // padded: kernels read the aligned 8-byte word holding an element
+    hipError_t e = hipMalloc(&cd.d, ((nb + 255) & ~(size_t)255) + 256);
     if (e == hipSuccess && nb) e = hipMemcpy(cd.d, cols[i].data, nb, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       if (cd.d) hipFree(cd.d);
@@ -565,8 +566,23 @@ extern "C" int64_t vq_join_table_capacity(int64_t keys) {
 // ===========================================================================
 namespace {
 
-constexpr int32_t kDenseTileRows = 64 * 64;  // rows per wave64 workgroup
 constexpr size_t kDenseLdsMax = 64 * 1024;
+
+// Tile rows for a scan of `rows` rows: a power of two between 256 and 16384
+// that cuts the scan into about 2048 tiles, so small (zone-pruned) scans still
+// spread over every CU and large ones keep per-tile overhead low.
+int32_t scan_tile_rows(int64_t rows) {
+  int64_t t = 256;
+  while (t < 16384 && t * 2048 < rows) t *= 2;
+  return (int32_t)t;
+}
+
+int64_t kept_rows(const vq_table* t, const std::vector<uint8_t>& keep) {
+  int64_t n = 0;
+  for (int32_t g = 0; g < t->nrg; g++)
+    if (keep[g]) n += t->rg_rows[g];
+  return n;
+}
 
 struct AccMap {
   int32_t acc[VQ_MAX_AGGS];  // accumulator index per aggregate (-1 = COUNT)
@@ -635,12 +651,8 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
     d.slot[s].type = t->cols[0].type;
     d.nslots = 1;
   }
-  if (G > 1) {
-    size_t lds = (size_t)G * d.nacc * 64 * 8 + (size_t)G * 64 * 8;
-    if (lds > kDenseLdsMax)
-      return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld groups x %d accumulators exceed LDS", (long long)G, d.nacc);
-  }
-  d.tile_rows = kDenseTileRows;
+  if (G > 1 && dense_lds_bytes(d) > kDenseLdsMax)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld groups x %d accumulators exceed LDS", (long long)G, d.nacc);
   return VQ_OK;
 }
 
@@ -658,13 +670,18 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card)) return rc;
   std::vector<uint8_t> keep;
   prune(t, pred, keep);
+  d.tile_rows = scan_tile_rows(kept_rows(t, keep));
   int64_t rows = 0, kept = 0;
   if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
 
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   const int NI = d.nacc + 2;
-  const size_t part_words = (size_t)std::max<int64_t>(d.ntiles, 1) * d.ngroups * NI;
+  std::string jerr;
+  int32_t grid = d.ntiles > 0 ? jit_dense_grid(d, &jerr) : 0;
+  const bool jit = grid >= 0;
+  if (!jit) grid = dense_grid(d);
+  const size_t part_words = (size_t)std::max<int32_t>(grid, 1) * d.ngroups * NI;
   const size_t out_words = (size_t)d.ngroups * NI;
   void* ar;
   if (int rc = arena_get(c, (part_words + out_words) * 8 + 1024, &ar)) return rc;
@@ -677,9 +694,10 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
 
   if (d.ntiles > 0) {
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    HIP_TRY(launch_agg_dense(d, part, c->stream));
+    if (jit) HIP_TRY(jit_launch_dense(d, grid, part, c->stream));
+    else HIP_TRY(launch_agg_dense(d, grid, part, c->stream));
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    HIP_TRY(launch_agg_final(d, part, fin, c->stream));
+    HIP_TRY(launch_agg_final(d, grid, part, fin, c->stream));
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipMemcpyAsync(hfin, fin, out_words * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -697,6 +715,8 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
     stats->rg_total = t->nrg;
     stats->rg_scanned = kept;
     stats->bytes_alg = rows * slot_bytes_per_row(t, sm);
+    stats->grid = grid;
+    stats->tile_rows = d.tile_rows;
   }
 
   // groups in first-occurrence order (the serial HashAggregate's key order)
@@ -743,7 +763,11 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
         out->ivals[o] = cnt;
         continue;
       }
-      const uint64_t w = r[am.acc[j]];
+      uint64_t w = r[am.acc[j]];
+      if ((a.kind == VQ_AGG_MIN || a.kind == VQ_AGG_MAX) && a.accum == VQ_F64) {
+        const int64_t o = (int64_t)w;  // order-preserving int64 -> IEEE bits
+        w = (uint64_t)(o >= 0 ? o : (int64_t)((uint64_t)o ^ 0x7FFFFFFFFFFFFFFFULL));
+      }
       if (a.kind == VQ_AGG_AVG) {
         const double s = a.arg.is_float ? bits_to_f64((int64_t)w) : (double)(int64_t)w;
         out->fvals[o] = s / (double)cnt;
@@ -778,9 +802,9 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
     d.nslots = 1;
   }
   d.ngroups = 1;
-  d.tile_rows = kDenseTileRows;
   std::vector<uint8_t> keep;
   prune(t, pred, keep);
+  d.tile_rows = scan_tile_rows(kept_rows(t, keep));
   int64_t rows = 0, kept = 0;
   if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
 
@@ -824,7 +848,8 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
     return rc;
   }
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  HIP_TRY(launch_filter(d, bits, tcount, toff, sv->rows, dcount, c->stream, c->ev[1], c->ev[2]));
+  const bool jit = d.ntiles > 0 && jit_available_filter(d, nullptr);
+  HIP_TRY(launch_filter(d, jit, bits, tcount, toff, sv->rows, dcount, c->stream, c->ev[1], c->ev[2]));
   HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   HIP_TRY(hipMemcpyAsync(hp, dcount, 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));

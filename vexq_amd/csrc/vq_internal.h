@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/vexq_gpu.h"
 
 namespace vq {
@@ -251,11 +253,22 @@ __host__ __device__ inline double fix_to_f64(const int64_t* limb, int nlimbs, Fi
 
 // Launchers (defined in the kernel translation units).
 namespace vq {
-hipError_t launch_agg_dense(const QDesc& d, uint64_t* part, hipStream_t s);
-hipError_t launch_agg_final(const QDesc& d, const uint64_t* part, uint64_t* out, hipStream_t s);
-hipError_t launch_filter(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,
+int32_t dense_grid(const QDesc& d);
+size_t dense_lds_bytes(const QDesc& d);
+hipError_t launch_agg_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s);
+hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part, uint64_t* out,
+                            hipStream_t s);
+hipError_t launch_filter(const QDesc& d, bool jit, uint64_t* bits, uint32_t* tile_counts,
                          uint32_t* tile_offsets, uint32_t* out_rows, uint64_t* out_count,
                          hipStream_t s, hipEvent_t ev_main0, hipEvent_t ev_main1);
+
+// Shape-specialized kernels compiled at run time (vq_jit.hip).  jit_dense_grid
+// returns the persistent grid, or -1 (with *err) when no kernel is available.
+int jit_dense_grid(const QDesc& d, std::string* err);
+hipError_t jit_launch_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s);
+bool jit_available_filter(const QDesc& d, std::string* err);
+hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,
+                                  hipStream_t s);
 
 struct JoinBufs {
   // partitioned build side

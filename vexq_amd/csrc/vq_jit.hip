@@ -1,0 +1,279 @@
+// vq_jit.hip — shape-specialized kernels compiled at run time with hipRTC.
+//
+// The query's shape (column types, predicate comparisons, key strides, the
+// aggregate folds) is written as compile-time constants in front of
+// jit_template.hip and compiled once for gfx950; the module is cached under
+// that text, so every later query of the same shape — whatever its literal
+// values — launches the cached kernel.  A compile failure is reported to the
+// caller, which then runs the generic kernels of k_scan.hip.
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "jit_template_src.h"  // generated by vexq_amd/build.py: kJitTemplateSrc
+#include "vq_internal.h"
+
+namespace vq {
+
+namespace {
+
+// Must match vqj::Args in jit_template.hip field for field.
+struct JitArgs {
+  const void* col[8];
+  int64_t ti[8];
+  double tf[8];
+  int64_t fi[8][4];
+  double ff[8][4];
+  int64_t sp_start[kMaxSpans];
+  int64_t sp_rows[kMaxSpans];
+  int64_t sp_tile0[kMaxSpans];
+  int32_t nspans;
+  int32_t tile_rows;
+  int64_t ntiles;
+  uint64_t* part;
+  uint64_t* bits;
+  uint32_t* tile_counts;
+  uint32_t* offsets;
+  const uint32_t* pstart;
+  int64_t* keys;
+  uint32_t* rows;
+  uint64_t* vals;
+  int64_t vstride;
+  int32_t P;
+  int32_t pad_;
+};
+
+struct Module {
+  hipModule_t mod = nullptr;
+  hipFunction_t dense = nullptr;
+  hipFunction_t filter_bits = nullptr;
+  int dense_per_cu = 0;
+  size_t dense_lds = 0;
+};
+
+std::mutex g_mu;
+std::unordered_map<std::string, Module*> g_cache;
+
+bool is_float_op(int op) { return op == ACC_SUM_F || op == ACC_MIN_F || op == ACC_MAX_F; }
+
+void arr(std::string& s, const char* name, const int* v, int n) {
+  s += "static constexpr int ";
+  s += name;
+  s += "[8] = {";
+  for (int i = 0; i < n; i++) s += (i ? "," : "") + std::to_string(v[i]);
+  s += "};\n";
+}
+
+void arr2(std::string& s, const char* name, const int (*v)[4]) {
+  s += "static constexpr int ";
+  s += name;
+  s += "[8][4] = {";
+  for (int i = 0; i < 8; i++) {
+    s += i ? ",{" : "{";
+    for (int k = 0; k < 4; k++) s += (k ? "," : "") + std::to_string(v[i][k]);
+    s += "}";
+  }
+  s += "};\n";
+}
+
+// Flattened predicate terms: (slot, op, ival, fval).
+struct Flat {
+  int nt = 0;
+  int ts[8] = {0}, to[8] = {0};
+  int64_t ti[8] = {0};
+  double tf[8] = {0};
+};
+
+Flat flatten_terms(const QDesc& d) {
+  Flat f;
+  for (int c = 0; c < d.nslots; c++)
+    for (int j = 0; j < d.slot[c].nterms && f.nt < 8; j++) {
+      f.ts[f.nt] = c;
+      f.to[f.nt] = d.slot[c].terms[j].op;
+      f.ti[f.nt] = d.slot[c].terms[j].ival;
+      f.tf[f.nt] = d.slot[c].terms[j].fval;
+      f.nt++;
+    }
+  return f;
+}
+
+int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
+
+std::string prologue(const QDesc& d, const Flat& f) {
+  int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
+  int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
+  for (int c = 0; c < d.nslots; c++) {
+    st[c] = d.slot[c].type;
+    ks[c] = d.slot[c].key_stride;
+  }
+  for (int j = 0; j < d.nacc; j++) {
+    aop[j] = d.acc[j].op;
+    aisf[j] = is_float_op(d.acc[j].op);
+    anf[j] = d.acc[j].e.nfactors;
+    for (int k = 0; k < d.acc[j].e.nfactors; k++) {
+      fk[j][k] = d.acc[j].e.f[k].kind;
+      fs[j][k] = d.acc[j].e.f[k].slot;
+      fo[j][k] = d.acc[j].e.f[k].op;
+      fc[j][k] = d.acc[j].e.f[k].combine;
+    }
+  }
+  std::string s = "#define VQ_JIT_PROLOGUE 1\n";
+  s += "#define VQ_NS " + std::to_string(d.nslots) + "\n";
+  s += "#define VQ_U " + std::to_string(rows_in_flight(d.nslots)) + "\n";
+  s += "#define VQ_G " + std::to_string(d.ngroups) + "\n";
+  s += "#define VQ_NA " + std::to_string(d.nacc) + "\n";
+  s += "#define VQ_NT " + std::to_string(f.nt) + "\n";
+  arr(s, "kST", st, 8);
+  arr(s, "kTS", f.ts, 8);
+  arr(s, "kTO", f.to, 8);
+  arr(s, "kKS", ks, 8);
+  arr(s, "kAOP", aop, 8);
+  arr(s, "kAISF", aisf, 8);
+  arr(s, "kANF", anf, 8);
+  arr2(s, "kFK", fk);
+  arr2(s, "kFS", fs);
+  arr2(s, "kFO", fo);
+  arr2(s, "kFC", fc);
+  return s;
+}
+
+int fail_jit(std::string* err, const std::string& msg) {
+  if (err) *err = msg;
+  return -1;
+}
+
+// Compiles (or finds) the module of this shape.
+Module* get_module(const std::string& pro, std::string* err) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_cache.find(pro);
+  if (it != g_cache.end()) return it->second;
+  const std::string src = pro + kJitTemplateSrc;
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "vq_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    if (err) *err = "hiprtcCreateProgram failed";
+    return nullptr;
+  }
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                        "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops"};
+  hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    if (err) *err = "hipRTC compile failed: " + log.substr(0, 400);
+    return nullptr;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::string code(cs, '\0');
+  hiprtcGetCode(prog, &code[0]);
+  hiprtcDestroyProgram(&prog);
+  Module* m = new Module();
+  if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess ||
+      hipModuleGetFunction(&m->dense, m->mod, "vq_dense") != hipSuccess ||
+      hipModuleGetFunction(&m->filter_bits, m->mod, "vq_filter_bits") != hipSuccess) {
+    delete m;
+    if (err) *err = "hipModuleLoadData failed";
+    return nullptr;
+  }
+  g_cache[pro] = m;
+  return m;
+}
+
+void fill_common(const QDesc& d, const Flat& f, JitArgs& a) {
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < d.nslots; c++) a.col[c] = d.slot[c].p;
+  for (int t = 0; t < f.nt; t++) {
+    a.ti[t] = f.ti[t];
+    a.tf[t] = f.tf[t];
+  }
+  for (int j = 0; j < d.nacc; j++)
+    for (int k = 0; k < d.acc[j].e.nfactors; k++) {
+      a.fi[j][k] = d.acc[j].e.f[k].ival;
+      a.ff[j][k] = d.acc[j].e.f[k].fval;
+    }
+  for (int s = 0; s < d.nspans; s++) {
+    a.sp_start[s] = d.span[s].start;
+    a.sp_rows[s] = d.span[s].rows;
+    a.sp_tile0[s] = d.span[s].tile0;
+  }
+  a.nspans = d.nspans;
+  a.tile_rows = d.tile_rows;
+  a.ntiles = d.ntiles;
+}
+
+hipError_t launch(hipFunction_t fn, unsigned grid, size_t lds, hipStream_t s, JitArgs& a) {
+  size_t sz = sizeof a;
+  void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz,
+                 HIP_LAUNCH_PARAM_END};
+  return hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, (unsigned)lds, s, nullptr, cfg);
+}
+
+bool jit_disabled() {
+  const char* e = getenv("VQ_NO_JIT");
+  return e && *e && *e != '0';
+}
+
+}  // namespace
+
+int jit_dense_grid(const QDesc& d, std::string* err) {
+  if (jit_disabled()) return fail_jit(err, "JIT disabled");
+  const Flat f = flatten_terms(d);
+  Module* m = get_module(prologue(d, f), err);
+  if (!m) return -1;
+  const size_t lds = dense_lds_bytes(d);
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+  }
+  int per_cu = 0;
+  hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->dense, 64, lds);
+  if (per_cu <= 0) per_cu = 1;
+  if (const char* e = getenv("VQ_WAVES_PER_CU")) {
+    const int v = atoi(e);
+    if (v > 0) per_cu = v;
+  }
+  const int64_t g = (int64_t)cus * per_cu;
+  return (int)(d.ntiles < g ? d.ntiles : g);
+}
+
+hipError_t jit_launch_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s) {
+  const Flat f = flatten_terms(d);
+  Module* m = get_module(prologue(d, f), nullptr);
+  if (!m) return hipErrorInvalidDeviceFunction;
+  JitArgs a;
+  fill_common(d, f, a);
+  a.part = part;
+  return launch(m->dense, (unsigned)grid, dense_lds_bytes(d), s, a);
+}
+
+bool jit_available_filter(const QDesc& d, std::string* err) {
+  if (jit_disabled()) return false;
+  const Flat f = flatten_terms(d);
+  return get_module(prologue(d, f), err) != nullptr;
+}
+
+hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,
+                                  hipStream_t s) {
+  const Flat f = flatten_terms(d);
+  Module* m = get_module(prologue(d, f), nullptr);
+  if (!m) return hipErrorInvalidDeviceFunction;
+  JitArgs a;
+  fill_common(d, f, a);
+  a.bits = bits;
+  a.tile_counts = tile_counts;
+  return launch(m->filter_bits, (unsigned)d.ntiles, 0, s, a);
+}
+
+}  // namespace vq

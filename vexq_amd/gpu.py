@@ -113,6 +113,16 @@ class Context:
         self.last_stats = st
         return rows
 
+    def filter_count(self, dt: "DeviceTable", pred) -> int:
+        """Filter -> selection vector left in HBM; returns the row count only."""
+        sv = C.c_void_p()
+        st = abi.Stats()
+        _check(lib().vq_filter(self.h, dt.h, _pp(pred), C.byref(sv), C.byref(st)))
+        n = lib().vq_selvec_count(sv)
+        lib().vq_selvec_destroy(sv)
+        self.last_stats = st
+        return int(n)
+
     def agg_dict(self, dt: "DeviceTable", pred, key_cols, aggs, capacity: int = 4096) -> abi.AggOut:
         keys = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
         arr = abi.agg_array(aggs)

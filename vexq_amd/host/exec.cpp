@@ -1,0 +1,405 @@
+// exec.cpp — the C++ operator layer over libvexqgpu.so (see exec.hpp).
+#include "exec.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace vexq {
+namespace exec {
+
+namespace {
+
+constexpr int kBlockRows = VQ_BLOCK_ROWS;  // storage.BlockRows (storage/format.go:8)
+
+int vq_type_of(const HostColumn& c) { return c.vq_type; }
+
+vq_expr expr_of(const HostTable& t, const AggExpr& a) {
+  if (a.expr.nfactors > 0 || a.col_idx < 0) return a.expr;
+  vq_expr e;
+  memset(&e, 0, sizeof e);
+  e.nfactors = 1;
+  e.is_float = t.columns[a.col_idx].vq_type == VQ_F64;
+  e.f[0].kind = VQ_F_COL;
+  e.f[0].col = a.col_idx;
+  return e;
+}
+
+vq_agg to_vq(const HostTable& t, const AggExpr& a) {
+  vq_agg g;
+  memset(&g, 0, sizeof g);
+  switch (a.kind) {
+    case AggKind::Count: g.kind = VQ_AGG_COUNT; break;
+    case AggKind::Sum: g.kind = VQ_AGG_SUM; break;
+    case AggKind::Avg: g.kind = VQ_AGG_AVG; break;
+    case AggKind::Min: g.kind = VQ_AGG_MIN; break;
+    case AggKind::Max: g.kind = VQ_AGG_MAX; break;
+  }
+  g.accum = a.accum_type == DataType::Float64 ? VQ_F64 : VQ_I64;
+  if (a.kind != AggKind::Count) g.arg = expr_of(t, a);
+  return g;
+}
+
+// Output column type of an aggregate (buildOutputBatch, exec/aggregate.go:780-876):
+// MIN/MAX keep the source type, AVG is float, COUNT int, SUM its accumulator.
+DataType out_type(const HostTable& t, const AggExpr& a) {
+  switch (a.kind) {
+    case AggKind::Count: return DataType::Int64;
+    case AggKind::Avg: return DataType::Float64;
+    case AggKind::Sum: return a.accum_type;
+    default:
+      if (a.expr.nfactors == 0 && a.col_idx >= 0) return t.columns[a.col_idx].type;
+      return a.accum_type;
+  }
+}
+
+// Agg result rows [lo, hi) -> one batch.
+Batch agg_batch(const Schema& schema, const vq_agg_result& r, int nkeys, int naggs,
+                const std::vector<std::shared_ptr<const std::vector<std::string>>>& key_dicts,
+                const std::vector<DataType>& key_types, const std::vector<DataType>& agg_types,
+                int64_t lo, int64_t hi) {
+  Batch b;
+  b.schema = schema;
+  b.length = (int)(hi - lo);
+  for (int k = 0; k < nkeys; k++) {
+    Vector v;
+    v.type = key_types[k];
+    for (int64_t i = lo; i < hi; i++) {
+      if (v.type == DataType::String) v.codes.push_back((uint32_t)r.keys[i * nkeys + k]);
+      else v.i64.push_back(r.keys[i * nkeys + k]);
+    }
+    v.dict = key_dicts[k];
+    b.vectors.push_back(std::move(v));
+  }
+  for (int j = 0; j < naggs; j++) {
+    Vector v;
+    v.type = agg_types[j];
+    for (int64_t i = lo; i < hi; i++) {
+      const int64_t o = i * naggs + j;
+      if (v.type == DataType::Float64) v.f64.push_back(r.fvals[o]);
+      else v.i64.push_back(r.ivals[o]);
+      v.valid.push_back(r.valid[o]);
+    }
+    b.vectors.push_back(std::move(v));
+  }
+  return b;
+}
+
+struct ResultBufs {
+  std::vector<int64_t> keys, counts, first, ivals;
+  std::vector<double> fvals;
+  std::vector<uint8_t> valid;
+  vq_agg_result r{};
+  void alloc(int64_t cap, int nkeys, int naggs) {
+    cap = std::max<int64_t>(cap, 1);
+    keys.assign(cap * std::max(nkeys, 1), 0);
+    counts.assign(cap, 0);
+    first.assign(cap, 0);
+    ivals.assign(cap * std::max(naggs, 1), 0);
+    fvals.assign(cap * std::max(naggs, 1), 0);
+    valid.assign(cap * std::max(naggs, 1), 0);
+    r.capacity = cap;
+    r.ngroups = 0;
+    r.keys = keys.data();
+    r.counts = counts.data();
+    r.first_row = first.data();
+    r.ivals = ivals.data();
+    r.fvals = fvals.data();
+    r.valid = valid.data();
+  }
+};
+
+// ---- filter scan ---------------------------------------------------------
+class GpuFilterScan : public Operator {
+ public:
+  GpuFilterScan(GpuTable* t, std::vector<int> cols, const vq_pred* pred)
+      : t_(t), cols_(std::move(cols)), has_pred_(pred != nullptr) {
+    if (pred) pred_ = *pred;
+    const Schema full = t->host().schema();
+    for (int c : cols_) schema_.fields.push_back(full.fields[c]);
+  }
+
+  Status Next(const Batch** out) override {
+    *out = nullptr;
+    if (!ran_) {
+      vq_selvec* sv = nullptr;
+      int rc = vq_filter(t_->device()->ctx(), t_->handle(), has_pred_ ? &pred_ : nullptr, &sv, nullptr);
+      if (rc) return Status::FromLib(rc, "filter");
+      rows_.resize((size_t)vq_selvec_count(sv));
+      rc = vq_selvec_copy(sv, rows_.data(), (int64_t)rows_.size());
+      vq_selvec_destroy(sv);
+      if (rc) return Status::FromLib(rc, "filter");
+      ran_ = true;
+      // physical 1024-row blocks inside each row group
+      int64_t st = 0;
+      for (int64_t n : t_->host().row_groups) {
+        for (int64_t b = 0; b < n; b += kBlockRows) blocks_.push_back({st + b, std::min<int64_t>(kBlockRows, n - b)});
+        st += n;
+      }
+    }
+    // Filter.Next skips blocks whose selection is empty (exec/filter.go:49-78).
+    while (bi_ < blocks_.size()) {
+      const int64_t b0 = blocks_[bi_].first, bn = blocks_[bi_].second;
+      bi_++;
+      size_t e = ri_;
+      while (e < rows_.size() && rows_[e] < b0 + bn) e++;
+      if (e == ri_) continue;
+      cur_ = Batch();
+      cur_.schema = schema_;
+      cur_.length = (int)(e - ri_);
+      for (size_t i = ri_; i < e; i++) cur_.sel.push_back((uint16_t)(rows_[i] - b0));
+      ri_ = e;
+      for (int c : cols_) {
+        const HostColumn& hc = t_->host().columns[c];
+        Vector v;
+        v.type = hc.type;
+        v.dict = hc.dict;
+        for (int64_t r = b0; r < b0 + bn; r++) {
+          switch (hc.vq_type) {
+            case VQ_U8: v.codes.push_back(((const uint8_t*)hc.data)[r]); break;
+            case VQ_U16: v.codes.push_back(((const uint16_t*)hc.data)[r]); break;
+            case VQ_DATE: v.i64.push_back(((const int32_t*)hc.data)[r]); break;
+            case VQ_I64: v.i64.push_back(((const int64_t*)hc.data)[r]); break;
+            default: v.f64.push_back(((const double*)hc.data)[r]); break;
+          }
+        }
+        if (!hc.dict && (hc.vq_type == VQ_U8 || hc.vq_type == VQ_U16)) {
+          for (uint32_t x : v.codes) v.i64.push_back(x);
+          v.codes.clear();
+        }
+        cur_.vectors.push_back(std::move(v));
+      }
+      *out = &cur_;
+      return Status::Ok();
+    }
+    return Status::Ok();
+  }
+  const Schema& schema() const override { return schema_; }
+  Status Close() override { return Status::Ok(); }
+
+ private:
+  GpuTable* t_;
+  std::vector<int> cols_;
+  bool has_pred_;
+  vq_pred pred_{};
+  Schema schema_;
+  bool ran_ = false;
+  std::vector<uint32_t> rows_;
+  std::vector<std::pair<int64_t, int64_t>> blocks_;
+  size_t bi_ = 0, ri_ = 0;
+  Batch cur_;
+};
+
+// ---- dictionary GROUP BY -----------------------------------------------------
+class GpuParallelHashAggregate : public Operator {
+ public:
+  GpuParallelHashAggregate(GpuTable* t, const vq_pred* pred, std::vector<int> group_by,
+                           std::vector<AggExpr> aggs)
+      : t_(t), has_pred_(pred != nullptr), group_by_(std::move(group_by)), aggs_(std::move(aggs)) {
+    if (pred) pred_ = *pred;
+    const HostTable& h = t->host();
+    const Schema full = h.schema();
+    for (int c : group_by_) {
+      schema_.fields.push_back(full.fields[c]);
+      key_dicts_.push_back(h.columns[c].dict);
+      key_types_.push_back(h.columns[c].type);
+    }
+    for (const AggExpr& a : aggs_) {
+      const DataType ty = out_type(h, a);
+      schema_.fields.push_back({a.out_name, ty, a.kind != AggKind::Count});
+      agg_types_.push_back(ty);
+    }
+  }
+
+  // Work is deferred to the first Next, as in ParallelHashAggregate.Next
+  // (exec/parallel.go:109-120).
+  Status Next(const Batch** out) override {
+    *out = nullptr;
+    if (!done_) {
+      const HostTable& h = t_->host();
+      int64_t cap = 1;
+      for (int c : group_by_) cap *= std::max<int64_t>(1, h.columns[c].dict ? (int64_t)h.columns[c].dict->size() : 1);
+      std::vector<vq_agg> va;
+      for (const AggExpr& a : aggs_) va.push_back(to_vq(h, a));
+      ResultBufs rb;
+      rb.alloc(cap, (int)group_by_.size(), (int)aggs_.size());
+      int rc = vq_agg_dict(t_->device()->ctx(), t_->handle(), has_pred_ ? &pred_ : nullptr,
+                           (int32_t)group_by_.size(), group_by_.data(), (int32_t)va.size(),
+                           va.data(), &rb.r, nullptr);
+      if (rc) return Status::FromLib(rc, "hash aggregate");
+      for (int64_t lo = 0; lo < rb.r.ngroups; lo += kBlockRows)
+        batches_.push_back(agg_batch(schema_, rb.r, (int)group_by_.size(), (int)aggs_.size(),
+                                     key_dicts_, key_types_, agg_types_, lo,
+                                     std::min<int64_t>(rb.r.ngroups, lo + kBlockRows)));
+      done_ = true;
+    }
+    if (pos_ < batches_.size()) *out = &batches_[pos_++];
+    return Status::Ok();
+  }
+  const Schema& schema() const override { return schema_; }
+  Status Close() override { return Status::Ok(); }
+
+ private:
+  GpuTable* t_;
+  bool has_pred_;
+  vq_pred pred_{};
+  std::vector<int> group_by_;
+  std::vector<AggExpr> aggs_;
+  Schema schema_;
+  std::vector<std::shared_ptr<const std::vector<std::string>>> key_dicts_;
+  std::vector<DataType> key_types_, agg_types_;
+  bool done_ = false;
+  std::vector<Batch> batches_;
+  size_t pos_ = 0;
+};
+
+}  // namespace
+
+// ---- small pieces ------------------------------------------------------------
+int Schema::IndexOf(const std::string& name) const {
+  for (size_t i = 0; i < fields.size(); i++)
+    if (fields[i].name == name) return (int)i;
+  return -1;
+}
+
+Status Status::FromLib(int code, const char* what) {
+  Status s;
+  s.code = code;
+  s.msg = std::string("exec: gpu ") + what + ": " + vq_last_error();
+  return s;
+}
+
+DataType AccumTypeFor(AggKind kind, DataType src) {
+  switch (kind) {
+    case AggKind::Avg: return DataType::Float64;
+    case AggKind::Sum: return src == DataType::Float64 ? DataType::Float64 : DataType::Int64;
+    case AggKind::Min:
+    case AggKind::Max:
+      if (src == DataType::Float64) return DataType::Float64;
+      if (src == DataType::String) return DataType::String;
+      return DataType::Int64;
+    default: return DataType::Int64;
+  }
+}
+
+Schema HostTable::schema() const {
+  Schema s;
+  for (const HostColumn& c : columns) s.fields.push_back({c.name, c.type, false});
+  return s;
+}
+
+Status Device::Open(int device, uint64_t budget, std::unique_ptr<Device>* out) {
+  std::unique_ptr<Device> d(new Device());
+  int rc = vq_ctx_create(device, budget, &d->ctx_);
+  if (rc) return Status::FromLib(rc, "open device");
+  *out = std::move(d);
+  return Status::Ok();
+}
+
+Device::~Device() { vq_ctx_destroy(ctx_); }
+
+Status GpuTable::Upload(Device* dev, const HostTable& host, std::unique_ptr<GpuTable>* out) {
+  std::vector<vq_col_desc> cols;
+  for (const HostColumn& c : host.columns) {
+    vq_col_desc d;
+    d.type = vq_type_of(c);
+    d.dict_size = c.dict ? (int32_t)c.dict->size() : 0;
+    d.data = c.data;
+    cols.push_back(d);
+  }
+  std::unique_ptr<GpuTable> t(new GpuTable());
+  t->dev_ = dev;
+  t->host_ = host;
+  int rc = vq_table_upload(dev->ctx(), host.nrows, (int32_t)cols.size(), cols.data(),
+                           (int32_t)host.row_groups.size(), host.row_groups.data(), nullptr, &t->t_);
+  if (rc) return Status::FromLib(rc, "table upload");
+  *out = std::move(t);
+  return Status::Ok();
+}
+
+GpuTable::~GpuTable() { vq_table_destroy(t_); }
+
+std::unique_ptr<Operator> NewGpuFilterScan(GpuTable* table, std::vector<int> cols, const vq_pred* pred) {
+  return std::unique_ptr<Operator>(new GpuFilterScan(table, std::move(cols), pred));
+}
+
+std::unique_ptr<Operator> NewGpuParallelHashAggregate(GpuTable* table, const vq_pred* pred,
+                                                      std::vector<int> group_by,
+                                                      std::vector<AggExpr> aggs) {
+  return std::unique_ptr<Operator>(
+      new GpuParallelHashAggregate(table, pred, std::move(group_by), std::move(aggs)));
+}
+
+// ---- join ----------------------------------------------------------------------
+GpuParallelHashJoinAggregate::GpuParallelHashJoinAggregate(GpuTable* build, int build_key,
+                                                           const vq_pred* build_pred,
+                                                           GpuTable* probe, int probe_key,
+                                                           const vq_pred* probe_pred,
+                                                           std::vector<AggExpr> aggs,
+                                                           JoinOptions opts)
+    : build_(build), probe_(probe), build_key_(build_key), probe_key_(probe_key),
+      has_bpred_(build_pred != nullptr), has_ppred_(probe_pred != nullptr),
+      aggs_(std::move(aggs)), opts_(opts) {
+  if (build_pred) bpred_ = *build_pred;
+  if (probe_pred) ppred_ = *probe_pred;
+  schema_.fields.push_back(build->host().schema().fields[build_key]);
+  for (const AggExpr& a : aggs_)
+    schema_.fields.push_back({a.out_name, out_type(probe->host(), a), a.kind != AggKind::Count});
+}
+
+Status GpuParallelHashJoinAggregate::Setup() {
+  std::vector<vq_agg> va;
+  for (const AggExpr& a : aggs_) va.push_back(to_vq(probe_->host(), a));
+  const int64_t cap = opts_.topk > 0 ? opts_.topk
+                                     : std::max<int64_t>(1, std::min(build_->host().nrows, probe_->host().nrows));
+  ResultBufs rb;
+  rb.alloc(cap, 1, (int)aggs_.size());
+  vq_stats st;
+  int rc = vq_join_agg(build_->device()->ctx(), build_->handle(), build_key_,
+                       has_bpred_ ? &bpred_ : nullptr, probe_->handle(), probe_key_,
+                       has_ppred_ ? &ppred_ : nullptr, (int32_t)va.size(), va.data(),
+                       opts_.radix_bits, opts_.topk, &rb.r, &st);
+  if (rc) return Status::FromLib(rc, "parallel join");
+  stats_.rows = st.build_rows;
+  stats_.keys = st.build_keys;
+  stats_.partitions = st.partitions;
+  stats_.elapsed_ms = st.main_kernel_ms;
+  std::vector<DataType> agg_types;
+  for (size_t j = 0; j < aggs_.size(); j++) agg_types.push_back(schema_.fields[j + 1].type);
+  const std::vector<std::shared_ptr<const std::vector<std::string>>> kd = {nullptr};
+  const std::vector<DataType> kt = {DataType::Int64};
+  for (int64_t lo = 0; lo < rb.r.ngroups; lo += kBlockRows)
+    out_.push_back(agg_batch(schema_, rb.r, 1, (int)aggs_.size(), kd, kt, agg_types, lo,
+                             std::min<int64_t>(rb.r.ngroups, lo + kBlockRows)));
+  return Status::Ok();
+}
+
+Status GpuParallelHashJoinAggregate::Next(const Batch** out) {
+  *out = nullptr;
+  if (!done_setup_) {
+    Status s = Setup();
+    if (!s.ok()) return s;
+    done_setup_ = true;
+  }
+  if (pos_ < out_.size()) *out = &out_[pos_++];
+  return Status::Ok();
+}
+
+std::unique_ptr<GpuParallelHashJoinAggregate> NewGpuParallelHashJoinAggregate(
+    GpuTable* build, int build_key, const vq_pred* build_pred, GpuTable* probe, int probe_key,
+    const vq_pred* probe_pred, std::vector<AggExpr> aggs, JoinOptions opts) {
+  return std::unique_ptr<GpuParallelHashJoinAggregate>(new GpuParallelHashJoinAggregate(
+      build, build_key, build_pred, probe, probe_key, probe_pred, std::move(aggs), opts));
+}
+
+Status Drain(Operator* op, std::vector<Batch>* out) {
+  for (;;) {
+    const Batch* b = nullptr;
+    Status s = op->Next(&b);
+    if (!s.ok()) return s;
+    if (!b) return Status::Ok();
+    out->push_back(*b);
+  }
+}
+
+}  // namespace exec
+}  // namespace vexq
