@@ -21,8 +21,10 @@
 #define VQ_G 6
 #define VQ_NA 5
 #define VQ_NT 1
-#define VQ_KEYSLOT 0
-#define VQ_NV 0
+#define VQ_KEYSLOT 2
+#define VQ_NV 5
+#define VQ_PROBE 1
+#define VQ_KERNELS 7
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
@@ -350,6 +352,9 @@ __device__ __forceinline__ void fold_ldss(u64* w, const Args& a, const i64* row,
 
 using namespace vqj;
 
+// VQ_KERNELS selects what a module holds: 1 dense aggregate, 2 filter bits,
+// 4 join partitioning.
+#if VQ_KERNELS & 1
 // ---------------------------------------------------------------------------
 // Dense GROUP BY (see k_agg_dense in k_scan.hip for the contract).
 // ---------------------------------------------------------------------------
@@ -427,6 +432,9 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
   }
 }
 
+#endif
+
+#if VQ_KERNELS & 2
 // ---------------------------------------------------------------------------
 // Filter pass bits (see k_filter_bits in k_scan.hip): one tile per wave.
 // ---------------------------------------------------------------------------
@@ -455,3 +463,137 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
   }
   if (lane == 0) a.tile_counts[it.t] = count;
 }
+#endif
+
+#if VQ_KERNELS & 4
+// ---------------------------------------------------------------------------
+// Radix partitioning of one join side (exec/parallel_join.go:322-347,422-550):
+// rows that pass the side's predicate go to partition radixHash(key) & (P-1).
+// A 1024-thread workgroup owns one tile of rows.
+//   vq_part_count:   offsets[t * P + p] = rows of tile t in partition p
+//   vq_part_scatter: each row to offsets[t * P + p] + rank, the offsets
+//                    having been turned into absolute positions,
+//                    as (key[, row, values of the aggregate arguments])
+// ---------------------------------------------------------------------------
+namespace vqj {
+
+constexpr int kPB = 1024;  // threads per partitioning workgroup
+
+__device__ __forceinline__ u64 radix_hash(u64 x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+template <int U, int... C>
+__device__ __forceinline__ void pload_one(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u,
+                                          Seq<C...>) {
+  ((raw[C][u] = ld<kST[C]>(a.col[C], r)), ...);
+}
+
+// Rows base + u*kPB + tid, clamped into the tile.
+template <int U>
+__device__ __forceinline__ void pload_rows(const Args& a, i64 base, i64 rend,
+                                           i64 (&raw)[VQ_NS][U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    i64 r = base + u * kPB + threadIdx.x;
+    r = r < rend ? r : rend - 1;
+    pload_one(a, r, raw, u, seq<VQ_NS>{});
+  }
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin.  Workgroup w takes tile
+// (w % 8) * per + w / 8, so each XCD works a contiguous block of tiles: the
+// ends of one tile's per-partition runs and the starts of the next tile's are
+// then written through the same L2 and merge there into whole lines.
+__device__ __forceinline__ i64 xcd_tile(i64 w, i64 ntiles) {
+  constexpr int kXcd = 8;
+  const i64 per = (ntiles + kXcd - 1) / kXcd;
+  const i64 full = ntiles - (per - 1) * kXcd;  // XCDs that get `per` tiles
+  const i64 x = w % kXcd, k = w / kXcd;
+  const i64 start = x < full ? x * per : full * per + (x - full) * (per - 1);
+  return start + k;
+}
+
+__device__ __forceinline__ void ptile(const Args& a, i64 t, i64& row0, i64& rend) {
+  int s = 0;
+  while (s + 1 < a.nspans && t >= a.sp_tile0[s + 1]) s++;
+  row0 = a.sp_start[s] + (t - a.sp_tile0[s]) * (i64)a.tile_rows;
+  const i64 send = a.sp_start[s] + a.sp_rows[s];
+  rend = row0 + a.tile_rows < send ? row0 + a.tile_rows : send;
+}
+
+// Value of aggregate argument J for one row, as 64 bits (float bits or int).
+template <int J>
+__device__ __forceinline__ u64 value_of(const Args& a, const i64* row) {
+  if constexpr (kAISF[J]) return (u64)f64_bits(fold_f<J, 0>(a, row, 0.0));
+  else return (u64)fold_i<J, 0>(a, row, 0);
+}
+
+template <int... J>
+__device__ __forceinline__ void write_values(const Args& a, const i64* row, u32 pos, Seq<J...>) {
+  ((a.vals[(i64)J * a.vstride + pos] = value_of<J>(a, row)), ...);
+}
+
+}  // namespace vqj
+
+extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
+  extern __shared__ u32 hist[];
+  constexpr int U = VQ_U;
+  const i64 t = xcd_tile(blockIdx.x, a.ntiles);
+  for (int p = threadIdx.x; p < a.P; p += kPB) hist[p] = 0;
+  __syncthreads();
+  i64 row0, rend;
+  ptile(a, t, row0, rend);
+  const u64 mask = (u64)a.P - 1;
+  for (i64 base = row0; base < rend; base += kPB * U) {
+    i64 raw[VQ_NS][U];
+    pload_rows<U>(a, base, rend, raw);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      i64 row[VQ_NS];
+#pragma unroll
+      for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
+      if (base + u * kPB + threadIdx.x < rend && pass_all(a, row, seq<VQ_NT>{}))
+        atomicAdd(&hist[radix_hash((u64)row[VQ_KEYSLOT]) & mask], 1u);
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < a.P; p += kPB) a.offsets[t * a.P + p] = hist[p];
+}
+
+extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a) {
+  extern __shared__ u32 lcnt[];
+  constexpr int U = VQ_U;
+  const i64 t = xcd_tile(blockIdx.x, a.ntiles);
+  for (int p = threadIdx.x; p < a.P; p += kPB) lcnt[p] = a.offsets[t * a.P + p];
+  __syncthreads();
+  i64 row0, rend;
+  ptile(a, t, row0, rend);
+  const u64 mask = (u64)a.P - 1;
+  for (i64 base = row0; base < rend; base += kPB * U) {
+    i64 raw[VQ_NS][U];
+    pload_rows<U>(a, base, rend, raw);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      i64 row[VQ_NS];
+#pragma unroll
+      for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
+      const i64 r = base + u * kPB + threadIdx.x;
+      if (r < rend && pass_all(a, row, seq<VQ_NT>{})) {
+        const i64 key = row[VQ_KEYSLOT];
+        const u32 pos = atomicAdd(&lcnt[radix_hash((u64)key) & mask], 1u);
+        a.keys[pos] = key;
+        if constexpr (VQ_PROBE) {
+          a.rows[pos] = (u32)r;
+          if constexpr (VQ_NV > 0) write_values(a, row, pos, seq<VQ_NV>{});
+        }
+      }
+    }
+  }
+}
+#endif

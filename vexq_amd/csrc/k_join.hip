@@ -2,16 +2,18 @@
 // GROUP BY on the join key (ParallelHashJoinAggregate, exec/parallel_join.go).
 //
 // Pipeline, per side (build, then probe):
-//   k_part_count   filter + radixHash + LDS histogram per tile -> counts[t][p]
-//   k_part_colscan per partition, exclusive prefix over tiles + total
-//   k_part_starts  exclusive scan of the partition totals (one block)
-//   k_part_scatter filter again, scatter (key[, row, values]) to its partition
-// then one workgroup per partition:
-//   k_join_agg     build the partition's open-addressing table in LDS from the
-//                  build keys, probe it with the partition's probe rows and
-//                  aggregate into per-slot accumulators in LDS, then emit the
-//                  partition's groups (or its top-k candidates)
-//   k_topk_merge   merge the per-partition candidates (top-k queries)
+//   vq_part_count    (run-time compiled, jit_template.hip) filter + radixHash
+//                    + LDS histogram per tile -> counts[t][p]
+//   k_off_*          counts -> absolute positions (2-D exclusive scan)
+//   vq_part_scatter  (run-time compiled) each passing row to its partition:
+//                    the key, and on the probe side its row id and the values
+//                    of the aggregate arguments
+// then:
+//   k_join_agg       one workgroup per partition: build the partition's
+//                    open-addressing table in LDS from its build keys, probe it
+//                    with its probe rows, aggregate into per-slot accumulators
+//                    in LDS, emit every matched slot as a group record
+//   k_topk           ORDER BY aggregate 0 DESC LIMIT k over the records
 //
 // The partition of a key is the low `bits` of radixHash (radixPart,
 // parallel_join.go:104-109) and its slot inside the partition table the high
@@ -27,185 +29,52 @@
 namespace vq {
 
 constexpr int kBlk = 256;
-constexpr int kJU = 4;
-constexpr int kSlotsPerThread = 8;  // table slots + 1 <= kBlk * kSlotsPerThread
 constexpr int64_t kEmptyKey = INT64_MIN;
 
-__device__ __forceinline__ int64_t jload(const QSlot& s, int64_t r) {
-  switch (s.type) {
-    case VQ_U8: return ((const uint8_t*)s.p)[r];
-    case VQ_U16: return ((const uint16_t*)s.p)[r];
-    case VQ_DATE: return ((const int32_t*)s.p)[r];
-    default: return ((const int64_t*)s.p)[r];
-  }
-}
+// ---- partition offsets: a 2-D exclusive scan over counts[t][p] --------------
+// Position of tile t's rows of partition p:
+//   pstart[p] + sum_{t' < t} counts[t'][p],  pstart = exclusive scan of totals.
+// Tiles are grouped in chunks of kChunk; three coalesced passes:
+//   k_off_chunks  csum[c][p] = sum of counts[t][p] over chunk c
+//   k_off_starts  cpre[c][p] = pstart[p] + sum of csum[c'][p] for c' < c (one block)
+//   k_off_tiles   counts[t][p] <- cpre[c][p] + sum_{t' in c, t' < t} counts[t'][p]
+constexpr int kChunk = 32;
 
-__device__ __forceinline__ bool jcmp(const QSlot& s, const QTerm& t, int64_t x) {
-  if (s.type == VQ_F64) {
-    const double a = bits_to_f64(x), b = t.fval;
-    switch (t.op) {
-      case VQ_EQ: return a == b;
-      case VQ_NE: return a != b;
-      case VQ_LT: return a < b;
-      case VQ_LE: return a <= b;
-      case VQ_GT: return a > b;
-      default: return a >= b;
-    }
-  }
-  switch (t.op) {
-    case VQ_EQ: return x == t.ival;
-    case VQ_NE: return x != t.ival;
-    case VQ_LT: return x < t.ival;
-    case VQ_LE: return x <= t.ival;
-    case VQ_GT: return x > t.ival;
-    default: return x >= t.ival;
-  }
-}
-
-__device__ __forceinline__ double jarith_f(int op, double a, double b) {
-  switch (op) {
-    case VQ_ADD: return a + b;
-    case VQ_SUB: return a - b;
-    case VQ_MUL: return a * b;
-    default: return b == 0.0 ? __longlong_as_double(0x7ff8000000000000LL) : a / b;
-  }
-}
-
-__device__ __forceinline__ int64_t jarith_i(int op, int64_t a, int64_t b) {
-  switch (op) {
-    case VQ_ADD: return (int64_t)((uint64_t)a + (uint64_t)b);
-    case VQ_SUB: return (int64_t)((uint64_t)a - (uint64_t)b);
-    case VQ_MUL: return (int64_t)((uint64_t)a * (uint64_t)b);
-    default:
-      if (b == 0) return 0;
-      if (a == INT64_MIN && b == -1) return INT64_MIN;
-      return a / b;
-  }
-}
-
-// Row-local value of an aggregate argument (one row, already loaded slots).
-__device__ __forceinline__ int64_t slot_val(const int64_t (&raw)[kMaxSlots], int slot) {
-  int64_t x = raw[0];
-#pragma unroll
-  for (int c = 1; c < kMaxSlots; c++)
-    if (slot == c) x = raw[c];
-  return x;
-}
-
-__device__ __forceinline__ uint64_t jexpr(const QExpr& e, const QSlot* slot,
-                                          const int64_t (&raw)[kMaxSlots]) {
-  if (e.is_float) {
-    double v = 0.0;
-    for (int k = 0; k < e.nfactors; k++) {
-      const QFactor f = e.f[k];
-      double x = f.fval;
-      if (f.kind != VQ_F_LIT) {
-        const int64_t xi = slot_val(raw, f.slot);
-        const double c = slot[f.slot].type == VQ_F64 ? bits_to_f64(xi) : (double)xi;
-        x = f.kind == VQ_F_COL ? c
-                               : (f.kind == VQ_F_LIT_OP_COL ? jarith_f(f.op, f.fval, c)
-                                                            : jarith_f(f.op, c, f.fval));
-      }
-      v = k == 0 ? x : jarith_f(f.combine, v, x);
-    }
-    return (uint64_t)f64_to_bits(v);
-  }
-  int64_t v = 0;
-  for (int k = 0; k < e.nfactors; k++) {
-    const QFactor f = e.f[k];
-    int64_t x = f.ival;
-    if (f.kind != VQ_F_LIT) {
-      const int64_t c = slot_val(raw, f.slot);
-      x = f.kind == VQ_F_COL ? c
-                             : (f.kind == VQ_F_LIT_OP_COL ? jarith_i(f.op, f.ival, c)
-                                                          : jarith_i(f.op, c, f.ival));
-    }
-    v = k == 0 ? x : jarith_i(f.combine, v, x);
-  }
-  return (uint64_t)v;
-}
-
-struct TileRange {
-  int64_t row0, rend;
-};
-
-__device__ __forceinline__ TileRange jtile(const JSide& sd, int64_t t) {
-  int s = 0;
-  while (s + 1 < sd.nspans && t >= sd.span[s + 1].tile0) s++;
-  TileRange r;
-  r.row0 = sd.span[s].start + (t - sd.span[s].tile0) * (int64_t)sd.tile_rows;
-  const int64_t send = sd.span[s].start + sd.span[s].rows;
-  r.rend = r.row0 + sd.tile_rows < send ? r.row0 + sd.tile_rows : send;
-  return r;
-}
-
-// Loads one row's slots; returns whether it passes the side's predicate.
-__device__ __forceinline__ bool jrow(const JSide& sd, int64_t r, int64_t (&raw)[kMaxSlots]) {
-  bool pass = true;
-#pragma unroll
-  for (int c = 0; c < kMaxSlots; c++) {
-    if (c < sd.nslots) {
-      raw[c] = jload(sd.slot[c], r);
-      for (int j = 0; j < sd.slot[c].nterms; j++) pass = pass && jcmp(sd.slot[c], sd.slot[c].terms[j], raw[c]);
-    } else {
-      raw[c] = 0;
-    }
-  }
-  return pass;
-}
-
-// counts[t * P + p] = rows of tile t that pass and fall in partition p.
-__global__ __launch_bounds__(kBlk) void k_part_count(const JSide sd, int32_t P,
-                                                     uint32_t* __restrict__ counts) {
-  extern __shared__ uint32_t hist[];
-  const int64_t t = blockIdx.x;
-  for (int p = threadIdx.x; p < P; p += kBlk) hist[p] = 0;
-  __syncthreads();
-  const TileRange tr = jtile(sd, t);
-  const uint64_t mask = (uint64_t)P - 1;
-  for (int64_t base = tr.row0; base < tr.rend; base += kBlk * kJU) {
-#pragma unroll
-    for (int u = 0; u < kJU; u++) {
-      const int64_t r = base + u * kBlk + threadIdx.x;
-      if (r < tr.rend) {
-        int64_t raw[kMaxSlots];
-        if (jrow(sd, r, raw)) {
-          const uint64_t h = radix_hash((uint64_t)slot_val(raw, sd.key_slot));
-          atomicAdd(&hist[h & mask], 1u);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  for (int p = threadIdx.x; p < P; p += kBlk) counts[t * P + p] = hist[p];
-}
-
-// Per partition p: offsets[t*P+p] = sum of counts[t'*P+p] for t' < t; total[p].
-__global__ __launch_bounds__(kBlk) void k_part_colscan(const uint32_t* __restrict__ counts,
-                                                       int64_t ntiles, int32_t P,
-                                                       uint32_t* __restrict__ offsets,
-                                                       uint32_t* __restrict__ totals) {
-  const int p = blockIdx.x * kBlk + threadIdx.x;
+__global__ __launch_bounds__(kBlk) void k_off_chunks(const uint32_t* __restrict__ counts,
+                                                     int64_t ntiles, int32_t P,
+                                                     uint32_t* __restrict__ csum) {
+  const int c = blockIdx.x;
+  const int p = blockIdx.y * kBlk + threadIdx.x;
   if (p >= P) return;
-  uint32_t run = 0;
-  for (int64_t t = 0; t < ntiles; t++) {
-    const uint32_t c = counts[t * P + p];
-    offsets[t * P + p] = run;
-    run += c;
-  }
-  totals[p] = run;
+  const int64_t t0 = (int64_t)c * kChunk;
+  const int64_t t1 = t0 + kChunk < ntiles ? t0 + kChunk : ntiles;
+  uint32_t s = 0;
+  for (int64_t t = t0; t < t1; t++) s += counts[t * P + p];
+  csum[(int64_t)c * P + p] = s;
 }
 
-// pstart[p] = exclusive prefix of totals; pstart[P] = grand total.  One block.
-__global__ __launch_bounds__(1024) void k_part_starts(const uint32_t* __restrict__ totals,
-                                                      int32_t P, uint32_t* __restrict__ pstart) {
+// One block: csum[c][p] -> cpre[c][p] (in place) and pstart[0..P].  The sums
+// are staged in LDS when they fit, so the per-partition walk over chunks runs
+// at LDS latency.
+constexpr int kStageMax = 12 * 1024;
+
+__global__ __launch_bounds__(1024) void k_off_starts(uint32_t* __restrict__ csum, int32_t nchunks,
+                                                     int32_t P, uint32_t* __restrict__ pstart) {
   __shared__ uint32_t s[1024];
+  __shared__ uint32_t stage[kStageMax];
   const int tid = threadIdx.x;
+  const int64_t n = (int64_t)nchunks * P;
+  const bool staged = n <= kStageMax;
+  uint32_t* cs = staged ? stage : csum;
+  if (staged)
+    for (int64_t i = tid; i < n; i += 1024) stage[i] = csum[i];
+  __syncthreads();
   const int per = (P + 1023) / 1024;
   const int lo = tid * per, hi = lo + per < P ? lo + per : P;
-  uint32_t sum = 0;
-  for (int i = lo; i < hi; i++) sum += totals[i];
-  s[tid] = sum;
+  uint32_t mine = 0;
+  for (int p = lo; p < hi; p++)
+    for (int c = 0; c < nchunks; c++) mine += cs[(int64_t)c * P + p];
+  s[tid] = mine;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
     const uint32_t x = tid >= off ? s[tid - off] : 0;
@@ -213,54 +82,34 @@ __global__ __launch_bounds__(1024) void k_part_starts(const uint32_t* __restrict
     s[tid] += x;
     __syncthreads();
   }
-  uint32_t run = s[tid] - sum;
-  for (int i = lo; i < hi; i++) {
-    pstart[i] = run;
-    run += totals[i];
+  uint32_t run = s[tid] - mine;
+  for (int p = lo; p < hi; p++) {
+    pstart[p] = run;
+    for (int c = 0; c < nchunks; c++) {
+      const uint32_t v = cs[(int64_t)c * P + p];
+      cs[(int64_t)c * P + p] = run;
+      run += v;
+    }
   }
   if (tid == 1023) pstart[P] = s[1023];
+  if (staged) {
+    __syncthreads();
+    for (int64_t i = tid; i < n; i += 1024) csum[i] = stage[i];
+  }
 }
 
-// Scatter each passing row to its partition: key (both sides), row id and
-// argument values (probe side).  Position inside a (tile, partition) segment
-// comes from an LDS counter.
-__global__ __launch_bounds__(kBlk) void k_part_scatter(const JSide sd, const JDesc jd, int32_t P,
-                                                       int32_t with_vals,
-                                                       const uint32_t* __restrict__ offsets,
-                                                       const uint32_t* __restrict__ pstart,
-                                                       int64_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ rows,
-                                                       uint64_t* __restrict__ vals, int64_t vstride) {
-  extern __shared__ uint32_t lcnt[];
-  const int64_t t = blockIdx.x;
-  for (int p = threadIdx.x; p < P; p += kBlk) lcnt[p] = pstart[p] + offsets[t * P + p];
-  __syncthreads();
-  const TileRange tr = jtile(sd, t);
-  const uint64_t mask = (uint64_t)P - 1;
-  for (int64_t base = tr.row0; base < tr.rend; base += kBlk * kJU) {
-#pragma unroll
-    for (int u = 0; u < kJU; u++) {
-      const int64_t r = base + u * kBlk + threadIdx.x;
-      if (r < tr.rend) {
-        int64_t raw[kMaxSlots];
-        if (jrow(sd, r, raw)) {
-          const int64_t key = slot_val(raw, sd.key_slot);
-          const uint64_t h = radix_hash((uint64_t)key);
-          const uint32_t pos = atomicAdd(&lcnt[h & mask], 1u);
-          keys[pos] = key;
-          if (with_vals) {
-            rows[pos] = (uint32_t)r;
-            int vi = 0;
-            for (int j = 0; j < jd.naggs; j++) {
-              if (jd.agg[j].e.nfactors > 0) {
-                vals[vi * vstride + pos] = jexpr(jd.agg[j].e, sd.slot, raw);
-                vi++;
-              }
-            }
-          }
-        }
-      }
-    }
+__global__ __launch_bounds__(kBlk) void k_off_tiles(uint32_t* __restrict__ counts, int64_t ntiles,
+                                                    int32_t P, const uint32_t* __restrict__ cpre) {
+  const int c = blockIdx.x;
+  const int p = blockIdx.y * kBlk + threadIdx.x;
+  if (p >= P) return;
+  const int64_t t0 = (int64_t)c * kChunk;
+  const int64_t t1 = t0 + kChunk < ntiles ? t0 + kChunk : ntiles;
+  uint32_t run = cpre[(int64_t)c * P + p];
+  for (int64_t t = t0; t < t1; t++) {
+    const uint32_t v = counts[t * P + p];
+    counts[t * P + p] = run;
+    run += v;
   }
 }
 
@@ -310,49 +159,6 @@ __device__ __forceinline__ void fix_add(unsigned long long* limb, double t, FixS
   atomicAdd(&limb[2], neg ? (unsigned long long)(-(int64_t)c2) : (unsigned long long)c2);
 }
 
-struct Cand {
-  int64_t v;       // sort value (aggregate 0, as the sort sees it)
-  uint32_t first;  // first-occurrence row (tie break)
-  uint32_t slot;   // LDS slot (or global record index)
-};
-
-__device__ __forceinline__ bool cand_better(const Cand& a, const Cand& b) {
-  return a.v > b.v || (a.v == b.v && a.first < b.first);
-}
-
-__device__ __forceinline__ Cand cand_none() {
-  Cand c;
-  c.v = INT64_MIN;
-  c.first = 0xFFFFFFFFu;
-  c.slot = 0xFFFFFFFFu;
-  return c;
-}
-
-__device__ __forceinline__ Cand shfl_cand(const Cand& c, int src) {
-  Cand o;
-  o.v = __shfl((long long)c.v, src, 64);
-  o.first = (uint32_t)__shfl((int)c.first, src, 64);
-  o.slot = (uint32_t)__shfl((int)c.slot, src, 64);
-  return o;
-}
-
-// Best candidate of the block (ties impossible: first rows are distinct).
-__device__ Cand block_best(Cand c, Cand* s_c) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    Cand o = shfl_cand(c, lane ^ m);
-    if (cand_better(o, c)) c = o;
-  }
-  if (lane == 0) s_c[wv] = c;
-  __syncthreads();
-  Cand b = s_c[0];
-  for (int i = 1; i < kBlk / 64; i++)
-    if (cand_better(s_c[i], b)) b = s_c[i];
-  __syncthreads();
-  return b;
-}
-
 // Record layout (64-bit words): key, count, first row, then one word per
 // aggregate (int64 or float64 bits).
 __device__ __forceinline__ void emit_record(const JDesc& jd, uint64_t* rec, int64_t key,
@@ -385,274 +191,385 @@ __device__ __forceinline__ void emit_record(const JDesc& jd, uint64_t* rec, int6
   }
 }
 
-// ExternalSort's key for aggregate 0 (exec/sort.go:130-163): the int64 the
-// row carries, i.e. a float's raw IEEE bits.
-__device__ __forceinline__ int64_t sort_key(const JDesc& jd, const uint64_t* rec) {
-  return jd.naggs > 0 ? (int64_t)rec[3] : 0;
-}
+// One 1024-thread workgroup per partition.  LDS:
+//   table  keys u64[cap+1] | mult u32[cap+1] | pidx u32[cap+1]
+//   pool   slot u32[pool] | cnt u32[pool] | first u32[pool] | acc u64[pool*nwords]
+// Slot `cap` holds the key INT64_MIN, which doubles as the empty marker of the
+// other slots.  Accumulators exist only for build keys a probe row matches: the
+// first match of a slot takes the next pool entry (a racing second claimant
+// loses the compare-and-swap on pidx, uses the winner's entry and leaves its
+// own unused).  flags[0] |= 1 table full, 2 pool full; flags[1] = records;
+// flags[2] = distinct build keys.
+constexpr int kJoinThreads = 1024;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-// One workgroup per partition.  LDS: keys[cap+1], mult[cap+1], cnt[cap+1],
-// first[cap+1], acc[(cap+1) * nwords]; slot `cap` holds the key INT64_MIN,
-// which doubles as the empty marker of the other slots.
-__global__ __launch_bounds__(kBlk) void k_join_agg(const JDesc jd, const JoinPlan plan,
-                                                   const int64_t* __restrict__ b_keys,
-                                                   const uint32_t* __restrict__ b_pstart,
-                                                   const int64_t* __restrict__ p_keys,
-                                                   const uint32_t* __restrict__ p_rows,
-                                                   const uint64_t* __restrict__ p_vals,
-                                                   int64_t vstride,
-                                                   const uint32_t* __restrict__ p_pstart,
-                                                   uint64_t* __restrict__ cand_out,
-                                                   uint64_t* __restrict__ full_out,
-                                                   uint32_t* __restrict__ full_count,
-                                                   uint32_t* __restrict__ flags) {
+__global__ __launch_bounds__(kJoinThreads) void k_join_agg(const JDesc jd, const JoinPlan plan,
+                                                           const int64_t* __restrict__ b_keys,
+                                                           const uint32_t* __restrict__ b_pstart,
+                                                           const int64_t* __restrict__ p_keys,
+                                                           const uint32_t* __restrict__ p_rows,
+                                                           const uint64_t* __restrict__ p_vals,
+                                                           int64_t vstride,
+                                                           const uint32_t* __restrict__ p_pstart,
+                                                           uint64_t* __restrict__ out,
+                                                           uint32_t* __restrict__ flags) {
   extern __shared__ unsigned long long smem[];
-  __shared__ Cand s_c[kBlk / 64];
-  __shared__ uint32_t s_fail;
+  __shared__ uint32_t s_fail, s_nrec, s_nkeys, s_base, s_pool;
   const int p = blockIdx.x;
-  const int cap = plan.cap;
-  const int ns = cap + 1;
+  const int cap = plan.cap, ns = cap + 1, NP = plan.pool;
   const int NW = jd.nwords;
   unsigned long long* keys = smem;
-  unsigned long long* acc = keys + ns;
-  uint32_t* mult = (uint32_t*)(acc + (size_t)ns * NW);
-  uint32_t* cnt = mult + ns;
-  uint32_t* first = cnt + ns;
+  unsigned long long* acc = keys + ns;                       // [NP * NW]
+  uint32_t* mult = (uint32_t*)(acc + (size_t)NP * NW);
+  uint32_t* pidx = mult + ns;
+  uint32_t* pslot = pidx + ns;
+  uint32_t* cnt = pslot + NP;
+  uint32_t* first = cnt + NP;
   const int lg = __ffs(cap) - 1;  // log2(cap)
-  if (threadIdx.x == 0) s_fail = 0;
-  for (int i = threadIdx.x; i < ns; i += kBlk) {
+  if (threadIdx.x == 0) {
+    s_fail = 0;
+    s_nrec = 0;
+    s_nkeys = 0;
+    s_pool = 0;
+  }
+  for (int i = threadIdx.x; i < ns; i += kJoinThreads) {
     keys[i] = (unsigned long long)kEmptyKey;
     mult[i] = 0;
+    pidx[i] = kNone;
+  }
+  for (int i = threadIdx.x; i < NP; i += kJoinThreads) {
     cnt[i] = 0;
-    first[i] = 0xFFFFFFFFu;
+    first[i] = kNone;
+  }
+  for (int i = threadIdx.x; i < NP * NW; i += kJoinThreads) {
+    const int w = i % NW;
+    unsigned long long init = 0;
     for (int j = 0; j < jd.naggs; j++) {
       const JAgg& a = jd.agg[j];
-      for (int k = 0; k < a.nwords; k++) {
-        unsigned long long init = 0;
+      if (w >= a.word0 && w < a.word0 + a.nwords) {
         if (a.kind == VQ_AGG_MIN) init = (unsigned long long)INT64_MAX;
         if (a.kind == VQ_AGG_MAX) init = (unsigned long long)INT64_MIN;
-        acc[(size_t)i * NW + a.word0 + k] = init;
       }
     }
+    acc[i] = init;
   }
   __syncthreads();
 
-  // build: insert the partition's build keys (multiplicity = rows per key)
+  // build: the partition's build keys; multiplicity = build rows per key
   const uint32_t b0 = b_pstart[p], b1 = b_pstart[p + 1];
-  for (uint32_t e = b0 + threadIdx.x; e < b1; e += kBlk) {
+  for (uint32_t e = b0 + threadIdx.x; e < b1; e += kJoinThreads) {
     const int64_t key = b_keys[e];
     if (key == kEmptyKey) {
-      atomicAdd(&mult[cap], 1u);
+      if (atomicAdd(&mult[cap], 1u) == 0) atomicAdd(&s_nkeys, 1u);
       continue;
     }
     const uint64_t h = radix_hash((uint64_t)key);
-    uint32_t s = lg ? (uint32_t)(h >> (64 - lg)) : 0;
+    uint32_t sl = lg ? (uint32_t)(h >> (64 - lg)) : 0;
     bool done = false;
     for (int probe = 0; probe < cap; probe++) {
       const unsigned long long old =
-          atomicCAS(&keys[s], (unsigned long long)kEmptyKey, (unsigned long long)key);
+          atomicCAS(&keys[sl], (unsigned long long)kEmptyKey, (unsigned long long)key);
       if (old == (unsigned long long)kEmptyKey || old == (unsigned long long)key) {
-        atomicAdd(&mult[s], 1u);
+        atomicAdd(&mult[sl], 1u);
+        if (old == (unsigned long long)kEmptyKey) atomicAdd(&s_nkeys, 1u);
         done = true;
         break;
       }
-      s = (s + 1) & (uint32_t)(cap - 1);
+      sl = (sl + 1) & (uint32_t)(cap - 1);
     }
     if (!done) s_fail = 1;
   }
   __syncthreads();
   if (s_fail) {
-    if (threadIdx.x == 0) atomicOr(&flags[0], 1u);
+    if (threadIdx.x == 0) atomicOr(&flags[0], s_fail);
     return;
   }
+  if (threadIdx.x == 0 && s_nkeys) atomicAdd(&flags[2], s_nkeys);
 
-  // probe + aggregate
+  // probe + aggregate; each thread has kPU entries' loads in flight
   const uint32_t q0 = p_pstart[p], q1 = p_pstart[p + 1];
-  for (uint32_t e = q0 + threadIdx.x; e < q1; e += kBlk) {
-    const int64_t key = p_keys[e];
-    int s = -1;
-    if (key == kEmptyKey) {
-      s = mult[cap] ? cap : -1;
-    } else {
-      const uint64_t h = radix_hash((uint64_t)key);
-      uint32_t i = lg ? (uint32_t)(h >> (64 - lg)) : 0;
-      for (int probe = 0; probe < cap; probe++) {
-        const int64_t k = (int64_t)keys[i];
-        if (k == kEmptyKey) break;
-        if (k == key) {
-          s = (int)i;
-          break;
-        }
-        i = (i + 1) & (uint32_t)(cap - 1);
-      }
+  constexpr int kPU = 4;
+  for (uint32_t e0 = q0 + threadIdx.x; e0 < q1; e0 += kJoinThreads * kPU) {
+    int64_t k_[kPU];
+    uint32_t r_[kPU];
+    uint64_t v_[kPU][2];
+#pragma unroll
+    for (int u = 0; u < kPU; u++) {
+      const uint32_t e = e0 + u * kJoinThreads;
+      const uint32_t ec = e < q1 ? e : q1 - 1;
+      k_[u] = p_keys[ec];
+      r_[u] = p_rows[ec];
+      v_[u][0] = plan.nvals > 0 ? p_vals[ec] : 0;
+      v_[u][1] = plan.nvals > 1 ? p_vals[vstride + ec] : 0;
     }
-    if (s < 0) continue;
-    const uint32_t row = p_rows[e];
-    atomicAdd(&cnt[s], 1u);
-    atomicMin(&first[s], row);
-    int vi = 0;
-    for (int j = 0; j < jd.naggs; j++) {
-      const JAgg& a = jd.agg[j];
-      if (a.e.nfactors == 0) continue;
-      const uint64_t v = p_vals[vi * vstride + e];
-      vi++;
-      unsigned long long* w = &acc[(size_t)s * NW + a.word0];
-      switch (a.kind) {
-        case VQ_AGG_SUM:
-        case VQ_AGG_AVG:
-          if (a.e.is_float) fix_add(w, bits_to_f64((int64_t)v), a.fs);
-          else atomicAdd(w, (unsigned long long)v);
-          break;
-        case VQ_AGG_MIN: {
-          const int64_t o = a.accum == VQ_F64 ? f64_ord(bits_to_f64((int64_t)v)) : (int64_t)v;
-          if (!(a.accum == VQ_F64 && bits_to_f64((int64_t)v) != bits_to_f64((int64_t)v)))
-            atomicMin((long long*)w, (long long)o);
-          break;
+#pragma unroll
+    for (int u = 0; u < kPU; u++) {
+      const uint32_t e = e0 + u * kJoinThreads;
+      if (e >= q1) continue;
+      const int64_t key = k_[u];
+      int sl = -1;
+      if (key == kEmptyKey) {
+        sl = mult[cap] ? cap : -1;
+      } else {
+        const uint64_t h = radix_hash((uint64_t)key);
+        uint32_t i = lg ? (uint32_t)(h >> (64 - lg)) : 0;
+        for (int probe = 0; probe < cap; probe++) {
+          const int64_t k = (int64_t)keys[i];
+          if (k == kEmptyKey) break;
+          if (k == key) {
+            sl = (int)i;
+            break;
+          }
+          i = (i + 1) & (uint32_t)(cap - 1);
         }
-        case VQ_AGG_MAX: {
-          const int64_t o = a.accum == VQ_F64 ? f64_ord(bits_to_f64((int64_t)v)) : (int64_t)v;
-          if (!(a.accum == VQ_F64 && bits_to_f64((int64_t)v) != bits_to_f64((int64_t)v)))
-            atomicMax((long long*)w, (long long)o);
-          break;
+      }
+      if (sl < 0) continue;
+      uint32_t pi = pidx[sl];
+      if (pi == kNone) {
+        const uint32_t mine = atomicAdd(&s_pool, 1u);
+        if (mine >= (uint32_t)NP) {
+          s_fail = 2;
+          continue;
         }
-        default:
-          break;
+        const uint32_t old = atomicCAS(&pidx[sl], kNone, mine);
+        pi = old == kNone ? mine : old;
+        if (old == kNone) pslot[mine] = (uint32_t)sl;
+      }
+      if (atomicAdd(&cnt[pi], 1u) == 0) atomicAdd(&s_nrec, 1u);
+      atomicMin(&first[pi], r_[u]);
+      int vi = 0;
+      for (int j = 0; j < jd.naggs; j++) {
+        const JAgg& a = jd.agg[j];
+        if (a.e.nfactors == 0) continue;
+        const uint64_t v = vi < 2 ? v_[u][vi] : p_vals[(int64_t)vi * vstride + e];
+        vi++;
+        unsigned long long* w = &acc[(size_t)pi * NW + a.word0];
+        switch (a.kind) {
+          case VQ_AGG_SUM:
+          case VQ_AGG_AVG:
+            if (a.e.is_float) fix_add(w, bits_to_f64((int64_t)v), a.fs);
+            else atomicAdd(w, (unsigned long long)v);
+            break;
+          case VQ_AGG_MIN:
+          case VQ_AGG_MAX: {
+            const double dv = bits_to_f64((int64_t)v);
+            if (a.accum == VQ_F64 && dv != dv) break;  // NaN never wins a compare
+            const int64_t o = a.accum == VQ_F64 ? f64_ord(dv) : (int64_t)v;
+            if (a.kind == VQ_AGG_MIN) atomicMin((long long*)w, (long long)o);
+            else atomicMax((long long*)w, (long long)o);
+            break;
+          }
+          default:
+            break;
+        }
       }
     }
   }
   __syncthreads();
-
-  const int RW = 3 + jd.naggs;
-  if (plan.topk <= 0) {
-    // full output: every matched slot, appended through a global counter
-    for (int i = threadIdx.x; i < ns; i += kBlk) {
-      if (cnt[i] == 0) continue;
-      const int64_t key = i == cap ? kEmptyKey : (int64_t)keys[i];
-      const uint32_t at = atomicAdd(full_count, 1u);
-      emit_record(jd, full_out + (size_t)at * RW, key, cnt[i], mult[i], first[i],
-                  &acc[(size_t)i * NW]);
-    }
+  if (s_fail) {
+    if (threadIdx.x == 0) atomicOr(&flags[0], s_fail);
     return;
   }
 
-  // top-k candidates of this partition: each thread ranks its own slots once
-  // (at most kSlotsPerThread of them), then k rounds of a block-wide best pick.
-  uint64_t* my_out = cand_out + (size_t)p * plan.topk * RW;
-  uint64_t recbuf[3 + VQ_MAX_AGGS];
-  Cand mine[kSlotsPerThread];
-#pragma unroll
-  for (int k = 0; k < kSlotsPerThread; k++) {
-    const int i = threadIdx.x + k * kBlk;
-    mine[k] = cand_none();
-    if (i < ns && cnt[i] != 0) {
-      const int64_t key = i == cap ? kEmptyKey : (int64_t)keys[i];
-      emit_record(jd, recbuf, key, cnt[i], mult[i], first[i], &acc[(size_t)i * NW]);
-      mine[k].v = sort_key(jd, recbuf);
-      mine[k].first = first[i];
-      mine[k].slot = (uint32_t)i;
-    }
+  // emit every matched group
+  if (threadIdx.x == 0) s_base = s_nrec ? atomicAdd(&flags[1], s_nrec) : 0;
+  __syncthreads();
+  if (s_nrec == 0) return;
+  const uint32_t used = s_pool < (uint32_t)NP ? s_pool : (uint32_t)NP;
+  __syncthreads();
+  if (threadIdx.x == 0) s_nrec = 0;
+  __syncthreads();
+  const int RW = 3 + jd.naggs;
+  for (uint32_t i = threadIdx.x; i < used; i += kJoinThreads) {
+    if (cnt[i] == 0) continue;
+    const uint32_t sl = pslot[i];
+    const int64_t key = (int)sl == cap ? kEmptyKey : (int64_t)keys[sl];
+    const uint32_t at = s_base + atomicAdd(&s_nrec, 1u);
+    emit_record(jd, out + (size_t)at * RW, key, cnt[i], mult[sl], first[i], &acc[(size_t)i * NW]);
   }
-  for (int round = 0; round < plan.topk; round++) {
-    Cand best = cand_none();
+}
+
+// ---- top-k over the group records (ExternalSort DESC + Limit) ----------------
+// One block.  Each thread keeps its kMaxK best records (sorted insertion),
+// each wave merges its 64 lists by k rounds of a wave-wide best pick, and
+// wave 0 merges the waves' lists the same way.
+constexpr int kMaxK = 16;
+constexpr int kTopThreads = 256;
+
+struct KItem {
+  int64_t v;
+  uint32_t first;
+  uint32_t idx;
+};
+
+__device__ __forceinline__ bool kbetter(const KItem& a, const KItem& b) {
+  return a.v > b.v || (a.v == b.v && a.first < b.first);
+}
+
+__device__ __forceinline__ KItem knone() {
+  KItem k;
+  k.v = INT64_MIN;
+  k.first = 0xFFFFFFFFu;
+  k.idx = 0xFFFFFFFFu;
+  return k;
+}
+
+__device__ __forceinline__ KItem kshfl(const KItem& c, int src) {
+  KItem o;
+  o.v = __shfl((long long)c.v, src, 64);
+  o.first = (uint32_t)__shfl((int)c.first, src, 64);
+  o.idx = (uint32_t)__shfl((int)c.idx, src, 64);
+  return o;
+}
+
+// k rounds: the wave's best remaining head wins, its owner advances.
+// Lane `r` of the result holds round r's winner.
+__device__ __forceinline__ KItem wave_topk(KItem (&list)[kMaxK], int k) {
+  const int lane = threadIdx.x & 63;
+  int pos = 0;
+  KItem mine = knone();
+  for (int r = 0; r < k; r++) {
+    KItem head = pos < kMaxK ? list[0] : knone();
 #pragma unroll
-    for (int k = 0; k < kSlotsPerThread; k++)
-      if (cand_better(mine[k], best)) best = mine[k];
-    const Cand b = block_best(best, s_c);
+    for (int q = 1; q < kMaxK; q++)
+      if (q == pos) head = list[q];
+    KItem best = head;
+    int owner = lane;
 #pragma unroll
-    for (int k = 0; k < kSlotsPerThread; k++)
-      if (mine[k].slot == b.slot) mine[k] = cand_none();
-    if (threadIdx.x == 0) {
-      uint64_t* rec = my_out + (size_t)round * RW;
-      if (b.slot == 0xFFFFFFFFu) {
-        for (int k = 0; k < RW; k++) rec[k] = 0;
-        rec[2] = 0xFFFFFFFFFFFFFFFFULL;  // no candidate
-      } else {
-        const int i = (int)b.slot;
-        const int64_t key = i == cap ? kEmptyKey : (int64_t)keys[i];
-        emit_record(jd, rec, key, cnt[i], mult[i], first[i], &acc[(size_t)i * NW]);
+    for (int m = 32; m >= 1; m >>= 1) {
+      const KItem o = kshfl(best, lane ^ m);
+      const int oo = __shfl(owner, lane ^ m, 64);
+      if (kbetter(o, best) || (!kbetter(best, o) && oo < owner && o.idx == best.idx)) {
+        best = o;
+        owner = oo;
       }
     }
+    if (lane == owner && best.idx != 0xFFFFFFFFu) pos++;
+    if (lane == r) mine = best;
+  }
+  return mine;
+}
+
+__device__ __forceinline__ void kinsert(KItem (&list)[kMaxK], const KItem& x) {
+  if (!kbetter(x, list[kMaxK - 1])) return;
+  KItem c = x;
+#pragma unroll
+  for (int q = 0; q < kMaxK; q++) {
+    if (kbetter(c, list[q])) {
+      const KItem t = list[q];
+      list[q] = c;
+      c = t;
+    }
   }
 }
 
-// Merge P * topk candidate records into the global top-k (one block).
-__global__ __launch_bounds__(kBlk) void k_topk_merge(const JDesc jd, int32_t P, int32_t topk,
-                                                     uint64_t* __restrict__ cand,
-                                                     uint64_t* __restrict__ out,
-                                                     uint32_t* __restrict__ out_n) {
-  __shared__ Cand s_c[kBlk / 64];
+// Block b of the grid ranks records [b*chunk, (b+1)*chunk) of n and writes
+// its best topk as full records to out[b*topk ...] (missing ones with first
+// row ~0).  Run with a grid of kTopBlocks over the group records, then with
+// one block over the kTopBlocks*topk candidates; the single-block run also
+// writes the count.
+constexpr int kTopBlocks = 128;
+
+__global__ __launch_bounds__(kTopThreads) void k_topk(const JDesc jd, int32_t topk,
+                                                       const uint64_t* __restrict__ recs,
+                                                       const uint32_t* __restrict__ n_dev,
+                                                       uint32_t n_static,
+                                                       uint64_t* __restrict__ out,
+                                                       uint32_t* __restrict__ out_n) {
+  __shared__ KItem s_w[kTopThreads / 64][kMaxK];
   const int RW = 3 + jd.naggs;
-  const int64_t n = (int64_t)P * topk;
-  int got = 0;
-  for (int round = 0; round < topk; round++) {
-    Cand best = cand_none();
-    for (int64_t i = threadIdx.x; i < n; i += kBlk) {
-      const uint64_t* rec = cand + i * RW;
-      if (rec[2] == 0xFFFFFFFFFFFFFFFFULL) continue;
-      Cand c;
-      c.v = sort_key(jd, rec);
-      c.first = (uint32_t)rec[2];
-      c.slot = (uint32_t)i;
-      if (cand_better(c, best)) best = c;
-    }
-    const Cand b = block_best(best, s_c);
-    if (b.slot == 0xFFFFFFFFu) break;
-    if (threadIdx.x == 0) {
-      uint64_t* rec = cand + (size_t)b.slot * RW;
-      for (int k = 0; k < RW; k++) out[(size_t)round * RW + k] = rec[k];
-      rec[2] = 0xFFFFFFFFFFFFFFFFULL;
-    }
-    got++;
-    __syncthreads();
+  const uint32_t n = n_dev ? *n_dev : n_static;
+  const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  KItem list[kMaxK];
+#pragma unroll
+  for (int q = 0; q < kMaxK; q++) list[q] = knone();
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kTopThreads) {
+    const uint64_t* r = recs + (size_t)i * RW;
+    if (r[2] == ~0ULL) continue;  // a missing candidate
+    KItem x;
+    x.v = jd.naggs > 0 ? (int64_t)r[3] : 0;
+    x.first = (uint32_t)r[2];
+    x.idx = i;
+    kinsert(list, x);
   }
-  if (threadIdx.x == 0) *out_n = (uint32_t)got;
+  const KItem w = wave_topk(list, topk);
+  if (lane < topk) s_w[wv][lane] = w;
+  __syncthreads();
+  if (wv != 0) return;
+#pragma unroll
+  for (int q = 0; q < kMaxK; q++) list[q] = knone();
+  for (int e = lane; e < (kTopThreads / 64) * topk; e += 64) kinsert(list, s_w[e / topk][e % topk]);
+  const KItem f = wave_topk(list, topk);
+  uint64_t* o = out + (size_t)blockIdx.x * topk * RW;
+  if (lane < topk) {
+    if (f.idx != 0xFFFFFFFFu) {
+      for (int q = 0; q < RW; q++) o[(size_t)lane * RW + q] = recs[(size_t)f.idx * RW + q];
+    } else {
+      for (int q = 0; q < RW; q++) o[(size_t)lane * RW + q] = 0;
+      o[(size_t)lane * RW + 2] = ~0ULL;
+    }
+  }
+  const uint32_t got = (uint32_t)__popcll(__ballot(lane < topk && f.idx != 0xFFFFFFFFu));
+  if (lane == 0 && out_n) *out_n = got;
 }
 
-static size_t join_lds_bytes(const JDesc& jd, const JoinPlan& plan) {
-  const size_t ns = (size_t)plan.cap + 1;
-  return ns * 8 + ns * jd.nwords * 8 + ns * 4 * 3;
+size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords) {
+  const size_t ns = (size_t)cap + 1;
+  return ns * (8 + 4 + 4) + (size_t)pool * (4 + 4 + 4 + 8 * (size_t)nwords);
+}
+
+// Turns counts[t][p] (written by the partition-count kernel) into absolute
+// positions and pstart[0..P].
+static void offsets(uint32_t* counts, int64_t ntiles, int32_t P, uint32_t* csum, uint32_t* pstart,
+                    hipStream_t s) {
+  const unsigned nch = (unsigned)((ntiles + kChunk - 1) / kChunk);
+  const dim3 g(nch, (unsigned)((P + kBlk - 1) / kBlk));
+  hipLaunchKernelGGL(k_off_chunks, g, dim3(kBlk), 0, s, counts, ntiles, P, csum);
+  hipLaunchKernelGGL(k_off_starts, dim3(1), dim3(1024), 0, s, csum, (int32_t)nch, P, pstart);
+  hipLaunchKernelGGL(k_off_tiles, g, dim3(kBlk), 0, s, counts, ntiles, P, csum);
 }
 
 hipError_t launch_join(const JSide& b, const JSide& p, const JDesc& jd, const JoinPlan& plan,
                        const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
   const int P = plan.P;
-  const size_t hist_bytes = (size_t)P * 4;
-  const unsigned cs_blocks = (unsigned)((P + kBlk - 1) / kBlk);
   if (ev) hipEventRecord(ev[0], s);
-  hipMemsetAsync(B.flags, 0, sizeof(uint32_t) * 2, s);
-  // build side
+  hipMemsetAsync(B.flags, 0, sizeof(uint32_t) * 4, s);
   if (b.ntiles > 0) {
-    hipLaunchKernelGGL(k_part_count, dim3((unsigned)b.ntiles), dim3(kBlk), hist_bytes, s, b, P,
-                       B.b_counts);
-    hipLaunchKernelGGL(k_part_colscan, dim3(cs_blocks), dim3(kBlk), 0, s, B.b_counts, b.ntiles, P,
-                       B.b_offsets, B.b_pstart + P + 1);
-    hipLaunchKernelGGL(k_part_starts, dim3(1), dim3(1024), 0, s, B.b_pstart + P + 1, P, B.b_pstart);
-    hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)b.ntiles), dim3(kBlk), hist_bytes, s, b, jd,
-                       P, 0, B.b_offsets, B.b_pstart, B.b_keys, (uint32_t*)nullptr,
-                       (uint64_t*)nullptr, (int64_t)0);
+    hipError_t e = jit_launch_part(b, nullptr, P, B.b_counts, nullptr, nullptr, nullptr, 0, s, true);
+    if (e != hipSuccess) return e;
+    offsets(B.b_counts, b.ntiles, P, B.b_csum, B.b_pstart, s);
+    e = jit_launch_part(b, nullptr, P, B.b_counts, B.b_keys, nullptr, nullptr, 0, s, false);
+    if (e != hipSuccess) return e;
   } else {
     hipMemsetAsync(B.b_pstart, 0, sizeof(uint32_t) * (P + 1), s);
   }
-  // probe side
   if (p.ntiles > 0) {
-    hipLaunchKernelGGL(k_part_count, dim3((unsigned)p.ntiles), dim3(kBlk), hist_bytes, s, p, P,
-                       B.p_counts);
-    hipLaunchKernelGGL(k_part_colscan, dim3(cs_blocks), dim3(kBlk), 0, s, B.p_counts, p.ntiles, P,
-                       B.p_offsets, B.p_pstart + P + 1);
-    hipLaunchKernelGGL(k_part_starts, dim3(1), dim3(1024), 0, s, B.p_pstart + P + 1, P, B.p_pstart);
-    hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)p.ntiles), dim3(kBlk), hist_bytes, s, p, jd,
-                       P, 1, B.p_offsets, B.p_pstart, B.p_keys, B.p_rows, B.p_vals, B.cap_p);
+    hipError_t e = jit_launch_part(p, &jd, P, B.p_counts, nullptr, nullptr, nullptr, 0, s, true);
+    if (e != hipSuccess) return e;
+    offsets(B.p_counts, p.ntiles, P, B.p_csum, B.p_pstart, s);
+    e = jit_launch_part(p, &jd, P, B.p_counts, B.p_keys, B.p_rows, B.p_vals, B.cap_p, s, false);
+    if (e != hipSuccess) return e;
   } else {
     hipMemsetAsync(B.p_pstart, 0, sizeof(uint32_t) * (P + 1), s);
   }
   if (ev) hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_join_agg, dim3((unsigned)P), dim3(kBlk), join_lds_bytes(jd, plan), s, jd,
+  const size_t lds = join_lds_for(plan.cap, plan.pool, jd.nwords);
+  if (lds > 64 * 1024)
+    hipFuncSetAttribute((const void*)k_join_agg, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        plan.lds_max);
+  hipLaunchKernelGGL(k_join_agg, dim3((unsigned)P), dim3(kJoinThreads), lds, s, jd,
                      plan, B.b_keys, B.b_pstart, B.p_keys, B.p_rows, B.p_vals, B.cap_p,
-                     B.p_pstart, B.cand, B.out, B.flags + 1, B.flags);
+                     B.p_pstart, B.recs, B.flags);
   if (ev) hipEventRecord(ev[2], s);
-  if (plan.topk > 0)
-    hipLaunchKernelGGL(k_topk_merge, dim3(1), dim3(kBlk), 0, s, jd, P, plan.topk, B.cand, B.out,
-                       B.flags + 1);
+  if (plan.topk > 0) {
+    hipLaunchKernelGGL(k_topk, dim3(kTopBlocks), dim3(kTopThreads), 0, s, jd, plan.topk, B.recs,
+                       B.flags + 1, 0u, B.cand, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_topk, dim3(1), dim3(kTopThreads), 0, s, jd, plan.topk, B.cand,
+                       (const uint32_t*)nullptr, (uint32_t)(kTopBlocks * plan.topk), B.out,
+                       B.flags + 3);
+  }
   if (ev) hipEventRecord(ev[3], s);
   return hipGetLastError();
 }

@@ -577,6 +577,15 @@ int32_t scan_tile_rows(int64_t rows) {
   return (int32_t)t;
 }
 
+// Tile rows of a partitioning pass (1024-thread workgroups, 4 rows per thread
+// per step): ~384 tiles at full size, which keeps a tile's rows of one
+// partition in long contiguous runs; never fewer than 4096 rows.
+int32_t join_tile_rows(int64_t rows) {
+  int64_t t = 4096;
+  while (t < (1 << 20) && t * 384 < rows) t *= 2;
+  return (int32_t)t;
+}
+
 int64_t kept_rows(const vq_table* t, const std::vector<uint8_t>& keep) {
   int64_t n = 0;
   for (int32_t g = 0; g < t->nrg; g++)
@@ -998,7 +1007,6 @@ int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg*
   }
   fill_slot_cols(t, sm, sd.slot);
   sd.nslots = sm.n;
-  sd.tile_rows = 256 * 4 * 16;
   return VQ_OK;
 }
 
@@ -1011,7 +1019,7 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
   if (!c || !build || !probe || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
   if (build->ctx != c || probe->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
   if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs)) return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
-  if (topk < 0 || topk > 64) return fail(VQ_E_UNSUPPORTED, "exec: gpu: top-k of %d", topk);
+  if (topk < 0) return fail(VQ_E_ARG, "exec: gpu: top-k of %d", topk);
   if (radix_bits > 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d radix bits", radix_bits);
   JDesc jd;
   memset(&jd, 0, sizeof jd);
@@ -1054,121 +1062,153 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     nw += a.nwords;
   }
   jd.nwords = nw;
+  std::string jerr_;
 
-  // partitioning plan: partitions small enough for an LDS table
-  const int64_t est = build->nrows;  // footer row count, as buildSideRadixOptions uses
-  int bits = radix_bits >= 0 ? radix_bits : std::max(0, ilog2_ceil((uint64_t)(est + 511) / 512));
-  const size_t slot_bytes = 8 + 8 * (size_t)nw + 12;
+  if (!jit_part_available(bs, nullptr, &jerr_) || !jit_part_available(ps, &jd, &jerr_))
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr_.c_str());
+
+  // Partitioning plan.  Each partition's table lives in one workgroup's LDS:
+  // cap slots of key, multiplicity and pool index, plus a pool of accumulator
+  // entries (count, first row, accumulator words) for the keys a probe row
+  // matches.  The build-side estimate is
+  // the scan's row count before filtering, as buildSideRadixOptions uses
+  // (planner/parallel_join.go:327-339); partitions are sized so that count
+  // fills at most 3/4 of a table.  A table that still fills up reports an
+  // overflow and the query is repartitioned one bit finer.
+  const int64_t est = std::max<int64_t>(build->nrows, 1);
+  // the largest table (<= 4096 slots) whose pool-of-half-as-many entries fits
+  // the device's per-workgroup LDS
+  int lds_max = 0;
+  hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+  if (lds_max <= 0) lds_max = 64 * 1024;
+  lds_max -= 1024;  // the kernel's own static LDS
+  int32_t cap_max = 4096;
+  while (cap_max > 8 && join_lds_for(cap_max, std::max(64, cap_max / 2), nw) > (size_t)lds_max)
+    cap_max /= 2;
+  const int32_t pool_max = cap_max / 2;
+  int bits = radix_bits;
+  if (bits < 0) {
+    bits = 0;
+    while (bits < 16 && (double)est / (1 << bits) > 0.75 * cap_max) bits++;
+  }
+  const bool dev_topk = topk > 0 && topk <= 16;
 
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  for (int attempt = 0; attempt < 8; attempt++) {
+  for (;;) {
     JoinPlan plan;
     memset(&plan, 0, sizeof plan);
     plan.bits = bits;
     plan.P = 1 << bits;
     const double per = (double)est / plan.P;
-    int64_t cap = 8;
-    while (cap < (int64_t)std::ceil(per * 1.5 / 0.7) + 1) cap *= 2;
-    while (cap > 1024 || (size_t)(cap + 1) * slot_bytes > 64 * 1024) {
-      if (bits >= 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: build side needs more than 2^16 partitions");
-      bits++;
-      plan.bits = bits;
-      plan.P = 1 << bits;
-      cap = 8;
-      const double per2 = (double)est / plan.P;
-      while (cap < (int64_t)std::ceil(per2 * 1.5 / 0.7) + 1) cap *= 2;
-    }
-    plan.cap = (int32_t)cap;
-    plan.topk = topk;
+    int32_t cap = 8;
+    while (cap < cap_max && cap * 0.75 < per) cap *= 2;
+    plan.cap = cap;
+    plan.pool = std::min<int32_t>(pool_max, std::max<int32_t>(64, cap / 2));
+    plan.lds_max = lds_max;
+    plan.topk = dev_topk ? topk : 0;
     plan.nvals = nvals;
-    plan.b_rows_total = build->nrows;
-    plan.p_rows_total = probe->nrows;
     const int P = plan.P;
 
     std::vector<uint8_t> bkeep, pkeep;
     prune(build, build_pred, bkeep);
     prune(probe, probe_pred, pkeep);
+    bs.tile_rows = join_tile_rows(kept_rows(build, bkeep));
+    ps.tile_rows = join_tile_rows(kept_rows(probe, pkeep));
     int64_t brows = 0, bkept = 0, prows = 0, pkept = 0;
     if (int rc = make_spans(build, bkeep, bs.tile_rows, bs.span, &bs.nspans, &bs.ntiles, &brows, &bkept)) return rc;
     if (int rc = make_spans(probe, pkeep, ps.tile_rows, ps.span, &ps.nspans, &ps.ntiles, &prows, &pkept)) return rc;
 
     const int RW = 3 + naggs;
-    const int64_t full_cap = topk > 0 ? topk : std::max<int64_t>(1, std::min(brows, prows));
+    const int64_t rec_cap = std::max<int64_t>(1, std::min(brows, prows));
     JoinBufs B;
     memset(&B, 0, sizeof B);
     B.cap_b = std::max<int64_t>(brows, 1);
     B.cap_p = std::max<int64_t>(prows, 1);
     const size_t nbt = (size_t)std::max<int64_t>(bs.ntiles, 1) * P;
     const size_t npt = (size_t)std::max<int64_t>(ps.ntiles, 1) * P;
+    const size_t nbc = (size_t)((std::max<int64_t>(bs.ntiles, 1) + 31) / 32) * P;
+    const size_t npc = (size_t)((std::max<int64_t>(ps.ntiles, 1) + 31) / 32) * P;
     size_t bytes = 0;
-    bytes += B.cap_b * 8 + nbt * 8 + (2 * P + 2) * 4;
-    bytes += B.cap_p * 8 + B.cap_p * 4 + (size_t)std::max(nvals, 1) * B.cap_p * 8 + npt * 8 + (2 * P + 2) * 4;
-    bytes += (size_t)P * std::max(topk, 1) * RW * 8 + (size_t)full_cap * RW * 8 + 64;
-    bytes += 16 * 256;
+    bytes += B.cap_b * 8 + (nbt + nbc) * 4 + (size_t)(P + 1) * 4;
+    bytes += B.cap_p * 12 + (size_t)std::max(nvals, 1) * B.cap_p * 8 + (npt + npc) * 4 + (size_t)(P + 1) * 4;
+    bytes += (size_t)rec_cap * RW * 8 + (size_t)(128 + 1) * 16 * RW * 8 + 64 + 16 * 256;
     void* ar;
     if (int rc = arena_get(c, bytes, &ar)) return rc;
     Carve cv;
     char* base = (char*)ar;
     B.b_keys = cv.take<int64_t>(base, B.cap_b);
     B.b_counts = cv.take<uint32_t>(base, nbt);
-    B.b_offsets = cv.take<uint32_t>(base, nbt);
-    B.b_pstart = cv.take<uint32_t>(base, 2 * P + 2);
+    B.b_csum = cv.take<uint32_t>(base, nbc);
+    B.b_pstart = cv.take<uint32_t>(base, P + 1);
     B.p_keys = cv.take<int64_t>(base, B.cap_p);
     B.p_rows = cv.take<uint32_t>(base, B.cap_p);
     B.p_vals = cv.take<uint64_t>(base, (size_t)std::max(nvals, 1) * B.cap_p);
     B.p_counts = cv.take<uint32_t>(base, npt);
-    B.p_offsets = cv.take<uint32_t>(base, npt);
-    B.p_pstart = cv.take<uint32_t>(base, 2 * P + 2);
-    B.cand = cv.take<uint64_t>(base, (size_t)P * std::max(topk, 1) * RW);
-    B.out = cv.take<uint64_t>(base, (size_t)full_cap * RW);
+    B.p_csum = cv.take<uint32_t>(base, npc);
+    B.p_pstart = cv.take<uint32_t>(base, P + 1);
+    B.recs = cv.take<uint64_t>(base, (size_t)rec_cap * RW);
+    B.cand = cv.take<uint64_t>(base, (size_t)128 * 16 * RW);
+    B.out = cv.take<uint64_t>(base, (size_t)16 * RW);
     B.flags = cv.take<uint32_t>(base, 16);
 
     HIP_TRY(launch_join(bs, ps, jd, plan, B, c->stream, c->ev));
-    // read back flags, build total and the records
-    const size_t rec_bytes = (size_t)full_cap * RW * 8;
     void* hp;
-    if (int rc = pinned_get(c, rec_bytes + 64, &hp)) return rc;
+    if (int rc = pinned_get(c, 64, &hp)) return rc;
     uint32_t* hflags = (uint32_t*)hp;
-    uint64_t* hrec = (uint64_t*)((char*)hp + 64);
-    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(hflags + 4, B.b_pstart + P, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (hflags[0]) {  // a partition table overflowed: repartition finer
+    if (hflags[0]) {  // a partition table or pool overflowed: repartition finer
       if (bits >= 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: join partitions overflow at 2^16");
       bits++;
       continue;
     }
-    const uint32_t nrec = hflags[1];
-    if (nrec > (uint64_t)full_cap) return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)full_cap);
-    if (nrec) HIP_TRY(hipMemcpy(hrec, B.out, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
+    const uint32_t ngroups_all = hflags[1];
+    const uint32_t nrec = dev_topk ? hflags[3] : ngroups_all;
+    if (nrec > (uint64_t)rec_cap) return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)rec_cap);
+    std::vector<uint64_t> hrec((size_t)nrec * RW + 1);
+    if (nrec) HIP_TRY(hipMemcpy(hrec.data(), dev_topk ? B.out : B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
     if (stats) {
       memset(stats, 0, sizeof *stats);
       stats->device_ms = elapsed(c->ev[0], c->ev[3]);
-      stats->main_kernel_ms = elapsed(c->ev[0], c->ev[1]);
+      stats->main_kernel_ms = elapsed(c->ev[1], c->ev[2]);
       stats->rows_scanned = brows + prows;
       stats->rg_total = build->nrg + probe->nrg;
       stats->rg_scanned = bkept + pkept;
       stats->bytes_alg = brows * slot_bytes_per_row(build, bsm) + prows * slot_bytes_per_row(probe, psm);
       stats->build_rows = hflags[4];
-      stats->build_keys = -1;
+      stats->build_keys = hflags[2];
       stats->partitions = P;
       stats->radix_bits = bits;
+      stats->grid = P;
+      stats->tile_rows = ps.tile_rows;
     }
-    // order: rank order for top-k; first-occurrence order otherwise
+    // rank order for a device top-k; first-occurrence order otherwise (then
+    // the host top-k, for k above the device limit, keeps the stable order)
     std::vector<std::pair<uint64_t, uint32_t>> ord(nrec);
-    for (uint32_t i = 0; i < nrec; i++) ord[i] = {topk > 0 ? (uint64_t)i : hrec[(size_t)i * RW + 2], i};
-    if (topk <= 0) std::sort(ord.begin(), ord.end());
-    out->ngroups = nrec;
-    if ((int64_t)nrec > out->capacity)
-      return fail(VQ_E_ARG, "exec: gpu: %u groups exceed result capacity %lld", nrec, (long long)out->capacity);
-    for (uint32_t i = 0; i < nrec; i++) {
+    for (uint32_t i = 0; i < nrec; i++) ord[i] = {dev_topk ? (uint64_t)i : hrec[(size_t)i * RW + 2], i};
+    if (!dev_topk) std::sort(ord.begin(), ord.end());
+    if (topk > 0 && !dev_topk) {
+      auto key = [&](uint32_t i) { return naggs > 0 ? (int64_t)hrec[(size_t)i * RW + 3] : (int64_t)0; };
+      std::stable_sort(ord.begin(), ord.end(), [&](const std::pair<uint64_t, uint32_t>& x,
+                                                   const std::pair<uint64_t, uint32_t>& y) {
+        return key(x.second) > key(y.second);
+      });
+      if ((int64_t)ord.size() > topk) ord.resize(topk);
+    }
+    const int64_t nout = (int64_t)ord.size();
+    out->ngroups = nout;
+    if (nout > out->capacity)
+      return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)nout,
+                  (long long)out->capacity);
+    for (int64_t i = 0; i < nout; i++) {
       const uint64_t* r = &hrec[(size_t)ord[i].second * RW];
       out->keys[i] = (int64_t)r[0];
       out->counts[i] = (int64_t)r[1];
       out->first_row[i] = (int64_t)r[2];
       for (int j = 0; j < naggs; j++) {
-        const int64_t o = (int64_t)i * naggs + j;
+        const int64_t o = i * naggs + j;
         out->valid[o] = 1;
         out->ivals[o] = 0;
         out->fvals[o] = 0;
@@ -1179,5 +1219,5 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     }
     return VQ_OK;
   }
-  return fail(VQ_E_UNSUPPORTED, "exec: gpu: join partitioning did not converge");
+
 }

@@ -273,35 +273,46 @@ hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile
 struct JoinBufs {
   // partitioned build side
   int64_t* b_keys;
-  uint32_t* b_counts;   // [nwg_b * P]
-  uint32_t* b_offsets;  // [nwg_b * P]
+  uint32_t* b_counts;   // [ntiles_b][P] counts, then absolute positions
+  uint32_t* b_csum;     // [nchunks_b][P]
   uint32_t* b_pstart;   // [P + 1]
   // partitioned probe side
   int64_t* p_keys;
   uint32_t* p_rows;
-  uint64_t* p_vals;     // [naggs_with_arg][n]
+  uint64_t* p_vals;     // [nvals][cap_p]
   uint32_t* p_counts;
-  uint32_t* p_offsets;
+  uint32_t* p_csum;
   uint32_t* p_pstart;
-  // per-partition top-k candidates / full group output
-  uint64_t* cand;       // candidate records
-  uint32_t* ngroups;    // [P + 1] groups per partition (then offsets)
-  uint32_t* flags;      // [0] = table overflow
-  uint64_t* out;        // final records
-  int64_t cap_p;        // probe entries capacity
-  int64_t cap_b;        // build entries capacity
+  // group records and the top-k
+  uint64_t* recs;       // one record per matched group
+  uint64_t* cand;       // top-k candidates, kTopBlocks * topk records
+  uint64_t* out;        // top-k records
+  uint32_t* flags;      // [0] overflow, [1] records, [2] build keys, [3] top-k count
+  int64_t cap_p;        // probe entries capacity (value stride)
+  int64_t cap_b;
 };
 
 struct JoinPlan {
   int32_t bits;
   int32_t P;
   int32_t cap;          // table slots per partition (power of two)
-  int32_t slot_words;   // 64-bit words per table slot in LDS
+  int32_t pool;         // accumulator entries per partition (matched keys)
   int32_t topk;
   int32_t nvals;        // aggregates that carry a per-row value
-  int64_t p_rows_total;
-  int64_t b_rows_total;
+  int32_t lds_max;      // per-workgroup LDS the device allows (bytes)
+  int32_t pad_;
 };
+
+// LDS bytes of one partition's table and accumulator pool.
+size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords);
+
+// Run-time compiled partition kernels of one join side (vq_jit.hip).  With
+// count = true, writes counts[t][p]; otherwise scatters to the positions in
+// counts.  jd (probe side) supplies the value expressions.
+hipError_t jit_launch_part(const JSide& sd, const JDesc* jd, int32_t P, uint32_t* counts,
+                           int64_t* keys, uint32_t* rows, uint64_t* vals, int64_t vstride,
+                           hipStream_t s, bool count);
+bool jit_part_available(const JSide& sd, const JDesc* jd, std::string* err);
 
 hipError_t launch_join(const JSide& b, const JSide& p, const JDesc& jd, const JoinPlan& plan,
                        const JoinBufs& bufs, hipStream_t s, hipEvent_t* ev);

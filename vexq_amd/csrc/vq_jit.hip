@@ -52,8 +52,8 @@ struct Module {
   hipModule_t mod = nullptr;
   hipFunction_t dense = nullptr;
   hipFunction_t filter_bits = nullptr;
-  int dense_per_cu = 0;
-  size_t dense_lds = 0;
+  hipFunction_t part_count = nullptr;
+  hipFunction_t part_scatter = nullptr;
 };
 
 std::mutex g_mu;
@@ -104,7 +104,8 @@ Flat flatten_terms(const QDesc& d) {
 
 int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
 
-std::string prologue(const QDesc& d, const Flat& f) {
+std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slot = 0,
+                     int nv = 0, int probe = 0) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
@@ -128,6 +129,10 @@ std::string prologue(const QDesc& d, const Flat& f) {
   s += "#define VQ_G " + std::to_string(d.ngroups) + "\n";
   s += "#define VQ_NA " + std::to_string(d.nacc) + "\n";
   s += "#define VQ_NT " + std::to_string(f.nt) + "\n";
+  s += "#define VQ_KERNELS " + std::to_string(kernels) + "\n";
+  s += "#define VQ_KEYSLOT " + std::to_string(key_slot) + "\n";
+  s += "#define VQ_NV " + std::to_string(nv) + "\n";
+  s += "#define VQ_PROBE " + std::to_string(probe) + "\n";
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
@@ -177,9 +182,14 @@ Module* get_module(const std::string& pro, std::string* err) {
   hiprtcGetCode(prog, &code[0]);
   hiprtcDestroyProgram(&prog);
   Module* m = new Module();
-  if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess ||
-      hipModuleGetFunction(&m->dense, m->mod, "vq_dense") != hipSuccess ||
-      hipModuleGetFunction(&m->filter_bits, m->mod, "vq_filter_bits") != hipSuccess) {
+  bool ok = hipModuleLoadData(&m->mod, code.data()) == hipSuccess;
+  if (ok && pro.find("#define VQ_KERNELS 3") != std::string::npos)
+    ok = hipModuleGetFunction(&m->dense, m->mod, "vq_dense") == hipSuccess &&
+         hipModuleGetFunction(&m->filter_bits, m->mod, "vq_filter_bits") == hipSuccess;
+  if (ok && pro.find("#define VQ_KERNELS 4") != std::string::npos)
+    ok = hipModuleGetFunction(&m->part_count, m->mod, "vq_part_count") == hipSuccess &&
+         hipModuleGetFunction(&m->part_scatter, m->mod, "vq_part_scatter") == hipSuccess;
+  if (!ok) {
     delete m;
     if (err) *err = "hipModuleLoadData failed";
     return nullptr;
@@ -210,11 +220,36 @@ void fill_common(const QDesc& d, const Flat& f, JitArgs& a) {
   a.ntiles = d.ntiles;
 }
 
-hipError_t launch(hipFunction_t fn, unsigned grid, size_t lds, hipStream_t s, JitArgs& a) {
+hipError_t launch(hipFunction_t fn, unsigned grid, size_t lds, hipStream_t s, JitArgs& a,
+                  unsigned block = 64) {
   size_t sz = sizeof a;
   void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz,
                  HIP_LAUNCH_PARAM_END};
-  return hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, (unsigned)lds, s, nullptr, cfg);
+  return hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, s, nullptr, cfg);
+}
+
+// A join side as a QDesc: its slots and predicate, and (probe side) the
+// aggregate arguments as value expressions.
+QDesc side_desc(const JSide& sd, const JDesc* jd, int* nv) {
+  QDesc d;
+  memset(&d, 0, sizeof d);
+  d.nslots = sd.nslots;
+  for (int c = 0; c < sd.nslots; c++) d.slot[c] = sd.slot[c];
+  d.ngroups = 1;
+  *nv = 0;
+  if (jd)
+    for (int j = 0; j < jd->naggs; j++) {
+      if (jd->agg[j].e.nfactors == 0) continue;
+      d.acc[*nv].op = jd->agg[j].e.is_float ? ACC_SUM_F : ACC_SUM_I;
+      d.acc[*nv].e = jd->agg[j].e;
+      (*nv)++;
+    }
+  d.nacc = *nv;
+  d.nspans = sd.nspans;
+  for (int i = 0; i < sd.nspans; i++) d.span[i] = sd.span[i];
+  d.tile_rows = sd.tile_rows;
+  d.ntiles = sd.ntiles;
+  return d;
 }
 
 bool jit_disabled() {
@@ -223,6 +258,34 @@ bool jit_disabled() {
 }
 
 }  // namespace
+
+bool jit_part_available(const JSide& sd, const JDesc* jd, std::string* err) {
+  if (jit_disabled()) return fail_jit(err, "JIT disabled") == 0;
+  int nv = 0;
+  const QDesc d = side_desc(sd, jd, &nv);
+  const Flat f = flatten_terms(d);
+  return get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr), err) != nullptr;
+}
+
+hipError_t jit_launch_part(const JSide& sd, const JDesc* jd, int32_t P, uint32_t* counts,
+                           int64_t* keys, uint32_t* rows, uint64_t* vals, int64_t vstride,
+                           hipStream_t s, bool count) {
+  int nv = 0;
+  const QDesc d = side_desc(sd, jd, &nv);
+  const Flat f = flatten_terms(d);
+  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr), nullptr);
+  if (!m) return hipErrorInvalidDeviceFunction;
+  JitArgs a;
+  fill_common(d, f, a);
+  a.offsets = counts;
+  a.keys = keys;
+  a.rows = rows;
+  a.vals = vals;
+  a.vstride = vstride;
+  a.P = P;
+  return launch(count ? m->part_count : m->part_scatter, (unsigned)d.ntiles, (size_t)P * 4, s, a,
+                1024);
+}
 
 int jit_dense_grid(const QDesc& d, std::string* err) {
   if (jit_disabled()) return fail_jit(err, "JIT disabled");
