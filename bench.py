@@ -76,10 +76,13 @@ def time_cpu(fn, budget_s=8.0, max_runs=10):
     return runs
 
 
-def operator_record(name, kernel, r, total, walls, mains, devs, steps, traffic, cpu, rows):
+def operator_record(name, kernel, r, total, walls, mains, devs, steps, traffic, cpu, rows,
+                    whole_pipeline=False):
+    """whole_pipeline: the roofline is taken over every kernel of the query
+    (first launch to last, one HIP event pair), not over one kernel."""
     bytes_alg = int(r.stats.bytes_alg)
     ms_step = total / steps * 1e3
-    main_ms = statistics.mean(mains)
+    main_ms = statistics.mean(devs if whole_pipeline else mains)
     achieved = bytes_alg / (main_ms * 1e-3) / 1e9 if main_ms > 0 else 0.0
     tr = traffic.get(name, {}).get("bytes_per_launch")
     rec = {
@@ -99,12 +102,45 @@ def operator_record(name, kernel, r, total, walls, mains, devs, steps, traffic, 
     return rec
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_record(runs, bytes_alg, cores, sample, kind="port"):
     med = statistics.median(runs)
     return {"value": round(bytes_alg / med / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": kind,
             "sample": sample, "ms_median": round(med * 1e3, 3), "ms_min": round(min(runs) * 1e3, 3),
-            "ms_max": round(max(runs) * 1e3, 3), "runs": len(runs),
+            "ms_max": round(max(runs) * 1e3, 3), "runs": len(runs), "cpu": cpu_model(),
             "label": "C restatement of vexq's CPU operators"}
+
+
+def copy_probe(steps=20, mib=1024):
+    """A plain device-to-device copy of a known size: the measured HBM
+    ceiling beside the published 8 TB/s, and the reference dispatch for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"bytes_read": n, "bytes_written": n, "ms": round(ms, 4),
+            "gbs_read_plus_write": round(2 * n / (ms * 1e-3) / 1e9, 1)}
 
 
 def main():
@@ -114,7 +150,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="comma list of operators to run")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time a 1 GiB device copy (HBM ceiling, counter scale)")
     args = ap.parse_args()
+    if args.copy_probe:
+        print(json.dumps({"copy_probe": copy_probe()}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding of the path is described, not built)", file=sys.stderr)
@@ -208,8 +249,9 @@ def main():
             runs = time_cpu(lambda: cpu_mod.join_agg(b, 0, bp, p, 0, pp, jaggs, topk=10,
                                                      nthreads=cores))
             cpu = cpu_record(runs, r.stats.bytes_alg, cores, f"the full join query, {len(runs)} runs")
-        rec = operator_record("join_agg_top10", "partition+k_join_agg", r, total, walls, mains, devs,
-                              K, traffic, cpu, b.nrows + p.nrows)
+        rec = operator_record("join_agg_top10", "join pipeline (partition, build+probe, top-k)", r,
+                              total, walls, mains, devs, K, traffic, cpu, b.nrows + p.nrows,
+                              whole_pipeline=True)
         rec["radix_bits"] = int(r.stats.radix_bits)
         rec["build_rows_inserted"] = int(r.stats.build_rows)
         ops["join_agg_top10"] = rec

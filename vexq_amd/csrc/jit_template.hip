@@ -25,6 +25,8 @@
 #define VQ_NV 5
 #define VQ_PROBE 1
 #define VQ_KERNELS 7
+#define VQ_P 256
+#define VQ_PB 8
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
@@ -86,10 +88,11 @@ struct Args {
   i64 ntiles;
   u64* part;          // vq_dense: partials [grid][G][NA+2]
   u64* bits;          // vq_filter_bits: pass words
-  u32* tile_counts;   // vq_filter_bits / vq_part_*: per-tile counts
+  u32* tile_counts;   // vq_filter_bits: per-tile counts
   // join partitioning
-  u32* offsets;       // [P][ntiles] exclusive offsets within a partition
-  const u32* pstart;  // [P + 1] partition starts
+  u32* counts;        // [G][P] rows per workgroup and partition, then positions
+  const u32* pstart;  // [P + 1] first entry of each partition
+  u32* perr;          // set to 1 if a write would pass its partition's end
   i64* keys;
   u32* rows;
   u64* vals;          // [NV][vstride]
@@ -469,15 +472,30 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
 // ---------------------------------------------------------------------------
 // Radix partitioning of one join side (exec/parallel_join.go:322-347,422-550):
 // rows that pass the side's predicate go to partition radixHash(key) & (P-1).
-// A 1024-thread workgroup owns one tile of rows.
-//   vq_part_count:   offsets[t * P + p] = rows of tile t in partition p
-//   vq_part_scatter: each row to offsets[t * P + p] + rank, the offsets
-//                    having been turned into absolute positions,
-//                    as (key[, row, values of the aggregate arguments])
+// A persistent grid of G 1024-thread workgroups; workgroup w takes the
+// contiguous tile range [w*ntiles/G, (w+1)*ntiles/G).  Both kernels of a side
+// run with the same G, so they see the same rows per workgroup.
+//   vq_part_count:   counts[p*G + w] = workgroup w's rows of partition p
+//                    (an LDS histogram)
+//   vq_part_scatter: workgroup w's run of partition p starts at
+//                    pstart[p] + counts[p*G + w] (k_part_runs, k_part_starts);
+//                    each row goes there as (key[, row id, values of the
+//                    aggregate arguments]).
+// The scatter gathers rows in an LDS buffer of VQ_PB entries per partition
+// and writes a full buffer as VQ_PB consecutive entries (a 128-byte line of
+// keys at VQ_PB = 16), advancing the workgroup's own cursor: no atomics
+// outside LDS, and the partial lines are only those at the ends of a
+// workgroup's run.  Where a row lands inside its workgroup's run depends on
+// LDS atomics, so the order inside a partition can differ from run to run;
+// every consumer aggregates with order-independent integer updates.
+// VQ_PB = 0 (partition counts too large for LDS buffers) writes each row
+// straight to its position.
 // ---------------------------------------------------------------------------
 namespace vqj {
 
 constexpr int kPB = 1024;  // threads per partitioning workgroup
+constexpr int kP = VQ_P;
+constexpr int kBuf = VQ_PB;
 
 __device__ __forceinline__ u64 radix_hash(u64 x) {
   x ^= x >> 33;
@@ -506,19 +524,6 @@ __device__ __forceinline__ void pload_rows(const Args& a, i64 base, i64 rend,
   }
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin.  Workgroup w takes tile
-// (w % 8) * per + w / 8, so each XCD works a contiguous block of tiles: the
-// ends of one tile's per-partition runs and the starts of the next tile's are
-// then written through the same L2 and merge there into whole lines.
-__device__ __forceinline__ i64 xcd_tile(i64 w, i64 ntiles) {
-  constexpr int kXcd = 8;
-  const i64 per = (ntiles + kXcd - 1) / kXcd;
-  const i64 full = ntiles - (per - 1) * kXcd;  // XCDs that get `per` tiles
-  const i64 x = w % kXcd, k = w / kXcd;
-  const i64 start = x < full ? x * per : full * per + (x - full) * (per - 1);
-  return start + k;
-}
-
 __device__ __forceinline__ void ptile(const Args& a, i64 t, i64& row0, i64& rend) {
   int s = 0;
   while (s + 1 < a.nspans && t >= a.sp_tile0[s + 1]) s++;
@@ -535,65 +540,241 @@ __device__ __forceinline__ u64 value_of(const Args& a, const i64* row) {
 }
 
 template <int... J>
-__device__ __forceinline__ void write_values(const Args& a, const i64* row, u32 pos, Seq<J...>) {
-  ((a.vals[(i64)J * a.vstride + pos] = value_of<J>(a, row)), ...);
+__device__ __forceinline__ void values_of(const Args& a, const i64* row, u64* v, Seq<J...>) {
+  ((v[J] = value_of<J>(a, row)), ...);
+}
+
+constexpr int kNV1 = VQ_NV > 0 ? VQ_NV : 1;
+
+// The rows one thread has in flight: partition, key, row id, values.
+template <int U>
+struct PRows {
+  u32 part[U];
+  i64 key[U];
+  u32 row[U];
+  u64 val[U][kNV1];
+  bool live[U];
+};
+
+template <int U, bool kVals>
+__device__ __forceinline__ void prows_from(const Args& a, i64 base, i64 rend,
+                                           const i64 (&raw)[VQ_NS][U], PRows<U>& q) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    i64 row[VQ_NS];
+#pragma unroll
+    for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
+    const i64 r = base + u * kPB + threadIdx.x;
+    q.live[u] = r < rend && pass_all(a, row, seq<VQ_NT>{});
+    q.key[u] = row[VQ_KEYSLOT];
+    q.part[u] = (u32)(radix_hash((u64)q.key[u]) & (u64)(kP - 1));
+    q.row[u] = (u32)r;
+    if constexpr (kVals && VQ_PROBE && VQ_NV > 0) values_of(a, row, q.val[u], seq<VQ_NV>{});
+  }
+}
+
+template <int U, bool kVals>
+__device__ __forceinline__ void prows_eval(const Args& a, i64 base, i64 rend, PRows<U>& q) {
+  i64 raw[VQ_NS][U];
+  pload_rows<U>(a, base, rend, raw);
+  prows_from<U, kVals>(a, base, rend, raw, q);
+}
+
+// The chunks (kPB * U rows, clamped to their tile) of the tile range
+// [t, t1): next() steps to the following chunk, false past the last.
+template <int U>
+struct Chunks {
+  i64 t, t1, base, rend;
+  __device__ __forceinline__ bool next(const Args& a) {
+    base += kPB * U;
+    while (base >= rend) {
+      if (++t >= t1) return false;
+      ptile(a, t, base, rend);
+    }
+    return true;
+  }
+};
+
+template <int... J>
+__device__ __forceinline__ void put_vals(const Args& a, u32 pos, const u64* v, Seq<J...>) {
+  ((a.vals[(i64)J * a.vstride + pos] = v[J]), ...);
 }
 
 }  // namespace vqj
 
 extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
-  extern __shared__ u32 hist[];
   constexpr int U = VQ_U;
-  const i64 t = xcd_tile(blockIdx.x, a.ntiles);
-  for (int p = threadIdx.x; p < a.P; p += kPB) hist[p] = 0;
+  const i64 t0 = (i64)blockIdx.x * a.ntiles / gridDim.x;
+  const i64 t1 = (i64)(blockIdx.x + 1) * a.ntiles / gridDim.x;
+  const i64 G = gridDim.x;
+#if VQ_P <= 32768
+  __shared__ u32 hist[kP];
+  for (int p = threadIdx.x; p < kP; p += kPB) hist[p] = 0;
+#else
+  for (int p = threadIdx.x; p < kP; p += kPB) a.counts[p * G + blockIdx.x] = 0;
+#endif
   __syncthreads();
-  i64 row0, rend;
-  ptile(a, t, row0, rend);
-  const u64 mask = (u64)a.P - 1;
-  for (i64 base = row0; base < rend; base += kPB * U) {
-    i64 raw[VQ_NS][U];
-    pload_rows<U>(a, base, rend, raw);
+  for (i64 t = t0; t < t1; t++) {
+    i64 row0, rend;
+    ptile(a, t, row0, rend);
+    for (i64 base = row0; base < rend; base += kPB * U) {
+      PRows<U> q;
+      prows_eval<U, false>(a, base, rend, q);
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      i64 row[VQ_NS];
-#pragma unroll
-      for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
-      if (base + u * kPB + threadIdx.x < rend && pass_all(a, row, seq<VQ_NT>{}))
-        atomicAdd(&hist[radix_hash((u64)row[VQ_KEYSLOT]) & mask], 1u);
-    }
-  }
-  __syncthreads();
-  for (int p = threadIdx.x; p < a.P; p += kPB) a.offsets[t * a.P + p] = hist[p];
-}
-
-extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a) {
-  extern __shared__ u32 lcnt[];
-  constexpr int U = VQ_U;
-  const i64 t = xcd_tile(blockIdx.x, a.ntiles);
-  for (int p = threadIdx.x; p < a.P; p += kPB) lcnt[p] = a.offsets[t * a.P + p];
-  __syncthreads();
-  i64 row0, rend;
-  ptile(a, t, row0, rend);
-  const u64 mask = (u64)a.P - 1;
-  for (i64 base = row0; base < rend; base += kPB * U) {
-    i64 raw[VQ_NS][U];
-    pload_rows<U>(a, base, rend, raw);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      i64 row[VQ_NS];
-#pragma unroll
-      for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
-      const i64 r = base + u * kPB + threadIdx.x;
-      if (r < rend && pass_all(a, row, seq<VQ_NT>{})) {
-        const i64 key = row[VQ_KEYSLOT];
-        const u32 pos = atomicAdd(&lcnt[radix_hash((u64)key) & mask], 1u);
-        a.keys[pos] = key;
-        if constexpr (VQ_PROBE) {
-          a.rows[pos] = (u32)r;
-          if constexpr (VQ_NV > 0) write_values(a, row, pos, seq<VQ_NV>{});
-        }
+      for (int u = 0; u < U; u++) {
+        if (!q.live[u]) continue;
+#if VQ_P <= 32768
+        atomicAdd(&hist[q.part[u]], 1u);
+#else
+        atomicAdd(&a.counts[q.part[u] * G + blockIdx.x], 1u);  // too many partitions for LDS
+#endif
       }
     }
   }
+#if VQ_P <= 32768
+  __syncthreads();
+  for (int p = threadIdx.x; p < kP; p += kPB) a.counts[p * G + blockIdx.x] = hist[p];
+#endif
+}
+
+namespace vqj {
+
+#if VQ_PB > 0
+constexpr int kGrp = 64 / kBuf;  // partitions one wave flushes per step
+
+struct PBuf {
+  u32 fill[kP];
+  u32 cur[kP];  // next position of the workgroup's run of each partition
+  i64 key[kP * kBuf];
+#if VQ_PROBE
+  u32 row[kP * kBuf];
+#if VQ_NV > 0
+  u64 val[VQ_NV][kP * kBuf];
+#endif
+#endif
+};
+
+// Writes the buffers of the partitions that are full (kTail = false) or hold
+// anything (kTail = true) at the workgroup's cursors.  Lanes
+// [kBuf*g, kBuf*(g+1)) of a wave carry partition p0 + g; each wave takes every
+// 16th group of partitions.
+template <bool kTail>
+__device__ __forceinline__ void flush(const Args& a, PBuf& b) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int e = lane % kBuf;
+  for (int p0 = wv * kGrp; p0 < kP; p0 += (kPB / 64) * kGrp) {
+    const int p = p0 + lane / kBuf;
+    const u32 f = p < kP ? b.fill[p] : 0;
+    const u32 n = kTail ? (f < (u32)kBuf ? f : (u32)kBuf) : (f >= (u32)kBuf ? (u32)kBuf : 0);
+    if (n == 0) continue;
+    const u32 g = b.cur[p];
+    const bool ok = g + n <= a.pstart[p + 1];  // never past the partition's end
+    if ((u32)e < n && ok) {
+      const u32 pos = g + e;
+      const int i = p * kBuf + e;
+      a.keys[pos] = b.key[i];
+#if VQ_PROBE
+      a.rows[pos] = b.row[i];
+#if VQ_NV > 0
+      u64 v[VQ_NV];
+#pragma unroll
+      for (int j = 0; j < VQ_NV; j++) v[j] = b.val[j][i];
+      put_vals(a, pos, v, seq<VQ_NV>{});
+#endif
+#endif
+    }
+    if (e == 0) {
+      if (!ok) atomicOr(a.perr, 1u);
+      b.cur[p] = g + n;
+      b.fill[p] = 0;
+    }
+  }
+}
+#endif
+
+}  // namespace vqj
+
+extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a) {
+  constexpr int U = VQ_U;
+  const i64 t0 = (i64)blockIdx.x * a.ntiles / gridDim.x;
+  const i64 t1 = (i64)(blockIdx.x + 1) * a.ntiles / gridDim.x;
+  const i64 G = gridDim.x;
+#if VQ_PB > 0
+  __shared__ PBuf b;
+  for (int p = threadIdx.x; p < kP; p += kPB) {
+    b.fill[p] = 0;
+    b.cur[p] = a.pstart[p] + a.counts[p * G + blockIdx.x];
+  }
+  __syncthreads();
+#else
+  // this workgroup's own cursors, advanced in place
+  for (int p = threadIdx.x; p < kP; p += kPB) a.counts[p * G + blockIdx.x] += a.pstart[p];
+  __syncthreads();
+#endif
+  // one chunk's column loads in flight while the previous chunk is placed
+  Chunks<U> ck{t0 - 1, t1, 0, 0};
+  ck.base = -(i64)kPB * U;  // next() steps onto tile t0
+  bool have = t0 < t1 && ck.next(a);
+  i64 raw[VQ_NS][U];
+  if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
+  while (have) {
+    const i64 base = ck.base, rend = ck.rend;
+    PRows<U> q;
+    prows_from<U, true>(a, base, rend, raw, q);
+    have = ck.next(a);
+    if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
+    {
+#if VQ_PB > 0
+      // insert what fits; flush full buffers; again until every row is in
+      for (;;) {
+        bool pend = false;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          if (!q.live[u]) continue;
+          const u32 s = atomicAdd(&b.fill[q.part[u]], 1u);
+          if (s < (u32)kBuf) {
+            const int i = q.part[u] * kBuf + s;
+            b.key[i] = q.key[u];
+#if VQ_PROBE
+            b.row[i] = q.row[u];
+#if VQ_NV > 0
+#pragma unroll
+            for (int j = 0; j < VQ_NV; j++) b.val[j][i] = q.val[u][j];
+#endif
+#endif
+            q.live[u] = false;
+          } else {
+            pend = true;
+          }
+        }
+        __syncthreads();
+        flush<false>(a, b);
+        if (!__syncthreads_or(pend)) break;
+      }
+#else
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (!q.live[u]) continue;
+        const u32 p = q.part[u];
+        const u32 pos = atomicAdd(&a.counts[p * G + blockIdx.x], 1u);
+        if (pos >= a.pstart[p + 1]) {
+          atomicOr(a.perr, 1u);
+          continue;
+        }
+        a.keys[pos] = q.key[u];
+#if VQ_PROBE
+        a.rows[pos] = q.row[u];
+#if VQ_NV > 0
+        put_vals(a, pos, q.val[u], seq<VQ_NV>{});
+#endif
+#endif
+      }
+#endif
+    }
+  }
+#if VQ_PB > 0
+  __syncthreads();
+  flush<true>(a, b);
+#endif
 }
 #endif

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "vq_internal.h"
@@ -67,6 +68,9 @@ struct vq_ctx {
   // cached selection-vector buffer
   void* sv_cache = nullptr;
   size_t sv_cache_bytes = 0;
+  // build rows that passed, by (build table, key column, predicate): the
+  // partition count of the next join over the same build side
+  std::unordered_map<std::string, int64_t> build_est;
 };
 
 struct ColDev {
@@ -518,6 +522,13 @@ extern "C" int vq_table_destroy(vq_table* t) {
   hipStreamSynchronize(t->ctx->stream);
   for (auto& x : t->cols) hipFree(x.d);
   release(t->ctx, t->bytes);
+  {
+    std::lock_guard<std::mutex> lk(t->ctx->mu);
+    char pre[32];
+    snprintf(pre, sizeof pre, "%p/", (const void*)t);
+    for (auto it = t->ctx->build_est.begin(); it != t->ctx->build_est.end();)
+      it = it->first.compare(0, strlen(pre), pre) == 0 ? t->ctx->build_est.erase(it) : std::next(it);
+  }
   delete t;
   return VQ_OK;
 }
@@ -580,9 +591,12 @@ int32_t scan_tile_rows(int64_t rows) {
 // Tile rows of a partitioning pass (1024-thread workgroups, 4 rows per thread
 // per step): ~384 tiles at full size, which keeps a tile's rows of one
 // partition in long contiguous runs; never fewer than 4096 rows.
+// Tiles of a join side: at least 4096 rows (one pass of a partitioning
+// workgroup), and about 2048 of them, so a persistent grid of one or two
+// workgroups per CU takes near-equal shares.
 int32_t join_tile_rows(int64_t rows) {
   int64_t t = 4096;
-  while (t < (1 << 20) && t * 384 < rows) t *= 2;
+  while (t < (1 << 20) && t * 2048 < rows) t *= 2;
   return (int32_t)t;
 }
 
@@ -991,6 +1005,22 @@ int ilog2_ceil(uint64_t x) {
   return l;
 }
 
+// Identity of a join's build side for the partition-count estimate: the
+// table, the key column and the predicate's terms, literals included.
+std::string build_est_key(const vq_table* t, int32_t key, const vq_pred* pred) {
+  char buf[96];
+  snprintf(buf, sizeof buf, "%p/%d", (const void*)t, key);
+  std::string k = buf;
+  if (pred)
+    for (int i = 0; i < pred->nterms && i < VQ_MAX_TERMS; i++) {
+      const vq_term& m = pred->terms[i];
+      snprintf(buf, sizeof buf, "|%d,%d,%lld,%llx", m.col, m.op, (long long)m.ival,
+               (unsigned long long)f64_to_bits(m.fval));
+      k += buf;
+    }
+  return k;
+}
+
 int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg* aggs,
               int32_t naggs, JSide& sd, JDesc* jd, SlotMap& sm) {
   memset(&sd, 0, sizeof sd);
@@ -1064,34 +1094,50 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
   jd.nwords = nw;
   std::string jerr_;
 
-  if (!jit_part_available(bs, nullptr, &jerr_) || !jit_part_available(ps, &jd, &jerr_))
-    return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr_.c_str());
-
   // Partitioning plan.  Each partition's table lives in one workgroup's LDS:
   // cap slots of key, multiplicity and pool index, plus a pool of accumulator
   // entries (count, first row, accumulator words) for the keys a probe row
-  // matches.  The build-side estimate is
-  // the scan's row count before filtering, as buildSideRadixOptions uses
-  // (planner/parallel_join.go:327-339); partitions are sized so that count
-  // fills at most 3/4 of a table.  A table that still fills up reports an
-  // overflow and the query is repartitioned one bit finer.
-  const int64_t est = std::max<int64_t>(build->nrows, 1);
+  // matches.  The build-side estimate is the scan's row count before
+  // filtering, as buildSideRadixOptions uses (planner/parallel_join.go:
+  // 327-339), until a query with the same build table, key and predicate has
+  // run: then the rows that passed last time.  Partitions are sized so the
+  // estimate fills at most 3/4 of a table.  A table that still fills up
+  // reports an overflow and the query is repartitioned one bit finer.
+  const std::string est_key = build_est_key(build, build_key, build_pred);
+  int64_t est = std::max<int64_t>(build->nrows, 1);
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->build_est.find(est_key);
+    if (it != c->build_est.end()) est = std::max<int64_t>(it->second, 1);
+  }
   // the largest table (<= 4096 slots) whose pool-of-half-as-many entries fits
   // the device's per-workgroup LDS
-  int lds_max = 0;
-  hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-  if (lds_max <= 0) lds_max = 64 * 1024;
-  lds_max -= 1024;  // the kernel's own static LDS
+  int lds_dev = 0;
+  hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+  if (lds_dev <= 0) lds_dev = 64 * 1024;
+  const int lds_max = lds_dev - 8192;  // the join kernel's own static LDS (about 4 KB)
+  // Table size: at most 4096 slots, filled to at most kLoad so that linear
+  // probing stays short for every lane of a wave (a wave waits for its
+  // longest probe); the pool of accumulators starts at half the table, or at
+  // twice the matched keys per partition of the last query over the same
+  // build and probe sides, and doubles when it fills.
+  constexpr double kLoad = 0.4;
   int32_t cap_max = 4096;
-  while (cap_max > 8 && join_lds_for(cap_max, std::max(64, cap_max / 2), nw) > (size_t)lds_max)
-    cap_max /= 2;
-  const int32_t pool_max = cap_max / 2;
+  while (cap_max > 8 && join_lds_for(cap_max, 64, nw) > (size_t)lds_max) cap_max /= 2;
   int bits = radix_bits;
   if (bits < 0) {
     bits = 0;
-    while (bits < 16 && (double)est / (1 << bits) > 0.75 * cap_max) bits++;
+    while (bits < 16 && (double)est / (1 << bits) > kLoad * cap_max) bits++;
+  }
+  const std::string grp_key = est_key + "#" + build_est_key(probe, probe_key, probe_pred);
+  int64_t groups_seen = -1;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->build_est.find(grp_key);
+    if (it != c->build_est.end()) groups_seen = it->second;
   }
   const bool dev_topk = topk > 0 && topk <= 16;
+  int32_t pool_req = 0;  // 0: from the estimate
 
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
@@ -1102,13 +1148,22 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     plan.P = 1 << bits;
     const double per = (double)est / plan.P;
     int32_t cap = 8;
-    while (cap < cap_max && cap * 0.75 < per) cap *= 2;
+    while (cap < cap_max && cap * kLoad < per) cap *= 2;
     plan.cap = cap;
-    plan.pool = std::min<int32_t>(pool_max, std::max<int32_t>(64, cap / 2));
+    // the largest pool the LDS and the kernel's top-k (2048 entries) allow
+    int32_t pool_max = std::min<int32_t>(2048, cap);
+    while (pool_max > 64 && join_lds_for(cap, pool_max, nw) > (size_t)lds_max) pool_max /= 2;
+    int32_t pool = groups_seen >= 0 ? (int32_t)std::min<int64_t>(2048, 2 * groups_seen / plan.P + 64)
+                                    : cap / 2;
+    if (pool_req) pool = pool_req;
+    plan.pool = std::max<int32_t>(64, std::min(pool, pool_max));
+    plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, nw);
     plan.lds_max = lds_max;
     plan.topk = dev_topk ? topk : 0;
     plan.nvals = nvals;
     const int P = plan.P;
+    plan.bbuf = part_buffer_entries(P, 8, lds_dev);
+    plan.pbuf = part_buffer_entries(P, 8 + 4 + 8 * nvals, lds_dev);
 
     std::vector<uint8_t> bkeep, pkeep;
     prune(build, build_pred, bkeep);
@@ -1123,47 +1178,70 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     const int64_t rec_cap = std::max<int64_t>(1, std::min(brows, prows));
     JoinBufs B;
     memset(&B, 0, sizeof B);
-    B.cap_b = std::max<int64_t>(brows, 1);
-    B.cap_p = std::max<int64_t>(prows, 1);
-    const size_t nbt = (size_t)std::max<int64_t>(bs.ntiles, 1) * P;
-    const size_t npt = (size_t)std::max<int64_t>(ps.ntiles, 1) * P;
-    const size_t nbc = (size_t)((std::max<int64_t>(bs.ntiles, 1) + 31) / 32) * P;
-    const size_t npc = (size_t)((std::max<int64_t>(ps.ntiles, 1) + 31) / 32) * P;
+    const int64_t cap_b = std::max<int64_t>(brows, 1);
+    const int64_t cap_p = std::max<int64_t>(prows, 1);
+    if (cap_b >= (int64_t)INT32_MAX || cap_p >= (int64_t)UINT32_MAX)
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: join side above 2^31 build / 2^32 probe rows");
+    PartLaunch bl, pl;
+    if (jit_part_prepare(bs, nullptr, P, plan.bbuf, &bl, &jerr_) ||
+        jit_part_prepare(ps, &jd, P, plan.pbuf, &pl, &jerr_))
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr_.c_str());
+    B.b.G = bl.grid;
+    B.p.G = pl.grid;
     size_t bytes = 0;
-    bytes += B.cap_b * 8 + (nbt + nbc) * 4 + (size_t)(P + 1) * 4;
-    bytes += B.cap_p * 12 + (size_t)std::max(nvals, 1) * B.cap_p * 8 + (npt + npc) * 4 + (size_t)(P + 1) * 4;
-    bytes += (size_t)rec_cap * RW * 8 + (size_t)(128 + 1) * 16 * RW * 8 + 64 + 16 * 256;
+    bytes += (size_t)cap_b * 8 + (size_t)(P + 1) * 4 + (size_t)B.b.G * P * 4;
+    bytes += (size_t)cap_p * 12 + (size_t)std::max(nvals, 1) * cap_p * 8 + (size_t)(P + 1) * 4 +
+             (size_t)B.p.G * P * 4;
+    bytes += 16 * 4 + (size_t)2 * P * 4;
+    bytes += dev_topk ? (size_t)P * topk * (RW * 8 + 16) + (size_t)16 * RW * 8 : (size_t)rec_cap * RW * 8;
+    bytes += 16 * 256;  // alignment of the carved pieces
     void* ar;
     if (int rc = arena_get(c, bytes, &ar)) return rc;
     Carve cv;
     char* base = (char*)ar;
-    B.b_keys = cv.take<int64_t>(base, B.cap_b);
-    B.b_counts = cv.take<uint32_t>(base, nbt);
-    B.b_csum = cv.take<uint32_t>(base, nbc);
-    B.b_pstart = cv.take<uint32_t>(base, P + 1);
-    B.p_keys = cv.take<int64_t>(base, B.cap_p);
-    B.p_rows = cv.take<uint32_t>(base, B.cap_p);
-    B.p_vals = cv.take<uint64_t>(base, (size_t)std::max(nvals, 1) * B.cap_p);
-    B.p_counts = cv.take<uint32_t>(base, npt);
-    B.p_csum = cv.take<uint32_t>(base, npc);
-    B.p_pstart = cv.take<uint32_t>(base, P + 1);
-    B.recs = cv.take<uint64_t>(base, (size_t)rec_cap * RW);
-    B.cand = cv.take<uint64_t>(base, (size_t)128 * 16 * RW);
-    B.out = cv.take<uint64_t>(base, (size_t)16 * RW);
     B.flags = cv.take<uint32_t>(base, 16);
+    B.b.perr = B.p.perr = B.flags + 5;
+    B.b.keys = cv.take<int64_t>(base, cap_b);
+    B.b.vstride = cap_b;
+    B.b.pstart = cv.take<uint32_t>(base, P + 1);
+    B.b.counts = cv.take<uint32_t>(base, (size_t)B.b.G * P);
+    B.b.tot = cv.take<uint32_t>(base, P);
+    B.p.keys = cv.take<int64_t>(base, cap_p);
+    B.p.rows = cv.take<uint32_t>(base, cap_p);
+    B.p.vals = cv.take<uint64_t>(base, (size_t)std::max(nvals, 1) * cap_p);
+    B.p.vstride = cap_p;
+    B.p.pstart = cv.take<uint32_t>(base, P + 1);
+    B.p.counts = cv.take<uint32_t>(base, (size_t)B.p.G * P);
+    B.p.tot = cv.take<uint32_t>(base, P);
+    if (dev_topk) {
+      B.cand = cv.take<uint64_t>(base, (size_t)P * topk * RW);
+      B.citems = (KItem*)cv.take<uint64_t>(base, (size_t)P * topk * 2);
+      B.out = cv.take<uint64_t>(base, (size_t)16 * RW);
+    } else {
+      B.recs = cv.take<uint64_t>(base, (size_t)rec_cap * RW);
+    }
 
-    HIP_TRY(launch_join(bs, ps, jd, plan, B, c->stream, c->ev));
+    HIP_TRY(launch_join(bl, pl, jd, plan, B, c->stream, c->ev));
     void* hp;
     if (int rc = pinned_get(c, 64, &hp)) return rc;
     uint32_t* hflags = (uint32_t*)hp;
-    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 16, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hflags + 4, B.b_pstart + P, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 32, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (hflags[0]) {  // a partition table or pool overflowed: repartition finer
+    if (hflags[5]) return fail(VQ_E_DEVICE, "exec: gpu: partition scatter wrote past a partition's end");
+    if (hflags[0]) {
+      // a partition's accumulator pool filled: a larger pool if the LDS has
+      // room, else (and when a table filled) one bit more of partitioning
+      if (!(hflags[0] & 1) && plan.pool < pool_max) {
+        pool_req = std::min(pool_max, plan.pool * 2);
+        continue;
+      }
       if (bits >= 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: join partitions overflow at 2^16");
       bits++;
+      pool_req = 0;
       continue;
     }
+    c->build_est[est_key] = hflags[6];
+    c->build_est[grp_key] = hflags[1];
     const uint32_t ngroups_all = hflags[1];
     const uint32_t nrec = dev_topk ? hflags[3] : ngroups_all;
     if (nrec > (uint64_t)rec_cap) return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)rec_cap);
@@ -1177,7 +1255,7 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
       stats->rg_total = build->nrg + probe->nrg;
       stats->rg_scanned = bkept + pkept;
       stats->bytes_alg = brows * slot_bytes_per_row(build, bsm) + prows * slot_bytes_per_row(probe, psm);
-      stats->build_rows = hflags[4];
+      stats->build_rows = hflags[6];
       stats->build_keys = hflags[2];
       stats->partitions = P;
       stats->radix_bits = bits;

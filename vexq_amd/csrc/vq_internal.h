@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/vexq_gpu.h"
+#include "fixpoint.h"
 
 namespace vq {
 
@@ -94,11 +95,6 @@ static_assert(sizeof(QDesc) < 4096, "QDesc must fit the kernarg segment");
 // int64 limb therefore has 63 - w bits of headroom, enough for 2^(62-w)
 // additions.  Integer addition is associative, so any order of atomics gives
 // the same bits.
-struct FixScale {
-  int32_t lsb;  // exponent of the least significant bit
-  int32_t w;    // chunk width in bits
-};
-
 // Join + aggregate descriptors.
 struct JAgg {
   int32_t kind;     // VQ_AGG_*
@@ -153,102 +149,6 @@ __host__ __device__ inline int64_t f64_to_bits(double d) {
   return u.i;
 }
 
-// Round a signed multi-limb fixed-point value  sum_i limb[i] * 2^(lsb + i w)
-// times the non-negative integer mult to the nearest float64 (ties to even).
-// Exact: the value is first assembled as a 256-bit two's complement integer.
-__host__ __device__ inline double fix_to_f64(const int64_t* limb, int nlimbs, FixScale fs,
-                                             uint64_t mult) {
-  // acc = sum limb[i] << (i w), in 4 x 64-bit words, two's complement
-  uint64_t a[4] = {0, 0, 0, 0};
-  for (int i = 0; i < nlimbs; i++) {
-    int64_t v = limb[i];
-    int sh = i * fs.w;
-    uint64_t ext = v < 0 ? ~0ULL : 0ULL;
-    // x = sign-extended v shifted left by sh into 256 bits
-    uint64_t x[4] = {(uint64_t)v, ext, ext, ext};
-    int ws = sh >> 6, bs = sh & 63;
-    uint64_t y[4] = {0, 0, 0, 0};
-    for (int k = 3; k >= 0; k--) {
-      int src = k - ws;
-      uint64_t lo = src >= 0 ? x[src] : 0;
-      uint64_t lo1 = src - 1 >= 0 ? x[src - 1] : 0;
-      y[k] = bs ? (lo << bs) | (lo1 >> (64 - bs)) : lo;
-    }
-    uint64_t c = 0;
-    for (int k = 0; k < 4; k++) {
-      uint64_t s = a[k] + y[k];
-      uint64_t c1 = s < a[k];
-      uint64_t s2 = s + c;
-      uint64_t c2 = s2 < s;
-      a[k] = s2;
-      c = c1 | c2;
-    }
-  }
-  bool neg = (int64_t)a[3] < 0;
-  if (neg) {  // negate
-    uint64_t c = 1;
-    for (int k = 0; k < 4; k++) {
-      uint64_t s = ~a[k] + c;
-      c = (c && s == 0) ? 1 : 0;
-      a[k] = s;
-    }
-  }
-  // multiply magnitude by mult (fits: |value| < 2^(3w+62) and mult < 2^32)
-  if (mult != 1) {
-    uint64_t m_lo = mult & 0xFFFFFFFFULL, m_hi = mult >> 32;
-    uint64_t r[4] = {0, 0, 0, 0};
-    // schoolbook over 32-bit halves of a
-    uint32_t h[8];
-    for (int k = 0; k < 4; k++) {
-      h[2 * k] = (uint32_t)a[k];
-      h[2 * k + 1] = (uint32_t)(a[k] >> 32);
-    }
-    uint32_t out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t m[2] = {(uint32_t)m_lo, (uint32_t)m_hi};
-    for (int j = 0; j < 2; j++) {
-      uint64_t carry = 0;
-      for (int k = 0; k + j < 8; k++) {
-        uint64_t t = (uint64_t)h[k] * m[j] + out[k + j] + carry;
-        out[k + j] = (uint32_t)t;
-        carry = t >> 32;
-      }
-    }
-    for (int k = 0; k < 4; k++) r[k] = (uint64_t)out[2 * k] | ((uint64_t)out[2 * k + 1] << 32);
-    for (int k = 0; k < 4; k++) a[k] = r[k];
-  }
-  int top = -1;
-  for (int k = 3; k >= 0; k--)
-    if (a[k]) {
-      top = k * 64 + 63 - __builtin_clzll(a[k]);
-      break;
-    }
-  if (top < 0) return 0.0;
-  auto bit = [&](int p) -> uint64_t { return p < 0 ? 0 : (a[p >> 6] >> (p & 63)) & 1ULL; };
-  double r;
-  if (top < 53) {
-    r = (double)a[0];
-    r = ldexp(r, fs.lsb);
-  } else {
-    uint64_t mant = 0;
-    for (int p = top; p >= top - 52; p--) mant = (mant << 1) | bit(p);
-    uint64_t rb = bit(top - 53);
-    uint64_t sticky = 0;
-    int lim = top - 54;
-    for (int k = 0; k < 4 && !sticky; k++) {
-      int lo = k * 64, hi = lo + 63;
-      if (lo > lim) break;
-      if (hi <= lim) sticky = a[k] != 0;
-      else {
-        uint64_t mask = (lim - lo + 1) >= 64 ? ~0ULL : ((1ULL << (lim - lo + 1)) - 1);
-        sticky = (a[k] & mask) != 0;
-      }
-    }
-    if (rb && (sticky || (mant & 1))) mant++;
-    r = ldexp((double)mant, top - 52 + fs.lsb);
-  }
-  return neg ? -r : r;
-}
-
 }  // namespace vq
 
 // Launchers (defined in the kernel translation units).
@@ -270,26 +170,35 @@ bool jit_available_filter(const QDesc& d, std::string* err);
 hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,
                                   hipStream_t s);
 
+// One partitioned join side.  Partition p's entries are
+// [pstart[p], pstart[p+1]).  A persistent grid of G workgroups partitions the
+// side; counts[p][w] holds workgroup w's rows of partition p after the count
+// pass, and after k_part_runs the rows of partition p in workgroups before w
+// (workgroup w's run starts at pstart[p] + counts[p][w]).
+struct PartBufs {
+  int64_t* keys;
+  uint32_t* rows;       // probe side: row id of each entry
+  uint64_t* vals;       // probe side: [nvals][vstride] aggregate argument values
+  int64_t vstride;      // entries capacity
+  uint32_t* counts;     // [P][G]
+  uint32_t* tot;        // [P] rows per partition
+  uint32_t* pstart;     // [P + 1]
+  uint32_t* perr;       // a write past its partition's end (must stay 0)
+  int32_t G;
+  int32_t pad_;
+};
+
 struct JoinBufs {
-  // partitioned build side
-  int64_t* b_keys;
-  uint32_t* b_counts;   // [ntiles_b][P] counts, then absolute positions
-  uint32_t* b_csum;     // [nchunks_b][P]
-  uint32_t* b_pstart;   // [P + 1]
-  // partitioned probe side
-  int64_t* p_keys;
-  uint32_t* p_rows;
-  uint64_t* p_vals;     // [nvals][cap_p]
-  uint32_t* p_counts;
-  uint32_t* p_csum;
-  uint32_t* p_pstart;
-  // group records and the top-k
-  uint64_t* recs;       // one record per matched group
-  uint64_t* cand;       // top-k candidates, kTopBlocks * topk records
+  PartBufs b, p;
+  uint64_t* recs;       // one record per matched group (full-output mode)
+  uint64_t* cand;       // [P][topk] per-partition top-k candidates (top-k mode)
+  struct KItem* citems; // [P][topk] their sort keys (value, first row, index)
   uint64_t* out;        // top-k records
-  uint32_t* flags;      // [0] overflow, [1] records, [2] build keys, [3] top-k count
-  int64_t cap_p;        // probe entries capacity (value stride)
-  int64_t cap_b;
+  // flags[0] overflow, [1] groups, [2] distinct build keys, [3] top-k count,
+  // [4] partitions finished, [5] partition write error, [6] build entries,
+  // [7] probe entries, [8..9] top-k lower bound; 16 words, zeroed by the
+  // pipeline's first launch
+  uint32_t* flags;
 };
 
 struct JoinPlan {
@@ -300,20 +209,62 @@ struct JoinPlan {
   int32_t topk;
   int32_t nvals;        // aggregates that carry a per-row value
   int32_t lds_max;      // per-workgroup LDS the device allows (bytes)
-  int32_t pad_;
+  int32_t bbuf;         // build scatter: LDS buffer entries per partition (0: none)
+  int32_t pbuf;         // probe scatter: the same
+  int32_t lds_bytes;    // the join kernel's dynamic LDS: join_lds_for(cap, pool, nwords)
 };
 
 // LDS bytes of one partition's table and accumulator pool.
 size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords);
 
-// Run-time compiled partition kernels of one join side (vq_jit.hip).  With
-// count = true, writes counts[t][p]; otherwise scatters to the positions in
-// counts.  jd (probe side) supplies the value expressions.
-hipError_t jit_launch_part(const JSide& sd, const JDesc* jd, int32_t P, uint32_t* counts,
-                           int64_t* keys, uint32_t* rows, uint64_t* vals, int64_t vstride,
-                           hipStream_t s, bool count);
-bool jit_part_available(const JSide& sd, const JDesc* jd, std::string* err);
+// Arguments of every run-time compiled kernel; must match vqj::Args in
+// jit_template.hip field for field.
+struct JitArgs {
+  const void* col[8];
+  int64_t ti[8];
+  double tf[8];
+  int64_t fi[8][4];
+  double ff[8][4];
+  int64_t sp_start[kMaxSpans];
+  int64_t sp_rows[kMaxSpans];
+  int64_t sp_tile0[kMaxSpans];
+  int32_t nspans;
+  int32_t tile_rows;
+  int64_t ntiles;
+  uint64_t* part;
+  uint64_t* bits;
+  uint32_t* tile_counts;
+  uint32_t* counts;
+  const uint32_t* pstart;
+  uint32_t* perr;
+  int64_t* keys;
+  uint32_t* rows;
+  uint64_t* vals;
+  int64_t vstride;
+  int32_t P;
+  int32_t pad_;
+};
 
-hipError_t launch_join(const JSide& b, const JSide& p, const JDesc& jd, const JoinPlan& plan,
-                       const JoinBufs& bufs, hipStream_t s, hipEvent_t* ev);
+// A join side's partition kernels, resolved once per query (vq_jit.hip):
+// the compiled count and scatter kernels of the side's shape, their
+// persistent grid (one workgroup per scatter slot the device holds at once,
+// at most one per tile) and the arguments that do not depend on buffers.
+struct PartLaunch {
+  hipFunction_t count = nullptr;
+  hipFunction_t scatter = nullptr;
+  int32_t grid = 0;
+  JitArgs args;
+};
+int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
+                     std::string* err);
+// count = true: adds the side's rows per partition and workgroup into counts;
+// otherwise scatters them into their runs.
+hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, hipStream_t s,
+                           bool count);
+// LDS buffer entries per partition for a scatter with entries of `entry`
+// bytes: the largest of 16, 8, 4, 2, 1 that fits, else 0.
+int32_t part_buffer_entries(int32_t P, int32_t entry, int32_t lds_max);
+
+hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd,
+                       const JoinPlan& plan, const JoinBufs& bufs, hipStream_t s, hipEvent_t* ev);
 }  // namespace vq

@@ -8,6 +8,7 @@
 // caller, which then runs the generic kernels of k_scan.hip.
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,32 +22,6 @@
 namespace vq {
 
 namespace {
-
-// Must match vqj::Args in jit_template.hip field for field.
-struct JitArgs {
-  const void* col[8];
-  int64_t ti[8];
-  double tf[8];
-  int64_t fi[8][4];
-  double ff[8][4];
-  int64_t sp_start[kMaxSpans];
-  int64_t sp_rows[kMaxSpans];
-  int64_t sp_tile0[kMaxSpans];
-  int32_t nspans;
-  int32_t tile_rows;
-  int64_t ntiles;
-  uint64_t* part;
-  uint64_t* bits;
-  uint32_t* tile_counts;
-  uint32_t* offsets;
-  const uint32_t* pstart;
-  int64_t* keys;
-  uint32_t* rows;
-  uint64_t* vals;
-  int64_t vstride;
-  int32_t P;
-  int32_t pad_;
-};
 
 struct Module {
   hipModule_t mod = nullptr;
@@ -105,7 +80,7 @@ Flat flatten_terms(const QDesc& d) {
 int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
 
 std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slot = 0,
-                     int nv = 0, int probe = 0) {
+                     int nv = 0, int probe = 0, int P = 0, int pbuf = 0) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
@@ -133,6 +108,10 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_KEYSLOT " + std::to_string(key_slot) + "\n";
   s += "#define VQ_NV " + std::to_string(nv) + "\n";
   s += "#define VQ_PROBE " + std::to_string(probe) + "\n";
+  if (kernels & 4) {
+    s += "#define VQ_P " + std::to_string(P) + "\n";
+    s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
+  }
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
@@ -259,32 +238,45 @@ bool jit_disabled() {
 
 }  // namespace
 
-bool jit_part_available(const JSide& sd, const JDesc* jd, std::string* err) {
-  if (jit_disabled()) return fail_jit(err, "JIT disabled") == 0;
+int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
+                     std::string* err) {
+  if (jit_disabled()) return fail_jit(err, "JIT disabled");
   int nv = 0;
   const QDesc d = side_desc(sd, jd, &nv);
   const Flat f = flatten_terms(d);
-  return get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr), err) != nullptr;
+  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, P, pbuf), err);
+  if (!m) return -1;
+  out->count = m->part_count;
+  out->scatter = m->part_scatter;
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+  }
+  int per_cu = 0;
+  hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->part_scatter, 1024, 0);
+  if (per_cu <= 0) per_cu = 1;
+  const int64_t g = (int64_t)cus * per_cu;
+  out->grid = (int32_t)std::max<int64_t>(1, d.ntiles < g ? d.ntiles : g);
+  fill_common(d, f, out->args);
+  out->args.P = P;
+  return 0;
 }
 
-hipError_t jit_launch_part(const JSide& sd, const JDesc* jd, int32_t P, uint32_t* counts,
-                           int64_t* keys, uint32_t* rows, uint64_t* vals, int64_t vstride,
-                           hipStream_t s, bool count) {
-  int nv = 0;
-  const QDesc d = side_desc(sd, jd, &nv);
-  const Flat f = flatten_terms(d);
-  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr), nullptr);
-  if (!m) return hipErrorInvalidDeviceFunction;
-  JitArgs a;
-  fill_common(d, f, a);
-  a.offsets = counts;
-  a.keys = keys;
-  a.rows = rows;
-  a.vals = vals;
-  a.vstride = vstride;
+hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, hipStream_t s,
+                           bool count) {
+  JitArgs a = pl.args;
+  a.counts = pb.counts;
+  a.pstart = pb.pstart;
+  a.perr = pb.perr;
+  a.keys = pb.keys;
+  a.rows = pb.rows;
+  a.vals = pb.vals;
+  a.vstride = pb.vstride;
   a.P = P;
-  return launch(count ? m->part_count : m->part_scatter, (unsigned)d.ntiles, (size_t)P * 4, s, a,
-                1024);
+  return launch(count ? pl.count : pl.scatter, (unsigned)pl.grid, 0, s, a, 1024);
 }
 
 int jit_dense_grid(const QDesc& d, std::string* err) {
