@@ -14,7 +14,11 @@ For each operator:
   value      algorithmic GB/s = referenced column bytes of the scanned row
              groups / wall time per query (inputs resident in HBM);
   roofline   the dominant kernel: algorithmic bytes per launch / its mean
-             HIP-event duration on the library's stream, against 8 TB/s;
+             HIP-event duration on the library's stream (20 back-to-back
+             launches between one event pair, so the events' overhead is
+             amortised), against 8 TB/s; for the join, whose work is spread
+             over eight kernels, the whole pipeline between its first and last
+             event;
              traffic from profiles/traffic.json (rocprofv3 FETCH_SIZE +
              WRITE_SIZE per dispatch) when present;
   cpu_baseline  the oracle's multithreaded restatement of the reference's
@@ -49,7 +53,14 @@ def load_traffic():
     return {}
 
 
-def time_gpu(fn, steps, warmup):
+KERNEL_REPEATS = 20
+
+
+def time_gpu(fn, steps, warmup, ctx=None):
+    """Times `steps` queries (after `warmup`).  Then, when ctx is given, the
+    query's main kernel alone: KERNEL_REPEATS back-to-back launches between
+    one pair of HIP events on the library's stream, three times; `mains`
+    holds the mean duration per launch."""
     for _ in range(warmup):
         fn()
     walls, mains, devs = [], [], []
@@ -62,6 +73,12 @@ def time_gpu(fn, steps, warmup):
         mains.append(r.stats.main_kernel_ms)
         devs.append(r.stats.device_ms)
     total = time.perf_counter() - t0
+    if ctx is not None:
+        ctx.set_kernel_repeats(KERNEL_REPEATS)
+        try:
+            mains = [fn().stats.main_kernel_ms for _ in range(3)]
+        finally:
+            ctx.set_kernel_repeats(1)
     return r, total, walls, mains, devs
 
 
@@ -186,7 +203,7 @@ def main():
         pred, keys, aggs = datagen.q_groupby(t)
         dt = ctx.upload(t)
         log(f"[bench] {name}: timing {K} steps")
-        r, total, walls, mains, devs = time_gpu(lambda: ctx.agg_dict(dt, pred, keys, aggs), K, W)
+        r, total, walls, mains, devs = time_gpu(lambda: ctx.agg_dict(dt, pred, keys, aggs), K, W, ctx)
         cpu = None
         if cpu_mod is not None:
             runs = time_cpu(lambda: cpu_mod.agg_dict(t, pred, keys, aggs, nthreads=cores))
@@ -202,7 +219,7 @@ def main():
         ts = datagen.lineitem_sorted(6_000_000)
         pred, keys, aggs = datagen.q_scan_sum(ts)
         dts = ctx.upload(ts)
-        r, total, walls, mains, devs = time_gpu(lambda: ctx.agg_dict(dts, pred, keys, aggs), K, W)
+        r, total, walls, mains, devs = time_gpu(lambda: ctx.agg_dict(dts, pred, keys, aggs), K, W, ctx)
         cpu = None
         if cpu_mod is not None:
             runs = time_cpu(lambda: cpu_mod.agg_dict(ts, pred, keys, aggs, nthreads=cores))
@@ -224,7 +241,7 @@ def main():
                 o = _R()
                 o.stats = ctx.last_stats
                 return o
-            r2, total, walls, mains, devs = time_gpu(filt, K, W)
+            r2, total, walls, mains, devs = time_gpu(filt, K, W, ctx)
             cpu = None
             if cpu_mod is not None:
                 runs = time_cpu(lambda: cpu_mod.filter_rows(ts, pred), max_runs=10)

@@ -164,6 +164,66 @@ __device__ __forceinline__ void load_rows(const Args& a, i64 base, i64 rend, int
   }
 }
 
+// Row-blocked loads for the dense GROUP BY: lane l takes the U consecutive
+// rows base + l*U .. base + l*U + U-1, read with 4- to 16-byte vector loads
+// (a wave still reads one contiguous span of each column).  r0 is clamped to
+// the last U-aligned start before rend, so a lane past the end re-reads rows
+// inside the column (plus at most U-1 rows of the upload's tail padding); the
+// caller masks rows >= rend.
+template <int T, int U>
+__device__ __forceinline__ void ld_block(const void* p, i64 r0, i64* out) {
+  if constexpr (T == VQ_U8) {
+    static_assert(U == 4 || U == 8, "U");
+    if constexpr (U == 4) {
+      const u32 w = *(const u32*)((const u8*)p + r0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) out[i] = (w >> (8 * i)) & 0xff;
+    } else {
+      const u64 w = *(const u64*)((const u8*)p + r0);
+#pragma unroll
+      for (int i = 0; i < 8; i++) out[i] = (i64)((w >> (8 * i)) & 0xff);
+    }
+  } else if constexpr (T == VQ_U16) {
+#pragma unroll
+    for (int h = 0; h < U / 4; h++) {
+      const u64 w = *(const u64*)((const u16*)p + r0 + 4 * h);
+#pragma unroll
+      for (int i = 0; i < 4; i++) out[4 * h + i] = (i64)((w >> (16 * i)) & 0xffff);
+    }
+  } else if constexpr (T == VQ_DATE) {
+#pragma unroll
+    for (int h = 0; h < U / 4; h++) {
+      const int4 w = *(const int4*)((const i32*)p + r0 + 4 * h);
+      out[4 * h + 0] = w.x;
+      out[4 * h + 1] = w.y;
+      out[4 * h + 2] = w.z;
+      out[4 * h + 3] = w.w;
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < U / 2; h++) {
+      const longlong2 w = *(const longlong2*)((const i64*)p + r0 + 2 * h);
+      out[2 * h + 0] = w.x;
+      out[2 * h + 1] = w.y;
+    }
+  }
+}
+
+template <int U, int... C>
+__device__ __forceinline__ void load_block_one(const Args& a, i64 r0, i64 (&raw)[VQ_NS][U],
+                                               Seq<C...>) {
+  (ld_block<kST[C], U>(a.col[C], r0, raw[C]), ...);
+}
+
+template <int U>
+__device__ __forceinline__ void load_rows_blocked(const Args& a, i64 base, i64 rend, int lane,
+                                                  i64 (&raw)[VQ_NS][U]) {
+  i64 r0 = base + (i64)lane * U;
+  const i64 last = ((rend - 1) / U) * U;
+  r0 = r0 < last ? r0 : last;
+  load_block_one(a, r0, raw, seq<VQ_NS>{});
+}
+
 template <int T>
 __device__ __forceinline__ bool term(const Args& a, const i64* rawrow) {
   constexpr int S = kTS[T];
@@ -382,7 +442,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
   TileIt it;
   tile_begin(a, blockIdx.x, it);
   i64 nraw[VQ_NS][U];
-  if (it.have) load_rows<U>(a, it.base, it.rend, lane, nraw);
+  if (it.have) load_rows_blocked<U>(a, it.base, it.rend, lane, nraw);
   while (it.have) {
     i64 raw[VQ_NS][U];
 #pragma unroll
@@ -391,13 +451,13 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
       for (int u = 0; u < U; u++) raw[c][u] = nraw[c][u];
     const i64 base = it.base, rend = it.rend;
     tile_advance<U>(a, it);
-    if (it.have) load_rows<U>(a, it.base, it.rend, lane, nraw);
+    if (it.have) load_rows_blocked<U>(a, it.base, it.rend, lane, nraw);
 #pragma unroll
     for (int u = 0; u < U; u++) {
       i64 row[VQ_NS];
 #pragma unroll
       for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
-      const i64 r = base + u * kWave + lane;
+      const i64 r = base + (i64)lane * U + u;
       if (!(r < rend && pass_all(a, row, seq<VQ_NT>{}))) continue;
       if constexpr (G == 1) {
         rfirst = rcnt == 0 ? (u32)r : rfirst;

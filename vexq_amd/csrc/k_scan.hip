@@ -659,15 +659,17 @@ hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part
 
 hipError_t launch_filter(const QDesc& d, bool jit, uint64_t* bits, uint32_t* tile_counts,
                          uint32_t* tile_offsets, uint32_t* out_rows, uint64_t* out_count,
-                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, int32_t repeats) {
   if (d.ntiles == 0) return hipMemsetAsync(out_count, 0, sizeof(uint64_t), s);
   if (ev0) hipEventRecord(ev0, s);
-  if (jit) {
-    hipError_t e = jit_launch_filter_bits(d, bits, tile_counts, s);
-    if (e != hipSuccess) return e;
-  } else {
-    hipLaunchKernelGGL((k_filter_bits<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, bits,
-                       tile_counts);
+  for (int32_t rep = 0; rep < repeats; rep++) {  // identical bits each time
+    if (jit) {
+      hipError_t e = jit_launch_filter_bits(d, bits, tile_counts, s);
+      if (e != hipSuccess) return e;
+    } else {
+      hipLaunchKernelGGL((k_filter_bits<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, bits,
+                         tile_counts);
+    }
   }
   if (ev1) hipEventRecord(ev1, s);
   hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(1024), 0, s, tile_counts, d.ntiles,

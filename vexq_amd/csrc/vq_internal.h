@@ -160,7 +160,8 @@ hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part
                             hipStream_t s);
 hipError_t launch_filter(const QDesc& d, bool jit, uint64_t* bits, uint32_t* tile_counts,
                          uint32_t* tile_offsets, uint32_t* out_rows, uint64_t* out_count,
-                         hipStream_t s, hipEvent_t ev_main0, hipEvent_t ev_main1);
+                         hipStream_t s, hipEvent_t ev_main0, hipEvent_t ev_main1,
+                         int32_t repeats);
 
 // Shape-specialized kernels compiled at run time (vq_jit.hip).  jit_dense_grid
 // returns the persistent grid, or -1 (with *err) when no kernel is available.

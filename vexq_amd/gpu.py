@@ -35,6 +35,7 @@ def lib():
     L.vq_last_error.restype = C.c_char_p
     L.vq_ctx_create.argtypes = [C.c_int, C.c_uint64, P(C.c_void_p)]
     L.vq_ctx_destroy.argtypes = [C.c_void_p]
+    L.vq_ctx_set_kernel_repeats.argtypes = [C.c_void_p, C.c_int32]
     L.vq_ctx_bytes_in_use.argtypes = [C.c_void_p]
     L.vq_ctx_bytes_in_use.restype = C.c_uint64
     L.vq_table_upload.argtypes = [C.c_void_p, C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32,
@@ -95,6 +96,11 @@ class Context:
     @property
     def bytes_in_use(self) -> int:
         return int(lib().vq_ctx_bytes_in_use(self.h))
+
+    def set_kernel_repeats(self, repeats: int) -> None:
+        """Timing aid: launch each query's main kernel `repeats` times inside
+        its event pair (stats.main_kernel_ms is the mean per launch)."""
+        _check(lib().vq_ctx_set_kernel_repeats(self.h, repeats))
 
     def upload(self, table, zones=None) -> "DeviceTable":
         return DeviceTable(self, table, zones)
