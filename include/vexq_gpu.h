@@ -265,6 +265,69 @@ int vq_join_agg(vq_ctx* ctx, const vq_table* build, int32_t build_key, const vq_
                 int32_t naggs, const vq_agg* aggs, int32_t radix_bits, int32_t topk,
                 vq_agg_result* out, vq_stats* stats);
 
+/* ---- concurrent and asynchronous queries (serving) ------------------------
+ * No counterpart in the reference's exec package: its operators run one
+ * query per call on the caller's goroutine.  A served query stream
+ * (planner.Parallel called from many request goroutines) needs queries that
+ * overlap on the device and an admission bound on their device memory.
+ *
+ * vq_ctx_set_slots: nslots (1..16) queries of this context may run at once,
+ * each on its own stream with its own scratch arena (calls from several host
+ * threads, or vq_submit, use them); queries started through vq_submit are
+ * admitted in arrival order while the device memory they are estimated to
+ * need, summed over the running ones, stays within serve_budget bytes
+ * (0: half the context budget); a query larger than the bound runs alone.
+ * Call with no query in flight. */
+int vq_ctx_set_slots(vq_ctx* ctx, int32_t nslots, uint64_t serve_budget);
+
+#define VQ_Q_AGG_DICT 1
+#define VQ_Q_JOIN_AGG 2
+
+/* One query for vq_submit: the arguments of vq_agg_dict (kind
+ * VQ_Q_AGG_DICT: table, pred, nkeys, key_cols, naggs, aggs) or of
+ * vq_join_agg (kind VQ_Q_JOIN_AGG: build, build_key, build_pred, table as the
+ * probe side, probe_key, pred as the probe predicate, naggs, aggs,
+ * radix_bits, topk).  The arrays are copied by vq_submit; the tables, out and
+ * stats must stay valid until vq_wait returns. */
+typedef struct vq_query {
+  int32_t kind;
+  int32_t nkeys;
+  const vq_table* table;
+  const vq_pred* pred;
+  const int32_t* key_cols;
+  const vq_table* build;
+  const vq_pred* build_pred;
+  int32_t build_key;
+  int32_t probe_key;
+  int32_t naggs;
+  int32_t radix_bits;
+  const vq_agg* aggs;
+  int32_t topk;
+  int32_t pad_;
+} vq_query;
+
+typedef struct vq_ticket vq_ticket;
+
+/* Queue a query; returns at once with a ticket.  The result is written to
+ * out/stats by the time vq_wait returns. */
+int vq_submit(vq_ctx* ctx, const vq_query* q, vq_agg_result* out, vq_stats* stats,
+              vq_ticket** ticket);
+/* 1 once the query has finished (successfully or not), else 0. */
+int vq_ticket_done(const vq_ticket* ticket);
+/* Block until the query has finished; returns its status (its message is then
+ * this thread's vq_last_error) and frees the ticket. */
+int vq_wait(vq_ticket* ticket);
+
+/* GROUP BY one integer column of any cardinality: HashAggregate's general
+ * key path (exec/aggregate.go:361-559; the int-key path's output order,
+ * exec/agg_intkey.go:206-361) for keys that are not small dictionary codes —
+ * u8/u16/date/int64 columns, up to millions of groups.  Rows passing pred are
+ * radix-partitioned on radixHash(key) and each partition aggregated in an
+ * LDS hash table; out->keys holds the key values, groups in first-occurrence
+ * order; float sums are exact (fixed point) and rounded once. */
+int vq_agg_hash(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t key_col,
+                int32_t naggs, const vq_agg* aggs, vq_agg_result* out, vq_stats* stats);
+
 /* RadixBitsFor (exec/parallel_join.go:73-82), radixHash (91-99) and
  * joinTableCapacity (exec/join_table.go:461-476), exported so host planners
  * and tests share the library's definitions. */

@@ -734,6 +734,45 @@ int or_agg_dict(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t n
   return rc;
 }
 
+/* Serial GROUP BY one integer column: TableScan(zone pruned) -> Filter ->
+ * HashAggregate over the general key path (exec/aggregate.go:361-559), groups
+ * in first-seen order (the order its output walks, aggregate.go:780-876). */
+int or_agg_hash(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                int32_t key_col, int32_t naggs, const vq_agg* aggs, int32_t exact,
+                vq_agg_result* out) {
+  (void)nrows;
+  if (key_col < 0 || key_col >= ncols || naggs < 0 || naggs > VQ_MAX_AGGS) return VQ_E_ARG;
+  uint8_t* keep = (uint8_t*)malloc((size_t)nrg + 1);
+  or_zone_prune(ncols, cols, nrg, zones, pred, keep);
+  kmap m;
+  kmap_init(&m, 1024);
+  glist l = {0, 0, 0};
+  int64_t base = 0;
+  for (int32_t rg = 0; rg < nrg; rg++) {
+    if (keep[rg])
+      for (int64_t r = base; r < base + rg_rows[rg]; r++) {
+        if (!pred_pass(cols, pred, r)) continue;
+        const int64_t key = col_i(&cols[key_col], r);
+        group* g = kmap_group(&m, &l, key, naggs, aggs, exact);
+        if (g->first_row < 0) {
+          g->first_row = r;
+          g->key[0] = key;
+        }
+        fold_row(g, cols, naggs, aggs, r);
+      }
+    base += rg_rows[rg];
+  }
+  int rc = check_cap(out, l.n);
+  if (rc == VQ_OK)
+    for (int64_t i = 0; i < l.n; i++) emit_group(&l.g[i], 1, naggs, aggs, out, i);
+  for (int64_t i = 0; i < l.n; i++) group_free(&l.g[i], naggs);
+  free(l.g);
+  kmap_free(&m);
+  free(keep);
+  return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* flat join table (exec/join_table.go)                                      */
 /* ------------------------------------------------------------------------ */

@@ -51,6 +51,13 @@ int or_agg_dict(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t n
                 int32_t nkeys, const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                 int32_t exact, vq_agg_result* out);
 
+/* Serial GROUP BY one integer column (HashAggregate's general key path):
+ * groups in first-seen order, key values in out->keys. */
+int or_agg_hash(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                int32_t key_col, int32_t naggs, const vq_agg* aggs, int32_t exact,
+                vq_agg_result* out);
+
 /* Serial HashJoin -> HashAggregate(GROUP BY join key) [-> Sort DESC -> Limit]. */
 int or_join_agg(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, int32_t b_key,
                 const vq_pred* b_pred, int64_t p_nrows, int32_t p_ncols,

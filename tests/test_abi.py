@@ -43,7 +43,8 @@ def test_struct_layout_matches_c(tmp_path):
     every struct's size and field offsets with the ctypes mirror."""
     structs = {"vq_zone": abi.Zone, "vq_col_desc": abi.ColDesc, "vq_term": abi.Term,
                "vq_pred": abi.Pred, "vq_factor": abi.Factor, "vq_expr": abi.Expr,
-               "vq_agg": abi.Agg, "vq_agg_result": abi.AggResult, "vq_stats": abi.Stats}
+               "vq_agg": abi.Agg, "vq_agg_result": abi.AggResult, "vq_stats": abi.Stats,
+               "vq_query": abi.Query}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

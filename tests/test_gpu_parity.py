@@ -270,3 +270,50 @@ def test_join_duplicates_extremes_empty(gpu_ctx):
     # key 5 has three build rows and two probe rows: six join rows
     i5 = list(got.keys[:, 0]).index(5)
     assert got.counts[i5] == 6 and got.fvals[i5, 0] == 3 * (0.5 + 7.5)
+
+
+# ---------------------------------------------------------------------------
+# GROUP BY one integer column of any cardinality (vq_agg_hash)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nrows,groups,key_type,skew", [
+    (10_000, 4, abi.VQ_I64, 0.0),
+    (200_000, 1000, abi.VQ_U16, 1.1),
+    (300_000, 60_000, abi.VQ_U16, 0.0),
+    (500_000, 100_000, abi.VQ_I64, 0.0),
+    (400_000, 20_000, abi.VQ_I64, 1.1),
+    (1_500_000, 1_000_000, abi.VQ_I64, 0.0),
+])
+def test_agg_hash_matches_oracle(gpu_ctx, nrows, groups, key_type, skew):
+    _print_case("agg_hash", nrows=nrows, groups=groups, key_type=key_type, skew=skew)
+    t = datagen.keyed_table(nrows, groups, key_type, skew, seed=groups % 97)
+    pred, key, aggs = datagen.q_keyed(t)
+    dt = gpu_ctx.upload(t)
+    try:
+        got = gpu_ctx.agg_hash(dt, pred, key, aggs)
+        again = gpu_ctx.agg_hash(dt, pred, key, aggs)
+    finally:
+        dt.close()
+    ref = O.agg_hash(t, pred, key, aggs, exact=True)
+    assert_agg_equal(got, ref, aggs)
+    assert same_bits(got, again)
+    naive = O.agg_hash(t, pred, key, aggs, exact=False)
+    assert_agg_equal(got, naive, aggs, rel=1e-9)
+
+
+def test_agg_hash_extreme_keys_and_empty(gpu_ctx):
+    _print_case("agg_hash_extremes")
+    keys = np.array([0, -1, np.iinfo(np.int64).min, np.iinfo(np.int64).max, 5, 0,
+                     np.iinfo(np.int64).min, -1, 7, np.iinfo(np.int64).max] * 300, dtype=np.int64)
+    n = len(keys)
+    price = np.arange(n, dtype=np.float64) * 0.25
+    t = datagen.Table([datagen._col("key", abi.VQ_I64, keys),
+                       datagen._col("price", abi.VQ_F64, price)], datagen.row_groups(n))
+    aggs = [abi.agg_sum("price", t), abi.count_star(t), abi.agg_max("price", t)]
+    dt = gpu_ctx.upload(t)
+    try:
+        got = gpu_ctx.agg_hash(dt, None, 0, aggs)
+        none = gpu_ctx.agg_hash(dt, abi.make_pred(t, [("price", "<", -1.0)]), 0, aggs)
+    finally:
+        dt.close()
+    assert_agg_equal(got, O.agg_hash(t, None, 0, aggs), aggs)
+    assert none.ngroups == 0

@@ -416,3 +416,37 @@ def test_join_empty_build():
     b, p = _join_pair([1, 2], [1, 2])
     r = O.join_agg(b, 0, abi.make_pred(b, [("pay", "<", 0)]), p, 0, None, [abi.count_star(p)])
     assert r.ngroups == 0
+
+
+def test_agg_hash_matches_numpy_and_the_dictionary_path():
+    """The general-key GROUP BY: first-seen group order, exact counts and
+    sums against numpy; on dictionary codes it equals the int-key path."""
+    import math
+    t = datagen.keyed_table(60_000, 500, abi.VQ_I64, 1.1, seed=3)
+    pred, key, aggs = datagen.q_keyed(t)
+    r = O.agg_hash(t, pred, key, aggs)
+    k = t.col("key")
+    keep = t.col("shipdate") <= datagen.DATE_MIN + datagen.SHIP_DAYS - 1 - 90
+    kk = k[keep]
+    uniq, first = np.unique(kk, return_index=True)
+    order = np.argsort(first)
+    assert r.ngroups == len(uniq)
+    np.testing.assert_array_equal(r.keys[:, 0], uniq[order])
+    rows = np.nonzero(keep)[0]
+    np.testing.assert_array_equal(r.first_row, rows[first[order]])
+    qty = t.col("quantity")[keep]
+    val = (t.col("price") * (1 - t.col("discount")))[keep]
+    for g, kv in enumerate(uniq[order]):
+        m = kk == kv
+        assert r.counts[g] == m.sum()
+        assert r.ivals[g, 1] == qty[m].sum()
+        assert r.fvals[g, 0] == math.fsum(val[m])
+    # dictionary codes: the two key paths agree
+    li = datagen.lineitem(20_000, seed=5)
+    p2, keys2, aggs2 = datagen.q_groupby(li)
+    a = O.agg_dict(li, p2, [keys2[0]], aggs2)
+    b = O.agg_hash(li, p2, keys2[0], aggs2)
+    assert a.ngroups == b.ngroups
+    np.testing.assert_array_equal(a.keys, b.keys)
+    np.testing.assert_array_equal(a.counts, b.counts)
+    np.testing.assert_array_equal(a.fvals.view(np.int64), b.fvals.view(np.int64))

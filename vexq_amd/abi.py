@@ -82,6 +82,20 @@ _OPS = {"==": VQ_EQ, "=": VQ_EQ, "!=": VQ_NE, "<>": VQ_NE, "<": VQ_LT, "<=": VQ_
         ">": VQ_GT, ">=": VQ_GE}
 
 
+VQ_Q_AGG_DICT = 1
+VQ_Q_JOIN_AGG = 2
+
+
+class Query(C.Structure):
+    """vq_query: one query for vq_submit."""
+    _fields_ = [("kind", C.c_int32), ("nkeys", C.c_int32), ("table", C.c_void_p),
+                ("pred", C.POINTER(Pred)), ("key_cols", C.POINTER(C.c_int32)),
+                ("build", C.c_void_p), ("build_pred", C.POINTER(Pred)),
+                ("build_key", C.c_int32), ("probe_key", C.c_int32), ("naggs", C.c_int32),
+                ("radix_bits", C.c_int32), ("aggs", C.POINTER(Agg)), ("topk", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 def make_pred(table, terms) -> Pred:
     """terms: iterable of (column name, op string, literal).  BETWEEN is two
     terms (>= lo, <= hi), the reference's own rewrite (exec/expr.go:1319-1336)."""

@@ -549,7 +549,8 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
 // LDS atomics, so the order inside a partition can differ from run to run;
 // every consumer aggregates with order-independent integer updates.
 // VQ_PB = 0 (partition counts too large for LDS buffers) writes each row
-// straight to its position.
+// straight to its position, claimed from the workgroup's cursors in LDS (or,
+// past 16384 partitions, in its own column of counts).
 // ---------------------------------------------------------------------------
 namespace vqj {
 
@@ -766,8 +767,13 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a)
     b.cur[p] = a.pstart[p] + a.counts[p * G + blockIdx.x];
   }
   __syncthreads();
+#elif VQ_P <= 16384
+  // this workgroup's own cursors, in LDS
+  __shared__ u32 cur[kP];
+  for (int p = threadIdx.x; p < kP; p += kPB) cur[p] = a.pstart[p] + a.counts[p * G + blockIdx.x];
+  __syncthreads();
 #else
-  // this workgroup's own cursors, advanced in place
+  // this workgroup's own cursors, advanced in place (too many for LDS)
   for (int p = threadIdx.x; p < kP; p += kPB) a.counts[p * G + blockIdx.x] += a.pstart[p];
   __syncthreads();
 #endif
@@ -816,7 +822,11 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a)
       for (int u = 0; u < U; u++) {
         if (!q.live[u]) continue;
         const u32 p = q.part[u];
+#if VQ_P <= 16384
+        const u32 pos = atomicAdd(&cur[p], 1u);
+#else
         const u32 pos = atomicAdd(&a.counts[p * G + blockIdx.x], 1u);
+#endif
         if (pos >= a.pstart[p + 1]) {
           atomicOr(a.perr, 1u);
           continue;

@@ -360,7 +360,8 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   __syncthreads();
 
   // build: the partition's build keys; multiplicity = build rows per key
-  const uint32_t b0 = B.b.pstart[p], b1 = B.b.pstart[p + 1];
+  // (GROUP BY mode has no build side: keys enter the table as rows arrive)
+  const uint32_t b0 = plan.groupby ? 0 : B.b.pstart[p], b1 = plan.groupby ? 0 : B.b.pstart[p + 1];
   for (uint32_t e = b0 + threadIdx.x; e < b1; e += kJoinThreads) {
     const int64_t key = B.b.keys[e];
     if (key == kEmptyKey) {
@@ -384,6 +385,68 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     if (!done) s_fail = 1;
   }
   __syncthreads();
+
+  // GROUP BY mode: the partition's keys are its rows' keys.  Every key goes
+  // into the table first, then each occupied slot gets the pool entry of its
+  // rank in slot order (so no two lanes ever race for a new group's entry),
+  // and the probe below finds every key claimed.
+  if (plan.groupby) {
+    const uint32_t g0 = B.p.pstart[p], g1 = B.p.pstart[p + 1];
+    for (uint32_t e = g0 + threadIdx.x; e < g1; e += kJoinThreads) {
+      const int64_t key = B.p.keys[e];
+      if (key == kEmptyKey) {
+        tag[cap] = 1;
+        continue;
+      }
+      const uint64_t h = radix_hash((uint64_t)key);
+      uint32_t sl = lg ? (uint32_t)(h >> (64 - lg)) : 0;
+      bool done = false;
+      for (int probe = 0; probe < cap; probe++) {
+        const unsigned long long old =
+            atomicCAS(&keys[sl], (unsigned long long)kEmptyKey, (unsigned long long)key);
+        if (old == (unsigned long long)kEmptyKey || old == (unsigned long long)key) {
+          if (old == (unsigned long long)kEmptyKey) tag[sl] = 1;
+          done = true;
+          break;
+        }
+        sl = (sl + 1) & (uint32_t)(cap - 1);
+      }
+      if (!done) s_fail = 1;
+    }
+    __syncthreads();
+    // rank the occupied slots: each thread a contiguous run of slots
+    const int per = (ns + kJoinThreads - 1) / kJoinThreads;
+    const int lo = threadIdx.x * per < ns ? threadIdx.x * per : ns;
+    const int hi = lo + per < ns ? lo + per : ns;
+    uint32_t mine = 0;
+    for (int i = lo; i < hi; i++) mine += tag[i] ? 1u : 0u;
+    uint32_t* sc = (uint32_t*)&s_cand[0];  // kJoinThreads words of scratch
+    sc[threadIdx.x] = mine;
+    __syncthreads();
+    for (int off = 1; off < kJoinThreads; off <<= 1) {
+      const uint32_t x = (int)threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += x;
+      __syncthreads();
+    }
+    uint32_t r = sc[threadIdx.x] - mine;
+    const uint32_t total = sc[kJoinThreads - 1];
+    for (int i = lo; i < hi; i++) {
+      if (!tag[i]) continue;
+      if (r < (uint32_t)NP) {
+        pslot[r] = (uint32_t)i;
+        pmult[r] = 1;
+      }
+      tag[i] = kClaimed | r;
+      r++;
+    }
+    if (threadIdx.x == 0) {
+      s_nkeys = total;
+      s_pool = total;
+      if (total > (uint32_t)NP) s_fail |= 2;  // more groups than pool entries
+    }
+    __syncthreads();
+  }
   const bool built = s_fail == 0;
   if (built && threadIdx.x == 0 && s_nkeys) atomicAdd(&flags[2], s_nkeys);
 
@@ -608,26 +671,28 @@ hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd
   const int P = plan.P;
   if (ev) hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, B.flags, (int64_t)16);
-  hipError_t e = jit_part_launch(b, B.b, P, s, true);
+  const bool has_build = !plan.groupby;
+  hipError_t e = has_build ? jit_part_launch(b, B.b, P, s, true) : hipSuccess;
   if (e != hipSuccess) return e;
   e = jit_part_launch(p, B.p, P, s, true);
   if (e != hipSuccess) return e;
+  // sides in the scan's order: build (when there is one), then probe
   PartScan ps;
   memset(&ps, 0, sizeof ps);
-  ps.counts[0] = B.b.counts;
-  ps.counts[1] = B.p.counts;
-  ps.tot[0] = B.b.tot;
-  ps.tot[1] = B.p.tot;
-  ps.pstart[0] = B.b.pstart;
-  ps.pstart[1] = B.p.pstart;
-  ps.total[0] = B.flags + 6;
-  ps.total[1] = B.flags + 7;
-  ps.G[0] = B.b.G;
-  ps.G[1] = B.p.G;
+  int ns = 0;
+  const PartBufs* sides[2] = {has_build ? &B.b : &B.p, &B.p};
+  uint32_t* totals[2] = {has_build ? B.flags + 6 : B.flags + 7, B.flags + 7};
+  for (ns = 0; ns < (has_build ? 2 : 1); ns++) {
+    ps.counts[ns] = sides[ns]->counts;
+    ps.tot[ns] = sides[ns]->tot;
+    ps.pstart[ns] = sides[ns]->pstart;
+    ps.total[ns] = totals[ns];
+    ps.G[ns] = sides[ns]->G;
+  }
   ps.P = P;
-  hipLaunchKernelGGL(k_part_runs, dim3((unsigned)P, 2), dim3(kScanThreads), 0, s, ps);
-  hipLaunchKernelGGL(k_part_starts, dim3(2), dim3(kScanThreads), 0, s, ps);
-  e = jit_part_launch(b, B.b, P, s, false);
+  hipLaunchKernelGGL(k_part_runs, dim3((unsigned)P, (unsigned)ns), dim3(kScanThreads), 0, s, ps);
+  hipLaunchKernelGGL(k_part_starts, dim3((unsigned)ns), dim3(kScanThreads), 0, s, ps);
+  e = has_build ? jit_part_launch(b, B.b, P, s, false) : hipSuccess;
   if (e != hipSuccess) return e;
   e = jit_part_launch(p, B.p, P, s, false);
   if (e != hipSuccess) return e;

@@ -6,8 +6,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -55,24 +58,43 @@ struct Arena {
 
 }  // namespace
 
-struct vq_ctx {
-  int device = 0;
+// One query slot: a stream and the per-query resources that go with it.  A
+// query holds one slot from its first launch to its result; queries on
+// different slots run concurrently on the device.
+struct Slot {
   hipStream_t stream = nullptr;
-  uint64_t budget = 0;
-  std::atomic<uint64_t> in_use{0};
-  std::mutex mu;  // one query at a time per context (arena + stream)
   Arena arena;
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   hipEvent_t ev[6] = {};
-  // cached selection-vector buffer
-  void* sv_cache = nullptr;
-  size_t sv_cache_bytes = 0;
+};
+
+struct Ticket;
+
+struct vq_ctx {
+  int device = 0;
+  uint64_t budget = 0;
+  std::atomic<uint64_t> in_use{0};
+  std::mutex mu;  // the slot pool and the caches below
+  std::condition_variable slot_cv;
+  std::vector<Slot*> slots, idle;
   // build rows that passed, by (build table, key column, predicate): the
   // partition count of the next join over the same build side
   std::unordered_map<std::string, int64_t> build_est;
   // launches of each query's main kernel inside its event pair (timing aid)
   int32_t kernel_repeats = 1;
+  // cached selection-vector buffer
+  void* sv_cache = nullptr;
+  size_t sv_cache_bytes = 0;
+  // asynchronous queries (vq_submit): FIFO admission against serve_budget
+  std::mutex qmu;
+  std::condition_variable qcv;
+  std::deque<Ticket*> queue;
+  std::vector<std::thread> workers;
+  uint64_t serve_budget = 0;
+  uint64_t serve_reserved = 0;
+  int running = 0;
+  bool stopping = false;
 };
 
 struct ColDev {
@@ -111,41 +133,98 @@ bool reserve(vq_ctx* c, uint64_t bytes) {
 
 void release(vq_ctx* c, uint64_t bytes) { c->in_use.fetch_sub(bytes); }
 
-int arena_get(vq_ctx* c, size_t bytes, void** out) {
+int arena_get(vq_ctx* c, Slot* S, size_t bytes, void** out) {
   bytes = (bytes + 255) & ~(size_t)255;
-  if (c->arena.bytes < bytes) {
-    size_t want = std::max(bytes, c->arena.bytes * 3 / 2);
-    if (c->arena.p) {
-      HIP_TRY(hipStreamSynchronize(c->stream));
-      HIP_TRY(hipFree(c->arena.p));
-      release(c, c->arena.bytes);
-      c->arena.p = nullptr;
-      c->arena.bytes = 0;
+  if (S->arena.bytes < bytes) {
+    size_t want = std::max(bytes, S->arena.bytes * 3 / 2);
+    if (S->arena.p) {
+      HIP_TRY(hipStreamSynchronize(S->stream));
+      HIP_TRY(hipFree(S->arena.p));
+      release(c, S->arena.bytes);
+      S->arena.p = nullptr;
+      S->arena.bytes = 0;
     }
     if (!reserve(c, want))
       return fail(VQ_E_BUDGET, "exec: gpu: query arena of %zu bytes exceeds the device budget", want);
-    hipError_t e = hipMalloc(&c->arena.p, want);
+    hipError_t e = hipMalloc(&S->arena.p, want);
     if (e != hipSuccess) {
       release(c, want);
       return fail(VQ_E_DEVICE, "exec: gpu: hipMalloc(%zu): %s", want, hipGetErrorString(e));
     }
-    c->arena.bytes = want;
+    S->arena.bytes = want;
   }
-  *out = c->arena.p;
+  *out = S->arena.p;
   return VQ_OK;
 }
 
-int pinned_get(vq_ctx* c, size_t bytes, void** out) {
-  if (c->pinned_bytes < bytes) {
-    if (c->pinned) HIP_TRY(hipHostFree(c->pinned));
-    c->pinned = nullptr;
+int pinned_get(Slot* S, size_t bytes, void** out) {
+  if (S->pinned_bytes < bytes) {
+    if (S->pinned) HIP_TRY(hipHostFree(S->pinned));
+    S->pinned = nullptr;
     size_t want = std::max(bytes, (size_t)1 << 16);
-    HIP_TRY(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
-    c->pinned_bytes = want;
+    HIP_TRY(hipHostMalloc(&S->pinned, want, hipHostMallocDefault));
+    S->pinned_bytes = want;
   }
-  *out = c->pinned;
+  *out = S->pinned;
   return VQ_OK;
 }
+
+int slot_create(Slot** out) {
+  Slot* S = new (std::nothrow) Slot();
+  if (!S) return fail(VQ_E_NOMEM, "exec: gpu: slot allocation");
+  hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  for (auto& ev : S->ev)
+    if (e == hipSuccess) e = hipEventCreate(&ev);
+  if (e != hipSuccess) {
+    for (auto& ev : S->ev)
+      if (ev) hipEventDestroy(ev);
+    if (S->stream) hipStreamDestroy(S->stream);
+    delete S;
+    return fail(VQ_E_DEVICE, "exec: gpu: stream/event creation: %s", hipGetErrorString(e));
+  }
+  *out = S;
+  return VQ_OK;
+}
+
+void slot_destroy(vq_ctx* c, Slot* S) {
+  hipStreamSynchronize(S->stream);
+  if (S->arena.p) {
+    hipFree(S->arena.p);
+    release(c, S->arena.bytes);
+  }
+  if (S->pinned) hipHostFree(S->pinned);
+  for (auto& ev : S->ev)
+    if (ev) hipEventDestroy(ev);
+  hipStreamDestroy(S->stream);
+  delete S;
+}
+
+void stop_workers(vq_ctx* c);
+
+// A query's hold on one slot: waits for an idle slot, gives it back at the
+// end of the scope.
+class SlotLease {
+ public:
+  explicit SlotLease(vq_ctx* c) : c_(c) {
+    std::unique_lock<std::mutex> lk(c->mu);
+    c->slot_cv.wait(lk, [&] { return !c->idle.empty(); });
+    s_ = c->idle.back();
+    c->idle.pop_back();
+  }
+  ~SlotLease() {
+    {
+      std::lock_guard<std::mutex> lk(c_->mu);
+      c_->idle.push_back(s_);
+    }
+    c_->slot_cv.notify_one();
+  }
+  Slot* operator->() const { return s_; }
+  Slot* get() const { return s_; }
+
+ private:
+  vq_ctx* c_;
+  Slot* s_;
+};
 
 // Bump allocator over the arena.
 struct Carve {
@@ -407,18 +486,13 @@ extern "C" int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out) {
     return fail(VQ_E_DEVICE, "exec: gpu: hipMemGetInfo: %s", hipGetErrorString(e));
   }
   c->budget = budget_bytes ? budget_bytes : (uint64_t)(fr * 0.9);
-  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
+  Slot* S = nullptr;
+  if (int rc = slot_create(&S)) {
     delete c;
-    return fail(VQ_E_DEVICE, "exec: gpu: hipStreamCreate: %s", hipGetErrorString(e));
+    return rc;
   }
-  for (auto& ev : c->ev) {
-    e = hipEventCreate(&ev);
-    if (e != hipSuccess) {
-      delete c;
-      return fail(VQ_E_DEVICE, "exec: gpu: hipEventCreate: %s", hipGetErrorString(e));
-    }
-  }
+  c->slots.push_back(S);
+  c->idle.push_back(S);
   *out = c;
   return VQ_OK;
 }
@@ -426,13 +500,9 @@ extern "C" int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out) {
 extern "C" int vq_ctx_destroy(vq_ctx* c) {
   if (!c) return VQ_OK;
   hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
-  if (c->arena.p) hipFree(c->arena.p);
+  stop_workers(c);
+  for (Slot* S : c->slots) slot_destroy(c, S);
   if (c->sv_cache) hipFree(c->sv_cache);
-  if (c->pinned) hipHostFree(c->pinned);
-  for (auto& ev : c->ev)
-    if (ev) hipEventDestroy(ev);
-  hipStreamDestroy(c->stream);
   delete c;
   return VQ_OK;
 }
@@ -528,7 +598,8 @@ extern "C" int vq_table_upload(vq_ctx* c, int64_t nrows, int32_t ncols, const vq
 extern "C" int vq_table_destroy(vq_table* t) {
   if (!t) return VQ_OK;
   hipSetDevice(t->ctx->device);
-  hipStreamSynchronize(t->ctx->stream);
+  // the caller ends every query on the table first; finish any in flight
+  for (Slot* S : t->ctx->slots) hipStreamSynchronize(S->stream);
   for (auto& x : t->cols) hipFree(x.d);
   release(t->ctx, t->bytes);
   {
@@ -703,7 +774,7 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   int64_t rows = 0, kept = 0;
   if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
 
-  std::lock_guard<std::mutex> lk(c->mu);
+  SlotLease S(c);
   hipSetDevice(c->device);
   const int NI = d.nacc + 2;
   std::string jerr;
@@ -713,25 +784,25 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   const size_t part_words = (size_t)std::max<int32_t>(grid, 1) * d.ngroups * NI;
   const size_t out_words = (size_t)d.ngroups * NI;
   void* ar;
-  if (int rc = arena_get(c, (part_words + out_words) * 8 + 1024, &ar)) return rc;
+  if (int rc = arena_get(c, S.get(), (part_words + out_words) * 8 + 1024, &ar)) return rc;
   Carve cv;
   uint64_t* part = cv.take<uint64_t>((char*)ar, part_words);
   uint64_t* fin = cv.take<uint64_t>((char*)ar, out_words);
   void* hp;
-  if (int rc = pinned_get(c, out_words * 8, &hp)) return rc;
+  if (int rc = pinned_get(S.get(), out_words * 8, &hp)) return rc;
   uint64_t* hfin = (uint64_t*)hp;
 
   if (d.ntiles > 0) {
-    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    HIP_TRY(hipEventRecord(S->ev[0], S->stream));
     for (int32_t rep = 0; rep < c->kernel_repeats; rep++) {  // identical partials each time
-      if (jit) HIP_TRY(jit_launch_dense(d, grid, part, c->stream));
-      else HIP_TRY(launch_agg_dense(d, grid, part, c->stream));
+      if (jit) HIP_TRY(jit_launch_dense(d, grid, part, S->stream));
+      else HIP_TRY(launch_agg_dense(d, grid, part, S->stream));
     }
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    HIP_TRY(launch_agg_final(d, grid, part, fin, c->stream));
-    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    HIP_TRY(hipMemcpyAsync(hfin, fin, out_words * 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipEventRecord(S->ev[1], S->stream));
+    HIP_TRY(launch_agg_final(d, grid, part, fin, S->stream));
+    HIP_TRY(hipEventRecord(S->ev[2], S->stream));
+    HIP_TRY(hipMemcpyAsync(hfin, fin, out_words * 8, hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(hipStreamSynchronize(S->stream));
   } else {
     for (int g = 0; g < d.ngroups; g++)
       for (int i = 0; i < NI; i++) hfin[g * NI + i] = (i == d.nacc + 1) ? 0xFFFFFFFFULL : 0;
@@ -739,8 +810,8 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   if (stats) {
     memset(stats, 0, sizeof *stats);
     if (d.ntiles > 0) {
-      stats->main_kernel_ms = elapsed(c->ev[0], c->ev[1]) / c->kernel_repeats;
-      stats->device_ms = stats->main_kernel_ms + elapsed(c->ev[1], c->ev[2]);
+      stats->main_kernel_ms = elapsed(S->ev[0], S->ev[1]) / c->kernel_repeats;
+      stats->device_ms = stats->main_kernel_ms + elapsed(S->ev[1], S->ev[2]);
     }
     stats->rows_scanned = rows;
     stats->rg_total = t->nrg;
@@ -839,12 +910,12 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
   int64_t rows = 0, kept = 0;
   if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
 
-  std::lock_guard<std::mutex> lk(c->mu);
+  SlotLease S(c);
   hipSetDevice(c->device);
   const size_t nwords = (size_t)(t->nrows / 64 + 2);
   const size_t nt = (size_t)std::max<int64_t>(d.ntiles, 1);
   void* ar;
-  if (int rc = arena_get(c, nwords * 8 + nt * 8 + 64 + 2048, &ar)) return rc;
+  if (int rc = arena_get(c, S.get(), nwords * 8 + nt * 8 + 64 + 2048, &ar)) return rc;
   Carve cv;
   uint64_t* bits = cv.take<uint64_t>((char*)ar, nwords);
   uint32_t* tcount = cv.take<uint32_t>((char*)ar, nt);
@@ -855,12 +926,16 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
   vq_selvec* sv = new (std::nothrow) vq_selvec();
   if (!sv) return fail(VQ_E_NOMEM, "exec: gpu: selvec allocation");
   sv->ctx = c;
-  if (c->sv_cache && c->sv_cache_bytes >= need) {
-    sv->rows = (uint32_t*)c->sv_cache;
-    sv->bytes = c->sv_cache_bytes;
-    c->sv_cache = nullptr;
-    c->sv_cache_bytes = 0;
-  } else {
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->sv_cache && c->sv_cache_bytes >= need) {
+      sv->rows = (uint32_t*)c->sv_cache;
+      sv->bytes = c->sv_cache_bytes;
+      c->sv_cache = nullptr;
+      c->sv_cache_bytes = 0;
+    }
+  }
+  if (!sv->rows) {
     if (!reserve(c, need)) {
       delete sv;
       return fail(VQ_E_BUDGET, "exec: gpu: selection vector of %zu bytes exceeds the budget", need);
@@ -874,22 +949,22 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
     sv->bytes = need;
   }
   void* hp;
-  if (int rc = pinned_get(c, 64, &hp)) {
+  if (int rc = pinned_get(S.get(), 64, &hp)) {
     vq_selvec_destroy(sv);
     return rc;
   }
-  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  HIP_TRY(hipEventRecord(S->ev[0], S->stream));
   const bool jit = d.ntiles > 0 && jit_available_filter(d, nullptr);
-  HIP_TRY(launch_filter(d, jit, bits, tcount, toff, sv->rows, dcount, c->stream, c->ev[1], c->ev[2],
+  HIP_TRY(launch_filter(d, jit, bits, tcount, toff, sv->rows, dcount, S->stream, S->ev[1], S->ev[2],
                         c->kernel_repeats));
-  HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-  HIP_TRY(hipMemcpyAsync(hp, dcount, 8, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipEventRecord(S->ev[3], S->stream));
+  HIP_TRY(hipMemcpyAsync(hp, dcount, 8, hipMemcpyDeviceToHost, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
   sv->count = (int64_t)*(uint64_t*)hp;
   if (stats) {
     memset(stats, 0, sizeof *stats);
-    stats->main_kernel_ms = d.ntiles ? elapsed(c->ev[1], c->ev[2]) / c->kernel_repeats : 0;
-    stats->device_ms = elapsed(c->ev[0], c->ev[3]) -
+    stats->main_kernel_ms = d.ntiles ? elapsed(S->ev[1], S->ev[2]) / c->kernel_repeats : 0;
+    stats->device_ms = elapsed(S->ev[0], S->ev[3]) -
                        (d.ntiles ? (c->kernel_repeats - 1) * stats->main_kernel_ms : 0);
     stats->rows_scanned = rows;
     stats->rg_total = t->nrg;
@@ -1015,8 +1090,10 @@ int ilog2_ceil(uint64_t x) {
   return l;
 }
 
-// Identity of a join's build side for the partition-count estimate: the
-// table, the key column and the predicate's terms, literals included.
+// Identity of a join side for the partition-count estimate: the table, the
+// key column and the predicate's columns and comparisons (not its literals,
+// which a served stream varies from query to query: the last query of the
+// same shape is the estimate, and a partition that overflows is retried).
 std::string build_est_key(const vq_table* t, int32_t key, const vq_pred* pred) {
   char buf[96];
   snprintf(buf, sizeof buf, "%p/%d", (const void*)t, key);
@@ -1024,8 +1101,7 @@ std::string build_est_key(const vq_table* t, int32_t key, const vq_pred* pred) {
   if (pred)
     for (int i = 0; i < pred->nterms && i < VQ_MAX_TERMS; i++) {
       const vq_term& m = pred->terms[i];
-      snprintf(buf, sizeof buf, "|%d,%d,%lld,%llx", m.col, m.op, (long long)m.ival,
-               (unsigned long long)f64_to_bits(m.fval));
+      snprintf(buf, sizeof buf, "|%d,%d", m.col, m.op);
       k += buf;
     }
   return k;
@@ -1052,12 +1128,18 @@ int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg*
 
 }  // namespace
 
-extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
-                           const vq_pred* build_pred, const vq_table* probe, int32_t probe_key,
-                           const vq_pred* probe_pred, int32_t naggs, const vq_agg* aggs,
-                           int32_t radix_bits, int32_t topk, vq_agg_result* out, vq_stats* stats) {
-  if (!c || !build || !probe || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
-  if (build->ctx != c || probe->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+namespace {
+
+// The join + aggregate pipeline; with groupby set there is no build side and
+// the probe side is grouped by its key column (every key enters the
+// partition tables as its rows arrive).
+int join_impl(vq_ctx* c, const vq_table* build, int32_t build_key, const vq_pred* build_pred,
+              const vq_table* probe, int32_t probe_key, const vq_pred* probe_pred, int32_t naggs,
+              const vq_agg* aggs, int32_t radix_bits, int32_t topk, vq_agg_result* out,
+              vq_stats* stats, bool groupby) {
+  if (!c || (!build && !groupby) || !probe || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if ((!groupby && build->ctx != c) || probe->ctx != c)
+    return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
   if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs)) return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
   if (topk < 0) return fail(VQ_E_ARG, "exec: gpu: top-k of %d", topk);
   if (radix_bits > 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d radix bits", radix_bits);
@@ -1066,7 +1148,9 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
   jd.naggs = naggs;
   JSide bs, ps;
   SlotMap bsm, psm;
-  if (int rc = plan_side(build, build_key, build_pred, nullptr, 0, bs, nullptr, bsm)) return rc;
+  memset(&bs, 0, sizeof bs);
+  if (!groupby)
+    if (int rc = plan_side(build, build_key, build_pred, nullptr, 0, bs, nullptr, bsm)) return rc;
   if (int rc = plan_side(probe, probe_key, probe_pred, aggs, naggs, ps, &jd, psm)) return rc;
   // accumulator words and fixed-point scales
   const int L = ilog2_ceil((uint64_t)probe->nrows + 1);
@@ -1113,8 +1197,9 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
   // run: then the rows that passed last time.  Partitions are sized so the
   // estimate fills at most 3/4 of a table.  A table that still fills up
   // reports an overflow and the query is repartitioned one bit finer.
-  const std::string est_key = build_est_key(build, build_key, build_pred);
-  int64_t est = std::max<int64_t>(build->nrows, 1);
+  const std::string est_key = groupby ? "G|" + build_est_key(probe, probe_key, probe_pred)
+                                      : build_est_key(build, build_key, build_pred);
+  int64_t est = std::max<int64_t>(groupby ? probe->nrows : build->nrows, 1);
   {
     std::lock_guard<std::mutex> lk(c->mu);
     auto it = c->build_est.find(est_key);
@@ -1139,7 +1224,8 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     bits = 0;
     while (bits < 16 && (double)est / (1 << bits) > kLoad * cap_max) bits++;
   }
-  const std::string grp_key = est_key + "#" + build_est_key(probe, probe_key, probe_pred);
+  const std::string grp_key =
+      groupby ? est_key : est_key + "#" + build_est_key(probe, probe_key, probe_pred);
   int64_t groups_seen = -1;
   {
     std::lock_guard<std::mutex> lk(c->mu);
@@ -1149,7 +1235,7 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
   const bool dev_topk = topk > 0 && topk <= 16;
   int32_t pool_req = 0;  // 0: from the estimate
 
-  std::lock_guard<std::mutex> lk(c->mu);
+  SlotLease S(c);
   hipSetDevice(c->device);
   for (;;) {
     JoinPlan plan;
@@ -1169,6 +1255,7 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     plan.pool = std::max<int32_t>(64, std::min(pool, pool_max));
     plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, nw);
     plan.lds_max = lds_max;
+    plan.groupby = groupby ? 1 : 0;
     plan.topk = dev_topk ? topk : 0;
     plan.nvals = nvals;
     const int P = plan.P;
@@ -1176,16 +1263,18 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     plan.pbuf = part_buffer_entries(P, 8 + 4 + 8 * nvals, lds_dev);
 
     std::vector<uint8_t> bkeep, pkeep;
-    prune(build, build_pred, bkeep);
-    prune(probe, probe_pred, pkeep);
-    bs.tile_rows = join_tile_rows(kept_rows(build, bkeep));
-    ps.tile_rows = join_tile_rows(kept_rows(probe, pkeep));
     int64_t brows = 0, bkept = 0, prows = 0, pkept = 0;
-    if (int rc = make_spans(build, bkeep, bs.tile_rows, bs.span, &bs.nspans, &bs.ntiles, &brows, &bkept)) return rc;
+    if (!groupby) {
+      prune(build, build_pred, bkeep);
+      bs.tile_rows = join_tile_rows(kept_rows(build, bkeep));
+      if (int rc = make_spans(build, bkeep, bs.tile_rows, bs.span, &bs.nspans, &bs.ntiles, &brows, &bkept)) return rc;
+    }
+    prune(probe, probe_pred, pkeep);
+    ps.tile_rows = join_tile_rows(kept_rows(probe, pkeep));
     if (int rc = make_spans(probe, pkeep, ps.tile_rows, ps.span, &ps.nspans, &ps.ntiles, &prows, &pkept)) return rc;
 
     const int RW = 3 + naggs;
-    const int64_t rec_cap = std::max<int64_t>(1, std::min(brows, prows));
+    const int64_t rec_cap = std::max<int64_t>(1, groupby ? prows : std::min(brows, prows));
     JoinBufs B;
     memset(&B, 0, sizeof B);
     const int64_t cap_b = std::max<int64_t>(brows, 1);
@@ -1193,10 +1282,10 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     if (cap_b >= (int64_t)INT32_MAX || cap_p >= (int64_t)UINT32_MAX)
       return fail(VQ_E_UNSUPPORTED, "exec: gpu: join side above 2^31 build / 2^32 probe rows");
     PartLaunch bl, pl;
-    if (jit_part_prepare(bs, nullptr, P, plan.bbuf, &bl, &jerr_) ||
+    if ((!groupby && jit_part_prepare(bs, nullptr, P, plan.bbuf, &bl, &jerr_)) ||
         jit_part_prepare(ps, &jd, P, plan.pbuf, &pl, &jerr_))
       return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr_.c_str());
-    B.b.G = bl.grid;
+    B.b.G = groupby ? 1 : bl.grid;
     B.p.G = pl.grid;
     size_t bytes = 0;
     bytes += (size_t)cap_b * 8 + (size_t)(P + 1) * 4 + (size_t)B.b.G * P * 4;
@@ -1206,7 +1295,7 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     bytes += dev_topk ? (size_t)P * topk * (RW * 8 + 16) + (size_t)16 * RW * 8 : (size_t)rec_cap * RW * 8;
     bytes += 16 * 256;  // alignment of the carved pieces
     void* ar;
-    if (int rc = arena_get(c, bytes, &ar)) return rc;
+    if (int rc = arena_get(c, S.get(), bytes, &ar)) return rc;
     Carve cv;
     char* base = (char*)ar;
     B.flags = cv.take<uint32_t>(base, 16);
@@ -1231,12 +1320,12 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
       B.recs = cv.take<uint64_t>(base, (size_t)rec_cap * RW);
     }
 
-    HIP_TRY(launch_join(bl, pl, jd, plan, B, c->stream, c->ev));
+    HIP_TRY(launch_join(bl, pl, jd, plan, B, S->stream, S->ev));
     void* hp;
-    if (int rc = pinned_get(c, 64, &hp)) return rc;
+    if (int rc = pinned_get(S.get(), 64, &hp)) return rc;
     uint32_t* hflags = (uint32_t*)hp;
-    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 32, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 32, hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(hipStreamSynchronize(S->stream));
     if (hflags[5]) return fail(VQ_E_DEVICE, "exec: gpu: partition scatter wrote past a partition's end");
     if (hflags[0]) {
       // a partition's accumulator pool filled: a larger pool if the LDS has
@@ -1250,8 +1339,11 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
       pool_req = 0;
       continue;
     }
-    c->build_est[est_key] = hflags[6];
-    c->build_est[grp_key] = hflags[1];
+    {
+      std::lock_guard<std::mutex> lk(c->mu);
+      c->build_est[est_key] = groupby ? hflags[2] : hflags[6];  // keys / passing build rows
+      c->build_est[grp_key] = hflags[1];
+    }
     const uint32_t ngroups_all = hflags[1];
     const uint32_t nrec = dev_topk ? hflags[3] : ngroups_all;
     if (nrec > (uint64_t)rec_cap) return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)rec_cap);
@@ -1259,13 +1351,14 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     if (nrec) HIP_TRY(hipMemcpy(hrec.data(), dev_topk ? B.out : B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
     if (stats) {
       memset(stats, 0, sizeof *stats);
-      stats->device_ms = elapsed(c->ev[0], c->ev[3]);
-      stats->main_kernel_ms = elapsed(c->ev[1], c->ev[2]);
+      stats->device_ms = elapsed(S->ev[0], S->ev[3]);
+      stats->main_kernel_ms = elapsed(S->ev[1], S->ev[2]);
       stats->rows_scanned = brows + prows;
-      stats->rg_total = build->nrg + probe->nrg;
+      stats->rg_total = (groupby ? 0 : build->nrg) + probe->nrg;
       stats->rg_scanned = bkept + pkept;
-      stats->bytes_alg = brows * slot_bytes_per_row(build, bsm) + prows * slot_bytes_per_row(probe, psm);
-      stats->build_rows = hflags[6];
+      stats->bytes_alg = (groupby ? 0 : brows * slot_bytes_per_row(build, bsm)) +
+                         prows * slot_bytes_per_row(probe, psm);
+      stats->build_rows = groupby ? 0 : hflags[6];
       stats->build_keys = hflags[2];
       stats->partitions = P;
       stats->radix_bits = bits;
@@ -1308,4 +1401,216 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
     return VQ_OK;
   }
 
+}
+
+}  // namespace
+
+extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
+                           const vq_pred* build_pred, const vq_table* probe, int32_t probe_key,
+                           const vq_pred* probe_pred, int32_t naggs, const vq_agg* aggs,
+                           int32_t radix_bits, int32_t topk, vq_agg_result* out, vq_stats* stats) {
+  return join_impl(c, build, build_key, build_pred, probe, probe_key, probe_pred, naggs, aggs,
+                   radix_bits, topk, out, stats, false);
+}
+
+extern "C" int vq_agg_hash(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t key_col,
+                           int32_t naggs, const vq_agg* aggs, vq_agg_result* out, vq_stats* stats) {
+  return join_impl(c, nullptr, 0, nullptr, t, key_col, pred, naggs, aggs, -1, 0, out, stats, true);
+}
+
+// ===========================================================================
+// concurrent and asynchronous queries
+// ===========================================================================
+struct Ticket {
+  vq_ctx* c = nullptr;
+  vq_query q;
+  vq_pred pred, build_pred;
+  bool has_pred = false, has_build_pred = false;
+  std::vector<int32_t> key_cols;
+  std::vector<vq_agg> aggs;
+  vq_agg_result* out = nullptr;
+  vq_stats* stats = nullptr;
+  uint64_t est = 0;  // device bytes the query is estimated to need
+  std::mutex mu;
+  std::condition_variable cv;
+  bool done = false;
+  int status = VQ_OK;
+  std::string err;
+};
+
+struct vq_ticket {
+  Ticket* t;
+};
+
+namespace {
+
+// Device memory of one query, from its tables' sizes: the join's partitioned
+// copies of both sides (keys, probe row ids and values) and its per-partition
+// bookkeeping; the aggregate's per-workgroup partials.
+uint64_t estimate_bytes(const Ticket& k) {
+  const vq_query& q = k.q;
+  if (q.kind == VQ_Q_JOIN_AGG) {
+    const uint64_t b = (uint64_t)std::max<int64_t>(q.build->nrows, 1);
+    const uint64_t p = (uint64_t)std::max<int64_t>(q.table->nrows, 1);
+    const uint64_t nv = (uint64_t)std::max(q.naggs, 1);
+    return b * 8 + p * (12 + 8 * nv) + (16ull << 20);
+  }
+  return 8ull << 20;
+}
+
+void run_ticket(Ticket* k) {
+  const vq_query& q = k->q;
+  int rc;
+  if (q.kind == VQ_Q_AGG_DICT)
+    rc = vq_agg_dict(k->c, q.table, k->has_pred ? &k->pred : nullptr, q.nkeys,
+                     k->key_cols.empty() ? nullptr : k->key_cols.data(), q.naggs,
+                     k->aggs.empty() ? nullptr : k->aggs.data(), k->out, k->stats);
+  else
+    rc = vq_join_agg(k->c, q.build, q.build_key, k->has_build_pred ? &k->build_pred : nullptr,
+                     q.table, q.probe_key, k->has_pred ? &k->pred : nullptr, q.naggs,
+                     k->aggs.empty() ? nullptr : k->aggs.data(), q.radix_bits, q.topk, k->out,
+                     k->stats);
+  std::lock_guard<std::mutex> lk(k->mu);
+  k->status = rc;
+  if (rc) k->err = g_err;
+  k->done = true;
+  k->cv.notify_all();
+}
+
+void worker_loop(vq_ctx* c) {
+  hipSetDevice(c->device);
+  for (;;) {
+    Ticket* k = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(c->qmu);
+      c->qcv.wait(lk, [&] {
+        if (c->stopping) return true;
+        if (c->queue.empty()) return false;
+        const Ticket* h = c->queue.front();
+        return c->running == 0 || c->serve_reserved + h->est <= c->serve_budget;
+      });
+      if (c->stopping && c->queue.empty()) return;
+      if (c->queue.empty()) continue;
+      k = c->queue.front();
+      c->queue.pop_front();
+      c->serve_reserved += k->est;
+      c->running++;
+    }
+    run_ticket(k);
+    {
+      std::lock_guard<std::mutex> lk(c->qmu);
+      c->serve_reserved -= k->est;
+      c->running--;
+    }
+    c->qcv.notify_all();
+  }
+}
+
+void stop_workers(vq_ctx* c) {
+  {
+    std::lock_guard<std::mutex> lk(c->qmu);
+    c->stopping = true;
+  }
+  c->qcv.notify_all();
+  for (auto& w : c->workers) w.join();
+  c->workers.clear();
+  c->stopping = false;
+}
+
+}  // namespace
+
+extern "C" int vq_ctx_set_slots(vq_ctx* c, int32_t nslots, uint64_t serve_budget) {
+  if (!c || nslots < 1 || nslots > 16) return fail(VQ_E_ARG, "exec: gpu: %d query slots", nslots);
+  {
+    std::lock_guard<std::mutex> lk(c->qmu);
+    if (c->running || !c->queue.empty()) return fail(VQ_E_ARG, "exec: gpu: queries in flight");
+  }
+  stop_workers(c);
+  hipSetDevice(c->device);
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->idle.size() != c->slots.size()) return fail(VQ_E_ARG, "exec: gpu: queries in flight");
+  while ((int32_t)c->slots.size() < nslots) {
+    Slot* S = nullptr;
+    if (int rc = slot_create(&S)) return rc;
+    c->slots.push_back(S);
+    c->idle.push_back(S);
+  }
+  while ((int32_t)c->slots.size() > nslots) {
+    Slot* S = c->slots.back();
+    c->slots.pop_back();
+    c->idle.erase(std::find(c->idle.begin(), c->idle.end(), S));
+    slot_destroy(c, S);
+  }
+  c->serve_budget = serve_budget ? serve_budget : c->budget / 2;
+  return VQ_OK;
+}
+
+extern "C" int vq_submit(vq_ctx* c, const vq_query* q, vq_agg_result* out, vq_stats* stats,
+                         vq_ticket** ticket) {
+  if (!c || !q || !out || !ticket) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  *ticket = nullptr;
+  if (q->kind != VQ_Q_AGG_DICT && q->kind != VQ_Q_JOIN_AGG)
+    return fail(VQ_E_ARG, "exec: gpu: query kind %d", q->kind);
+  if (!q->table || (q->kind == VQ_Q_JOIN_AGG && !q->build))
+    return fail(VQ_E_ARG, "exec: gpu: query without its table");
+  if (q->naggs < 0 || q->naggs > VQ_MAX_AGGS || (q->naggs > 0 && !q->aggs))
+    return fail(VQ_E_ARG, "exec: gpu: %d aggregates", q->naggs);
+  if (q->kind == VQ_Q_AGG_DICT && (q->nkeys < 0 || q->nkeys > 4 || (q->nkeys > 0 && !q->key_cols)))
+    return fail(VQ_E_ARG, "exec: gpu: %d group keys", q->nkeys);
+  Ticket* k = new (std::nothrow) Ticket();
+  vq_ticket* h = new (std::nothrow) vq_ticket();
+  if (!k || !h) {
+    delete k;
+    delete h;
+    return fail(VQ_E_NOMEM, "exec: gpu: ticket allocation");
+  }
+  k->c = c;
+  k->q = *q;
+  if (q->pred) {
+    k->pred = *q->pred;
+    k->has_pred = true;
+  }
+  if (q->build_pred) {
+    k->build_pred = *q->build_pred;
+    k->has_build_pred = true;
+  }
+  if (q->kind == VQ_Q_AGG_DICT && q->nkeys > 0) k->key_cols.assign(q->key_cols, q->key_cols + q->nkeys);
+  if (q->naggs > 0) k->aggs.assign(q->aggs, q->aggs + q->naggs);
+  k->out = out;
+  k->stats = stats;
+  k->est = estimate_bytes(*k);
+  h->t = k;
+  {
+    std::lock_guard<std::mutex> lk(c->qmu);
+    if (c->serve_budget == 0) c->serve_budget = c->budget / 2;
+    if (c->workers.empty()) {
+      const size_t n = c->slots.size();
+      for (size_t i = 0; i < n; i++) c->workers.emplace_back(worker_loop, c);
+    }
+    c->queue.push_back(k);
+  }
+  c->qcv.notify_all();
+  *ticket = h;
+  return VQ_OK;
+}
+
+extern "C" int vq_ticket_done(const vq_ticket* h) {
+  if (!h || !h->t) return 1;
+  std::lock_guard<std::mutex> lk(h->t->mu);
+  return h->t->done ? 1 : 0;
+}
+
+extern "C" int vq_wait(vq_ticket* h) {
+  if (!h || !h->t) return fail(VQ_E_ARG, "exec: gpu: NULL ticket");
+  Ticket* k = h->t;
+  int rc;
+  {
+    std::unique_lock<std::mutex> lk(k->mu);
+    k->cv.wait(lk, [&] { return k->done; });
+    rc = k->status;
+    if (rc) g_err = k->err;
+  }
+  delete k;
+  delete h;
+  return rc;
 }

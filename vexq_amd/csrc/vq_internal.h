@@ -213,6 +213,8 @@ struct JoinPlan {
   int32_t bbuf;         // build scatter: LDS buffer entries per partition (0: none)
   int32_t pbuf;         // probe scatter: the same
   int32_t lds_bytes;    // the join kernel's dynamic LDS: join_lds_for(cap, pool, nwords)
+  int32_t groupby;      // no build side: GROUP BY the probe side's key
+  int32_t pad_;
 };
 
 // LDS bytes of one partition's table and accumulator pool.

@@ -222,6 +222,82 @@ def lineitem_probe(build: Table, nrows: int, seed: int = 42) -> Table:
     return Table(cols, row_groups(nrows))
 
 
+def fact_serving(dim: Table, nrows: int, seed: int = 42) -> Table:
+    """The served stream's fact table: lineitem's dictionary and measure
+    columns plus the join probe side's foreign key (each dimension key 1..7
+    times, shuffled) with shipdate = orderdate + U{1..121}."""
+    p = lineitem_probe(dim, nrows, seed)
+    li = lineitem(nrows, seed)
+    cols = [li.columns[li.index("flagA")], li.columns[li.index("flagB")],
+            p.columns[p.index("quantity")], p.columns[p.index("price")],
+            p.columns[p.index("discount")], li.columns[li.index("tax")],
+            p.columns[p.index("shipdate")], p.columns[p.index("fk")]]
+    return Table(cols, row_groups(nrows))
+
+
+def _ranks(seed, stream, n, groups, skew):
+    """n draws of a rank in [0, groups): uniform (skew 0) or Zipf(skew)
+    truncated to `groups` ranks (inverse CDF over the rank weights)."""
+    u = (rand_u64(seed, stream, n) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    if skew <= 0:
+        return np.minimum((u * groups).astype(np.int64), groups - 1)
+    w = 1.0 / np.arange(1, groups + 1, dtype=np.float64) ** skew
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    return np.minimum(np.searchsorted(cdf, u, side="right"), groups - 1).astype(np.int64)
+
+
+def keyed_table(nrows: int, groups: int, key_type: int = abi.VQ_I64, skew: float = 0.0,
+                seed: int = 42) -> Table:
+    """A GROUP BY instance of the batched workload: a key column with
+    `groups` distinct values (int64 keys spread over the int64 range, or u16
+    codes), uniform or Zipf-skewed, and config 0's measures."""
+    n = nrows
+    rank = _ranks(seed, 30, n, groups, skew)
+    if key_type == abi.VQ_U16:
+        if groups > 65536:
+            raise ValueError("u16 keys hold at most 65536 groups")
+        perm = np.argsort(rand_u64(seed, 31, groups), kind="stable")
+        key = perm[rank].astype(np.uint16)
+    else:
+        key = _mix(rank.astype(np.uint64) + np.uint64(seed * 7919)).view(np.int64)
+    quantity = rand_int(seed, 32, n, 1, 50)
+    price = _price(seed, 33, quantity, n)
+    discount = rand_int(seed, 34, n, 0, 10).astype(np.float64) / 100.0
+    shipdate = rand_int(seed, 35, n, DATE_MIN, DATE_MIN + SHIP_DAYS - 1)
+    cols = [
+        _col("key", key_type, key),
+        _col("quantity", abi.VQ_I64, quantity),
+        _col("price", abi.VQ_F64, price),
+        _col("discount", abi.VQ_F64, discount),
+        _col("shipdate", abi.VQ_DATE, shipdate),
+    ]
+    return Table(cols, row_groups(n))
+
+
+def join_instance(nbuild: int, ratio: float, skew: float = 0.0, seed: int = 42):
+    """A join instance of the batched workload: a build side of unique keys
+    with an orderdate, and a probe side of ratio x nbuild rows whose foreign
+    keys are drawn from the build keys (uniform or Zipf-skewed), with
+    shipdate and config 0's measures."""
+    b = orders(nbuild, seed)
+    npr = max(1, int(nbuild * ratio))
+    rank = _ranks(seed, 40, npr, nbuild, skew)
+    fk = b.col("key")[rank]
+    quantity = rand_int(seed, 41, npr, 1, 50)
+    price = _price(seed, 42, quantity, npr)
+    discount = rand_int(seed, 43, npr, 0, 10).astype(np.float64) / 100.0
+    shipdate = b.col("orderdate")[rank].astype(np.int64) + rand_int(seed, 44, npr, 1, 121)
+    p = Table([
+        _col("fk", abi.VQ_I64, fk),
+        _col("shipdate", abi.VQ_DATE, shipdate),
+        _col("quantity", abi.VQ_I64, quantity),
+        _col("price", abi.VQ_F64, price),
+        _col("discount", abi.VQ_F64, discount),
+    ], row_groups(npr))
+    return b, p
+
+
 # ---- the workload queries ---------------------------------------------------
 
 def q_groupby(t: Table):
@@ -268,3 +344,40 @@ def q_join(build: Table, probe: Table, cutoff: int = JOIN_CUTOFF):
     ppred = abi.make_pred(probe, [("shipdate", ">", cutoff)])
     aggs = [abi.agg_sum(X.col("price") * (1 - X.col("discount")), probe)]
     return bpred, ppred, aggs
+
+
+def q_groupby_cut(t: Table, cut: int):
+    """q_groupby with the shipdate cutoff `cut` (shipdate <= cut)."""
+    pred, keys, aggs = q_groupby(t)
+    pred = abi.make_pred(t, [("shipdate", "<=", int(cut))])
+    return pred, keys, aggs
+
+
+def serve_stream(fact: Table, dim: Table, nq: int, seed: int = 7):
+    """A served query mix: 50% range scans with random windows (q_scan_sum),
+    30% GROUP BYs with random shipdate cutoffs, 20% joins with random date
+    cutoffs.  Returns [(kind, args)] with kind in scan/groupby/join."""
+    r = rand_u64(seed, 200, 3 * nq)
+    out = []
+    for i in range(nq):
+        u = int(r[3 * i] % np.uint64(100))
+        if u < 50:
+            out.append(("scan", q_scan_sum(fact, seed, which=1000 + i)))
+        elif u < 80:
+            cut = DATE_MIN + SHIP_DAYS - 1 - int(r[3 * i + 1] % np.uint64(366))
+            out.append(("groupby", q_groupby_cut(fact, cut)))
+        else:
+            cutoff = DATE_MIN + 900 + int(r[3 * i + 2] % np.uint64(501))
+            out.append(("join", q_join(dim, fact, cutoff)))
+    return out
+
+
+def q_keyed(t: Table, cut: int = DATE_MIN + SHIP_DAYS - 1 - 90):
+    """GROUP BY key: SUM(price*(1-discount)), SUM(quantity), AVG(price),
+    COUNT(*), MIN(price), MAX(quantity) WHERE shipdate <= cut."""
+    X = abi.X
+    pred = abi.make_pred(t, [("shipdate", "<=", int(cut))])
+    aggs = [abi.agg_sum(X.col("price") * (1 - X.col("discount")), t), abi.agg_sum("quantity", t),
+            abi.agg_avg("price", t), abi.count_star(t), abi.agg_min("price", t),
+            abi.agg_max("quantity", t)]
+    return pred, t.index("key"), aggs

@@ -36,6 +36,11 @@ def lib():
     L.vq_ctx_create.argtypes = [C.c_int, C.c_uint64, P(C.c_void_p)]
     L.vq_ctx_destroy.argtypes = [C.c_void_p]
     L.vq_ctx_set_kernel_repeats.argtypes = [C.c_void_p, C.c_int32]
+    L.vq_ctx_set_slots.argtypes = [C.c_void_p, C.c_int32, C.c_uint64]
+    L.vq_submit.argtypes = [C.c_void_p, P(abi.Query), P(abi.AggResult), P(abi.Stats),
+                            P(C.c_void_p)]
+    L.vq_ticket_done.argtypes = [C.c_void_p]
+    L.vq_wait.argtypes = [C.c_void_p]
     L.vq_ctx_bytes_in_use.argtypes = [C.c_void_p]
     L.vq_ctx_bytes_in_use.restype = C.c_uint64
     L.vq_table_upload.argtypes = [C.c_void_p, C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32,
@@ -55,6 +60,8 @@ def lib():
     L.vq_join_agg.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_void_p,
                               C.c_int32, P(abi.Pred), C.c_int32, P(abi.Agg), C.c_int32,
                               C.c_int32, P(abi.AggResult), P(abi.Stats)]
+    L.vq_agg_hash.argtypes = [C.c_void_p, C.c_void_p, P(abi.Pred), C.c_int32, C.c_int32,
+                              P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
     L.vq_radix_bits_for.argtypes = [C.c_int64]
     L.vq_radix_bits_for.restype = C.c_int32
     L.vq_radix_hash.argtypes = [C.c_int64]
@@ -105,6 +112,56 @@ class Context:
     def upload(self, table, zones=None) -> "DeviceTable":
         return DeviceTable(self, table, zones)
 
+    def set_slots(self, nslots: int, serve_budget: int = 0) -> None:
+        """nslots concurrent query slots; vq_submit admission bound in bytes."""
+        _check(lib().vq_ctx_set_slots(self.h, nslots, serve_budget))
+
+    # -- asynchronous queries ------------------------------------------------
+    def submit_agg_dict(self, dt: "DeviceTable", pred, key_cols, aggs,
+                        capacity: int = 4096) -> "Ticket":
+        q = abi.Query()
+        q.kind = abi.VQ_Q_AGG_DICT
+        q.table = dt.h
+        keep = [abi.agg_array(aggs), (C.c_int32 * max(1, len(key_cols)))(*key_cols)]
+        if pred is not None:
+            keep.append(pred)
+            q.pred = C.pointer(pred)
+        q.nkeys = len(key_cols)
+        q.key_cols = C.cast(keep[1], C.POINTER(C.c_int32))
+        q.naggs = len(aggs)
+        q.aggs = C.cast(keep[0], C.POINTER(abi.Agg))
+        return self._submit(q, abi.AggOut(capacity, len(key_cols), len(aggs)), keep)
+
+    def submit_join_agg(self, build: "DeviceTable", build_key: int, build_pred,
+                        probe: "DeviceTable", probe_key: int, probe_pred, aggs,
+                        radix_bits: int = -1, topk: int = 0,
+                        capacity: int | None = None) -> "Ticket":
+        q = abi.Query()
+        q.kind = abi.VQ_Q_JOIN_AGG
+        q.build = build.h
+        q.table = probe.h
+        keep = [abi.agg_array(aggs)]
+        if build_pred is not None:
+            keep.append(build_pred)
+            q.build_pred = C.pointer(build_pred)
+        if probe_pred is not None:
+            keep.append(probe_pred)
+            q.pred = C.pointer(probe_pred)
+        q.build_key, q.probe_key = build_key, probe_key
+        q.naggs = len(aggs)
+        q.aggs = C.cast(keep[0], C.POINTER(abi.Agg))
+        q.radix_bits, q.topk = radix_bits, topk
+        if capacity is None:
+            capacity = topk if topk > 0 else max(1, min(build.nrows, probe.nrows))
+        return self._submit(q, abi.AggOut(capacity, 1, len(aggs)), keep)
+
+    def _submit(self, q, out, keep) -> "Ticket":
+        st = abi.Stats()
+        h = C.c_void_p()
+        _check(lib().vq_submit(self.h, C.byref(q), C.byref(out.c), C.byref(st), C.byref(h)))
+        out.stats = st
+        return Ticket(h, out, keep)
+
     # -- operators ----------------------------------------------------------
     def filter(self, dt: "DeviceTable", pred) -> np.ndarray:
         sv = C.c_void_p()
@@ -139,6 +196,19 @@ class Context:
         out.stats = st
         return out
 
+    def agg_hash(self, dt: "DeviceTable", pred, key_col: int, aggs,
+                 capacity: int | None = None) -> abi.AggOut:
+        """GROUP BY one integer column of any cardinality."""
+        arr = abi.agg_array(aggs)
+        if capacity is None:
+            capacity = max(1, dt.nrows)
+        out = abi.AggOut(capacity, 1, len(aggs))
+        st = abi.Stats()
+        _check(lib().vq_agg_hash(self.h, dt.h, _pp(pred), key_col, len(aggs), arr,
+                                 C.byref(out.c), C.byref(st)))
+        out.stats = st
+        return out
+
     def join_agg(self, build: "DeviceTable", build_key: int, build_pred, probe: "DeviceTable",
                  probe_key: int, probe_pred, aggs, radix_bits: int = -1, topk: int = 0,
                  capacity: int | None = None) -> abi.AggOut:
@@ -152,6 +222,22 @@ class Context:
                                  C.byref(out.c), C.byref(st)))
         out.stats = st
         return out
+
+
+class Ticket:
+    """A submitted query: wait() returns its result (raises on failure)."""
+
+    def __init__(self, h, out, keep):
+        self.h, self.out, self._keep = h, out, keep
+
+    def done(self) -> bool:
+        return self.h is None or bool(lib().vq_ticket_done(self.h))
+
+    def wait(self) -> abi.AggOut:
+        if self.h is not None:
+            h, self.h = self.h, None
+            _check(lib().vq_wait(h))
+        return self.out
 
 
 class DeviceTable:
