@@ -282,6 +282,7 @@ int vq_ctx_set_slots(vq_ctx* ctx, int32_t nslots, uint64_t serve_budget);
 
 #define VQ_Q_AGG_DICT 1
 #define VQ_Q_JOIN_AGG 2
+#define VQ_Q_AGG_HASH 3 /* vq_agg_hash: table, pred, key_cols[0] (nkeys 1), naggs, aggs */
 
 /* One query for vq_submit: the arguments of vq_agg_dict (kind
  * VQ_Q_AGG_DICT: table, pred, nkeys, key_cols, naggs, aggs) or of
@@ -327,6 +328,30 @@ int vq_wait(vq_ticket* ticket);
  * order; float sums are exact (fixed point) and rounded once. */
 int vq_agg_hash(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t key_col,
                 int32_t naggs, const vq_agg* aggs, vq_agg_result* out, vq_stats* stats);
+
+/* ---- batches of independent queries ----------------------------------------
+ * No counterpart in the reference (its operators run one query per call).
+ * Runs n independent queries as one unit: every join and hash GROUP BY is
+ * planned on the host, its scratch carved from one arena, and all their
+ * launches captured into one hipGraph of eight concurrent branches (instance
+ * i on branch i mod 8), which is instantiated once and replayed while the
+ * batch's launch arguments stay the same.  Dictionary GROUP BYs
+ * (VQ_Q_AGG_DICT) run one after another after the graph.  An instance whose
+ * partition tables overflow is run again on its own.  outs[i] / stats[i]
+ * (stats may be NULL) receive instance i's result; stats[i].device_ms is the
+ * whole graph's time.  Returns the first failing instance's status.
+ *
+ * Flags: VQ_BATCH_TUNE — before planning, autotune each instance class not
+ * yet tuned in this context (radix bits around the planner's choice, the
+ * scatter's LDS buffer size, the tile size; the fastest device time wins,
+ * results are identical under every choice) and keep the choice (also
+ * appended to the file named by the VQ_TUNE_FILE environment variable,
+ * which a new context reads back); VQ_BATCH_NO_GRAPH — enqueue the same
+ * launches on the eight branch streams directly instead of through a graph. */
+#define VQ_BATCH_TUNE 1
+#define VQ_BATCH_NO_GRAPH 2
+int vq_batch_run(vq_ctx* ctx, int32_t n, const vq_query* qs, vq_agg_result* outs,
+                 vq_stats* stats, int32_t flags);
 
 /* RadixBitsFor (exec/parallel_join.go:73-82), radixHash (91-99) and
  * joinTableCapacity (exec/join_table.go:461-476), exported so host planners

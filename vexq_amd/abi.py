@@ -84,6 +84,9 @@ _OPS = {"==": VQ_EQ, "=": VQ_EQ, "!=": VQ_NE, "<>": VQ_NE, "<": VQ_LT, "<=": VQ_
 
 VQ_Q_AGG_DICT = 1
 VQ_Q_JOIN_AGG = 2
+VQ_Q_AGG_HASH = 3
+VQ_BATCH_TUNE = 1
+VQ_BATCH_NO_GRAPH = 2
 
 
 class Query(C.Structure):

@@ -31,6 +31,7 @@
 // result does not depend on which lane or workgroup got there first, nor on
 // the order of entries inside a partition.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 
@@ -666,6 +667,13 @@ int32_t part_buffer_entries(int32_t P, int32_t entry, int32_t lds_max) {
   return 0;
 }
 
+void join_allow_lds(int32_t bytes) {
+  static std::atomic<int32_t> allowed{0};
+  if (allowed.load() >= bytes) return;
+  hipFuncSetAttribute((const void*)k_join_agg, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  allowed.store(bytes);
+}
+
 hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd,
                        const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
   const int P = plan.P;
@@ -698,9 +706,7 @@ hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd
   if (e != hipSuccess) return e;
   if (ev) hipEventRecord(ev[1], s);
   const size_t lds = plan.lds_bytes;
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute((const void*)k_join_agg, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        plan.lds_max);
+  if (lds > 64 * 1024) join_allow_lds(plan.lds_max);
   hipLaunchKernelGGL(k_join_agg, dim3((unsigned)P), dim3(kJoinThreads), lds, s, jd, plan, B);
   if (ev) {
     hipEventRecord(ev[2], s);

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -67,6 +68,16 @@ struct Slot {
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   hipEvent_t ev[6] = {};
+  // batches: branch streams forked from `stream` while a graph is captured,
+  // their join events, and the instantiated graphs of recent batches
+  static constexpr int kBranches = 8;
+  hipStream_t side[kBranches] = {};
+  hipEvent_t fork = nullptr, join[kBranches] = {};
+  struct CachedGraph {
+    uint64_t h1, h2;
+    hipGraphExec_t exec;
+  };
+  std::vector<CachedGraph> graphs;
 };
 
 struct Ticket;
@@ -86,6 +97,8 @@ struct vq_ctx {
   // cached selection-vector buffer
   void* sv_cache = nullptr;
   size_t sv_cache_bytes = 0;
+  // the autotuner's choices per instance class (under mu)
+  std::unordered_map<std::string, std::string> tune;
   // asynchronous queries (vq_submit): FIFO admission against serve_budget
   std::mutex qmu;
   std::condition_variable qcv;
@@ -188,6 +201,12 @@ int slot_create(Slot** out) {
 
 void slot_destroy(vq_ctx* c, Slot* S) {
   hipStreamSynchronize(S->stream);
+  for (auto& g : S->graphs) hipGraphExecDestroy(g.exec);
+  for (int k = 0; k < Slot::kBranches; k++) {
+    if (S->side[k]) hipStreamDestroy(S->side[k]);
+    if (S->join[k]) hipEventDestroy(S->join[k]);
+  }
+  if (S->fork) hipEventDestroy(S->fork);
   if (S->arena.p) {
     hipFree(S->arena.p);
     release(c, S->arena.bytes);
@@ -200,6 +219,7 @@ void slot_destroy(vq_ctx* c, Slot* S) {
 }
 
 void stop_workers(vq_ctx* c);
+void tune_load(vq_ctx* c);
 
 // A query's hold on one slot: waits for an idle slot, gives it back at the
 // end of the scope.
@@ -493,6 +513,7 @@ extern "C" int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out) {
   }
   c->slots.push_back(S);
   c->idle.push_back(S);
+  tune_load(c);
   *out = c;
   return VQ_OK;
 }
@@ -1130,51 +1151,98 @@ int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg*
 
 namespace {
 
-// The join + aggregate pipeline; with groupby set there is no build side and
-// the probe side is grouped by its key column (every key enters the
-// partition tables as its rows arrive).
-int join_impl(vq_ctx* c, const vq_table* build, int32_t build_key, const vq_pred* build_pred,
-              const vq_table* probe, int32_t probe_key, const vq_pred* probe_pred, int32_t naggs,
-              const vq_agg* aggs, int32_t radix_bits, int32_t topk, vq_agg_result* out,
-              vq_stats* stats, bool groupby) {
-  if (!c || (!build && !groupby) || !probe || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
-  if ((!groupby && build->ctx != c) || probe->ctx != c)
-    return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
-  if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs)) return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
-  if (topk < 0) return fail(VQ_E_ARG, "exec: gpu: top-k of %d", topk);
-  if (radix_bits > 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d radix bits", radix_bits);
+// Tuning of one join / hash GROUP BY (the autotuner's choices; the results do
+// not depend on them): radix bits relative to the planner's choice, the
+// probe scatter's LDS buffer entries per partition (-1: the largest that
+// fits) and the tile size relative to the default (a power of two).
+struct JoinTune {
+  int32_t bits_delta = 0;
+  int32_t pbuf = -1;
+  int32_t tile_shift = 0;
+};
+
+constexpr int kRetry = -1;  // internal: repartition and run again
+
+// One join + aggregate (or, with groupby, one GROUP BY of the probe side's
+// key with no build side): planned on the host, bound to device and pinned
+// memory, enqueued on a stream (or captured into a graph), then turned into
+// a result.  A partition table or accumulator pool that overflows makes the
+// result step ask for another attempt with a larger pool or one more bit.
+struct JoinJob {
+  vq_ctx* c = nullptr;
+  const vq_table* build = nullptr;
+  const vq_table* probe = nullptr;
+  int32_t build_key = 0, probe_key = 0;
+  vq_pred bpred, ppred;
+  const vq_pred* bp = nullptr;
+  const vq_pred* pp = nullptr;
+  int32_t naggs = 0;
+  vq_agg aggs[VQ_MAX_AGGS];
+  int32_t radix_bits = -1, topk = 0;
+  bool groupby = false;
+  JoinTune tune;
+  // fixed by setup
   JDesc jd;
-  memset(&jd, 0, sizeof jd);
-  jd.naggs = naggs;
   JSide bs, ps;
   SlotMap bsm, psm;
-  memset(&bs, 0, sizeof bs);
-  if (!groupby)
-    if (int rc = plan_side(build, build_key, build_pred, nullptr, 0, bs, nullptr, bsm)) return rc;
-  if (int rc = plan_side(probe, probe_key, probe_pred, aggs, naggs, ps, &jd, psm)) return rc;
-  // accumulator words and fixed-point scales
-  const int L = ilog2_ceil((uint64_t)probe->nrows + 1);
   int nw = 0, nvals = 0;
-  for (int j = 0; j < naggs; j++) {
-    if (int rc = check_agg(aggs[j])) return rc;
-    JAgg& a = jd.agg[j];
-    a.kind = aggs[j].kind;
-    a.accum = aggs[j].accum;
+  std::string est_key, grp_key;
+  int64_t est = 1, groups_seen = -1;
+  int lds_dev = 0, lds_max = 0;
+  int32_t cap_max = 4096;
+  bool dev_topk = false;
+  int bits = 0;
+  int32_t pool_req = 0;
+  // per attempt
+  JoinPlan plan;
+  int32_t pool_max = 0;
+  PartLaunch bl, pl;
+  JoinBufs B;
+  int64_t brows = 0, bkept = 0, prows = 0, pkept = 0, cap_b = 1, cap_p = 1, rec_cap = 1;
+  int RW = 3;
+  size_t bytes = 0;        // device scratch
+  size_t host_bytes = 0;   // pinned result staging
+  uint32_t* hflags = nullptr;
+  uint64_t* hout = nullptr;
+};
+
+constexpr double kLoad = 0.4;
+
+int join_setup(JoinJob& j) {
+  vq_ctx* c = j.c;
+  if (!c || (!j.build && !j.groupby) || !j.probe) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if ((!j.groupby && j.build->ctx != c) || j.probe->ctx != c)
+    return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  if (j.naggs < 0 || j.naggs > VQ_MAX_AGGS) return fail(VQ_E_ARG, "exec: gpu: %d aggregates", j.naggs);
+  if (j.topk < 0) return fail(VQ_E_ARG, "exec: gpu: top-k of %d", j.topk);
+  if (j.radix_bits > 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d radix bits", j.radix_bits);
+  memset(&j.jd, 0, sizeof j.jd);
+  j.jd.naggs = j.naggs;
+  memset(&j.bs, 0, sizeof j.bs);
+  if (!j.groupby)
+    if (int rc = plan_side(j.build, j.build_key, j.bp, nullptr, 0, j.bs, nullptr, j.bsm)) return rc;
+  if (int rc = plan_side(j.probe, j.probe_key, j.pp, j.aggs, j.naggs, j.ps, &j.jd, j.psm)) return rc;
+  // accumulator words and fixed-point scales
+  const int L = ilog2_ceil((uint64_t)j.probe->nrows + 1);
+  int nw = 0, nvals = 0;
+  for (int k = 0; k < j.naggs; k++) {
+    if (int rc = check_agg(j.aggs[k])) return rc;
+    JAgg& a = j.jd.agg[k];
+    a.kind = j.aggs[k].kind;
+    a.accum = j.aggs[k].accum;
     a.word0 = nw;
     if (a.kind == VQ_AGG_COUNT) {
       a.nwords = 0;
       continue;
     }
     nvals++;
-    if ((a.kind == VQ_AGG_SUM || a.kind == VQ_AGG_AVG) && aggs[j].arg.is_float) {
-      Interval iv = expr_interval(probe, aggs[j].arg);
+    if ((a.kind == VQ_AGG_SUM || a.kind == VQ_AGG_AVG) && j.aggs[k].arg.is_float) {
+      Interval iv = expr_interval(j.probe, j.aggs[k].arg);
       if (!iv.ok)
-        return fail(VQ_E_UNSUPPORTED, "exec: gpu: aggregate %d has no finite bound for exact summation", j);
+        return fail(VQ_E_UNSUPPORTED, "exec: gpu: aggregate %d has no finite bound for exact summation", k);
       double m = std::max(std::fabs(iv.lo), std::fabs(iv.hi)) * (1 + 1e-9);
       int E = 0;
-      if (m > 0) {
-        std::frexp(m, &E);  // m < 2^E
-      }
+      if (m > 0) std::frexp(m, &E);  // m < 2^E
       const int w = std::min(40, 62 - L);
       if (w < 20) return fail(VQ_E_UNSUPPORTED, "exec: gpu: probe side too large for exact sums");
       a.fs.w = w;
@@ -1185,222 +1253,302 @@ int join_impl(vq_ctx* c, const vq_table* build, int32_t build_key, const vq_pred
     }
     nw += a.nwords;
   }
-  jd.nwords = nw;
-  std::string jerr_;
-
+  j.jd.nwords = nw;
+  j.nw = nw;
+  j.nvals = nvals;
   // Partitioning plan.  Each partition's table lives in one workgroup's LDS:
-  // cap slots of key, multiplicity and pool index, plus a pool of accumulator
-  // entries (count, first row, accumulator words) for the keys a probe row
-  // matches.  The build-side estimate is the scan's row count before
-  // filtering, as buildSideRadixOptions uses (planner/parallel_join.go:
-  // 327-339), until a query with the same build table, key and predicate has
-  // run: then the rows that passed last time.  Partitions are sized so the
-  // estimate fills at most 3/4 of a table.  A table that still fills up
-  // reports an overflow and the query is repartitioned one bit finer.
-  const std::string est_key = groupby ? "G|" + build_est_key(probe, probe_key, probe_pred)
-                                      : build_est_key(build, build_key, build_pred);
-  int64_t est = std::max<int64_t>(groupby ? probe->nrows : build->nrows, 1);
+  // cap slots of key and tag (multiplicity, then pool index), plus a pool of
+  // accumulator entries (count, first row, multiplicity, accumulator words)
+  // for the keys a probe row matches.  The build-side estimate is the scan's
+  // row count before filtering, as buildSideRadixOptions uses
+  // (planner/parallel_join.go:327-339), until a query of the same shape over
+  // the same build side has run: then the rows that passed last time (for a
+  // GROUP BY, the distinct keys).  A table at most 4096 slots filled to at
+  // most kLoad keeps linear probing short for every lane of a wave (a wave
+  // waits for its longest probe).  The pool starts at half the table, or at
+  // twice the matched keys per partition of the last query of the same shape,
+  // and doubles when it fills; a table that fills adds one bit.
+  j.est_key = j.groupby ? "G|" + build_est_key(j.probe, j.probe_key, j.pp)
+                        : build_est_key(j.build, j.build_key, j.bp);
+  j.grp_key = j.groupby ? j.est_key : j.est_key + "#" + build_est_key(j.probe, j.probe_key, j.pp);
+  j.est = std::max<int64_t>(j.groupby ? j.probe->nrows : j.build->nrows, 1);
   {
     std::lock_guard<std::mutex> lk(c->mu);
-    auto it = c->build_est.find(est_key);
-    if (it != c->build_est.end()) est = std::max<int64_t>(it->second, 1);
+    auto it = c->build_est.find(j.est_key);
+    if (it != c->build_est.end()) j.est = std::max<int64_t>(it->second, 1);
+    it = c->build_est.find(j.grp_key);
+    if (it != c->build_est.end()) j.groups_seen = it->second;
   }
-  // the largest table (<= 4096 slots) whose pool-of-half-as-many entries fits
-  // the device's per-workgroup LDS
-  int lds_dev = 0;
-  hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-  if (lds_dev <= 0) lds_dev = 64 * 1024;
-  const int lds_max = lds_dev - 8192;  // the join kernel's own static LDS (about 4 KB)
-  // Table size: at most 4096 slots, filled to at most kLoad so that linear
-  // probing stays short for every lane of a wave (a wave waits for its
-  // longest probe); the pool of accumulators starts at half the table, or at
-  // twice the matched keys per partition of the last query over the same
-  // build and probe sides, and doubles when it fills.
-  constexpr double kLoad = 0.4;
-  int32_t cap_max = 4096;
-  while (cap_max > 8 && join_lds_for(cap_max, 64, nw) > (size_t)lds_max) cap_max /= 2;
-  int bits = radix_bits;
-  if (bits < 0) {
-    bits = 0;
-    while (bits < 16 && (double)est / (1 << bits) > kLoad * cap_max) bits++;
+  hipDeviceGetAttribute(&j.lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+  if (j.lds_dev <= 0) j.lds_dev = 64 * 1024;
+  j.lds_max = j.lds_dev - 8192;  // the join kernel's own static LDS (about 4 KB)
+  j.cap_max = 4096;
+  while (j.cap_max > 8 && join_lds_for(j.cap_max, 64, nw) > (size_t)j.lds_max) j.cap_max /= 2;
+  j.bits = j.radix_bits;
+  if (j.bits < 0) {
+    j.bits = 0;
+    while (j.bits < 16 && (double)j.est / (1 << j.bits) > kLoad * j.cap_max) j.bits++;
+    j.bits = std::max(0, std::min(16, j.bits + j.tune.bits_delta));
   }
-  const std::string grp_key =
-      groupby ? est_key : est_key + "#" + build_est_key(probe, probe_key, probe_pred);
-  int64_t groups_seen = -1;
-  {
-    std::lock_guard<std::mutex> lk(c->mu);
-    auto it = c->build_est.find(grp_key);
-    if (it != c->build_est.end()) groups_seen = it->second;
-  }
-  const bool dev_topk = topk > 0 && topk <= 16;
-  int32_t pool_req = 0;  // 0: from the estimate
+  j.dev_topk = j.topk > 0 && j.topk <= 16;
+  return VQ_OK;
+}
 
+int join_plan(JoinJob& j) {
+  std::string jerr;
+  JoinPlan& plan = j.plan;
+  memset(&plan, 0, sizeof plan);
+  plan.bits = j.bits;
+  plan.P = 1 << j.bits;
+  const double per = (double)j.est / plan.P;
+  int32_t cap = 8;
+  while (cap < j.cap_max && cap * kLoad < per) cap *= 2;
+  plan.cap = cap;
+  // the largest pool the LDS and the kernel's top-k (2048 entries) allow
+  j.pool_max = std::min<int32_t>(2048, cap);
+  while (j.pool_max > 64 && join_lds_for(cap, j.pool_max, j.nw) > (size_t)j.lds_max) j.pool_max /= 2;
+  int32_t pool = j.groups_seen >= 0 ? (int32_t)std::min<int64_t>(2048, 2 * j.groups_seen / plan.P + 64)
+                                    : cap / 2;
+  if (j.pool_req) pool = j.pool_req;
+  plan.pool = std::max<int32_t>(64, std::min(pool, j.pool_max));
+  plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, j.nw);
+  plan.lds_max = j.lds_max;
+  plan.groupby = j.groupby ? 1 : 0;
+  plan.topk = j.dev_topk ? j.topk : 0;
+  plan.nvals = j.nvals;
+  const int P = plan.P;
+  plan.bbuf = part_buffer_entries(P, 8, j.lds_dev);
+  plan.pbuf = part_buffer_entries(P, 8 + 4 + 8 * j.nvals, j.lds_dev);
+  if (j.tune.pbuf >= 0) plan.pbuf = std::min(plan.pbuf, j.tune.pbuf);
+
+  std::vector<uint8_t> bkeep, pkeep;
+  j.brows = j.bkept = j.prows = j.pkept = 0;
+  const int ts = std::max(0, std::min(4, j.tune.tile_shift));
+  if (!j.groupby) {
+    prune(j.build, j.bp, bkeep);
+    j.bs.tile_rows = join_tile_rows(kept_rows(j.build, bkeep)) << ts;
+    if (int rc = make_spans(j.build, bkeep, j.bs.tile_rows, j.bs.span, &j.bs.nspans, &j.bs.ntiles,
+                           &j.brows, &j.bkept))
+      return rc;
+  }
+  prune(j.probe, j.pp, pkeep);
+  j.ps.tile_rows = join_tile_rows(kept_rows(j.probe, pkeep)) << ts;
+  if (int rc = make_spans(j.probe, pkeep, j.ps.tile_rows, j.ps.span, &j.ps.nspans, &j.ps.ntiles,
+                         &j.prows, &j.pkept))
+    return rc;
+  j.RW = 3 + j.naggs;
+  j.rec_cap = std::max<int64_t>(1, j.groupby ? j.prows : std::min(j.brows, j.prows));
+  memset(&j.B, 0, sizeof j.B);
+  j.cap_b = std::max<int64_t>(j.brows, 1);
+  j.cap_p = std::max<int64_t>(j.prows, 1);
+  if (j.cap_b >= (int64_t)INT32_MAX || j.cap_p >= (int64_t)UINT32_MAX)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: join side above 2^31 build / 2^32 probe rows");
+  if ((!j.groupby && jit_part_prepare(j.bs, nullptr, P, plan.bbuf, &j.bl, &jerr)) ||
+      jit_part_prepare(j.ps, &j.jd, P, plan.pbuf, &j.pl, &jerr))
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr.c_str());
+  j.B.b.G = j.groupby ? 1 : j.bl.grid;
+  j.B.p.G = j.pl.grid;
+  size_t bytes = 0;
+  bytes += (size_t)j.cap_b * 8 + (size_t)(P + 1) * 4 + (size_t)j.B.b.G * P * 4;
+  bytes += (size_t)j.cap_p * 12 + (size_t)std::max(j.nvals, 1) * j.cap_p * 8 + (size_t)(P + 1) * 4 +
+           (size_t)j.B.p.G * P * 4;
+  bytes += 16 * 4 + (size_t)2 * P * 4;
+  bytes += j.dev_topk ? (size_t)P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8
+                      : (size_t)j.rec_cap * j.RW * 8;
+  bytes += 16 * 256;  // alignment of the carved pieces
+  j.bytes = (bytes + 255) & ~(size_t)255;
+  j.host_bytes = 64 + (size_t)16 * j.RW * 8;
+  return VQ_OK;
+}
+
+void join_bind(JoinJob& j, char* dev, char* host) {
+  const int P = j.plan.P;
+  Carve cv;
+  JoinBufs& B = j.B;
+  B.flags = cv.take<uint32_t>(dev, 16);
+  B.b.perr = B.p.perr = B.flags + 5;
+  B.b.keys = cv.take<int64_t>(dev, j.cap_b);
+  B.b.vstride = j.cap_b;
+  B.b.pstart = cv.take<uint32_t>(dev, P + 1);
+  B.b.counts = cv.take<uint32_t>(dev, (size_t)B.b.G * P);
+  B.b.tot = cv.take<uint32_t>(dev, P);
+  B.p.keys = cv.take<int64_t>(dev, j.cap_p);
+  B.p.rows = cv.take<uint32_t>(dev, j.cap_p);
+  B.p.vals = cv.take<uint64_t>(dev, (size_t)std::max(j.nvals, 1) * j.cap_p);
+  B.p.vstride = j.cap_p;
+  B.p.pstart = cv.take<uint32_t>(dev, P + 1);
+  B.p.counts = cv.take<uint32_t>(dev, (size_t)B.p.G * P);
+  B.p.tot = cv.take<uint32_t>(dev, P);
+  if (j.dev_topk) {
+    B.cand = cv.take<uint64_t>(dev, (size_t)P * j.topk * j.RW);
+    B.citems = (KItem*)cv.take<uint64_t>(dev, (size_t)P * j.topk * 2);
+    B.out = cv.take<uint64_t>(dev, (size_t)16 * j.RW);
+  } else {
+    B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
+  }
+  j.hflags = (uint32_t*)host;
+  j.hout = (uint64_t*)(host + 64);
+}
+
+// Every launch and copy of one attempt, in stream order (capturable: no
+// memset nodes, no host synchronisation).
+hipError_t join_enqueue(JoinJob& j, hipStream_t s, hipEvent_t* ev) {
+  hipError_t e = launch_join(j.bl, j.pl, j.jd, j.plan, j.B, s, ev);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(j.hflags, j.B.flags, 32, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && j.dev_topk)
+    e = hipMemcpyAsync(j.hout, j.B.out, (size_t)16 * j.RW * 8, hipMemcpyDeviceToHost, s);
+  return e;
+}
+
+// After the stream (or graph) has finished: the result, or kRetry with the
+// next attempt's pool / bits set.
+int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev) {
+  vq_ctx* c = j.c;
+  const uint32_t* hflags = j.hflags;
+  if (hflags[5]) return fail(VQ_E_DEVICE, "exec: gpu: partition scatter wrote past a partition's end");
+  if (hflags[0]) {
+    // a partition's accumulator pool filled: a larger pool if the LDS has
+    // room, else (and when a table filled) one bit more of partitioning
+    if (!(hflags[0] & 1) && j.plan.pool < j.pool_max) {
+      j.pool_req = std::min(j.pool_max, j.plan.pool * 2);
+      return kRetry;
+    }
+    if (j.bits >= 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: join partitions overflow at 2^16");
+    j.bits++;
+    j.pool_req = 0;
+    return kRetry;
+  }
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->build_est[j.est_key] = j.groupby ? hflags[2] : hflags[6];  // keys / passing build rows
+    c->build_est[j.grp_key] = hflags[1];
+  }
+  const int RW = j.RW, naggs = j.naggs;
+  const bool dev_topk = j.dev_topk;
+  const uint32_t ngroups_all = hflags[1];
+  const uint32_t nrec = dev_topk ? hflags[3] : ngroups_all;
+  if (nrec > (uint64_t)j.rec_cap)
+    return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)j.rec_cap);
+  std::vector<uint64_t> hrec;
+  const uint64_t* recs = j.hout;
+  if (!dev_topk) {
+    hrec.resize((size_t)nrec * RW + 1);
+    if (nrec) HIP_TRY(hipMemcpy(hrec.data(), j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
+    recs = hrec.data();
+  }
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    if (ev) {
+      stats->device_ms = elapsed(ev[0], ev[3]);
+      stats->main_kernel_ms = elapsed(ev[1], ev[2]);
+    }
+    stats->rows_scanned = j.brows + j.prows;
+    stats->rg_total = (j.groupby ? 0 : j.build->nrg) + j.probe->nrg;
+    stats->rg_scanned = j.bkept + j.pkept;
+    stats->bytes_alg = (j.groupby ? 0 : j.brows * slot_bytes_per_row(j.build, j.bsm)) +
+                       j.prows * slot_bytes_per_row(j.probe, j.psm);
+    stats->build_rows = j.groupby ? 0 : hflags[6];
+    stats->build_keys = hflags[2];
+    stats->partitions = j.plan.P;
+    stats->radix_bits = j.bits;
+    stats->grid = j.plan.P;
+    stats->tile_rows = j.ps.tile_rows;
+  }
+  // rank order for a device top-k; first-occurrence order otherwise (then
+  // the host top-k, for k above the device limit, keeps the stable order)
+  std::vector<std::pair<uint64_t, uint32_t>> ord(nrec);
+  for (uint32_t i = 0; i < nrec; i++) ord[i] = {dev_topk ? (uint64_t)i : recs[(size_t)i * RW + 2], i};
+  if (!dev_topk) std::sort(ord.begin(), ord.end());
+  if (j.topk > 0 && !dev_topk) {
+    auto key = [&](uint32_t i) { return naggs > 0 ? (int64_t)recs[(size_t)i * RW + 3] : (int64_t)0; };
+    std::stable_sort(ord.begin(), ord.end(), [&](const std::pair<uint64_t, uint32_t>& x,
+                                                 const std::pair<uint64_t, uint32_t>& y) {
+      return key(x.second) > key(y.second);
+    });
+    if ((int64_t)ord.size() > j.topk) ord.resize(j.topk);
+  }
+  const int64_t nout = (int64_t)ord.size();
+  out->ngroups = nout;
+  if (nout > out->capacity)
+    return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)nout,
+                (long long)out->capacity);
+  for (int64_t i = 0; i < nout; i++) {
+    const uint64_t* r = &recs[(size_t)ord[i].second * RW];
+    out->keys[i] = (int64_t)r[0];
+    out->counts[i] = (int64_t)r[1];
+    out->first_row[i] = (int64_t)r[2];
+    for (int k = 0; k < naggs; k++) {
+      const int64_t o = i * naggs + k;
+      out->valid[o] = 1;
+      out->ivals[o] = 0;
+      out->fvals[o] = 0;
+      const vq_agg& a = j.aggs[k];
+      const bool isf = a.kind == VQ_AGG_AVG || (a.kind != VQ_AGG_COUNT && a.accum == VQ_F64);
+      if (isf) out->fvals[o] = bits_to_f64((int64_t)r[3 + k]);
+      else out->ivals[o] = (int64_t)r[3 + k];
+    }
+  }
+  return VQ_OK;
+}
+
+void join_job_init(JoinJob& j, vq_ctx* c, const vq_table* build, int32_t build_key,
+                   const vq_pred* build_pred, const vq_table* probe, int32_t probe_key,
+                   const vq_pred* probe_pred, int32_t naggs, const vq_agg* aggs,
+                   int32_t radix_bits, int32_t topk, bool groupby) {
+  j.c = c;
+  j.build = build;
+  j.probe = probe;
+  j.build_key = build_key;
+  j.probe_key = probe_key;
+  if (build_pred) {
+    j.bpred = *build_pred;
+    j.bp = &j.bpred;
+  }
+  if (probe_pred) {
+    j.ppred = *probe_pred;
+    j.pp = &j.ppred;
+  }
+  j.naggs = naggs;
+  for (int k = 0; k < naggs && k < VQ_MAX_AGGS; k++) j.aggs[k] = aggs[k];
+  j.radix_bits = radix_bits;
+  j.topk = topk;
+  j.groupby = groupby;
+}
+
+// One join (or hash GROUP BY) on one slot, start to finish.
+int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
+  if (int rc = join_setup(j)) return rc;
+  vq_ctx* c = j.c;
   SlotLease S(c);
   hipSetDevice(c->device);
   for (;;) {
-    JoinPlan plan;
-    memset(&plan, 0, sizeof plan);
-    plan.bits = bits;
-    plan.P = 1 << bits;
-    const double per = (double)est / plan.P;
-    int32_t cap = 8;
-    while (cap < cap_max && cap * kLoad < per) cap *= 2;
-    plan.cap = cap;
-    // the largest pool the LDS and the kernel's top-k (2048 entries) allow
-    int32_t pool_max = std::min<int32_t>(2048, cap);
-    while (pool_max > 64 && join_lds_for(cap, pool_max, nw) > (size_t)lds_max) pool_max /= 2;
-    int32_t pool = groups_seen >= 0 ? (int32_t)std::min<int64_t>(2048, 2 * groups_seen / plan.P + 64)
-                                    : cap / 2;
-    if (pool_req) pool = pool_req;
-    plan.pool = std::max<int32_t>(64, std::min(pool, pool_max));
-    plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, nw);
-    plan.lds_max = lds_max;
-    plan.groupby = groupby ? 1 : 0;
-    plan.topk = dev_topk ? topk : 0;
-    plan.nvals = nvals;
-    const int P = plan.P;
-    plan.bbuf = part_buffer_entries(P, 8, lds_dev);
-    plan.pbuf = part_buffer_entries(P, 8 + 4 + 8 * nvals, lds_dev);
-
-    std::vector<uint8_t> bkeep, pkeep;
-    int64_t brows = 0, bkept = 0, prows = 0, pkept = 0;
-    if (!groupby) {
-      prune(build, build_pred, bkeep);
-      bs.tile_rows = join_tile_rows(kept_rows(build, bkeep));
-      if (int rc = make_spans(build, bkeep, bs.tile_rows, bs.span, &bs.nspans, &bs.ntiles, &brows, &bkept)) return rc;
-    }
-    prune(probe, probe_pred, pkeep);
-    ps.tile_rows = join_tile_rows(kept_rows(probe, pkeep));
-    if (int rc = make_spans(probe, pkeep, ps.tile_rows, ps.span, &ps.nspans, &ps.ntiles, &prows, &pkept)) return rc;
-
-    const int RW = 3 + naggs;
-    const int64_t rec_cap = std::max<int64_t>(1, groupby ? prows : std::min(brows, prows));
-    JoinBufs B;
-    memset(&B, 0, sizeof B);
-    const int64_t cap_b = std::max<int64_t>(brows, 1);
-    const int64_t cap_p = std::max<int64_t>(prows, 1);
-    if (cap_b >= (int64_t)INT32_MAX || cap_p >= (int64_t)UINT32_MAX)
-      return fail(VQ_E_UNSUPPORTED, "exec: gpu: join side above 2^31 build / 2^32 probe rows");
-    PartLaunch bl, pl;
-    if ((!groupby && jit_part_prepare(bs, nullptr, P, plan.bbuf, &bl, &jerr_)) ||
-        jit_part_prepare(ps, &jd, P, plan.pbuf, &pl, &jerr_))
-      return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr_.c_str());
-    B.b.G = groupby ? 1 : bl.grid;
-    B.p.G = pl.grid;
-    size_t bytes = 0;
-    bytes += (size_t)cap_b * 8 + (size_t)(P + 1) * 4 + (size_t)B.b.G * P * 4;
-    bytes += (size_t)cap_p * 12 + (size_t)std::max(nvals, 1) * cap_p * 8 + (size_t)(P + 1) * 4 +
-             (size_t)B.p.G * P * 4;
-    bytes += 16 * 4 + (size_t)2 * P * 4;
-    bytes += dev_topk ? (size_t)P * topk * (RW * 8 + 16) + (size_t)16 * RW * 8 : (size_t)rec_cap * RW * 8;
-    bytes += 16 * 256;  // alignment of the carved pieces
+    if (int rc = join_plan(j)) return rc;
     void* ar;
-    if (int rc = arena_get(c, S.get(), bytes, &ar)) return rc;
-    Carve cv;
-    char* base = (char*)ar;
-    B.flags = cv.take<uint32_t>(base, 16);
-    B.b.perr = B.p.perr = B.flags + 5;
-    B.b.keys = cv.take<int64_t>(base, cap_b);
-    B.b.vstride = cap_b;
-    B.b.pstart = cv.take<uint32_t>(base, P + 1);
-    B.b.counts = cv.take<uint32_t>(base, (size_t)B.b.G * P);
-    B.b.tot = cv.take<uint32_t>(base, P);
-    B.p.keys = cv.take<int64_t>(base, cap_p);
-    B.p.rows = cv.take<uint32_t>(base, cap_p);
-    B.p.vals = cv.take<uint64_t>(base, (size_t)std::max(nvals, 1) * cap_p);
-    B.p.vstride = cap_p;
-    B.p.pstart = cv.take<uint32_t>(base, P + 1);
-    B.p.counts = cv.take<uint32_t>(base, (size_t)B.p.G * P);
-    B.p.tot = cv.take<uint32_t>(base, P);
-    if (dev_topk) {
-      B.cand = cv.take<uint64_t>(base, (size_t)P * topk * RW);
-      B.citems = (KItem*)cv.take<uint64_t>(base, (size_t)P * topk * 2);
-      B.out = cv.take<uint64_t>(base, (size_t)16 * RW);
-    } else {
-      B.recs = cv.take<uint64_t>(base, (size_t)rec_cap * RW);
-    }
-
-    HIP_TRY(launch_join(bl, pl, jd, plan, B, S->stream, S->ev));
     void* hp;
-    if (int rc = pinned_get(S.get(), 64, &hp)) return rc;
-    uint32_t* hflags = (uint32_t*)hp;
-    HIP_TRY(hipMemcpyAsync(hflags, B.flags, 32, hipMemcpyDeviceToHost, S->stream));
+    if (int rc = arena_get(c, S.get(), j.bytes, &ar)) return rc;
+    if (int rc = pinned_get(S.get(), j.host_bytes, &hp)) return rc;
+    join_bind(j, (char*)ar, (char*)hp);
+    HIP_TRY(join_enqueue(j, S->stream, S->ev));
     HIP_TRY(hipStreamSynchronize(S->stream));
-    if (hflags[5]) return fail(VQ_E_DEVICE, "exec: gpu: partition scatter wrote past a partition's end");
-    if (hflags[0]) {
-      // a partition's accumulator pool filled: a larger pool if the LDS has
-      // room, else (and when a table filled) one bit more of partitioning
-      if (!(hflags[0] & 1) && plan.pool < pool_max) {
-        pool_req = std::min(pool_max, plan.pool * 2);
-        continue;
-      }
-      if (bits >= 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: join partitions overflow at 2^16");
-      bits++;
-      pool_req = 0;
-      continue;
-    }
-    {
-      std::lock_guard<std::mutex> lk(c->mu);
-      c->build_est[est_key] = groupby ? hflags[2] : hflags[6];  // keys / passing build rows
-      c->build_est[grp_key] = hflags[1];
-    }
-    const uint32_t ngroups_all = hflags[1];
-    const uint32_t nrec = dev_topk ? hflags[3] : ngroups_all;
-    if (nrec > (uint64_t)rec_cap) return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)rec_cap);
-    std::vector<uint64_t> hrec((size_t)nrec * RW + 1);
-    if (nrec) HIP_TRY(hipMemcpy(hrec.data(), dev_topk ? B.out : B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
-    if (stats) {
-      memset(stats, 0, sizeof *stats);
-      stats->device_ms = elapsed(S->ev[0], S->ev[3]);
-      stats->main_kernel_ms = elapsed(S->ev[1], S->ev[2]);
-      stats->rows_scanned = brows + prows;
-      stats->rg_total = (groupby ? 0 : build->nrg) + probe->nrg;
-      stats->rg_scanned = bkept + pkept;
-      stats->bytes_alg = (groupby ? 0 : brows * slot_bytes_per_row(build, bsm)) +
-                         prows * slot_bytes_per_row(probe, psm);
-      stats->build_rows = groupby ? 0 : hflags[6];
-      stats->build_keys = hflags[2];
-      stats->partitions = P;
-      stats->radix_bits = bits;
-      stats->grid = P;
-      stats->tile_rows = ps.tile_rows;
-    }
-    // rank order for a device top-k; first-occurrence order otherwise (then
-    // the host top-k, for k above the device limit, keeps the stable order)
-    std::vector<std::pair<uint64_t, uint32_t>> ord(nrec);
-    for (uint32_t i = 0; i < nrec; i++) ord[i] = {dev_topk ? (uint64_t)i : hrec[(size_t)i * RW + 2], i};
-    if (!dev_topk) std::sort(ord.begin(), ord.end());
-    if (topk > 0 && !dev_topk) {
-      auto key = [&](uint32_t i) { return naggs > 0 ? (int64_t)hrec[(size_t)i * RW + 3] : (int64_t)0; };
-      std::stable_sort(ord.begin(), ord.end(), [&](const std::pair<uint64_t, uint32_t>& x,
-                                                   const std::pair<uint64_t, uint32_t>& y) {
-        return key(x.second) > key(y.second);
-      });
-      if ((int64_t)ord.size() > topk) ord.resize(topk);
-    }
-    const int64_t nout = (int64_t)ord.size();
-    out->ngroups = nout;
-    if (nout > out->capacity)
-      return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)nout,
-                  (long long)out->capacity);
-    for (int64_t i = 0; i < nout; i++) {
-      const uint64_t* r = &hrec[(size_t)ord[i].second * RW];
-      out->keys[i] = (int64_t)r[0];
-      out->counts[i] = (int64_t)r[1];
-      out->first_row[i] = (int64_t)r[2];
-      for (int j = 0; j < naggs; j++) {
-        const int64_t o = i * naggs + j;
-        out->valid[o] = 1;
-        out->ivals[o] = 0;
-        out->fvals[o] = 0;
-        const bool isf = aggs[j].kind == VQ_AGG_AVG || (aggs[j].kind != VQ_AGG_COUNT && aggs[j].accum == VQ_F64);
-        if (isf) out->fvals[o] = bits_to_f64((int64_t)r[3 + j]);
-        else out->ivals[o] = (int64_t)r[3 + j];
-      }
-    }
-    return VQ_OK;
+    const int rc = join_result(j, out, stats, S->ev);
+    if (rc != kRetry) return rc;
   }
+}
 
+int join_impl(vq_ctx* c, const vq_table* build, int32_t build_key, const vq_pred* build_pred,
+              const vq_table* probe, int32_t probe_key, const vq_pred* probe_pred, int32_t naggs,
+              const vq_agg* aggs, int32_t radix_bits, int32_t topk, vq_agg_result* out,
+              vq_stats* stats, bool groupby, const JoinTune* tune = nullptr) {
+  if (!out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs))
+    return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
+  std::unique_ptr<JoinJob> j(new (std::nothrow) JoinJob());
+  if (!j) return fail(VQ_E_NOMEM, "exec: gpu: join job allocation");
+  join_job_init(*j, c, build, build_key, build_pred, probe, probe_key, probe_pred, naggs, aggs,
+                radix_bits, topk, groupby);
+  if (tune) j->tune = *tune;
+  return join_run(*j, out, stats);
 }
 
 }  // namespace
@@ -1613,4 +1761,262 @@ extern "C" int vq_wait(vq_ticket* h) {
   delete k;
   delete h;
   return rc;
+}
+
+// ===========================================================================
+// batches of independent queries
+// ===========================================================================
+namespace {
+
+// FNV-1a over bytes, with two seeds: a 128-bit fingerprint of everything a
+// batch's launches take as arguments.
+struct Fp {
+  uint64_t h1 = 0xcbf29ce484222325ULL, h2 = 0x84222325cbf29ce4ULL;
+  void add(const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) {
+      h1 = (h1 ^ b[i]) * 0x100000001b3ULL;
+      h2 = (h2 ^ b[i]) * 0x100000001b3ULL + 0x9e3779b97f4a7c15ULL;
+    }
+  }
+};
+
+int slot_branches(Slot* S) {
+  if (S->fork) return VQ_OK;
+  hipError_t e = hipEventCreateWithFlags(&S->fork, hipEventDisableTiming);
+  for (int k = 0; k < Slot::kBranches && e == hipSuccess; k++) {
+    e = hipStreamCreateWithFlags(&S->side[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->join[k], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) return fail(VQ_E_DEVICE, "exec: gpu: batch streams: %s", hipGetErrorString(e));
+  return VQ_OK;
+}
+
+// The class an instance is tuned for: its kind, key type, and the binary
+// order of magnitude of each side's rows and of its estimated keys.
+std::string tune_class(const JoinJob& j) {
+  auto lg = [](int64_t x) { return ilog2_ceil((uint64_t)std::max<int64_t>(x, 1)); };
+  char buf[128];
+  snprintf(buf, sizeof buf, "%s/k%d/p%d/b%d/e%d/a%d", j.groupby ? "hash" : "join",
+           j.probe->cols[j.probe_key].type, lg(j.probe->nrows), j.groupby ? 0 : lg(j.build->nrows),
+           lg(j.est), j.naggs);
+  return buf;
+}
+
+bool tune_lookup(vq_ctx* c, const std::string& cls, JoinTune* t) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->tune.find(cls);
+  if (it == c->tune.end()) return false;
+  return sscanf(it->second.c_str(), "%d %d %d", &t->bits_delta, &t->pbuf, &t->tile_shift) == 3;
+}
+
+void tune_store(vq_ctx* c, const std::string& cls, const JoinTune& t) {
+  char buf[64];
+  snprintf(buf, sizeof buf, "%d %d %d", t.bits_delta, t.pbuf, t.tile_shift);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->tune[cls] = buf;
+  if (const char* f = getenv("VQ_TUNE_FILE")) {
+    if (FILE* fp = fopen(f, "a")) {
+      fprintf(fp, "%s %s\n", cls.c_str(), buf);
+      fclose(fp);
+    }
+  }
+}
+
+void tune_load(vq_ctx* c) {
+  const char* f = getenv("VQ_TUNE_FILE");
+  FILE* fp = f ? fopen(f, "r") : nullptr;
+  if (!fp) return;
+  char cls[128];
+  int a, b, d;
+  std::lock_guard<std::mutex> lk(c->mu);
+  while (fscanf(fp, "%127s %d %d %d", cls, &a, &b, &d) == 4) {
+    char buf[64];
+    snprintf(buf, sizeof buf, "%d %d %d", a, b, d);
+    c->tune[cls] = buf;
+  }
+  fclose(fp);
+}
+
+// The autotuner: run the instance under each candidate (radix bits one
+// below / at / above the planner's choice, the largest or an 8-entry scatter
+// buffer, default or 4x tiles), keep the fastest device time.  Every
+// candidate computes the same result (the accumulation is exact and
+// order-free), so tuning never changes an answer.
+JoinTune autotune(const JoinJob& proto, vq_agg_result* out) {
+  JoinTune best;
+  double best_ms = 1e30;
+  for (int bd = -1; bd <= 1; bd++)
+    for (int pb : {-1, 8})
+      for (int ts : {0, 2}) {
+        JoinTune t;
+        t.bits_delta = bd;
+        t.pbuf = pb;
+        t.tile_shift = ts;
+        double ms = 1e30;
+        for (int rep = 0; rep < 2; rep++) {
+          vq_stats st;
+          const int rc = join_impl(proto.c, proto.build, proto.build_key, proto.bp, proto.probe,
+                                   proto.probe_key, proto.pp, proto.naggs, proto.aggs,
+                                   proto.radix_bits, proto.topk, out, &st, proto.groupby, &t);
+          if (rc) {
+            ms = 1e30;
+            break;
+          }
+          ms = std::min(ms, st.device_ms);
+        }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+  return best;
+}
+
+}  // namespace
+
+extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_result* outs,
+                            vq_stats* stats, int32_t flags) {
+  if (!c || n < 0 || (n > 0 && (!qs || !outs))) return fail(VQ_E_ARG, "exec: gpu: bad batch arguments");
+  hipSetDevice(c->device);
+  std::vector<std::unique_ptr<JoinJob>> jobs(n);
+  std::vector<char> direct(n, 0);  // dictionary GROUP BYs run one by one
+  for (int32_t i = 0; i < n; i++) {
+    const vq_query& q = qs[i];
+    if (q.kind == VQ_Q_AGG_DICT) {
+      direct[i] = 1;
+      continue;
+    }
+    if (q.kind != VQ_Q_JOIN_AGG && q.kind != VQ_Q_AGG_HASH)
+      return fail(VQ_E_ARG, "exec: gpu: batch query %d: kind %d", i, q.kind);
+    if (q.naggs < 0 || q.naggs > VQ_MAX_AGGS || (q.naggs > 0 && !q.aggs))
+      return fail(VQ_E_ARG, "exec: gpu: batch query %d: %d aggregates", i, q.naggs);
+    const bool gb = q.kind == VQ_Q_AGG_HASH;
+    if (gb && (q.nkeys != 1 || !q.key_cols)) return fail(VQ_E_ARG, "exec: gpu: batch query %d: one key", i);
+    jobs[i].reset(new (std::nothrow) JoinJob());
+    if (!jobs[i]) return fail(VQ_E_NOMEM, "exec: gpu: batch job allocation");
+    JoinJob& j = *jobs[i];
+    join_job_init(j, c, gb ? nullptr : q.build, q.build_key, q.build_pred, q.table,
+                  gb ? q.key_cols[0] : q.probe_key, q.pred, q.naggs, q.aggs, gb ? -1 : q.radix_bits,
+                  gb ? 0 : q.topk, gb);
+    if (int rc = join_setup(j)) return rc;
+    const std::string cls = tune_class(j);
+    JoinTune t;
+    if (!tune_lookup(c, cls, &t) && (flags & VQ_BATCH_TUNE)) {
+      t = autotune(j, &outs[i]);
+      tune_store(c, cls, t);
+    }
+    if (t.bits_delta || t.pbuf >= 0 || t.tile_shift) {
+      j.tune = t;
+      if (int rc = join_setup(j)) return rc;  // the bits again, with the tuning
+    }
+    if (int rc = join_plan(j)) return rc;
+  }
+  std::vector<int> rcs(n, VQ_OK);
+  {
+    SlotLease S(c);
+    size_t dev = 0, host = 0;
+    for (auto& j : jobs)
+      if (j) {
+        dev += j->bytes;
+        host += (j->host_bytes + 255) & ~(size_t)255;
+      }
+    if (dev > 0) {
+      void* ar;
+      void* hp;
+      if (int rc = arena_get(c, S.get(), dev, &ar)) return rc;
+      if (int rc = pinned_get(S.get(), host, &hp)) return rc;
+      size_t od = 0, oh = 0;
+      for (auto& j : jobs)
+        if (j) {
+          join_bind(*j, (char*)ar + od, (char*)hp + oh);
+          od += j->bytes;
+          oh += (j->host_bytes + 255) & ~(size_t)255;
+        }
+      if (int rc = slot_branches(S.get())) return rc;
+      for (auto& j : jobs)
+        if (j && j->plan.lds_bytes > 64 * 1024) join_allow_lds(j->plan.lds_max);
+      const int K = Slot::kBranches;
+      if (flags & VQ_BATCH_NO_GRAPH) {
+        int b = 0;
+        for (auto& j : jobs)
+          if (j) HIP_TRY(join_enqueue(*j, S->side[b++ % K], nullptr));
+        for (int k = 0; k < K; k++) HIP_TRY(hipStreamSynchronize(S->side[k]));
+      } else {
+        // the batch's graph: instance i on branch i mod K, forked from and
+        // joined back into the slot's stream; reused while the batch's
+        // launch arguments are the same
+        Fp fp;
+        for (auto& j : jobs)
+          if (j) {
+            fp.add(&j->plan, sizeof j->plan);
+            fp.add(&j->jd, sizeof j->jd);
+            fp.add(&j->B, sizeof j->B);
+            fp.add(&j->bl.args, sizeof j->bl.args);
+            fp.add(&j->pl.args, sizeof j->pl.args);
+            fp.add(&j->bl.grid, sizeof j->bl.grid);
+            fp.add(&j->pl.grid, sizeof j->pl.grid);
+            fp.add(&j->hflags, sizeof j->hflags);
+          }
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : S->graphs)
+          if (g.h1 == fp.h1 && g.h2 == fp.h2) exec = g.exec;
+        if (!exec) {
+          hipGraph_t graph = nullptr;
+          HIP_TRY(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+          hipError_t e = hipEventRecord(S->fork, S->stream);
+          for (int k = 0; k < K && e == hipSuccess; k++) e = hipStreamWaitEvent(S->side[k], S->fork, 0);
+          int b = 0;
+          for (auto& j : jobs)
+            if (j && e == hipSuccess) e = join_enqueue(*j, S->side[b++ % K], nullptr);
+          for (int k = 0; k < K && e == hipSuccess; k++) {
+            e = hipEventRecord(S->join[k], S->side[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(S->stream, S->join[k], 0);
+          }
+          const hipError_t e2 = hipStreamEndCapture(S->stream, &graph);
+          if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return fail(VQ_E_DEVICE, "exec: gpu: batch capture: %s",
+                        hipGetErrorString(e != hipSuccess ? e : e2));
+          }
+          e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+          hipGraphDestroy(graph);
+          if (e != hipSuccess) return fail(VQ_E_DEVICE, "exec: gpu: batch graph: %s", hipGetErrorString(e));
+          if (S->graphs.size() >= 4) {
+            hipGraphExecDestroy(S->graphs.front().exec);
+            S->graphs.erase(S->graphs.begin());
+          }
+          S->graphs.push_back({fp.h1, fp.h2, exec});
+        }
+        HIP_TRY(hipEventRecord(S->ev[0], S->stream));
+        HIP_TRY(hipGraphLaunch(exec, S->stream));
+        HIP_TRY(hipEventRecord(S->ev[1], S->stream));
+        HIP_TRY(hipStreamSynchronize(S->stream));
+      }
+      const float batch_ms = (flags & VQ_BATCH_NO_GRAPH) ? 0.0f : elapsed(S->ev[0], S->ev[1]);
+      for (int32_t i = 0; i < n; i++) {
+        if (!jobs[i]) continue;
+        rcs[i] = join_result(*jobs[i], &outs[i], stats ? &stats[i] : nullptr, nullptr);
+        if (stats && rcs[i] == VQ_OK) stats[i].device_ms = batch_ms;  // the whole batch's graph
+      }
+    }
+  }
+  // instances that overflowed a partition run again on their own (with the
+  // estimates this run has learned), and dictionary GROUP BYs run directly
+  int first_rc = VQ_OK;
+  for (int32_t i = 0; i < n; i++) {
+    const vq_query& q = qs[i];
+    int rc = rcs[i];
+    if (direct[i]) {
+      rc = vq_agg_dict(c, q.table, q.pred, q.nkeys, q.key_cols, q.naggs, q.aggs, &outs[i],
+                       stats ? &stats[i] : nullptr);
+    } else if (rc == kRetry) {
+      JoinJob& j = *jobs[i];
+      rc = join_impl(c, j.build, j.build_key, j.bp, j.probe, j.probe_key, j.pp, j.naggs, j.aggs,
+                     j.groupby ? -1 : j.radix_bits, j.topk, &outs[i], stats ? &stats[i] : nullptr,
+                     j.groupby, &j.tune);
+    }
+    if (rc && first_rc == VQ_OK) first_rc = rc;
+  }
+  return first_rc;
 }

@@ -268,6 +268,10 @@ hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, 
 // bytes: the largest of 16, 8, 4, 2, 1 that fits, else 0.
 int32_t part_buffer_entries(int32_t P, int32_t entry, int32_t lds_max);
 
+// Lets the join kernel launch with up to `bytes` of dynamic LDS (once per
+// process and size; outside any graph capture).
+void join_allow_lds(int32_t bytes);
+
 hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd,
                        const JoinPlan& plan, const JoinBufs& bufs, hipStream_t s, hipEvent_t* ev);
 }  // namespace vq

@@ -23,6 +23,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import math
+
 import numpy as np
 
 from . import abi
@@ -381,3 +383,39 @@ def q_keyed(t: Table, cut: int = DATE_MIN + SHIP_DAYS - 1 - 90):
             abi.agg_avg("price", t), abi.count_star(t), abi.agg_min("price", t),
             abi.agg_max("quantity", t)]
     return pred, t.index("key"), aggs
+
+
+def batch_specs(n: int = 256, seed: int = 42, min_rows: int = 10_000, max_rows: int = 4_000_000):
+    """The batched workload's instances: half GROUP BYs, half joins; rows
+    log-uniform in [min_rows, max_rows] (the probe side's, for a join); GROUP
+    BY cardinality log-uniform in [4, 1M] (at most the rows), u16 keys when
+    it fits in 16 bits and the draw says so, else int64; join build:probe
+    uniform in [1:2, 1:16]; Zipf(1.1) key skew on every other instance."""
+    r = rand_u64(seed, 300, 6 * n)
+    u = (r >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    specs = []
+    for i in range(n):
+        rows = int(round(math.exp(math.log(min_rows) + u[6 * i] * (math.log(max_rows) - math.log(min_rows)))))
+        skew = 1.1 if i % 2 else 0.0
+        if u[6 * i + 1] < 0.5:
+            groups = int(round(math.exp(math.log(4) + u[6 * i + 2] * (math.log(1_000_000) - math.log(4)))))
+            groups = max(4, min(groups, rows))
+            kt = abi.VQ_U16 if (groups <= 65536 and u[6 * i + 3] < 0.5) else abi.VQ_I64
+            specs.append({"kind": "hash", "rows": rows, "groups": groups, "key_type": kt,
+                          "skew": skew, "seed": 1000 + i})
+        else:
+            ratio = 2.0 + 14.0 * u[6 * i + 4]
+            specs.append({"kind": "join", "rows": rows, "build": max(16, int(rows / ratio)),
+                          "ratio": ratio, "skew": skew, "seed": 1000 + i})
+    return specs
+
+
+def q_join_instance(build: Table, probe: Table, cutoff: int = JOIN_CUTOFF + 100):
+    """A batched join instance's query: orderdate < c, shipdate > c - 60,
+    GROUP BY key, SUM(price*(1-discount)), COUNT(*), MAX(quantity)."""
+    X = abi.X
+    bpred = abi.make_pred(build, [("orderdate", "<", cutoff)])
+    ppred = abi.make_pred(probe, [("shipdate", ">", cutoff - 60)])
+    aggs = [abi.agg_sum(X.col("price") * (1 - X.col("discount")), probe), abi.count_star(probe),
+            abi.agg_max("quantity", probe)]
+    return bpred, ppred, aggs

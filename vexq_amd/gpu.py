@@ -40,6 +40,8 @@ def lib():
     L.vq_submit.argtypes = [C.c_void_p, P(abi.Query), P(abi.AggResult), P(abi.Stats),
                             P(C.c_void_p)]
     L.vq_ticket_done.argtypes = [C.c_void_p]
+    L.vq_batch_run.argtypes = [C.c_void_p, C.c_int32, P(abi.Query), P(abi.AggResult),
+                               P(abi.Stats), C.c_int32]
     L.vq_wait.argtypes = [C.c_void_p]
     L.vq_ctx_bytes_in_use.argtypes = [C.c_void_p]
     L.vq_ctx_bytes_in_use.restype = C.c_uint64
@@ -154,6 +156,60 @@ class Context:
         if capacity is None:
             capacity = topk if topk > 0 else max(1, min(build.nrows, probe.nrows))
         return self._submit(q, abi.AggOut(capacity, 1, len(aggs)), keep)
+
+    # -- batches ------------------------------------------------------------
+    @staticmethod
+    def batch_hash(dt: "DeviceTable", pred, key_col: int, aggs, capacity: int | None = None):
+        """A hash GROUP BY instance for batch_run."""
+        q = abi.Query()
+        q.kind = abi.VQ_Q_AGG_HASH
+        q.table = dt.h
+        kc = (C.c_int32 * 1)(key_col)
+        keep = [abi.agg_array(aggs), kc]
+        if pred is not None:
+            keep.append(pred)
+            q.pred = C.pointer(pred)
+        q.nkeys = 1
+        q.key_cols = C.cast(kc, C.POINTER(C.c_int32))
+        q.naggs = len(aggs)
+        q.aggs = C.cast(keep[0], C.POINTER(abi.Agg))
+        return q, abi.AggOut(capacity or max(1, dt.nrows), 1, len(aggs)), keep
+
+    @staticmethod
+    def batch_join(build: "DeviceTable", build_key: int, build_pred, probe: "DeviceTable",
+                   probe_key: int, probe_pred, aggs, topk: int = 0, capacity: int | None = None):
+        """A join + aggregate instance for batch_run."""
+        q = abi.Query()
+        q.kind = abi.VQ_Q_JOIN_AGG
+        q.build, q.table = build.h, probe.h
+        keep = [abi.agg_array(aggs)]
+        if build_pred is not None:
+            keep.append(build_pred)
+            q.build_pred = C.pointer(build_pred)
+        if probe_pred is not None:
+            keep.append(probe_pred)
+            q.pred = C.pointer(probe_pred)
+        q.build_key, q.probe_key = build_key, probe_key
+        q.naggs = len(aggs)
+        q.aggs = C.cast(keep[0], C.POINTER(abi.Agg))
+        q.radix_bits, q.topk = -1, topk
+        cap = capacity or (topk if topk > 0 else max(1, min(build.nrows, probe.nrows)))
+        return q, abi.AggOut(cap, 1, len(aggs)), keep
+
+    def batch_run(self, instances, flags: int = 0):
+        """Runs [batch_hash / batch_join instances] as one batch; returns
+        their results (AggOut, with .stats)."""
+        n = len(instances)
+        qs = (abi.Query * max(n, 1))(*[q for q, _, _ in instances])
+        outs = (abi.AggResult * max(n, 1))(*[o.c for _, o, _ in instances])
+        sts = (abi.Stats * max(n, 1))()
+        _check(lib().vq_batch_run(self.h, n, qs, outs, sts, flags))
+        res = []
+        for i, (_, o, _) in enumerate(instances):
+            o.c.ngroups = outs[i].ngroups
+            o.stats = sts[i]
+            res.append(o)
+        return res
 
     def _submit(self, q, out, keep) -> "Ticket":
         st = abi.Stats()
