@@ -265,6 +265,84 @@ def serve_bench(ctx, scale, nq, slots=8):
             "qps_ratio": round(best["served_qps"] / seq_qps, 2) if best else None}
 
 
+def batch_bench(ctx, n, max_rows, steps=5):
+    """The batched workload (config 3): n independent instances (half hash
+    GROUP BYs on u16/int64 keys of 4..1M groups, half joins of build:probe
+    1:2..1:16, rows log-uniform in [10K, max_rows], Zipf(1.1) skew on every
+    other one).  First each instance as its own plain query, one after
+    another; then the whole set through vq_batch_run: autotuned classes, one
+    captured graph of eight branches, replayed."""
+    import numpy as np
+    from vexq_amd import abi, datagen
+    specs = datagen.batch_specs(n, max_rows=max_rows)
+    log(f"[bench] batch: generating {n} instances, {sum(s['rows'] for s in specs) / 1e6:.1f}M rows")
+    made = []
+    for sp in specs:
+        if sp["kind"] == "hash":
+            t = datagen.keyed_table(sp["rows"], sp["groups"], sp["key_type"], sp["skew"], sp["seed"])
+            made.append(("hash", (ctx.upload(t),), datagen.q_keyed(t)))
+        else:
+            b, p = datagen.join_instance(sp["build"], sp["ratio"], sp["skew"], sp["seed"])
+            made.append(("join", (ctx.upload(b), ctx.upload(p)), datagen.q_join_instance(b, p)))
+
+    def alone():
+        res = []
+        for kind, dts, q in made:
+            if kind == "hash":
+                res.append(ctx.agg_hash(dts[0], q[0], q[1], q[2]))
+            else:
+                res.append(ctx.join_agg(dts[0], 0, q[0], dts[1], 0, q[1], q[2]))
+        return res
+
+    def instances():
+        out = []
+        for kind, dts, q in made:
+            if kind == "hash":
+                out.append(ctx.batch_hash(dts[0], q[0], q[1], q[2]))
+            else:
+                out.append(ctx.batch_join(dts[0], 0, q[0], dts[1], 0, q[1], q[2]))
+        return out
+
+    log("[bench] batch: plain launches")
+    alone()  # compile each shape, learn estimates
+    t0 = time.perf_counter()
+    ref = None
+    for _ in range(steps):
+        ref = alone()
+    plain_ms = (time.perf_counter() - t0) / steps * 1e3
+    bytes_alg = sum(int(r.stats.bytes_alg) for r in ref)
+    rows = sum(int(r.stats.rows_scanned) for r in ref)
+    log("[bench] batch: autotune + capture")
+    ctx.batch_run(instances(), abi.VQ_BATCH_TUNE)  # tune the classes
+    got = ctx.batch_run(instances())                # capture the graph
+    same = all(a.ngroups == b.ngroups and np.array_equal(a.keys, b.keys)
+               and np.array_equal(a.counts, b.counts) and np.array_equal(a.ivals, b.ivals)
+               and np.array_equal(a.fvals.view(np.int64), b.fvals.view(np.int64))
+               for a, b in zip(ref, got))
+    t0 = time.perf_counter()
+    dev = []
+    for _ in range(steps):
+        res = ctx.batch_run(instances())
+        dev.append(res[0].stats.device_ms)
+    batch_ms = (time.perf_counter() - t0) / steps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.batch_run(instances(), abi.VQ_BATCH_NO_GRAPH)
+    nograph_ms = (time.perf_counter() - t0) / steps * 1e3
+    for _, dts, _ in made:
+        for d in dts:
+            d.close()
+    return {"unit": "ms per batch", "instances": n, "rows": rows, "bytes_alg": bytes_alg,
+            "plain_launches_ms": round(plain_ms, 3), "batched_graph_ms": round(batch_ms, 3),
+            "batched_graph_device_ms": round(statistics.median(dev), 3),
+            "batched_no_graph_ms": round(nograph_ms, 3),
+            "speedup_vs_plain": round(plain_ms / batch_ms, 2),
+            "batched_gbs": round(bytes_alg / (batch_ms * 1e-3) / 1e9, 2),
+            "plain_gbs": round(bytes_alg / (plain_ms * 1e-3) / 1e9, 2),
+            "batched_rows_per_s": round(rows / (batch_ms * 1e-3), 1),
+            "results_identical_to_plain": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +355,8 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="fraction of the served stream's 60M/15M-row tables")
     ap.add_argument("--serve-queries", type=int, default=240)
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-max-rows", type=int, default=4_000_000)
     args = ap.parse_args()
     if args.copy_probe:
         print(json.dumps({"copy_probe": copy_probe()}), flush=True)
@@ -394,14 +474,17 @@ def main():
         dp.close()
         log(f"[bench] join: {rec['value']} GB/s")
 
+    if want("batch_mixed"):
+        ops["batch_mixed"] = batch_bench(ctx, args.batch_instances, args.batch_max_rows)
+        log(f"[bench] batch: {ops['batch_mixed']}")
     if want("serve_mixed"):
         ops["serve_mixed"] = serve_bench(ctx, args.serve_scale, args.serve_queries)
 
     ctx.close()
     head_name = "dict_groupby_6m" if "dict_groupby_6m" in ops else next(
         (k for k in ops if "value" in ops[k]), None)
-    if head_name is None:  # only the served stream was asked for
-        print(json.dumps({"serve_mixed": ops.get("serve_mixed")}), flush=True)
+    if head_name is None:  # only the served stream / the batch was asked for
+        print(json.dumps(ops), flush=True)
         return
     head = ops[head_name]
     line = {

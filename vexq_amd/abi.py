@@ -279,12 +279,13 @@ class AggOut:
     def __init__(self, capacity: int, nkeys: int, naggs: int):
         cap = max(1, int(capacity))
         self.nkeys, self.naggs = nkeys, naggs
-        self._keys = _np.zeros(cap * max(nkeys, 1), dtype=_np.int64)
-        self._counts = _np.zeros(cap, dtype=_np.int64)
-        self._first = _np.zeros(cap, dtype=_np.int64)
-        self._ivals = _np.zeros(cap * max(naggs, 1), dtype=_np.int64)
-        self._fvals = _np.zeros(cap * max(naggs, 1), dtype=_np.float64)
-        self._valid = _np.zeros(cap * max(naggs, 1), dtype=_np.uint8)
+        # the library writes every field of the ngroups rows it returns
+        self._keys = _np.empty(cap * max(nkeys, 1), dtype=_np.int64)
+        self._counts = _np.empty(cap, dtype=_np.int64)
+        self._first = _np.empty(cap, dtype=_np.int64)
+        self._ivals = _np.empty(cap * max(naggs, 1), dtype=_np.int64)
+        self._fvals = _np.empty(cap * max(naggs, 1), dtype=_np.float64)
+        self._valid = _np.empty(cap * max(naggs, 1), dtype=_np.uint8)
         r = AggResult()
         r.capacity = cap
         r.ngroups = 0

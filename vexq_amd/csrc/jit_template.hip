@@ -25,7 +25,6 @@
 #define VQ_NV 5
 #define VQ_PROBE 1
 #define VQ_KERNELS 7
-#define VQ_P 256
 #define VQ_PB 8
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
@@ -98,7 +97,7 @@ struct Args {
   u64* vals;          // [NV][vstride]
   i64 vstride;
   i32 P;
-  i32 pad_;
+  u32 lds_words;      // dynamic LDS the launch provides, in 4-byte words
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -531,12 +530,15 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
 #if VQ_KERNELS & 4
 // ---------------------------------------------------------------------------
 // Radix partitioning of one join side (exec/parallel_join.go:322-347,422-550):
-// rows that pass the side's predicate go to partition radixHash(key) & (P-1).
-// A persistent grid of G 1024-thread workgroups; workgroup w takes the
-// contiguous tile range [w*ntiles/G, (w+1)*ntiles/G).  Both kernels of a side
-// run with the same G, so they see the same rows per workgroup.
+// rows that pass the side's predicate go to partition radixHash(key) & (P-1),
+// P = a.P a power of two chosen per query (a run-time value: one module serves
+// every partition count).  A persistent grid of G 1024-thread workgroups;
+// workgroup w takes the contiguous tile range [w*ntiles/G, (w+1)*ntiles/G).
+// Both kernels of a side run with the same G, so they see the same rows per
+// workgroup.
 //   vq_part_count:   counts[p*G + w] = workgroup w's rows of partition p
-//                    (an LDS histogram)
+//                    (an LDS histogram in dynamic LDS of P words when the
+//                    launch provides it, else global atomics)
 //   vq_part_scatter: workgroup w's run of partition p starts at
 //                    pstart[p] + counts[p*G + w] (k_part_runs, k_part_starts);
 //                    each row goes there as (key[, row id, values of the
@@ -550,12 +552,16 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
 // every consumer aggregates with order-independent integer updates.
 // VQ_PB = 0 (partition counts too large for LDS buffers) writes each row
 // straight to its position, claimed from the workgroup's cursors in LDS (or,
-// past 16384 partitions, in its own column of counts).
+// when the launch gives no LDS for them, in its own column of counts).
+// Dynamic LDS layouts (bytes from the host, part_lds_bytes):
+//   count:            hist u32[P]                       (or none)
+//   scatter, PB > 0:  fill u32[P] | cur u32[P] | key i64[P*PB]
+//                     [| row u32[P*PB] | val u64[NV][P*PB]]   (probe side)
+//   scatter, PB = 0:  cur u32[P]                        (or none)
 // ---------------------------------------------------------------------------
 namespace vqj {
 
 constexpr int kPB = 1024;  // threads per partitioning workgroup
-constexpr int kP = VQ_P;
 constexpr int kBuf = VQ_PB;
 
 __device__ __forceinline__ u64 radix_hash(u64 x) {
@@ -620,6 +626,7 @@ struct PRows {
 template <int U, bool kVals>
 __device__ __forceinline__ void prows_from(const Args& a, i64 base, i64 rend,
                                            const i64 (&raw)[VQ_NS][U], PRows<U>& q) {
+  const u64 mask = (u64)a.P - 1;
 #pragma unroll
   for (int u = 0; u < U; u++) {
     i64 row[VQ_NS];
@@ -628,7 +635,7 @@ __device__ __forceinline__ void prows_from(const Args& a, i64 base, i64 rend,
     const i64 r = base + u * kPB + threadIdx.x;
     q.live[u] = r < rend && pass_all(a, row, seq<VQ_NT>{});
     q.key[u] = row[VQ_KEYSLOT];
-    q.part[u] = (u32)(radix_hash((u64)q.key[u]) & (u64)(kP - 1));
+    q.part[u] = (u32)(radix_hash((u64)q.key[u]) & mask);
     q.row[u] = (u32)r;
     if constexpr (kVals && VQ_PROBE && VQ_NV > 0) values_of(a, row, q.val[u], seq<VQ_NV>{});
   }
@@ -664,16 +671,17 @@ __device__ __forceinline__ void put_vals(const Args& a, u32 pos, const u64* v, S
 }  // namespace vqj
 
 extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
+  extern __shared__ u32 dyn[];
   constexpr int U = VQ_U;
+  const int P = a.P;
+  const bool lds = a.lds_words >= (u32)P;
   const i64 t0 = (i64)blockIdx.x * a.ntiles / gridDim.x;
   const i64 t1 = (i64)(blockIdx.x + 1) * a.ntiles / gridDim.x;
   const i64 G = gridDim.x;
-#if VQ_P <= 32768
-  __shared__ u32 hist[kP];
-  for (int p = threadIdx.x; p < kP; p += kPB) hist[p] = 0;
-#else
-  for (int p = threadIdx.x; p < kP; p += kPB) a.counts[p * G + blockIdx.x] = 0;
-#endif
+  for (int p = threadIdx.x; p < P; p += kPB) {
+    if (lds) dyn[p] = 0;
+    else a.counts[p * G + blockIdx.x] = 0;
+  }
   __syncthreads();
   for (i64 t = t0; t < t1; t++) {
     i64 row0, rend;
@@ -684,18 +692,14 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (!q.live[u]) continue;
-#if VQ_P <= 32768
-        atomicAdd(&hist[q.part[u]], 1u);
-#else
-        atomicAdd(&a.counts[q.part[u] * G + blockIdx.x], 1u);  // too many partitions for LDS
-#endif
+        if (lds) atomicAdd(&dyn[q.part[u]], 1u);
+        else atomicAdd(&a.counts[q.part[u] * G + blockIdx.x], 1u);
       }
     }
   }
-#if VQ_P <= 32768
   __syncthreads();
-  for (int p = threadIdx.x; p < kP; p += kPB) a.counts[p * G + blockIdx.x] = hist[p];
-#endif
+  if (lds)
+    for (int p = threadIdx.x; p < P; p += kPB) a.counts[p * G + blockIdx.x] = dyn[p];
 }
 
 namespace vqj {
@@ -703,16 +707,24 @@ namespace vqj {
 #if VQ_PB > 0
 constexpr int kGrp = 64 / kBuf;  // partitions one wave flushes per step
 
+// The scatter's LDS buffers, carved from dynamic LDS for P partitions.
 struct PBuf {
-  u32 fill[kP];
-  u32 cur[kP];  // next position of the workgroup's run of each partition
-  i64 key[kP * kBuf];
-#if VQ_PROBE
-  u32 row[kP * kBuf];
-#if VQ_NV > 0
-  u64 val[VQ_NV][kP * kBuf];
-#endif
-#endif
+  u32* fill;
+  u32* cur;  // next position of the workgroup's run of each partition
+  i64* key;
+  u32* row;
+  u64* val;  // [NV][P * kBuf]
+  int P;
+  // word offsets: fill 0, cur P, key 2P (even, so 8-byte aligned), row
+  // 2P + 2P*kBuf, val after row rounded up to an even word
+  __device__ __forceinline__ void carve(u32* dyn, int p) {
+    P = p;
+    fill = dyn;
+    cur = dyn + P;
+    key = (i64*)(dyn + 2 * P);
+    row = dyn + 2 * P + 2 * P * kBuf;
+    val = (u64*)(dyn + 2 * P + 2 * P * kBuf + ((P * kBuf + 1) & ~1));
+  }
 };
 
 // Writes the buffers of the partitions that are full (kTail = false) or hold
@@ -723,9 +735,10 @@ template <bool kTail>
 __device__ __forceinline__ void flush(const Args& a, PBuf& b) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int e = lane % kBuf;
-  for (int p0 = wv * kGrp; p0 < kP; p0 += (kPB / 64) * kGrp) {
+  const int P = b.P;
+  for (int p0 = wv * kGrp; p0 < P; p0 += (kPB / 64) * kGrp) {
     const int p = p0 + lane / kBuf;
-    const u32 f = p < kP ? b.fill[p] : 0;
+    const u32 f = p < P ? b.fill[p] : 0;
     const u32 n = kTail ? (f < (u32)kBuf ? f : (u32)kBuf) : (f >= (u32)kBuf ? (u32)kBuf : 0);
     if (n == 0) continue;
     const u32 g = b.cur[p];
@@ -739,7 +752,7 @@ __device__ __forceinline__ void flush(const Args& a, PBuf& b) {
 #if VQ_NV > 0
       u64 v[VQ_NV];
 #pragma unroll
-      for (int j = 0; j < VQ_NV; j++) v[j] = b.val[j][i];
+      for (int j = 0; j < VQ_NV; j++) v[j] = b.val[(i64)j * P * kBuf + i];
       put_vals(a, pos, v, seq<VQ_NV>{});
 #endif
 #endif
@@ -756,27 +769,29 @@ __device__ __forceinline__ void flush(const Args& a, PBuf& b) {
 }  // namespace vqj
 
 extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a) {
+  extern __shared__ u32 dyn[];
   constexpr int U = VQ_U;
+  const int P = a.P;
   const i64 t0 = (i64)blockIdx.x * a.ntiles / gridDim.x;
   const i64 t1 = (i64)(blockIdx.x + 1) * a.ntiles / gridDim.x;
   const i64 G = gridDim.x;
 #if VQ_PB > 0
-  __shared__ PBuf b;
-  for (int p = threadIdx.x; p < kP; p += kPB) {
+  PBuf b;
+  b.carve(dyn, P);
+  for (int p = threadIdx.x; p < P; p += kPB) {
     b.fill[p] = 0;
     b.cur[p] = a.pstart[p] + a.counts[p * G + blockIdx.x];
   }
-  __syncthreads();
-#elif VQ_P <= 16384
-  // this workgroup's own cursors, in LDS
-  __shared__ u32 cur[kP];
-  for (int p = threadIdx.x; p < kP; p += kPB) cur[p] = a.pstart[p] + a.counts[p * G + blockIdx.x];
-  __syncthreads();
 #else
-  // this workgroup's own cursors, advanced in place (too many for LDS)
-  for (int p = threadIdx.x; p < kP; p += kPB) a.counts[p * G + blockIdx.x] += a.pstart[p];
-  __syncthreads();
+  // this workgroup's own cursors, in LDS when the launch gives room for them,
+  // else advanced in place in its own column of counts
+  const bool lds = a.lds_words >= (u32)P;
+  for (int p = threadIdx.x; p < P; p += kPB) {
+    if (lds) dyn[p] = a.pstart[p] + a.counts[p * G + blockIdx.x];
+    else a.counts[p * G + blockIdx.x] += a.pstart[p];
+  }
 #endif
+  __syncthreads();
   // one chunk's column loads in flight while the previous chunk is placed
   Chunks<U> ck{t0 - 1, t1, 0, 0};
   ck.base = -(i64)kPB * U;  // next() steps onto tile t0
@@ -789,58 +804,52 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a)
     prows_from<U, true>(a, base, rend, raw, q);
     have = ck.next(a);
     if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
-    {
 #if VQ_PB > 0
-      // insert what fits; flush full buffers; again until every row is in
-      for (;;) {
-        bool pend = false;
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          if (!q.live[u]) continue;
-          const u32 s = atomicAdd(&b.fill[q.part[u]], 1u);
-          if (s < (u32)kBuf) {
-            const int i = q.part[u] * kBuf + s;
-            b.key[i] = q.key[u];
-#if VQ_PROBE
-            b.row[i] = q.row[u];
-#if VQ_NV > 0
-#pragma unroll
-            for (int j = 0; j < VQ_NV; j++) b.val[j][i] = q.val[u][j];
-#endif
-#endif
-            q.live[u] = false;
-          } else {
-            pend = true;
-          }
-        }
-        __syncthreads();
-        flush<false>(a, b);
-        if (!__syncthreads_or(pend)) break;
-      }
-#else
+    // insert what fits; flush full buffers; again until every row is in
+    for (;;) {
+      bool pend = false;
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (!q.live[u]) continue;
-        const u32 p = q.part[u];
-#if VQ_P <= 16384
-        const u32 pos = atomicAdd(&cur[p], 1u);
-#else
-        const u32 pos = atomicAdd(&a.counts[p * G + blockIdx.x], 1u);
-#endif
-        if (pos >= a.pstart[p + 1]) {
-          atomicOr(a.perr, 1u);
-          continue;
-        }
-        a.keys[pos] = q.key[u];
+        const u32 s = atomicAdd(&b.fill[q.part[u]], 1u);
+        if (s < (u32)kBuf) {
+          const int i = q.part[u] * kBuf + s;
+          b.key[i] = q.key[u];
 #if VQ_PROBE
-        a.rows[pos] = q.row[u];
+          b.row[i] = q.row[u];
 #if VQ_NV > 0
-        put_vals(a, pos, q.val[u], seq<VQ_NV>{});
+#pragma unroll
+          for (int j = 0; j < VQ_NV; j++) b.val[(i64)j * P * kBuf + i] = q.val[u][j];
 #endif
 #endif
+          q.live[u] = false;
+        } else {
+          pend = true;
+        }
       }
+      __syncthreads();
+      flush<false>(a, b);
+      if (!__syncthreads_or(pend)) break;
+    }
+#else
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!q.live[u]) continue;
+      const u32 p = q.part[u];
+      const u32 pos = lds ? atomicAdd(&dyn[p], 1u) : atomicAdd(&a.counts[p * G + blockIdx.x], 1u);
+      if (pos >= a.pstart[p + 1]) {
+        atomicOr(a.perr, 1u);
+        continue;
+      }
+      a.keys[pos] = q.key[u];
+#if VQ_PROBE
+      a.rows[pos] = q.row[u];
+#if VQ_NV > 0
+      put_vals(a, pos, q.val[u], seq<VQ_NV>{});
+#endif
 #endif
     }
+#endif
   }
 #if VQ_PB > 0
   __syncthreads();

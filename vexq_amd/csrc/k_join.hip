@@ -658,12 +658,23 @@ size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords) {
   return ns * (8 + 4) + (size_t)pool * (4 + 4 + 4 + 4 + 8 * (size_t)nwords);
 }
 
+uint32_t part_scatter_lds(int32_t P, int32_t pbuf, int32_t entry) {
+  if (pbuf <= 0) return P <= 16384 ? (uint32_t)P * 4 : 0;
+  // fill + cur words and the keys; on the probe side (entries beyond the
+  // 8-byte key: a 4-byte row id and 8-byte values) the row ids, rounded up to
+  // an even word count, and the values
+  const int64_t n = (int64_t)P * pbuf;
+  int64_t words = 2 * (int64_t)P + 2 * n;
+  if (entry > 8) words += ((n + 1) & ~(int64_t)1) + n * ((entry - 12) / 4);
+  return (uint32_t)(words * 4);
+}
+
 int32_t part_buffer_entries(int32_t P, int32_t entry, int32_t lds_max) {
   int32_t want = 16;
   if (const char* e = getenv("VQ_PART_BUF")) want = atoi(e);
-  const int64_t limit = (int64_t)lds_max - 2048;
+  const int64_t limit = (int64_t)lds_max - 1024;
   for (int32_t b = 16; b >= 1; b >>= 1)
-    if (b <= want && (int64_t)P * 8 + (int64_t)P * b * entry <= limit) return b;
+    if (b <= want && (int64_t)part_scatter_lds(P, b, entry) <= limit) return b;
   return 0;
 }
 

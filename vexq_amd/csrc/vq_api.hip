@@ -1286,8 +1286,15 @@ int join_setup(JoinJob& j) {
   while (j.cap_max > 8 && join_lds_for(j.cap_max, 64, nw) > (size_t)j.lds_max) j.cap_max /= 2;
   j.bits = j.radix_bits;
   if (j.bits < 0) {
+    // enough partitions for the keys to fit their tables, and enough for the
+    // work: one workgroup per partition takes at most ~16K probe rows
+    // (a handful of keys over millions of rows still spreads over the CUs)
     j.bits = 0;
     while (j.bits < 16 && (double)j.est / (1 << j.bits) > kLoad * j.cap_max) j.bits++;
+    const int64_t prows = j.probe->nrows;
+    int pbits = 0;
+    while (pbits < 10 && (prows >> pbits) > 16384) pbits++;
+    j.bits = std::max(j.bits, pbits);
     j.bits = std::max(0, std::min(16, j.bits + j.tune.bits_delta));
   }
   j.dev_topk = j.topk > 0 && j.topk <= 16;
@@ -1459,7 +1466,20 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   // the host top-k, for k above the device limit, keeps the stable order)
   std::vector<std::pair<uint64_t, uint32_t>> ord(nrec);
   for (uint32_t i = 0; i < nrec; i++) ord[i] = {dev_topk ? (uint64_t)i : recs[(size_t)i * RW + 2], i};
-  if (!dev_topk) std::sort(ord.begin(), ord.end());
+  if (!dev_topk) {
+    // first rows are distinct row ids below the probe table's size: place
+    // each record at its first row, then read the places in row order
+    const int64_t nr = j.probe->nrows;
+    if (nrec > 4096 && nr <= 64 * (int64_t)nrec) {
+      std::vector<uint32_t> at((size_t)nr, 0xFFFFFFFFu);
+      for (uint32_t i = 0; i < nrec; i++) at[(size_t)ord[i].first] = i;
+      uint32_t k = 0;
+      for (int64_t r = 0; r < nr; r++)
+        if (at[(size_t)r] != 0xFFFFFFFFu) ord[k++] = {(uint64_t)r, at[(size_t)r]};
+    } else {
+      std::sort(ord.begin(), ord.end());
+    }
+  }
   if (j.topk > 0 && !dev_topk) {
     auto key = [&](uint32_t i) { return naggs > 0 ? (int64_t)recs[(size_t)i * RW + 3] : (int64_t)0; };
     std::stable_sort(ord.begin(), ord.end(), [&](const std::pair<uint64_t, uint32_t>& x,

@@ -245,7 +245,7 @@ struct JitArgs {
   uint64_t* vals;
   int64_t vstride;
   int32_t P;
-  int32_t pad_;
+  uint32_t lds_words;  // dynamic LDS of the launch, in 4-byte words
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -256,6 +256,8 @@ struct PartLaunch {
   hipFunction_t count = nullptr;
   hipFunction_t scatter = nullptr;
   int32_t grid = 0;
+  uint32_t count_lds = 0;    // dynamic LDS bytes of each kernel
+  uint32_t scatter_lds = 0;
   JitArgs args;
 };
 int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
@@ -267,6 +269,9 @@ hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, 
 // LDS buffer entries per partition for a scatter with entries of `entry`
 // bytes: the largest of 16, 8, 4, 2, 1 that fits, else 0.
 int32_t part_buffer_entries(int32_t P, int32_t entry, int32_t lds_max);
+// Dynamic LDS bytes of the scatter: buffers of pbuf entries of `entry` bytes
+// per partition, or the cursors alone when pbuf is 0 (0 when they do not fit).
+uint32_t part_scatter_lds(int32_t P, int32_t pbuf, int32_t entry);
 
 // Lets the join kernel launch with up to `bytes` of dynamic LDS (once per
 // process and size; outside any graph capture).

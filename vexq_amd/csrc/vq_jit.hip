@@ -80,7 +80,7 @@ Flat flatten_terms(const QDesc& d) {
 int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
 
 std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slot = 0,
-                     int nv = 0, int probe = 0, int P = 0, int pbuf = 0) {
+                     int nv = 0, int probe = 0, int pbuf = 0) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
@@ -108,10 +108,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_KEYSLOT " + std::to_string(key_slot) + "\n";
   s += "#define VQ_NV " + std::to_string(nv) + "\n";
   s += "#define VQ_PROBE " + std::to_string(probe) + "\n";
-  if (kernels & 4) {
-    s += "#define VQ_P " + std::to_string(P) + "\n";
-    s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
-  }
+  if (kernels & 4) s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
@@ -244,10 +241,13 @@ int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, 
   int nv = 0;
   const QDesc d = side_desc(sd, jd, &nv);
   const Flat f = flatten_terms(d);
-  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, P, pbuf), err);
+  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, pbuf), err);
   if (!m) return -1;
   out->count = m->part_count;
   out->scatter = m->part_scatter;
+  const int32_t entry = jd ? 8 + 4 + 8 * nv : 8;
+  out->count_lds = P <= 16384 ? (uint32_t)P * 4 : 0;
+  out->scatter_lds = part_scatter_lds(P, pbuf, entry);
   static int cus = 0;
   if (!cus) {
     int dev = 0;
@@ -256,7 +256,8 @@ int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, 
     if (cus <= 0) cus = 256;
   }
   int per_cu = 0;
-  hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->part_scatter, 1024, 0);
+  hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->part_scatter, 1024,
+                                                     out->scatter_lds);
   if (per_cu <= 0) per_cu = 1;
   const int64_t g = (int64_t)cus * per_cu;
   out->grid = (int32_t)std::max<int64_t>(1, d.ntiles < g ? d.ntiles : g);
@@ -276,7 +277,9 @@ hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, 
   a.vals = pb.vals;
   a.vstride = pb.vstride;
   a.P = P;
-  return launch(count ? pl.count : pl.scatter, (unsigned)pl.grid, 0, s, a, 1024);
+  const uint32_t lds = count ? pl.count_lds : pl.scatter_lds;
+  a.lds_words = lds / 4;
+  return launch(count ? pl.count : pl.scatter, (unsigned)pl.grid, lds, s, a, 1024);
 }
 
 int jit_dense_grid(const QDesc& d, std::string* err) {
