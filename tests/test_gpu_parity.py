@@ -282,6 +282,13 @@ def test_join_duplicates_extremes_empty(gpu_ctx):
     (500_000, 100_000, abi.VQ_I64, 0.0),
     (400_000, 20_000, abi.VQ_I64, 1.1),
     (1_500_000, 1_000_000, abi.VQ_I64, 0.0),
+    # few keys over many rows: per-workgroup partials
+    (3_000_000, 5, abi.VQ_I64, 0.0),
+    (2_000_000, 1, abi.VQ_U16, 0.0),
+    (800_000, 70, abi.VQ_I64, 1.1),
+    (1_000_000, 256, abi.VQ_U16, 0.0),
+    # one key more than the partials hold: partitions
+    (1_000_000, 257, abi.VQ_I64, 0.0),
 ])
 def test_agg_hash_matches_oracle(gpu_ctx, nrows, groups, key_type, skew):
     _print_case("agg_hash", nrows=nrows, groups=groups, key_type=key_type, skew=skew)
@@ -298,6 +305,25 @@ def test_agg_hash_matches_oracle(gpu_ctx, nrows, groups, key_type, skew):
     assert same_bits(got, again)
     naive = O.agg_hash(t, pred, key, aggs, exact=False)
     assert_agg_equal(got, naive, aggs, rel=1e-9)
+
+
+def test_agg_hash_path_by_key_count(gpu_ctx):
+    """Up to 256 keys aggregate through per-workgroup partials (no
+    partitions); more go through the partitions, on the first query already
+    and directly on later ones.  Every run gives the same bits."""
+    _print_case("agg_hash_paths")
+    for groups, local in ((200, True), (300, False)):
+        t = datagen.keyed_table(400_000, groups, abi.VQ_I64, 0.0, seed=groups)
+        pred, key, aggs = datagen.q_keyed(t)
+        dt = gpu_ctx.upload(t)
+        try:
+            runs = [gpu_ctx.agg_hash(dt, pred, key, aggs) for _ in range(3)]
+        finally:
+            dt.close()
+        for r in runs:
+            assert (r.stats.partitions == 0) == local, (groups, r.stats.partitions)
+            assert same_bits(r, runs[0])
+        assert_agg_equal(runs[0], O.agg_hash(t, pred, key, aggs, exact=True), aggs)
 
 
 def test_agg_hash_extreme_keys_and_empty(gpu_ctx):

@@ -33,12 +33,16 @@ FLAGS = [
 
 
 def embed_template() -> None:
-    """Writes csrc/jit_template_src.h: jit_template.hip as a C++ raw string."""
+    """Writes csrc/jit_template_src.h: fixchunks.h and jit_template.hip as C++
+    raw strings (the run-time source is the prologue, then both)."""
     with open(os.path.join(CSRC, JIT_TEMPLATE)) as f:
         text = f.read()
-    assert ")VQJIT\"" not in text
-    out = ("// generated by vexq_amd/build.py from jit_template.hip; do not edit\n"
+    with open(os.path.join(CSRC, "fixchunks.h")) as f:
+        chunks = f.read()
+    assert ")VQJIT\"" not in text and ")VQJIT\"" not in chunks
+    out = ("// generated by vexq_amd/build.py from fixchunks.h and jit_template.hip; do not edit\n"
            "#pragma once\n"
+           f"static const char kFixChunksSrc[] = R\"VQJIT({chunks})VQJIT\";\n"
            f"static const char kJitTemplateSrc[] = R\"VQJIT({text})VQJIT\";\n")
     path = os.path.join(CSRC, JIT_HEADER)
     if not os.path.exists(path) or open(path).read() != out:

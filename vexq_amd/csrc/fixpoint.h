@@ -24,18 +24,12 @@
 
 #include <cmath>
 
-#ifdef __HIPCC__
-#define VQ_FX_HD __host__ __device__ __forceinline__
-#else
-#define VQ_FX_HD inline
-#endif
+#include "fixchunks.h"
 
 namespace vq {
 
-struct FixScale {
-  int32_t lsb;  // weight of limb 0's least significant bit: 2^lsb
-  int32_t w;    // bits per chunk
-};
+using vqfx::FixScale;
+using vqfx::fix_chunks;
 
 // 256-bit two's complement integer, least significant word first.
 struct U256 {
@@ -123,49 +117,6 @@ VQ_FX_HD int u256_top(const U256& a) {
   if (a.w1) return 64 + 63 - __builtin_clzll(a.w1);
   if (a.w0) return 63 - __builtin_clzll(a.w0);
   return -1;
-}
-
-// The three w-bit chunks of float64 t on scale fs, each carrying t's sign:
-// t = (c0 + c1 2^w + c2 2^2w) 2^lsb for the bits of t at or above 2^lsb.
-// Returns false for a zero addend (nothing to add).
-VQ_FX_HD bool fix_chunks(double t, FixScale fs, int64_t& c0, int64_t& c1, int64_t& c2) {
-  union {
-    double d;
-    uint64_t u;
-  } cv;
-  cv.d = t;
-  const uint64_t b = cv.u;
-  const int ex = (int)((b >> 52) & 0x7FF);
-  uint64_t m = b & ((1ULL << 52) - 1);
-  if (ex == 0 && m == 0) return false;
-  int e;
-  if (ex == 0) {
-    e = -1074;
-  } else {
-    m |= 1ULL << 52;
-    e = ex - 1075;
-  }
-  const int sh = e - fs.lsb;  // |t| = m * 2^sh units of 2^lsb
-  uint64_t lo, hi;
-  if (sh >= 0) {
-    if (sh >= 128) return false;  // outside the planned range (the planner's bound excludes it)
-    lo = sh >= 64 ? 0 : m << sh;
-    hi = sh == 0 ? 0 : (sh >= 64 ? m << (sh - 64) : m >> (64 - sh));
-  } else {
-    if (-sh >= 64) return false;
-    lo = m >> (-sh);
-    hi = 0;
-  }
-  const int w = fs.w;
-  const uint64_t wm = (1ULL << w) - 1;
-  const uint64_t u0 = lo & wm;
-  const uint64_t u1 = ((lo >> w) | (hi << (64 - w))) & wm;
-  const uint64_t u2 = (2 * w >= 64 ? hi >> (2 * w - 64) : (lo >> (2 * w)) | (hi << (64 - 2 * w))) & wm;
-  const bool neg = (b >> 63) != 0;
-  c0 = neg ? -(int64_t)u0 : (int64_t)u0;
-  c1 = neg ? -(int64_t)u1 : (int64_t)u1;
-  c2 = neg ? -(int64_t)u2 : (int64_t)u2;
-  return true;
 }
 
 // (limb0 + limb1 2^w + limb2 2^2w) * 2^lsb * mult, rounded once to the

@@ -13,6 +13,7 @@
 //   vq_dense       per-wave partial aggregates (persistent grid, prefetch)
 //   vq_filter_bits pass bits + per-tile counts for the selection vector
 //   vq_part_count / vq_part_scatter   radix partitioning of a join side
+//   vq_hash_local  GROUP BY of few keys into per-workgroup partials
 #ifndef VQ_JIT_PROLOGUE
 // Stand-alone build check (hipcc -c): the 8-aggregate GROUP BY shape.
 #define VQ_JIT_PROLOGUE 1
@@ -24,7 +25,7 @@
 #define VQ_KEYSLOT 2
 #define VQ_NV 5
 #define VQ_PROBE 1
-#define VQ_KERNELS 7
+#define VQ_KERNELS 15
 #define VQ_PB 8
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
@@ -38,6 +39,11 @@ static constexpr int kFS[8][4] = {{2}, {3}, {3, 4}, {3, 4, 5}, {4}};
 static constexpr int kFO[8][4] = {{0}, {0}, {0, 1}, {0, 1, 0}, {0}};
 static constexpr int kFC[8][4] = {{0}, {0}, {0, 2}, {0, 2, 2}, {0}};
 #include <hip/hip_runtime.h>
+#endif
+// exact fixed-point chunks of a float64 (the host prepends this header's
+// text to the run-time source; the stand-alone check includes it)
+#ifndef VQ_FIXCHUNKS_H
+#include "fixchunks.h"
 #endif
 
 // No headers: hipRTC compiles this with its built-in runtime declarations only.
@@ -98,6 +104,18 @@ struct Args {
   i64 vstride;
   i32 P;
   u32 lds_words;      // dynamic LDS the launch provides, in 4-byte words
+  // GROUP BY through per-workgroup partials (vq_hash_local)
+  const i64* lkeys;   // [lcap] the distinct keys (kI64Min = empty slot)
+  u32* lflags;        // the pipeline's flags: [0] overflow bits, [2] keys,
+                      // [10] key INT64_MIN present
+  i64* lgkey;         // [groups] the key of each group index
+  i32 lcap;           // table slots (a power of two)
+  i32 lnw;            // accumulator words per group
+  i32 vop[8];         // per aggregate value: 0 float sum, 1 int sum,
+                      // 2/3 float min/max, 4/5 int min/max
+  i32 vword[8];       // the value's first accumulator word
+  i32 vlsb[8];        // float sums: the fixed-point scale
+  i32 vw[8];
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -527,7 +545,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
 }
 #endif
 
-#if VQ_KERNELS & 4
+#if VQ_KERNELS & 12
 // ---------------------------------------------------------------------------
 // Radix partitioning of one join side (exec/parallel_join.go:322-347,422-550):
 // rows that pass the side's predicate go to partition radixHash(key) & (P-1),
@@ -669,7 +687,9 @@ __device__ __forceinline__ void put_vals(const Args& a, u32 pos, const u64* v, S
 }
 
 }  // namespace vqj
+#endif
 
+#if VQ_KERNELS & 4
 extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
   extern __shared__ u32 dyn[];
   constexpr int U = VQ_U;
@@ -855,5 +875,252 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a)
   __syncthreads();
   flush<true>(a, b);
 #endif
+}
+#endif
+
+#if VQ_KERNELS & 8
+// ---------------------------------------------------------------------------
+// GROUP BY of at most a few hundred keys (HashAggregate's int-key path and
+// ParallelHashAggregate's partials + merge, exec/agg_intkey.go:206-361,
+// exec/parallel.go:18-314): every workgroup takes an even share of the rows
+// and aggregates them into its own accumulators in LDS; k_local_merge then
+// combines the workgroups' partials per group.  The distinct keys were
+// collected beforehand (k_local_keys) into one table that every workgroup
+// copies, so a group has the same index in every workgroup.
+//
+// LDS: keys i64[lcap] | index u32[lcap] | acc u64[groups][lnw][S] |
+//      cnt u32[groups][S] | first u32[groups][S]
+// S copies of the accumulators, as many as fit (at most one per thread):
+// thread t updates copy t mod S, so the lanes of a wave that share a group
+// spread their atomics over up to S addresses (a handful of groups would
+// otherwise queue every lane of a wave on one word).  Every update is an
+// integer add, minimum or maximum, so neither the copy nor the order changes
+// the result.  Partials: part[wg][group][lnw + 2] (words, count, first row).
+// ---------------------------------------------------------------------------
+namespace vqj {
+
+constexpr int kLT = 512;  // threads per workgroup
+constexpr u32 kLocalOverflow = 4u;
+
+template <int U, int... C>
+__device__ __forceinline__ void lload_one(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u,
+                                          Seq<C...>) {
+  ((raw[C][u] = ld<kST[C]>(a.col[C], r)), ...);
+}
+
+// Rows base + u*kLT + tid, clamped into the tile.
+template <int U>
+__device__ __forceinline__ void lload_rows(const Args& a, i64 base, i64 rend,
+                                           i64 (&raw)[VQ_NS][U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    i64 r = base + u * kLT + threadIdx.x;
+    r = r < rend ? r : rend - 1;
+    lload_one(a, r, raw, u, seq<VQ_NS>{});
+  }
+}
+
+// One aggregate value of one row into copy c of group g.
+template <int J>
+__device__ __forceinline__ void local_fold(const Args& a, u64* acc, u32 S, u32 g, u32 c, u64 v) {
+  const int op = a.vop[J];
+  u64* w = acc + ((u64)g * a.lnw + a.vword[J]) * S + c;
+  if (op == 0) {
+    vqfx::FixScale fs;
+    fs.lsb = a.vlsb[J];
+    fs.w = a.vw[J];
+    i64 c0, c1, c2;
+    if (!vqfx::fix_chunks(as_f64((i64)v), fs, c0, c1, c2)) return;
+    if (c0) atomicAdd(w, (u64)c0);
+    if (c1) atomicAdd(w + S, (u64)c1);
+    if (c2) atomicAdd(w + 2 * S, (u64)c2);
+  } else if (op == 1) {
+    atomicAdd(w, v);
+  } else if (op == 2 || op == 3) {
+    const double d = as_f64((i64)v);
+    if (d != d) return;  // NaN never wins a compare
+    if (op == 2) atomicMin((long long*)w, (long long)f64_ord(d));
+    else atomicMax((long long*)w, (long long)f64_ord(d));
+  } else if (op == 4) {
+    atomicMin((long long*)w, (long long)v);
+  } else {
+    atomicMax((long long*)w, (long long)v);
+  }
+}
+
+template <int... J>
+__device__ __forceinline__ void local_folds(const Args& a, u64* acc, u32 S, u32 g, u32 c,
+                                            const u64* v, Seq<J...>) {
+  (local_fold<J>(a, acc, S, g, c, v[J]), ...);
+}
+
+}  // namespace vqj
+
+extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
+  extern __shared__ u64 lds[];
+  __shared__ u32 s_scan[kLT];
+  __shared__ u8 s_wop[32];  // per accumulator word: 0 add, 1 min, 2 max
+  constexpr int U = VQ_U;
+  const int tid = threadIdx.x;
+  if (a.lflags[0] & kLocalOverflow) return;  // more keys than this path holds
+  const int cap = a.lcap;
+  const int lg = 31 - __clz(cap);
+  const u32 nk = a.lflags[2];
+  const bool has_min = a.lflags[10] != 0;
+  const int NW = a.lnw;
+  i64* tk = (i64*)lds;
+  u32* tix = (u32*)(tk + cap);
+  u64* acc = (u64*)(tix + cap + (cap & 1));
+  const u64 room = (u64)a.lds_words * 4 - (u64)cap * 12 - (cap & 1) * 4;
+  const u64 per_copy = (u64)nk * (NW * 8 + 8);
+  u32 S = kLT;
+  while (S > 1 && S * per_copy > room) S >>= 1;
+  if (S * per_copy > room) {  // (the host sizes LDS for its key limit)
+    if (tid == 0) atomicOr(&a.lflags[0], kLocalOverflow);
+    return;
+  }
+  u32* cnt = (u32*)(acc + (u64)nk * NW * S);
+  u32* first = cnt + (u64)nk * S;
+
+  // the key table and each occupied slot's index (its rank in slot order;
+  // key INT64_MIN, kept apart, takes the last index)
+  const int per = (cap + kLT - 1) / kLT;
+  const int s0 = tid * per < cap ? tid * per : cap;
+  const int s1 = s0 + per < cap ? s0 + per : cap;
+  u32 mine = 0;
+  for (int i = s0; i < s1; i++) {
+    const i64 k = a.lkeys[i];
+    tk[i] = k;
+    mine += k != kI64Min;
+  }
+  s_scan[tid] = mine;
+  if (tid < NW) {
+    u8 op = 0;
+    for (int j = 0; j < VQ_NV; j++) {
+      const int nw = a.vop[j] == 0 ? 3 : 1;
+      if (tid >= a.vword[j] && tid < a.vword[j] + nw)
+        op = (a.vop[j] == 2 || a.vop[j] == 4) ? 1 : (a.vop[j] == 3 || a.vop[j] == 5) ? 2 : 0;
+    }
+    s_wop[tid] = op;
+  }
+  __syncthreads();
+  for (int off = 1; off < kLT; off <<= 1) {
+    const u32 x = tid >= off ? s_scan[tid - off] : 0;
+    __syncthreads();
+    s_scan[tid] += x;
+    __syncthreads();
+  }
+  u32 r = s_scan[tid] - mine;
+  const u32 nreg = s_scan[kLT - 1];
+  for (int i = s0; i < s1; i++) {
+    if (tk[i] == kI64Min) {
+      tix[i] = 0xFFFFFFFFu;
+      continue;
+    }
+    if (blockIdx.x == 0) a.lgkey[r] = tk[i];
+    tix[i] = r++;
+  }
+  if (blockIdx.x == 0 && tid == 0 && has_min) a.lgkey[nreg] = kI64Min;
+  for (u64 x = tid; x < (u64)nk * NW * S; x += kLT) {
+    const int op = s_wop[(x / S) % NW];
+    acc[x] = op == 1 ? (u64)kI64Max : op == 2 ? (u64)kI64Min : 0;
+  }
+  for (u64 x = tid; x < (u64)nk * S; x += kLT) {
+    cnt[x] = 0;
+    first[x] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+
+  // this workgroup's tiles [t0, t1), kLT * U rows at a time, the next rows'
+  // loads in flight while the current ones are folded
+  const u32 c = (u32)tid & (S - 1);
+  const i64 t0 = (i64)blockIdx.x * a.ntiles / gridDim.x;
+  const i64 t1 = (i64)(blockIdx.x + 1) * a.ntiles / gridDim.x;
+  i64 t = t0 - 1, base = 0, rend = 0;
+  auto step = [&]() -> bool {
+    base += (i64)kLT * U;
+    while (base >= rend) {
+      if (++t >= t1) return false;
+      ptile(a, t, base, rend);
+    }
+    return true;
+  };
+  base = -(i64)kLT * U;
+  bool have = t0 < t1 && step();
+  i64 raw[VQ_NS][U];
+  if (have) lload_rows<U>(a, base, rend, raw);
+  while (have) {
+    i64 cur[VQ_NS][U];
+#pragma unroll
+    for (int q = 0; q < VQ_NS; q++)
+#pragma unroll
+      for (int u = 0; u < U; u++) cur[q][u] = raw[q][u];
+    const i64 cbase = base, cend = rend;
+    have = step();
+    if (have) lload_rows<U>(a, base, rend, raw);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      i64 row[VQ_NS];
+#pragma unroll
+      for (int q = 0; q < VQ_NS; q++) row[q] = cur[q][u];
+      const i64 rr = cbase + u * kLT + tid;
+      if (!(rr < cend && pass_all(a, row, seq<VQ_NT>{}))) continue;
+      const i64 key = row[VQ_KEYSLOT];
+      u32 g = 0xFFFFFFFFu;
+      if (key == kI64Min) {
+        if (has_min) g = nreg;
+      } else {
+        u32 i = (u32)(radix_hash((u64)key) >> (64 - lg));
+        for (int pr = 0; pr < cap; pr++) {
+          const i64 k = tk[i];
+          if (k == key) {
+            g = tix[i];
+            break;
+          }
+          if (k == kI64Min) break;
+          i = (i + 1) & (u32)(cap - 1);
+        }
+      }
+      if (g >= nk) {  // not collected: the key pass and this pass disagree
+        atomicOr(&a.lflags[0], kLocalOverflow);
+        continue;
+      }
+      atomicAdd(&cnt[(u64)g * S + c], 1u);
+      atomicMin(&first[(u64)g * S + c], (u32)rr);
+#if VQ_NV > 0
+      u64 v[VQ_NV];
+      values_of(a, row, v, seq<VQ_NV>{});
+      local_folds(a, acc, S, g, c, v, seq<VQ_NV>{});
+#endif
+    }
+  }
+  __syncthreads();
+  // fold the copies into this workgroup's partials
+  const int NI = NW + 2;
+  u64* out = a.part + (u64)blockIdx.x * nk * NI;
+  for (u64 x = tid; x < (u64)nk * NI; x += kLT) {
+    const u32 g = (u32)(x / NI);
+    const int w = (int)(x % NI);
+    u64 v;
+    if (w < NW) {
+      const int op = s_wop[w];
+      const u64* p = acc + ((u64)g * NW + w) * S;
+      v = p[0];
+      for (u32 k = 1; k < S; k++) {
+        const u64 o = p[k];
+        if (op == 0) v += o;
+        else if (op == 1) v = (i64)o < (i64)v ? o : v;
+        else v = (i64)o > (i64)v ? o : v;
+      }
+    } else if (w == NW) {
+      v = 0;
+      for (u32 k = 0; k < S; k++) v += cnt[(u64)g * S + k];
+    } else {
+      u32 f = 0xFFFFFFFFu;
+      for (u32 k = 0; k < S; k++) f = first[(u64)g * S + k] < f ? first[(u64)g * S + k] : f;
+      v = f;
+    }
+    out[x] = v;
+  }
 }
 #endif

@@ -299,18 +299,25 @@ __device__ __noinline__ KItem wave_topk_two(KItem x0, KItem x1, int k) {
   return mine;
 }
 
+__device__ __forceinline__ unsigned long long lds_load_key(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // One 1024-thread workgroup per partition.  LDS:
-//   table  keys u64[cap+1] | acc u64[pool*nwords] | tag u32[cap+1]
-//   pool   slot u32[pool] | cnt u32[pool] | first u32[pool] | mult u32[pool]
+//   table  keys u64[cap+1] | acc u64[reps][pool][nwords] | tag u32[cap+1]
+//   pool   slot u32[pool] | mult u32[pool] | cnt u32[reps][pool] |
+//          first u32[reps][pool]
 // Slot `cap` holds the key INT64_MIN, which doubles as the empty marker of the
 // other slots.  A slot's tag counts its build rows (the key's multiplicity)
 // while the table is built.  Accumulators exist only for build keys a probe
-// row matches: the first match of a slot takes the next pool entry, moves the
-// multiplicity there and replaces the tag with kClaimed | entry (a racing
-// second claimant loses the compare-and-swap on the tag, uses the winner's
-// entry and leaves its own unused).  flags: see JoinBufs.
+// row matches: the first match of a slot marks the tag pending by a
+// compare-and-swap, takes the next pool entry, moves the multiplicity there
+// and sets the tag to kClaimed | entry; lanes that lose the swap wait for
+// the entry.  Each wave updates replica (wave mod reps) of the accumulators;
+// the replicas are folded before the results are read.  flags: see JoinBufs.
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kClaimed = 0x80000000u;
+constexpr uint32_t kPending = 0xFFFFFFFFu;  // a lane is taking the slot's pool entry
 constexpr int kPoolPerThread = 2;  // pool <= kPoolPerThread * kJoinThreads
 
 // Eight waves per SIMD (at most 64 VGPRs): two workgroups per CU when the
@@ -320,18 +327,22 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   extern __shared__ unsigned long long smem[];
   __shared__ uint32_t s_fail, s_nrec, s_nkeys, s_base, s_pool, s_last;
   __shared__ KItem s_cand[kWaves * kMaxK];
+  __shared__ uint8_t s_wop[3 * VQ_MAX_AGGS];  // per accumulator word: 0 add, 1 min, 2 max
   const int p = blockIdx.x;
-  const int cap = plan.cap, ns = cap + 1, NP = plan.pool;
+  const int cap = plan.cap, ns = cap + 1, NP = plan.pool, R = plan.reps;
   const int NW = jd.nwords;
   uint32_t* flags = B.flags;
   unsigned long long* keys = smem;
-  unsigned long long* acc = keys + ns;                       // [NP * NW]
-  uint32_t* tag = (uint32_t*)(acc + (size_t)NP * NW);
+  unsigned long long* acc = keys + ns;                       // [R][NP][NW]
+  uint32_t* tag = (uint32_t*)(acc + (size_t)R * NP * NW);
   uint32_t* pslot = tag + ns;
-  uint32_t* cnt = pslot + NP;
-  uint32_t* first = cnt + NP;
-  uint32_t* pmult = first + NP;
+  uint32_t* pmult = pslot + NP;
+  uint32_t* cnt = pmult + NP;                                // [R][NP]
+  uint32_t* first = cnt + (size_t)R * NP;                    // [R][NP]
   const int lg = __ffs(cap) - 1;  // log2(cap)
+  // wave w updates accumulator replica w mod R: with few hot keys the
+  // replicas spread what would be one LDS address's atomics over R addresses
+  const int myrep = (int)(threadIdx.x >> 6) & (R - 1);
   if (threadIdx.x == 0) {
     s_fail = 0;
     s_nrec = 0;
@@ -342,21 +353,23 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     keys[i] = (unsigned long long)kEmptyKey;
     tag[i] = 0;
   }
-  for (int i = threadIdx.x; i < NP; i += kJoinThreads) {
+  for (int i = threadIdx.x; i < R * NP; i += kJoinThreads) {
     cnt[i] = 0;
     first[i] = kNone;
   }
-  for (int i = threadIdx.x; i < NP * NW; i += kJoinThreads) {
-    const int w = i % NW;
-    unsigned long long init = 0;
+  if (threadIdx.x < NW) {
+    uint8_t op = 0;
     for (int j = 0; j < jd.naggs; j++) {
       const JAgg& a = jd.agg[j];
-      if (w >= a.word0 && w < a.word0 + a.nwords) {
-        if (a.kind == VQ_AGG_MIN) init = (unsigned long long)INT64_MAX;
-        if (a.kind == VQ_AGG_MAX) init = (unsigned long long)INT64_MIN;
-      }
+      if ((int)threadIdx.x >= a.word0 && (int)threadIdx.x < a.word0 + a.nwords)
+        op = a.kind == VQ_AGG_MIN ? 1 : a.kind == VQ_AGG_MAX ? 2 : 0;
     }
-    acc[i] = init;
+    s_wop[threadIdx.x] = op;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < R * NP * NW; i += kJoinThreads) {
+    const int op = s_wop[i % NW];
+    acc[i] = op == 1 ? (unsigned long long)INT64_MAX : op == 2 ? (unsigned long long)INT64_MIN : 0ULL;
   }
   __syncthreads();
 
@@ -373,8 +386,9 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     uint32_t sl = lg ? (uint32_t)(h >> (64 - lg)) : 0;
     bool done = false;
     for (int probe = 0; probe < cap; probe++) {
-      const unsigned long long old =
-          atomicCAS(&keys[sl], (unsigned long long)kEmptyKey, (unsigned long long)key);
+      unsigned long long old = lds_load_key(&keys[sl]);
+      if (old == (unsigned long long)kEmptyKey)
+        old = atomicCAS(&keys[sl], (unsigned long long)kEmptyKey, (unsigned long long)key);
       if (old == (unsigned long long)kEmptyKey || old == (unsigned long long)key) {
         atomicAdd(&tag[sl], 1u);
         if (old == (unsigned long long)kEmptyKey) atomicAdd(&s_nkeys, 1u);
@@ -403,8 +417,11 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
       uint32_t sl = lg ? (uint32_t)(h >> (64 - lg)) : 0;
       bool done = false;
       for (int probe = 0; probe < cap; probe++) {
-        const unsigned long long old =
-            atomicCAS(&keys[sl], (unsigned long long)kEmptyKey, (unsigned long long)key);
+        // (a key already in the table is found without an atomic: a hot
+        // key's rows do not serialize on its slot)
+        unsigned long long old = lds_load_key(&keys[sl]);
+        if (old == (unsigned long long)kEmptyKey)
+          old = atomicCAS(&keys[sl], (unsigned long long)kEmptyKey, (unsigned long long)key);
         if (old == (unsigned long long)kEmptyKey || old == (unsigned long long)key) {
           if (old == (unsigned long long)kEmptyKey) tag[sl] = 1;
           done = true;
@@ -451,87 +468,98 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   const bool built = s_fail == 0;
   if (built && threadIdx.x == 0 && s_nkeys) atomicAdd(&flags[2], s_nkeys);
 
-  // probe + aggregate; each thread has kPU entries' loads in flight
+  // probe + aggregate
   const uint32_t q0 = B.p.pstart[p], q1 = B.p.pstart[p + 1];
   const int64_t vstride = B.p.vstride;
-  constexpr int kPU = 4;
-  for (uint32_t e0 = q0 + threadIdx.x; built && e0 < q1; e0 += kJoinThreads * kPU) {
-    int64_t k_[kPU];
-    uint32_t r_[kPU];
-    uint64_t va_[kPU], vb_[kPU];
-#pragma unroll
-    for (int u = 0; u < kPU; u++) {
-      const uint32_t e = e0 + u * kJoinThreads;
-      const uint32_t ec = e < q1 ? e : q1 - 1;
-      k_[u] = B.p.keys[ec];
-      r_[u] = B.p.rows[ec];
-      va_[u] = plan.nvals > 0 ? B.p.vals[ec] : 0;
-      vb_[u] = plan.nvals > 1 ? B.p.vals[vstride + ec] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < kPU; u++) {
-      const uint32_t e = e0 + u * kJoinThreads;
-      if (e >= q1) continue;
-      const int64_t key = k_[u];
-      int sl = -1;
-      if (key == kEmptyKey) {
-        sl = tag[cap] ? cap : -1;
-      } else {
-        const uint64_t h = radix_hash((uint64_t)key);
-        uint32_t i = lg ? (uint32_t)(h >> (64 - lg)) : 0;
-        for (int probe = 0; probe < cap; probe++) {
-          const int64_t k = (int64_t)keys[i];
-          if (k == kEmptyKey) break;
-          if (k == key) {
-            sl = (int)i;
-            break;
-          }
-          i = (i + 1) & (uint32_t)(cap - 1);
+  // one entry per lane and iteration, the next entry's loads issued before
+  // this one's update (an empty partition runs no iteration, and loads
+  // nothing)
+  uint32_t e = q0 + threadIdx.x;
+  int64_t nkey = 0;
+  uint32_t nrow = 0;
+  uint64_t nva = 0, nvb = 0;
+  auto fetch = [&](uint32_t x) {
+    const uint32_t ec = x < q1 ? x : q1 - 1;
+    nkey = B.p.keys[ec];
+    nrow = B.p.rows[ec];
+    nva = plan.nvals > 0 ? B.p.vals[ec] : 0;
+    nvb = plan.nvals > 1 ? B.p.vals[vstride + ec] : 0;
+  };
+  if (built && q1 > q0) fetch(e);
+  for (; built && e < q1; e += kJoinThreads) {
+    const int64_t key = nkey;
+    const uint32_t row = nrow;
+    const uint64_t va = nva, vb = nvb;
+    if (e + kJoinThreads < q1) fetch(e + kJoinThreads);
+    int sl = -1;
+    if (key == kEmptyKey) {
+      sl = tag[cap] ? cap : -1;
+    } else {
+      const uint64_t h = radix_hash((uint64_t)key);
+      uint32_t i = lg ? (uint32_t)(h >> (64 - lg)) : 0;
+      for (int probe = 0; probe < cap; probe++) {
+        const int64_t k = (int64_t)keys[i];
+        if (k == kEmptyKey) break;
+        if (k == key) {
+          sl = (int)i;
+          break;
         }
+        i = (i + 1) & (uint32_t)(cap - 1);
       }
-      if (sl < 0) continue;
-      uint32_t pi = tag[sl];
-      if (!(pi & kClaimed)) {  // first match of this key: claim a pool entry
+    }
+    if (sl < 0) continue;
+    uint32_t pi = tag[sl];
+    if (!(pi & kClaimed)) {
+      // first match of this key: the lane whose compare-and-swap marks the
+      // slot pending takes the next pool entry; the others wait for it, so a
+      // key claims exactly one entry
+      const uint32_t old = atomicCAS(&tag[sl], pi, kPending);
+      if (old == pi) {
         const uint32_t mine = atomicAdd(&s_pool, 1u);
-        if (mine >= (uint32_t)NP) {
-          s_fail = 2;
-          continue;
-        }
-        const uint32_t old = atomicCAS(&tag[sl], pi, kClaimed | mine);
-        if (old == pi) {
+        if (mine < (uint32_t)NP) {
           pslot[mine] = (uint32_t)sl;
           pmult[mine] = pi;
         }
-        pi = old == pi ? kClaimed | mine : old;
+        pi = kClaimed | (mine < (uint32_t)NP ? mine : (uint32_t)NP);
+        __hip_atomic_store(&tag[sl], pi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else {
+        pi = old;
       }
-      pi &= ~kClaimed;
-      if (atomicAdd(&cnt[pi], 1u) == 0) atomicAdd(&s_nrec, 1u);
-      atomicMin(&first[pi], r_[u]);
-      int vi = 0;
-      for (int j = 0; j < jd.naggs; j++) {
-        const JAgg& a = jd.agg[j];
-        if (a.e.nfactors == 0) continue;
-        const uint64_t v = vi == 0 ? va_[u] : vi == 1 ? vb_[u] : B.p.vals[(int64_t)vi * vstride + e];
-        vi++;
-        unsigned long long* w = &acc[(size_t)pi * NW + a.word0];
-        switch (a.kind) {
-          case VQ_AGG_SUM:
-          case VQ_AGG_AVG:
-            if (a.e.is_float) fix_add(w, bits_to_f64((int64_t)v), a.fs);
-            else atomicAdd(w, (unsigned long long)v);
-            break;
-          case VQ_AGG_MIN:
-          case VQ_AGG_MAX: {
-            const double dv = bits_to_f64((int64_t)v);
-            if (a.accum == VQ_F64 && dv != dv) break;  // NaN never wins a compare
-            const int64_t o = a.accum == VQ_F64 ? f64_ord(dv) : (int64_t)v;
-            if (a.kind == VQ_AGG_MIN) atomicMin((long long*)w, (long long)o);
-            else atomicMax((long long*)w, (long long)o);
-            break;
-          }
-          default:
-            break;
+    }
+    // (the tag read above may itself be another lane's pending mark)
+    while (pi == kPending)
+      pi = __hip_atomic_load(&tag[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    pi &= ~kClaimed;
+    if (pi >= (uint32_t)NP) {  // the pool is full
+      s_fail = 2;
+      continue;
+    }
+    atomicAdd(&cnt[myrep * NP + pi], 1u);
+    atomicMin(&first[myrep * NP + pi], row);
+    int vi = 0;
+    for (int j = 0; j < jd.naggs; j++) {
+      const JAgg& a = jd.agg[j];
+      if (a.e.nfactors == 0) continue;
+      const uint64_t v = vi == 0 ? va : vi == 1 ? vb : B.p.vals[(int64_t)vi * vstride + e];
+      vi++;
+      unsigned long long* w = &acc[((size_t)myrep * NP + pi) * NW + a.word0];
+      switch (a.kind) {
+        case VQ_AGG_SUM:
+        case VQ_AGG_AVG:
+          if (a.e.is_float) fix_add(w, bits_to_f64((int64_t)v), a.fs);
+          else atomicAdd(w, (unsigned long long)v);
+          break;
+        case VQ_AGG_MIN:
+        case VQ_AGG_MAX: {
+          const double dv = bits_to_f64((int64_t)v);
+          if (a.accum == VQ_F64 && dv != dv) break;  // NaN never wins a compare
+          const int64_t o = a.accum == VQ_F64 ? f64_ord(dv) : (int64_t)v;
+          if (a.kind == VQ_AGG_MIN) atomicMin((long long*)w, (long long)o);
+          else atomicMax((long long*)w, (long long)o);
+          break;
         }
+        default:
+          break;
       }
     }
   }
@@ -539,6 +567,35 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   const uint32_t fail = s_fail;
   if (fail && threadIdx.x == 0) atomicOr(&flags[0], fail);
   const uint32_t used = s_pool < (uint32_t)NP ? s_pool : (uint32_t)NP;
+  // fold the replicas into replica 0 (integer adds, minima and maxima: the
+  // same result in any order), then count the groups
+  if (R > 1 && !fail) {
+    for (uint32_t x = threadIdx.x; x < used * (uint32_t)NW; x += kJoinThreads) {
+      const uint32_t i = x / NW, w = x % NW;
+      const int op = s_wop[w];
+      unsigned long long v = acc[(size_t)i * NW + w];
+      for (int r = 1; r < R; r++) {
+        const unsigned long long o = acc[((size_t)r * NP + i) * NW + w];
+        if (op == 0) v += o;
+        else if (op == 1) v = (long long)o < (long long)v ? o : v;
+        else v = (long long)o > (long long)v ? o : v;
+      }
+      acc[(size_t)i * NW + w] = v;
+    }
+    for (uint32_t i = threadIdx.x; i < used; i += kJoinThreads) {
+      uint32_t c = cnt[i], f = first[i];
+      for (int r = 1; r < R; r++) {
+        c += cnt[r * NP + i];
+        f = first[r * NP + i] < f ? first[r * NP + i] : f;
+      }
+      cnt[i] = c;
+      first[i] = f;
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; !fail && i < used; i += kJoinThreads)
+    if (cnt[i]) atomicAdd(&s_nrec, 1u);
+  __syncthreads();
   const int RW = 3 + jd.naggs;
 
   if (plan.topk == 0) {
@@ -653,9 +710,9 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   }
 }
 
-size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords) {
+size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps) {
   const size_t ns = (size_t)cap + 1;
-  return ns * (8 + 4) + (size_t)pool * (4 + 4 + 4 + 4 + 8 * (size_t)nwords);
+  return ns * (8 + 4) + (size_t)pool * (4 + 4) + (size_t)reps * pool * (4 + 4 + 8 * (size_t)nwords);
 }
 
 uint32_t part_scatter_lds(int32_t P, int32_t pbuf, int32_t entry) {
@@ -683,6 +740,243 @@ void join_allow_lds(int32_t bytes) {
   if (allowed.load() >= bytes) return;
   hipFuncSetAttribute((const void*)k_join_agg, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   allowed.store(bytes);
+}
+
+// ---- GROUP BY of few keys through per-workgroup partials ---------------------
+// (ParallelHashAggregate's per-worker partials and mergePartialAgg,
+// exec/parallel.go:18-314.)  Four launches:
+//   k_local_init   the key table to empty, the flags to zero
+//   k_local_keys   every distinct key of the scanned rows (predicates
+//                  aside: a key whose rows all fail them makes a group of no
+//                  rows, which emits nothing) into one table of lcap slots;
+//                  more than `limit` keys sets flags[0] bit 4, and the query
+//                  runs through the partitions instead
+//   vq_hash_local  (run-time compiled) each workgroup aggregates an even
+//                  share of the rows into LDS accumulators per group index
+//                  and writes its partials
+//   k_local_merge  one block per group: the workgroups' partials combined,
+//                  the group's record emitted (the same record as k_join_agg)
+constexpr uint32_t kLocalOverflow = 4u;
+constexpr int kLocalThreads = 512;
+
+struct LocalKeys {
+  const void* col;
+  int32_t type;
+  int32_t lcap;
+  int32_t limit;
+  int32_t nspans;
+  int64_t tile_rows;
+  int64_t ntiles;
+  int64_t sp_start[kMaxSpans];
+  int64_t sp_rows[kMaxSpans];
+  int64_t sp_tile0[kMaxSpans];
+  int64_t* keys;
+  uint32_t* flags;
+};
+
+__global__ __launch_bounds__(256) void k_local_init(int64_t* __restrict__ keys, int32_t lcap,
+                                                    uint32_t* __restrict__ flags) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < lcap; i += gridDim.x * 256) keys[i] = kEmptyKey;
+  if (blockIdx.x == 0 && threadIdx.x < 16) flags[threadIdx.x] = 0;
+}
+
+// A key column's value as the run-time kernels read it: codes zero-extended,
+// dates sign-extended, 64-bit columns as their bits.
+__device__ __forceinline__ int64_t load_key(const void* col, int32_t type, int64_t r) {
+  switch (type) {
+    case VQ_U8: return ((const uint8_t*)col)[r];
+    case VQ_U16: return ((const uint16_t*)col)[r];
+    case VQ_DATE: return ((const int32_t*)col)[r];
+    default: return ((const int64_t*)col)[r];
+  }
+}
+
+// Inserts key into an open-addressing table of cap slots; returns 1 if it
+// was new, 0 if present, -1 if the table is full.
+template <int SCOPE>
+__device__ __forceinline__ int table_insert(unsigned long long* t, int cap, int lg, int64_t key) {
+  uint32_t i = (uint32_t)(radix_hash((uint64_t)key) >> (64 - lg));
+  for (int pr = 0; pr < cap; pr++) {
+    unsigned long long cur = __hip_atomic_load(&t[i], __ATOMIC_RELAXED, SCOPE);
+    if (cur == (unsigned long long)key) return 0;
+    if (cur == (unsigned long long)kEmptyKey) {
+      cur = atomicCAS(&t[i], (unsigned long long)kEmptyKey, (unsigned long long)key);
+      if (cur == (unsigned long long)kEmptyKey) return 1;
+      if (cur == (unsigned long long)key) return 0;
+    }
+    i = (i + 1) & (uint32_t)(cap - 1);
+  }
+  return -1;
+}
+
+// Each workgroup collects the keys of its tiles in LDS (a key repeated over
+// millions of rows costs a read of its slot, not an atomic), then adds the
+// ones it found to the global table.
+__global__ __launch_bounds__(kLocalThreads) void k_local_keys(const LocalKeys k) {
+  extern __shared__ unsigned long long lt[];
+  __shared__ uint32_t s_n, s_min, s_stop;
+  const int cap = k.lcap, lg = __ffs(cap) - 1;
+  for (int i = threadIdx.x; i < cap; i += kLocalThreads) lt[i] = (unsigned long long)kEmptyKey;
+  if (threadIdx.x == 0) {
+    s_n = 0;
+    s_min = 0;
+    s_stop = 0;
+  }
+  __syncthreads();
+  for (int64_t t = blockIdx.x; t < k.ntiles; t += gridDim.x) {
+    int s = 0;
+    while (s + 1 < k.nspans && t >= k.sp_tile0[s + 1]) s++;
+    const int64_t row0 = k.sp_start[s] + (t - k.sp_tile0[s]) * k.tile_rows;
+    const int64_t send = k.sp_start[s] + k.sp_rows[s];
+    const int64_t rend = row0 + k.tile_rows < send ? row0 + k.tile_rows : send;
+    bool over = false;
+    for (int64_t r = row0 + threadIdx.x; r < rend; r += kLocalThreads) {
+      const int64_t key = load_key(k.col, k.type, r);
+      if (key == kEmptyKey) {
+        s_min = 1;
+        continue;
+      }
+      const int got = table_insert<__HIP_MEMORY_SCOPE_WORKGROUP>(lt, cap, lg, key);
+      if (got < 0 || (got > 0 && atomicAdd(&s_n, 1u) + 1 > (uint32_t)k.limit)) over = true;
+    }
+    if (over) s_stop = 1;
+    __syncthreads();
+    // stop early once this workgroup or another has seen too many keys
+    if (threadIdx.x == 0 && (__hip_atomic_load(&k.flags[0], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT) & kLocalOverflow))
+      s_stop = 1;
+    __syncthreads();
+    if (s_stop) break;
+  }
+  if (s_stop) {
+    if (threadIdx.x == 0) atomicOr(&k.flags[0], kLocalOverflow);
+    return;
+  }
+  for (int i = threadIdx.x; i < cap; i += kLocalThreads) {
+    const int64_t key = (int64_t)lt[i];
+    if (key == kEmptyKey) continue;
+    const int got = table_insert<__HIP_MEMORY_SCOPE_AGENT>((unsigned long long*)k.keys, cap, lg, key);
+    if (got < 0 || (got > 0 && atomicAdd(&k.flags[2], 1u) + 1 > (uint32_t)k.limit))
+      atomicOr(&k.flags[0], kLocalOverflow);
+  }
+  if (threadIdx.x == 0 && s_min && atomicCAS(&k.flags[10], 0u, 1u) == 0u &&
+      atomicAdd(&k.flags[2], 1u) + 1 > (uint32_t)k.limit)
+    atomicOr(&k.flags[0], kLocalOverflow);
+}
+
+struct LocalMerge {
+  const uint64_t* part;
+  const int64_t* gkey;
+  uint32_t* flags;
+  uint64_t* recs;
+  int32_t grid;  // workgroups that wrote partials
+  int32_t pad_;
+};
+
+constexpr int kMergeThreads = 256;
+
+// One block per group index: each word of the group's partials combined over
+// the workgroups (integer adds, minima and maxima), then its record.
+__global__ __launch_bounds__(kMergeThreads) void k_local_merge(const JDesc jd, const LocalMerge m) {
+  __shared__ unsigned long long s_red[kMergeThreads];
+  __shared__ unsigned long long s_words[3 * VQ_MAX_AGGS + 2];
+  __shared__ uint8_t s_wop[3 * VQ_MAX_AGGS + 2];
+  const uint32_t g = blockIdx.x;
+  if (m.flags[0] & kLocalOverflow) return;
+  const uint32_t nk = m.flags[2];
+  if (g >= nk) return;
+  const int NW = jd.nwords, NI = NW + 2;
+  if ((int)threadIdx.x < NI) {
+    uint8_t op = 0;  // add
+    if ((int)threadIdx.x == NI - 1) op = 3;  // first row: unsigned minimum
+    for (int j = 0; j < jd.naggs; j++) {
+      const JAgg& a = jd.agg[j];
+      if ((int)threadIdx.x >= a.word0 && (int)threadIdx.x < a.word0 + a.nwords)
+        op = a.kind == VQ_AGG_MIN ? 1 : a.kind == VQ_AGG_MAX ? 2 : 0;
+    }
+    s_wop[threadIdx.x] = op;
+  }
+  __syncthreads();
+  for (int i = 0; i < NI; i++) {
+    const int op = s_wop[i];
+    const unsigned long long id = op == 1 ? (unsigned long long)INT64_MAX
+                                  : op == 2 ? (unsigned long long)INT64_MIN
+                                  : op == 3 ? 0xFFFFFFFFull : 0ull;
+    unsigned long long v = id;
+    for (int w = threadIdx.x; w < m.grid; w += kMergeThreads) {
+      const unsigned long long o = m.part[((size_t)w * nk + g) * NI + i];
+      if (op == 0) v += o;
+      else if (op == 1) v = (long long)o < (long long)v ? o : v;
+      else if (op == 2) v = (long long)o > (long long)v ? o : v;
+      else v = o < v ? o : v;
+    }
+    s_red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kMergeThreads / 2; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) {
+        const unsigned long long o = s_red[threadIdx.x + off];
+        unsigned long long x = s_red[threadIdx.x];
+        if (op == 0) x += o;
+        else if (op == 1) x = (long long)o < (long long)x ? o : x;
+        else if (op == 2) x = (long long)o > (long long)x ? o : x;
+        else x = o < x ? o : x;
+        s_red[threadIdx.x] = x;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) s_words[i] = s_red[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const unsigned long long cnt = s_words[NW];
+    if (cnt == 0) return;  // a key whose rows all failed the predicate
+    const uint32_t at = atomicAdd(&m.flags[1], 1u);
+    emit_record(jd, m.recs + (size_t)at * (3 + jd.naggs), m.gkey[g], cnt, 1,
+                (uint32_t)s_words[NW + 1], s_words);
+  }
+}
+
+size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords) {
+  return (size_t)lcap * 12 + (size_t)limit * (8 * (size_t)nwords + 8);
+}
+
+hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
+                        const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
+  if (ev) hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL(k_local_init, dim3((unsigned)std::max(1, plan.lcap / 256)), dim3(256), 0, s,
+                     B.l.keys, plan.lcap, B.flags);
+  LocalKeys k;
+  memset(&k, 0, sizeof k);
+  k.col = ps.slot[ps.key_slot].p;
+  k.type = ps.slot[ps.key_slot].type;
+  k.lcap = plan.lcap;
+  k.limit = plan.limit;
+  k.nspans = ps.nspans;
+  k.tile_rows = ps.tile_rows;
+  k.ntiles = ps.ntiles;
+  for (int i = 0; i < ps.nspans; i++) {
+    k.sp_start[i] = ps.span[i].start;
+    k.sp_rows[i] = ps.span[i].rows;
+    k.sp_tile0[i] = ps.span[i].tile0;
+  }
+  k.keys = B.l.keys;
+  k.flags = B.flags;
+  hipLaunchKernelGGL(k_local_keys, dim3((unsigned)plan.kgrid), dim3(kLocalThreads),
+                     (size_t)plan.lcap * 8, s, k);
+  if (ev) hipEventRecord(ev[1], s);
+  hipError_t e = jit_local_launch(ll, B, s);
+  if (e != hipSuccess) return e;
+  if (ev) hipEventRecord(ev[2], s);
+  LocalMerge m;
+  m.part = B.l.part;
+  m.gkey = B.l.gkey;
+  m.flags = B.flags;
+  m.recs = B.recs;
+  m.grid = ll.grid;
+  m.pad_ = 0;
+  hipLaunchKernelGGL(k_local_merge, dim3((unsigned)plan.limit), dim3(kMergeThreads), 0, s, jd, m);
+  if (ev) hipEventRecord(ev[3], s);
+  return hipGetLastError();
 }
 
 hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd,

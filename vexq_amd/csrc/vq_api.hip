@@ -628,7 +628,7 @@ extern "C" int vq_table_destroy(vq_table* t) {
     char pre[32];
     snprintf(pre, sizeof pre, "%p/", (const void*)t);
     for (auto it = t->ctx->build_est.begin(); it != t->ctx->build_est.end();)
-      it = it->first.compare(0, strlen(pre), pre) == 0 ? t->ctx->build_est.erase(it) : std::next(it);
+      it = it->first.find(pre) != std::string::npos ? t->ctx->build_est.erase(it) : std::next(it);
   }
   delete t;
   return VQ_OK;
@@ -1159,6 +1159,7 @@ struct JoinTune {
   int32_t bits_delta = 0;
   int32_t pbuf = -1;
   int32_t tile_shift = 0;
+  int32_t reps = 0;  // accumulator replicas (0: the planner's choice)
 };
 
 constexpr int kRetry = -1;  // internal: repartition and run again
@@ -1193,6 +1194,8 @@ struct JoinJob {
   bool dev_topk = false;
   int bits = 0;
   int32_t pool_req = 0;
+  bool no_local = false;   // GROUP BY: too many keys for per-workgroup partials
+  LocalLaunch ll;
   // per attempt
   JoinPlan plan;
   int32_t pool_max = 0;
@@ -1207,6 +1210,11 @@ struct JoinJob {
 };
 
 constexpr double kLoad = 0.4;
+// A GROUP BY of at most kLocalLimit keys runs through per-workgroup partials
+// (one table of kLocalCap slots); one of more, or whose key count is not
+// known yet and turns out larger, through the partitions.
+constexpr int32_t kLocalCap = 512;
+constexpr int32_t kLocalLimit = kLocalCap / 2;
 
 int join_setup(JoinJob& j) {
   vq_ctx* c = j.c;
@@ -1278,6 +1286,9 @@ int join_setup(JoinJob& j) {
     if (it != c->build_est.end()) j.est = std::max<int64_t>(it->second, 1);
     it = c->build_est.find(j.grp_key);
     if (it != c->build_est.end()) j.groups_seen = it->second;
+    if (j.groupby && (c->build_est.count("L|" + j.est_key) ||
+                      (j.groups_seen >= 0 && j.est > kLocalLimit)))
+      j.no_local = true;
   }
   hipDeviceGetAttribute(&j.lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
   if (j.lds_dev <= 0) j.lds_dev = 64 * 1024;
@@ -1301,10 +1312,53 @@ int join_setup(JoinJob& j) {
   return VQ_OK;
 }
 
+int join_plan_local(JoinJob& j) {
+  std::string jerr;
+  JoinPlan& plan = j.plan;
+  plan.local = 1;
+  plan.groupby = 1;
+  plan.P = 1;
+  plan.lcap = kLocalCap;
+  plan.limit = kLocalLimit;
+  plan.lds_max = j.lds_max;
+  if (local_lds_for(plan.lcap, plan.limit, j.nw) > (size_t)kLocalLds) {
+    j.no_local = true;
+    return kRetry;
+  }
+  std::vector<uint8_t> pkeep;
+  j.brows = j.bkept = j.prows = j.pkept = 0;
+  const int ts = std::max(0, std::min(4, j.tune.tile_shift));
+  prune(j.probe, j.pp, pkeep);
+  j.ps.tile_rows = join_tile_rows(kept_rows(j.probe, pkeep)) << ts;
+  if (int rc = make_spans(j.probe, pkeep, j.ps.tile_rows, j.ps.span, &j.ps.nspans, &j.ps.ntiles,
+                         &j.prows, &j.pkept))
+    return rc;
+  if (j.prows >= (int64_t)UINT32_MAX)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: GROUP BY above 2^32 rows");
+  j.RW = 3 + j.naggs;
+  j.rec_cap = plan.limit;
+  memset(&j.B, 0, sizeof j.B);
+  if (jit_local_prepare(j.ps, j.jd, plan.lcap, &j.ll, &jerr))
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: GROUP BY kernels unavailable: %s", jerr.c_str());
+  plan.lgrid = j.ll.grid;
+  plan.kgrid = (int32_t)std::max<int64_t>(1, std::min<int64_t>(j.ps.ntiles, 2 * (int64_t)j.ll.grid));
+  size_t bytes = 16 * 4 + (size_t)plan.lcap * 8 + (size_t)plan.limit * 8 +
+                 (size_t)plan.lgrid * plan.limit * (j.nw + 2) * 8 + (size_t)j.rec_cap * j.RW * 8;
+  bytes += 8 * 256;  // alignment of the carved pieces
+  j.bytes = (bytes + 255) & ~(size_t)255;
+  j.host_bytes = 64 + (size_t)16 * j.RW * 8;
+  return VQ_OK;
+}
+
 int join_plan(JoinJob& j) {
   std::string jerr;
   JoinPlan& plan = j.plan;
   memset(&plan, 0, sizeof plan);
+  if (j.groupby && !j.no_local) {
+    const int rc = join_plan_local(j);
+    if (rc != kRetry) return rc;
+    memset(&plan, 0, sizeof plan);
+  }
   plan.bits = j.bits;
   plan.P = 1 << j.bits;
   const double per = (double)j.est / plan.P;
@@ -1318,7 +1372,22 @@ int join_plan(JoinJob& j) {
                                     : cap / 2;
   if (j.pool_req) pool = j.pool_req;
   plan.pool = std::max<int32_t>(64, std::min(pool, j.pool_max));
-  plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, j.nw);
+  // accumulator replicas when a partition's keys take many rows each (their
+  // atomics would queue on a few LDS addresses), as many as fit without
+  // costing the kernel its second workgroup per CU (or the LDS limit when it
+  // has none)
+  plan.reps = 1;
+  {
+    const double keys_pp = std::max(1.0, (double)(j.groups_seen >= 0 ? j.groups_seen : j.est) / plan.P);
+    const double rows_pk = (double)j.probe->nrows / plan.P / keys_pp;
+    const size_t one = join_lds_for(cap, plan.pool, j.nw, 1);
+    const size_t half = (size_t)j.lds_dev / 2 - 8192;
+    const size_t budget = one <= half ? half : (size_t)j.lds_max;
+    int reps = j.tune.reps > 0 ? j.tune.reps : rows_pk >= 64 ? 16 : 1;
+    while (reps > 1 && join_lds_for(cap, plan.pool, j.nw, reps) > budget) reps /= 2;
+    plan.reps = reps;
+  }
+  plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, j.nw, plan.reps);
   plan.lds_max = j.lds_max;
   plan.groupby = j.groupby ? 1 : 0;
   plan.topk = j.dev_topk ? j.topk : 0;
@@ -1373,6 +1442,15 @@ void join_bind(JoinJob& j, char* dev, char* host) {
   Carve cv;
   JoinBufs& B = j.B;
   B.flags = cv.take<uint32_t>(dev, 16);
+  j.hflags = (uint32_t*)host;
+  j.hout = (uint64_t*)(host + 64);
+  if (j.plan.local) {
+    B.l.keys = cv.take<int64_t>(dev, j.plan.lcap);
+    B.l.gkey = cv.take<int64_t>(dev, j.plan.limit);
+    B.l.part = cv.take<uint64_t>(dev, (size_t)j.plan.lgrid * j.plan.limit * (j.nw + 2));
+    B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
+    return;
+  }
   B.b.perr = B.p.perr = B.flags + 5;
   B.b.keys = cv.take<int64_t>(dev, j.cap_b);
   B.b.vstride = j.cap_b;
@@ -1393,14 +1471,13 @@ void join_bind(JoinJob& j, char* dev, char* host) {
   } else {
     B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
   }
-  j.hflags = (uint32_t*)host;
-  j.hout = (uint64_t*)(host + 64);
 }
 
 // Every launch and copy of one attempt, in stream order (capturable: no
 // memset nodes, no host synchronisation).
 hipError_t join_enqueue(JoinJob& j, hipStream_t s, hipEvent_t* ev) {
-  hipError_t e = launch_join(j.bl, j.pl, j.jd, j.plan, j.B, s, ev);
+  hipError_t e = j.plan.local ? launch_local(j.ll, j.ps, j.jd, j.plan, j.B, s, ev)
+                              : launch_join(j.bl, j.pl, j.jd, j.plan, j.B, s, ev);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(j.hflags, j.B.flags, 32, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess && j.dev_topk)
@@ -1413,6 +1490,15 @@ hipError_t join_enqueue(JoinJob& j, hipStream_t s, hipEvent_t* ev) {
 int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev) {
   vq_ctx* c = j.c;
   const uint32_t* hflags = j.hflags;
+  if (j.plan.local && (hflags[0] & 4)) {
+    // more keys than the per-workgroup path holds: through the partitions,
+    // now and for later queries of this shape
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->build_est["L|" + j.est_key] = 1;
+    j.no_local = true;
+    return kRetry;
+  }
+  if (j.plan.local && hflags[0]) return fail(VQ_E_DEVICE, "exec: gpu: GROUP BY flags %#x", hflags[0]);
   if (hflags[5]) return fail(VQ_E_DEVICE, "exec: gpu: partition scatter wrote past a partition's end");
   if (hflags[0]) {
     // a partition's accumulator pool filled: a larger pool if the LDS has
@@ -1457,9 +1543,9 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
                        j.prows * slot_bytes_per_row(j.probe, j.psm);
     stats->build_rows = j.groupby ? 0 : hflags[6];
     stats->build_keys = hflags[2];
-    stats->partitions = j.plan.P;
-    stats->radix_bits = j.bits;
-    stats->grid = j.plan.P;
+    stats->partitions = j.plan.local ? 0 : j.plan.P;
+    stats->radix_bits = j.plan.local ? 0 : j.bits;
+    stats->grid = j.plan.local ? j.plan.lgrid : j.plan.P;
     stats->tile_rows = j.ps.tile_rows;
   }
   // rank order for a device top-k; first-occurrence order otherwise (then
@@ -1827,12 +1913,13 @@ bool tune_lookup(vq_ctx* c, const std::string& cls, JoinTune* t) {
   std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->tune.find(cls);
   if (it == c->tune.end()) return false;
-  return sscanf(it->second.c_str(), "%d %d %d", &t->bits_delta, &t->pbuf, &t->tile_shift) == 3;
+  return sscanf(it->second.c_str(), "%d %d %d %d", &t->bits_delta, &t->pbuf, &t->tile_shift,
+                &t->reps) == 4;
 }
 
 void tune_store(vq_ctx* c, const std::string& cls, const JoinTune& t) {
   char buf[64];
-  snprintf(buf, sizeof buf, "%d %d %d", t.bits_delta, t.pbuf, t.tile_shift);
+  snprintf(buf, sizeof buf, "%d %d %d %d", t.bits_delta, t.pbuf, t.tile_shift, t.reps);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune[cls] = buf;
   if (const char* f = getenv("VQ_TUNE_FILE")) {
@@ -1848,11 +1935,11 @@ void tune_load(vq_ctx* c) {
   FILE* fp = f ? fopen(f, "r") : nullptr;
   if (!fp) return;
   char cls[128];
-  int a, b, d;
+  int a, b, d, r;
   std::lock_guard<std::mutex> lk(c->mu);
-  while (fscanf(fp, "%127s %d %d %d", cls, &a, &b, &d) == 4) {
+  while (fscanf(fp, "%127s %d %d %d %d", cls, &a, &b, &d, &r) == 5) {
     char buf[64];
-    snprintf(buf, sizeof buf, "%d %d %d", a, b, d);
+    snprintf(buf, sizeof buf, "%d %d %d %d", a, b, d, r);
     c->tune[cls] = buf;
   }
   fclose(fp);
@@ -1860,7 +1947,8 @@ void tune_load(vq_ctx* c) {
 
 // The autotuner: run the instance under each candidate (radix bits one
 // below / at / above the planner's choice, the largest or an 8-entry scatter
-// buffer, default or 4x tiles), keep the fastest device time.  Every
+// buffer, default or 4x tiles, the planner's or a single accumulator
+// replica), keep the fastest device time.  Every
 // candidate computes the same result (the accumulation is exact and
 // order-free), so tuning never changes an answer.
 JoinTune autotune(const JoinJob& proto, vq_agg_result* out) {
@@ -1868,28 +1956,30 @@ JoinTune autotune(const JoinJob& proto, vq_agg_result* out) {
   double best_ms = 1e30;
   for (int bd = -1; bd <= 1; bd++)
     for (int pb : {-1, 8})
-      for (int ts : {0, 2}) {
-        JoinTune t;
-        t.bits_delta = bd;
-        t.pbuf = pb;
-        t.tile_shift = ts;
-        double ms = 1e30;
-        for (int rep = 0; rep < 2; rep++) {
-          vq_stats st;
-          const int rc = join_impl(proto.c, proto.build, proto.build_key, proto.bp, proto.probe,
-                                   proto.probe_key, proto.pp, proto.naggs, proto.aggs,
-                                   proto.radix_bits, proto.topk, out, &st, proto.groupby, &t);
-          if (rc) {
-            ms = 1e30;
-            break;
+      for (int ts : {0, 2})
+        for (int rp : {0, 1}) {
+          JoinTune t;
+          t.bits_delta = bd;
+          t.pbuf = pb;
+          t.tile_shift = ts;
+          t.reps = rp;
+          double ms = 1e30;
+          for (int rep = 0; rep < 2; rep++) {
+            vq_stats st;
+            const int rc = join_impl(proto.c, proto.build, proto.build_key, proto.bp, proto.probe,
+                                     proto.probe_key, proto.pp, proto.naggs, proto.aggs,
+                                     proto.radix_bits, proto.topk, out, &st, proto.groupby, &t);
+            if (rc) {
+              ms = 1e30;
+              break;
+            }
+            ms = std::min(ms, st.device_ms);
           }
-          ms = std::min(ms, st.device_ms);
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+          }
         }
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = t;
-        }
-      }
   return best;
 }
 
@@ -1926,7 +2016,7 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
       t = autotune(j, &outs[i]);
       tune_store(c, cls, t);
     }
-    if (t.bits_delta || t.pbuf >= 0 || t.tile_shift) {
+    if (t.bits_delta || t.pbuf >= 0 || t.tile_shift || t.reps) {
       j.tune = t;
       if (int rc = join_setup(j)) return rc;  // the bits again, with the tuning
     }

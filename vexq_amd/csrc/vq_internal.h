@@ -189,16 +189,27 @@ struct PartBufs {
   int32_t pad_;
 };
 
+// GROUP BY through per-workgroup partials: the distinct keys in one table
+// (slots of kEmptyKey are free; key INT64_MIN is flagged in flags[10]), the
+// key of each group index, and the workgroups' partial accumulators.
+struct LocalBufs {
+  int64_t* keys;        // [lcap]
+  int64_t* gkey;        // [limit]
+  uint64_t* part;       // [grid][groups][nwords + 2]
+};
+
 struct JoinBufs {
   PartBufs b, p;
+  LocalBufs l;
   uint64_t* recs;       // one record per matched group (full-output mode)
   uint64_t* cand;       // [P][topk] per-partition top-k candidates (top-k mode)
   struct KItem* citems; // [P][topk] their sort keys (value, first row, index)
   uint64_t* out;        // top-k records
-  // flags[0] overflow, [1] groups, [2] distinct build keys, [3] top-k count,
-  // [4] partitions finished, [5] partition write error, [6] build entries,
-  // [7] probe entries, [8..9] top-k lower bound; 16 words, zeroed by the
-  // pipeline's first launch
+  // flags[0] overflow (1 a table, 2 a pool, 4 more keys than the
+  // per-workgroup path holds), [1] groups, [2] distinct build keys, [3] top-k
+  // count, [4] partitions finished, [5] partition write error, [6] build
+  // entries, [7] probe entries, [8..9] top-k lower bound, [10] key INT64_MIN
+  // seen; 16 words, zeroed by the pipeline's first launch
   uint32_t* flags;
 };
 
@@ -214,11 +225,19 @@ struct JoinPlan {
   int32_t pbuf;         // probe scatter: the same
   int32_t lds_bytes;    // the join kernel's dynamic LDS: join_lds_for(cap, pool, nwords)
   int32_t groupby;      // no build side: GROUP BY the probe side's key
+  int32_t reps;         // accumulator replicas (power of two, 1..16)
+  // GROUP BY of few keys through per-workgroup partials instead of partitions
+  int32_t local;
+  int32_t lcap;         // its key table's slots (a power of two)
+  int32_t limit;        // the most keys it holds (lcap / 2)
+  int32_t lgrid;        // workgroups that aggregate (partials)
+  int32_t kgrid;        // workgroups that collect the keys
   int32_t pad_;
 };
 
-// LDS bytes of one partition's table and accumulator pool.
-size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords);
+// LDS bytes of one partition's table and accumulator pool (with `reps`
+// replicas of the accumulators).
+size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps = 1);
 
 // Arguments of every run-time compiled kernel; must match vqj::Args in
 // jit_template.hip field for field.
@@ -246,6 +265,16 @@ struct JitArgs {
   int64_t vstride;
   int32_t P;
   uint32_t lds_words;  // dynamic LDS of the launch, in 4-byte words
+  // GROUP BY through per-workgroup partials (vq_hash_local)
+  const int64_t* lkeys;
+  uint32_t* lflags;
+  int64_t* lgkey;
+  int32_t lcap;
+  int32_t lnw;
+  int32_t vop[8];   // per aggregate value: 0 float sum, 1 int sum, 2/3 float min/max, 4/5 int min/max
+  int32_t vword[8];
+  int32_t vlsb[8];
+  int32_t vw[8];
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -279,4 +308,23 @@ void join_allow_lds(int32_t bytes);
 
 hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd,
                        const JoinPlan& plan, const JoinBufs& bufs, hipStream_t s, hipEvent_t* ev);
+
+// The run-time compiled per-workgroup GROUP BY of one probe side
+// (vq_hash_local): its kernel, grid and dynamic LDS, and the arguments that
+// do not depend on buffers.
+struct LocalLaunch {
+  hipFunction_t fn = nullptr;
+  int32_t grid = 0;
+  uint32_t lds = 0;
+  JitArgs args;
+};
+constexpr int32_t kLocalLds = 60 * 1024;  // vq_hash_local's dynamic LDS
+int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunch* out,
+                      std::string* err);
+hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s);
+// LDS the per-workgroup path needs for `limit` keys of `nwords` words.
+size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords);
+hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
+                        const JoinPlan& plan, const JoinBufs& bufs, hipStream_t s,
+                        hipEvent_t* ev);
 }  // namespace vq

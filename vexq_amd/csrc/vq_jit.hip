@@ -16,7 +16,7 @@
 #include <string>
 #include <unordered_map>
 
-#include "jit_template_src.h"  // generated by vexq_amd/build.py: kJitTemplateSrc
+#include "jit_template_src.h"  // generated by vexq_amd/build.py: kFixChunksSrc, kJitTemplateSrc
 #include "vq_internal.h"
 
 namespace vq {
@@ -29,6 +29,7 @@ struct Module {
   hipFunction_t filter_bits = nullptr;
   hipFunction_t part_count = nullptr;
   hipFunction_t part_scatter = nullptr;
+  hipFunction_t hash_local = nullptr;
 };
 
 std::mutex g_mu;
@@ -108,7 +109,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_KEYSLOT " + std::to_string(key_slot) + "\n";
   s += "#define VQ_NV " + std::to_string(nv) + "\n";
   s += "#define VQ_PROBE " + std::to_string(probe) + "\n";
-  if (kernels & 4) s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
+  s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
@@ -133,7 +134,7 @@ Module* get_module(const std::string& pro, std::string* err) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cache.find(pro);
   if (it != g_cache.end()) return it->second;
-  const std::string src = pro + kJitTemplateSrc;
+  const std::string src = pro + kFixChunksSrc + kJitTemplateSrc;
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "vq_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
     if (err) *err = "hiprtcCreateProgram failed";
@@ -165,6 +166,8 @@ Module* get_module(const std::string& pro, std::string* err) {
   if (ok && pro.find("#define VQ_KERNELS 4") != std::string::npos)
     ok = hipModuleGetFunction(&m->part_count, m->mod, "vq_part_count") == hipSuccess &&
          hipModuleGetFunction(&m->part_scatter, m->mod, "vq_part_scatter") == hipSuccess;
+  if (ok && pro.find("#define VQ_KERNELS 8") != std::string::npos)
+    ok = hipModuleGetFunction(&m->hash_local, m->mod, "vq_hash_local") == hipSuccess;
   if (!ok) {
     delete m;
     if (err) *err = "hipModuleLoadData failed";
@@ -280,6 +283,59 @@ hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, 
   const uint32_t lds = count ? pl.count_lds : pl.scatter_lds;
   a.lds_words = lds / 4;
   return launch(count ? pl.count : pl.scatter, (unsigned)pl.grid, lds, s, a, 1024);
+}
+
+int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunch* out,
+                      std::string* err) {
+  if (jit_disabled()) return fail_jit(err, "JIT disabled");
+  int nv = 0;
+  const QDesc d = side_desc(sd, &jd, &nv);
+  const Flat f = flatten_terms(d);
+  Module* m = get_module(prologue(d, f, 8, sd.key_slot, nv, 1, 0), err);
+  if (!m) return -1;
+  out->fn = m->hash_local;
+  out->lds = kLocalLds;
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+  }
+  int per_cu = 0;
+  hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->hash_local, 512, out->lds);
+  if (per_cu <= 0) per_cu = 1;
+  const int64_t g = (int64_t)cus * per_cu;
+  out->grid = (int32_t)std::max<int64_t>(1, d.ntiles < g ? d.ntiles : g);
+  fill_common(d, f, out->args);
+  JitArgs& a = out->args;
+  a.lcap = lcap;
+  a.lnw = jd.nwords;
+  int v = 0;
+  for (int j = 0; j < jd.naggs; j++) {
+    const JAgg& g = jd.agg[j];
+    if (g.e.nfactors == 0) continue;
+    int op;
+    if (g.kind == VQ_AGG_SUM || g.kind == VQ_AGG_AVG) op = g.e.is_float ? 0 : 1;
+    else if (g.kind == VQ_AGG_MIN) op = g.accum == VQ_F64 ? 2 : 4;
+    else op = g.accum == VQ_F64 ? 3 : 5;
+    a.vop[v] = op;
+    a.vword[v] = g.word0;
+    a.vlsb[v] = g.fs.lsb;
+    a.vw[v] = g.fs.w;
+    v++;
+  }
+  return 0;
+}
+
+hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s) {
+  JitArgs a = ll.args;
+  a.lkeys = B.l.keys;
+  a.lflags = B.flags;
+  a.lgkey = B.l.gkey;
+  a.part = B.l.part;
+  a.lds_words = ll.lds / 4;
+  return launch(ll.fn, (unsigned)ll.grid, ll.lds, s, a, 512);
 }
 
 int jit_dense_grid(const QDesc& d, std::string* err) {
