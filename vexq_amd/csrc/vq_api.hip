@@ -1196,6 +1196,7 @@ struct JoinJob {
   int32_t pool_req = 0;
   bool no_local = false;   // GROUP BY: too many keys for per-workgroup partials
   LocalLaunch ll;
+  int attempts = 0;        // plans run (more than one: a table or pool overflowed)
   // per attempt
   JoinPlan plan;
   int32_t pool_max = 0;
@@ -1210,6 +1211,16 @@ struct JoinJob {
 };
 
 constexpr double kLoad = 0.4;
+
+// Partition tables sized for the keys a partition may get, not just the mean:
+// `per` keys on average, with Poisson spread over the partitions.
+double keys_high(double per) { return per + 4 * std::sqrt(per) + 8; }
+int32_t table_cap_for(double per, int32_t cap_max) {
+  int32_t cap = 8;
+  while (cap < cap_max && (cap * kLoad < per || cap < keys_high(per))) cap *= 2;
+  return cap;
+}
+constexpr int32_t kPoolMax = 2048;  // the join kernel's top-k holds 2 entries per thread
 // A GROUP BY of at most kLocalLimit keys runs through per-workgroup partials
 // (one table of kLocalCap slots); one of more, or whose key count is not
 // known yet and turns out larger, through the partitions.
@@ -1300,8 +1311,18 @@ int join_setup(JoinJob& j) {
     // enough partitions for the keys to fit their tables, and enough for the
     // work: one workgroup per partition takes at most ~16K probe rows
     // (a handful of keys over millions of rows still spreads over the CUs)
+    // (a GROUP BY needs a pool entry for every key of its partition, so
+    // the partition's keys must fit the pool next to the table as well)
+    auto fits = [&](int b) {
+      const double per = (double)j.est / (1 << b);
+      if (per > kLoad * j.cap_max) return false;
+      if (!j.groupby) return true;
+      const double need = keys_high(per) + 56;
+      return need <= kPoolMax &&
+             join_lds_for(table_cap_for(per, j.cap_max), (int32_t)need, nw) <= (size_t)j.lds_max;
+    };
     j.bits = 0;
-    while (j.bits < 16 && (double)j.est / (1 << j.bits) > kLoad * j.cap_max) j.bits++;
+    while (j.bits < 16 && !fits(j.bits)) j.bits++;
     const int64_t prows = j.probe->nrows;
     int pbits = 0;
     while (pbits < 10 && (prows >> pbits) > 16384) pbits++;
@@ -1362,14 +1383,18 @@ int join_plan(JoinJob& j) {
   plan.bits = j.bits;
   plan.P = 1 << j.bits;
   const double per = (double)j.est / plan.P;
-  int32_t cap = 8;
-  while (cap < j.cap_max && cap * kLoad < per) cap *= 2;
+  const int32_t cap = table_cap_for(per, j.cap_max);
   plan.cap = cap;
-  // the largest pool the LDS and the kernel's top-k (2048 entries) allow
-  j.pool_max = std::min<int32_t>(2048, cap);
-  while (j.pool_max > 64 && join_lds_for(cap, j.pool_max, j.nw) > (size_t)j.lds_max) j.pool_max /= 2;
-  int32_t pool = j.groups_seen >= 0 ? (int32_t)std::min<int64_t>(2048, 2 * j.groups_seen / plan.P + 64)
-                                    : cap / 2;
+  // the largest pool the LDS and the kernel's top-k allow
+  j.pool_max = std::min<int32_t>(kPoolMax, cap);
+  while (j.pool_max > 64 && join_lds_for(cap, j.pool_max, j.nw) > (size_t)j.lds_max) j.pool_max -= 64;
+  // twice the matched keys per partition of the last query of this shape
+  // (a GROUP BY matches every key: at least their high estimate), else half
+  // the table
+  const double gper = (double)j.groups_seen / plan.P;
+  int32_t pool = j.groups_seen >= 0
+                     ? (int32_t)std::min<double>(kPoolMax, std::max(2 * gper, j.groupby ? keys_high(gper) : 0) + 64)
+                     : cap / 2;
   if (j.pool_req) pool = j.pool_req;
   plan.pool = std::max<int32_t>(64, std::min(pool, j.pool_max));
   // accumulator replicas when a partition's keys take many rows each (their
@@ -1523,12 +1548,13 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   const uint32_t nrec = dev_topk ? hflags[3] : ngroups_all;
   if (nrec > (uint64_t)j.rec_cap)
     return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)j.rec_cap);
-  std::vector<uint64_t> hrec;
+  std::unique_ptr<uint64_t[]> hrec;  // (not zero-filled: the copy writes every word read)
   const uint64_t* recs = j.hout;
   if (!dev_topk) {
-    hrec.resize((size_t)nrec * RW + 1);
-    if (nrec) HIP_TRY(hipMemcpy(hrec.data(), j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
-    recs = hrec.data();
+    hrec.reset(new (std::nothrow) uint64_t[(size_t)nrec * RW + 1]);
+    if (!hrec) return fail(VQ_E_NOMEM, "exec: gpu: %u result records", nrec);
+    if (nrec) HIP_TRY(hipMemcpy(hrec.get(), j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
+    recs = hrec.get();
   }
   if (stats) {
     memset(stats, 0, sizeof *stats);
@@ -1630,6 +1656,7 @@ int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
   hipSetDevice(c->device);
   for (;;) {
     if (int rc = join_plan(j)) return rc;
+    j.attempts++;
     void* ar;
     void* hp;
     if (int rc = arena_get(c, S.get(), j.bytes, &ar)) return rc;
@@ -1965,11 +1992,17 @@ JoinTune autotune(const JoinJob& proto, vq_agg_result* out) {
           t.reps = rp;
           double ms = 1e30;
           for (int rep = 0; rep < 2; rep++) {
+            // (a candidate whose first plan overflows is not taken: its
+            // device time would not count the attempts before the last)
             vq_stats st;
-            const int rc = join_impl(proto.c, proto.build, proto.build_key, proto.bp, proto.probe,
-                                     proto.probe_key, proto.pp, proto.naggs, proto.aggs,
-                                     proto.radix_bits, proto.topk, out, &st, proto.groupby, &t);
-            if (rc) {
+            std::unique_ptr<JoinJob> j(new (std::nothrow) JoinJob());
+            if (!j) break;
+            join_job_init(*j, proto.c, proto.build, proto.build_key, proto.bp, proto.probe,
+                          proto.probe_key, proto.pp, proto.naggs, proto.aggs, proto.radix_bits,
+                          proto.topk, proto.groupby);
+            j->tune = t;
+            const int rc = join_run(*j, out, &st);
+            if (rc || j->attempts > 1) {
               ms = 1e30;
               break;
             }
@@ -2023,6 +2056,7 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
     if (int rc = join_plan(j)) return rc;
   }
   std::vector<int> rcs(n, VQ_OK);
+  std::vector<std::string> errs(n);
   {
     SlotLease S(c);
     size_t dev = 0, host = 0;
@@ -2104,19 +2138,33 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
         HIP_TRY(hipStreamSynchronize(S->stream));
       }
       const float batch_ms = (flags & VQ_BATCH_NO_GRAPH) ? 0.0f : elapsed(S->ev[0], S->ev[1]);
-      for (int32_t i = 0; i < n; i++) {
-        if (!jobs[i]) continue;
-        rcs[i] = join_result(*jobs[i], &outs[i], stats ? &stats[i] : nullptr, nullptr);
-        if (stats && rcs[i] == VQ_OK) stats[i].device_ms = batch_ms;  // the whole batch's graph
-      }
+      // the instances' results, assembled by a few host threads (copying and
+      // ordering the groups of a large batch is host-memory bound)
+      std::atomic<int32_t> next{0};
+      auto assemble = [&]() {
+        hipSetDevice(c->device);
+        for (int32_t i; (i = next.fetch_add(1)) < n;) {
+          if (!jobs[i]) continue;
+          rcs[i] = join_result(*jobs[i], &outs[i], stats ? &stats[i] : nullptr, nullptr);
+          if (rcs[i] != VQ_OK && rcs[i] != kRetry) errs[i] = g_err;
+          if (stats && rcs[i] == VQ_OK) stats[i].device_ms = batch_ms;  // the whole batch's graph
+        }
+      };
+      const int nt = std::max(1, std::min<int>({8, n, (int)std::thread::hardware_concurrency()}));
+      std::vector<std::thread> helpers;
+      for (int k = 1; k < nt; k++) helpers.emplace_back(assemble);
+      assemble();
+      for (auto& h : helpers) h.join();
     }
   }
   // instances that overflowed a partition run again on their own (with the
   // estimates this run has learned), and dictionary GROUP BYs run directly
   int first_rc = VQ_OK;
+  std::string first_err;
   for (int32_t i = 0; i < n; i++) {
     const vq_query& q = qs[i];
     int rc = rcs[i];
+    if (rc != VQ_OK && rc != kRetry) g_err = errs[i];
     if (direct[i]) {
       rc = vq_agg_dict(c, q.table, q.pred, q.nkeys, q.key_cols, q.naggs, q.aggs, &outs[i],
                        stats ? &stats[i] : nullptr);
@@ -2126,7 +2174,11 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
                      j.groupby ? -1 : j.radix_bits, j.topk, &outs[i], stats ? &stats[i] : nullptr,
                      j.groupby, &j.tune);
     }
-    if (rc && first_rc == VQ_OK) first_rc = rc;
+    if (rc && first_rc == VQ_OK) {
+      first_rc = rc;
+      first_err = g_err;
+    }
   }
+  if (first_rc != VQ_OK) g_err = first_err;
   return first_rc;
 }
