@@ -164,13 +164,29 @@ __device__ __forceinline__ uint64_t agg_word_of(const JAgg& a, uint64_t cnt, uin
 // Record layout (64-bit words): key, count, first row, then one word per
 // aggregate (int64 or float64 bits).  A missing top-k candidate has first
 // row ~0.
+// Stores and loads at the device's coherence point (past this L2), for data
+// one workgroup hands to another inside a kernel: with them the hand-over
+// needs no cache write-back, only the writer's stores acknowledged before its
+// signal.
+__device__ __forceinline__ void st_dev(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_dev(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool kDev = false>
 __device__ __forceinline__ void emit_record(const JDesc& jd, uint64_t* rec, int64_t key,
                                             uint64_t cnt, uint32_t mult, uint32_t first,
                                             const unsigned long long* acc) {
-  rec[0] = (uint64_t)key;
-  rec[1] = cnt * mult;
-  rec[2] = first;
-  for (int j = 0; j < jd.naggs; j++) rec[3 + j] = agg_word_of(jd.agg[j], cnt, mult, acc);
+  auto put = [&](int i, uint64_t v) {
+    if (kDev) st_dev(&rec[i], v);
+    else rec[i] = v;
+  };
+  put(0, (uint64_t)key);
+  put(1, cnt * mult);
+  put(2, first);
+  for (int j = 0; j < jd.naggs; j++) put(3 + j, agg_word_of(jd.agg[j], cnt, mult, acc));
 }
 
 // ---- top-k (ExternalSort on aggregate 0 DESC + Limit, exec/sort.go:130-163):
@@ -193,12 +209,29 @@ __device__ __forceinline__ bool kbetter(const KItem& a, const KItem& b) {
   return a.v > b.v || (a.v == b.v && a.first < b.first);
 }
 
+__device__ __forceinline__ KItem knone();
+__device__ __forceinline__ void st_item(KItem* p, const KItem& x);
+__device__ __forceinline__ KItem ld_item(const KItem* p);
+
 __device__ __forceinline__ KItem knone() {
   KItem k;
   k.v = INT64_MIN;
   k.first = 0xFFFFFFFFu;
   k.idx = 0xFFFFFFFFu;
   return k;
+}
+
+__device__ __forceinline__ void st_item(KItem* p, const KItem& x) {
+  st_dev((uint64_t*)p, (uint64_t)x.v);
+  st_dev((uint64_t*)p + 1, (uint64_t)x.first | ((uint64_t)x.idx << 32));
+}
+__device__ __forceinline__ KItem ld_item(const KItem* p) {
+  KItem x;
+  x.v = (int64_t)ld_dev((const uint64_t*)p);
+  const uint64_t w = ld_dev((const uint64_t*)p + 1);
+  x.first = (uint32_t)w;
+  x.idx = (uint32_t)(w >> 32);
+  return x;
 }
 
 __device__ __forceinline__ KItem kshfl(const KItem& c, int src) {
@@ -303,6 +336,21 @@ __device__ __forceinline__ unsigned long long lds_load_key(unsigned long long* p
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The number of items a[0..n) that rank ahead of x (an absent item never
+// does), with eight LDS reads in flight at a time.
+__device__ __forceinline__ uint32_t rank_in(const KItem* a, uint32_t n, const KItem& x) {
+  uint32_t r = 0, b = 0;
+  for (; b + 8 <= n; b += 8) {
+    KItem y[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) y[q] = a[b + q];
+#pragma unroll
+    for (int q = 0; q < 8; q++) r += kbetter(y[q], x) ? 1u : 0u;
+  }
+  for (; b < n; b++) r += kbetter(a[b], x) ? 1u : 0u;
+  return r;
+}
+
 // One 1024-thread workgroup per partition.  LDS:
 //   table  keys u64[cap+1] | acc u64[reps][pool][nwords] | tag u32[cap+1]
 //   pool   slot u32[pool] | mult u32[pool] | cnt u32[reps][pool] |
@@ -318,7 +366,68 @@ __device__ __forceinline__ unsigned long long lds_load_key(unsigned long long* p
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kClaimed = 0x80000000u;
 constexpr uint32_t kPending = 0xFFFFFFFFu;  // a lane is taking the slot's pool entry
+
+// Position of this lane's entry when the lanes with `pred` append to the
+// list *ctr counts: one LDS atomic per wave.  Call from the whole wave.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* ctr) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return 0;
+  const uint32_t below =
+      __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+  return (uint32_t)__builtin_amdgcn_readlane((int)base, leader) + below;
+}
+
+// Drops from a[0..n) (n <= kJoinThreads, in LDS) every item that cannot make
+// the best k, and absent items; the survivors are compacted to the front of
+// a, and their count is returned.  The bound: when a holds `lists` runs of k
+// items each in rank order, the best of the runs' k-th items; otherwise the
+// k-th best of a sample (the first 64, ranked by wave 0 lane against lane).
+// Either is at or ahead of the whole set's k-th best, so the best k
+// survive.  Call from the whole workgroup.
+__device__ uint32_t prune_topk(KItem* a, uint32_t n, int k, uint32_t* s_cnt, KItem* s_thr,
+                               uint32_t lists) {
+  const bool bound = lists > 0 || n > 64;
+  if (lists > 0) {
+    if (threadIdx.x < 64) {
+      const int l = threadIdx.x & 63;
+      const KItem w = wave_best((uint32_t)l < lists ? a[(size_t)l * k + k - 1] : knone());
+      if (threadIdx.x == 0) *s_thr = w;
+    }
+  } else if (n > 64 && threadIdx.x < 64) {
+    const KItem x = a[threadIdx.x];
+    const bool valid = x.idx != kNone;
+    const uint32_t xlo = (uint32_t)(uint64_t)x.v, xhi = (uint32_t)((uint64_t)x.v >> 32);
+    uint32_t r = 0;
+    for (int l = 0; l < 64; l++) {
+      KItem y;
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)xlo, l);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)xhi, l);
+      y.v = (int64_t)(((uint64_t)hi << 32) | lo);
+      y.first = (uint32_t)__builtin_amdgcn_readlane((int)x.first, l);
+      y.idx = (uint32_t)__builtin_amdgcn_readlane((int)x.idx, l);
+      r += (y.idx != kNone && kbetter(y, x)) ? 1u : 0u;
+    }
+    const uint64_t m = __ballot(valid && r == (uint32_t)(k - 1));
+    if (threadIdx.x == 0) *s_thr = knone();
+    if (m && (int)threadIdx.x == __ffsll((unsigned long long)m) - 1) *s_thr = x;
+  }
+  const KItem x = threadIdx.x < n ? a[threadIdx.x] : knone();
+  if (threadIdx.x == 0) *s_cnt = 0;
+  __syncthreads();
+  const KItem t = bound ? *s_thr : knone();
+  const bool keep = x.idx != kNone && (t.idx == kNone || !kbetter(t, x));
+  const uint32_t at = wave_append(keep, s_cnt);
+  if (keep) a[at] = x;
+  __syncthreads();
+  return *s_cnt;
+}
+
 constexpr int kPoolPerThread = 2;  // pool <= kPoolPerThread * kJoinThreads
+constexpr int kRankMax = 512;      // top-k by counting ranks up to this many groups
+static_assert(kTopkFan * kMaxK <= kRankMax, "a merge's candidates fit the ranking room");
 
 // Eight waves per SIMD (at most 64 VGPRs): two workgroups per CU when the
 // tables leave room in LDS.
@@ -340,6 +449,10 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   uint32_t* cnt = pmult + NP;                                // [R][NP]
   uint32_t* first = cnt + (size_t)R * NP;                    // [R][NP]
   const int lg = __ffs(cap) - 1;  // log2(cap)
+  // the top-k's ranking room after the pool, 16-byte aligned (an offset from
+  // smem, so the compiler keeps it an LDS address)
+  KItem* ritems = (KItem*)(smem + ((((size_t)ns * 12 + (size_t)R * NP * NW * 8 + (size_t)NP * 8 +
+                                     (size_t)R * NP * 8) + 15) / 16) * 2);
   // wave w updates accumulator replica w mod R: with few hot keys the
   // replicas spread what would be one LDS address's atomics over R addresses
   const int myrep = (int)(threadIdx.x >> 6) & (R - 1);
@@ -618,101 +731,141 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     return;
   }
 
-  // top-k: the partition's best k as candidate records ...
+  // top-k: the partition's best k as candidates (records), then a tree of
+  // merges over the partitions, kTopkFan children to a node, where the last
+  // child to finish merges its parent's candidates, up to the final k
   const int k = plan.topk;
   const int lane = threadIdx.x & 63;
-  if (!fail && threadIdx.x == 0 && s_nrec) atomicAdd(&flags[1], s_nrec);
-  KItem it[kPoolPerThread];
-#pragma unroll
-  for (int j = 0; j < kPoolPerThread; j++) {
-    const uint32_t i = threadIdx.x + j * kJoinThreads;
-    it[j] = knone();
-    if (!fail && i < used && cnt[i] != 0) {
-      it[j].v = jd.naggs > 0 ? (int64_t)agg_word_of(jd.agg[0], cnt[i], pmult[i], &acc[(size_t)i * NW])
-                             : 0;
-      it[j].first = first[i];
-      it[j].idx = i;
-    }
-  }
-  // each wave's best k of its lanes' items, then wave 0's best k of those
-  static_assert(kPoolPerThread == 2, "items per lane");
   const int wv = threadIdx.x >> 6;
-  KItem w = wave_topk_two(it[0], it[1], k);
-  if (lane < k) s_cand[wv * k + lane] = w;
-  __syncthreads();
-  if (wv == 0) {
-    w = wave_topk_mem(s_cand, 0, (uint32_t)(kWaves * k), k);
-    // The partition's k-th best bounds the final k-th best from below: no
-    // candidate ranked under the largest such bound can make the top k.
-    // (kept as value ^ sign bit, an unsigned order-preserving form whose
-    // zero-filled start is the smallest value)
-    const KItem kth = kshfl(w, k - 1);
-    if (lane == 0 && kth.idx != kNone)
-      atomicMax((unsigned long long*)(flags + 8), (unsigned long long)kth.v ^ (1ULL << 63));
-    if (lane < k) {
-      const uint32_t c = (uint32_t)p * k + lane;
-      uint64_t* rec = B.cand + (size_t)c * RW;
-      KItem ci = knone();
-      if (w.idx != kNone) {
-        const uint32_t i = w.idx, sl = pslot[i];
-        const int64_t key = (int)sl == cap ? kEmptyKey : (int64_t)keys[sl];
-        emit_record(jd, rec, key, cnt[i], pmult[i], first[i], &acc[(size_t)i * NW]);
-        ci.v = w.v;
-        ci.first = w.first;
-        ci.idx = c;
+  if (!fail && threadIdx.x == 0 && s_nrec) atomicAdd(&flags[1], s_nrec);
+  if (fail || used <= (uint32_t)kRankMax) {
+    // few groups: each one's rank is the number of groups ahead of it
+    KItem* items = ritems;
+    const uint32_t nu = fail ? 0u : used;
+    for (uint32_t i = threadIdx.x; i < nu; i += kJoinThreads) {
+      KItem x = knone();
+      if (cnt[i] != 0) {
+        x.v = jd.naggs > 0 ? (int64_t)agg_word_of(jd.agg[0], cnt[i], pmult[i], &acc[(size_t)i * NW]) : 0;
+        x.first = first[i];
+        x.idx = i;
       }
-      B.citems[c] = ci;
+      items[i] = x;
+    }
+    __syncthreads();
+    const uint32_t n = prune_topk(items, nu, k, &s_nkeys, &s_cand[0], 0);
+    for (uint32_t a = threadIdx.x; a < n; a += kJoinThreads) {
+      const KItem x = items[a];
+      const uint32_t r = rank_in(items, n, x);
+      if (r >= (uint32_t)k) continue;
+      const uint32_t c = (uint32_t)p * k + r;
+      const uint32_t i = x.idx, sl = pslot[i];
+      const int64_t key = (int)sl == cap ? kEmptyKey : (int64_t)keys[sl];
+      emit_record<true>(jd, B.cand + (size_t)c * RW, key, cnt[i], pmult[i], first[i], &acc[(size_t)i * NW]);
+      KItem ci = x;
+      ci.idx = c;
+      st_item(&B.citems[c], ci);
+    }
+    for (uint32_t r = n + threadIdx.x; r < (uint32_t)k; r += kJoinThreads)
+      st_item(&B.citems[(uint32_t)p * k + r], knone());
+  } else {
+    // many groups: each wave's best k of its lanes' items, then wave 0's
+    // best k of those
+    KItem it[kPoolPerThread];
+#pragma unroll
+    for (int j = 0; j < kPoolPerThread; j++) {
+      const uint32_t i = threadIdx.x + j * kJoinThreads;
+      it[j] = knone();
+      if (!fail && i < used && cnt[i] != 0) {
+        it[j].v = jd.naggs > 0 ? (int64_t)agg_word_of(jd.agg[0], cnt[i], pmult[i], &acc[(size_t)i * NW])
+                               : 0;
+        it[j].first = first[i];
+        it[j].idx = i;
+      }
+    }
+    static_assert(kPoolPerThread == 2, "items per lane");
+    KItem w = wave_topk_two(it[0], it[1], k);
+    if (lane < k) s_cand[wv * k + lane] = w;
+    __syncthreads();
+    if (wv == 0) {
+      w = wave_topk_mem(s_cand, 0, (uint32_t)(kWaves * k), k);
+      if (lane < k) {
+        const uint32_t c = (uint32_t)p * k + lane;
+        KItem ci = knone();
+        if (w.idx != kNone) {
+          const uint32_t i = w.idx, sl = pslot[i];
+          const int64_t key = (int)sl == cap ? kEmptyKey : (int64_t)keys[sl];
+          emit_record<true>(jd, B.cand + (size_t)c * RW, key, cnt[i], pmult[i], first[i], &acc[(size_t)i * NW]);
+          ci = w;
+          ci.idx = c;
+        }
+        st_item(&B.citems[c], ci);
+      }
     }
   }
-  // ... and the last workgroup to finish merges every partition's candidates
-  if (threadIdx.x < 64) __threadfence();  // wave 0 wrote them: release
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(&flags[4], 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // every partition's candidates that reach the floor (the largest
-  // partition k-th best), compacted into LDS, then ranked by wave 0
-  const uint32_t ncand = gridDim.x * (uint32_t)k;
-  const int64_t floor_v = (int64_t)(*(volatile unsigned long long*)(flags + 8) ^ (1ULL << 63));
-  // (an absent bound reads as the smallest value)
-  const uint32_t room = (uint32_t)((size_t)plan.lds_bytes / sizeof(KItem));
-  KItem* keep = (KItem*)smem;
-  __syncthreads();
-  if (threadIdx.x == 0) s_nrec = 0;
-  __syncthreads();
-  for (uint32_t ci = threadIdx.x; ci < ncand; ci += kJoinThreads) {
-    const KItem x = B.citems[ci];
-    if (x.idx == kNone) continue;
-    if (x.v < floor_v) continue;
-    const uint32_t at = atomicAdd(&s_nrec, 1u);
-    if (at < room) keep[at] = x;
+  // Climb: the last of a node's children to arrive merges the children's
+  // candidates into the node's best k (ranked by counting, at most
+  // kTopkFan * k items).  Candidates keep the index of their partition
+  // record; the root's best k become the output.  Candidates and records go
+  // through the device's coherence point (st_dev / ld_dev): every wave waits
+  // until its stores are acknowledged, then after a barrier one thread
+  // signals.  (A device-scope fence instead writes back the whole L2, which
+  // costs microseconds each time.)
+  uint32_t node = p, nl = gridDim.x, off_up = 0;
+  const KItem* level = B.citems;
+  while (nl > 1) {
+    const uint32_t parent = node / kTopkFan;
+    const uint32_t nup = (nl + kTopkFan - 1) / kTopkFan;
+    const uint32_t nch = nl - parent * kTopkFan < (uint32_t)kTopkFan ? nl - parent * kTopkFan : kTopkFan;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&B.tctr[off_up + parent], 1u) == nch - 1;
+    __syncthreads();
+    if (!s_last) return;
+    const uint32_t nitems = nch * (uint32_t)k;  // at most kTopkFan * kMaxK
+    KItem* mi = ritems;
+    KItem x = knone();
+    if (threadIdx.x < nitems) x = ld_item(&level[(size_t)parent * kTopkFan * k + threadIdx.x]);
+    if (threadIdx.x < nitems) mi[threadIdx.x] = x;
+    const int nv = __syncthreads_count(threadIdx.x < nitems && x.idx != kNone);
+    const uint32_t ns_ = prune_topk(mi, nitems, k, &s_nkeys, &s_cand[0], nch);
+    KItem* dst = B.tree + (size_t)(off_up + parent) * k;
+    if (threadIdx.x < ns_) {
+      x = mi[threadIdx.x];
+      const uint32_t r = rank_in(mi, ns_, x);
+      if (r < (uint32_t)k) st_item(&dst[r], x);
+    }
+    if ((int)threadIdx.x >= nv && (int)threadIdx.x < k) st_item(&dst[threadIdx.x], knone());
+    level = B.tree + (size_t)off_up * k;
+    node = parent;
+    off_up += nup;
+    nl = nup;
   }
+  // the root: its best k, as records (written by this workgroup's waves
+  // when there is a single partition)
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  const uint32_t nkeep = s_nrec;
-  // every wave ranks a slice of the survivors, wave 0 the waves' best (too
-  // many survivors for LDS, from ties at the bound: all candidates, from
-  // memory)
-  const KItem* src = nkeep <= room ? keep : B.citems;
-  const uint32_t n = nkeep <= room ? nkeep : ncand;
-  const uint32_t slice = (n + kWaves - 1) / kWaves;
-  const uint32_t lo = (uint32_t)wv * slice < n ? (uint32_t)wv * slice : n;
-  const uint32_t hi = lo + slice < n ? lo + slice : n;
-  w = wave_topk_mem(src, lo, hi, k);
-  if (lane < k) s_cand[wv * k + lane] = w;
-  __syncthreads();
-  if (wv == 0) {
-    w = wave_topk_mem(s_cand, 0, (uint32_t)(kWaves * k), k);
-    if (lane < k && w.idx != kNone)
-      for (int q = 0; q < RW; q++) B.out[(size_t)lane * RW + q] = B.cand[(size_t)w.idx * RW + q];
-    const uint32_t got = (uint32_t)__popcll(__ballot(lane < k && w.idx != kNone));
-    if (lane == 0) flags[3] = got;
-  }
+  const KItem* root = level + (size_t)node * k;
+  KItem x = knone();
+  if ((int)threadIdx.x < k) x = ld_item(&root[threadIdx.x]);
+  if ((int)threadIdx.x < k && x.idx != kNone)
+    for (int q = 0; q < RW; q++) B.out[(size_t)threadIdx.x * RW + q] = ld_dev(&B.cand[(size_t)x.idx * RW + q]);
+  const int got = __syncthreads_count((int)threadIdx.x < k && x.idx != kNone);
+  if (threadIdx.x == 0) flags[3] = (uint32_t)got;
 }
 
-size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps) {
+size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps, bool topk) {
   const size_t ns = (size_t)cap + 1;
-  return ns * (8 + 4) + (size_t)pool * (4 + 4) + (size_t)reps * pool * (4 + 4 + 8 * (size_t)nwords);
+  return ns * (8 + 4) + (size_t)pool * (4 + 4) + (size_t)reps * pool * (4 + 4 + 8 * (size_t)nwords) +
+         (topk ? 16 * (size_t)kRankMax + 16 : 0);
+}
+
+int32_t topk_tree_nodes(int32_t P) {
+  int32_t total = 0;
+  for (int32_t n = P; n > 1;) {
+    n = (n + kTopkFan - 1) / kTopkFan;
+    total += n;
+  }
+  return total;
 }
 
 uint32_t part_scatter_lds(int32_t P, int32_t pbuf, int32_t entry) {
@@ -983,7 +1136,7 @@ hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd
                        const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
   const int P = plan.P;
   if (ev) hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, B.flags, (int64_t)16);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, B.flags, (int64_t)16 + plan.tnodes);
   const bool has_build = !plan.groupby;
   hipError_t e = has_build ? jit_part_launch(b, B.b, P, s, true) : hipSuccess;
   if (e != hipSuccess) return e;

@@ -1304,8 +1304,10 @@ int join_setup(JoinJob& j) {
   hipDeviceGetAttribute(&j.lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
   if (j.lds_dev <= 0) j.lds_dev = 64 * 1024;
   j.lds_max = j.lds_dev - 8192;  // the join kernel's own static LDS (about 4 KB)
+  j.dev_topk = j.topk > 0 && j.topk <= 16;
   j.cap_max = 4096;
-  while (j.cap_max > 8 && join_lds_for(j.cap_max, 64, nw) > (size_t)j.lds_max) j.cap_max /= 2;
+  while (j.cap_max > 8 && join_lds_for(j.cap_max, 64, nw, 1, j.dev_topk) > (size_t)j.lds_max)
+    j.cap_max /= 2;
   j.bits = j.radix_bits;
   if (j.bits < 0) {
     // enough partitions for the keys to fit their tables, and enough for the
@@ -1329,7 +1331,6 @@ int join_setup(JoinJob& j) {
     j.bits = std::max(j.bits, pbits);
     j.bits = std::max(0, std::min(16, j.bits + j.tune.bits_delta));
   }
-  j.dev_topk = j.topk > 0 && j.topk <= 16;
   return VQ_OK;
 }
 
@@ -1387,7 +1388,8 @@ int join_plan(JoinJob& j) {
   plan.cap = cap;
   // the largest pool the LDS and the kernel's top-k allow
   j.pool_max = std::min<int32_t>(kPoolMax, cap);
-  while (j.pool_max > 64 && join_lds_for(cap, j.pool_max, j.nw) > (size_t)j.lds_max) j.pool_max -= 64;
+  while (j.pool_max > 64 && join_lds_for(cap, j.pool_max, j.nw, 1, j.dev_topk) > (size_t)j.lds_max)
+    j.pool_max -= 64;
   // twice the matched keys per partition of the last query of this shape
   // (a GROUP BY matches every key: at least their high estimate), else half
   // the table
@@ -1405,17 +1407,18 @@ int join_plan(JoinJob& j) {
   {
     const double keys_pp = std::max(1.0, (double)(j.groups_seen >= 0 ? j.groups_seen : j.est) / plan.P);
     const double rows_pk = (double)j.probe->nrows / plan.P / keys_pp;
-    const size_t one = join_lds_for(cap, plan.pool, j.nw, 1);
+    const size_t one = join_lds_for(cap, plan.pool, j.nw, 1, j.dev_topk);
     const size_t half = (size_t)j.lds_dev / 2 - 8192;
     const size_t budget = one <= half ? half : (size_t)j.lds_max;
     int reps = j.tune.reps > 0 ? j.tune.reps : rows_pk >= 64 ? 16 : 1;
-    while (reps > 1 && join_lds_for(cap, plan.pool, j.nw, reps) > budget) reps /= 2;
+    while (reps > 1 && join_lds_for(cap, plan.pool, j.nw, reps, j.dev_topk) > budget) reps /= 2;
     plan.reps = reps;
   }
-  plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, j.nw, plan.reps);
+  plan.lds_bytes = (int32_t)join_lds_for(cap, plan.pool, j.nw, plan.reps, j.dev_topk);
   plan.lds_max = j.lds_max;
   plan.groupby = j.groupby ? 1 : 0;
   plan.topk = j.dev_topk ? j.topk : 0;
+  plan.tnodes = j.dev_topk ? topk_tree_nodes(plan.P) : 0;
   plan.nvals = j.nvals;
   const int P = plan.P;
   plan.bbuf = part_buffer_entries(P, 8, j.lds_dev);
@@ -1453,8 +1456,9 @@ int join_plan(JoinJob& j) {
   bytes += (size_t)j.cap_b * 8 + (size_t)(P + 1) * 4 + (size_t)j.B.b.G * P * 4;
   bytes += (size_t)j.cap_p * 12 + (size_t)std::max(j.nvals, 1) * j.cap_p * 8 + (size_t)(P + 1) * 4 +
            (size_t)j.B.p.G * P * 4;
-  bytes += 16 * 4 + (size_t)2 * P * 4;
-  bytes += j.dev_topk ? (size_t)P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8
+  bytes += (16 + (size_t)plan.tnodes) * 4 + (size_t)2 * P * 4;
+  bytes += j.dev_topk ? (size_t)P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8 +
+                            (size_t)plan.tnodes * j.topk * 16
                       : (size_t)j.rec_cap * j.RW * 8;
   bytes += 16 * 256;  // alignment of the carved pieces
   j.bytes = (bytes + 255) & ~(size_t)255;
@@ -1466,7 +1470,8 @@ void join_bind(JoinJob& j, char* dev, char* host) {
   const int P = j.plan.P;
   Carve cv;
   JoinBufs& B = j.B;
-  B.flags = cv.take<uint32_t>(dev, 16);
+  B.flags = cv.take<uint32_t>(dev, 16 + (size_t)j.plan.tnodes);
+  B.tctr = B.flags + 16;
   j.hflags = (uint32_t*)host;
   j.hout = (uint64_t*)(host + 64);
   if (j.plan.local) {
@@ -1493,6 +1498,7 @@ void join_bind(JoinJob& j, char* dev, char* host) {
     B.cand = cv.take<uint64_t>(dev, (size_t)P * j.topk * j.RW);
     B.citems = (KItem*)cv.take<uint64_t>(dev, (size_t)P * j.topk * 2);
     B.out = cv.take<uint64_t>(dev, (size_t)16 * j.RW);
+    B.tree = (KItem*)cv.take<uint64_t>(dev, (size_t)std::max(j.plan.tnodes, 1) * j.topk * 2);
   } else {
     B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
   }
