@@ -368,7 +368,7 @@ def main():
 
     import numpy as np  # noqa: F401
 
-    from vexq_amd import datagen, gpu
+    from vexq_amd import abi, datagen, gpu
     cpu_mod = None
     if not args.no_cpu:
         from oracle import oracle_ffi as cpu_mod
@@ -473,6 +473,28 @@ def main():
         db.close()
         dp.close()
         log(f"[bench] join: {rec['value']} GB/s")
+
+    # ---- GROUP BY one int64 key of any cardinality (hash path) ---------------
+    for name, groups in (("hash_groupby_6m_115k", 115_000), ("hash_groupby_6m_6", 6)):
+        if not want(name):
+            continue
+        log(f"[bench] {name}: generating")
+        t = datagen.keyed_table(6_000_000, groups, abi.VQ_I64, 0.0, seed=7)
+        pred, key, aggs = datagen.q_keyed(t)
+        dt = ctx.upload(t)
+        r, total, walls, mains, devs = time_gpu(lambda: ctx.agg_hash(dt, pred, key, aggs), K, W)
+        cpu = None
+        if cpu_mod is not None:
+            runs = time_cpu(lambda: cpu_mod.agg_hash(t, pred, key, aggs), max_runs=3)
+            cpu = cpu_record(runs, r.stats.bytes_alg, 1, f"the full query, {len(runs)} runs, one thread")
+        rec = operator_record(name, "hash GROUP BY pipeline (partitions, or per-workgroup partials)",
+                              r, total, walls, mains, devs, K, traffic, cpu, t.nrows,
+                              whole_pipeline=True)
+        rec["groups"] = int(r.ngroups)
+        rec["path"] = "partitions" if r.stats.partitions > 0 else "per-workgroup partials"
+        ops[name] = rec
+        dt.close()
+        log(f"[bench] {name}: {rec['value']} GB/s ({rec['path']})")
 
     if want("batch_mixed"):
         ops["batch_mixed"] = batch_bench(ctx, args.batch_instances, args.batch_max_rows)

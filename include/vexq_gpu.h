@@ -322,10 +322,14 @@ int vq_wait(vq_ticket* ticket);
 /* GROUP BY one integer column of any cardinality: HashAggregate's general
  * key path (exec/aggregate.go:361-559; the int-key path's output order,
  * exec/agg_intkey.go:206-361) for keys that are not small dictionary codes —
- * u8/u16/date/int64 columns, up to millions of groups.  Rows passing pred are
+ * u8/u16/date/int64 columns, up to millions of groups.  Up to 256 distinct
+ * keys: every workgroup aggregates an even share of the rows into its own
+ * partials, merged per group (ParallelHashAggregate's partials and
+ * mergePartialAgg, exec/parallel.go:18-314); more: rows passing pred are
  * radix-partitioned on radixHash(key) and each partition aggregated in an
- * LDS hash table; out->keys holds the key values, groups in first-occurrence
- * order; float sums are exact (fixed point) and rounded once. */
+ * LDS hash table.  out->keys holds the key values, groups in first-occurrence
+ * order; float sums are exact (fixed point) and rounded once, so both ways
+ * give the same bits. */
 int vq_agg_hash(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t key_col,
                 int32_t naggs, const vq_agg* aggs, vq_agg_result* out, vq_stats* stats);
 

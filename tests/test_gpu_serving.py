@@ -74,3 +74,27 @@ def test_submit_reports_errors(gpu_ctx, serving_tables):
     t = gpu_ctx.submit_agg_dict(dfact, pred, [fact.index("price")], aggs)  # float key
     with pytest.raises(gpu.GpuError):
         t.wait()
+
+
+def test_hash_groupbys_alone_and_in_the_mix(gpu_ctx, serving_tables):
+    """Submitted hash GROUP BYs (few keys: per-workgroup partials; many keys:
+    partitions), overlapping with each other, give the bits they give alone."""
+    from vexq_amd import abi
+    X = abi.X
+    dim, fact, ddim, dfact = serving_tables
+    aggs = [abi.agg_sum(X.col("price") * (1 - X.col("discount")), fact), abi.count_star(fact),
+            abi.agg_max("quantity", fact), abi.agg_avg("discount", fact)]
+    cases = []
+    for i, key in enumerate(["quantity", "fk", "flagA", "quantity", "fk", "flagB"]):
+        pred = abi.make_pred(fact, [("shipdate", "<=", int(datagen.DATE_MIN + 400 + 300 * i))])
+        cases.append((pred, fact.index(key)))
+    gpu_ctx.set_slots(1)
+    alone = [gpu_ctx.agg_hash(dfact, p, k, aggs) for p, k in cases]
+    assert {r.stats.partitions > 0 for r in alone} == {True, False}  # both ways ran
+    gpu_ctx.set_slots(4)
+    try:
+        mixed = [t.wait() for t in [gpu_ctx.submit_agg_hash(dfact, p, k, aggs) for p, k in cases]]
+    finally:
+        gpu_ctx.set_slots(1)
+    for a, b in zip(alone, mixed):
+        same(a, b)

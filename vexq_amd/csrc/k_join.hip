@@ -983,11 +983,17 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_keys(const LocalKeys k)
     const int64_t send = k.sp_start[s] + k.sp_rows[s];
     const int64_t rend = row0 + k.tile_rows < send ? row0 + k.tile_rows : send;
     bool over = false;
-    for (int64_t r = row0 + threadIdx.x; r < rend; r += kLocalThreads) {
+    for (int64_t r = row0 + threadIdx.x; r < rend && !over; r += kLocalThreads) {
       const int64_t key = load_key(k.col, k.type, r);
       if (key == kEmptyKey) {
         s_min = 1;
         continue;
+      }
+      // (past the limit every thread stops: a table that fills makes each
+      // further insert probe every slot)
+      if (__hip_atomic_load(&s_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > (uint32_t)k.limit) {
+        over = true;
+        break;
       }
       const int got = table_insert<__HIP_MEMORY_SCOPE_WORKGROUP>(lt, cap, lg, key);
       if (got < 0 || (got > 0 && atomicAdd(&s_n, 1u) + 1 > (uint32_t)k.limit)) over = true;
@@ -1026,12 +1032,21 @@ struct LocalMerge {
   int32_t pad_;
 };
 
-constexpr int kMergeThreads = 256;
+constexpr int kMergeThreads = 1024;
+
+__device__ __forceinline__ unsigned long long merge_op(int op, unsigned long long a,
+                                                       unsigned long long b) {
+  if (op == 0) return a + b;
+  if (op == 1) return (long long)b < (long long)a ? b : a;
+  if (op == 2) return (long long)b > (long long)a ? b : a;
+  return b < a ? b : a;
+}
 
 // One block per group index: each word of the group's partials combined over
-// the workgroups (integer adds, minima and maxima), then its record.
+// the workgroups (integer adds, minima and maxima) by one wave — its lanes'
+// loads all in flight, then a butterfly over the lanes — and then the
+// group's record.
 __global__ __launch_bounds__(kMergeThreads) void k_local_merge(const JDesc jd, const LocalMerge m) {
-  __shared__ unsigned long long s_red[kMergeThreads];
   __shared__ unsigned long long s_words[3 * VQ_MAX_AGGS + 2];
   __shared__ uint8_t s_wop[3 * VQ_MAX_AGGS + 2];
   const uint32_t g = blockIdx.x;
@@ -1050,36 +1065,18 @@ __global__ __launch_bounds__(kMergeThreads) void k_local_merge(const JDesc jd, c
     s_wop[threadIdx.x] = op;
   }
   __syncthreads();
-  for (int i = 0; i < NI; i++) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i = wv; i < NI; i += kMergeThreads / 64) {
     const int op = s_wop[i];
-    const unsigned long long id = op == 1 ? (unsigned long long)INT64_MAX
-                                  : op == 2 ? (unsigned long long)INT64_MIN
-                                  : op == 3 ? 0xFFFFFFFFull : 0ull;
-    unsigned long long v = id;
-    for (int w = threadIdx.x; w < m.grid; w += kMergeThreads) {
-      const unsigned long long o = m.part[((size_t)w * nk + g) * NI + i];
-      if (op == 0) v += o;
-      else if (op == 1) v = (long long)o < (long long)v ? o : v;
-      else if (op == 2) v = (long long)o > (long long)v ? o : v;
-      else v = o < v ? o : v;
-    }
-    s_red[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = kMergeThreads / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) {
-        const unsigned long long o = s_red[threadIdx.x + off];
-        unsigned long long x = s_red[threadIdx.x];
-        if (op == 0) x += o;
-        else if (op == 1) x = (long long)o < (long long)x ? o : x;
-        else if (op == 2) x = (long long)o > (long long)x ? o : x;
-        else x = o < x ? o : x;
-        s_red[threadIdx.x] = x;
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) s_words[i] = s_red[0];
-    __syncthreads();
+    unsigned long long v = op == 1 ? (unsigned long long)INT64_MAX
+                           : op == 2 ? (unsigned long long)INT64_MIN
+                           : op == 3 ? 0xFFFFFFFFull : 0ull;
+    for (int w = lane; w < m.grid; w += 64) v = merge_op(op, v, m.part[((size_t)w * nk + g) * NI + i]);
+    for (int off = 32; off > 0; off >>= 1)
+      v = merge_op(op, v, (unsigned long long)__shfl_xor((long long)v, off, 64));
+    if (lane == 0) s_words[i] = v;
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned long long cnt = s_words[NW];
     if (cnt == 0) return;  // a key whose rows all failed the predicate

@@ -1585,15 +1585,24 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   std::vector<std::pair<uint64_t, uint32_t>> ord(nrec);
   for (uint32_t i = 0; i < nrec; i++) ord[i] = {dev_topk ? (uint64_t)i : recs[(size_t)i * RW + 2], i};
   if (!dev_topk) {
-    // first rows are distinct row ids below the probe table's size: place
-    // each record at its first row, then read the places in row order
-    const int64_t nr = j.probe->nrows;
-    if (nrec > 4096 && nr <= 64 * (int64_t)nrec) {
-      std::vector<uint32_t> at((size_t)nr, 0xFFFFFFFFu);
-      for (uint32_t i = 0; i < nrec; i++) at[(size_t)ord[i].first] = i;
-      uint32_t k = 0;
-      for (int64_t r = 0; r < nr; r++)
-        if (at[(size_t)r] != 0xFFFFFFFFu) ord[k++] = {(uint64_t)r, at[(size_t)r]};
+    // first rows are distinct 32-bit row ids: a least-significant-digit
+    // radix sort on them (two 16-bit digits) for many groups
+    if (nrec > 4096) {
+      std::vector<std::pair<uint64_t, uint32_t>> tmp(nrec);
+      std::vector<uint32_t> pos(1 << 16);
+      for (int d = 0; d < 2; d++) {
+        const int sh = 16 * d;
+        std::fill(pos.begin(), pos.end(), 0u);
+        for (uint32_t i = 0; i < nrec; i++) pos[(ord[i].first >> sh) & 0xFFFF]++;
+        uint32_t sum = 0;
+        for (auto& x : pos) {
+          const uint32_t c = x;
+          x = sum;
+          sum += c;
+        }
+        for (uint32_t i = 0; i < nrec; i++) tmp[pos[(ord[i].first >> sh) & 0xFFFF]++] = ord[i];
+        ord.swap(tmp);
+      }
     } else {
       std::sort(ord.begin(), ord.end());
     }
@@ -1742,6 +1751,11 @@ uint64_t estimate_bytes(const Ticket& k) {
     const uint64_t nv = (uint64_t)std::max(q.naggs, 1);
     return b * 8 + p * (12 + 8 * nv) + (16ull << 20);
   }
+  if (q.kind == VQ_Q_AGG_HASH) {  // the partitioned rows, and the groups' records
+    const uint64_t p = (uint64_t)std::max<int64_t>(q.table->nrows, 1);
+    const uint64_t nv = (uint64_t)std::max(q.naggs, 1);
+    return p * (12 + 8 * nv) + p * 8 * (3 + nv) + (16ull << 20);
+  }
   return 8ull << 20;
 }
 
@@ -1751,6 +1765,9 @@ void run_ticket(Ticket* k) {
   if (q.kind == VQ_Q_AGG_DICT)
     rc = vq_agg_dict(k->c, q.table, k->has_pred ? &k->pred : nullptr, q.nkeys,
                      k->key_cols.empty() ? nullptr : k->key_cols.data(), q.naggs,
+                     k->aggs.empty() ? nullptr : k->aggs.data(), k->out, k->stats);
+  else if (q.kind == VQ_Q_AGG_HASH)
+    rc = vq_agg_hash(k->c, q.table, k->has_pred ? &k->pred : nullptr, k->key_cols[0], q.naggs,
                      k->aggs.empty() ? nullptr : k->aggs.data(), k->out, k->stats);
   else
     rc = vq_join_agg(k->c, q.build, q.build_key, k->has_build_pred ? &k->build_pred : nullptr,
@@ -1836,7 +1853,7 @@ extern "C" int vq_submit(vq_ctx* c, const vq_query* q, vq_agg_result* out, vq_st
                          vq_ticket** ticket) {
   if (!c || !q || !out || !ticket) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
   *ticket = nullptr;
-  if (q->kind != VQ_Q_AGG_DICT && q->kind != VQ_Q_JOIN_AGG)
+  if (q->kind != VQ_Q_AGG_DICT && q->kind != VQ_Q_JOIN_AGG && q->kind != VQ_Q_AGG_HASH)
     return fail(VQ_E_ARG, "exec: gpu: query kind %d", q->kind);
   if (!q->table || (q->kind == VQ_Q_JOIN_AGG && !q->build))
     return fail(VQ_E_ARG, "exec: gpu: query without its table");
@@ -1844,6 +1861,8 @@ extern "C" int vq_submit(vq_ctx* c, const vq_query* q, vq_agg_result* out, vq_st
     return fail(VQ_E_ARG, "exec: gpu: %d aggregates", q->naggs);
   if (q->kind == VQ_Q_AGG_DICT && (q->nkeys < 0 || q->nkeys > 4 || (q->nkeys > 0 && !q->key_cols)))
     return fail(VQ_E_ARG, "exec: gpu: %d group keys", q->nkeys);
+  if (q->kind == VQ_Q_AGG_HASH && (q->nkeys != 1 || !q->key_cols))
+    return fail(VQ_E_ARG, "exec: gpu: a hash GROUP BY takes one key, not %d", q->nkeys);
   Ticket* k = new (std::nothrow) Ticket();
   vq_ticket* h = new (std::nothrow) vq_ticket();
   if (!k || !h) {
@@ -1861,7 +1880,7 @@ extern "C" int vq_submit(vq_ctx* c, const vq_query* q, vq_agg_result* out, vq_st
     k->build_pred = *q->build_pred;
     k->has_build_pred = true;
   }
-  if (q->kind == VQ_Q_AGG_DICT && q->nkeys > 0) k->key_cols.assign(q->key_cols, q->key_cols + q->nkeys);
+  if (q->kind != VQ_Q_JOIN_AGG && q->nkeys > 0) k->key_cols.assign(q->key_cols, q->key_cols + q->nkeys);
   if (q->naggs > 0) k->aggs.assign(q->aggs, q->aggs + q->naggs);
   k->out = out;
   k->stats = stats;

@@ -157,6 +157,11 @@ class Context:
             capacity = topk if topk > 0 else max(1, min(build.nrows, probe.nrows))
         return self._submit(q, abi.AggOut(capacity, 1, len(aggs)), keep)
 
+    def submit_agg_hash(self, dt: "DeviceTable", pred, key_col: int, aggs,
+                        capacity: int | None = None) -> "Ticket":
+        q, out, keep = self.batch_hash(dt, pred, key_col, aggs, capacity)
+        return self._submit(q, out, keep)
+
     # -- batches ------------------------------------------------------------
     @staticmethod
     def batch_hash(dt: "DeviceTable", pred, key_col: int, aggs, capacity: int | None = None):
