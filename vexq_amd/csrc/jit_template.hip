@@ -27,6 +27,7 @@
 #define VQ_PROBE 1
 #define VQ_KERNELS 15
 #define VQ_PB 8
+#define VQ_BLOOM 1
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
@@ -116,6 +117,12 @@ struct Args {
   i32 vword[8];       // the value's first accumulator word
   i32 vlsb[8];        // float sums: the fixed-point scale
   i32 vw[8];
+  // semi-join filter of the build side's keys (VQ_BLOOM 1: the probe side
+  // drops rows whose two bits are not both set): a blocked Bloom filter,
+  // 2^bloom_lpp 64-byte lines per partition
+  const u32* bloom;
+  u32 bloom_lpp;      // log2 of the lines per partition
+  i32 pad_bloom;
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -597,7 +604,44 @@ __device__ __forceinline__ void pload_one(const Args& a, i64 r, i64 (&raw)[VQ_NS
   ((raw[C][u] = ld<kST[C]>(a.col[C], r)), ...);
 }
 
-// Rows base + u*kPB + tid, clamped into the tile.
+// A column the row's liveness depends on: the key or a predicate column.
+// With the semi-join filter the others (aggregate arguments) are loaded
+// only for the rows that live, after the test: most probe rows of a
+// selective join fail it, and a 64-byte line none of whose rows lives is
+// never fetched.
+template <int C>
+__device__ constexpr bool live_slot() {
+  if (C == VQ_KEYSLOT) return true;
+  for (int t = 0; t < VQ_NT; t++)
+    if (kTS[t] == C) return true;
+  return false;
+}
+constexpr bool kLateVals = (VQ_BLOOM != 0) && (VQ_PROBE != 0);
+
+template <int C, int U>
+__device__ __forceinline__ void pload_slot(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u) {
+  if constexpr (!kLateVals || live_slot<C>()) raw[C][u] = ld<kST[C]>(a.col[C], r);
+  else raw[C][u] = 0;
+}
+
+template <int U, int... C>
+__device__ __forceinline__ void pload_live_one(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u,
+                                               Seq<C...>) {
+  (pload_slot<C, U>(a, r, raw, u), ...);
+}
+
+template <int C>
+__device__ __forceinline__ void late_slot(const Args& a, i64 r, i64* row) {
+  if constexpr (kLateVals && !live_slot<C>()) row[C] = ld<kST[C]>(a.col[C], r);
+}
+
+template <int... C>
+__device__ __forceinline__ void late_load(const Args& a, i64 r, i64* row, Seq<C...>) {
+  (late_slot<C>(a, r, row), ...);
+}
+
+// Rows base + u*kPB + tid, clamped into the tile (with the semi-join
+// filter, the probe side's live columns only).
 template <int U>
 __device__ __forceinline__ void pload_rows(const Args& a, i64 base, i64 rend,
                                            i64 (&raw)[VQ_NS][U]) {
@@ -605,7 +649,7 @@ __device__ __forceinline__ void pload_rows(const Args& a, i64 base, i64 rend,
   for (int u = 0; u < U; u++) {
     i64 r = base + u * kPB + threadIdx.x;
     r = r < rend ? r : rend - 1;
-    pload_one(a, r, raw, u, seq<VQ_NS>{});
+    pload_live_one(a, r, raw, u, seq<VQ_NS>{});
   }
 }
 
@@ -627,6 +671,17 @@ __device__ __forceinline__ u64 value_of(const Args& a, const i64* row) {
 template <int... J>
 __device__ __forceinline__ void values_of(const Args& a, const i64* row, u64* v, Seq<J...>) {
   ((v[J] = value_of<J>(a, row)), ...);
+}
+
+// The semi-join filter's test of a key with hash h in partition p: one
+// 64-byte line of the partition's block (hash bits 20..), two bits of it
+// (hash bits 40..48 and 49..57); the same bits k_bloom_build sets.
+__device__ __forceinline__ bool bloom_test(const Args& a, u64 h, u32 p) {
+  const u32 lines = 1u << a.bloom_lpp;
+  const u32 line = p * lines + ((u32)(h >> 20) & (lines - 1));
+  const u32 b1 = (u32)(h >> 40) & 511u, b2 = (u32)(h >> 49) & 511u;
+  const u32* w = a.bloom + (u64)line * 16;
+  return ((w[b1 >> 5] >> (b1 & 31)) & (w[b2 >> 5] >> (b2 & 31)) & 1u) != 0;
 }
 
 constexpr int kNV1 = VQ_NV > 0 ? VQ_NV : 1;
@@ -653,9 +708,16 @@ __device__ __forceinline__ void prows_from(const Args& a, i64 base, i64 rend,
     const i64 r = base + u * kPB + threadIdx.x;
     q.live[u] = r < rend && pass_all(a, row, seq<VQ_NT>{});
     q.key[u] = row[VQ_KEYSLOT];
-    q.part[u] = (u32)(radix_hash((u64)q.key[u]) & mask);
+    const u64 h = radix_hash((u64)q.key[u]);
+    q.part[u] = (u32)(h & mask);
+#if VQ_BLOOM
+    q.live[u] = q.live[u] && bloom_test(a, h, q.part[u]);
+#endif
     q.row[u] = (u32)r;
-    if constexpr (kVals && VQ_PROBE && VQ_NV > 0) values_of(a, row, q.val[u], seq<VQ_NV>{});
+    if constexpr (kVals && VQ_PROBE && VQ_NV > 0) {
+      if (q.live[u]) late_load(a, r, row, seq<VQ_NS>{});
+      values_of(a, row, q.val[u], seq<VQ_NV>{});
+    }
   }
 }
 

@@ -1,17 +1,22 @@
 // k_join.hip — gfx950 kernels for the radix-partitioned hash join feeding a
 // GROUP BY on the join key (ParallelHashJoinAggregate, exec/parallel_join.go).
 //
-// Pipeline (one stream, eight launches):
+// Pipeline (one stream; eight launches, eleven with the semi-join filter):
 //   k_zero_u32       the flags to zero
 //   vq_part_count    (run-time compiled, jit_template.hip) per side: filter +
 //                    radixHash + LDS histogram per persistent workgroup ->
 //                    counts[p][w]
-//   k_part_runs,     both sides: counts -> partition starts and the start of
+//   k_part_runs,     per side: counts -> partition starts and the start of
 //   k_part_starts    each workgroup's run inside each partition
 //   vq_part_scatter  (run-time compiled) per side: each passing row into its
 //                    workgroup's run through LDS write-combining buffers: the
 //                    key, and on the probe side its row id and the values of
 //                    the aggregate arguments
+//   k_bloom_build    with the filter, between the build side's scatter and
+//                    the probe side's count: each partition's build keys as
+//                    a blocked Bloom filter, which the probe side's rows must
+//                    pass (most probe rows of a selective join match no
+//                    passing build key, and are then never copied)
 //   k_join_agg       one workgroup per partition: build the partition's
 //                    open-addressing table in LDS from its build keys, probe it
 //                    with its probe rows, aggregate into per-key accumulators
@@ -114,6 +119,31 @@ __global__ __launch_bounds__(kScanThreads) void k_part_starts(const PartScan a) 
     ps[P] = t;
     *a.total[side] = t;
   }
+}
+
+// ---- semi-join filter ---------------------------------------------------------
+// One block per partition builds the partition's filter block (2^lpp lines
+// of 64 bytes) in LDS from its build entries — two bits per key, in one line
+// chosen by the key's hash (the bits vq_part_count / vq_part_scatter test,
+// jit_template.hip bloom_test) — and writes it whole: no global atomics, no
+// zero fill.
+__global__ __launch_bounds__(256) void k_bloom_build(const int64_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ pstart,
+                                                     uint32_t* __restrict__ bloom, int32_t lpp) {
+  extern __shared__ uint32_t seg[];
+  const uint32_t p = blockIdx.x, lines = 1u << lpp, words = lines * 16;
+  for (uint32_t i = threadIdx.x; i < words; i += 256) seg[i] = 0;
+  __syncthreads();
+  for (uint32_t e = pstart[p] + threadIdx.x; e < pstart[p + 1]; e += 256) {
+    const uint64_t h = radix_hash((uint64_t)keys[e]);
+    const uint32_t line = (uint32_t)(h >> 20) & (lines - 1);
+    const uint32_t b1 = (uint32_t)(h >> 40) & 511u, b2 = (uint32_t)(h >> 49) & 511u;
+    atomicOr(&seg[line * 16 + (b1 >> 5)], 1u << (b1 & 31));
+    atomicOr(&seg[line * 16 + (b2 >> 5)], 1u << (b2 & 31));
+  }
+  __syncthreads();
+  uint32_t* out = bloom + (size_t)p * words;
+  for (uint32_t i = threadIdx.x; i < words; i += 256) out[i] = seg[i];
 }
 
 // ---- per-partition table ----------------------------------------------------
@@ -1135,30 +1165,45 @@ hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd
   if (ev) hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, B.flags, (int64_t)16 + plan.tnodes);
   const bool has_build = !plan.groupby;
-  hipError_t e = has_build ? jit_part_launch(b, B.b, P, s, true) : hipSuccess;
-  if (e != hipSuccess) return e;
-  e = jit_part_launch(p, B.p, P, s, true);
-  if (e != hipSuccess) return e;
-  // sides in the scan's order: build (when there is one), then probe
-  PartScan ps;
-  memset(&ps, 0, sizeof ps);
-  int ns = 0;
-  const PartBufs* sides[2] = {has_build ? &B.b : &B.p, &B.p};
-  uint32_t* totals[2] = {has_build ? B.flags + 6 : B.flags + 7, B.flags + 7};
-  for (ns = 0; ns < (has_build ? 2 : 1); ns++) {
-    ps.counts[ns] = sides[ns]->counts;
-    ps.tot[ns] = sides[ns]->tot;
-    ps.pstart[ns] = sides[ns]->pstart;
-    ps.total[ns] = totals[ns];
-    ps.G[ns] = sides[ns]->G;
+  // the run and start scans of one or two sides (build, then probe)
+  auto scans = [&](const PartBufs* s0, uint32_t* t0, const PartBufs* s1, uint32_t* t1) {
+    PartScan ps;
+    memset(&ps, 0, sizeof ps);
+    const PartBufs* sd[2] = {s0, s1};
+    uint32_t* tt[2] = {t0, t1};
+    const int ns = s1 ? 2 : 1;
+    for (int i = 0; i < ns; i++) {
+      ps.counts[i] = sd[i]->counts;
+      ps.tot[i] = sd[i]->tot;
+      ps.pstart[i] = sd[i]->pstart;
+      ps.total[i] = tt[i];
+      ps.G[i] = sd[i]->G;
+    }
+    ps.P = P;
+    hipLaunchKernelGGL(k_part_runs, dim3((unsigned)P, (unsigned)ns), dim3(kScanThreads), 0, s, ps);
+    hipLaunchKernelGGL(k_part_starts, dim3((unsigned)ns), dim3(kScanThreads), 0, s, ps);
+  };
+  hipError_t e;
+  if (has_build && B.bloom) {
+    // build side partitioned first, its filter, then the probe side
+    if ((e = jit_part_launch(b, B.b, P, s, true)) != hipSuccess) return e;
+    scans(&B.b, B.flags + 6, nullptr, nullptr);
+    if ((e = jit_part_launch(b, B.b, P, s, false)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bloom_build, dim3((unsigned)P), dim3(256), (size_t)64 << plan.bloom_lpp, s,
+                       B.b.keys, B.b.pstart, B.bloom, plan.bloom_lpp);
+    if ((e = jit_part_launch(p, B.p, P, s, true)) != hipSuccess) return e;
+    scans(&B.p, B.flags + 7, nullptr, nullptr);
+    if ((e = jit_part_launch(p, B.p, P, s, false)) != hipSuccess) return e;
+  } else {
+    e = has_build ? jit_part_launch(b, B.b, P, s, true) : hipSuccess;
+    if (e != hipSuccess) return e;
+    if ((e = jit_part_launch(p, B.p, P, s, true)) != hipSuccess) return e;
+    if (has_build) scans(&B.b, B.flags + 6, &B.p, B.flags + 7);
+    else scans(&B.p, B.flags + 7, nullptr, nullptr);
+    e = has_build ? jit_part_launch(b, B.b, P, s, false) : hipSuccess;
+    if (e != hipSuccess) return e;
+    if ((e = jit_part_launch(p, B.p, P, s, false)) != hipSuccess) return e;
   }
-  ps.P = P;
-  hipLaunchKernelGGL(k_part_runs, dim3((unsigned)P, (unsigned)ns), dim3(kScanThreads), 0, s, ps);
-  hipLaunchKernelGGL(k_part_starts, dim3((unsigned)ns), dim3(kScanThreads), 0, s, ps);
-  e = has_build ? jit_part_launch(b, B.b, P, s, false) : hipSuccess;
-  if (e != hipSuccess) return e;
-  e = jit_part_launch(p, B.p, P, s, false);
-  if (e != hipSuccess) return e;
   if (ev) hipEventRecord(ev[1], s);
   const size_t lds = plan.lds_bytes;
   if (lds > 64 * 1024) join_allow_lds(plan.lds_max);

@@ -1419,6 +1419,16 @@ int join_plan(JoinJob& j) {
   plan.groupby = j.groupby ? 1 : 0;
   plan.topk = j.dev_topk ? j.topk : 0;
   plan.tnodes = j.dev_topk ? topk_tree_nodes(plan.P) : 0;
+  // the semi-join filter: about 16 bits per expected build key (two set per
+  // key), in 64-byte lines, a power of two of them per partition; at most
+  // 2^25 bits (4 MB, so it stays in one XCD's L2) and 64 KB per partition
+  plan.bloom_lpp = -1;
+  if (!j.groupby) {
+    const double want = 16.0 * (double)std::max<int64_t>(j.est, 4096) / 512.0 / plan.P;
+    int lpp = 0;
+    while (lpp < 10 && (double)(1u << lpp) < want && plan.bits + lpp + 9 < 25) lpp++;
+    plan.bloom_lpp = lpp;
+  }
   plan.nvals = j.nvals;
   const int P = plan.P;
   plan.bbuf = part_buffer_entries(P, 8, j.lds_dev);
@@ -1447,8 +1457,9 @@ int join_plan(JoinJob& j) {
   j.cap_p = std::max<int64_t>(j.prows, 1);
   if (j.cap_b >= (int64_t)INT32_MAX || j.cap_p >= (int64_t)UINT32_MAX)
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: join side above 2^31 build / 2^32 probe rows");
+  const bool bloom = plan.bloom_lpp >= 0;
   if ((!j.groupby && jit_part_prepare(j.bs, nullptr, P, plan.bbuf, &j.bl, &jerr)) ||
-      jit_part_prepare(j.ps, &j.jd, P, plan.pbuf, &j.pl, &jerr))
+      jit_part_prepare(j.ps, &j.jd, P, plan.pbuf, &j.pl, &jerr, bloom))
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr.c_str());
   j.B.b.G = j.groupby ? 1 : j.bl.grid;
   j.B.p.G = j.pl.grid;
@@ -1460,6 +1471,7 @@ int join_plan(JoinJob& j) {
   bytes += j.dev_topk ? (size_t)P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8 +
                             (size_t)plan.tnodes * j.topk * 16
                       : (size_t)j.rec_cap * j.RW * 8;
+  bytes += bloom ? (size_t)P * 64 << plan.bloom_lpp : 0;
   bytes += 16 * 256;  // alignment of the carved pieces
   j.bytes = (bytes + 255) & ~(size_t)255;
   j.host_bytes = 64 + (size_t)16 * j.RW * 8;
@@ -1482,6 +1494,11 @@ void join_bind(JoinJob& j, char* dev, char* host) {
     return;
   }
   B.b.perr = B.p.perr = B.flags + 5;
+  if (j.plan.bloom_lpp >= 0) {
+    B.bloom = cv.take<uint32_t>(dev, (size_t)P * 16 << j.plan.bloom_lpp);
+    B.p.bloom = B.bloom;
+    B.p.bloom_lpp = (uint32_t)j.plan.bloom_lpp;
+  }
   B.b.keys = cv.take<int64_t>(dev, j.cap_b);
   B.b.vstride = j.cap_b;
   B.b.pstart = cv.take<uint32_t>(dev, P + 1);

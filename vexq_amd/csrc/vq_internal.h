@@ -185,8 +185,9 @@ struct PartBufs {
   uint32_t* tot;        // [P] rows per partition
   uint32_t* pstart;     // [P + 1]
   uint32_t* perr;       // a write past its partition's end (must stay 0)
+  const uint32_t* bloom; // the join's semi-join filter (null: none)
+  uint32_t bloom_lpp;    // log2 of its 64-byte lines per partition
   int32_t G;
-  int32_t pad_;
 };
 
 // GROUP BY through per-workgroup partials: the distinct keys in one table
@@ -205,6 +206,7 @@ struct JoinBufs {
   uint64_t* cand;       // [P][topk] per-partition top-k candidates (top-k mode)
   struct KItem* citems; // [P][topk] their sort keys (value, first row, index)
   uint64_t* out;        // top-k records
+  uint32_t* bloom;      // semi-join filter of the build keys (null: none)
   uint32_t* tctr;       // top-k merge tree: arrivals per node (zeroed with the flags)
   struct KItem* tree;   // [nodes][topk] each tree node's best candidates
   // flags[0] overflow (1 a table, 2 a pool, 4 more keys than the
@@ -235,6 +237,9 @@ struct JoinPlan {
   int32_t lgrid;        // workgroups that aggregate (partials)
   int32_t kgrid;        // workgroups that collect the keys
   int32_t tnodes;       // top-k merge tree nodes above the partitions
+  int32_t bloom_lpp;    // semi-join filter: log2 of its 64-byte lines per
+                        // partition (-1: no filter)
+  int32_t pad2_;
 };
 
 // LDS bytes of one partition's table and accumulator pool (with `reps`
@@ -281,6 +286,10 @@ struct JitArgs {
   int32_t vword[8];
   int32_t vlsb[8];
   int32_t vw[8];
+  // semi-join filter of the build keys (the probe side's partition kernels)
+  const uint32_t* bloom;
+  uint32_t bloom_lpp;
+  int32_t pad_bloom;
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -295,8 +304,10 @@ struct PartLaunch {
   uint32_t scatter_lds = 0;
   JitArgs args;
 };
+// bloom: this (probe) side keeps only the rows that pass the join's
+// semi-join filter.
 int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
-                     std::string* err);
+                     std::string* err, bool bloom = false);
 // count = true: adds the side's rows per partition and workgroup into counts;
 // otherwise scatters them into their runs.
 hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, hipStream_t s,

@@ -81,7 +81,7 @@ Flat flatten_terms(const QDesc& d) {
 int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
 
 std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slot = 0,
-                     int nv = 0, int probe = 0, int pbuf = 0) {
+                     int nv = 0, int probe = 0, int pbuf = 0, int bloom = 0) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
@@ -110,6 +110,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_NV " + std::to_string(nv) + "\n";
   s += "#define VQ_PROBE " + std::to_string(probe) + "\n";
   s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
+  s += "#define VQ_BLOOM " + std::to_string(bloom) + "\n";
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
@@ -239,12 +240,12 @@ bool jit_disabled() {
 }  // namespace
 
 int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
-                     std::string* err) {
+                     std::string* err, bool bloom) {
   if (jit_disabled()) return fail_jit(err, "JIT disabled");
   int nv = 0;
   const QDesc d = side_desc(sd, jd, &nv);
   const Flat f = flatten_terms(d);
-  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, pbuf), err);
+  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, pbuf, bloom ? 1 : 0), err);
   if (!m) return -1;
   out->count = m->part_count;
   out->scatter = m->part_scatter;
@@ -280,6 +281,8 @@ hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, 
   a.vals = pb.vals;
   a.vstride = pb.vstride;
   a.P = P;
+  a.bloom = pb.bloom;
+  a.bloom_lpp = pb.bloom_lpp;
   const uint32_t lds = count ? pl.count_lds : pl.scatter_lds;
   a.lds_words = lds / 4;
   return launch(count ? pl.count : pl.scatter, (unsigned)pl.grid, lds, s, a, 1024);
