@@ -39,6 +39,9 @@ static constexpr int kFK[8][4] = {{0}, {0}, {0, 2}, {0, 2, 2}, {0}};
 static constexpr int kFS[8][4] = {{2}, {3}, {3, 4}, {3, 4, 5}, {4}};
 static constexpr int kFO[8][4] = {{0}, {0}, {0, 1}, {0, 1, 0}, {0}};
 static constexpr int kFC[8][4] = {{0}, {0}, {0, 2}, {0, 2, 2}, {0}};
+#define VQ_NAG 5
+static constexpr int kLOP[8] = {0, 1, 0, 2, 5, 0, 0, 0};
+static constexpr int kLSRC[8] = {0, 1, 2, 2, 1, 0, 0, 0};
 #include <hip/hip_runtime.h>
 #endif
 // exact fixed-point chunks of a float64 (the host prepends this header's
@@ -112,11 +115,13 @@ struct Args {
   i64* lgkey;         // [groups] the key of each group index
   i32 lcap;           // table slots (a power of two)
   i32 lnw;            // accumulator words per group
-  i32 vop[8];         // per aggregate value: 0 float sum, 1 int sum,
-                      // 2/3 float min/max, 4/5 int min/max
-  i32 vword[8];       // the value's first accumulator word
+  i32 vop[8];         // per aggregate with an argument (kLOP, kLSRC in the
+                      // prologue: its fold and its argument's value index):
+  i32 vword[8];       // its first accumulator word
   i32 vlsb[8];        // float sums: the fixed-point scale
   i32 vw[8];
+  i32 vsrc[8];
+  i32 nvop;
   // semi-join filter of the build side's keys (VQ_BLOOM 1: the probe side
   // drops rows whose two bits are not both set): a blocked Bloom filter,
   // 2^bloom_lpp 64-byte lines per partition
@@ -982,12 +987,16 @@ __device__ __forceinline__ void lload_rows(const Args& a, i64 base, i64 rend,
   }
 }
 
-// One aggregate value of one row into copy c of group g.
+// Aggregate J's value of one row into copy c of group g (fold kLOP[J]: 0
+// float sum, 1 int sum, 2/3 float min/max, 4/5 int min/max, of value
+// kLSRC[J]).
 template <int J>
-__device__ __forceinline__ void local_fold(const Args& a, u64* acc, u32 S, u32 g, u32 c, u64 v) {
-  const int op = a.vop[J];
+__device__ __forceinline__ void local_fold(const Args& a, u64* acc, u32 S, u32 g, u32 c,
+                                           const u64* vals) {
+  constexpr int op = kLOP[J];
+  const u64 v = vals[kLSRC[J]];
   u64* w = acc + ((u64)g * a.lnw + a.vword[J]) * S + c;
-  if (op == 0) {
+  if constexpr (op == 0) {
     vqfx::FixScale fs;
     fs.lsb = a.vlsb[J];
     fs.w = a.vw[J];
@@ -996,14 +1005,14 @@ __device__ __forceinline__ void local_fold(const Args& a, u64* acc, u32 S, u32 g
     if (c0) atomicAdd(w, (u64)c0);
     if (c1) atomicAdd(w + S, (u64)c1);
     if (c2) atomicAdd(w + 2 * S, (u64)c2);
-  } else if (op == 1) {
+  } else if constexpr (op == 1) {
     atomicAdd(w, v);
-  } else if (op == 2 || op == 3) {
+  } else if constexpr (op == 2 || op == 3) {
     const double d = as_f64((i64)v);
     if (d != d) return;  // NaN never wins a compare
-    if (op == 2) atomicMin((long long*)w, (long long)f64_ord(d));
+    if constexpr (op == 2) atomicMin((long long*)w, (long long)f64_ord(d));
     else atomicMax((long long*)w, (long long)f64_ord(d));
-  } else if (op == 4) {
+  } else if constexpr (op == 4) {
     atomicMin((long long*)w, (long long)v);
   } else {
     atomicMax((long long*)w, (long long)v);
@@ -1013,7 +1022,7 @@ __device__ __forceinline__ void local_fold(const Args& a, u64* acc, u32 S, u32 g
 template <int... J>
 __device__ __forceinline__ void local_folds(const Args& a, u64* acc, u32 S, u32 g, u32 c,
                                             const u64* v, Seq<J...>) {
-  (local_fold<J>(a, acc, S, g, c, v[J]), ...);
+  (local_fold<J>(a, acc, S, g, c, v), ...);
 }
 
 }  // namespace vqj
@@ -1058,7 +1067,7 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
   s_scan[tid] = mine;
   if (tid < NW) {
     u8 op = 0;
-    for (int j = 0; j < VQ_NV; j++) {
+    for (int j = 0; j < a.nvop; j++) {
       const int nw = a.vop[j] == 0 ? 3 : 1;
       if (tid >= a.vword[j] && tid < a.vword[j] + nw)
         op = (a.vop[j] == 2 || a.vop[j] == 4) ? 1 : (a.vop[j] == 3 || a.vop[j] == 5) ? 2 : 0;
@@ -1152,7 +1161,7 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
 #if VQ_NV > 0
       u64 v[VQ_NV];
       values_of(a, row, v, seq<VQ_NV>{});
-      local_folds(a, acc, S, g, c, v, seq<VQ_NV>{});
+      local_folds(a, acc, S, g, c, v, seq<VQ_NAG>{});
 #endif
     }
   }

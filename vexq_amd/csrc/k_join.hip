@@ -679,12 +679,11 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     }
     atomicAdd(&cnt[myrep * NP + pi], 1u);
     atomicMin(&first[myrep * NP + pi], row);
-    int vi = 0;
     for (int j = 0; j < jd.naggs; j++) {
       const JAgg& a = jd.agg[j];
-      if (a.e.nfactors == 0) continue;
+      const int vi = a.vidx;
+      if (vi < 0) continue;
       const uint64_t v = vi == 0 ? va : vi == 1 ? vb : B.p.vals[(int64_t)vi * vstride + e];
-      vi++;
       unsigned long long* w = &acc[((size_t)myrep * NP + pi) * NW + a.word0];
       switch (a.kind) {
         case VQ_AGG_SUM:

@@ -1250,11 +1250,17 @@ int join_setup(JoinJob& j) {
     a.kind = j.aggs[k].kind;
     a.accum = j.aggs[k].accum;
     a.word0 = nw;
+    a.vidx = -1;
     if (a.kind == VQ_AGG_COUNT) {
       a.nwords = 0;
       continue;
     }
-    nvals++;
+    // one value per distinct argument expression (partition entries carry
+    // each once)
+    for (int k2 = 0; k2 < k && a.vidx < 0; k2++)
+      if (j.jd.agg[k2].vidx >= 0 && memcmp(&j.jd.agg[k2].e, &a.e, sizeof a.e) == 0)
+        a.vidx = j.jd.agg[k2].vidx;
+    if (a.vidx < 0) a.vidx = nvals++;
     if ((a.kind == VQ_AGG_SUM || a.kind == VQ_AGG_AVG) && j.aggs[k].arg.is_float) {
       Interval iv = expr_interval(j.probe, j.aggs[k].arg);
       if (!iv.ok)
@@ -1571,13 +1577,21 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   const uint32_t nrec = dev_topk ? hflags[3] : ngroups_all;
   if (nrec > (uint64_t)j.rec_cap)
     return fail(VQ_E_DEVICE, "exec: gpu: join produced %u records > %lld", nrec, (long long)j.rec_cap);
-  std::unique_ptr<uint64_t[]> hrec;  // (not zero-filled: the copy writes every word read)
+  // the records, copied into this thread's staging vector (kept between
+  // calls: its pages are already mapped, and it is never zero-filled again)
+  thread_local std::vector<uint64_t> hrec;
   const uint64_t* recs = j.hout;
   if (!dev_topk) {
-    hrec.reset(new (std::nothrow) uint64_t[(size_t)nrec * RW + 1]);
-    if (!hrec) return fail(VQ_E_NOMEM, "exec: gpu: %u result records", nrec);
-    if (nrec) HIP_TRY(hipMemcpy(hrec.get(), j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
-    recs = hrec.get();
+    const size_t words = (size_t)nrec * RW + 1;
+    if (hrec.size() < words) {
+      try {
+        hrec.resize(words);
+      } catch (const std::bad_alloc&) {
+        return fail(VQ_E_NOMEM, "exec: gpu: %u result records", nrec);
+      }
+    }
+    if (nrec) HIP_TRY(hipMemcpy(hrec.data(), j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
+    recs = hrec.data();
   }
   if (stats) {
     memset(stats, 0, sizeof *stats);

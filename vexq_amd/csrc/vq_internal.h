@@ -102,6 +102,9 @@ struct JAgg {
   int32_t nwords;   // 64-bit accumulator words this aggregate holds per slot
   int32_t word0;    // first word index within a slot's accumulator block
   FixScale fs;      // for float sums
+  int32_t vidx;     // its argument's value index (aggregates with the same
+                    // argument share one); -1 for COUNT(*)
+  int32_t pad_;
   QExpr e;
 };
 
@@ -282,10 +285,12 @@ struct JitArgs {
   int64_t* lgkey;
   int32_t lcap;
   int32_t lnw;
-  int32_t vop[8];   // per aggregate value: 0 float sum, 1 int sum, 2/3 float min/max, 4/5 int min/max
+  int32_t vop[8];   // per aggregate with an argument: 0 float sum, 1 int sum, 2/3 float min/max, 4/5 int min/max
   int32_t vword[8];
   int32_t vlsb[8];
   int32_t vw[8];
+  int32_t vsrc[8];  // its argument's value index
+  int32_t nvop;     // aggregates with an argument
   // semi-join filter of the build keys (the probe side's partition kernels)
   const uint32_t* bloom;
   uint32_t bloom_lpp;

@@ -81,7 +81,8 @@ Flat flatten_terms(const QDesc& d) {
 int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
 
 std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slot = 0,
-                     int nv = 0, int probe = 0, int pbuf = 0, int bloom = 0) {
+                     int nv = 0, int probe = 0, int pbuf = 0, int bloom = 0,
+                     const int* lop = nullptr, const int* lsrc = nullptr, int nag = 0) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
@@ -122,6 +123,11 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   arr2(s, "kFS", fs);
   arr2(s, "kFO", fo);
   arr2(s, "kFC", fc);
+  if (kernels & 8) {
+    s += "#define VQ_NAG " + std::to_string(nag) + "\n";
+    arr(s, "kLOP", lop, nag);
+    arr(s, "kLSRC", lsrc, nag);
+  }
   return s;
 }
 
@@ -219,7 +225,7 @@ QDesc side_desc(const JSide& sd, const JDesc* jd, int* nv) {
   *nv = 0;
   if (jd)
     for (int j = 0; j < jd->naggs; j++) {
-      if (jd->agg[j].e.nfactors == 0) continue;
+      if (jd->agg[j].vidx != *nv) continue;  // COUNT(*), or an argument already listed
       d.acc[*nv].op = jd->agg[j].e.is_float ? ACC_SUM_F : ACC_SUM_I;
       d.acc[*nv].e = jd->agg[j].e;
       (*nv)++;
@@ -294,7 +300,18 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
   int nv = 0;
   const QDesc d = side_desc(sd, &jd, &nv);
   const Flat f = flatten_terms(d);
-  Module* m = get_module(prologue(d, f, 8, sd.key_slot, nv, 1, 0), err);
+  // each aggregate with an argument: its fold and its value index
+  int lop[8] = {0}, lsrc[8] = {0}, nag = 0;
+  for (int j = 0; j < jd.naggs && nag < 8; j++) {
+    const JAgg& g = jd.agg[j];
+    if (g.vidx < 0) continue;
+    if (g.kind == VQ_AGG_SUM || g.kind == VQ_AGG_AVG) lop[nag] = g.e.is_float ? 0 : 1;
+    else if (g.kind == VQ_AGG_MIN) lop[nag] = g.accum == VQ_F64 ? 2 : 4;
+    else lop[nag] = g.accum == VQ_F64 ? 3 : 5;
+    lsrc[nag] = g.vidx;
+    nag++;
+  }
+  Module* m = get_module(prologue(d, f, 8, sd.key_slot, nv, 1, 0, 0, lop, lsrc, nag), err);
   if (!m) return -1;
   out->fn = m->hash_local;
   out->lds = kLocalLds;
@@ -317,7 +334,7 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
   int v = 0;
   for (int j = 0; j < jd.naggs; j++) {
     const JAgg& g = jd.agg[j];
-    if (g.e.nfactors == 0) continue;
+    if (g.vidx < 0) continue;
     int op;
     if (g.kind == VQ_AGG_SUM || g.kind == VQ_AGG_AVG) op = g.e.is_float ? 0 : 1;
     else if (g.kind == VQ_AGG_MIN) op = g.accum == VQ_F64 ? 2 : 4;
@@ -326,8 +343,10 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
     a.vword[v] = g.word0;
     a.vlsb[v] = g.fs.lsb;
     a.vw[v] = g.fs.w;
+    a.vsrc[v] = g.vidx;
     v++;
   }
+  a.nvop = v;
   return 0;
 }
 
