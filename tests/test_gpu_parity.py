@@ -286,9 +286,10 @@ def test_join_duplicates_extremes_empty(gpu_ctx):
     (3_000_000, 5, abi.VQ_I64, 0.0),
     (2_000_000, 1, abi.VQ_U16, 0.0),
     (800_000, 70, abi.VQ_I64, 1.1),
-    (1_000_000, 256, abi.VQ_U16, 0.0),
-    # one key more than the partials hold: partitions
-    (1_000_000, 257, abi.VQ_I64, 0.0),
+    (1_000_000, 512, abi.VQ_U16, 0.0),
+    # one key more than the partials hold (512 with these 9 accumulator
+    # words): partitions
+    (1_000_000, 513, abi.VQ_I64, 0.0),
 ])
 def test_agg_hash_matches_oracle(gpu_ctx, nrows, groups, key_type, skew):
     _print_case("agg_hash", nrows=nrows, groups=groups, key_type=key_type, skew=skew)
@@ -308,11 +309,12 @@ def test_agg_hash_matches_oracle(gpu_ctx, nrows, groups, key_type, skew):
 
 
 def test_agg_hash_path_by_key_count(gpu_ctx):
-    """Up to 256 keys aggregate through per-workgroup partials (no
-    partitions); more go through the partitions, on the first query already
-    and directly on later ones.  Every run gives the same bits."""
+    """Up to 512 keys (with these aggregates' 9 accumulator words) aggregate
+    through per-workgroup partials (no partitions); more go through the
+    partitions, on the first query already and directly on later ones.
+    Every run gives the same bits."""
     _print_case("agg_hash_paths")
-    for groups, local in ((200, True), (300, False)):
+    for groups, local in ((400, True), (700, False)):
         t = datagen.keyed_table(400_000, groups, abi.VQ_I64, 0.0, seed=groups)
         pred, key, aggs = datagen.q_keyed(t)
         dt = gpu_ctx.upload(t)

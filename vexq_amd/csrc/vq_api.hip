@@ -1221,11 +1221,16 @@ int32_t table_cap_for(double per, int32_t cap_max) {
   return cap;
 }
 constexpr int32_t kPoolMax = 2048;  // the join kernel's top-k holds 2 entries per thread
-// A GROUP BY of at most kLocalLimit keys runs through per-workgroup partials
-// (one table of kLocalCap slots); one of more, or whose key count is not
-// known yet and turns out larger, through the partitions.
-constexpr int32_t kLocalCap = 512;
-constexpr int32_t kLocalLimit = kLocalCap / 2;
+// A GROUP BY of at most local_limit(nwords) keys runs through per-workgroup
+// partials (one table of twice as many slots): 1024, 512 or 256 keys, as many
+// as one copy of their accumulators leaves room for in the kernel's LDS; one
+// of more, or whose key count is not known yet and turns out larger, through
+// the partitions.
+int32_t local_limit(int32_t nwords) {
+  for (int32_t limit = 1024; limit >= 256; limit /= 2)
+    if (local_lds_for(2 * limit, limit, nwords) <= (size_t)kLocalLds) return limit;
+  return 0;
+}
 
 int join_setup(JoinJob& j) {
   vq_ctx* c = j.c;
@@ -1304,7 +1309,7 @@ int join_setup(JoinJob& j) {
     it = c->build_est.find(j.grp_key);
     if (it != c->build_est.end()) j.groups_seen = it->second;
     if (j.groupby && (c->build_est.count("L|" + j.est_key) ||
-                      (j.groups_seen >= 0 && j.est > kLocalLimit)))
+                      (j.groups_seen >= 0 && j.est > local_limit(nw))))
       j.no_local = true;
   }
   hipDeviceGetAttribute(&j.lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
@@ -1346,10 +1351,10 @@ int join_plan_local(JoinJob& j) {
   plan.local = 1;
   plan.groupby = 1;
   plan.P = 1;
-  plan.lcap = kLocalCap;
-  plan.limit = kLocalLimit;
+  plan.limit = local_limit(j.nw);
+  plan.lcap = 2 * plan.limit;
   plan.lds_max = j.lds_max;
-  if (local_lds_for(plan.lcap, plan.limit, j.nw) > (size_t)kLocalLds) {
+  if (plan.limit == 0) {
     j.no_local = true;
     return kRetry;
   }
