@@ -205,7 +205,7 @@ def _join_tables(nb, npr, seed=42):
     return b, p
 
 
-@pytest.mark.parametrize("bits", [-1, 0, 3, 9])
+@pytest.mark.parametrize("bits", [-1, 0, 3, 9, 11, 13])  # 11, 13: two partitioning passes
 def test_join_full_groups(gpu_ctx, bits):
     _print_case("join_full", bits=bits)
     b, p = _join_tables(150_000, 600_000)
@@ -223,6 +223,25 @@ def test_join_full_groups(gpu_ctx, bits):
     ref = O.join_agg(b, 0, bp, p, 0, pp, aggs, exact=True)
     assert_agg_equal(got, ref, aggs)
     assert same_bits(got, again)
+
+
+def test_join_top10_two_pass_partitioning(gpu_ctx):
+    """4096 partitions (a first pass into 256, then 16 each), top-10 through
+    the merge tree and the semi-join filter: the oracle's top 10."""
+    _print_case("join_top10_two_pass")
+    b, p = _join_tables(400_000, 1_600_000, seed=5)
+    bp, pp, aggs = datagen.q_join(b, p)
+    db, dp = gpu_ctx.upload(b), gpu_ctx.upload(p)
+    try:
+        got = gpu_ctx.join_agg(db, 0, bp, dp, 0, pp, aggs, radix_bits=12, topk=10)
+        auto = gpu_ctx.join_agg(db, 0, bp, dp, 0, pp, aggs, topk=10)
+    finally:
+        db.close()
+        dp.close()
+    assert got.stats.partitions == 4096
+    ref = O.join_agg(b, 0, bp, p, 0, pp, aggs, topk=10, exact=True)
+    assert_agg_equal(got, ref, aggs)
+    assert same_bits(got, auto)
 
 
 def test_join_top10_full_size(gpu_ctx):

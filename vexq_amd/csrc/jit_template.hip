@@ -28,6 +28,7 @@
 #define VQ_KERNELS 15
 #define VQ_PB 8
 #define VQ_BLOOM 1
+#define VQ_LIVEBITS 1
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
@@ -127,7 +128,11 @@ struct Args {
   // 2^bloom_lpp 64-byte lines per partition
   const u32* bloom;
   u32 bloom_lpp;      // log2 of the lines per partition
-  i32 pad_bloom;
+  u32 bloom_pmask;    // the join's partitions - 1 (the filter's blocks; the
+                      // kernels here may write fewer, coarser partitions)
+  // VQ_LIVEBITS: the count pass writes one bit per row (predicate and filter
+  // passed), word row / 64; the scatter reads them instead of testing again
+  u64* livebits;
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -678,10 +683,11 @@ __device__ __forceinline__ void values_of(const Args& a, const i64* row, u64* v,
   ((v[J] = value_of<J>(a, row)), ...);
 }
 
-// The semi-join filter's test of a key with hash h in partition p: one
-// 64-byte line of the partition's block (hash bits 20..), two bits of it
-// (hash bits 40..48 and 49..57); the same bits k_bloom_build sets.
-__device__ __forceinline__ bool bloom_test(const Args& a, u64 h, u32 p) {
+// The semi-join filter's test of a key with hash h: one 64-byte line of its
+// partition's block (hash bits 20..), two bits of it (hash bits 40..48 and
+// 49..57); the same bits k_bloom_build sets.
+__device__ __forceinline__ bool bloom_test(const Args& a, u64 h) {
+  const u32 p = (u32)h & a.bloom_pmask;
   const u32 lines = 1u << a.bloom_lpp;
   const u32 line = p * lines + ((u32)(h >> 20) & (lines - 1));
   const u32 b1 = (u32)(h >> 40) & 511u, b2 = (u32)(h >> 49) & 511u;
@@ -716,7 +722,7 @@ __device__ __forceinline__ void prows_from(const Args& a, i64 base, i64 rend,
     const u64 h = radix_hash((u64)q.key[u]);
     q.part[u] = (u32)(h & mask);
 #if VQ_BLOOM
-    q.live[u] = q.live[u] && bloom_test(a, h, q.part[u]);
+    q.live[u] = q.live[u] && bloom_test(a, h);
 #endif
     q.row[u] = (u32)r;
     if constexpr (kVals && VQ_PROBE && VQ_NV > 0) {
@@ -732,6 +738,48 @@ __device__ __forceinline__ void prows_eval(const Args& a, i64 base, i64 rend, PR
   pload_rows<U>(a, base, rend, raw);
   prows_from<U, kVals>(a, base, rend, raw, q);
 }
+
+#if VQ_LIVEBITS
+// The live-bit words of this wave's U groups of 64 rows (rows base + u*kPB +
+// wave*64 ..; the host keeps row groups 64-aligned for this).
+template <int U>
+__device__ __forceinline__ void lw_load(const Args& a, i64 base, i64 rend, u64 (&lw)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const i64 r0 = base + u * kPB + (threadIdx.x & ~63);
+    lw[u] = r0 < rend ? a.livebits[r0 >> 6] : 0ULL;
+  }
+}
+
+template <int... C>
+__device__ __forceinline__ void load_all(const Args& a, i64 r, i64* row, Seq<C...>) {
+  ((row[C] = ld<kST[C]>(a.col[C], r)), ...);
+}
+
+// The scatter's rows from the count pass's live bits: only live rows load
+// their columns (key and aggregate arguments).
+template <int U>
+__device__ __forceinline__ void prows_live(const Args& a, i64 base, i64 rend, const u64 (&lw)[U],
+                                           PRows<U>& q) {
+  const u64 mask = (u64)a.P - 1;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const i64 r = base + u * kPB + threadIdx.x;
+    q.live[u] = r < rend && ((lw[u] >> lane) & 1ULL);
+    q.row[u] = (u32)r;
+    q.key[u] = 0;
+    q.part[u] = 0;
+    if (q.live[u]) {
+      i64 row[VQ_NS];
+      load_all(a, r, row, seq<VQ_NS>{});
+      q.key[u] = row[VQ_KEYSLOT];
+      q.part[u] = (u32)(radix_hash((u64)q.key[u]) & mask);
+      if constexpr (VQ_NV > 0) values_of(a, row, q.val[u], seq<VQ_NV>{});
+    }
+  }
+}
+#endif
 
 // The chunks (kPB * U rows, clamped to their tile) of the tile range
 // [t, t1): next() steps to the following chunk, false past the last.
@@ -776,6 +824,14 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
     for (i64 base = row0; base < rend; base += kPB * U) {
       PRows<U> q;
       prows_eval<U, false>(a, base, rend, q);
+#if VQ_LIVEBITS
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const i64 r0 = base + u * kPB + (threadIdx.x & ~63);
+        const u64 m = __ballot(q.live[u]);
+        if ((threadIdx.x & 63) == 0 && r0 < rend) a.livebits[r0 >> 6] = m;
+      }
+#endif
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (!q.live[u]) continue;
@@ -883,14 +939,25 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_scatter(const Args a)
   Chunks<U> ck{t0 - 1, t1, 0, 0};
   ck.base = -(i64)kPB * U;  // next() steps onto tile t0
   bool have = t0 < t1 && ck.next(a);
+#if VQ_LIVEBITS
+  u64 lw[U];
+  if (have) lw_load<U>(a, ck.base, ck.rend, lw);
+#else
   i64 raw[VQ_NS][U];
   if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
+#endif
   while (have) {
     const i64 base = ck.base, rend = ck.rend;
     PRows<U> q;
+#if VQ_LIVEBITS
+    prows_live<U>(a, base, rend, lw, q);
+    have = ck.next(a);
+    if (have) lw_load<U>(a, ck.base, ck.rend, lw);
+#else
     prows_from<U, true>(a, base, rend, raw, q);
     have = ck.next(a);
     if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
+#endif
 #if VQ_PB > 0
     // insert what fits; flush full buffers; again until every row is in
     for (;;) {

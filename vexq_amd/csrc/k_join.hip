@@ -129,12 +129,17 @@ __global__ __launch_bounds__(kScanThreads) void k_part_starts(const PartScan a) 
 // zero fill.
 __global__ __launch_bounds__(256) void k_bloom_build(const int64_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ pstart,
-                                                     uint32_t* __restrict__ bloom, int32_t lpp) {
+                                                     uint32_t* __restrict__ bloom, int32_t lpp,
+                                                     int32_t P1, int32_t P2) {
   extern __shared__ uint32_t seg[];
-  const uint32_t p = blockIdx.x, lines = 1u << lpp, words = lines * 16;
+  // block b holds layout partition b: coarse b / P2, fine b % P2, which is
+  // hash partition (b / P2) + (b % P2) * P1 (the filter's block index)
+  const uint32_t b = blockIdx.x;
+  const uint32_t p = b / (uint32_t)P2 + (b % (uint32_t)P2) * (uint32_t)P1;
+  const uint32_t lines = 1u << lpp, words = lines * 16;
   for (uint32_t i = threadIdx.x; i < words; i += 256) seg[i] = 0;
   __syncthreads();
-  for (uint32_t e = pstart[p] + threadIdx.x; e < pstart[p + 1]; e += 256) {
+  for (uint32_t e = pstart[b] + threadIdx.x; e < pstart[b + 1]; e += 256) {
     const uint64_t h = radix_hash((uint64_t)keys[e]);
     const uint32_t line = (uint32_t)(h >> 20) & (lines - 1);
     const uint32_t b1 = (uint32_t)(h >> 40) & 511u, b2 = (uint32_t)(h >> 49) & 511u;
@@ -144,6 +149,49 @@ __global__ __launch_bounds__(256) void k_bloom_build(const int64_t* __restrict__
   __syncthreads();
   uint32_t* out = bloom + (size_t)p * words;
   for (uint32_t i = threadIdx.x; i < words; i += 256) out[i] = seg[i];
+}
+
+// ---- second partitioning pass ------------------------------------------------
+// One block per coarse partition c (hash bits [0, bits1)): its entries split
+// by the next hash bits into P2 fine partitions, written in fine order to the
+// final arrays inside c's range (an LDS histogram, a scan, then LDS
+// cursors); fpstart[c * P2 + f] is fine partition f's start.  A coarse
+// partition's writes stay within its own range, so they combine in L2.
+constexpr int kFineThreads = 1024;
+
+__global__ __launch_bounds__(kFineThreads) void k_part_fine(const PartBufs s, int32_t P1,
+                                                            int32_t bits1, int32_t P2,
+                                                            int32_t nvals, int32_t probe) {
+  __shared__ uint32_t hist[256], cur[256];
+  const uint32_t c = blockIdx.x;
+  const uint32_t cs = s.pstart[c], ce = s.pstart[c + 1];
+  for (int f = threadIdx.x; f < P2; f += kFineThreads) hist[f] = 0;
+  __syncthreads();
+  for (uint32_t e = cs + threadIdx.x; e < ce; e += kFineThreads) {
+    const uint32_t f = (uint32_t)(radix_hash((uint64_t)s.keys[e]) >> bits1) & (uint32_t)(P2 - 1);
+    atomicAdd(&hist[f], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t at = cs;
+    for (int f = 0; f < P2; f++) {
+      cur[f] = at;
+      s.fpstart[(size_t)c * P2 + f] = at;
+      at += hist[f];
+    }
+    if ((int)c == P1 - 1) s.fpstart[(size_t)P1 * P2] = ce;
+  }
+  __syncthreads();
+  for (uint32_t e = cs + threadIdx.x; e < ce; e += kFineThreads) {
+    const int64_t key = s.keys[e];
+    const uint32_t f = (uint32_t)(radix_hash((uint64_t)key) >> bits1) & (uint32_t)(P2 - 1);
+    const uint32_t pos = atomicAdd(&cur[f], 1u);
+    s.fkeys[pos] = key;
+    if (probe) {
+      s.frows[pos] = s.rows[e];
+      for (int v = 0; v < nvals; v++) s.fvals[(size_t)v * s.vstride + pos] = s.vals[(size_t)v * s.vstride + e];
+    }
+  }
 }
 
 // ---- per-partition table ----------------------------------------------------
@@ -518,9 +566,9 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
 
   // build: the partition's build keys; multiplicity = build rows per key
   // (GROUP BY mode has no build side: keys enter the table as rows arrive)
-  const uint32_t b0 = plan.groupby ? 0 : B.b.pstart[p], b1 = plan.groupby ? 0 : B.b.pstart[p + 1];
+  const uint32_t b0 = plan.groupby ? 0 : B.b.fpstart[p], b1 = plan.groupby ? 0 : B.b.fpstart[p + 1];
   for (uint32_t e = b0 + threadIdx.x; e < b1; e += kJoinThreads) {
-    const int64_t key = B.b.keys[e];
+    const int64_t key = B.b.fkeys[e];
     if (key == kEmptyKey) {
       if (atomicAdd(&tag[cap], 1u) == 0) atomicAdd(&s_nkeys, 1u);
       continue;
@@ -549,9 +597,9 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   // rank in slot order (so no two lanes ever race for a new group's entry),
   // and the probe below finds every key claimed.
   if (plan.groupby) {
-    const uint32_t g0 = B.p.pstart[p], g1 = B.p.pstart[p + 1];
+    const uint32_t g0 = B.p.fpstart[p], g1 = B.p.fpstart[p + 1];
     for (uint32_t e = g0 + threadIdx.x; e < g1; e += kJoinThreads) {
-      const int64_t key = B.p.keys[e];
+      const int64_t key = B.p.fkeys[e];
       if (key == kEmptyKey) {
         tag[cap] = 1;
         continue;
@@ -612,7 +660,7 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   if (built && threadIdx.x == 0 && s_nkeys) atomicAdd(&flags[2], s_nkeys);
 
   // probe + aggregate
-  const uint32_t q0 = B.p.pstart[p], q1 = B.p.pstart[p + 1];
+  const uint32_t q0 = B.p.fpstart[p], q1 = B.p.fpstart[p + 1];
   const int64_t vstride = B.p.vstride;
   // one entry per lane and iteration, the next entry's loads issued before
   // this one's update (an empty partition runs no iteration, and loads
@@ -623,10 +671,10 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   uint64_t nva = 0, nvb = 0;
   auto fetch = [&](uint32_t x) {
     const uint32_t ec = x < q1 ? x : q1 - 1;
-    nkey = B.p.keys[ec];
-    nrow = B.p.rows[ec];
-    nva = plan.nvals > 0 ? B.p.vals[ec] : 0;
-    nvb = plan.nvals > 1 ? B.p.vals[vstride + ec] : 0;
+    nkey = B.p.fkeys[ec];
+    nrow = B.p.frows[ec];
+    nva = plan.nvals > 0 ? B.p.fvals[ec] : 0;
+    nvb = plan.nvals > 1 ? B.p.fvals[vstride + ec] : 0;
   };
   if (built && q1 > q0) fetch(e);
   for (; built && e < q1; e += kJoinThreads) {
@@ -683,7 +731,7 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
       const JAgg& a = jd.agg[j];
       const int vi = a.vidx;
       if (vi < 0) continue;
-      const uint64_t v = vi == 0 ? va : vi == 1 ? vb : B.p.vals[(int64_t)vi * vstride + e];
+      const uint64_t v = vi == 0 ? va : vi == 1 ? vb : B.p.fvals[(int64_t)vi * vstride + e];
       unsigned long long* w = &acc[((size_t)myrep * NP + pi) * NW + a.word0];
       switch (a.kind) {
         case VQ_AGG_SUM:
@@ -1160,7 +1208,13 @@ hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
 
 hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd,
                        const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
-  const int P = plan.P;
+  const int P = plan.P1;  // the partition kernels' (coarse) partitions
+  const int P2 = plan.P / plan.P1;
+  auto fine = [&](const PartBufs& side, bool probe) {
+    if (P2 > 1)
+      hipLaunchKernelGGL(k_part_fine, dim3((unsigned)P), dim3(kFineThreads), 0, s, side, P,
+                         plan.bits1, P2, probe ? plan.nvals : 0, probe ? 1 : 0);
+  };
   if (ev) hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, B.flags, (int64_t)16 + plan.tnodes);
   const bool has_build = !plan.groupby;
@@ -1188,11 +1242,13 @@ hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd
     if ((e = jit_part_launch(b, B.b, P, s, true)) != hipSuccess) return e;
     scans(&B.b, B.flags + 6, nullptr, nullptr);
     if ((e = jit_part_launch(b, B.b, P, s, false)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bloom_build, dim3((unsigned)P), dim3(256), (size_t)64 << plan.bloom_lpp, s,
-                       B.b.keys, B.b.pstart, B.bloom, plan.bloom_lpp);
+    fine(B.b, false);
+    hipLaunchKernelGGL(k_bloom_build, dim3((unsigned)plan.P), dim3(256), (size_t)64 << plan.bloom_lpp, s,
+                       B.b.fkeys, B.b.fpstart, B.bloom, plan.bloom_lpp, P, P2);
     if ((e = jit_part_launch(p, B.p, P, s, true)) != hipSuccess) return e;
     scans(&B.p, B.flags + 7, nullptr, nullptr);
     if ((e = jit_part_launch(p, B.p, P, s, false)) != hipSuccess) return e;
+    fine(B.p, true);
   } else {
     e = has_build ? jit_part_launch(b, B.b, P, s, true) : hipSuccess;
     if (e != hipSuccess) return e;
@@ -1202,11 +1258,13 @@ hipError_t launch_join(const PartLaunch& b, const PartLaunch& p, const JDesc& jd
     e = has_build ? jit_part_launch(b, B.b, P, s, false) : hipSuccess;
     if (e != hipSuccess) return e;
     if ((e = jit_part_launch(p, B.p, P, s, false)) != hipSuccess) return e;
+    if (has_build) fine(B.b, false);
+    fine(B.p, true);
   }
   if (ev) hipEventRecord(ev[1], s);
   const size_t lds = plan.lds_bytes;
   if (lds > 64 * 1024) join_allow_lds(plan.lds_max);
-  hipLaunchKernelGGL(k_join_agg, dim3((unsigned)P), dim3(kJoinThreads), lds, s, jd, plan, B);
+  hipLaunchKernelGGL(k_join_agg, dim3((unsigned)plan.P), dim3(kJoinThreads), lds, s, jd, plan, B);
   if (ev) {
     hipEventRecord(ev[2], s);
     hipEventRecord(ev[3], s);

@@ -1195,6 +1195,7 @@ struct JoinJob {
   int bits = 0;
   int32_t pool_req = 0;
   bool no_local = false;   // GROUP BY: too many keys for per-workgroup partials
+  bool livebits = false;   // the probe side's count pass records its live rows
   LocalLaunch ll;
   int attempts = 0;        // plans run (more than one: a table or pool overflowed)
   // per attempt
@@ -1442,8 +1443,14 @@ int join_plan(JoinJob& j) {
   }
   plan.nvals = j.nvals;
   const int P = plan.P;
-  plan.bbuf = part_buffer_entries(P, 8, j.lds_dev);
-  plan.pbuf = part_buffer_entries(P, 8 + 4 + 8 * j.nvals, j.lds_dev);
+  // more than 512 partitions: two passes, the first into 256 coarse ones
+  // (so its LDS buffers stay deep enough to write whole lines)
+  plan.P1 = P > 512 ? 256 : P;
+  plan.bits1 = P > 512 ? 8 : plan.bits;
+  const int P1 = plan.P1;
+  const bool two = P1 < P;
+  plan.bbuf = part_buffer_entries(P1, 8, j.lds_dev);
+  plan.pbuf = part_buffer_entries(P1, 8 + 4 + 8 * j.nvals, j.lds_dev);
   if (j.tune.pbuf >= 0) plan.pbuf = std::min(plan.pbuf, j.tune.pbuf);
 
   std::vector<uint8_t> bkeep, pkeep;
@@ -1469,20 +1476,29 @@ int join_plan(JoinJob& j) {
   if (j.cap_b >= (int64_t)INT32_MAX || j.cap_p >= (int64_t)UINT32_MAX)
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: join side above 2^31 build / 2^32 probe rows");
   const bool bloom = plan.bloom_lpp >= 0;
-  if ((!j.groupby && jit_part_prepare(j.bs, nullptr, P, plan.bbuf, &j.bl, &jerr)) ||
-      jit_part_prepare(j.ps, &j.jd, P, plan.pbuf, &j.pl, &jerr, bloom))
+  // with the filter, the probe side's count pass hands its live rows to the
+  // scatter as bits (row groups starting at multiples of 64 rows)
+  j.livebits = bloom;
+  for (int i = 0; i < j.ps.nspans; i++)
+    if (j.ps.span[i].start % 64) j.livebits = false;
+  if ((!j.groupby && jit_part_prepare(j.bs, nullptr, P1, plan.bbuf, &j.bl, &jerr)) ||
+      jit_part_prepare(j.ps, &j.jd, P1, plan.pbuf, &j.pl, &jerr, bloom, j.livebits))
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: join kernels unavailable: %s", jerr.c_str());
   j.B.b.G = j.groupby ? 1 : j.bl.grid;
   j.B.p.G = j.pl.grid;
   size_t bytes = 0;
-  bytes += (size_t)j.cap_b * 8 + (size_t)(P + 1) * 4 + (size_t)j.B.b.G * P * 4;
-  bytes += (size_t)j.cap_p * 12 + (size_t)std::max(j.nvals, 1) * j.cap_p * 8 + (size_t)(P + 1) * 4 +
-           (size_t)j.B.p.G * P * 4;
+  bytes += (size_t)j.cap_b * 8 + (size_t)(P1 + 1) * 4 + (size_t)j.B.b.G * P1 * 4;
+  bytes += (size_t)j.cap_p * 12 + (size_t)std::max(j.nvals, 1) * j.cap_p * 8 + (size_t)(P1 + 1) * 4 +
+           (size_t)j.B.p.G * P1 * 4;
+  if (two)  // the final layout of both sides
+    bytes += (size_t)j.cap_b * 8 + (size_t)j.cap_p * 12 + (size_t)std::max(j.nvals, 1) * j.cap_p * 8 +
+             (size_t)2 * (P + 1) * 4;
   bytes += (16 + (size_t)plan.tnodes) * 4 + (size_t)2 * P * 4;
   bytes += j.dev_topk ? (size_t)P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8 +
                             (size_t)plan.tnodes * j.topk * 16
                       : (size_t)j.rec_cap * j.RW * 8;
   bytes += bloom ? (size_t)P * 64 << plan.bloom_lpp : 0;
+  bytes += j.livebits ? ((size_t)j.probe->nrows / 64 + 2) * 8 : 0;
   bytes += 16 * 256;  // alignment of the carved pieces
   j.bytes = (bytes + 255) & ~(size_t)255;
   j.host_bytes = 64 + (size_t)16 * j.RW * 8;
@@ -1510,18 +1526,36 @@ void join_bind(JoinJob& j, char* dev, char* host) {
     B.p.bloom = B.bloom;
     B.p.bloom_lpp = (uint32_t)j.plan.bloom_lpp;
   }
+  const int P1 = j.plan.P1;
   B.b.keys = cv.take<int64_t>(dev, j.cap_b);
   B.b.vstride = j.cap_b;
-  B.b.pstart = cv.take<uint32_t>(dev, P + 1);
-  B.b.counts = cv.take<uint32_t>(dev, (size_t)B.b.G * P);
-  B.b.tot = cv.take<uint32_t>(dev, P);
+  B.b.pstart = cv.take<uint32_t>(dev, P1 + 1);
+  B.b.counts = cv.take<uint32_t>(dev, (size_t)B.b.G * P1);
+  B.b.tot = cv.take<uint32_t>(dev, P1);
   B.p.keys = cv.take<int64_t>(dev, j.cap_p);
   B.p.rows = cv.take<uint32_t>(dev, j.cap_p);
   B.p.vals = cv.take<uint64_t>(dev, (size_t)std::max(j.nvals, 1) * j.cap_p);
   B.p.vstride = j.cap_p;
-  B.p.pstart = cv.take<uint32_t>(dev, P + 1);
-  B.p.counts = cv.take<uint32_t>(dev, (size_t)B.p.G * P);
-  B.p.tot = cv.take<uint32_t>(dev, P);
+  B.p.pstart = cv.take<uint32_t>(dev, P1 + 1);
+  B.p.counts = cv.take<uint32_t>(dev, (size_t)B.p.G * P1);
+  B.p.tot = cv.take<uint32_t>(dev, P1);
+  if (P1 < P) {
+    B.b.fkeys = cv.take<int64_t>(dev, j.cap_b);
+    B.b.fpstart = cv.take<uint32_t>(dev, P + 1);
+    B.p.fkeys = cv.take<int64_t>(dev, j.cap_p);
+    B.p.frows = cv.take<uint32_t>(dev, j.cap_p);
+    B.p.fvals = cv.take<uint64_t>(dev, (size_t)std::max(j.nvals, 1) * j.cap_p);
+    B.p.fpstart = cv.take<uint32_t>(dev, P + 1);
+  } else {
+    B.b.fkeys = B.b.keys;
+    B.b.fpstart = B.b.pstart;
+    B.p.fkeys = B.p.keys;
+    B.p.frows = B.p.rows;
+    B.p.fvals = B.p.vals;
+    B.p.fpstart = B.p.pstart;
+  }
+  B.p.bloom_pmask = (uint32_t)(P - 1);
+  if (j.livebits) B.p.livebits = cv.take<uint64_t>(dev, (size_t)j.probe->nrows / 64 + 2);
   if (j.dev_topk) {
     B.cand = cv.take<uint64_t>(dev, (size_t)P * j.topk * j.RW);
     B.citems = (KItem*)cv.take<uint64_t>(dev, (size_t)P * j.topk * 2);

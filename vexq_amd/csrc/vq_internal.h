@@ -179,18 +179,31 @@ hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile
 // side; counts[p][w] holds workgroup w's rows of partition p after the count
 // pass, and after k_part_runs the rows of partition p in workgroups before w
 // (workgroup w's run starts at pstart[p] + counts[p][w]).
+// With many partitions (P > 512) a side is partitioned in two passes: the
+// partition kernels write P1 = 256 coarse partitions (keys .. pstart), and
+// k_part_fine splits each into P / P1 fine ones (fkeys .. fpstart, in
+// coarse-major order: coarse c's fine partition f is layout index
+// c * P2 + f).  With one pass the f* arrays are the same arrays.  The join
+// kernel reads the f* arrays.
 struct PartBufs {
   int64_t* keys;
   uint32_t* rows;       // probe side: row id of each entry
   uint64_t* vals;       // probe side: [nvals][vstride] aggregate argument values
   int64_t vstride;      // entries capacity
-  uint32_t* counts;     // [P][G]
-  uint32_t* tot;        // [P] rows per partition
-  uint32_t* pstart;     // [P + 1]
+  uint32_t* counts;     // [P1][G]
+  uint32_t* tot;        // [P1] rows per partition
+  uint32_t* pstart;     // [P1 + 1]
   uint32_t* perr;       // a write past its partition's end (must stay 0)
   const uint32_t* bloom; // the join's semi-join filter (null: none)
   uint32_t bloom_lpp;    // log2 of its 64-byte lines per partition
+  uint32_t bloom_pmask;  // P - 1 (the filter is kept per final partition)
   int32_t G;
+  int32_t pad_;
+  uint64_t* livebits;    // the count pass's live rows (with the filter; null: none)
+  int64_t* fkeys;       // the final layout
+  uint32_t* frows;
+  uint64_t* fvals;
+  uint32_t* fpstart;    // [P + 1]
 };
 
 // GROUP BY through per-workgroup partials: the distinct keys in one table
@@ -242,7 +255,9 @@ struct JoinPlan {
   int32_t tnodes;       // top-k merge tree nodes above the partitions
   int32_t bloom_lpp;    // semi-join filter: log2 of its 64-byte lines per
                         // partition (-1: no filter)
-  int32_t pad2_;
+  int32_t P1;           // coarse partitions of the partition kernels (P: one pass)
+  int32_t bits1;        // log2(P1)
+  int32_t pad3_;
 };
 
 // LDS bytes of one partition's table and accumulator pool (with `reps`
@@ -294,7 +309,8 @@ struct JitArgs {
   // semi-join filter of the build keys (the probe side's partition kernels)
   const uint32_t* bloom;
   uint32_t bloom_lpp;
-  int32_t pad_bloom;
+  uint32_t bloom_pmask;
+  uint64_t* livebits;  // count pass -> scatter: one bit per live row
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -310,9 +326,10 @@ struct PartLaunch {
   JitArgs args;
 };
 // bloom: this (probe) side keeps only the rows that pass the join's
-// semi-join filter.
+// semi-join filter (and livebits: its count pass records them for the
+// scatter, which then loads only their columns).
 int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
-                     std::string* err, bool bloom = false);
+                     std::string* err, bool bloom = false, bool livebits = false);
 // count = true: adds the side's rows per partition and workgroup into counts;
 // otherwise scatters them into their runs.
 hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, hipStream_t s,

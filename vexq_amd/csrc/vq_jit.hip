@@ -82,7 +82,8 @@ int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
 
 std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slot = 0,
                      int nv = 0, int probe = 0, int pbuf = 0, int bloom = 0,
-                     const int* lop = nullptr, const int* lsrc = nullptr, int nag = 0) {
+                     const int* lop = nullptr, const int* lsrc = nullptr, int nag = 0,
+                     int livebits = 0) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
@@ -112,6 +113,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_PROBE " + std::to_string(probe) + "\n";
   s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
   s += "#define VQ_BLOOM " + std::to_string(bloom) + "\n";
+  s += "#define VQ_LIVEBITS " + std::to_string(livebits) + "\n";
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
@@ -246,12 +248,14 @@ bool jit_disabled() {
 }  // namespace
 
 int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
-                     std::string* err, bool bloom) {
+                     std::string* err, bool bloom, bool livebits) {
   if (jit_disabled()) return fail_jit(err, "JIT disabled");
   int nv = 0;
   const QDesc d = side_desc(sd, jd, &nv);
   const Flat f = flatten_terms(d);
-  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, pbuf, bloom ? 1 : 0), err);
+  Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, pbuf, bloom ? 1 : 0,
+                                  nullptr, nullptr, 0, livebits ? 1 : 0),
+                         err);
   if (!m) return -1;
   out->count = m->part_count;
   out->scatter = m->part_scatter;
@@ -289,6 +293,8 @@ hipError_t jit_part_launch(const PartLaunch& pl, const PartBufs& pb, int32_t P, 
   a.P = P;
   a.bloom = pb.bloom;
   a.bloom_lpp = pb.bloom_lpp;
+  a.bloom_pmask = pb.bloom_pmask;
+  a.livebits = pb.livebits;
   const uint32_t lds = count ? pl.count_lds : pl.scatter_lds;
   a.lds_words = lds / 4;
   return launch(count ? pl.count : pl.scatter, (unsigned)pl.grid, lds, s, a, 1024);
