@@ -428,6 +428,35 @@ __device__ __forceinline__ u64 wave_reduce(int op, u64 v) {
   return v;
 }
 
+// One accumulator's 64 lane columns folded in lane order from column r0.
+template <int OP>
+__device__ __forceinline__ u64 fold_col(const u64* c, int r0) {
+  u64 v = c[r0 & (kWave - 1)];
+  for (int l = 1; l < kWave; l++) v = combine(OP, v, c[(r0 + l) & (kWave - 1)]);
+  return v;
+}
+
+__device__ __forceinline__ u64 fold_columns(int op, const u64* c, int r0) {
+  switch (op) {
+    case 0: return fold_col<0>(c, r0);
+    case 1: return fold_col<1>(c, r0);
+    case 2:
+    case 4: return fold_col<2>(c, r0);
+    default: return fold_col<3>(c, r0);
+  }
+}
+
+// The counts (sum) or first rows (unsigned minimum) of one group's columns.
+template <bool kMin>
+__device__ __forceinline__ u64 fold_counts(const u32* c, int r0) {
+  u32 v = c[r0 & (kWave - 1)];
+  for (int l = 1; l < kWave; l++) {
+    const u32 x = c[(r0 + l) & (kWave - 1)];
+    v = kMin ? (x < v ? x : v) : v + x;
+  }
+  return v;
+}
+
 template <int... J>
 __device__ __forceinline__ void init_regs(u64* r, Seq<J...>) {
   ((r[J] = acc_init<kAOP[J]>()), ...);
@@ -505,9 +534,23 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
       }
     }
   }
-  if constexpr (G > 1) __syncthreads();
   constexpr int NI = NA + 2;
   u64* out = a.part + (u64)blockIdx.x * G * NI;
+  if constexpr (G > 1) {
+    // the partials: lane q folds item q (group q / NI, accumulator, count
+    // or first row q % NI) over the 64 lanes' columns, starting at column q
+    // (so the lanes read different banks); one fixed order per item
+    __syncthreads();
+    for (int q = lane; q < G * NI; q += kWave) {
+      const int gi = q / NI, item = q - gi * NI;
+      u64 v;
+      if (item < NA) v = fold_columns(kAOP[item], &lds[(gi * NA + item) * kWave], q);
+      else if (item == NA) v = fold_counts<false>(&cnt[gi * kWave], q);
+      else v = fold_counts<true>(&first[gi * kWave], q);
+      out[q] = v;
+    }
+    return;
+  }
   for (int gi = 0; gi < G; gi++) {
 #pragma unroll
     for (int item = 0; item < NI; item++) {

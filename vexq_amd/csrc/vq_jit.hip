@@ -144,6 +144,15 @@ Module* get_module(const std::string& pro, std::string* err) {
   auto it = g_cache.find(pro);
   if (it != g_cache.end()) return it->second;
   const std::string src = pro + kFixChunksSrc + kJitTemplateSrc;
+  // VQ_JIT_DUMP=<dir>: each module's prologue (its shape) written there, so
+  // the same kernel can be compiled and inspected offline
+  if (const char* dir = getenv("VQ_JIT_DUMP")) {
+    const std::string path = std::string(dir) + "/module" + std::to_string(g_cache.size()) + ".pro";
+    if (FILE* fp = fopen(path.c_str(), "w")) {
+      fwrite(pro.data(), 1, pro.size(), fp);
+      fclose(fp);
+    }
+  }
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "vq_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
     if (err) *err = "hiprtcCreateProgram failed";
