@@ -417,15 +417,43 @@ __device__ __forceinline__ u64 combine(int op, u64 a, u64 b) {
   }
 }
 
-__device__ __forceinline__ u64 wave_reduce(int op, u64 v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    const u64 o = (u64)__shfl_xor((long long)v, m, kWave);
-    const u64 lo = (threadIdx.x & m) ? o : v;
-    const u64 hi = (threadIdx.x & m) ? v : o;
-    v = combine(op, lo, hi);
+// The value of lane (this lane's source under DPP control CTRL) in rows
+// ROWM; lanes without a source, and rows outside ROWM, take id.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ u64 dpp_u64(u64 x, u64 id) {
+  const u32 lo = (u32)__builtin_amdgcn_update_dpp((int)(u32)id, (int)(u32)x, CTRL, ROWM, 0xf, false);
+  const u32 hi =
+      (u32)__builtin_amdgcn_update_dpp((int)(u32)(id >> 32), (int)(u32)(x >> 32), CTRL, ROWM, 0xf, false);
+  return ((u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ u64 op_identity(int op) {
+  switch (op) {
+    case 0: return 0x8000000000000000ULL;  // -0.0
+    case 2:
+    case 4: return (u64)kI64Max;
+    case 3:
+    case 5: return (u64)kI64Min;
+    case 101: return ~0ULL;
+    default: return 0;
   }
-  return v;
+}
+
+// Fixed-order reduction over the wave: an inclusive scan inside each 16-lane
+// row (row_shr 1, 2, 4, 8), rows carried by row_bcast 15 and 31, so that
+// lane 63 holds the total, which every lane receives.  Call with the whole
+// wave active.
+__device__ __forceinline__ u64 wave_reduce(int op, u64 v) {
+  const u64 id = op_identity(op);
+  v = combine(op, dpp_u64<0x111, 0xf>(v, id), v);
+  v = combine(op, dpp_u64<0x112, 0xf>(v, id), v);
+  v = combine(op, dpp_u64<0x114, 0xf>(v, id), v);
+  v = combine(op, dpp_u64<0x118, 0xf>(v, id), v);
+  v = combine(op, dpp_u64<0x142, 0xa>(v, id), v);
+  v = combine(op, dpp_u64<0x143, 0xc>(v, id), v);
+  const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)v, 63);
+  const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(v >> 32), 63);
+  return ((u64)hi << 32) | lo;
 }
 
 // One accumulator's 64 lane columns folded in lane order from column r0.

@@ -1145,10 +1145,22 @@ __global__ __launch_bounds__(kMergeThreads) void k_local_merge(const JDesc jd, c
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int i = wv; i < NI; i += kMergeThreads / 64) {
     const int op = s_wop[i];
-    unsigned long long v = op == 1 ? (unsigned long long)INT64_MAX
-                           : op == 2 ? (unsigned long long)INT64_MIN
-                           : op == 3 ? 0xFFFFFFFFull : 0ull;
-    for (int w = lane; w < m.grid; w += 64) v = merge_op(op, v, m.part[((size_t)w * nk + g) * NI + i]);
+    const unsigned long long id = op == 1 ? (unsigned long long)INT64_MAX
+                                  : op == 2 ? (unsigned long long)INT64_MIN
+                                  : op == 3 ? 0xFFFFFFFFull : 0ull;
+    unsigned long long v = id;
+    // eight partials per lane in flight at a time (integer adds, minima and
+    // maxima: any order gives the same words)
+    for (int w0 = lane; w0 < m.grid; w0 += 64 * 8) {
+      unsigned long long x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int w = w0 + u * 64;
+        x[u] = w < m.grid ? m.part[((size_t)w * nk + g) * NI + i] : id;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) v = merge_op(op, v, x[u]);
+    }
     for (int off = 32; off > 0; off >>= 1)
       v = merge_op(op, v, (unsigned long long)__shfl_xor((long long)v, off, 64));
     if (lane == 0) s_words[i] = v;

@@ -283,16 +283,44 @@ __device__ __forceinline__ uint64_t combine_item(int op, uint64_t a, uint64_t b)
   }
 }
 
-// Fixed-order butterfly over the wave; every lane ends with the same value.
-__device__ __forceinline__ uint64_t wave_reduce(int op, uint64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    const uint64_t o = (uint64_t)__shfl_xor((long long)v, m, kWave);
-    const uint64_t lo = (threadIdx.x & m) ? o : v;
-    const uint64_t hi = (threadIdx.x & m) ? v : o;
-    v = combine_item(op, lo, hi);
+// The value of lane (this lane's source under DPP control CTRL) in rows
+// ROWM; lanes without a source, and rows outside ROWM, take id.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t x, uint64_t id) {
+  const uint32_t lo =
+      (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)id, (int)(uint32_t)x, CTRL, ROWM, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(id >> 32),
+                                                            (int)(uint32_t)(x >> 32), CTRL, ROWM, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t item_identity(int op) {
+  switch (op) {
+    case ACC_SUM_F: return 0x8000000000000000ULL;  // -0.0
+    case ACC_MIN_F:
+    case ACC_MIN_I: return (uint64_t)INT64_MAX;
+    case ACC_MAX_F:
+    case ACC_MAX_I: return (uint64_t)INT64_MIN;
+    case ITEM_FIRST: return ~0ULL;
+    default: return 0;
   }
-  return v;
+}
+
+// Fixed-order reduction over the wave: an inclusive scan inside each 16-lane
+// row (row_shr 1, 2, 4, 8), rows carried by row_bcast 15 and 31, so that
+// lane 63 holds the total, which every lane receives.  Call with the whole
+// wave active.
+__device__ __forceinline__ uint64_t wave_reduce(int op, uint64_t v) {
+  const uint64_t id = item_identity(op);
+  v = combine_item(op, dpp_u64<0x111, 0xf>(v, id), v);
+  v = combine_item(op, dpp_u64<0x112, 0xf>(v, id), v);
+  v = combine_item(op, dpp_u64<0x114, 0xf>(v, id), v);
+  v = combine_item(op, dpp_u64<0x118, 0xf>(v, id), v);
+  v = combine_item(op, dpp_u64<0x142, 0xa>(v, id), v);
+  v = combine_item(op, dpp_u64<0x143, 0xc>(v, id), v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 struct TileIt {

@@ -67,6 +67,10 @@ struct Slot {
   Arena arena;
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // full-output GROUP BY and join records on their way to the host (grown on
+  // demand; a pageable copy of megabytes costs milliseconds)
+  void* rec_pinned = nullptr;
+  size_t rec_pinned_bytes = 0;
   hipEvent_t ev[6] = {};
   // batches: branch streams forked from `stream` while a graph is captured,
   // their join events, and the instantiated graphs of recent batches
@@ -212,6 +216,7 @@ void slot_destroy(vq_ctx* c, Slot* S) {
     release(c, S->arena.bytes);
   }
   if (S->pinned) hipHostFree(S->pinned);
+  if (S->rec_pinned) hipHostFree(S->rec_pinned);
   for (auto& ev : S->ev)
     if (ev) hipEventDestroy(ev);
   hipStreamDestroy(S->stream);
@@ -1194,6 +1199,7 @@ struct JoinJob {
   bool dev_topk = false;
   int bits = 0;
   int32_t pool_req = 0;
+  Slot* slot = nullptr;    // the leased slot, when the job runs alone (its record staging)
   bool no_local = false;   // GROUP BY: too many keys for per-workgroup partials
   bool livebits = false;   // the probe side's count pass records its live rows
   LocalLaunch ll;
@@ -1620,7 +1626,22 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   // calls: its pages are already mapped, and it is never zero-filled again)
   thread_local std::vector<uint64_t> hrec;
   const uint64_t* recs = j.hout;
-  if (!dev_topk) {
+  if (!dev_topk && j.slot) {
+    // through the slot's pinned staging
+    Slot* S = j.slot;
+    const size_t bytes = (size_t)nrec * RW * 8 + 8;
+    if (S->rec_pinned_bytes < bytes) {
+      if (S->rec_pinned) HIP_TRY(hipHostFree(S->rec_pinned));
+      S->rec_pinned = nullptr;
+      S->rec_pinned_bytes = 0;
+      size_t want = (size_t)1 << 20;
+      while (want < bytes) want *= 2;
+      HIP_TRY(hipHostMalloc(&S->rec_pinned, want, hipHostMallocDefault));
+      S->rec_pinned_bytes = want;
+    }
+    if (nrec) HIP_TRY(hipMemcpy(S->rec_pinned, j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
+    recs = (const uint64_t*)S->rec_pinned;
+  } else if (!dev_topk) {
     const size_t words = (size_t)nrec * RW + 1;
     if (hrec.size() < words) {
       try {
@@ -1650,48 +1671,67 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     stats->grid = j.plan.local ? j.plan.lgrid : j.plan.P;
     stats->tile_rows = j.ps.tile_rows;
   }
-  // rank order for a device top-k; first-occurrence order otherwise (then
-  // the host top-k, for k above the device limit, keeps the stable order)
-  std::vector<std::pair<uint64_t, uint32_t>> ord(nrec);
-  for (uint32_t i = 0; i < nrec; i++) ord[i] = {dev_topk ? (uint64_t)i : recs[(size_t)i * RW + 2], i};
-  if (!dev_topk) {
-    // first rows are distinct 32-bit row ids: a least-significant-digit
-    // radix sort on them (two 16-bit digits) for many groups
-    if (nrec > 4096) {
-      std::vector<std::pair<uint64_t, uint32_t>> tmp(nrec);
-      std::vector<uint32_t> pos(1 << 16);
-      for (int d = 0; d < 2; d++) {
-        const int sh = 16 * d;
-        std::fill(pos.begin(), pos.end(), 0u);
-        for (uint32_t i = 0; i < nrec; i++) pos[(ord[i].first >> sh) & 0xFFFF]++;
-        uint32_t sum = 0;
-        for (auto& x : pos) {
-          const uint32_t c = x;
-          x = sum;
-          sum += c;
-        }
-        for (uint32_t i = 0; i < nrec; i++) tmp[pos[(ord[i].first >> sh) & 0xFFFF]++] = ord[i];
-        ord.swap(tmp);
-      }
-    } else {
-      std::sort(ord.begin(), ord.end());
+  // the records' order, as (first row << 32 | record index): rank order for
+  // a device top-k, first-occurrence order otherwise (then the host top-k,
+  // for k above the device limit, keeps the stable order).  The buffers are
+  // this thread's, kept between calls.
+  thread_local std::vector<uint64_t> ord, tmp;
+  thread_local std::vector<uint32_t> pos;
+  try {
+    if (ord.size() < nrec) ord.resize(nrec);
+  } catch (const std::bad_alloc&) {
+    return fail(VQ_E_NOMEM, "exec: gpu: %u result records", nrec);
+  }
+  uint64_t* o = ord.data();
+  uint64_t fmax = 0;
+  for (uint32_t i = 0; i < nrec; i++) {
+    const uint64_t f = dev_topk ? 0 : (recs[(size_t)i * RW + 2] & 0xFFFFFFFFull);
+    fmax = f > fmax ? f : fmax;
+    o[i] = f << 32 | i;
+  }
+  if (!dev_topk && nrec > 4096) {
+    // first rows are distinct 32-bit row ids: a least-significant-digit radix
+    // sort on their 11-bit digits, as many as the largest one has
+    try {
+      if (tmp.size() < nrec) tmp.resize(nrec);
+      pos.resize(1 << 11);
+    } catch (const std::bad_alloc&) {
+      return fail(VQ_E_NOMEM, "exec: gpu: %u result records", nrec);
     }
+    uint64_t* a = o;
+    uint64_t* b = tmp.data();
+    for (int sh = 32; sh < 64 && (fmax >> (sh - 32)) != 0; sh += 11) {
+      std::fill(pos.begin(), pos.end(), 0u);
+      for (uint32_t i = 0; i < nrec; i++) pos[(a[i] >> sh) & 0x7FF]++;
+      uint32_t sum = 0;
+      for (auto& x : pos) {
+        const uint32_t c = x;
+        x = sum;
+        sum += c;
+      }
+      for (uint32_t i = 0; i < nrec; i++) b[pos[(a[i] >> sh) & 0x7FF]++] = a[i];
+      std::swap(a, b);
+    }
+    o = a;
+  } else if (!dev_topk) {
+    std::sort(o, o + nrec);
   }
+  size_t nord = nrec;
   if (j.topk > 0 && !dev_topk) {
-    auto key = [&](uint32_t i) { return naggs > 0 ? (int64_t)recs[(size_t)i * RW + 3] : (int64_t)0; };
-    std::stable_sort(ord.begin(), ord.end(), [&](const std::pair<uint64_t, uint32_t>& x,
-                                                 const std::pair<uint64_t, uint32_t>& y) {
-      return key(x.second) > key(y.second);
-    });
-    if ((int64_t)ord.size() > j.topk) ord.resize(j.topk);
+    auto key = [&](uint64_t x) {
+      return naggs > 0 ? (int64_t)recs[(size_t)(uint32_t)x * RW + 3] : (int64_t)0;
+    };
+    std::stable_sort(o, o + nrec, [&](uint64_t x, uint64_t y) { return key(x) > key(y); });
+    if ((int64_t)nord > j.topk) nord = (size_t)j.topk;
   }
-  const int64_t nout = (int64_t)ord.size();
+  const int64_t nout = (int64_t)nord;
   out->ngroups = nout;
   if (nout > out->capacity)
     return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)nout,
                 (long long)out->capacity);
   for (int64_t i = 0; i < nout; i++) {
-    const uint64_t* r = &recs[(size_t)ord[i].second * RW];
+    if (i + 8 < nout) __builtin_prefetch(&recs[(size_t)(uint32_t)o[i + 8] * RW]);
+    const uint64_t* r = &recs[(size_t)(uint32_t)o[i] * RW];
     out->keys[i] = (int64_t)r[0];
     out->counts[i] = (int64_t)r[1];
     out->first_row[i] = (int64_t)r[2];
@@ -1747,6 +1787,7 @@ int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
     if (int rc = arena_get(c, S.get(), j.bytes, &ar)) return rc;
     if (int rc = pinned_get(S.get(), j.host_bytes, &hp)) return rc;
     join_bind(j, (char*)ar, (char*)hp);
+    j.slot = S.get();
     HIP_TRY(join_enqueue(j, S->stream, S->ev));
     HIP_TRY(hipStreamSynchronize(S->stream));
     const int rc = join_result(j, out, stats, S->ev);
