@@ -397,6 +397,33 @@ __device__ __noinline__ KItem wave_topk_mem(const KItem* a, uint32_t lo, uint32_
   return best;
 }
 
+// wave_topk_mem over items other workgroups of this kernel wrote (read at
+// the device's coherence point).
+__device__ __noinline__ KItem wave_topk_dev(const KItem* a, uint32_t n, int k) {
+  const int lane = threadIdx.x & 63;
+  KItem best = knone();
+  for (uint32_t b = 0; b < n; b += 64 * kRegItems) {
+    KItem x[kRegItems];
+#pragma unroll
+    for (int j = 0; j < kRegItems; j++) {
+      const uint32_t i = b + j * 64 + lane;
+      x[j] = i < n ? ld_item(&a[i]) : knone();
+    }
+    KItem thr = knone(), mine = knone();
+    for (int r = 0; r < k; r++) {
+      KItem c = kafter(best, thr, r == 0);
+#pragma unroll
+      for (int j = 0; j < kRegItems; j++) c = kmax(c, kafter(x[j], thr, r == 0));
+      const KItem w = wave_best(c);
+      if (w.idx == 0xFFFFFFFFu) break;
+      if (lane == r) mine = w;
+      thr = w;
+    }
+    best = mine;
+  }
+  return best;
+}
+
 __device__ __noinline__ KItem wave_topk_two(KItem x0, KItem x1, int k) {
   const int lane = threadIdx.x & 63;
   KItem thr = knone(), mine = knone();
@@ -460,38 +487,34 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* ctr) {
 
 // Drops from a[0..n) (n <= kJoinThreads, in LDS) every item that cannot make
 // the best k, and absent items; the survivors are compacted to the front of
-// a, and their count is returned.  The bound: when a holds `lists` runs of k
-// items each in rank order, the best of the runs' k-th items; otherwise the
-// k-th best of a sample (the first 64, ranked by wave 0 lane against lane).
-// Either is at or ahead of the whole set's k-th best, so the best k
-// survive.  Call from the whole workgroup.
-__device__ uint32_t prune_topk(KItem* a, uint32_t n, int k, uint32_t* s_cnt, KItem* s_thr,
-                               uint32_t lists) {
-  const bool bound = lists > 0 || n > 64;
-  if (lists > 0) {
-    if (threadIdx.x < 64) {
-      const int l = threadIdx.x & 63;
-      const KItem w = wave_best((uint32_t)l < lists ? a[(size_t)l * k + k - 1] : knone());
-      if (threadIdx.x == 0) *s_thr = w;
-    }
-  } else if (n > 64 && threadIdx.x < 64) {
-    const KItem x = a[threadIdx.x];
-    const bool valid = x.idx != kNone;
-    const uint32_t xlo = (uint32_t)(uint64_t)x.v, xhi = (uint32_t)((uint64_t)x.v >> 32);
-    uint32_t r = 0;
-    for (int l = 0; l < 64; l++) {
-      KItem y;
-      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)xlo, l);
-      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)xhi, l);
-      y.v = (int64_t)(((uint64_t)hi << 32) | lo);
-      y.first = (uint32_t)__builtin_amdgcn_readlane((int)x.first, l);
-      y.idx = (uint32_t)__builtin_amdgcn_readlane((int)x.idx, l);
-      r += (y.idx != kNone && kbetter(y, x)) ? 1u : 0u;
-    }
-    const uint64_t m = __ballot(valid && r == (uint32_t)(k - 1));
-    if (threadIdx.x == 0) *s_thr = knone();
-    if (m && (int)threadIdx.x == __ffsll((unsigned long long)m) - 1) *s_thr = x;
+// a, and their count is returned.  The bound: the k-th best of a sample (the
+// first 64, ranked by wave 0 lane against lane), which is at or ahead of the
+// whole set's k-th best, so the best k survive.  Call from the whole
+// workgroup.
+// The k-th best of the wave's lane items (each lane ranked against every
+// other), written to *out by the lane that holds it; none when fewer than k
+// items are valid.  Call from the whole wave.
+__device__ __forceinline__ void wave_kth(const KItem& x, int k, KItem* out) {
+  const bool valid = x.idx != kNone;
+  const uint32_t xlo = (uint32_t)(uint64_t)x.v, xhi = (uint32_t)((uint64_t)x.v >> 32);
+  uint32_t r = 0;
+  for (int l = 0; l < 64; l++) {
+    KItem y;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)xlo, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)xhi, l);
+    y.v = (int64_t)(((uint64_t)hi << 32) | lo);
+    y.first = (uint32_t)__builtin_amdgcn_readlane((int)x.first, l);
+    y.idx = (uint32_t)__builtin_amdgcn_readlane((int)x.idx, l);
+    r += (y.idx != kNone && kbetter(y, x)) ? 1u : 0u;
   }
+  const uint64_t m = __ballot(valid && r == (uint32_t)(k - 1));
+  if ((threadIdx.x & 63) == 0) *out = knone();
+  if (m && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) *out = x;
+}
+
+__device__ uint32_t prune_topk(KItem* a, uint32_t n, int k, uint32_t* s_cnt, KItem* s_thr) {
+  const bool bound = n > 64;
+  if (n > 64 && threadIdx.x < 64) wave_kth(a[threadIdx.x], k, s_thr);
   const KItem x = threadIdx.x < n ? a[threadIdx.x] : knone();
   if (threadIdx.x == 0) *s_cnt = 0;
   __syncthreads();
@@ -505,7 +528,6 @@ __device__ uint32_t prune_topk(KItem* a, uint32_t n, int k, uint32_t* s_cnt, KIt
 
 constexpr int kPoolPerThread = 2;  // pool <= kPoolPerThread * kJoinThreads
 constexpr int kRankMax = 512;      // top-k by counting ranks up to this many groups
-static_assert(kTopkFan * kMaxK <= kRankMax, "a merge's candidates fit the ranking room");
 
 // Eight waves per SIMD (at most 64 VGPRs): two workgroups per CU when the
 // tables leave room in LDS.
@@ -513,6 +535,7 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
                                                            const JoinBufs B) {
   extern __shared__ unsigned long long smem[];
   __shared__ uint32_t s_fail, s_nrec, s_nkeys, s_base, s_pool, s_last;
+  __shared__ KItem s_bound;
   __shared__ KItem s_cand[kWaves * kMaxK];
   __shared__ uint8_t s_wop[3 * VQ_MAX_AGGS];  // per accumulator word: 0 add, 1 min, 2 max
   const int p = blockIdx.x;
@@ -534,6 +557,25 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   // wave w updates accumulator replica w mod R: with few hot keys the
   // replicas spread what would be one LDS address's atomics over R addresses
   const int myrep = (int)(threadIdx.x >> 6) & (R - 1);
+  // the partition's bounds, this thread's first build key and first probe
+  // entry: loading while the tables are cleared
+  const uint32_t b0 = plan.groupby ? 0 : B.b.fpstart[p], b1 = plan.groupby ? 0 : B.b.fpstart[p + 1];
+  const uint32_t q0 = B.p.fpstart[p], q1 = B.p.fpstart[p + 1];
+  const int64_t vstride = B.p.vstride;
+  int64_t bkey0 = 0;
+  if (b0 + threadIdx.x < b1) bkey0 = B.b.fkeys[b0 + threadIdx.x];
+  uint32_t e = q0 + threadIdx.x;
+  int64_t nkey = 0;
+  uint32_t nrow = 0;
+  uint64_t nva = 0, nvb = 0;
+  auto fetch = [&](uint32_t x) {
+    const uint32_t ec = x < q1 ? x : q1 - 1;
+    nkey = B.p.fkeys[ec];
+    nrow = B.p.frows[ec];
+    nva = plan.nvals > 0 ? B.p.fvals[ec] : 0;
+    nvb = plan.nvals > 1 ? B.p.fvals[vstride + ec] : 0;
+  };
+  if (q1 > q0) fetch(e);
   if (threadIdx.x == 0) {
     s_fail = 0;
     s_nrec = 0;
@@ -566,9 +608,8 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
 
   // build: the partition's build keys; multiplicity = build rows per key
   // (GROUP BY mode has no build side: keys enter the table as rows arrive)
-  const uint32_t b0 = plan.groupby ? 0 : B.b.fpstart[p], b1 = plan.groupby ? 0 : B.b.fpstart[p + 1];
-  for (uint32_t e = b0 + threadIdx.x; e < b1; e += kJoinThreads) {
-    const int64_t key = B.b.fkeys[e];
+  for (uint32_t eb = b0 + threadIdx.x; eb < b1; eb += kJoinThreads) {
+    const int64_t key = eb == b0 + threadIdx.x ? bkey0 : B.b.fkeys[eb];
     if (key == kEmptyKey) {
       if (atomicAdd(&tag[cap], 1u) == 0) atomicAdd(&s_nkeys, 1u);
       continue;
@@ -597,9 +638,8 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   // rank in slot order (so no two lanes ever race for a new group's entry),
   // and the probe below finds every key claimed.
   if (plan.groupby) {
-    const uint32_t g0 = B.p.fpstart[p], g1 = B.p.fpstart[p + 1];
-    for (uint32_t e = g0 + threadIdx.x; e < g1; e += kJoinThreads) {
-      const int64_t key = B.p.fkeys[e];
+    for (uint32_t eg = q0 + threadIdx.x; eg < q1; eg += kJoinThreads) {
+      const int64_t key = eg == q0 + threadIdx.x ? nkey : B.p.fkeys[eg];
       if (key == kEmptyKey) {
         tag[cap] = 1;
         continue;
@@ -659,24 +699,9 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   const bool built = s_fail == 0;
   if (built && threadIdx.x == 0 && s_nkeys) atomicAdd(&flags[2], s_nkeys);
 
-  // probe + aggregate
-  const uint32_t q0 = B.p.fpstart[p], q1 = B.p.fpstart[p + 1];
-  const int64_t vstride = B.p.vstride;
-  // one entry per lane and iteration, the next entry's loads issued before
-  // this one's update (an empty partition runs no iteration, and loads
-  // nothing)
-  uint32_t e = q0 + threadIdx.x;
-  int64_t nkey = 0;
-  uint32_t nrow = 0;
-  uint64_t nva = 0, nvb = 0;
-  auto fetch = [&](uint32_t x) {
-    const uint32_t ec = x < q1 ? x : q1 - 1;
-    nkey = B.p.fkeys[ec];
-    nrow = B.p.frows[ec];
-    nva = plan.nvals > 0 ? B.p.fvals[ec] : 0;
-    nvb = plan.nvals > 1 ? B.p.fvals[vstride + ec] : 0;
-  };
-  if (built && q1 > q0) fetch(e);
+  // probe + aggregate: one entry per lane and iteration, the next entry's
+  // loads issued before this one's update (an empty partition runs no
+  // iteration, and loads nothing)
   for (; built && e < q1; e += kJoinThreads) {
     const int64_t key = nkey;
     const uint32_t row = nrow;
@@ -808,9 +833,8 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     return;
   }
 
-  // top-k: the partition's best k as candidates (records), then a tree of
-  // merges over the partitions, kTopkFan children to a node, where the last
-  // child to finish merges its parent's candidates, up to the final k
+  // top-k: the partition's best k as candidates (records, and their sort
+  // keys in rank order), then the last workgroup to finish merges them
   const int k = plan.topk;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -829,7 +853,7 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
       items[i] = x;
     }
     __syncthreads();
-    const uint32_t n = prune_topk(items, nu, k, &s_nkeys, &s_cand[0], 0);
+    const uint32_t n = prune_topk(items, nu, k, &s_nkeys, &s_cand[0]);
     for (uint32_t a = threadIdx.x; a < n; a += kJoinThreads) {
       const KItem x = items[a];
       const uint32_t r = rank_in(items, n, x);
@@ -845,8 +869,11 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     for (uint32_t r = n + threadIdx.x; r < (uint32_t)k; r += kJoinThreads)
       st_item(&B.citems[(uint32_t)p * k + r], knone());
   } else {
-    // many groups: each wave's best k of its lanes' items, then wave 0's
-    // best k of those
+    // many groups: the k-th best of a sample (the first 64 groups) bounds
+    // the best k from below; the groups at or ahead of it are ranked by
+    // counting, in the ranking room.  When more survive than the room holds:
+    // each wave's best k by rounds of wave maxima, then wave 0's best k of
+    // those.
     KItem it[kPoolPerThread];
 #pragma unroll
     for (int j = 0; j < kPoolPerThread; j++) {
@@ -860,6 +887,34 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
       }
     }
     static_assert(kPoolPerThread == 2, "items per lane");
+    if (wv == 0) wave_kth(it[0], k, &s_bound);
+    if (threadIdx.x == 0) s_nkeys = 0;
+    __syncthreads();
+    const KItem t = s_bound;
+#pragma unroll
+    for (int j = 0; j < kPoolPerThread; j++) {
+      const bool keep = it[j].idx != kNone && (t.idx == kNone || !kbetter(t, it[j]));
+      const uint32_t at = wave_append(keep, &s_nkeys);
+      if (keep && at < (uint32_t)kRankMax) ritems[at] = it[j];
+    }
+    __syncthreads();
+    const uint32_t m = s_nkeys;
+    if (m <= (uint32_t)kRankMax) {
+      for (uint32_t a = threadIdx.x; a < m; a += kJoinThreads) {
+        const KItem x = ritems[a];
+        const uint32_t r = rank_in(ritems, m, x);
+        if (r >= (uint32_t)k) continue;
+        const uint32_t c = (uint32_t)p * k + r;
+        const uint32_t i = x.idx, sl = pslot[i];
+        const int64_t key = (int)sl == cap ? kEmptyKey : (int64_t)keys[sl];
+        emit_record<true>(jd, B.cand + (size_t)c * RW, key, cnt[i], pmult[i], first[i], &acc[(size_t)i * NW]);
+        KItem ci = x;
+        ci.idx = c;
+        st_item(&B.citems[c], ci);
+      }
+      for (uint32_t r = m + threadIdx.x; r < (uint32_t)k; r += kJoinThreads)
+        st_item(&B.citems[(uint32_t)p * k + r], knone());
+    } else {
     KItem w = wave_topk_two(it[0], it[1], k);
     if (lane < k) s_cand[wv * k + lane] = w;
     __syncthreads();
@@ -878,56 +933,75 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
         st_item(&B.citems[c], ci);
       }
     }
-  }
-  // Climb: the last of a node's children to arrive merges the children's
-  // candidates into the node's best k (ranked by counting, at most
-  // kTopkFan * k items).  Candidates keep the index of their partition
-  // record; the root's best k become the output.  Candidates and records go
-  // through the device's coherence point (st_dev / ld_dev): every wave waits
-  // until its stores are acknowledged, then after a barrier one thread
-  // signals.  (A device-scope fence instead writes back the whole L2, which
-  // costs microseconds each time.)
-  uint32_t node = p, nl = gridDim.x, off_up = 0;
-  const KItem* level = B.citems;
-  while (nl > 1) {
-    const uint32_t parent = node / kTopkFan;
-    const uint32_t nup = (nl + kTopkFan - 1) / kTopkFan;
-    const uint32_t nch = nl - parent * kTopkFan < (uint32_t)kTopkFan ? nl - parent * kTopkFan : kTopkFan;
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&B.tctr[off_up + parent], 1u) == nch - 1;
-    __syncthreads();
-    if (!s_last) return;
-    const uint32_t nitems = nch * (uint32_t)k;  // at most kTopkFan * kMaxK
-    KItem* mi = ritems;
-    KItem x = knone();
-    if (threadIdx.x < nitems) x = ld_item(&level[(size_t)parent * kTopkFan * k + threadIdx.x]);
-    if (threadIdx.x < nitems) mi[threadIdx.x] = x;
-    const int nv = __syncthreads_count(threadIdx.x < nitems && x.idx != kNone);
-    const uint32_t ns_ = prune_topk(mi, nitems, k, &s_nkeys, &s_cand[0], nch);
-    KItem* dst = B.tree + (size_t)(off_up + parent) * k;
-    if (threadIdx.x < ns_) {
-      x = mi[threadIdx.x];
-      const uint32_t r = rank_in(mi, ns_, x);
-      if (r < (uint32_t)k) st_item(&dst[r], x);
     }
-    if ((int)threadIdx.x >= nv && (int)threadIdx.x < k) st_item(&dst[threadIdx.x], knone());
-    level = B.tree + (size_t)off_up * k;
-    node = parent;
-    off_up += nup;
-    nl = nup;
   }
-  // the root: its best k, as records (written by this workgroup's waves
-  // when there is a single partition)
+  // The merge: the last workgroup to finish takes every partition's
+  // candidates.  Candidates and records go through the device's coherence
+  // point (st_dev / ld_dev): every wave waits until its stores are
+  // acknowledged, then after a barrier one thread signals.  (A device-scope
+  // fence instead writes back the whole L2, which costs microseconds.)  The
+  // bound: each lane of wave 0 takes the best of its share of the lists'
+  // first items, and the k-th best of those 64 is at or ahead of the k-th
+  // best of all (k distinct items are at or ahead of it); the items at or
+  // ahead of it are ranked by counting.  When more survive than the ranking
+  // room holds, wave 0 selects over all items by rounds of wave maxima.
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  const KItem* root = level + (size_t)node * k;
-  KItem x = knone();
-  if ((int)threadIdx.x < k) x = ld_item(&root[threadIdx.x]);
-  if ((int)threadIdx.x < k && x.idx != kNone)
-    for (int q = 0; q < RW; q++) B.out[(size_t)threadIdx.x * RW + q] = ld_dev(&B.cand[(size_t)x.idx * RW + q]);
-  const int got = __syncthreads_count((int)threadIdx.x < k && x.idx != kNone);
-  if (threadIdx.x == 0) flags[3] = (uint32_t)got;
+  if (gridDim.x > 1) {
+    if (threadIdx.x == 0) s_last = atomicAdd(&B.tctr[0], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+  }
+  const uint32_t P = gridDim.x, n = P * (uint32_t)k;
+  if (threadIdx.x == 0) s_nkeys = 0;
+  // (four loads per lane in flight at a time: with thousands of partitions
+  // the lists hold tens of thousands of items)
+  constexpr int kIn = 4;
+  if (wv == 0) {
+    KItem b = knone();
+    for (uint32_t l0 = lane; l0 < P; l0 += 64 * kIn) {
+      KItem xs[kIn];
+#pragma unroll
+      for (int u = 0; u < kIn; u++) {
+        const uint32_t l = l0 + u * 64;
+        xs[u] = l < P ? ld_item(&B.citems[(size_t)l * k]) : knone();
+      }
+#pragma unroll
+      for (int u = 0; u < kIn; u++) b = kmax(b, xs[u]);
+    }
+    wave_kth(b, k, &s_bound);
+  }
+  __syncthreads();
+  const KItem t = s_bound;
+  for (uint32_t i0 = 0; i0 < n; i0 += kJoinThreads * kIn) {
+    KItem xs[kIn];
+#pragma unroll
+    for (int u = 0; u < kIn; u++) {
+      const uint32_t i = i0 + u * kJoinThreads + threadIdx.x;
+      xs[u] = i < n ? ld_item(&B.citems[i]) : knone();
+    }
+#pragma unroll
+    for (int u = 0; u < kIn; u++) {
+      const bool keep = xs[u].idx != kNone && (t.idx == kNone || !kbetter(t, xs[u]));
+      const uint32_t at = wave_append(keep, &s_nkeys);
+      if (keep && at < (uint32_t)kRankMax) ritems[at] = xs[u];
+    }
+  }
+  __syncthreads();
+  const uint32_t m = s_nkeys;
+  if (m <= (uint32_t)kRankMax) {
+    for (uint32_t a = threadIdx.x; a < m; a += kJoinThreads) {
+      const KItem x = ritems[a];
+      const uint32_t r = rank_in(ritems, m, x);
+      if (r >= (uint32_t)k) continue;
+      for (int q = 0; q < RW; q++) B.out[(size_t)r * RW + q] = ld_dev(&B.cand[(size_t)x.idx * RW + q]);
+    }
+  } else if (wv == 0) {
+    const KItem x = wave_topk_dev(B.citems, n, k);
+    if (lane < k && x.idx != kNone)
+      for (int q = 0; q < RW; q++) B.out[(size_t)lane * RW + q] = ld_dev(&B.cand[(size_t)x.idx * RW + q]);
+  }
+  if (threadIdx.x == 0) flags[3] = m < (uint32_t)k ? m : (uint32_t)k;
 }
 
 size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps, bool topk) {
@@ -936,14 +1010,7 @@ size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps, boo
          (topk ? 16 * (size_t)kRankMax + 16 : 0);
 }
 
-int32_t topk_tree_nodes(int32_t P) {
-  int32_t total = 0;
-  for (int32_t n = P; n > 1;) {
-    n = (n + kTopkFan - 1) / kTopkFan;
-    total += n;
-  }
-  return total;
-}
+int32_t topk_tree_nodes(int32_t P) { return P > 1 ? 1 : 0; }
 
 uint32_t part_scatter_lds(int32_t P, int32_t pbuf, int32_t entry) {
   if (pbuf <= 0) return P <= 16384 ? (uint32_t)P * 4 : 0;

@@ -1500,8 +1500,7 @@ int join_plan(JoinJob& j) {
     bytes += (size_t)j.cap_b * 8 + (size_t)j.cap_p * 12 + (size_t)std::max(j.nvals, 1) * j.cap_p * 8 +
              (size_t)2 * (P + 1) * 4;
   bytes += (16 + (size_t)plan.tnodes) * 4 + (size_t)2 * P * 4;
-  bytes += j.dev_topk ? (size_t)P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8 +
-                            (size_t)plan.tnodes * j.topk * 16
+  bytes += j.dev_topk ? (size_t)P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8
                       : (size_t)j.rec_cap * j.RW * 8;
   bytes += bloom ? (size_t)P * 64 << plan.bloom_lpp : 0;
   bytes += j.livebits ? ((size_t)j.probe->nrows / 64 + 2) * 8 : 0;
@@ -1566,7 +1565,6 @@ void join_bind(JoinJob& j, char* dev, char* host) {
     B.cand = cv.take<uint64_t>(dev, (size_t)P * j.topk * j.RW);
     B.citems = (KItem*)cv.take<uint64_t>(dev, (size_t)P * j.topk * 2);
     B.out = cv.take<uint64_t>(dev, (size_t)16 * j.RW);
-    B.tree = (KItem*)cv.take<uint64_t>(dev, (size_t)std::max(j.plan.tnodes, 1) * j.topk * 2);
   } else {
     B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
   }

@@ -223,13 +223,12 @@ struct JoinBufs {
   struct KItem* citems; // [P][topk] their sort keys (value, first row, index)
   uint64_t* out;        // top-k records
   uint32_t* bloom;      // semi-join filter of the build keys (null: none)
-  uint32_t* tctr;       // top-k merge tree: arrivals per node (zeroed with the flags)
-  struct KItem* tree;   // [nodes][topk] each tree node's best candidates
+  uint32_t* tctr;       // top-k merge: workgroups finished (zeroed with the flags)
   // flags[0] overflow (1 a table, 2 a pool, 4 more keys than the
   // per-workgroup path holds), [1] groups, [2] distinct build keys, [3] top-k
   // count, [4] partitions finished, [5] partition write error, [6] build
   // entries, [7] probe entries, [10] key INT64_MIN seen; 16 words (then
-  // the top-k tree's counters), zeroed by the pipeline's first launch
+  // the top-k merge's counter), zeroed by the pipeline's first launch
   uint32_t* flags;
 };
 
@@ -252,7 +251,7 @@ struct JoinPlan {
   int32_t limit;        // the most keys it holds (lcap / 2)
   int32_t lgrid;        // workgroups that aggregate (partials)
   int32_t kgrid;        // workgroups that collect the keys
-  int32_t tnodes;       // top-k merge tree nodes above the partitions
+  int32_t tnodes;       // top-k merge counters (1 with more than one partition)
   int32_t bloom_lpp;    // semi-join filter: log2 of its 64-byte lines per
                         // partition (-1: no filter)
   int32_t P1;           // coarse partitions of the partition kernels (P: one pass)
@@ -263,9 +262,7 @@ struct JoinPlan {
 // LDS bytes of one partition's table and accumulator pool (with `reps`
 // replicas of the accumulators, and the top-k's ranking room).
 size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps = 1, bool topk = false);
-// Partition candidates of a top-k merge in a tree of kTopkFan children per
-// node; topk_tree_nodes(P) is the number of nodes above the P partitions.
-constexpr int kTopkFan = 32;
+// Counters of the top-k merge over P partitions' candidates.
 int32_t topk_tree_nodes(int32_t P);
 
 // Arguments of every run-time compiled kernel; must match vqj::Args in
