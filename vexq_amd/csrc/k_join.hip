@@ -567,13 +567,14 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   uint32_t e = q0 + threadIdx.x;
   int64_t nkey = 0;
   uint32_t nrow = 0;
-  uint64_t nva = 0, nvb = 0;
+  uint64_t nva = 0, nvb = 0, nvc = 0;
   auto fetch = [&](uint32_t x) {
     const uint32_t ec = x < q1 ? x : q1 - 1;
     nkey = B.p.fkeys[ec];
     nrow = B.p.frows[ec];
     nva = plan.nvals > 0 ? B.p.fvals[ec] : 0;
     nvb = plan.nvals > 1 ? B.p.fvals[vstride + ec] : 0;
+    nvc = plan.nvals > 2 ? B.p.fvals[2 * vstride + ec] : 0;
   };
   if (q1 > q0) fetch(e);
   if (threadIdx.x == 0) {
@@ -705,7 +706,7 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   for (; built && e < q1; e += kJoinThreads) {
     const int64_t key = nkey;
     const uint32_t row = nrow;
-    const uint64_t va = nva, vb = nvb;
+    const uint64_t va = nva, vb = nvb, vc = nvc;
     if (e + kJoinThreads < q1) fetch(e + kJoinThreads);
     int sl = -1;
     if (key == kEmptyKey) {
@@ -756,7 +757,7 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
       const JAgg& a = jd.agg[j];
       const int vi = a.vidx;
       if (vi < 0) continue;
-      const uint64_t v = vi == 0 ? va : vi == 1 ? vb : B.p.fvals[(int64_t)vi * vstride + e];
+      const uint64_t v = vi == 0 ? va : vi == 1 ? vb : vi == 2 ? vc : B.p.fvals[(int64_t)vi * vstride + e];
       unsigned long long* w = &acc[((size_t)myrep * NP + pi) * NW + a.word0];
       switch (a.kind) {
         case VQ_AGG_SUM:
