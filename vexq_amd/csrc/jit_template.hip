@@ -889,12 +889,21 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
     else a.counts[p * G + blockIdx.x] = 0;
   }
   __syncthreads();
-  for (i64 t = t0; t < t1; t++) {
-    i64 row0, rend;
-    ptile(a, t, row0, rend);
-    for (i64 base = row0; base < rend; base += kPB * U) {
+  // the next chunk's column loads issued before this chunk's rows are
+  // evaluated (and their filter lines read), so the two wait together
+  Chunks<U> ck{t0 - 1, t1, 0, 0};
+  ck.base = -(i64)kPB * U;  // next() steps onto tile t0
+  bool have = t0 < t1 && ck.next(a);
+  i64 raw[VQ_NS][U];
+  if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
+  while (have) {
+    const i64 base = ck.base, rend = ck.rend;
+    have = ck.next(a);
+    i64 nraw[VQ_NS][U];
+    if (have) pload_rows<U>(a, ck.base, ck.rend, nraw);
+    {
       PRows<U> q;
-      prows_eval<U, false>(a, base, rend, q);
+      prows_from<U, false>(a, base, rend, raw, q);
 #if VQ_LIVEBITS
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -909,6 +918,12 @@ extern "C" __global__ __launch_bounds__(1024) void vq_part_count(const Args a) {
         if (lds) atomicAdd(&dyn[q.part[u]], 1u);
         else atomicAdd(&a.counts[q.part[u] * G + blockIdx.x], 1u);
       }
+    }
+    if (have) {
+#pragma unroll
+      for (int c = 0; c < VQ_NS; c++)
+#pragma unroll
+        for (int u = 0; u < U; u++) raw[c][u] = nraw[c][u];
     }
   }
   __syncthreads();

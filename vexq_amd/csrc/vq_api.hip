@@ -694,13 +694,27 @@ int32_t scan_tile_rows(int64_t rows) {
   return (int32_t)t;
 }
 
-// Tiles of a join side: at least 4096 rows (one pass of a partitioning
-// workgroup), and about 2048 of them, so a persistent grid of one or two
-// workgroups per CU takes near-equal shares.
-int32_t join_tile_rows(int64_t rows) {
-  int64_t t = 4096;
-  while (t < (1 << 20) && t * 2048 < rows) t *= 2;
-  return (int32_t)t;
+// Tiles of a join side: at least 4096 rows, and about 2048 of them, so a
+// persistent grid of one or two workgroups per CU takes near-equal shares.
+// Given the partition kernels' chunk (rows one workgroup loads per step, a
+// step never crossing a tile), the tile that needs the fewest steps of the
+// busiest of `grid` workgroups (a smaller tile on a tie).
+int32_t join_tile_rows(int64_t rows, int64_t chunk = 0, int64_t grid = 256) {
+  if (chunk <= 0) {
+    int64_t t = 4096;
+    while (t < (1 << 20) && t * 2048 < rows) t *= 2;
+    return (int32_t)t;
+  }
+  int64_t best = 4096, best_cost = INT64_MAX;
+  for (int64_t t = 1024; t <= (1 << 20); t *= 2) {
+    const int64_t tiles = (rows + t - 1) / t;
+    const int64_t cost = ((tiles + grid - 1) / grid) * ((t + chunk - 1) / chunk);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = t;
+    }
+  }
+  return (int32_t)best;
 }
 
 int64_t kept_rows(const vq_table* t, const std::vector<uint8_t>& keep) {
@@ -1464,13 +1478,13 @@ int join_plan(JoinJob& j) {
   const int ts = std::max(0, std::min(4, j.tune.tile_shift));
   if (!j.groupby) {
     prune(j.build, j.bp, bkeep);
-    j.bs.tile_rows = join_tile_rows(kept_rows(j.build, bkeep)) << ts;
+    j.bs.tile_rows = join_tile_rows(kept_rows(j.build, bkeep), part_chunk_rows(j.bs.nslots), device_cus()) << ts;
     if (int rc = make_spans(j.build, bkeep, j.bs.tile_rows, j.bs.span, &j.bs.nspans, &j.bs.ntiles,
                            &j.brows, &j.bkept))
       return rc;
   }
   prune(j.probe, j.pp, pkeep);
-  j.ps.tile_rows = join_tile_rows(kept_rows(j.probe, pkeep)) << ts;
+  j.ps.tile_rows = join_tile_rows(kept_rows(j.probe, pkeep), part_chunk_rows(j.ps.nslots), device_cus()) << ts;
   if (int rc = make_spans(j.probe, pkeep, j.ps.tile_rows, j.ps.span, &j.ps.nspans, &j.ps.ntiles,
                          &j.prows, &j.pkept))
     return rc;

@@ -158,6 +158,10 @@ __host__ __device__ inline int64_t f64_to_bits(double d) {
 namespace vq {
 int32_t dense_grid(const QDesc& d);
 size_t dense_lds_bytes(const QDesc& d);
+// Rows one partitioning workgroup (1024 threads) loads per step, for a side
+// of `nslots` columns; the device's compute units.
+int64_t part_chunk_rows(int nslots);
+int device_cus();
 hipError_t launch_agg_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s);
 hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part, uint64_t* out,
                             hipStream_t s);

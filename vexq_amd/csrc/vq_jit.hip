@@ -255,6 +255,18 @@ bool jit_disabled() {
 }
 
 }  // namespace
+int64_t part_chunk_rows(int nslots) { return (int64_t)1024 * rows_in_flight(nslots); }
+
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
 
 int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, PartLaunch* out,
                      std::string* err, bool bloom, bool livebits) {
