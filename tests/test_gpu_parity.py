@@ -328,12 +328,12 @@ def test_agg_hash_matches_oracle(gpu_ctx, nrows, groups, key_type, skew):
 
 
 def test_agg_hash_path_by_key_count(gpu_ctx):
-    """Up to 512 keys (with these aggregates' 9 accumulator words) aggregate
-    through per-workgroup partials (no partitions); more go through the
-    partitions, on the first query already and directly on later ones.
-    Every run gives the same bits."""
+    """Up to 4 x 512 keys (with these aggregates' 9 accumulator words)
+    aggregate through per-workgroup partials (no partitions), in passes of
+    512 keys; more go through the partitions, on the first query already and
+    directly on later ones.  Every run gives the same bits."""
     _print_case("agg_hash_paths")
-    for groups, local in ((400, True), (700, False)):
+    for groups, local in ((400, True), (700, True), (2000, True), (2100, False)):
         t = datagen.keyed_table(400_000, groups, abi.VQ_I64, 0.0, seed=groups)
         pred, key, aggs = datagen.q_keyed(t)
         dt = gpu_ctx.upload(t)
@@ -345,6 +345,28 @@ def test_agg_hash_path_by_key_count(gpu_ctx):
             assert (r.stats.partitions == 0) == local, (groups, r.stats.partitions)
             assert same_bits(r, runs[0])
         assert_agg_equal(runs[0], O.agg_hash(t, pred, key, aggs, exact=True), aggs)
+
+
+def test_agg_hash_passes_skewed_with_min_key(gpu_ctx):
+    """1,500 Zipf-skewed keys (three passes of 512) plus key INT64_MIN, which
+    takes the last group index and so lands in the last pass; against the
+    oracle, and the same bits on a rerun."""
+    _print_case("agg_hash_passes_skewed")
+    t = datagen.keyed_table(600_000, 1500, abi.VQ_I64, 1.1, seed=11)
+    key = t.columns[t.index("key")].data
+    key[::97] = np.iinfo(np.int64).min
+    pred, kc, aggs = datagen.q_keyed(t)
+    dt = gpu_ctx.upload(t)
+    try:
+        runs = [gpu_ctx.agg_hash(dt, pred, kc, aggs) for _ in range(3)]
+    finally:
+        dt.close()
+    for r in runs:
+        assert r.stats.partitions == 0
+        assert same_bits(r, runs[0])
+    ref = O.agg_hash(t, pred, kc, aggs, exact=True)
+    assert np.iinfo(np.int64).min in set(ref.keys.ravel().tolist())
+    assert_agg_equal(runs[0], ref, aggs)
 
 
 def test_agg_hash_extreme_keys_and_empty(gpu_ctx):

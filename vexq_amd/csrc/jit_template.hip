@@ -133,6 +133,8 @@ struct Args {
   // VQ_LIVEBITS: the count pass writes one bit per row (predicate and filter
   // passed), word row / 64; the scatter reads them instead of testing again
   u64* livebits;
+  int lk0;
+  int lpad_;
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -1182,15 +1184,23 @@ __device__ __forceinline__ void local_folds(const Args& a, u64* acc, u32 S, u32 
 
 extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
   extern __shared__ u64 lds[];
-  __shared__ u32 s_scan[kLT];
   __shared__ u8 s_wop[32];  // per accumulator word: 0 add, 1 min, 2 max
   constexpr int U = VQ_U;
   const int tid = threadIdx.x;
   if (a.lflags[0] & kLocalOverflow) return;  // more keys than this path holds
+  // this pass's keys: ranks [k0, k1) of the key list (the regular keys in
+  // slot order of the key table, then INT64_MIN when a row has it)
   const int cap = a.lcap;
   const int lg = 31 - __clz(cap);
-  const u32 nk = a.lflags[2];
   const bool has_min = a.lflags[10] != 0;
+  const u32 total = a.lflags[2];
+  const u32 nreg = total - (has_min ? 1u : 0u);
+  const u32 k0 = (u32)a.lk0;
+  if (k0 >= total) return;  // (a pass beyond the keys)
+  const u32 k1 = total - k0 < (u32)(cap / 2) ? total : k0 + (u32)(cap / 2);
+  const u32 nk = k1 - k0;
+  const bool whole = k0 == 0 && k1 == total;  // every key in this pass
+  const int min_g = has_min && nreg >= k0 && nreg < k1 ? (int)(nreg - k0) : -1;
   const int NW = a.lnw;
   i64* tk = (i64*)lds;
   u32* tix = (u32*)(tk + cap);
@@ -1206,18 +1216,7 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
   u32* cnt = (u32*)(acc + (u64)nk * NW * S);
   u32* first = cnt + (u64)nk * S;
 
-  // the key table and each occupied slot's index (its rank in slot order;
-  // key INT64_MIN, kept apart, takes the last index)
-  const int per = (cap + kLT - 1) / kLT;
-  const int s0 = tid * per < cap ? tid * per : cap;
-  const int s1 = s0 + per < cap ? s0 + per : cap;
-  u32 mine = 0;
-  for (int i = s0; i < s1; i++) {
-    const i64 k = a.lkeys[i];
-    tk[i] = k;
-    mine += k != kI64Min;
-  }
-  s_scan[tid] = mine;
+  for (int i = tid; i < cap; i += kLT) tk[i] = kI64Min;
   if (tid < NW) {
     u8 op = 0;
     for (int j = 0; j < a.nvop; j++) {
@@ -1228,23 +1227,23 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
     s_wop[tid] = op;
   }
   __syncthreads();
-  for (int off = 1; off < kLT; off <<= 1) {
-    const u32 x = tid >= off ? s_scan[tid - off] : 0;
-    __syncthreads();
-    s_scan[tid] += x;
-    __syncthreads();
-  }
-  u32 r = s_scan[tid] - mine;
-  const u32 nreg = s_scan[kLT - 1];
-  for (int i = s0; i < s1; i++) {
-    if (tk[i] == kI64Min) {
-      tix[i] = 0xFFFFFFFFu;
-      continue;
+  // the pass's regular keys into the LDS table, each with its group index
+  const u32 kr1 = k1 < nreg ? k1 : nreg;
+  // (slots from the hash's bits 20 and up, not its top bits: the key list is
+  // in the key table's slot order, by top bits, so a pass's keys share them)
+  for (u32 r = k0 + tid; r < kr1; r += kLT) {
+    const i64 key = a.lgkey[r];
+    u32 i = (u32)(radix_hash((u64)key) >> 20) & (u32)(cap - 1);
+    for (int pr = 0; pr < cap; pr++) {
+      const u64 old = atomicCAS((unsigned long long*)&tk[i], (unsigned long long)kI64Min,
+                                (unsigned long long)key);
+      if (old == (u64)kI64Min) {
+        tix[i] = r - k0;
+        break;
+      }
+      i = (i + 1) & (u32)(cap - 1);
     }
-    if (blockIdx.x == 0) a.lgkey[r] = tk[i];
-    tix[i] = r++;
   }
-  if (blockIdx.x == 0 && tid == 0 && has_min) a.lgkey[nreg] = kI64Min;
   for (u64 x = tid; x < (u64)nk * NW * S; x += kLT) {
     const int op = s_wop[(x / S) % NW];
     acc[x] = op == 1 ? (u64)kI64Max : op == 2 ? (u64)kI64Min : 0;
@@ -1292,9 +1291,9 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
       const i64 key = row[VQ_KEYSLOT];
       u32 g = 0xFFFFFFFFu;
       if (key == kI64Min) {
-        if (has_min) g = nreg;
+        if (min_g >= 0) g = (u32)min_g;
       } else {
-        u32 i = (u32)(radix_hash((u64)key) >> (64 - lg));
+        u32 i = (u32)(radix_hash((u64)key) >> 20) & (u32)(cap - 1);
         for (int pr = 0; pr < cap; pr++) {
           const i64 k = tk[i];
           if (k == key) {
@@ -1305,8 +1304,8 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
           i = (i + 1) & (u32)(cap - 1);
         }
       }
-      if (g >= nk) {  // not collected: the key pass and this pass disagree
-        atomicOr(&a.lflags[0], kLocalOverflow);
+      if (g >= nk) {  // another pass's key, or (one pass) one the key pass missed
+        if (whole) atomicOr(&a.lflags[0], kLocalOverflow);
         continue;
       }
       atomicAdd(&cnt[(u64)g * S + c], 1u);

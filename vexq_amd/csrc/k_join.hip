@@ -1168,12 +1168,41 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_keys(const LocalKeys k)
     atomicOr(&k.flags[0], kLocalOverflow);
 }
 
+// The key list: the table's regular keys in slot order (a key's rank is its
+// group index), then INT64_MIN when a row has it.  One workgroup.
+constexpr int kRankThreads = 1024;
+__global__ __launch_bounds__(kRankThreads) void k_local_rank(const int64_t* __restrict__ keys,
+                                                             int32_t gcap, uint32_t* __restrict__ flags,
+                                                             int64_t* __restrict__ gkey) {
+  __shared__ uint32_t s_scan[kRankThreads];
+  if (flags[0] & kLocalOverflow) return;
+  const int per = (gcap + kRankThreads - 1) / kRankThreads;
+  const int s0 = (int)threadIdx.x * per < gcap ? (int)threadIdx.x * per : gcap;
+  const int s1 = s0 + per < gcap ? s0 + per : gcap;
+  uint32_t mine = 0;
+  for (int i = s0; i < s1; i++) mine += keys[i] != kEmptyKey ? 1u : 0u;
+  s_scan[threadIdx.x] = mine;
+  __syncthreads();
+  for (int off = 1; off < kRankThreads; off <<= 1) {
+    const uint32_t x = (int)threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    s_scan[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t r = s_scan[threadIdx.x] - mine;
+  for (int i = s0; i < s1; i++)
+    if (keys[i] != kEmptyKey) gkey[r++] = keys[i];
+  if (threadIdx.x == kRankThreads - 1 && flags[10]) gkey[s_scan[kRankThreads - 1]] = kEmptyKey;
+}
+
 struct LocalMerge {
   const uint64_t* part;
-  const int64_t* gkey;
+  const int64_t* gkey;  // the key list
   uint32_t* flags;
   uint64_t* recs;
-  int32_t grid;  // workgroups that wrote partials
+  int32_t grid;   // workgroups that wrote partials
+  int32_t k0;     // this pass's first key
+  int32_t limit;  // keys a pass holds
   int32_t pad_;
 };
 
@@ -1196,7 +1225,9 @@ __global__ __launch_bounds__(kMergeThreads) void k_local_merge(const JDesc jd, c
   __shared__ uint8_t s_wop[3 * VQ_MAX_AGGS + 2];
   const uint32_t g = blockIdx.x;
   if (m.flags[0] & kLocalOverflow) return;
-  const uint32_t nk = m.flags[2];
+  const uint32_t total = m.flags[2], k0 = (uint32_t)m.k0;
+  if (k0 >= total) return;
+  const uint32_t nk = total - k0 < (uint32_t)m.limit ? total - k0 : (uint32_t)m.limit;
   if (g >= nk) return;
   const int NW = jd.nwords, NI = NW + 2;
   if ((int)threadIdx.x < NI) {
@@ -1238,7 +1269,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_local_merge(const JDesc jd, c
     const unsigned long long cnt = s_words[NW];
     if (cnt == 0) return;  // a key whose rows all failed the predicate
     const uint32_t at = atomicAdd(&m.flags[1], 1u);
-    emit_record(jd, m.recs + (size_t)at * (3 + jd.naggs), m.gkey[g], cnt, 1,
+    emit_record(jd, m.recs + (size_t)at * (3 + jd.naggs), m.gkey[k0 + g], cnt, 1,
                 (uint32_t)s_words[NW + 1], s_words);
   }
 }
@@ -1250,14 +1281,14 @@ size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords) {
 hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
                         const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
   if (ev) hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_local_init, dim3((unsigned)std::max(1, plan.lcap / 256)), dim3(256), 0, s,
-                     B.l.keys, plan.lcap, B.flags);
+  hipLaunchKernelGGL(k_local_init, dim3((unsigned)std::max(1, plan.gcap / 256)), dim3(256), 0, s,
+                     B.l.keys, plan.gcap, B.flags);
   LocalKeys k;
   memset(&k, 0, sizeof k);
   k.col = ps.slot[ps.key_slot].p;
   k.type = ps.slot[ps.key_slot].type;
-  k.lcap = plan.lcap;
-  k.limit = plan.limit;
+  k.lcap = plan.gcap;
+  k.limit = plan.limit * plan.passes;
   k.nspans = ps.nspans;
   k.tile_rows = ps.tile_rows;
   k.ntiles = ps.ntiles;
@@ -1269,19 +1300,26 @@ hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
   k.keys = B.l.keys;
   k.flags = B.flags;
   hipLaunchKernelGGL(k_local_keys, dim3((unsigned)plan.kgrid), dim3(kLocalThreads),
-                     (size_t)plan.lcap * 8, s, k);
+                     (size_t)plan.gcap * 8, s, k);
+  hipLaunchKernelGGL(k_local_rank, dim3(1), dim3(kRankThreads), 0, s, (const int64_t*)B.l.keys,
+                     plan.gcap, B.flags, B.l.gkey);
   if (ev) hipEventRecord(ev[1], s);
-  hipError_t e = jit_local_launch(ll, B, s);
-  if (e != hipSuccess) return e;
-  if (ev) hipEventRecord(ev[2], s);
+  // one pass over the rows per `limit` keys: partials, then their merge
   LocalMerge m;
   m.part = B.l.part;
   m.gkey = B.l.gkey;
   m.flags = B.flags;
   m.recs = B.recs;
   m.grid = ll.grid;
+  m.limit = plan.limit;
   m.pad_ = 0;
-  hipLaunchKernelGGL(k_local_merge, dim3((unsigned)plan.limit), dim3(kMergeThreads), 0, s, jd, m);
+  for (int c = 0; c < plan.passes; c++) {
+    hipError_t e = jit_local_launch(ll, B, s, c * plan.limit);
+    if (e != hipSuccess) return e;
+    if (ev && c == plan.passes - 1) hipEventRecord(ev[2], s);
+    m.k0 = c * plan.limit;
+    hipLaunchKernelGGL(k_local_merge, dim3((unsigned)plan.limit), dim3(kMergeThreads), 0, s, jd, m);
+  }
   if (ev) hipEventRecord(ev[3], s);
   return hipGetLastError();
 }

@@ -1242,15 +1242,28 @@ int32_t table_cap_for(double per, int32_t cap_max) {
   return cap;
 }
 constexpr int32_t kPoolMax = 2048;  // the join kernel's top-k holds 2 entries per thread
-// A GROUP BY of at most local_limit(nwords) keys runs through per-workgroup
-// partials (one table of twice as many slots): 1024, 512 or 256 keys, as many
-// as one copy of their accumulators leaves room for in the kernel's LDS; one
-// of more, or whose key count is not known yet and turns out larger, through
-// the partitions.
+// A GROUP BY of few keys runs through per-workgroup partials, in passes over
+// the rows of local_limit(nwords) keys each: 1024, 512 or 256 keys, as many
+// as one copy of their accumulators leaves room for in the kernel's LDS.  Up
+// to kLocalPasses passes (a pass reads every row, so more keys go through the
+// partitions, which read them about three times); a GROUP BY of more keys,
+// or whose key count is not known yet and turns out larger, through the
+// partitions.
+// The key table (every pass's keys at load <= 0.5) is at most kLocalSlots
+// slots: the key-collecting kernel holds a copy in LDS.
+constexpr int32_t kLocalPasses = 4;
+constexpr int32_t kLocalSlots = 4096;
 int32_t local_limit(int32_t nwords) {
   for (int32_t limit = 1024; limit >= 256; limit /= 2)
     if (local_lds_for(2 * limit, limit, nwords) <= (size_t)kLocalLds) return limit;
   return 0;
+}
+int32_t local_passes_max(int32_t limit) {
+  return limit > 0 ? std::max(1, std::min(kLocalPasses, kLocalSlots / (2 * limit))) : 0;
+}
+int64_t local_max_keys(int32_t nwords) {
+  const int32_t limit = local_limit(nwords);
+  return (int64_t)limit * local_passes_max(limit);
 }
 
 int join_setup(JoinJob& j) {
@@ -1330,7 +1343,7 @@ int join_setup(JoinJob& j) {
     it = c->build_est.find(j.grp_key);
     if (it != c->build_est.end()) j.groups_seen = it->second;
     if (j.groupby && (c->build_est.count("L|" + j.est_key) ||
-                      (j.groups_seen >= 0 && j.est > local_limit(nw))))
+                      (j.groups_seen >= 0 && j.est > local_max_keys(nw))))
       j.no_local = true;
   }
   hipDeviceGetAttribute(&j.lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
@@ -1374,6 +1387,14 @@ int join_plan_local(JoinJob& j) {
   plan.P = 1;
   plan.limit = local_limit(j.nw);
   plan.lcap = 2 * plan.limit;
+  // passes: as many as the keys need (all of them when the key count is not
+  // known yet); the key table holds every pass's keys at load <= 0.5
+  plan.passes = local_passes_max(plan.limit);
+  if (plan.limit > 0 && j.groups_seen >= 0)
+    plan.passes = (int32_t)std::max<int64_t>(
+        1, std::min<int64_t>(plan.passes, (j.est + plan.limit - 1) / plan.limit));
+  plan.gcap = plan.lcap;
+  while (plan.gcap < 2 * plan.limit * plan.passes) plan.gcap *= 2;
   plan.lds_max = j.lds_max;
   if (plan.limit == 0) {
     j.no_local = true;
@@ -1390,13 +1411,13 @@ int join_plan_local(JoinJob& j) {
   if (j.prows >= (int64_t)UINT32_MAX)
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: GROUP BY above 2^32 rows");
   j.RW = 3 + j.naggs;
-  j.rec_cap = plan.limit;
+  j.rec_cap = (int64_t)plan.limit * plan.passes;
   memset(&j.B, 0, sizeof j.B);
   if (jit_local_prepare(j.ps, j.jd, plan.lcap, &j.ll, &jerr))
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: GROUP BY kernels unavailable: %s", jerr.c_str());
   plan.lgrid = j.ll.grid;
   plan.kgrid = (int32_t)std::max<int64_t>(1, std::min<int64_t>(j.ps.ntiles, 2 * (int64_t)j.ll.grid));
-  size_t bytes = 16 * 4 + (size_t)plan.lcap * 8 + (size_t)plan.limit * 8 +
+  size_t bytes = 16 * 4 + (size_t)plan.gcap * 8 + (size_t)plan.limit * plan.passes * 8 +
                  (size_t)plan.lgrid * plan.limit * (j.nw + 2) * 8 + (size_t)j.rec_cap * j.RW * 8;
   bytes += 8 * 256;  // alignment of the carved pieces
   j.bytes = (bytes + 255) & ~(size_t)255;
@@ -1533,8 +1554,8 @@ void join_bind(JoinJob& j, char* dev, char* host) {
   j.hflags = (uint32_t*)host;
   j.hout = (uint64_t*)(host + 64);
   if (j.plan.local) {
-    B.l.keys = cv.take<int64_t>(dev, j.plan.lcap);
-    B.l.gkey = cv.take<int64_t>(dev, j.plan.limit);
+    B.l.keys = cv.take<int64_t>(dev, j.plan.gcap);
+    B.l.gkey = cv.take<int64_t>(dev, (size_t)j.plan.limit * j.plan.passes);
     B.l.part = cv.take<uint64_t>(dev, (size_t)j.plan.lgrid * j.plan.limit * (j.nw + 2));
     B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
     return;

@@ -214,9 +214,9 @@ struct PartBufs {
 // (slots of kEmptyKey are free; key INT64_MIN is flagged in flags[10]), the
 // key of each group index, and the workgroups' partial accumulators.
 struct LocalBufs {
-  int64_t* keys;        // [lcap]
-  int64_t* gkey;        // [limit]
-  uint64_t* part;       // [grid][groups][nwords + 2]
+  int64_t* keys;        // [gcap] the key table
+  int64_t* gkey;        // [limit * passes] the keys in rank order
+  uint64_t* part;       // [grid][limit][nwords + 2] one pass's partials
 };
 
 struct JoinBufs {
@@ -251,8 +251,10 @@ struct JoinPlan {
   int32_t reps;         // accumulator replicas (power of two, 1..16)
   // GROUP BY of few keys through per-workgroup partials instead of partitions
   int32_t local;
-  int32_t lcap;         // its key table's slots (a power of two)
-  int32_t limit;        // the most keys it holds (lcap / 2)
+  int32_t lcap;         // a pass's LDS key table slots (a power of two)
+  int32_t limit;        // the most keys a pass holds (lcap / 2)
+  int32_t passes;       // passes over the rows, `limit` keys each
+  int32_t gcap;         // the key table's slots (2 * limit * passes, rounded up)
   int32_t lgrid;        // workgroups that aggregate (partials)
   int32_t kgrid;        // workgroups that collect the keys
   int32_t tnodes;       // top-k merge counters (1 with more than one partition)
@@ -312,6 +314,8 @@ struct JitArgs {
   uint32_t bloom_lpp;
   uint32_t bloom_pmask;
   uint64_t* livebits;  // count pass -> scatter: one bit per live row
+  int32_t lk0;         // vq_hash_local: the first key (rank) of this pass
+  int32_t lpad_;
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -361,7 +365,7 @@ struct LocalLaunch {
 constexpr int32_t kLocalLds = 60 * 1024;  // vq_hash_local's dynamic LDS
 int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunch* out,
                       std::string* err);
-hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s);
+hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s, int32_t k0);
 // LDS the per-workgroup path needs for `limit` keys of `nwords` words.
 size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords);
 hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,

@@ -377,8 +377,9 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
   return 0;
 }
 
-hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s) {
+hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s, int32_t k0) {
   JitArgs a = ll.args;
+  a.lk0 = k0;
   a.lkeys = B.l.keys;
   a.lflags = B.flags;
   a.lgkey = B.l.gkey;
