@@ -328,12 +328,12 @@ def test_agg_hash_matches_oracle(gpu_ctx, nrows, groups, key_type, skew):
 
 
 def test_agg_hash_path_by_key_count(gpu_ctx):
-    """Up to 4 x 512 keys (with these aggregates' 9 accumulator words)
+    """Up to 8 x 512 keys (with these aggregates' 9 accumulator words)
     aggregate through per-workgroup partials (no partitions), in passes of
     512 keys; more go through the partitions, on the first query already and
     directly on later ones.  Every run gives the same bits."""
     _print_case("agg_hash_paths")
-    for groups, local in ((400, True), (700, True), (2000, True), (2100, False)):
+    for groups, local in ((400, True), (700, True), (4000, True), (4200, False)):
         t = datagen.keyed_table(400_000, groups, abi.VQ_I64, 0.0, seed=groups)
         pred, key, aggs = datagen.q_keyed(t)
         dt = gpu_ctx.upload(t)

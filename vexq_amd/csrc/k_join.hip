@@ -1299,6 +1299,15 @@ hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
   }
   k.keys = B.l.keys;
   k.flags = B.flags;
+  {
+    // the key table's LDS copy: up to 64 KB
+    static std::atomic<int32_t> allowed{0};
+    const int32_t bytes = plan.gcap * 8;
+    if (allowed.load() < bytes) {
+      hipFuncSetAttribute((const void*)k_local_keys, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+      allowed.store(bytes);
+    }
+  }
   hipLaunchKernelGGL(k_local_keys, dim3((unsigned)plan.kgrid), dim3(kLocalThreads),
                      (size_t)plan.gcap * 8, s, k);
   hipLaunchKernelGGL(k_local_rank, dim3(1), dim3(kRankThreads), 0, s, (const int64_t*)B.l.keys,

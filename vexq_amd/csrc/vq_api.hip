@@ -1251,8 +1251,8 @@ constexpr int32_t kPoolMax = 2048;  // the join kernel's top-k holds 2 entries p
 // partitions.
 // The key table (every pass's keys at load <= 0.5) is at most kLocalSlots
 // slots: the key-collecting kernel holds a copy in LDS.
-constexpr int32_t kLocalPasses = 4;
-constexpr int32_t kLocalSlots = 4096;
+constexpr int32_t kLocalPasses = 8;
+constexpr int32_t kLocalSlots = 8192;
 int32_t local_limit(int32_t nwords) {
   for (int32_t limit = 1024; limit >= 256; limit /= 2)
     if (local_lds_for(2 * limit, limit, nwords) <= (size_t)kLocalLds) return limit;
