@@ -322,10 +322,10 @@ int vq_wait(vq_ticket* ticket);
 /* GROUP BY one integer column of any cardinality: HashAggregate's general
  * key path (exec/aggregate.go:361-559; the int-key path's output order,
  * exec/agg_intkey.go:206-361) for keys that are not small dictionary codes —
- * u8/u16/date/int64 columns, up to millions of groups.  Up to 256-1024
+ * u8/u16/date/int64 columns, up to millions of groups.  Up to 8 x 256-1024
  * distinct keys (fewer for more accumulator words): every workgroup
- * aggregates an even share of the rows into its own
- * partials, merged per group (ParallelHashAggregate's partials and
+ * aggregates an even share of the rows into its own partials, in passes of
+ * up to 256-1024 keys, merged per group (ParallelHashAggregate's partials and
  * mergePartialAgg, exec/parallel.go:18-314); more: rows passing pred are
  * radix-partitioned on radixHash(key) and each partition aggregated in an
  * LDS hash table.  out->keys holds the key values, groups in first-occurrence
