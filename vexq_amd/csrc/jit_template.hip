@@ -757,15 +757,17 @@ __device__ __forceinline__ void values_of(const Args& a, const i64* row, u64* v,
 }
 
 // The semi-join filter's test of a key with hash h: one 64-byte line of its
-// partition's block (hash bits 20..), two bits of it (hash bits 40..48 and
-// 49..57); the same bits k_bloom_build sets.
+// partition's block (hash bits 20..), one 64-bit word of the line (bits
+// 40..42), two bits of the word (bits 43..48 and 49..54); the same bits
+// k_bloom_build sets.
 __device__ __forceinline__ bool bloom_test(const Args& a, u64 h) {
   const u32 p = (u32)h & a.bloom_pmask;
   const u32 lines = 1u << a.bloom_lpp;
   const u32 line = p * lines + ((u32)(h >> 20) & (lines - 1));
-  const u32 b1 = (u32)(h >> 40) & 511u, b2 = (u32)(h >> 49) & 511u;
-  const u32* w = a.bloom + (u64)line * 16;
-  return ((w[b1 >> 5] >> (b1 & 31)) & (w[b2 >> 5] >> (b2 & 31)) & 1u) != 0;
+  // both bits in one 64-bit word of the line: one load
+  const u64 x = ((const u64*)a.bloom)[(u64)line * 8 + ((u32)(h >> 40) & 7u)];
+  const u32 b1 = (u32)(h >> 43) & 63u, b2 = (u32)(h >> 49) & 63u;
+  return ((x >> b1) & (x >> b2) & 1ull) != 0;
 }
 
 constexpr int kNV1 = VQ_NV > 0 ? VQ_NV : 1;

@@ -123,8 +123,9 @@ __global__ __launch_bounds__(kScanThreads) void k_part_starts(const PartScan a) 
 
 // ---- semi-join filter ---------------------------------------------------------
 // One block per partition builds the partition's filter block (2^lpp lines
-// of 64 bytes) in LDS from its build entries — two bits per key, in one line
-// chosen by the key's hash (the bits vq_part_count / vq_part_scatter test,
+// of 64 bytes) in LDS from its build entries — two bits per key, in one
+// 64-bit word of one line, both chosen by the key's hash (the bits
+// vq_part_count / vq_part_scatter test,
 // jit_template.hip bloom_test) — and writes it whole: no global atomics, no
 // zero fill.
 __global__ __launch_bounds__(256) void k_bloom_build(const int64_t* __restrict__ keys,
@@ -141,10 +142,12 @@ __global__ __launch_bounds__(256) void k_bloom_build(const int64_t* __restrict__
   __syncthreads();
   for (uint32_t e = pstart[b] + threadIdx.x; e < pstart[b + 1]; e += 256) {
     const uint64_t h = radix_hash((uint64_t)keys[e]);
+    // both bits in one 64-bit word of the line (one load per test)
     const uint32_t line = (uint32_t)(h >> 20) & (lines - 1);
-    const uint32_t b1 = (uint32_t)(h >> 40) & 511u, b2 = (uint32_t)(h >> 49) & 511u;
-    atomicOr(&seg[line * 16 + (b1 >> 5)], 1u << (b1 & 31));
-    atomicOr(&seg[line * 16 + (b2 >> 5)], 1u << (b2 & 31));
+    const uint32_t w = line * 16 + (((uint32_t)(h >> 40) & 7u) << 1);
+    const uint32_t b1 = (uint32_t)(h >> 43) & 63u, b2 = (uint32_t)(h >> 49) & 63u;
+    atomicOr(&seg[w + (b1 >> 5)], 1u << (b1 & 31));
+    atomicOr(&seg[w + (b2 >> 5)], 1u << (b2 & 31));
   }
   __syncthreads();
   uint32_t* out = bloom + (size_t)p * words;
