@@ -557,9 +557,10 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
   // smem, so the compiler keeps it an LDS address)
   KItem* ritems = (KItem*)(smem + ((((size_t)ns * 12 + (size_t)R * NP * NW * 8 + (size_t)NP * 8 +
                                      (size_t)R * NP * 8) + 15) / 16) * 2);
-  // wave w updates accumulator replica w mod R: with few hot keys the
-  // replicas spread what would be one LDS address's atomics over R addresses
-  const int myrep = (int)(threadIdx.x >> 6) & (R - 1);
+  // lane l of every wave updates accumulator replica l mod R: with few hot
+  // keys the replicas spread what would be one LDS address's atomics (lanes
+  // of one instruction on the same address serialize) over R addresses
+  const int myrep = (int)threadIdx.x & (R - 1);
   // the partition's bounds, this thread's first build key and first probe
   // entry: loading while the tables are cleared
   const uint32_t b0 = plan.groupby ? 0 : B.b.fpstart[p], b1 = plan.groupby ? 0 : B.b.fpstart[p + 1];
