@@ -1269,13 +1269,20 @@ __global__ __launch_bounds__(kMergeThreads) void k_local_merge(const JDesc jd, c
     if (lane == 0) s_words[i] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned long long cnt = s_words[NW];
-    if (cnt == 0) return;  // a key whose rows all failed the predicate
-    const uint32_t at = atomicAdd(&m.flags[1], 1u);
-    emit_record(jd, m.recs + (size_t)at * (3 + jd.naggs), m.gkey[k0 + g], cnt, 1,
-                (uint32_t)s_words[NW + 1], s_words);
+  // the record, by wave 0: lane j the word of aggregate j (their roundings
+  // run side by side), lane 0 the key, count and first row
+  const unsigned long long cnt = s_words[NW];
+  if (cnt == 0 || wv != 0) return;  // (a key whose rows all failed the predicate)
+  uint32_t at = 0;
+  if (lane == 0) at = atomicAdd(&m.flags[1], 1u);
+  at = (uint32_t)__shfl((int)at, 0, 64);
+  uint64_t* rec = m.recs + (size_t)at * (3 + jd.naggs);
+  if (lane == 0) {
+    rec[0] = (uint64_t)m.gkey[k0 + g];
+    rec[1] = cnt;
+    rec[2] = (uint32_t)s_words[NW + 1];
   }
+  if (lane < jd.naggs) rec[3 + lane] = agg_word_of(jd.agg[lane], cnt, 1, s_words);
 }
 
 size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords) {
