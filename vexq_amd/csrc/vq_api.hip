@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -1762,22 +1763,45 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   if (nout > out->capacity)
     return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)nout,
                 (long long)out->capacity);
-  for (int64_t i = 0; i < nout; i++) {
-    if (i + 8 < nout) __builtin_prefetch(&recs[(size_t)(uint32_t)o[i + 8] * RW]);
-    const uint64_t* r = &recs[(size_t)(uint32_t)o[i] * RW];
-    out->keys[i] = (int64_t)r[0];
-    out->counts[i] = (int64_t)r[1];
-    out->first_row[i] = (int64_t)r[2];
-    for (int k = 0; k < naggs; k++) {
-      const int64_t o = i * naggs + k;
-      out->valid[o] = 1;
-      out->ivals[o] = 0;
-      out->fvals[o] = 0;
-      const vq_agg& a = j.aggs[k];
-      const bool isf = a.kind == VQ_AGG_AVG || (a.kind != VQ_AGG_COUNT && a.accum == VQ_F64);
-      if (isf) out->fvals[o] = bits_to_f64((int64_t)r[3 + k]);
-      else out->ivals[o] = (int64_t)r[3 + k];
+  bool isf[VQ_MAX_AGGS];
+  for (int k = 0; k < naggs; k++) {
+    const vq_agg& a = j.aggs[k];
+    isf[k] = a.kind == VQ_AGG_AVG || (a.kind != VQ_AGG_COUNT && a.accum == VQ_F64);
+  }
+  auto fill = [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; i++) {
+      if (i + 8 < i1) __builtin_prefetch(&recs[(size_t)(uint32_t)o[i + 8] * RW]);
+      const uint64_t* r = &recs[(size_t)(uint32_t)o[i] * RW];
+      out->keys[i] = (int64_t)r[0];
+      out->counts[i] = (int64_t)r[1];
+      out->first_row[i] = (int64_t)r[2];
+      for (int k = 0; k < naggs; k++) {
+        const int64_t q = i * naggs + k;
+        out->valid[q] = 1;
+        out->ivals[q] = isf[k] ? 0 : (int64_t)r[3 + k];
+        out->fvals[q] = isf[k] ? bits_to_f64((int64_t)r[3 + k]) : 0.0;
+      }
     }
+  };
+  // many groups: the rows filled by several threads (most of the time goes
+  // to first touches of the caller's fresh result pages)
+  const int64_t kRowsPerThread = 32768;
+  const int nt = (int)std::min<int64_t>(8, nout / kRowsPerThread);
+  if (nt >= 2) {
+    std::vector<std::thread> th;
+    const int64_t step = (nout + nt - 1) / nt;
+    for (int t = 1; t < nt; t++) {
+      const int64_t i0 = std::min(nout, t * step), i1 = std::min(nout, (t + 1) * step);
+      try {
+        th.emplace_back(fill, i0, i1);
+      } catch (const std::system_error&) {
+        fill(i0, i1);  // (no thread to be had: this one fills the range)
+      }
+    }
+    fill(0, std::min(nout, step));
+    for (auto& x : th) x.join();
+  } else {
+    fill(0, nout);
   }
   return VQ_OK;
 }
