@@ -1232,14 +1232,19 @@ struct JoinJob {
   uint64_t* hout = nullptr;
 };
 
+// Table fill: a GROUP BY's rows all find their keys, but fill the table by
+// compare-and-swap as they arrive; a join's build keys go in first, and the
+// semi-join filter leaves few probe rows whose key is missing (a miss walks
+// to an empty slot), so a join's tables fill further.
 constexpr double kLoad = 0.4;
+constexpr double kLoadJoin = 0.55;
 
 // Partition tables sized for the keys a partition may get, not just the mean:
 // `per` keys on average, with Poisson spread over the partitions.
 double keys_high(double per) { return per + 4 * std::sqrt(per) + 8; }
-int32_t table_cap_for(double per, int32_t cap_max) {
+int32_t table_cap_for(double per, int32_t cap_max, double load = kLoad) {
   int32_t cap = 8;
-  while (cap < cap_max && (cap * kLoad < per || cap < keys_high(per))) cap *= 2;
+  while (cap < cap_max && (cap * load < per || cap < keys_high(per))) cap *= 2;
   return cap;
 }
 constexpr int32_t kPoolMax = 2048;  // the join kernel's top-k holds 2 entries per thread
@@ -1329,8 +1334,8 @@ int join_setup(JoinJob& j) {
   // (planner/parallel_join.go:327-339), until a query of the same shape over
   // the same build side has run: then the rows that passed last time (for a
   // GROUP BY, the distinct keys).  A table at most 4096 slots filled to at
-  // most kLoad keeps linear probing short for every lane of a wave (a wave
-  // waits for its longest probe).  The pool starts at half the table, or at
+  // most kLoad (a join: kLoadJoin) keeps linear probing short for every lane
+  // of a wave (a wave waits for its longest probe).  The pool starts at half the table, or at
   // twice the matched keys per partition of the last query of the same shape,
   // and doubles when it fills; a table that fills adds one bit.
   j.est_key = j.groupby ? "G|" + build_est_key(j.probe, j.probe_key, j.pp)
@@ -1363,7 +1368,7 @@ int join_setup(JoinJob& j) {
     // the partition's keys must fit the pool next to the table as well)
     auto fits = [&](int b) {
       const double per = (double)j.est / (1 << b);
-      if (per > kLoad * j.cap_max) return false;
+      if (per > (j.groupby ? kLoad : kLoadJoin) * j.cap_max) return false;
       if (!j.groupby) return true;
       const double need = keys_high(per) + 56;
       return need <= kPoolMax &&
@@ -1438,7 +1443,7 @@ int join_plan(JoinJob& j) {
   plan.bits = j.bits;
   plan.P = 1 << j.bits;
   const double per = (double)j.est / plan.P;
-  const int32_t cap = table_cap_for(per, j.cap_max);
+  const int32_t cap = table_cap_for(per, j.cap_max, j.groupby ? kLoad : kLoadJoin);
   plan.cap = cap;
   // the largest pool the LDS and the kernel's top-k allow
   j.pool_max = std::min<int32_t>(kPoolMax, cap);
