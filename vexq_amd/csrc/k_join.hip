@@ -1094,11 +1094,17 @@ __device__ __forceinline__ int64_t load_key(const void* col, int32_t type, int64
 }
 
 // Inserts key into an open-addressing table of cap slots; returns 1 if it
-// was new, 0 if present, -1 if the table is full.
+// was new, 0 if present, -1 if it found no place within kMaxProbes slots (a
+// table filled past its key limit: a full one would cost a walk over every
+// slot for each key).  At the tables' load (at most one half while within
+// the limit) a probe sequence that long does not occur in practice; when it
+// does, the query runs through the partitions.
+constexpr int kMaxProbes = 128;
 template <int SCOPE>
 __device__ __forceinline__ int table_insert(unsigned long long* t, int cap, int lg, int64_t key) {
   uint32_t i = (uint32_t)(radix_hash((uint64_t)key) >> (64 - lg));
-  for (int pr = 0; pr < cap; pr++) {
+  const int np = cap < kMaxProbes ? cap : kMaxProbes;
+  for (int pr = 0; pr < np; pr++) {
     unsigned long long cur = __hip_atomic_load(&t[i], __ATOMIC_RELAXED, SCOPE);
     if (cur == (unsigned long long)key) return 0;
     if (cur == (unsigned long long)kEmptyKey) {
@@ -1163,6 +1169,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_keys(const LocalKeys k)
   for (int i = threadIdx.x; i < cap; i += kLocalThreads) {
     const int64_t key = (int64_t)lt[i];
     if (key == kEmptyKey) continue;
+    // (past the limit, stop: another workgroup's keys already overflowed)
+    if (__hip_atomic_load(&k.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kLocalOverflow) break;
     const int got = table_insert<__HIP_MEMORY_SCOPE_AGENT>((unsigned long long*)k.keys, cap, lg, key);
     if (got < 0 || (got > 0 && atomicAdd(&k.flags[2], 1u) + 1 > (uint32_t)k.limit))
       atomicOr(&k.flags[0], kLocalOverflow);
