@@ -643,13 +643,7 @@ size_t dense_lds_bytes(const QDesc& d) {
 
 // Waves the persistent dense kernel runs with: what fits the device at once.
 int32_t dense_grid(const QDesc& d) {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) cus = 256;
-  }
+  const int cus = device_cus();
   int per_cu = 0;
   if (d.ngroups == 1)
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_agg_dense<true, kU>, kWave, 0);

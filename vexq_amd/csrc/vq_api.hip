@@ -69,9 +69,14 @@ struct Slot {
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   // full-output GROUP BY and join records on their way to the host (grown on
-  // demand; a pageable copy of megabytes costs milliseconds)
-  void* rec_pinned = nullptr;
-  size_t rec_pinned_bytes = 0;
+  // demand; a pageable copy of megabytes costs milliseconds): one for a query
+  // run alone, one per result-assembling thread of a batch
+  struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  static constexpr int kAssemblers = 8;
+  Pinned rec_pin[kAssemblers];
   hipEvent_t ev[6] = {};
   // batches: branch streams forked from `stream` while a graph is captured,
   // their join events, and the instantiated graphs of recent batches
@@ -217,7 +222,8 @@ void slot_destroy(vq_ctx* c, Slot* S) {
     release(c, S->arena.bytes);
   }
   if (S->pinned) hipHostFree(S->pinned);
-  if (S->rec_pinned) hipHostFree(S->rec_pinned);
+  for (auto& r : S->rec_pin)
+    if (r.p) hipHostFree(r.p);
   for (auto& ev : S->ev)
     if (ev) hipEventDestroy(ev);
   hipStreamDestroy(S->stream);
@@ -1214,7 +1220,7 @@ struct JoinJob {
   bool dev_topk = false;
   int bits = 0;
   int32_t pool_req = 0;
-  Slot* slot = nullptr;    // the leased slot, when the job runs alone (its record staging)
+  Slot::Pinned* pin = nullptr;  // pinned staging for the records (null: pageable)
   bool no_local = false;   // GROUP BY: too many keys for per-workgroup partials
   bool livebits = false;   // the probe side's count pass records its live rows
   LocalLaunch ll;
@@ -1665,21 +1671,21 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   // calls: its pages are already mapped, and it is never zero-filled again)
   thread_local std::vector<uint64_t> hrec;
   const uint64_t* recs = j.hout;
-  if (!dev_topk && j.slot) {
-    // through the slot's pinned staging
-    Slot* S = j.slot;
+  if (!dev_topk && j.pin) {
+    // through pinned staging (the slot's)
+    Slot::Pinned& pin = *j.pin;
     const size_t bytes = (size_t)nrec * RW * 8 + 8;
-    if (S->rec_pinned_bytes < bytes) {
-      if (S->rec_pinned) HIP_TRY(hipHostFree(S->rec_pinned));
-      S->rec_pinned = nullptr;
-      S->rec_pinned_bytes = 0;
+    if (pin.bytes < bytes) {
+      if (pin.p) HIP_TRY(hipHostFree(pin.p));
+      pin.p = nullptr;
+      pin.bytes = 0;
       size_t want = (size_t)1 << 20;
       while (want < bytes) want *= 2;
-      HIP_TRY(hipHostMalloc(&S->rec_pinned, want, hipHostMallocDefault));
-      S->rec_pinned_bytes = want;
+      HIP_TRY(hipHostMalloc(&pin.p, want, hipHostMallocDefault));
+      pin.bytes = want;
     }
-    if (nrec) HIP_TRY(hipMemcpy(S->rec_pinned, j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
-    recs = (const uint64_t*)S->rec_pinned;
+    if (nrec) HIP_TRY(hipMemcpy(pin.p, j.B.recs, (size_t)nrec * RW * 8, hipMemcpyDeviceToHost));
+    recs = (const uint64_t*)pin.p;
   } else if (!dev_topk) {
     const size_t words = (size_t)nrec * RW + 1;
     if (hrec.size() < words) {
@@ -1849,7 +1855,7 @@ int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
     if (int rc = arena_get(c, S.get(), j.bytes, &ar)) return rc;
     if (int rc = pinned_get(S.get(), j.host_bytes, &hp)) return rc;
     join_bind(j, (char*)ar, (char*)hp);
-    j.slot = S.get();
+    j.pin = &S->rec_pin[0];
     HIP_TRY(join_enqueue(j, S->stream, S->ev));
     HIP_TRY(hipStreamSynchronize(S->stream));
     const int rc = join_result(j, out, stats, S->ev);
@@ -2236,10 +2242,14 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
     if (gb && (q.nkeys != 1 || !q.key_cols)) return fail(VQ_E_ARG, "exec: gpu: batch query %d: one key", i);
     jobs[i].reset(new (std::nothrow) JoinJob());
     if (!jobs[i]) return fail(VQ_E_NOMEM, "exec: gpu: batch job allocation");
-    JoinJob& j = *jobs[i];
-    join_job_init(j, c, gb ? nullptr : q.build, q.build_key, q.build_pred, q.table,
+    join_job_init(*jobs[i], c, gb ? nullptr : q.build, q.build_key, q.build_pred, q.table,
                   gb ? q.key_cols[0] : q.probe_key, q.pred, q.naggs, q.aggs, gb ? -1 : q.radix_bits,
                   gb ? 0 : q.topk, gb);
+  }
+  // each instance's plan (its class's tuning first; with VQ_BATCH_TUNE a
+  // class not tuned yet is tuned here, by running it)
+  auto plan_one = [&](int32_t i) -> int {
+    JoinJob& j = *jobs[i];
     if (int rc = join_setup(j)) return rc;
     const std::string cls = tune_class(j);
     JoinTune t;
@@ -2251,7 +2261,42 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
       j.tune = t;
       if (int rc = join_setup(j)) return rc;  // the bits again, with the tuning
     }
-    if (int rc = join_plan(j)) return rc;
+    return join_plan(j);
+  };
+  if (flags & VQ_BATCH_TUNE) {
+    for (int32_t i = 0; i < n; i++)
+      if (jobs[i])
+        if (int rc = plan_one(i)) return rc;
+  } else {
+    // planned by up to eight host threads (zone pruning, spans and kernel
+    // lookups per instance); the first failure in instance order is reported
+    std::vector<int> prc(n, VQ_OK);
+    std::vector<std::string> perr(n);
+    std::atomic<int32_t> next{0};
+    auto worker = [&]() {
+      hipSetDevice(c->device);
+      for (int32_t i; (i = next.fetch_add(1)) < n;) {
+        if (!jobs[i]) continue;
+        prc[i] = plan_one(i);
+        if (prc[i] != VQ_OK) perr[i] = g_err;
+      }
+    };
+    const int nt = (int)std::min<int32_t>(8, std::max<int32_t>(1, n / 16));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) {
+      try {
+        th.emplace_back(worker);
+      } catch (const std::system_error&) {
+        break;  // (the threads there are, this one included, plan them all)
+      }
+    }
+    worker();
+    for (auto& x : th) x.join();
+    for (int32_t i = 0; i < n; i++)
+      if (prc[i] != VQ_OK) {
+        g_err = perr[i];
+        return prc[i];
+      }
   }
   std::vector<int> rcs(n, VQ_OK);
   std::vector<std::string> errs(n);
@@ -2339,19 +2384,27 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
       // the instances' results, assembled by a few host threads (copying and
       // ordering the groups of a large batch is host-memory bound)
       std::atomic<int32_t> next{0};
-      auto assemble = [&]() {
+      auto assemble = [&](int k) {
         hipSetDevice(c->device);
         for (int32_t i; (i = next.fetch_add(1)) < n;) {
           if (!jobs[i]) continue;
+          jobs[i]->pin = &S->rec_pin[k];
           rcs[i] = join_result(*jobs[i], &outs[i], stats ? &stats[i] : nullptr, nullptr);
+          jobs[i]->pin = nullptr;
           if (rcs[i] != VQ_OK && rcs[i] != kRetry) errs[i] = g_err;
           if (stats && rcs[i] == VQ_OK) stats[i].device_ms = batch_ms;  // the whole batch's graph
         }
       };
-      const int nt = std::max(1, std::min<int>({8, n, (int)std::thread::hardware_concurrency()}));
+      const int nt = std::max(1, std::min<int>({Slot::kAssemblers, n, (int)std::thread::hardware_concurrency()}));
       std::vector<std::thread> helpers;
-      for (int k = 1; k < nt; k++) helpers.emplace_back(assemble);
-      assemble();
+      for (int k = 1; k < nt; k++) {
+        try {
+          helpers.emplace_back(assemble, k);
+        } catch (const std::system_error&) {
+          break;  // (the threads there are take every instance)
+        }
+      }
+      assemble(0);
       for (auto& h : helpers) h.join();
     }
   }

@@ -258,13 +258,13 @@ bool jit_disabled() {
 int64_t part_chunk_rows(int nslots) { return (int64_t)1024 * rows_in_flight(nslots); }
 
 int device_cus() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
+  // (initialised once, by whichever thread asks first)
+  static const int cus = [] {
+    int dev = 0, n = 0;
     hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) cus = 256;
-  }
+    hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    return n > 0 ? n : 256;
+  }();
   return cus;
 }
 
@@ -283,13 +283,7 @@ int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, 
   const int32_t entry = jd ? 8 + 4 + 8 * nv : 8;
   out->count_lds = P <= 16384 ? (uint32_t)P * 4 : 0;
   out->scatter_lds = part_scatter_lds(P, pbuf, entry);
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) cus = 256;
-  }
+  const int cus = device_cus();
   int per_cu = 0;
   hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->part_scatter, 1024,
                                                      out->scatter_lds);
@@ -342,13 +336,7 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
   if (!m) return -1;
   out->fn = m->hash_local;
   out->lds = kLocalLds;
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) cus = 256;
-  }
+  const int cus = device_cus();
   int per_cu = 0;
   hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->hash_local, 512, out->lds);
   if (per_cu <= 0) per_cu = 1;
@@ -394,13 +382,7 @@ int jit_dense_grid(const QDesc& d, std::string* err) {
   Module* m = get_module(prologue(d, f), err);
   if (!m) return -1;
   const size_t lds = dense_lds_bytes(d);
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) cus = 256;
-  }
+  const int cus = device_cus();
   int per_cu = 0;
   hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->dense, 64, lds);
   if (per_cu <= 0) per_cu = 1;
