@@ -83,6 +83,17 @@ typedef struct vq_col_desc {
 #define VQ_LE 3
 #define VQ_GT 4
 #define VQ_GE 5
+/* col IN (set) over dictionary codes or small integers (InExpr,
+ * exec/expr.go:1188-1238, with the planner's string literals already turned
+ * into global codes): ival is a 64-bit mask, a row passes when its value v
+ * is in [0, 64) and bit v is set.  Integer-typed columns only.  Never prunes
+ * a row group. */
+#define VQ_IN 6
+/* op | VQ_COLCMP compares two columns of the row, col OP column ival (a
+ * BinOp compare of two column references, exec/expr.go:320-896, e.g.
+ * l_commitdate < l_receiptdate): both float64, or both integer-typed
+ * (codes, dates and int64 compared as int64).  Never prunes a row group. */
+#define VQ_COLCMP 16
 
 #define VQ_MAX_TERMS 8
 typedef struct vq_term {

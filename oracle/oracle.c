@@ -225,8 +225,20 @@ static inline int pred_pass(const vq_col_desc* cols, const vq_pred* p, int64_t r
   for (int t = 0; t < p->nterms; t++) {
     const vq_term* tm = &p->terms[t];
     const vq_col_desc* c = &cols[tm->col];
-    int ok = c->type == VQ_F64 ? cmp_f(tm->op, col_f(c, r), tm->fval)
-                               : cmp_i(tm->op, col_i(c, r), tm->ival);
+    int ok;
+    if (tm->op == VQ_IN) {
+      /* InExpr (exec/expr.go:1188-1238) over codes: the value is one of the
+       * set's members, given as a mask over 0..63 */
+      const int64_t v = col_i(c, r);
+      ok = v >= 0 && v < 64 && (((uint64_t)tm->ival >> v) & 1u);
+    } else if (tm->op & VQ_COLCMP) {
+      /* a BinOp compare of two columns (exec/expr.go:320-896) */
+      const vq_col_desc* c2 = &cols[tm->ival];
+      const int op = tm->op & ~VQ_COLCMP;
+      ok = c->type == VQ_F64 ? cmp_f(op, col_f(c, r), col_f(c2, r)) : cmp_i(op, col_i(c, r), col_i(c2, r));
+    } else {
+      ok = c->type == VQ_F64 ? cmp_f(tm->op, col_f(c, r), tm->fval) : cmp_i(tm->op, col_i(c, r), tm->ival);
+    }
     if (!ok) return 0;
   }
   return 1;
@@ -360,6 +372,7 @@ void or_zone_maps(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t
  * match; NE never prunes. */
 static int zone_term_keep(const vq_col_desc* c, const vq_zone* z, const vq_term* t) {
   if (!z->has_minmax) return 1;
+  if (t->op == VQ_IN || (t->op & VQ_COLCMP)) return 1; /* (never prunes) */
   if ((c->type == VQ_U8 || c->type == VQ_U16) && c->dict_size > 0) return 1;
   int64_t v;
   if (c->type == VQ_F64) memcpy(&v, &t->fval, 8);

@@ -78,6 +78,77 @@ def test_filter_matches_oracle(gpu_ctx, nrows, kernel_path):
         dt.close()
 
 
+def test_filter_in_and_column_compares(gpu_ctx, kernel_path):
+    """IN over codes and comparisons of two columns of the row (the Q12
+    shape), alone and with literal terms, against the oracle."""
+    from test_oracle import q12_table
+    _print_case("filter_in_colcmp", path=kernel_path)
+    t = q12_table(300_000, seed=4)
+    X = abi.X
+    preds = [
+        [("shipmode", "in", [1, 4])],
+        [("commitdate", "<", X.col("receiptdate"))],
+        [("shipmode", "in", [0, 2, 6]), ("commitdate", "<", X.col("receiptdate")),
+         ("shipdate", "<", X.col("commitdate")), ("receiptdate", ">=", 9000)],
+        [("discount", ">", X.col("tax")), ("quantity", "!=", X.col("quantity"))],
+        [("flagA", "in", [])],
+    ]
+    dt = gpu_ctx.upload(t)
+    try:
+        for terms in preds:
+            p = abi.make_pred(t, terms)
+            np.testing.assert_array_equal(gpu_ctx.filter(dt, p), O.filter_rows(t, p))
+    finally:
+        dt.close()
+
+
+def test_q12_shape_groupby_and_join(gpu_ctx, kernel_path):
+    """A Q12-shape predicate (shipmode IN (..), commitdate < receiptdate,
+    shipdate < commitdate, receiptdate in a range) under the dictionary GROUP
+    BY, the hash GROUP BY and the join's probe side (those two have run-time
+    compiled kernels only), against the oracle."""
+    from test_oracle import q12_table
+    _print_case("q12_shape", path=kernel_path)
+    X = abi.X
+    t = q12_table(400_000, seed=6)
+    terms = [("shipmode", "in", [1, 5]), ("commitdate", "<", X.col("receiptdate")),
+             ("shipdate", "<", X.col("commitdate")), ("receiptdate", ">=", 9500),
+             ("receiptdate", "<", 10500)]
+    pred = abi.make_pred(t, terms)
+    aggs = [abi.count_star(t), abi.agg_sum("price", t), abi.agg_sum(X.col("price") * (1 - X.col("discount")), t)]
+    dt = gpu_ctx.upload(t)
+    try:
+        got = gpu_ctx.agg_dict(dt, pred, [t.index("shipmode")], aggs)
+        assert_agg_equal(got, O.agg_dict(t, pred, [t.index("shipmode")], aggs, exact=True), aggs)
+        if kernel_path == "jit":
+            got = gpu_ctx.agg_hash(dt, pred, t.index("quantity"), aggs)
+            assert_agg_equal(got, O.agg_hash(t, pred, t.index("quantity"), aggs, exact=True), aggs)
+    finally:
+        dt.close()
+    if kernel_path != "jit":
+        return
+    # the join: the probe side carries commit / receipt dates
+    b, p = _join_tables(100_000, 400_000, seed=8)
+    ship = p.col("shipdate").astype(np.int64)
+    n = p.nrows
+    commit = ship + datagen.rand_int(8, 90, n, -30, 30)
+    receipt = ship + datagen.rand_int(8, 91, n, 1, 30)
+    p = datagen.Table(list(p.columns) + [datagen.Column("commitdate", abi.VQ_DATE, commit.astype(np.int32)),
+                                         datagen.Column("receiptdate", abi.VQ_DATE, receipt.astype(np.int32))],
+                      datagen.row_groups(n))
+    bp, _, jaggs = datagen.q_join(b, p)
+    pp = abi.make_pred(p, [("commitdate", "<", X.col("receiptdate")), ("shipdate", "<", X.col("commitdate"))])
+    db, dp = gpu_ctx.upload(b), gpu_ctx.upload(p)
+    try:
+        got = gpu_ctx.join_agg(db, 0, bp, dp, 0, pp, jaggs)
+        top = gpu_ctx.join_agg(db, 0, bp, dp, 0, pp, jaggs, topk=10)
+    finally:
+        db.close()
+        dp.close()
+    assert_agg_equal(got, O.join_agg(b, 0, bp, p, 0, pp, jaggs, exact=True), jaggs)
+    assert_agg_equal(top, O.join_agg(b, 0, bp, p, 0, pp, jaggs, topk=10, exact=True), jaggs)
+
+
 def test_filter_pruned_sorted(gpu_ctx):
     t = datagen.lineitem_sorted(1_000_000, seed=3)
     dt = gpu_ctx.upload(t)

@@ -217,6 +217,42 @@ def test_filter_matches_numpy_predicates():
         np.testing.assert_array_equal(O.filter_rows(t, abi.make_pred(t, terms)), np.nonzero(mask)[0])
 
 
+def q12_table(n, seed=9):
+    """lineitem plus the columns of a Q12-shape predicate: commit and receipt
+    dates around the ship date, and a ship mode code (7 values)."""
+    t = datagen.lineitem(n, seed=seed)
+    ship = t.columns[t.index("shipdate")].data.astype(np.int64)
+    commit = ship + datagen.rand_int(seed, 90, n, -30, 30)
+    receipt = ship + datagen.rand_int(seed, 91, n, 1, 30)
+    mode = datagen.rand_int(seed, 92, n, 0, 6).astype(np.uint8)
+    cols = list(t.columns) + [datagen.Column("commitdate", abi.VQ_DATE, commit.astype(np.int32)),
+                              datagen.Column("receiptdate", abi.VQ_DATE, receipt.astype(np.int32)),
+                              datagen.Column("shipmode", abi.VQ_U8, mode, 7)]
+    return datagen.Table(cols, datagen.row_groups(n))
+
+
+def test_filter_in_and_column_compares_match_numpy():
+    """IN over codes (a mask over [0, 64)) and comparisons of two columns of
+    the row (the Q12 shape: shipmode IN (...), commitdate < receiptdate,
+    shipdate < commitdate), alone and with literal terms."""
+    t = q12_table(200_000)
+    d = {c.name: c.data for c in t.columns}
+    X = abi.X
+    cases = [
+        ([("shipmode", "in", [1, 4])], np.isin(d["shipmode"], [1, 4])),
+        ([("commitdate", "<", X.col("receiptdate"))], d["commitdate"] < d["receiptdate"]),
+        ([("shipmode", "in", [0, 2, 6]), ("commitdate", "<", X.col("receiptdate")),
+          ("shipdate", "<", X.col("commitdate")), ("receiptdate", ">=", 9000)],
+         np.isin(d["shipmode"], [0, 2, 6]) & (d["commitdate"] < d["receiptdate"])
+         & (d["shipdate"] < d["commitdate"]) & (d["receiptdate"] >= 9000)),
+        ([("price", ">=", X.col("price"))], np.ones(t.nrows, bool)),
+        ([("discount", ">", X.col("tax"))], d["discount"] > d["tax"]),
+        ([("flagA", "in", [])], np.zeros(t.nrows, bool)),
+    ]
+    for terms, mask in cases:
+        np.testing.assert_array_equal(O.filter_rows(t, abi.make_pred(t, terms)), np.nonzero(mask)[0])
+
+
 # ---- dictionary-code GROUP BY -----------------------------------------------
 
 def test_intkey_remap_kat():

@@ -16,6 +16,8 @@ import ctypes as C
 VQ_OK, VQ_E_ARG, VQ_E_DEVICE, VQ_E_BUDGET, VQ_E_UNSUPPORTED, VQ_E_NOMEM = range(6)
 VQ_U8, VQ_U16, VQ_DATE, VQ_I64, VQ_F64 = 1, 2, 3, 4, 5
 VQ_EQ, VQ_NE, VQ_LT, VQ_LE, VQ_GT, VQ_GE = range(6)
+VQ_IN = 6        # column IN (set of values in [0, 64)): ival is the mask
+VQ_COLCMP = 16   # op | VQ_COLCMP: column OP column ival
 VQ_F_COL, VQ_F_LIT, VQ_F_LIT_OP_COL, VQ_F_COL_OP_LIT = range(4)
 VQ_ADD, VQ_SUB, VQ_MUL, VQ_DIV = range(4)
 VQ_AGG_COUNT, VQ_AGG_SUM, VQ_AGG_AVG, VQ_AGG_MIN, VQ_AGG_MAX = range(5)
@@ -101,7 +103,9 @@ class Query(C.Structure):
 
 def make_pred(table, terms) -> Pred:
     """terms: iterable of (column name, op string, literal).  BETWEEN is two
-    terms (>= lo, <= hi), the reference's own rewrite (exec/expr.go:1319-1336)."""
+    terms (>= lo, <= hi), the reference's own rewrite (exec/expr.go:1319-1336).
+    op "in" takes an iterable of values in [0, 64) (dictionary codes); a
+    literal X.col(name) compares with another column of the row."""
     terms = list(terms)
     if len(terms) > VQ_MAX_TERMS:
         raise ValueError("too many predicate terms")
@@ -111,6 +115,21 @@ def make_pred(table, terms) -> Pred:
         ci = table.index(name)
         t = p.terms[i]
         t.col = ci
+        if op == "in":
+            mask = 0
+            for v in lit:
+                if not 0 <= int(v) < 64:
+                    raise ValueError("IN values must lie in [0, 64)")
+                mask |= 1 << int(v)
+            t.op = VQ_IN
+            t.ival = mask - (1 << 64) if mask >= 1 << 63 else mask
+            continue
+        if isinstance(lit, X):
+            if not lit._single():
+                raise TypeError("a column compare takes one column, X.col(name)")
+            t.op = _OPS[op] | VQ_COLCMP
+            t.ival = table.index(lit.factors[0][1].col)
+            continue
         t.op = _OPS[op]
         if table.columns[ci].type == VQ_F64:
             t.fval = float(lit)

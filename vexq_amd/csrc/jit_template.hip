@@ -32,6 +32,7 @@
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kTS2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kKS[8] = {2, 1, 0, 0, 0, 0, 0, 0};
 static constexpr int kAOP[8] = {1, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kAISF[8] = {0, 1, 1, 1, 1, 0, 0, 0};
@@ -260,11 +261,23 @@ __device__ __forceinline__ void load_rows_blocked(const Args& a, i64 base, i64 r
   load_block_one(a, r0, raw, seq<VQ_NS>{});
 }
 
+// One predicate term: column OP literal; column IN set (a mask over
+// [0, 64)); or column OP column (op | 16).
 template <int T>
 __device__ __forceinline__ bool term(const Args& a, const i64* rawrow) {
-  constexpr int S = kTS[T];
-  if constexpr (kST[S] == VQ_F64) return cmp<kTO[T]>(as_f64(rawrow[S]), a.tf[T]);
-  else return cmp<kTO[T]>(rawrow[S], a.ti[T]);
+  constexpr int S = kTS[T], OP = kTO[T];
+  if constexpr (OP == 6) {
+    const i64 v = rawrow[S];
+    return (u64)v < 64 && (((u64)a.ti[T] >> v) & 1ull);
+  } else if constexpr (OP >= 16) {
+    constexpr int S2 = kTS2[T];
+    if constexpr (kST[S] == VQ_F64) return cmp<OP - 16>(as_f64(rawrow[S]), as_f64(rawrow[S2]));
+    else return cmp<OP - 16>(rawrow[S], rawrow[S2]);
+  } else if constexpr (kST[S] == VQ_F64) {
+    return cmp<OP>(as_f64(rawrow[S]), a.tf[T]);
+  } else {
+    return cmp<OP>(rawrow[S], a.ti[T]);
+  }
 }
 
 template <int... T>
@@ -696,7 +709,7 @@ template <int C>
 __device__ constexpr bool live_slot() {
   if (C == VQ_KEYSLOT) return true;
   for (int t = 0; t < VQ_NT; t++)
-    if (kTS[t] == C) return true;
+    if (kTS[t] == C || (kTO[t] >= 16 && kTS2[t] == C)) return true;  // (either side of a compare)
   return false;
 }
 constexpr bool kLateVals = (VQ_BLOOM != 0) && (VQ_PROBE != 0);

@@ -112,36 +112,6 @@ __device__ __forceinline__ void load_rows(const SlotLd* ld, int nslots, int64_t 
   }
 }
 
-template <int U>
-__device__ __forceinline__ void eval_pred(const QSlot* slot, int nslots, int64_t base, int64_t rend,
-                                          int lane, const int64_t (&raw)[kMaxSlots][U],
-                                          bool (&pass)[U], int (&g)[U]) {
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    pass[u] = base + u * kWave + lane < rend;
-    g[u] = 0;
-  }
-#pragma unroll
-  for (int c = 0; c < kMaxSlots; c++) {
-    if (c < nslots) {
-      const QSlot& s = slot[c];
-      const bool isf = s.type == VQ_F64;
-      for (int j = 0; j < s.nterms; j++) {
-        const QTerm t = s.terms[j];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          bool ok = isf ? cmp_f(t.op, bits_to_f64(raw[c][u]), t.fval) : cmp_i(t.op, raw[c][u], t.ival);
-          pass[u] = pass[u] && ok;
-        }
-      }
-      if (s.key_stride) {
-#pragma unroll
-        for (int u = 0; u < U; u++) g[u] += s.key_stride * (int)raw[c][u];
-      }
-    }
-  }
-}
-
 // Selects column `slot` of the loaded rows (slot is wave-uniform: the branch
 // is scalar and the copy is a register move).
 template <int U>
@@ -161,6 +131,53 @@ __device__ __forceinline__ void pick(const int64_t (&raw)[kMaxSlots][U], int slo
     default:
       _Pragma("unroll") for (int u = 0; u < U; u++) x[u] = raw[7][u];
 #undef VQ_PICK
+  }
+}
+
+template <int U>
+__device__ __forceinline__ void eval_pred(const QSlot* slot, int nslots, int64_t base, int64_t rend,
+                                          int lane, const int64_t (&raw)[kMaxSlots][U],
+                                          bool (&pass)[U], int (&g)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    pass[u] = base + u * kWave + lane < rend;
+    g[u] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < kMaxSlots; c++) {
+    if (c < nslots) {
+      const QSlot& s = slot[c];
+      const bool isf = s.type == VQ_F64;
+      for (int j = 0; j < s.nterms; j++) {
+        const QTerm t = s.terms[j];
+        if (t.op == VQ_IN) {  // a mask over [0, 64)
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const bool ok = (uint64_t)raw[c][u] < 64 && (((uint64_t)t.ival >> raw[c][u]) & 1ull);
+            pass[u] = pass[u] && ok;
+          }
+        } else if (t.op & VQ_COLCMP) {  // column OP column
+          int64_t y[U];
+          pick(raw, t.slot2, y);
+          const int op = t.op & ~VQ_COLCMP;
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const bool ok = isf ? cmp_f(op, bits_to_f64(raw[c][u]), bits_to_f64(y[u])) : cmp_i(op, raw[c][u], y[u]);
+            pass[u] = pass[u] && ok;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            bool ok = isf ? cmp_f(t.op, bits_to_f64(raw[c][u]), t.fval) : cmp_i(t.op, raw[c][u], t.ival);
+            pass[u] = pass[u] && ok;
+          }
+        }
+      }
+      if (s.key_stride) {
+#pragma unroll
+        for (int u = 0; u < U; u++) g[u] += s.key_stride * (int)raw[c][u];
+      }
+    }
   }
 }
 

@@ -340,6 +340,7 @@ void compute_zones(const vq_col_desc* cols, int32_t ncols, int32_t nrg,
 // zoneRangePred (planner/physical.go:667-750) on one conjunct.
 bool zone_term_keep(const ColDev& c, const vq_zone& z, const vq_term& t) {
   if (!z.has_minmax) return true;
+  if (t.op == VQ_IN || (t.op & VQ_COLCMP)) return true;  // (never prunes)
   if ((c.type == VQ_U8 || c.type == VQ_U16) && c.dict_size > 0) return true;
   int64_t v;
   if (c.type == VQ_F64) memcpy(&v, &t.fval, 8);
@@ -414,6 +415,25 @@ int check_col(const vq_table* t, int32_t c) {
   return VQ_OK;
 }
 
+// A term's operator and operands: a comparison with a literal; IN (a mask
+// over [0, 64)) on an integer-typed column; or a comparison of two columns
+// of the same kind (both float64, or both integer-typed).
+int check_term(const vq_table* t, const vq_term& tm) {
+  const bool isf = t->cols[tm.col].type == VQ_F64;
+  if (tm.op >= VQ_EQ && tm.op <= VQ_GE) return VQ_OK;
+  if (tm.op == VQ_IN) {
+    if (isf) return fail(VQ_E_UNSUPPORTED, "exec: gpu: IN on a float64 column");
+    return VQ_OK;
+  }
+  const int base = tm.op & ~VQ_COLCMP;
+  if (!(tm.op & VQ_COLCMP) || base < VQ_EQ || base > VQ_GE)
+    return fail(VQ_E_ARG, "exec: gpu: bad comparison op %d", tm.op);
+  if (tm.ival < 0 || tm.ival >= t->ncols) return fail(VQ_E_ARG, "exec: gpu: column %lld out of range", (long long)tm.ival);
+  if ((t->cols[tm.ival].type == VQ_F64) != isf)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: a comparison of a float64 column with an integer one");
+  return VQ_OK;
+}
+
 int add_pred(const vq_table* t, const vq_pred* pred, SlotMap& sm, QSlot* slots) {
   if (!pred) return VQ_OK;
   if (pred->nterms < 0 || pred->nterms > VQ_MAX_TERMS)
@@ -421,14 +441,20 @@ int add_pred(const vq_table* t, const vq_pred* pred, SlotMap& sm, QSlot* slots) 
   for (int k = 0; k < pred->nterms; k++) {
     const vq_term& tm = pred->terms[k];
     if (int rc = check_col(t, tm.col)) return rc;
-    if (tm.op < VQ_EQ || tm.op > VQ_GE) return fail(VQ_E_ARG, "exec: gpu: bad comparison op %d", tm.op);
+    if (int rc = check_term(t, tm)) return rc;
     int s = sm.get(tm.col);
     if (s < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+    int s2 = 0;
+    if (tm.op & VQ_COLCMP) {
+      s2 = sm.get((int32_t)tm.ival);
+      if (s2 < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+    }
     QSlot& sl = slots[s];
     if (sl.nterms == kMaxSlotTerms)
       return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d terms on one column", kMaxSlotTerms);
     QTerm& q = sl.terms[sl.nterms++];
     q.op = tm.op;
+    q.slot2 = s2;
     q.ival = tm.ival;
     q.fval = tm.fval;
   }
@@ -659,8 +685,10 @@ extern "C" int vq_zone_prune(const vq_table* t, const vq_pred* pred, uint8_t* ke
   if (!t || !keep) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
   if (pred) {
     if (pred->nterms < 0 || pred->nterms > VQ_MAX_TERMS) return fail(VQ_E_ARG, "exec: gpu: bad predicate");
-    for (int k = 0; k < pred->nterms; k++)
+    for (int k = 0; k < pred->nterms; k++) {
       if (int rc = check_col(t, pred->terms[k].col)) return rc;
+      if (int rc = check_term(t, pred->terms[k])) return rc;
+    }
   }
   std::vector<uint8_t> k;
   prune(t, pred, k);

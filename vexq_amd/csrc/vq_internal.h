@@ -30,8 +30,8 @@ enum AccOp : int32_t {
 };
 
 struct QTerm {
-  int32_t op;
-  int32_t pad_;
+  int32_t op;     // VQ_EQ..VQ_GE, VQ_IN (ival: mask), or op | VQ_COLCMP
+  int32_t slot2;  // VQ_COLCMP: the other column's slot
   int64_t ival;
   double fval;
 };

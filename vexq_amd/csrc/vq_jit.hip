@@ -60,7 +60,7 @@ void arr2(std::string& s, const char* name, const int (*v)[4]) {
 // Flattened predicate terms: (slot, op, ival, fval).
 struct Flat {
   int nt = 0;
-  int ts[8] = {0}, to[8] = {0};
+  int ts[8] = {0}, to[8] = {0}, ts2[8] = {0};
   int64_t ti[8] = {0};
   double tf[8] = {0};
 };
@@ -71,6 +71,7 @@ Flat flatten_terms(const QDesc& d) {
     for (int j = 0; j < d.slot[c].nterms && f.nt < 8; j++) {
       f.ts[f.nt] = c;
       f.to[f.nt] = d.slot[c].terms[j].op;
+      f.ts2[f.nt] = d.slot[c].terms[j].slot2;
       f.ti[f.nt] = d.slot[c].terms[j].ival;
       f.tf[f.nt] = d.slot[c].terms[j].fval;
       f.nt++;
@@ -117,6 +118,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
+  arr(s, "kTS2", f.ts2, 8);
   arr(s, "kKS", ks, 8);
   arr(s, "kAOP", aop, 8);
   arr(s, "kAISF", aisf, 8);
