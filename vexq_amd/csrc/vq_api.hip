@@ -845,7 +845,21 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card)) return rc;
   std::vector<uint8_t> keep;
   prune(t, pred, keep);
-  d.tile_rows = scan_tile_rows(kept_rows(t, keep));
+  // tiles: an even share of the kept rows for each wave the run-time
+  // compiled kernel keeps on the device (a step of 64 lanes x U rows never
+  // crosses a tile), at least 1024 rows
+  {
+    std::string derr;
+    const int64_t kr = kept_rows(t, keep);
+    const int slots = jit_dense_slots(d, &derr);
+    if (slots > 0 && kr >= (int64_t)1 << 20) {
+      const int64_t chunk = dense_chunk_rows(d.nslots);
+      const int64_t share = (kr + slots - 1) / slots;
+      d.tile_rows = (int32_t)std::max<int64_t>(1024, (share + chunk - 1) / chunk * chunk);
+    } else {
+      d.tile_rows = scan_tile_rows(kr);
+    }
+  }
   int64_t rows = 0, kept = 0;
   if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
 

@@ -161,6 +161,8 @@ size_t dense_lds_bytes(const QDesc& d);
 // Rows one partitioning workgroup (1024 threads) loads per step, for a side
 // of `nslots` columns; the device's compute units.
 int64_t part_chunk_rows(int nslots);
+// Rows one wave of the dense kernel loads per step.
+int64_t dense_chunk_rows(int nslots);
 int device_cus();
 hipError_t launch_agg_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s);
 hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part, uint64_t* out,
@@ -173,6 +175,8 @@ hipError_t launch_filter(const QDesc& d, bool jit, uint64_t* bits, uint32_t* til
 // Shape-specialized kernels compiled at run time (vq_jit.hip).  jit_dense_grid
 // returns the persistent grid, or -1 (with *err) when no kernel is available.
 int jit_dense_grid(const QDesc& d, std::string* err);
+// The waves the device holds at once for d's dense kernel (-1: no JIT).
+int jit_dense_slots(const QDesc& d, std::string* err);
 hipError_t jit_launch_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s);
 bool jit_available_filter(const QDesc& d, std::string* err);
 hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,

@@ -258,6 +258,7 @@ bool jit_disabled() {
 
 }  // namespace
 int64_t part_chunk_rows(int nslots) { return (int64_t)1024 * rows_in_flight(nslots); }
+int64_t dense_chunk_rows(int nslots) { return (int64_t)64 * rows_in_flight(nslots); }
 
 int device_cus() {
   // (initialised once, by whichever thread asks first)
@@ -378,7 +379,7 @@ hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_
   return launch(ll.fn, (unsigned)ll.grid, ll.lds, s, a, 512);
 }
 
-int jit_dense_grid(const QDesc& d, std::string* err) {
+int jit_dense_slots(const QDesc& d, std::string* err) {
   if (jit_disabled()) return fail_jit(err, "JIT disabled");
   const Flat f = flatten_terms(d);
   Module* m = get_module(prologue(d, f), err);
@@ -392,7 +393,12 @@ int jit_dense_grid(const QDesc& d, std::string* err) {
     const int v = atoi(e);
     if (v > 0) per_cu = v;
   }
-  const int64_t g = (int64_t)cus * per_cu;
+  return cus * per_cu;
+}
+
+int jit_dense_grid(const QDesc& d, std::string* err) {
+  const int g = jit_dense_slots(d, err);
+  if (g < 0) return g;
   return (int)(d.ntiles < g ? d.ntiles : g);
 }
 
