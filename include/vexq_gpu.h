@@ -91,8 +91,11 @@ typedef struct vq_col_desc {
 #define VQ_IN 6
 /* op | VQ_COLCMP compares two columns of the row, col OP column ival (a
  * BinOp compare of two column references, exec/expr.go:320-896, e.g.
- * l_commitdate < l_receiptdate): both float64, or both integer-typed
- * (codes, dates and int64 compared as int64).  Never prunes a row group. */
+ * l_commitdate < l_receiptdate): both float64, or both integer-typed (dates
+ * and int64 compared as int64).  A dictionary-coded (string) column on either
+ * side returns VQ_E_UNSUPPORTED: strings compare by value, which codes do not
+ * order, and two dictionaries need not share codes.  Never prunes a row
+ * group. */
 #define VQ_COLCMP 16
 
 #define VQ_MAX_TERMS 8
@@ -160,9 +163,11 @@ typedef struct vq_agg {
 } vq_agg;
 
 /* Aggregate output, one row per group.  The caller sizes the arrays for
- * `capacity` groups; ngroups is set even when it exceeds capacity (then the
- * call returns VQ_E_ARG and the caller retries with more room — the two-call
- * size query).  Groups come out in first-occurrence order (the order the
+ * `capacity` groups.  When the result needs more rows the call returns
+ * VQ_E_ARG with ngroups set to the rows needed and nothing written to the
+ * arrays (the two-call size query: retry with that capacity); on any other
+ * failure ngroups is 0, so a reused result buffer never shows a failed
+ * call's rows.  Groups come out in first-occurrence order (the order the
  * serial HashAggregate inserts keys, exec/aggregate.go:83-129), or for a
  * top-k join in rank order.  Per aggregate j of group g:
  *   ivals[g*naggs+j] for VQ_I64 accumulators, fvals[g*naggs+j] for VQ_F64,

@@ -63,6 +63,7 @@ typedef struct sacc {
   int64_t d[SA_DIGITS];
   int64_t nadds;
   int32_t nan, pinf, ninf;
+  uint64_t nan_bits; /* the first NaN added (an IEEE sum carries its payload) */
 } sacc;
 
 static void sa_norm(sacc* a) {
@@ -81,8 +82,10 @@ static void sa_add(sacc* a, double x) {
   int ex = (int)((bits >> 52) & 0x7FF);
   uint64_t frac = bits & ((1ULL << 52) - 1);
   if (ex == 0x7FF) {
-    if (frac) a->nan = 1;
-    else if (sign) a->ninf = 1;
+    if (frac) {
+      if (!a->nan) a->nan_bits = bits | (1ULL << 51); /* quieted */
+      a->nan = 1;
+    } else if (sign) a->ninf = 1;
     else a->pinf = 1;
     return;
   }
@@ -120,7 +123,12 @@ static int sa_bit(const int64_t* d, int64_t pos) {
 
 /* Correctly rounded (nearest, ties to even) value of the accumulator. */
 static double sa_round(const sacc* src) {
-  if (src->nan || (src->pinf && src->ninf)) return NAN;
+  if (src->nan) {
+    double d;
+    memcpy(&d, &src->nan_bits, 8);
+    return d;
+  }
+  if (src->pinf && src->ninf) return NAN;
   if (src->pinf) return INFINITY;
   if (src->ninf) return -INFINITY;
   sacc a = *src;
@@ -244,13 +252,21 @@ static inline int pred_pass(const vq_col_desc* cols, const vq_pred* p, int64_t r
   return 1;
 }
 
-/* applyArithFloat64 (exec/expr.go:1005-1020): x/0 is NaN. */
+/* Go's math.NaN(): the bits of uvnan (0x7FF8000000000001). */
+static inline double go_nan(void) {
+  const uint64_t b = 0x7FF8000000000001ULL;
+  double d;
+  memcpy(&d, &b, 8);
+  return d;
+}
+
+/* applyArithFloat64 (exec/expr.go:1005-1020): x/0 is math.NaN(). */
 static inline double arith_f(int op, double a, double b) {
   switch (op) {
     case VQ_ADD: return a + b;
     case VQ_SUB: return a - b;
     case VQ_MUL: return a * b;
-    default: return b == 0 ? NAN : a / b;
+    default: return b == 0 ? go_nan() : a / b;
   }
 }
 

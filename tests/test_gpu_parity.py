@@ -43,6 +43,10 @@ def assert_agg_equal(gpu, ref, aggs, rel=REL_TOL):
             v = gpu.valid[:, j].astype(bool)
             np.testing.assert_allclose(gpu.fvals[v, j], ref.fvals[v, j], rtol=rel, atol=0,
                                        equal_nan=True, err_msg=f"aggregate {j}")
+            # a NaN result carries the bits of Go's math.NaN() (x/0)
+            nan = v & np.isnan(ref.fvals[:, j])
+            np.testing.assert_array_equal(gpu.fvals[nan, j].view(np.uint64),
+                                          ref.fvals[nan, j].view(np.uint64))
         else:
             np.testing.assert_array_equal(gpu.ivals[:, j], ref.ivals[:, j])
 
@@ -268,6 +272,65 @@ def test_scan_sum_pruned(gpu_ctx, kernel_path):
 
 
 # ---------------------------------------------------------------------------
+# Row groups of any size: spans (and tiles) that start at unaligned rows
+# ---------------------------------------------------------------------------
+def _odd_layout(kind, n):
+    """Row-group sizes summing to n: 1003-row groups, or 65536-row groups
+    with short odd ones between them."""
+    if kind == "1003":
+        sizes = [1003] * (n // 1003)
+    else:
+        sizes, i = [], 0
+        pattern = [65536, 1003, 1003, 777, 65536, 1, 63, 65, 4097]
+        while sum(sizes) + pattern[i % len(pattern)] <= n:
+            sizes.append(pattern[i % len(pattern)])
+            i += 1
+    rest = n - sum(sizes)
+    if rest:
+        sizes.append(rest)
+    return np.array(sizes, dtype=np.int64)
+
+
+@pytest.mark.parametrize("layout", ["1003", "mixed"])
+def test_odd_row_groups_pruned(gpu_ctx, layout, kernel_path):
+    """Zone pruning over row groups of odd sizes keeps spans that start at
+    rows that are not multiples of the kernels' row blocks: the dictionary
+    GROUP BY, the pruned scan-sum and the filter still fold every kept row
+    exactly once."""
+    n = 600_000
+    t = datagen.lineitem_sorted(n, seed=13)
+    t = datagen.Table(t.columns, _odd_layout(layout, n))
+    X = abi.X
+    sd = t.col("shipdate")
+    lo, hi = int(sd[n // 5]) + 1, int(sd[n // 2])
+    window = [("shipdate", ">=", lo), ("shipdate", "<", hi)]
+    _, keys, aggs = datagen.q_groupby(t)
+    pred = abi.make_pred(t, window)
+    dt = gpu_ctx.upload(t)
+    try:
+        keep = dt.prune(pred)
+        assert 0 < keep.sum() < len(keep)
+        _print_case("odd_rg_groupby", layout=layout, path=kernel_path, kept=int(keep.sum()))
+        got = gpu_ctx.agg_dict(dt, pred, keys, aggs)
+        assert_agg_equal(got, O.agg_dict(t, pred, keys, aggs, exact=True), aggs)
+        _print_case("odd_rg_scan_sum", layout=layout, path=kernel_path)
+        sp = abi.make_pred(t, window + [("quantity", "<", 24)])
+        sa = [abi.agg_sum(X.col("price") * X.col("discount"), t), abi.count_star(t)]
+        got = gpu_ctx.agg_dict(dt, sp, [], sa)
+        assert_agg_equal(got, O.agg_dict(t, sp, [], sa, exact=True), sa)
+        _print_case("odd_rg_filter", layout=layout, path=kernel_path)
+        np.testing.assert_array_equal(gpu_ctx.filter(dt, sp), O.filter_rows(t, sp))
+        np.testing.assert_array_equal(gpu_ctx.filter(dt, None), O.filter_rows(t, None))
+        if kernel_path == "jit":
+            _print_case("odd_rg_agg_hash", layout=layout)
+            ha = [abi.count_star(t), abi.agg_sum("price", t)]
+            got = gpu_ctx.agg_hash(dt, pred, t.index("quantity"), ha)
+            assert_agg_equal(got, O.agg_hash(t, pred, t.index("quantity"), ha, exact=True), ha)
+    finally:
+        dt.close()
+
+
+# ---------------------------------------------------------------------------
 # Radix-partitioned join + aggregate
 # ---------------------------------------------------------------------------
 def _join_tables(nb, npr, seed=42):
@@ -294,6 +357,28 @@ def test_join_full_groups(gpu_ctx, bits):
     ref = O.join_agg(b, 0, bp, p, 0, pp, aggs, exact=True)
     assert_agg_equal(got, ref, aggs)
     assert same_bits(got, again)
+
+
+def test_join_odd_probe_row_groups(gpu_ctx):
+    """A probe side sorted by shipdate in odd-sized row groups: its predicate
+    prunes the leading groups, so the kept span starts at a row that is not a
+    multiple of 64 (no live-bit words for the scatter)."""
+    b, p = _join_tables(100_000, 400_000, seed=9)
+    p = p.take(np.argsort(p.col("shipdate"), kind="stable"))
+    p = datagen.Table(p.columns, _odd_layout("mixed", p.nrows))
+    bp, pp, aggs = datagen.q_join(b, p)
+    db, dp = gpu_ctx.upload(b), gpu_ctx.upload(p)
+    try:
+        keep = dp.prune(pp)
+        assert 0 < keep.sum() < len(keep)
+        for bits, topk in [(-1, 0), (3, 10)]:
+            _print_case("join_odd_rg", bits=bits, topk=topk)
+            got = gpu_ctx.join_agg(db, 0, bp, dp, 0, pp, aggs, radix_bits=bits, topk=topk)
+            ref = O.join_agg(b, 0, bp, p, 0, pp, aggs, topk=topk, exact=True)
+            assert_agg_equal(got, ref, aggs)
+    finally:
+        db.close()
+        dp.close()
 
 
 def test_join_top10_two_pass_partitioning(gpu_ctx):

@@ -125,6 +125,38 @@ def test_exact_sum_specials():
     assert math.isnan(O.exact_sum([float("inf"), float("-inf")]))
 
 
+GO_NAN_BITS = 0x7FF8000000000001  # Go's math.NaN() (uvnan)
+
+
+def test_float_division_by_zero_is_go_nan():
+    """x/0 in a float expression is math.NaN() (exec/expr.go:1016-1018):
+    the bits 0x7FF8000000000001.  A SUM or AVG over such a row carries those
+    bits (an IEEE sum returns its NaN operand), in both summation modes; a
+    group with no zero divisor keeps a number."""
+    t = datagen.lineitem(20_000, seed=5)
+    X = abi.X
+    aggs = [abi.agg_sum(X.col("price") / X.col("discount"), t),
+            abi.agg_avg(X.col("price") / X.col("discount"), t)]
+    zero_b = np.isin(np.arange(2), t.col("flagB")[t.col("discount") == 0])
+    assert zero_b.any()
+    for exact in (True, False):
+        r = O.agg_dict(t, None, [t.index("flagB")], aggs, exact=exact)
+        for i in range(r.ngroups):
+            b = int(r.keys[i, 0])
+            for j in range(2):
+                bits = int(r.fvals[i:i + 1, j].view(np.uint64)[0])
+                if zero_b[b]:
+                    assert bits == GO_NAN_BITS, hex(bits)
+                else:
+                    assert not math.isnan(r.fvals[i, j])
+    # a group whose divisors are all nonzero
+    p = abi.make_pred(t, [("discount", ">", 0.0)])
+    r = O.agg_dict(t, p, [], aggs, exact=True)
+    assert not np.isnan(r.fvals[0]).any()
+    nz = t.col("discount") > 0
+    assert r.fvals[0, 0] == math.fsum(t.col("price")[nz] / t.col("discount")[nz])
+
+
 # ---- zone maps and pruning ---------------------------------------------------
 
 def test_zone_pruning_kat():

@@ -61,6 +61,9 @@ typedef unsigned short u16;
 typedef unsigned char u8;
 constexpr i64 kI64Min = -0x7fffffffffffffffLL - 1;
 constexpr i64 kI64Max = 0x7fffffffffffffffLL;
+// The bits of Go's math.NaN() (uvnan), which applyArithFloat64 returns for
+// x/0 (exec/expr.go:1016-1018).
+constexpr i64 kGoNaNBits = 0x7FF8000000000001LL;
 
 #define VQ_U8 1
 #define VQ_U16 2
@@ -166,7 +169,7 @@ __device__ __forceinline__ double arf(double a, double b) {
   if constexpr (OP == 0) return a + b;
   else if constexpr (OP == 1) return a - b;
   else if constexpr (OP == 2) return a * b;
-  else return b == 0.0 ? __longlong_as_double(0x7ff8000000000000LL) : a / b;
+  else return b == 0.0 ? __longlong_as_double(kGoNaNBits) : a / b;
 }
 
 template <int OP>
@@ -203,10 +206,12 @@ __device__ __forceinline__ void load_rows(const Args& a, i64 base, i64 rend, int
 
 // Row-blocked loads for the dense GROUP BY: lane l takes the U consecutive
 // rows base + l*U .. base + l*U + U-1, read with 4- to 16-byte vector loads
-// (a wave still reads one contiguous span of each column).  r0 is clamped to
-// the last U-aligned start before rend, so a lane past the end re-reads rows
-// inside the column (plus at most U-1 rows of the upload's tail padding); the
-// caller masks rows >= rend.
+// (a wave still reads one contiguous span of each column).  base is a
+// multiple of U (tile_begin<U>), so every lane's r0 is too and the loads are
+// aligned.  r0 is clamped to the last U-aligned start before rend: a lane
+// whose rows all lie at or past rend re-reads rows inside the column (plus at
+// most U-1 rows of the upload's tail padding) instead; the caller masks rows
+// outside [lo, rend), so a clamped lane contributes nothing.
 template <int T, int U>
 __device__ __forceinline__ void ld_block(const void* p, i64 r0, i64* out) {
   if constexpr (T == VQ_U8) {
@@ -397,26 +402,33 @@ __device__ __forceinline__ void fold_lds(u64* w, const Args& a, const i64* rawro
 }
 
 // ---- tile iteration (persistent grid) ---------------------------------------
+// Tile t holds rows [lo, rend).  A row group can hold any number of rows, so
+// a span (and each of its tiles) can start at any row; base, where the
+// tile's chunks start, is lo rounded down to a multiple of A, so that
+// row-blocked vector loads stay aligned.  Rows below lo (the previous
+// tile's) and at or past rend are masked by the caller.
 struct TileIt {
-  i64 t, base, rend;
+  i64 t, lo, base, rend;
   bool have;
 };
 
+template <int A>
 __device__ __forceinline__ void tile_begin(const Args& a, i64 t, TileIt& it) {
   it.t = t;
   it.have = t < a.ntiles;
   if (!it.have) return;
   int s = 0;
   while (s + 1 < a.nspans && t >= a.sp_tile0[s + 1]) s++;
-  it.base = a.sp_start[s] + (t - a.sp_tile0[s]) * (i64)a.tile_rows;
+  it.lo = a.sp_start[s] + (t - a.sp_tile0[s]) * (i64)a.tile_rows;
+  it.base = it.lo - (it.lo % A);
   const i64 send = a.sp_start[s] + a.sp_rows[s];
-  it.rend = it.base + a.tile_rows < send ? it.base + a.tile_rows : send;
+  it.rend = it.lo + a.tile_rows < send ? it.lo + a.tile_rows : send;
 }
 
 template <int U>
 __device__ __forceinline__ void tile_advance(const Args& a, TileIt& it) {
   it.base += kWave * U;
-  if (it.base >= it.rend) tile_begin(a, it.t + gridDim.x, it);
+  if (it.base >= it.rend) tile_begin<U>(a, it.t + gridDim.x, it);
 }
 
 __device__ __forceinline__ u64 combine(int op, u64 a, u64 b) {
@@ -546,7 +558,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
     }
   }
   TileIt it;
-  tile_begin(a, blockIdx.x, it);
+  tile_begin<U>(a, blockIdx.x, it);
   i64 nraw[VQ_NS][U];
   if (it.have) load_rows_blocked<U>(a, it.base, it.rend, lane, nraw);
   while (it.have) {
@@ -555,7 +567,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
     for (int c = 0; c < VQ_NS; c++)
 #pragma unroll
       for (int u = 0; u < U; u++) raw[c][u] = nraw[c][u];
-    const i64 base = it.base, rend = it.rend;
+    const i64 base = it.base, lo = it.lo, rend = it.rend;
     tile_advance<U>(a, it);
     if (it.have) load_rows_blocked<U>(a, it.base, it.rend, lane, nraw);
 #pragma unroll
@@ -564,7 +576,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
 #pragma unroll
       for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
       const i64 r = base + (i64)lane * U + u;
-      if (!(r < rend && pass_all(a, row, seq<VQ_NT>{}))) continue;
+      if (!(r >= lo && r < rend && pass_all(a, row, seq<VQ_NT>{}))) continue;
       if constexpr (G == 1) {
         rfirst = rcnt == 0 ? (u32)r : rfirst;
         rcnt++;
@@ -625,10 +637,13 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
   constexpr int U = VQ_U;
   const int lane = threadIdx.x;
   TileIt it;
-  tile_begin(a, blockIdx.x, it);
+  tile_begin<1>(a, blockIdx.x, it);
   if (!it.have) return;
   u32 count = 0;
-  for (i64 base = it.base; base < it.rend; base += kWave * U) {
+  // word k of tile t (rows lo + 64k ..) sits at t * tile_rows / 64 + k, so
+  // tiles starting at any row never share a word
+  u64* words = a.bits + it.t * (i64)(a.tile_rows >> 6);
+  for (i64 base = it.lo; base < it.rend; base += kWave * U) {
     i64 raw[VQ_NS][U];
     load_rows<U>(a, base, it.rend, lane, raw);
     u64 mine = 0;
@@ -642,7 +657,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
       count += (u32)__popcll(m);
       if (lane == u) mine = m;
     }
-    if (lane < U && base + lane * kWave < it.rend) a.bits[(base >> 6) + lane] = mine;
+    if (lane < U && base + lane * kWave < it.rend) words[((base - it.lo) >> 6) + lane] = mine;
   }
   if (lane == 0) a.tile_counts[it.t] = count;
 }

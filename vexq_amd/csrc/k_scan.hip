@@ -78,7 +78,7 @@ __device__ __forceinline__ double arith_f(int op, double a, double b) {
     case VQ_ADD: return a + b;
     case VQ_SUB: return a - b;
     case VQ_MUL: return a * b;
-    default: return b == 0.0 ? __longlong_as_double(0x7ff8000000000000LL) : a / b;
+    default: return b == 0.0 ? __longlong_as_double(kGoNaNBits) : a / b;
   }
 }
 
@@ -572,8 +572,10 @@ __global__ __launch_bounds__(256) void k_agg_final(const QDesc d, int32_t nparts
 // Selection vector: (1) per tile, evaluate the predicate and store the pass
 // bits (one ballot word per 64 rows) and the tile's count; (2) exclusive scan
 // of tile counts; (3) per tile, expand the words into ascending row ids with
-// coalesced writes.  Tiles start on 64-row boundaries (row groups are 65,536
-// rows, tiles a multiple of 64), so a word never straddles two tiles.
+// coalesced writes.  A tile can start at any row (row groups hold any number
+// of rows); its words sit at tile_index * tile_rows / 64 onward (tile_rows is
+// a multiple of 64), word k holding rows lo + 64k .. lo + 64k + 63, so no
+// word is shared by two tiles.
 // ---------------------------------------------------------------------------
 template <int U>
 __global__ __launch_bounds__(kWave) void k_filter_bits(const QDesc d, uint64_t* __restrict__ bits,
@@ -585,8 +587,11 @@ __global__ __launch_bounds__(kWave) void k_filter_bits(const QDesc d, uint64_t* 
   for (int c = 0; c < kMaxSlots; c++) ld[c] = slot_ld(d.slot[c < NS ? c : 0]);
   TileIt it;
   tile_begin(d, blockIdx.x, it);
-  const int64_t t0 = it.t;
+  const int64_t t0 = it.t, lo = it.base;
   uint32_t count = 0;
+  // word k of tile t (rows lo + 64k ..) at t * tile_rows / 64 + k: tiles
+  // starting at any row never share a word
+  uint64_t* words = bits + t0 * (int64_t)(d.tile_rows >> 6);
   // this kernel runs one tile per wave (grid = ntiles)
   while (it.have && it.t == t0) {
     int64_t raw[kMaxSlots][U];
@@ -601,7 +606,7 @@ __global__ __launch_bounds__(kWave) void k_filter_bits(const QDesc d, uint64_t* 
       count += (uint32_t)__popcll(m);
       if (lane == u) mine = m;
     }
-    if (lane < U && it.base + lane * kWave < it.rend) bits[(it.base >> 6) + lane] = mine;
+    if (lane < U && it.base + lane * kWave < it.rend) words[((it.base - lo) >> 6) + lane] = mine;
     it.base += kWave * U;
     if (it.base >= it.rend) it.have = false;
   }
@@ -643,8 +648,9 @@ __global__ __launch_bounds__(kWave) void k_filter_write(const QDesc d, const uin
   if (!it.have) return;
   uint32_t pos = offsets[it.t];
   const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  const uint64_t* words = bits + it.t * (int64_t)(d.tile_rows >> 6);
   for (int64_t wb = it.base; wb < it.rend; wb += kWave) {
-    const uint64_t m = bits[wb >> 6];
+    const uint64_t m = words[(wb - it.base) >> 6];
     if ((m >> lane) & 1ULL) out_rows[pos + (uint32_t)__popcll(m & below)] = (uint32_t)(wb + lane);
     pos += (uint32_t)__popcll(m);
   }

@@ -431,6 +431,10 @@ int check_term(const vq_table* t, const vq_term& tm) {
   if (tm.ival < 0 || tm.ival >= t->ncols) return fail(VQ_E_ARG, "exec: gpu: column %lld out of range", (long long)tm.ival);
   if ((t->cols[tm.ival].type == VQ_F64) != isf)
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: a comparison of a float64 column with an integer one");
+  // two dictionary columns compare as strings (exec/expr.go:320-896), which
+  // their codes do not order, and two dictionaries need not agree on a code
+  if (t->cols[tm.col].dict_size > 0 || t->cols[tm.ival].dict_size > 0)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: a column comparison of dictionary-coded (string) columns");
   return VQ_OK;
 }
 
@@ -719,6 +723,10 @@ extern "C" int64_t vq_join_table_capacity(int64_t keys) {
 namespace {
 
 constexpr size_t kDenseLdsMax = 64 * 1024;
+// Dense group slots: one slot when there is no GROUP BY; with one, every
+// slot holds at least a count and a first row per lane (8 bytes x 64 lanes)
+// inside kDenseLdsMax, so at most 128.
+constexpr int kDenseGroupsMax = 128;
 
 // Tile rows for a scan of `rows` rows: a power of two between 256 and 16384
 // that cuts the scan into about 2048 tiles, so small (zone-pruned) scans still
@@ -826,8 +834,82 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
     d.slot[s].type = t->cols[0].type;
     d.nslots = 1;
   }
-  if (G > 1 && dense_lds_bytes(d) > kDenseLdsMax)
+  if ((G > 1 && dense_lds_bytes(d) > kDenseLdsMax) || G > kDenseGroupsMax)
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld groups x %d accumulators exceed LDS", (long long)G, d.nacc);
+  return VQ_OK;
+}
+
+// The final words of a dense GROUP BY ([G][nacc + 2]: accumulators, count,
+// first row) -> the caller's result: groups in first-occurrence order (the
+// serial HashAggregate's key order), AVG divided on the host, float MIN/MAX
+// mapped back from their order-preserving integers, the empty global
+// aggregate as buildEmptyGlobalResult makes it.
+int dense_result(const uint64_t* hfin, const QDesc& d, const AccMap& am, const int32_t card[4],
+                 int32_t nkeys, int32_t naggs, const vq_agg* aggs, vq_agg_result* out) {
+  const int NI = d.nacc + 2;
+  // groups in first-occurrence order (the serial HashAggregate's key order)
+  // (at most kDenseGroupsMax groups: no heap allocation here)
+  std::pair<uint64_t, int> order[kDenseGroupsMax];
+  int64_t norder = 0;
+  for (int g = 0; g < d.ngroups; g++) {
+    const uint64_t cnt = hfin[g * NI + d.nacc];
+    if (cnt) order[norder++] = {hfin[g * NI + d.nacc + 1], g};
+  }
+  std::sort(order, order + norder);
+  const bool empty_global = norder == 0 && nkeys == 0;
+  const int64_t ng = empty_global ? 1 : norder;
+  if (ng > out->capacity) {
+    out->ngroups = ng;  // the size query: groups needed, the arrays hold no result
+    return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)ng,
+                (long long)out->capacity);
+  }
+  out->ngroups = ng;
+  if (empty_global) {  // buildEmptyGlobalResult (exec/aggregate.go:961-982)
+    out->counts[0] = 0;
+    out->first_row[0] = -1;
+    for (int j = 0; j < naggs; j++) {
+      out->ivals[j] = 0;
+      out->fvals[j] = 0;
+      out->valid[j] = aggs[j].kind == VQ_AGG_COUNT;
+    }
+    return VQ_OK;
+  }
+  for (int64_t i = 0; i < ng; i++) {
+    const int g = order[i].second;
+    const uint64_t* r = &hfin[g * NI];
+    const int64_t cnt = (int64_t)r[d.nacc];
+    int64_t rem = g;
+    for (int k = nkeys - 1; k >= 0; k--) {
+      out->keys[i * nkeys + k] = rem % card[k];
+      rem /= card[k];
+    }
+    out->counts[i] = cnt;
+    out->first_row[i] = (int64_t)r[d.nacc + 1];
+    for (int j = 0; j < naggs; j++) {
+      const int64_t o = i * naggs + j;
+      out->ivals[o] = 0;
+      out->fvals[o] = 0;
+      out->valid[o] = 1;
+      const vq_agg& a = aggs[j];
+      if (a.kind == VQ_AGG_COUNT) {
+        out->ivals[o] = cnt;
+        continue;
+      }
+      uint64_t w = r[am.acc[j]];
+      if ((a.kind == VQ_AGG_MIN || a.kind == VQ_AGG_MAX) && a.accum == VQ_F64) {
+        const int64_t o = (int64_t)w;  // order-preserving int64 -> IEEE bits
+        w = (uint64_t)(o >= 0 ? o : (int64_t)((uint64_t)o ^ 0x7FFFFFFFFFFFFFFFULL));
+      }
+      if (a.kind == VQ_AGG_AVG) {
+        const double s = a.arg.is_float ? bits_to_f64((int64_t)w) : (double)(int64_t)w;
+        out->fvals[o] = s / (double)cnt;
+      } else if (a.accum == VQ_F64) {
+        out->fvals[o] = bits_to_f64((int64_t)w);
+      } else {
+        out->ivals[o] = (int64_t)w;
+      }
+    }
+  }
   return VQ_OK;
 }
 
@@ -837,6 +919,7 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
                            const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                            vq_agg_result* out, vq_stats* stats) {
   if (!c || !t || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  out->ngroups = 0;  // no result unless this call succeeds
   if (t->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
   QDesc d;
   AccMap am;
@@ -910,66 +993,7 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
     stats->tile_rows = d.tile_rows;
   }
 
-  // groups in first-occurrence order (the serial HashAggregate's key order)
-  std::vector<std::pair<uint64_t, int>> order;
-  for (int g = 0; g < d.ngroups; g++) {
-    const uint64_t cnt = hfin[g * NI + d.nacc];
-    if (cnt) order.push_back({hfin[g * NI + d.nacc + 1], g});
-  }
-  std::sort(order.begin(), order.end());
-  const bool empty_global = order.empty() && nkeys == 0;
-  const int64_t ng = empty_global ? 1 : (int64_t)order.size();
-  out->ngroups = ng;
-  if (ng > out->capacity)
-    return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)ng,
-                (long long)out->capacity);
-  if (empty_global) {  // buildEmptyGlobalResult (exec/aggregate.go:961-982)
-    out->counts[0] = 0;
-    out->first_row[0] = -1;
-    for (int j = 0; j < naggs; j++) {
-      out->ivals[j] = 0;
-      out->fvals[j] = 0;
-      out->valid[j] = aggs[j].kind == VQ_AGG_COUNT;
-    }
-    return VQ_OK;
-  }
-  for (int64_t i = 0; i < ng; i++) {
-    const int g = order[i].second;
-    const uint64_t* r = &hfin[g * NI];
-    const int64_t cnt = (int64_t)r[d.nacc];
-    int64_t rem = g;
-    for (int k = nkeys - 1; k >= 0; k--) {
-      out->keys[i * nkeys + k] = rem % card[k];
-      rem /= card[k];
-    }
-    out->counts[i] = cnt;
-    out->first_row[i] = (int64_t)r[d.nacc + 1];
-    for (int j = 0; j < naggs; j++) {
-      const int64_t o = i * naggs + j;
-      out->ivals[o] = 0;
-      out->fvals[o] = 0;
-      out->valid[o] = 1;
-      const vq_agg& a = aggs[j];
-      if (a.kind == VQ_AGG_COUNT) {
-        out->ivals[o] = cnt;
-        continue;
-      }
-      uint64_t w = r[am.acc[j]];
-      if ((a.kind == VQ_AGG_MIN || a.kind == VQ_AGG_MAX) && a.accum == VQ_F64) {
-        const int64_t o = (int64_t)w;  // order-preserving int64 -> IEEE bits
-        w = (uint64_t)(o >= 0 ? o : (int64_t)((uint64_t)o ^ 0x7FFFFFFFFFFFFFFFULL));
-      }
-      if (a.kind == VQ_AGG_AVG) {
-        const double s = a.arg.is_float ? bits_to_f64((int64_t)w) : (double)(int64_t)w;
-        out->fvals[o] = s / (double)cnt;
-      } else if (a.accum == VQ_F64) {
-        out->fvals[o] = bits_to_f64((int64_t)w);
-      } else {
-        out->ivals[o] = (int64_t)w;
-      }
-    }
-  }
-  return VQ_OK;
+  return dense_result(hfin, d, am, card, nkeys, naggs, aggs, out);
 }
 
 // ===========================================================================
@@ -1001,8 +1025,9 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
 
   SlotLease S(c);
   hipSetDevice(c->device);
-  const size_t nwords = (size_t)(t->nrows / 64 + 2);
   const size_t nt = (size_t)std::max<int64_t>(d.ntiles, 1);
+  // pass words: tile_rows / 64 per tile (a tile's words start at its index)
+  const size_t nwords = nt * (size_t)(d.tile_rows / 64) + 2;
   void* ar;
   if (int rc = arena_get(c, S.get(), nwords * 8 + nt * 8 + 64 + 2048, &ar)) return rc;
   Carve cv;
@@ -1812,10 +1837,12 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     if ((int64_t)nord > j.topk) nord = (size_t)j.topk;
   }
   const int64_t nout = (int64_t)nord;
-  out->ngroups = nout;
-  if (nout > out->capacity)
+  if (nout > out->capacity) {
+    out->ngroups = nout;  // the size query: groups needed, the arrays hold no result
     return fail(VQ_E_ARG, "exec: gpu: %lld groups exceed result capacity %lld", (long long)nout,
                 (long long)out->capacity);
+  }
+  out->ngroups = nout;
   bool isf[VQ_MAX_AGGS];
   for (int k = 0; k < naggs; k++) {
     const vq_agg& a = j.aggs[k];
@@ -1889,7 +1916,14 @@ int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
   vq_ctx* c = j.c;
   SlotLease S(c);
   hipSetDevice(c->device);
+  // Each retry doubles the accumulator pool (up to its LDS bound) or adds a
+  // radix bit (up to 16), so a query converges within kMaxAttempts plans; a
+  // query that has not is reported instead of planning ever more partitions.
+  constexpr int kMaxAttempts = 40;
   for (;;) {
+    if (j.attempts >= kMaxAttempts)
+      return fail(VQ_E_DEVICE, "exec: gpu: join/GROUP BY plan did not converge after %d attempts (bits %d)",
+                  j.attempts, j.bits);
     if (int rc = join_plan(j)) return rc;
     j.attempts++;
     void* ar;
@@ -1910,6 +1944,7 @@ int join_impl(vq_ctx* c, const vq_table* build, int32_t build_key, const vq_pred
               const vq_agg* aggs, int32_t radix_bits, int32_t topk, vq_agg_result* out,
               vq_stats* stats, bool groupby, const JoinTune* tune = nullptr) {
   if (!out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  out->ngroups = 0;  // no result unless this call succeeds
   if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs))
     return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
   std::unique_ptr<JoinJob> j(new (std::nothrow) JoinJob());
@@ -2267,6 +2302,7 @@ JoinTune autotune(const JoinJob& proto, vq_agg_result* out) {
 extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_result* outs,
                             vq_stats* stats, int32_t flags) {
   if (!c || n < 0 || (n > 0 && (!qs || !outs))) return fail(VQ_E_ARG, "exec: gpu: bad batch arguments");
+  for (int32_t i = 0; i < n; i++) outs[i].ngroups = 0;  // no result unless its instance succeeds
   hipSetDevice(c->device);
   std::vector<std::unique_ptr<JoinJob>> jobs(n);
   std::vector<char> direct(n, 0);  // dictionary GROUP BYs run one by one

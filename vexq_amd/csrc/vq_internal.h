@@ -18,6 +18,9 @@ constexpr int kMaxSlots = 8;      // distinct columns one query touches
 constexpr int kMaxSlotTerms = 4;  // predicate terms on one column
 constexpr int kMaxAcc = 8;        // accumulators after de-duplication
 constexpr int kMaxSpans = 24;     // contiguous runs of surviving row groups
+// The bits of Go's math.NaN() (uvnan), which applyArithFloat64 returns for
+// x/0 (exec/expr.go:1016-1018).
+constexpr long long kGoNaNBits = 0x7FF8000000000001LL;
 
 // Accumulator operations of the dense (dictionary-code) path.
 enum AccOp : int32_t {

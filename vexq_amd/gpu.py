@@ -18,9 +18,11 @@ _lib = None
 
 
 class GpuError(RuntimeError):
-    def __init__(self, code: int, msg: str):
+    def __init__(self, code: int, msg: str, needed: int = 0):
         super().__init__(f"{abi.STATUS_NAMES.get(code, code)}: {msg}")
         self.code = code
+        # result rows a call needed when it failed for want of capacity
+        self.needed = needed
 
 
 def lib():
@@ -74,9 +76,16 @@ def lib():
     return L
 
 
-def _check(rc: int):
+def _check(rc: int, out: "abi.AggOut | None" = None):
+    """Raises GpuError on a failed call.  With `out`, the result buffer the
+    call was given: on failure it is left empty (ngroups 0) and the rows a
+    capacity failure needed are carried in the error's `needed`."""
     if rc != abi.VQ_OK:
-        raise GpuError(rc, lib().vq_last_error().decode())
+        needed = 0
+        if out is not None:
+            needed = int(out.c.ngroups)
+            out.c.ngroups = 0
+        raise GpuError(rc, lib().vq_last_error().decode(), needed)
 
 
 def _pp(x):
@@ -253,7 +262,7 @@ class Context:
         out = abi.AggOut(capacity, len(key_cols), len(aggs))
         st = abi.Stats()
         _check(lib().vq_agg_dict(self.h, dt.h, _pp(pred), len(key_cols), keys, len(aggs), arr,
-                                 C.byref(out.c), C.byref(st)))
+                                 C.byref(out.c), C.byref(st)), out)
         out.stats = st
         return out
 
@@ -271,7 +280,7 @@ class Context:
             raise ValueError("agg_hash: result buffer shaped for another query")
         st = abi.Stats()
         _check(lib().vq_agg_hash(self.h, dt.h, _pp(pred), key_col, len(aggs), arr,
-                                 C.byref(out.c), C.byref(st)))
+                                 C.byref(out.c), C.byref(st)), out)
         out.stats = st
         return out
 
@@ -285,7 +294,7 @@ class Context:
         st = abi.Stats()
         _check(lib().vq_join_agg(self.h, build.h, build_key, _pp(build_pred), probe.h, probe_key,
                                  _pp(probe_pred), len(aggs), arr, radix_bits, topk,
-                                 C.byref(out.c), C.byref(st)))
+                                 C.byref(out.c), C.byref(st)), out)
         out.stats = st
         return out
 
@@ -302,8 +311,19 @@ class Ticket:
     def wait(self) -> abi.AggOut:
         if self.h is not None:
             h, self.h = self.h, None
-            _check(lib().vq_wait(h))
+            _check(lib().vq_wait(h), self.out)
         return self.out
+
+    def __del__(self):
+        # The library writes into this ticket's result arrays and stats until
+        # the query ends: a ticket dropped unwaited waits here, so they are
+        # not freed under it, and its vq_ticket is released.
+        if getattr(self, "h", None) is not None:
+            try:
+                h, self.h = self.h, None
+                lib().vq_wait(h)
+            except Exception:
+                pass
 
 
 class DeviceTable:
