@@ -265,6 +265,22 @@ int vq_agg_dict(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t nke
                 const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                 vq_agg_result* out, vq_stats* stats);
 
+/* The same dictionary-code GROUP BY with no device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, semantics and result as vq_agg_dict;
+ * it runs the precompiled kernels only (no run-time compilation) and keeps
+ * its partials and final words in a fixed 2 MiB device array that the library
+ * holds on each device for the life of the process (no arena, no pinned
+ * buffer; fewer waves when their partials would not fit).  A shape whose
+ * final words alone exceed that array returns VQ_E_UNSUPPORTED.  Every call
+ * on one device shares that device's array: calls from this library take
+ * turns on it (a host lock held until the result is on the host), and work
+ * that overlaps it on the same device in any other way races. */
+int vq_agg_dict_noalloc(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t nkeys,
+                        const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
+                        vq_agg_result* out, vq_stats* stats);
+
 /* ---- radix-partitioned hash join feeding an aggregate ---------------------
  * ParallelHashJoinAggregate (exec/parallel_join.go:665-758,780-988) over
  * BuildSharedHashTableParallel (parallel_join.go:389-552) and the flat table

@@ -232,6 +232,7 @@ void slot_destroy(vq_ctx* c, Slot* S) {
 
 void stop_workers(vq_ctx* c);
 void tune_load(vq_ctx* c);
+int scratch_setup(int device);
 
 // A query's hold on one slot: waits for an idle slot, gives it back at the
 // end of the scope.
@@ -555,6 +556,11 @@ extern "C" int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out) {
   }
   c->slots.push_back(S);
   c->idle.push_back(S);
+  if (int rc = scratch_setup(device)) {
+    slot_destroy(c, S);
+    delete c;
+    return rc;
+  }
   tune_load(c);
   *out = c;
   return VQ_OK;
@@ -993,6 +999,105 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
     stats->tile_rows = d.tile_rows;
   }
 
+  return dense_result(hfin, d, am, card, nkeys, naggs, aggs, out);
+}
+
+// ===========================================================================
+// dictionary-code GROUP BY that allocates nothing
+// ===========================================================================
+// The partials and final words of vq_agg_dict_noalloc live in this file-scope
+// device array (each device holds its own copy for the life of the process).
+// In host code `scratch` names a host-side stand-in, never the device memory:
+// its device address is looked up once per device (scratch_setup, from
+// vq_ctx_create with that device current) and only that address is used.
+constexpr size_t kNoAllocScratchBytes = (size_t)2 << 20;
+constexpr size_t kNoAllocScratchWords = kNoAllocScratchBytes / 8;
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchWords];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch[kNoAllocDevices];  // scratch's device address, per device
+// calls on one device take turns on its scratch and host staging
+std::mutex g_noalloc_mu[kNoAllocDevices];
+uint64_t g_noalloc_host[kNoAllocDevices][kDenseGroupsMax * (kMaxAcc + 2)];
+
+int scratch_setup(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return VQ_OK;  // (the entry point then refuses)
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return VQ_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(VQ_E_DEVICE, "exec: gpu: device %d: scratch address: %s", device, hipGetErrorString(e));
+  g_scratch[device] = (uint64_t*)p;
+  return VQ_OK;
+}
+
+}  // namespace
+
+extern "C" int vq_agg_dict_noalloc(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
+                                   const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
+                                   vq_agg_result* out, vq_stats* stats) {
+  if (!c || !t || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  out->ngroups = 0;  // no result unless this call succeeds
+  if (t->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  if (c->device < 0 || c->device >= kNoAllocDevices || !g_scratch[c->device])
+    return fail(VQ_E_DEVICE, "exec: gpu: no scratch for device %d", c->device);
+  QDesc d;
+  AccMap am;
+  SlotMap sm;
+  int32_t card[4] = {1, 1, 1, 1};
+  if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card)) return rc;
+  const int NI = d.nacc + 2;
+  // final words [G][NI] first (256-byte aligned), then partials [grid][G][NI]
+  const size_t fin_words = (size_t)d.ngroups * NI;
+  const size_t part_off = (fin_words + 31) & ~(size_t)31;
+  if (part_off + fin_words > kNoAllocScratchWords)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d groups x %d words exceed the %zu-byte scratch", d.ngroups,
+                NI, kNoAllocScratchBytes);
+  std::vector<uint8_t> keep;
+  prune(t, pred, keep);
+  d.tile_rows = scan_tile_rows(kept_rows(t, keep));
+  int64_t rows = 0, kept = 0;
+  if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
+  // the persistent kernel's waves: as many as the device holds, fewer when
+  // their partials would not fit the scratch (each wave then takes more tiles)
+  int64_t grid = d.ntiles > 0 ? dense_grid(d) : 0;
+  grid = std::min<int64_t>(grid, (int64_t)((kNoAllocScratchWords - part_off) / fin_words));
+
+  std::lock_guard<std::mutex> lk(g_noalloc_mu[c->device]);
+  SlotLease S(c);
+  hipSetDevice(c->device);
+  uint64_t* fin = g_scratch[c->device];
+  uint64_t* part = fin + part_off;
+  uint64_t* hfin = g_noalloc_host[c->device];
+  if (d.ntiles > 0) {
+    HIP_TRY(hipEventRecord(S->ev[0], S->stream));
+    HIP_TRY(launch_agg_dense(d, (int32_t)grid, part, S->stream));
+    HIP_TRY(hipEventRecord(S->ev[1], S->stream));
+    HIP_TRY(launch_agg_final(d, (int32_t)grid, part, fin, S->stream));
+    HIP_TRY(hipEventRecord(S->ev[2], S->stream));
+    HIP_TRY(hipMemcpyAsync(hfin, fin, fin_words * 8, hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(hipStreamSynchronize(S->stream));
+  } else {
+    for (int g = 0; g < d.ngroups; g++)
+      for (int i = 0; i < NI; i++) hfin[g * NI + i] = (i == d.nacc + 1) ? 0xFFFFFFFFULL : 0;
+  }
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    if (d.ntiles > 0) {
+      stats->main_kernel_ms = elapsed(S->ev[0], S->ev[1]);
+      stats->device_ms = elapsed(S->ev[0], S->ev[2]);
+    }
+    stats->rows_scanned = rows;
+    stats->rg_total = t->nrg;
+    stats->rg_scanned = kept;
+    stats->bytes_alg = rows * slot_bytes_per_row(t, sm);
+    stats->grid = (int32_t)grid;
+    stats->tile_rows = d.tile_rows;
+  }
   return dense_result(hfin, d, am, card, nkeys, naggs, aggs, out);
 }
 

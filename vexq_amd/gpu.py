@@ -61,6 +61,8 @@ def lib():
     L.vq_selvec_destroy.argtypes = [C.c_void_p]
     L.vq_agg_dict.argtypes = [C.c_void_p, C.c_void_p, P(abi.Pred), C.c_int32, P(C.c_int32),
                               C.c_int32, P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
+    L.vq_agg_dict_noalloc.argtypes = [C.c_void_p, C.c_void_p, P(abi.Pred), C.c_int32, P(C.c_int32),
+                                      C.c_int32, P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
     L.vq_join_agg.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_void_p,
                               C.c_int32, P(abi.Pred), C.c_int32, P(abi.Agg), C.c_int32,
                               C.c_int32, P(abi.AggResult), P(abi.Stats)]
@@ -263,6 +265,19 @@ class Context:
         st = abi.Stats()
         _check(lib().vq_agg_dict(self.h, dt.h, _pp(pred), len(key_cols), keys, len(aggs), arr,
                                  C.byref(out.c), C.byref(st)), out)
+        out.stats = st
+        return out
+
+    def agg_dict_noalloc(self, dt: "DeviceTable", pred, key_cols, aggs,
+                         capacity: int = 4096) -> abi.AggOut:
+        """agg_dict through vq_agg_dict_noalloc: no device allocation (the
+        library's fixed per-device scratch array)."""
+        keys = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
+        arr = abi.agg_array(aggs)
+        out = abi.AggOut(capacity, len(key_cols), len(aggs))
+        st = abi.Stats()
+        _check(lib().vq_agg_dict_noalloc(self.h, dt.h, _pp(pred), len(key_cols), keys, len(aggs), arr,
+                                         C.byref(out.c), C.byref(st)), out)
         out.stats = st
         return out
 
