@@ -257,10 +257,17 @@ int vq_selvec_destroy(vq_selvec* sv);
 /* ---- dictionary-code GROUP BY ---------------------------------------------
  * ParallelHashAggregate over Scan -> Filter -> Project -> HashAggregate with
  * the int-key fast path (exec/parallel.go:78-86,124-314, exec/agg_intkey.go:
- * 206-361, exec/aggregate.go:361-876).  key_cols name 0..4 dictionary columns
+ * 206-361, exec/aggregate.go:361-876).  key_cols name 0..8 dictionary columns
  * (VQ_U8/VQ_U16 with dict_size > 0); nkeys = 0 is the global aggregate
  * (accumulateDirect, aggregate.go:567-745; one row COUNT 0 / NULLs on empty
- * input, aggregate.go:961-982). */
+ * input, aggregate.go:961-982).  Up to 4 columns whose code space (the
+ * product of their dict_size) is at most 128 slots, with room for the
+ * accumulators in the kernel's per-lane LDS slots, run the dense kernel;
+ * any other key set has its codes packed into one 64-bit key as the int-key
+ * path packs them (32 bits per column for up to 2 columns, 16 for up to 4, 8
+ * for up to 8; exec/agg_intkey.go:67-100) and runs through vq_agg_hash's
+ * kernels, float sums then exact; codes wider than their packed width return
+ * VQ_E_UNSUPPORTED (the reference's string-key path). */
 int vq_agg_dict(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t nkeys,
                 const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                 vq_agg_result* out, vq_stats* stats);
@@ -272,8 +279,9 @@ int vq_agg_dict(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t nke
  * it runs the precompiled kernels only (no run-time compilation) and keeps
  * its partials and final words in a fixed 2 MiB device array that the library
  * holds on each device for the life of the process (no arena, no pinned
- * buffer; fewer waves when their partials would not fit).  A shape whose
- * final words alone exceed that array returns VQ_E_UNSUPPORTED.  Every call
+ * buffer; fewer waves when their partials would not fit).  Only the dense
+ * kernel's shapes: a key set beyond its slots (which vq_agg_dict packs into
+ * the hash kernels) returns VQ_E_UNSUPPORTED.  Every call
  * on one device shares that device's array: calls from this library take
  * turns on it (a host lock held until the result is on the host), and work
  * that overlaps it on the same device in any other way races. */

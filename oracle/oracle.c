@@ -441,7 +441,7 @@ int64_t or_filter(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t
  * stored as IEEE bits (exec/aggregate.go:83-129); newAccums seeds MIN/MAX with
  * the extreme sentinel (aggregate.go:255-279). */
 typedef struct group {
-  int64_t key[4];
+  int64_t key[8];
   int64_t count;
   int64_t first_row;
   int64_t acc[VQ_MAX_AGGS];
@@ -698,19 +698,33 @@ static int check_cap(vq_agg_result* out, int64_t n) {
 }
 
 /* Group-key packing of the int-key path (exec/agg_intkey.go:67-100): 32 bits
- * per column for <= 2 columns, 16 for <= 4. */
+ * per column for <= 2 columns, 16 for <= 4, 8 for <= 8. */
+static inline int key_width(int32_t nkeys) { return nkeys <= 2 ? 32 : nkeys <= 4 ? 16 : 8; }
+
 static inline int64_t pack_key(const vq_col_desc* cols, int32_t nkeys, const int32_t* key_cols,
                                int64_t r) {
-  int shift = nkeys <= 2 ? 32 : 16;
+  const int shift = key_width(nkeys);
   uint64_t k = 0;
   for (int i = 0; i < nkeys; i++) k |= (uint64_t)col_i(&cols[key_cols[i]], r) << (shift * i);
   return (int64_t)k;
 }
 
 static void unpack_key(int64_t packed, int32_t nkeys, int64_t* key) {
-  int shift = nkeys <= 2 ? 32 : 16;
-  uint64_t mask = nkeys <= 2 ? 0xFFFFFFFFULL : 0xFFFFULL;
+  const int shift = key_width(nkeys);
+  const uint64_t mask = shift == 32 ? 0xFFFFFFFFULL : ((1ULL << shift) - 1);
   for (int i = 0; i < nkeys; i++) key[i] = (int64_t)(((uint64_t)packed >> (shift * i)) & mask);
+}
+
+/* Every code of a key column fits its packed width (the reference leaves the
+ * int-key path for the string-key one otherwise, agg_intkey.go:80-100; this
+ * restatement then refuses the query). */
+static int keys_fit(const vq_col_desc* cols, int32_t nkeys, const int32_t* key_cols) {
+  const int w = key_width(nkeys);
+  for (int i = 0; i < nkeys; i++) {
+    const vq_col_desc* c = &cols[key_cols[i]];
+    if (c->dict_size <= 0 || (w < 32 && c->dict_size > (1 << w))) return 0;
+  }
+  return 1;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -726,7 +740,8 @@ int or_agg_dict(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t n
                 int32_t nkeys, const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                 int32_t exact, vq_agg_result* out) {
   (void)nrows;
-  if (nkeys < 0 || nkeys > 4 || naggs < 0 || naggs > VQ_MAX_AGGS) return VQ_E_ARG;
+  if (nkeys < 0 || nkeys > 8 || naggs < 0 || naggs > VQ_MAX_AGGS) return VQ_E_ARG;
+  if (!keys_fit(cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
   uint8_t* keep = (uint8_t*)malloc((size_t)nrg + 1);
   or_zone_prune(ncols, cols, nrg, zones, pred, keep);
   kmap m;
@@ -1020,6 +1035,7 @@ typedef struct agg_job {
   const uint8_t* keep;
   const vq_pred* pred;
   int32_t nkeys;
+  int32_t hash_col; /* >= 0: GROUP BY this one integer column (general key path) */
   const int32_t* key_cols;
   int32_t naggs;
   const vq_agg* aggs;
@@ -1038,11 +1054,14 @@ static void* agg_worker(void* arg) {
     if (!j->keep[rg]) continue;
     for (int64_t r = j->rg_base[rg]; r < j->rg_base[rg] + j->rg_rows[rg]; r++) {
       if (!pred_pass(j->cols, j->pred, r)) continue;
-      int64_t pk = j->nkeys ? pack_key(j->cols, j->nkeys, j->key_cols, r) : 0;
+      int64_t pk = j->hash_col >= 0 ? col_i(&j->cols[j->hash_col], r)
+                   : j->nkeys    ? pack_key(j->cols, j->nkeys, j->key_cols, r)
+                                 : 0;
       group* g = kmap_group(&j->m, &j->l, pk, j->naggs, j->aggs, 0);
       if (g->first_row < 0) {
         g->first_row = r;
-        unpack_key(pk, j->nkeys, g->key);
+        if (j->hash_col >= 0) g->key[0] = pk;
+        else unpack_key(pk, j->nkeys, g->key);
       }
       fold_row(g, j->cols, j->naggs, j->aggs, r);
     }
@@ -1050,11 +1069,12 @@ static void* agg_worker(void* arg) {
   return NULL;
 }
 
-int or_agg_dict_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
-                         const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
-                         int32_t nkeys, const int32_t* key_cols, int32_t naggs,
-                         const vq_agg* aggs, int32_t nthreads, vq_agg_result* out) {
-  (void)nrows;
+/* ParallelHashAggregate (exec/parallel.go:124-314) over the dictionary-code
+ * keys (hash_col < 0) or one integer key column (hash_col >= 0). */
+static int agg_parallel(int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                        const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                        int32_t nkeys, const int32_t* key_cols, int32_t hash_col, int32_t naggs,
+                        const vq_agg* aggs, int32_t nthreads, vq_agg_result* out) {
   if (nthreads < 1) nthreads = 1;
   if (nthreads > nrg && nrg > 0) nthreads = nrg;
   uint8_t* keep = (uint8_t*)malloc((size_t)nrg + 1);
@@ -1075,6 +1095,7 @@ int or_agg_dict_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, 
     j->keep = keep;
     j->pred = pred;
     j->nkeys = nkeys;
+    j->hash_col = hash_col;
     j->key_cols = key_cols;
     j->naggs = naggs;
     j->aggs = aggs;
@@ -1091,8 +1112,10 @@ int or_agg_dict_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, 
     for (int64_t k = 0; k < j->l.n; k++) {
       group* src = &j->l.g[k];
       int64_t pk = 0;
-      int shift = nkeys <= 2 ? 32 : 16;
-      for (int c = 0; c < nkeys; c++) pk |= (int64_t)((uint64_t)src->key[c] << (shift * c));
+      const int shift = key_width(nkeys);
+      if (hash_col >= 0) pk = src->key[0];
+      else
+        for (int c = 0; c < nkeys; c++) pk |= (int64_t)((uint64_t)src->key[c] << (shift * c));
       int f;
       int64_t s = kmap_find(&m, pk, &f);
       if (f) merge_group(&l.g[m.idx[s]], src, naggs, aggs);
@@ -1128,6 +1151,27 @@ int or_agg_dict_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, 
   free(rg_base);
   free(keep);
   return rc;
+}
+
+int or_agg_dict_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                         const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                         int32_t nkeys, const int32_t* key_cols, int32_t naggs,
+                         const vq_agg* aggs, int32_t nthreads, vq_agg_result* out) {
+  (void)nrows;
+  if (nkeys < 0 || nkeys > 8 || naggs < 0 || naggs > VQ_MAX_AGGS) return VQ_E_ARG;
+  if (!keys_fit(cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
+  return agg_parallel(ncols, cols, nrg, rg_rows, zones, pred, nkeys, key_cols, -1, naggs, aggs,
+                      nthreads, out);
+}
+
+int or_agg_hash_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                         const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                         int32_t key_col, int32_t naggs, const vq_agg* aggs, int32_t nthreads,
+                         vq_agg_result* out) {
+  (void)nrows;
+  if (key_col < 0 || key_col >= ncols || naggs < 0 || naggs > VQ_MAX_AGGS) return VQ_E_ARG;
+  return agg_parallel(ncols, cols, nrg, rg_rows, zones, pred, 1, NULL, key_col, naggs, aggs,
+                      nthreads, out);
 }
 
 /* ---- parallel join: two-pass radix build, morsel-parallel probe --------- */

@@ -74,6 +74,13 @@ int or_agg_dict_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, 
                          int32_t nkeys, const int32_t* key_cols, int32_t naggs,
                          const vq_agg* aggs, int32_t nthreads, vq_agg_result* out);
 
+/* The same partial-aggregate plan over one integer key column (the general
+ * key path, exec/aggregate.go:361-559, partials merged as parallel.go:186-209). */
+int or_agg_hash_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                         const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                         int32_t key_col, int32_t naggs, const vq_agg* aggs, int32_t nthreads,
+                         vq_agg_result* out);
+
 int or_join_agg_parallel(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols,
                          int32_t b_key, const vq_pred* b_pred, int64_t p_nrows, int32_t p_ncols,
                          const vq_col_desc* p_cols, int32_t p_key, const vq_pred* p_pred,

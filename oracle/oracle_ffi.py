@@ -58,6 +58,9 @@ def lib():
     L.or_agg_hash.argtypes = [C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32, P(C.c_int64),
                               P(abi.Zone), P(abi.Pred), C.c_int32, C.c_int32, P(abi.Agg),
                               C.c_int32, P(abi.AggResult)]
+    L.or_agg_hash_parallel.argtypes = [C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32,
+                                       P(C.c_int64), P(abi.Zone), P(abi.Pred), C.c_int32,
+                                       C.c_int32, P(abi.Agg), C.c_int32, P(abi.AggResult)]
     L.or_agg_dict_parallel.argtypes = [C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32,
                                        P(C.c_int64), P(abi.Zone), P(abi.Pred), C.c_int32,
                                        P(C.c_int32), C.c_int32, P(abi.Agg), C.c_int32,
@@ -142,15 +145,20 @@ def agg_dict(table, pred, key_cols, aggs, exact: bool = True, capacity: int = 1 
 
 
 def agg_hash(table, pred, key_col, aggs, exact: bool = True, capacity: int | None = None,
-             zones=None) -> abi.AggOut:
-    """Serial GROUP BY one integer column (HashAggregate's general key path)."""
+             zones=None, nthreads: int = 0) -> abi.AggOut:
+    """Serial GROUP BY one integer column (HashAggregate's general key path),
+    or (nthreads > 0) the morsel-parallel CPU baseline of the same query."""
     zones = zones if zones is not None else zone_maps(table)
     descs = abi.col_descs(table)
     rg, rgp = abi.rg_array(table)
     arr = abi.agg_array(aggs)
     out = abi.AggOut(capacity or max(1, table.nrows), 1, len(aggs))
-    rc = lib().or_agg_hash(table.nrows, len(table.columns), descs, len(rg), rgp, zones, _pp(pred),
-                           key_col, len(aggs), arr, int(exact), C.byref(out.c))
+    if nthreads:
+        rc = lib().or_agg_hash_parallel(table.nrows, len(table.columns), descs, len(rg), rgp, zones,
+                                        _pp(pred), key_col, len(aggs), arr, nthreads, C.byref(out.c))
+    else:
+        rc = lib().or_agg_hash(table.nrows, len(table.columns), descs, len(rg), rgp, zones,
+                               _pp(pred), key_col, len(aggs), arr, int(exact), C.byref(out.c))
     if rc:
         raise RuntimeError(f"oracle agg_hash failed: {rc}")
     return out

@@ -250,6 +250,68 @@ def test_groupby_variants(gpu_ctx, kernel_path):
         dt.close()
 
 
+def _coded_table(n, cards, seed):
+    """n rows: one dictionary-code column per cardinality in `cards` (u8 up
+    to 256 codes, else u16; codes in first-occurrence order), plus the
+    lineitem measures."""
+    rng = np.random.default_rng(seed)
+    li = datagen.lineitem(n, seed=seed)
+    cols = []
+    for i, c in enumerate(cards):
+        raw = rng.integers(0, c, n)
+        codes, _ = datagen.first_occurrence_codes(raw, [f"v{j}" for j in range(c)])
+        typ = abi.VQ_U8 if c <= 256 else abi.VQ_U16
+        cols.append(datagen._col(f"k{i}", typ, codes, [f"v{j}" for j in range(c)]))
+    cols += [li.columns[li.index(x)] for x in ("quantity", "price", "discount", "tax", "shipdate")]
+    return datagen.Table(cols, datagen.row_groups(n))
+
+
+@pytest.mark.parametrize("cards,n", [((20, 30), 300_000), ((3, 2, 50), 400_000),
+                                     ((1000, 1000), 2_000_000), ((5, 7, 3, 4, 6), 500_000),
+                                     ((40000,), 1_000_000)])
+def test_groupby_beyond_dense_slots_packs_keys(gpu_ctx, cards, n):
+    """A dictionary GROUP BY whose code space exceeds the dense kernel's
+    slots: 2 keys of 600 groups, 3 keys of 300, 2 u16 keys of ~860K groups,
+    5 keys (8-bit packing), one u16 key of 40,000 codes -- the codes packed
+    into one key as the int-key path packs them, grouped by the hash
+    kernels, against the oracle."""
+    _print_case("groupby_packed", cards=cards, n=n)
+    t = _coded_table(n, cards, seed=len(cards) * 7 + n % 97)
+    keys = list(range(len(cards)))
+    X = abi.X
+    pred = abi.make_pred(t, [("shipdate", "<=", datagen.DATE_MIN + datagen.SHIP_DAYS - 1 - 90)])
+    aggs = [abi.agg_sum("quantity", t), abi.agg_sum("price", t),
+            abi.agg_sum(X.col("price") * (1 - X.col("discount")), t),
+            abi.agg_avg("discount", t), abi.count_star(t), abi.agg_max("tax", t)]
+    if len(cards) > 3:  # a query touches at most 8 columns
+        aggs = [abi.agg_sum("price", t), abi.agg_avg("discount", t), abi.count_star(t)]
+    dt = gpu_ctx.upload(t)
+    try:
+        got = gpu_ctx.agg_dict(dt, pred, keys, aggs)
+        again = gpu_ctx.agg_dict(dt, pred, keys, aggs)
+    finally:
+        dt.close()
+    ref = O.agg_dict(t, pred, keys, aggs, exact=True, capacity=t.nrows)
+    assert ref.ngroups > 128
+    assert_agg_equal(got, ref, aggs)
+    assert same_bits(got, again)
+
+
+def test_packed_keys_wider_than_their_width_are_unsupported(gpu_ctx):
+    """Five key columns pack 8 bits each: a column of 300 codes does not fit,
+    and the query is refused (the reference's string-key path)."""
+    _print_case("groupby_packed_unsupported")
+    from vexq_amd import gpu as G
+    t = _coded_table(20_000, (300, 2, 3, 4, 5), seed=3)
+    dt = gpu_ctx.upload(t)
+    try:
+        with pytest.raises(G.GpuError) as e:
+            gpu_ctx.agg_dict(dt, None, [0, 1, 2, 3, 4], [abi.count_star(t)])
+        assert e.value.code == abi.VQ_E_UNSUPPORTED
+    finally:
+        dt.close()
+
+
 # ---------------------------------------------------------------------------
 # Zone-pruned scan + SUM(expr)
 # ---------------------------------------------------------------------------

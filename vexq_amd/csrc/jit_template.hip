@@ -29,6 +29,9 @@
 #define VQ_PB 8
 #define VQ_BLOOM 1
 #define VQ_LIVEBITS 1
+#define VQ_NKEY 1
+static constexpr int kKSL[8] = {2, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kKSH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
 static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
@@ -723,11 +726,25 @@ __device__ __forceinline__ void pload_one(const Args& a, i64 r, i64 (&raw)[VQ_NS
 template <int C>
 __device__ constexpr bool live_slot() {
   if (C == VQ_KEYSLOT) return true;
+  for (int k = 0; k < VQ_NKEY; k++)
+    if (kKSL[k] == C) return true;
   for (int t = 0; t < VQ_NT; t++)
     if (kTS[t] == C || (kTO[t] >= 16 && kTS2[t] == C)) return true;  // (either side of a compare)
   return false;
 }
 constexpr bool kLateVals = (VQ_BLOOM != 0) && (VQ_PROBE != 0);
+
+// The row's key: its key column, or (a GROUP BY of several dictionary-code
+// columns) their codes packed as the int-key path packs them
+// (exec/agg_intkey.go:67-100).
+template <int... K>
+__device__ __forceinline__ i64 packed_key(const i64* row, Seq<K...>) {
+  return (i64)(0ull | ... | ((u64)row[kKSL[K]] << kKSH[K]));
+}
+__device__ __forceinline__ i64 row_key(const i64* row) {
+  if constexpr (VQ_NKEY > 1) return packed_key(row, seq<VQ_NKEY>{});
+  else return row[VQ_KEYSLOT];
+}
 
 template <int C, int U>
 __device__ __forceinline__ void pload_slot(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u) {
@@ -821,7 +838,7 @@ __device__ __forceinline__ void prows_from(const Args& a, i64 base, i64 rend,
     for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
     const i64 r = base + u * kPB + threadIdx.x;
     q.live[u] = r < rend && pass_all(a, row, seq<VQ_NT>{});
-    q.key[u] = row[VQ_KEYSLOT];
+    q.key[u] = row_key(row);
     const u64 h = radix_hash((u64)q.key[u]);
     q.part[u] = (u32)(h & mask);
 #if VQ_BLOOM
@@ -876,7 +893,7 @@ __device__ __forceinline__ void prows_live(const Args& a, i64 base, i64 rend, co
     if (q.live[u]) {
       i64 row[VQ_NS];
       load_all(a, r, row, seq<VQ_NS>{});
-      q.key[u] = row[VQ_KEYSLOT];
+      q.key[u] = row_key(row);
       q.part[u] = (u32)(radix_hash((u64)q.key[u]) & mask);
       if constexpr (VQ_NV > 0) values_of(a, row, q.val[u], seq<VQ_NV>{});
     }
@@ -1318,7 +1335,7 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
       for (int q = 0; q < VQ_NS; q++) row[q] = cur[q][u];
       const i64 rr = cbase + u * kLT + tid;
       if (!(rr < cend && pass_all(a, row, seq<VQ_NT>{}))) continue;
-      const i64 key = row[VQ_KEYSLOT];
+      const i64 key = row_key(row);
       u32 g = 0xFFFFFFFFu;
       if (key == kI64Min) {
         if (min_g >= 0) g = (u32)min_g;

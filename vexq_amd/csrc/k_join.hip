@@ -1064,6 +1064,11 @@ constexpr int kLocalThreads = 512;
 struct LocalKeys {
   const void* col;
   int32_t type;
+  // several dictionary-code columns packed into one key (JSide.nkeys > 1)
+  int32_t nkeys;
+  const void* kcol[8];
+  int32_t ktype[8];
+  int32_t kshift[8];
   int32_t lcap;
   int32_t limit;
   int32_t nspans;
@@ -1139,7 +1144,14 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_keys(const LocalKeys k)
     const int64_t rend = row0 + k.tile_rows < send ? row0 + k.tile_rows : send;
     bool over = false;
     for (int64_t r = row0 + threadIdx.x; r < rend && !over; r += kLocalThreads) {
-      const int64_t key = load_key(k.col, k.type, r);
+      int64_t key;
+      if (k.nkeys > 1) {
+        uint64_t pk = 0;
+        for (int c = 0; c < k.nkeys; c++) pk |= (uint64_t)load_key(k.kcol[c], k.ktype[c], r) << k.kshift[c];
+        key = (int64_t)pk;
+      } else {
+        key = load_key(k.col, k.type, r);
+      }
       if (key == kEmptyKey) {
         s_min = 1;
         continue;
@@ -1306,6 +1318,12 @@ hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
   memset(&k, 0, sizeof k);
   k.col = ps.slot[ps.key_slot].p;
   k.type = ps.slot[ps.key_slot].type;
+  k.nkeys = ps.nkeys;
+  for (int c = 0; c < ps.nkeys && c < 8; c++) {
+    k.kcol[c] = ps.slot[ps.key_slots[c]].p;
+    k.ktype[c] = ps.slot[ps.key_slots[c]].type;
+    k.kshift[c] = ps.key_shift[c];
+  }
   k.lcap = plan.gcap;
   k.limit = plan.limit * plan.passes;
   k.nspans = ps.nspans;

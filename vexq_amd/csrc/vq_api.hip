@@ -777,12 +777,20 @@ struct AccMap {
   int32_t acc[VQ_MAX_AGGS];  // accumulator index per aggregate (-1 = COUNT)
 };
 
+// *over is set when the query is well formed but its group slots exceed what
+// the dense kernel's per-lane accumulators hold (more than 4 key columns,
+// more than kDenseGroupsMax slots, or more LDS than kDenseLdsMax).
 int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int32_t* key_cols,
                int32_t naggs, const vq_agg* aggs, QDesc& d, AccMap& am, SlotMap& sm,
-               int32_t card[4]) {
+               int32_t card[4], bool* over = nullptr) {
   memset(&d, 0, sizeof d);
-  if (nkeys < 0 || nkeys > 4 || (nkeys > 0 && !key_cols))
+  if (over) *over = false;
+  if (nkeys < 0 || nkeys > 8 || (nkeys > 0 && !key_cols))
     return fail(VQ_E_ARG, "exec: gpu: %d group-by columns", nkeys);
+  if (nkeys > 4) {
+    if (over) *over = true;
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d group-by columns for the dense slots", nkeys);
+  }
   if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs))
     return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
   if (naggs == 0 && nkeys == 0)
@@ -795,9 +803,12 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
     if ((c.type != VQ_U8 && c.type != VQ_U16) || c.dict_size <= 0)
       return fail(VQ_E_UNSUPPORTED, "exec: gpu: group-by column %d is not a dictionary code", key_cols[i]);
     card[i] = c.dict_size;
-    G *= c.dict_size;
+    G = std::min<int64_t>(G * c.dict_size, (int64_t)1 << 40);  // (no overflow past 4 columns)
   }
-  if (G > 4096) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld dense group slots", (long long)G);
+  if (G > kDenseGroupsMax) {
+    if (over) *over = true;
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld dense group slots", (long long)G);
+  }
   int64_t stride = 1;
   for (int i = nkeys - 1; i >= 0; i--) {
     int s = sm.get(key_cols[i]);
@@ -840,8 +851,10 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
     d.slot[s].type = t->cols[0].type;
     d.nslots = 1;
   }
-  if ((G > 1 && dense_lds_bytes(d) > kDenseLdsMax) || G > kDenseGroupsMax)
+  if (G > 1 && dense_lds_bytes(d) > kDenseLdsMax) {
+    if (over) *over = true;
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: %lld groups x %d accumulators exceed LDS", (long long)G, d.nacc);
+  }
   return VQ_OK;
 }
 
@@ -919,6 +932,10 @@ int dense_result(const uint64_t* hfin, const QDesc& d, const AccMap& am, const i
   return VQ_OK;
 }
 
+int agg_packed(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
+               const int32_t* key_cols, int32_t naggs, const vq_agg* aggs, vq_agg_result* out,
+               vq_stats* stats);
+
 }  // namespace
 
 extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
@@ -931,7 +948,13 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   AccMap am;
   SlotMap sm;
   int32_t card[4] = {1, 1, 1, 1};
-  if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card)) return rc;
+  bool over = false;
+  if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card, &over)) {
+    // more dense group slots than the per-lane accumulators hold: the codes
+    // packed into one key, through the hash GROUP BY's kernels
+    if (over) return agg_packed(c, t, pred, nkeys, key_cols, naggs, aggs, out, stats);
+    return rc;
+  }
   std::vector<uint8_t> keep;
   prune(t, pred, keep);
   // tiles: an even share of the kept rows for each wave the run-time
@@ -1327,7 +1350,8 @@ std::string build_est_key(const vq_table* t, int32_t key, const vq_pred* pred) {
 }
 
 int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg* aggs,
-              int32_t naggs, JSide& sd, JDesc* jd, SlotMap& sm) {
+              int32_t naggs, JSide& sd, JDesc* jd, SlotMap& sm, int32_t npack = 0,
+              const int32_t* pack_cols = nullptr, const int32_t* pack_shift = nullptr) {
   memset(&sd, 0, sizeof sd);
   if (int rc = check_col(t, key)) return rc;
   if (t->cols[key].type == VQ_F64) return fail(VQ_E_UNSUPPORTED, "exec: gpu: float join key");
@@ -1335,6 +1359,17 @@ int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg*
   int ks = sm.get(key);
   if (ks < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
   sd.key_slot = ks;
+  sd.nkeys = 1;
+  sd.key_slots[0] = ks;
+  if (npack > 1) {
+    sd.nkeys = npack;
+    for (int i = 0; i < npack; i++) {
+      const int s2 = sm.get(pack_cols[i]);
+      if (s2 < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+      sd.key_slots[i] = s2;
+      sd.key_shift[i] = pack_shift[i];
+    }
+  }
   if (jd) {
     for (int j = 0; j < naggs; j++)
       if (aggs[j].kind != VQ_AGG_COUNT)
@@ -1379,6 +1414,10 @@ struct JoinJob {
   vq_agg aggs[VQ_MAX_AGGS];
   int32_t radix_bits = -1, topk = 0;
   bool groupby = false;
+  // GROUP BY several dictionary-code columns: their codes packed into one
+  // key (exec/agg_intkey.go:67-100), unpacked again into the result
+  int32_t npack = 0;
+  int32_t pack_cols[8] = {0}, pack_shift[8] = {0};
   JoinTune tune;
   // fixed by setup
   JDesc jd;
@@ -1463,7 +1502,9 @@ int join_setup(JoinJob& j) {
   memset(&j.bs, 0, sizeof j.bs);
   if (!j.groupby)
     if (int rc = plan_side(j.build, j.build_key, j.bp, nullptr, 0, j.bs, nullptr, j.bsm)) return rc;
-  if (int rc = plan_side(j.probe, j.probe_key, j.pp, j.aggs, j.naggs, j.ps, &j.jd, j.psm)) return rc;
+  if (int rc = plan_side(j.probe, j.probe_key, j.pp, j.aggs, j.naggs, j.ps, &j.jd, j.psm, j.npack,
+                         j.pack_cols, j.pack_shift))
+    return rc;
   // accumulator words and fixed-point scales
   const int L = ilog2_ceil((uint64_t)j.probe->nrows + 1);
   int nw = 0, nvals = 0;
@@ -1518,6 +1559,7 @@ int join_setup(JoinJob& j) {
   // and doubles when it fills; a table that fills adds one bit.
   j.est_key = j.groupby ? "G|" + build_est_key(j.probe, j.probe_key, j.pp)
                         : build_est_key(j.build, j.build_key, j.bp);
+  for (int i = 0; j.npack > 1 && i < j.npack; i++) j.est_key += "#k" + std::to_string(j.pack_cols[i]);
   j.grp_key = j.groupby ? j.est_key : j.est_key + "#" + build_est_key(j.probe, j.probe_key, j.pp);
   j.est = std::max<int64_t>(j.groupby ? j.probe->nrows : j.build->nrows, 1);
   {
@@ -1957,7 +1999,14 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     for (int64_t i = i0; i < i1; i++) {
       if (i + 8 < i1) __builtin_prefetch(&recs[(size_t)(uint32_t)o[i + 8] * RW]);
       const uint64_t* r = &recs[(size_t)(uint32_t)o[i] * RW];
-      out->keys[i] = (int64_t)r[0];
+      if (j.npack > 1) {  // the codes of the packed key, in GROUP BY order
+        const int w = j.npack <= 2 ? 32 : j.npack <= 4 ? 16 : 8;
+        const uint64_t mask = (1ULL << w) - 1;
+        for (int k = 0; k < j.npack; k++)
+          out->keys[i * j.npack + k] = (int64_t)(((uint64_t)r[0] >> j.pack_shift[k]) & mask);
+      } else {
+        out->keys[i] = (int64_t)r[0];
+      }
       out->counts[i] = (int64_t)r[1];
       out->first_row[i] = (int64_t)r[2];
       for (int k = 0; k < naggs; k++) {
@@ -2057,6 +2106,40 @@ int join_impl(vq_ctx* c, const vq_table* build, int32_t build_key, const vq_pred
   join_job_init(*j, c, build, build_key, build_pred, probe, probe_key, probe_pred, naggs, aggs,
                 radix_bits, topk, groupby);
   if (tune) j->tune = *tune;
+  return join_run(*j, out, stats);
+}
+
+// A dictionary-code GROUP BY whose code space exceeds the dense slots: the
+// int-key path's packing (exec/agg_intkey.go:67-100: 32 bits per column for
+// up to 2 columns, 16 for up to 4, 8 for up to 8) makes one 64-bit key per
+// row, grouped by the hash GROUP BY's kernels and unpacked into the result.
+// Codes wider than their packed width would collide: the reference then
+// leaves the int-key path for its string keys, and this returns
+// VQ_E_UNSUPPORTED (fall back).
+int agg_packed(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
+               const int32_t* key_cols, int32_t naggs, const vq_agg* aggs, vq_agg_result* out,
+               vq_stats* stats) {
+  if (nkeys < 1 || nkeys > 8 || !key_cols) return fail(VQ_E_ARG, "exec: gpu: %d group-by columns", nkeys);
+  if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs))
+    return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
+  const int w = nkeys <= 2 ? 32 : nkeys <= 4 ? 16 : 8;
+  for (int i = 0; i < nkeys; i++) {
+    if (int rc = check_col(t, key_cols[i])) return rc;
+    const ColDev& cd = t->cols[key_cols[i]];
+    if ((cd.type != VQ_U8 && cd.type != VQ_U16) || cd.dict_size <= 0)
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: group-by column %d is not a dictionary code", key_cols[i]);
+    if (w < 32 && cd.dict_size > (1 << w))
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d codes of column %d exceed the %d-bit packed key", cd.dict_size,
+                  key_cols[i], w);
+  }
+  std::unique_ptr<JoinJob> j(new (std::nothrow) JoinJob());
+  if (!j) return fail(VQ_E_NOMEM, "exec: gpu: GROUP BY job allocation");
+  join_job_init(*j, c, nullptr, 0, nullptr, t, key_cols[0], pred, naggs, aggs, -1, 0, true);
+  j->npack = nkeys;
+  for (int i = 0; i < nkeys; i++) {
+    j->pack_cols[i] = key_cols[i];
+    j->pack_shift[i] = w * i;
+  }
   return join_run(*j, out, stats);
 }
 

@@ -119,6 +119,12 @@ struct JSide {
   int64_t ntiles;
   QSlot slot[kMaxSlots];
   QSpan span[kMaxSpans];
+  // a GROUP BY of several dictionary-code columns: the key is their codes
+  // packed as the int-key path packs them (exec/agg_intkey.go:67-100), code i
+  // shifted left by key_shift[i]; nkeys 1 is the one key column key_slot
+  int32_t nkeys;
+  int32_t key_slots[8];
+  int32_t key_shift[8];
 };
 
 struct JDesc {

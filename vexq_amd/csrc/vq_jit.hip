@@ -84,7 +84,7 @@ int rows_in_flight(int nslots) { return nslots <= 3 ? 8 : 4; }
 std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slot = 0,
                      int nv = 0, int probe = 0, int pbuf = 0, int bloom = 0,
                      const int* lop = nullptr, const int* lsrc = nullptr, int nag = 0,
-                     int livebits = 0) {
+                     int livebits = 0, const JSide* keys = nullptr) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
@@ -115,6 +115,18 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
   s += "#define VQ_BLOOM " + std::to_string(bloom) + "\n";
   s += "#define VQ_LIVEBITS " + std::to_string(livebits) + "\n";
+  // the key: one column (key_slot), or several dictionary codes packed
+  int nkey = 1, ksl[8] = {key_slot}, ksh[8] = {0};
+  if (keys && keys->nkeys > 1) {
+    nkey = keys->nkeys;
+    for (int k = 0; k < nkey; k++) {
+      ksl[k] = keys->key_slots[k];
+      ksh[k] = keys->key_shift[k];
+    }
+  }
+  s += "#define VQ_NKEY " + std::to_string(nkey) + "\n";
+  arr(s, "kKSL", ksl, 8);
+  arr(s, "kKSH", ksh, 8);
   arr(s, "kST", st, 8);
   arr(s, "kTS", f.ts, 8);
   arr(s, "kTO", f.to, 8);
@@ -278,7 +290,7 @@ int jit_part_prepare(const JSide& sd, const JDesc* jd, int32_t P, int32_t pbuf, 
   const QDesc d = side_desc(sd, jd, &nv);
   const Flat f = flatten_terms(d);
   Module* m = get_module(prologue(d, f, 4, sd.key_slot, nv, jd != nullptr, pbuf, bloom ? 1 : 0,
-                                  nullptr, nullptr, 0, livebits ? 1 : 0),
+                                  nullptr, nullptr, 0, livebits ? 1 : 0, &sd),
                          err);
   if (!m) return -1;
   out->count = m->part_count;
@@ -335,7 +347,7 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
     lsrc[nag] = g.vidx;
     nag++;
   }
-  Module* m = get_module(prologue(d, f, 8, sd.key_slot, nv, 1, 0, 0, lop, lsrc, nag), err);
+  Module* m = get_module(prologue(d, f, 8, sd.key_slot, nv, 1, 0, 0, lop, lsrc, nag, 0, &sd), err);
   if (!m) return -1;
   out->fn = m->hash_local;
   out->lds = kLocalLds;

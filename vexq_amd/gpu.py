@@ -258,7 +258,12 @@ class Context:
         self.last_stats = st
         return int(n)
 
-    def agg_dict(self, dt: "DeviceTable", pred, key_cols, aggs, capacity: int = 4096) -> abi.AggOut:
+    def agg_dict(self, dt: "DeviceTable", pred, key_cols, aggs, capacity: int | None = None) -> abi.AggOut:
+        if capacity is None:  # at most one group per code tuple, and per row
+            space = 1
+            for k in key_cols:
+                space *= max(1, dt.table.columns[k].dict_size)
+            capacity = max(1, min(space, dt.nrows))
         keys = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
         arr = abi.agg_array(aggs)
         out = abi.AggOut(capacity, len(key_cols), len(aggs))
