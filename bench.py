@@ -114,20 +114,42 @@ def operator_record(name, kernel, r, total, walls, mains, devs, steps, traffic, 
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": tr, "launch_ms": round(main_ms, 5)},
     }
+    if tr is not None:
+        rec["roofline"]["traffic_source"] = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and "
+                                             "WRITE_SIZE per dispatch, scaled by a copy of known size")
     if cpu is not None:
         rec["cpu_baseline"] = cpu
     return rec
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it: model name, family / model /
+    stepping, logical CPUs listed."""
+    info, ncpu = {}, 0
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k == "processor":
+                    ncpu += 1
+                elif k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                    info[k] = v
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, model "
+            f"{info.get('model', '?')}, stepping {info.get('stepping', '?')}; {ncpu} logical CPUs listed)")
+
+
+def host_cores():
+    """Threads for the CPU baseline: the CPUs this process may run on
+    (sched_getaffinity), capped by OMP_NUM_THREADS when that is set (the
+    host's share of CPUs per GPU).  Returns (threads, affinity CPUs)."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    n = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    return max(1, n), aff
 
 
 def cpu_record(runs, bytes_alg, cores, sample, kind="port"):
@@ -135,7 +157,14 @@ def cpu_record(runs, bytes_alg, cores, sample, kind="port"):
     return {"value": round(bytes_alg / med / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": kind,
             "sample": sample, "ms_median": round(med * 1e3, 3), "ms_min": round(min(runs) * 1e3, 3),
             "ms_max": round(max(runs) * 1e3, 3), "runs": len(runs), "cpu": cpu_model(),
+            "cpus_in_affinity": host_cores()[1],
             "label": "C restatement of vexq's CPU operators"}
+
+
+def roofline(achieved_gbs, kernel, launch_ms=None, traffic=None):
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "launch_ms": None if launch_ms is None else round(launch_ms, 5)}
 
 
 def copy_probe(steps=20, mib=1024):
@@ -165,7 +194,7 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, int(round(q * (len(xs) - 1))))]
 
 
-def serve_bench(ctx, scale, nq, slots=8):
+def serve_bench(ctx, scale, nq, slots=4, cpu_mod=None, cores=1):
     """The served stream (config 4): a 60M-row fact table and a 15M-row
     dimension table resident; a 50/30/20 mix of range scans, GROUP BYs and
     joins with random windows and cutoffs.  First the one-query-per-launch
@@ -195,16 +224,29 @@ def serve_bench(ctx, scale, nq, slots=8):
         return ctx.submit_agg_dict(dfact, a[0], a[1], a[2])
 
     ctx.set_slots(1)
+    # the dictionary GROUP BY over the 60M-row fact table (2.28 GB of
+    # referenced columns, nine times the 256 MB Infinity Cache): the dense
+    # kernel's HBM rate with no cache to help
+    cold = None
+    if nf >= 20_000_000:
+        pred, keys, aggs = datagen.q_groupby(fact)
+        r, total, walls, mains, devs = time_gpu(lambda: ctx.agg_dict(dfact, pred, keys, aggs), 10, 2, ctx)
+        cold = operator_record("dict_groupby_60m", "vq_dense", r, total, walls, mains, devs, 10, {},
+                               None, nf)
+        cold["groups"] = int(r.ngroups)
+        log(f"[bench] serve: 60M-row GROUP BY kernel {cold['roofline']['achieved']} GB/s")
     log("[bench] serve: warm-up pass (compiles each shape, sizes the arenas)")
     for kind, a in stream:
         run(kind, a)
     log("[bench] serve: one query per launch")
-    lat = []
+    lat, qbytes, qdev = [], [], []
     t0 = time.perf_counter()
     for kind, a in stream:
         s = time.perf_counter()
-        run(kind, a)
+        r = run(kind, a)
         lat.append(time.perf_counter() - s)
+        qbytes.append(int(r.stats.bytes_alg))
+        qdev.append(float(r.stats.device_ms))
     seq_s = time.perf_counter() - t0
     seq_qps = len(stream) / seq_s
     by_kind = {}
@@ -256,16 +298,45 @@ def serve_bench(ctx, scale, nq, slots=8):
     dfact.close()
     ok = [lv for lv in levels if lv["p99_ms"] <= 2 * single["p99_ms"]]
     best = max(ok, key=lambda lv: lv["served_qps"]) if ok else None
-    return {"unit": "queries/s", "fact_rows": nf, "dim_rows": nd, "queries": len(stream),
-            "mix": "50% scan (random 365-day window), 30% GROUP BY (random cutoff), "
-                   "20% join + top-10 (random cutoff)",
-            "slots": slots, "one_query_per_launch": single, "open_loop": levels,
-            "capacity_qps": round(cap_qps, 1),
-            "served_qps_at_p99_within_2x_single": best["served_qps"] if best else None,
-            "qps_ratio": round(best["served_qps"] / seq_qps, 2) if best else None}
+    tot_bytes, tot_dev = sum(qbytes), sum(qdev)
+    rec = {"unit": "queries/s", "fact_rows": nf, "dim_rows": nd, "queries": len(stream),
+           "mix": "50% scan (random 365-day window), 30% GROUP BY (random cutoff), "
+                  "20% join + top-10 (random cutoff)",
+           "slots": slots, "one_query_per_launch": single, "open_loop": levels,
+           "capacity_qps": round(cap_qps, 1),
+           "served_qps_at_p99_within_2x_single": best["served_qps"] if best else None,
+           "qps_ratio": round(best["served_qps"] / seq_qps, 2) if best else None,
+           "bytes_alg": tot_bytes,
+           # the stream's algorithmic bytes over the device time its queries
+           # took one per launch (HIP events around each query's kernels)
+           "roofline": roofline(tot_bytes / (tot_dev * 1e-3) / 1e9 if tot_dev > 0 else 0.0,
+                                "every kernel of the stream's 240 queries, one per launch"),
+           "served_capacity_gbs": round(tot_bytes * cap_qps / len(stream) / 1e9, 2)}
+    if cpu_mod is not None:
+        # the reference's CPU operators on the same queries, in stream order,
+        # for about 10 s of host time
+        done, nbytes, t0 = 0, 0, time.perf_counter()
+        for (kind, a), b in zip(stream, qbytes):
+            if kind == "join":
+                cpu_mod.join_agg(dim, 0, a[0], fact, fk, a[1], a[2], topk=10, nthreads=cores)
+            else:
+                cpu_mod.agg_dict(fact, a[0], a[1], a[2], nthreads=cores)
+            done += 1
+            nbytes += b
+            if time.perf_counter() - t0 > 10.0 and len({k for k, _ in stream[:done]}) == 3:
+                break
+        el = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(done / el, 2), "unit": "queries/s", "cores": cores,
+                               "kind": "port", "gbs": round(nbytes / el / 1e9, 3),
+                               "sample": f"the first {done} queries of the stream, one after another",
+                               "cpu": cpu_model(), "cpus_in_affinity": host_cores()[1],
+                               "label": "C restatement of vexq's CPU operators"}
+    if cold is not None:
+        rec["dict_groupby_60m"] = cold
+    return rec
 
 
-def batch_bench(ctx, n, max_rows, steps=5):
+def batch_bench(ctx, n, max_rows, steps=5, cpu_mod=None, cores=1):
     """The batched workload (config 3): n independent instances (half hash
     GROUP BYs on u16/int64 keys of 4..1M groups, half joins of build:probe
     1:2..1:16, rows log-uniform in [10K, max_rows], Zipf(1.1) skew on every
@@ -276,14 +347,16 @@ def batch_bench(ctx, n, max_rows, steps=5):
     from vexq_amd import abi, datagen
     specs = datagen.batch_specs(n, max_rows=max_rows)
     log(f"[bench] batch: generating {n} instances, {sum(s['rows'] for s in specs) / 1e6:.1f}M rows")
-    made = []
+    made, hosts = [], []
     for sp in specs:
         if sp["kind"] == "hash":
             t = datagen.keyed_table(sp["rows"], sp["groups"], sp["key_type"], sp["skew"], sp["seed"])
             made.append(("hash", (ctx.upload(t),), datagen.q_keyed(t)))
+            hosts.append((t,))
         else:
             b, p = datagen.join_instance(sp["build"], sp["ratio"], sp["skew"], sp["seed"])
             made.append(("join", (ctx.upload(b), ctx.upload(p)), datagen.q_join_instance(b, p)))
+            hosts.append((b, p))
 
     def alone():
         res = []
@@ -332,7 +405,31 @@ def batch_bench(ctx, n, max_rows, steps=5):
     for _, dts, _ in made:
         for d in dts:
             d.close()
+    dev_ms = statistics.median(dev)
+    cpu = None
+    if cpu_mod is not None:
+        # the reference's CPU operators instance by instance, for about 10 s
+        done, nbytes, t0 = 0, 0, time.perf_counter()
+        for (kind, _, q), tabs, r in zip(made, hosts, ref):
+            if kind == "hash":
+                cpu_mod.agg_hash(tabs[0], q[0], q[1], q[2], nthreads=cores)
+            else:
+                cpu_mod.join_agg(tabs[0], 0, q[0], tabs[1], 0, q[1], q[2], nthreads=cores)
+            done += 1
+            nbytes += int(r.stats.bytes_alg)
+            if time.perf_counter() - t0 > 10.0:
+                break
+        el = time.perf_counter() - t0
+        cpu = {"value": round(nbytes / el / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+               "sample": f"the first {done} of the {n} instances, one after another",
+               "instances_per_s": round(done / el, 2), "cpu": cpu_model(),
+               "cpus_in_affinity": host_cores()[1], "label": "C restatement of vexq's CPU operators"}
     return {"unit": "ms per batch", "instances": n, "rows": rows, "bytes_alg": bytes_alg,
+            # the batch's algorithmic bytes over the captured graph's device
+            # time (HIP events around the replay on the slot's stream)
+            "roofline": roofline(bytes_alg / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else 0.0,
+                                 "every kernel of the 256-instance graph", dev_ms),
+            "cpu_baseline": cpu,
             "plain_launches_ms": round(plain_ms, 3), "batched_graph_ms": round(batch_ms, 3),
             "batched_graph_device_ms": round(statistics.median(dev), 3),
             "batched_no_graph_ms": round(nograph_ms, 3),
@@ -372,7 +469,7 @@ def main():
     cpu_mod = None
     if not args.no_cpu:
         from oracle import oracle_ffi as cpu_mod
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores, _ = host_cores()
     traffic = load_traffic()
     only = set(x for x in args.only.split(",") if x)
     K, W = args.steps, args.warmup
@@ -442,8 +539,8 @@ def main():
             r2, total, walls, mains, devs = time_gpu(filt, K, W, ctx)
             cpu = None
             if cpu_mod is not None:
-                runs = time_cpu(lambda: cpu_mod.filter_rows(ts, pred), max_runs=10)
-                cpu = cpu_record(runs, r2.stats.bytes_alg, 1, f"the full scan, {len(runs)} runs")
+                runs = time_cpu(lambda: cpu_mod.filter_rows(ts, pred, nthreads=cores), max_runs=10)
+                cpu = cpu_record(runs, r2.stats.bytes_alg, cores, f"the full scan, {len(runs)} runs")
             ops["filter_selvec"] = operator_record("filter_selvec", "vq_filter_bits", r2, total, walls,
                                                    mains, devs, K, traffic, cpu,
                                                    int(r2.stats.rows_scanned))
@@ -488,8 +585,8 @@ def main():
             lambda: ctx.agg_hash(dt, pred, key, aggs, out=ro), K, W)
         cpu = None
         if cpu_mod is not None:
-            runs = time_cpu(lambda: cpu_mod.agg_hash(t, pred, key, aggs), max_runs=3)
-            cpu = cpu_record(runs, r.stats.bytes_alg, 1, f"the full query, {len(runs)} runs, one thread")
+            runs = time_cpu(lambda: cpu_mod.agg_hash(t, pred, key, aggs, nthreads=cores), max_runs=10)
+            cpu = cpu_record(runs, r.stats.bytes_alg, cores, f"the full query, {len(runs)} runs")
         rec = operator_record(name, "hash GROUP BY pipeline (partitions, or per-workgroup partials)",
                               r, total, walls, mains, devs, K, traffic, cpu, t.nrows,
                               whole_pipeline=True)
@@ -500,10 +597,14 @@ def main():
         log(f"[bench] {name}: {rec['value']} GB/s ({rec['path']})")
 
     if want("batch_mixed"):
-        ops["batch_mixed"] = batch_bench(ctx, args.batch_instances, args.batch_max_rows)
+        ops["batch_mixed"] = batch_bench(ctx, args.batch_instances, args.batch_max_rows,
+                                         cpu_mod=cpu_mod, cores=cores)
         log(f"[bench] batch: {ops['batch_mixed']}")
     if want("serve_mixed"):
-        ops["serve_mixed"] = serve_bench(ctx, args.serve_scale, args.serve_queries)
+        ops["serve_mixed"] = serve_bench(ctx, args.serve_scale, args.serve_queries,
+                                         cpu_mod=cpu_mod, cores=cores)
+        if "dict_groupby_60m" in ops["serve_mixed"]:
+            ops["dict_groupby_60m"] = ops["serve_mixed"].pop("dict_groupby_60m")
 
     ctx.close()
     head_name = "dict_groupby_6m" if "dict_groupby_6m" in ops else next(
@@ -532,6 +633,9 @@ def main():
         "cpu_baseline": head.get("cpu_baseline"),
         "operators": ops,
     }
+    if "dict_groupby_60m" in ops:
+        # the same kernel over a table nine times the Infinity Cache
+        line["roofline_cache_cold"] = dict(ops["dict_groupby_60m"]["roofline"], rows=60_000_000)
     print(json.dumps(line), flush=True)
 
 

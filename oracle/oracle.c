@@ -433,6 +433,70 @@ int64_t or_filter(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t
   return n;
 }
 
+/* The same selection over row-group morsels claimed from an atomic cursor
+ * by nthreads workers (the morsel plan of exec/parallel.go:18-36,143-183):
+ * each morsel's passing rows go to that row group's own stretch of out_rows
+ * (its rows are ascending and every row group starts after the previous
+ * one's), then the stretches are closed up in row-group order, so the
+ * result is the serial one, ascending. */
+typedef struct filter_job {
+  const vq_col_desc* cols;
+  const vq_pred* pred;
+  const int64_t* rg_base;
+  const int64_t* rg_rows;
+  const uint8_t* keep;
+  int64_t* rg_count;
+  uint32_t* out_rows;
+  int64_t next, end; /* the shared cursor (first job's) */
+  struct filter_job* shared;
+} filter_job;
+
+static void* filter_worker(void* arg) {
+  filter_job* j = (filter_job*)arg;
+  for (;;) {
+    const int64_t g = __atomic_fetch_add(&j->shared->next, 1, __ATOMIC_RELAXED);
+    if (g >= j->shared->end) break;
+    int64_t n = 0;
+    if (j->keep[g])
+      for (int64_t r = j->rg_base[g]; r < j->rg_base[g] + j->rg_rows[g]; r++)
+        if (pred_pass(j->cols, j->pred, r)) j->out_rows[j->rg_base[g] + n++] = (uint32_t)r;
+    j->rg_count[g] = n;
+  }
+  return NULL;
+}
+
+int64_t or_filter_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                           const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                           int32_t nthreads, uint32_t* out_rows) {
+  (void)nrows;
+  if (nthreads < 1) nthreads = 1;
+  uint8_t* keep = (uint8_t*)malloc((size_t)nrg + 1);
+  or_zone_prune(ncols, cols, nrg, zones, pred, keep);
+  int64_t* rg_base = (int64_t*)malloc(sizeof(int64_t) * ((size_t)nrg + 1));
+  int64_t* rg_count = (int64_t*)calloc((size_t)nrg + 1, sizeof(int64_t));
+  rg_base[0] = 0;
+  for (int32_t g = 0; g < nrg; g++) rg_base[g + 1] = rg_base[g] + rg_rows[g];
+  filter_job* jobs = (filter_job*)calloc((size_t)nthreads, sizeof(filter_job));
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  jobs[0].end = nrg;
+  for (int i = 0; i < nthreads; i++) {
+    jobs[i] = (filter_job){cols, pred, rg_base, rg_rows, keep, rg_count, out_rows, 0, nrg, &jobs[0]};
+    pthread_create(&th[i], NULL, filter_worker, &jobs[i]);
+  }
+  for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
+  int64_t n = 0;
+  for (int32_t g = 0; g < nrg; g++) {
+    if (n != rg_base[g]) memmove(&out_rows[n], &out_rows[rg_base[g]], (size_t)rg_count[g] * 4);
+    n += rg_count[g];
+  }
+  free(th);
+  free(jobs);
+  free(rg_count);
+  free(rg_base);
+  free(keep);
+  return n;
+}
+
 /* ------------------------------------------------------------------------ */
 /* aggregate state                                                           */
 /* ------------------------------------------------------------------------ */

@@ -42,6 +42,11 @@ void or_zone_prune(int32_t ncols, const vq_col_desc* cols, int32_t nrg, const vq
 int64_t or_filter(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
                   const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
                   uint32_t* out_rows);
+/* The same selection by nthreads workers over row-group morsels (the CPU
+ * baseline); the result is the serial one. */
+int64_t or_filter_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
+                           const int64_t* rg_rows, const vq_zone* zones, const vq_pred* pred,
+                           int32_t nthreads, uint32_t* out_rows);
 
 /* Serial Scan->Filter->Project->HashAggregate over dictionary keys.
  * exact = 0: float sums are the reference's row-order left fold (bit pattern

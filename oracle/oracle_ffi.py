@@ -58,6 +58,9 @@ def lib():
     L.or_agg_hash.argtypes = [C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32, P(C.c_int64),
                               P(abi.Zone), P(abi.Pred), C.c_int32, C.c_int32, P(abi.Agg),
                               C.c_int32, P(abi.AggResult)]
+    L.or_filter_parallel.argtypes = [C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32, P(C.c_int64),
+                                     P(abi.Zone), P(abi.Pred), C.c_int32, P(C.c_uint32)]
+    L.or_filter_parallel.restype = C.c_int64
     L.or_agg_hash_parallel.argtypes = [C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32,
                                        P(C.c_int64), P(abi.Zone), P(abi.Pred), C.c_int32,
                                        C.c_int32, P(abi.Agg), C.c_int32, P(abi.AggResult)]
@@ -112,13 +115,19 @@ def zone_prune(table, zones, pred) -> np.ndarray:
     return keep[: len(table.rg_rows)]
 
 
-def filter_rows(table, pred, zones=None) -> np.ndarray:
+def filter_rows(table, pred, zones=None, nthreads: int = 0) -> np.ndarray:
+    """Serial selection vector, or (nthreads > 0) the morsel-parallel CPU
+    baseline of the same scan (the same rows)."""
     zones = zones if zones is not None else zone_maps(table)
     descs = abi.col_descs(table)
     rg, rgp = abi.rg_array(table)
     out = np.empty(max(table.nrows, 1), dtype=np.uint32)
-    n = lib().or_filter(table.nrows, len(table.columns), descs, len(rg), rgp, zones, _pp(pred),
-                        out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if nthreads:
+        n = lib().or_filter_parallel(table.nrows, len(table.columns), descs, len(rg), rgp, zones,
+                                     _pp(pred), nthreads, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    else:
+        n = lib().or_filter(table.nrows, len(table.columns), descs, len(rg), rgp, zones, _pp(pred),
+                            out.ctypes.data_as(C.POINTER(C.c_uint32)))
     return out[:n].copy()
 
 

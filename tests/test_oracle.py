@@ -399,6 +399,17 @@ def test_parallel_baseline_agrees():
         np.testing.assert_allclose(r.fvals, ref.fvals, rtol=1e-9)
 
 
+def test_parallel_filter_baseline_agrees():
+    """The morsel-parallel selection (the filter's CPU baseline) gives the
+    serial selection vector, ascending, with pruning, for odd row groups."""
+    t = datagen.lineitem_sorted(300_001, seed=3)
+    t = datagen.Table(t.columns, np.array([1003] * 299 + [300_001 - 1003 * 299], dtype=np.int64))
+    for w in range(3):
+        pred, _, _ = datagen.q_scan_sum(t, which=w)
+        np.testing.assert_array_equal(O.filter_rows(t, pred, nthreads=5), O.filter_rows(t, pred))
+    np.testing.assert_array_equal(O.filter_rows(t, None, nthreads=3), np.arange(t.nrows, dtype=np.uint32))
+
+
 def test_parallel_hash_baseline_agrees():
     """The morsel-parallel restatement of ParallelHashAggregate over one
     integer key (the hash GROUP BY's CPU baseline) against the serial one:
