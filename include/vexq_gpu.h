@@ -188,7 +188,9 @@ typedef struct vq_agg_result {
 /* Per-call measurements, filled when a non-NULL pointer is passed. */
 typedef struct vq_stats {
   double device_ms;      /* HIP-event time of the call's kernel sequence  */
-  double main_kernel_ms; /* HIP-event time of its dominant kernel         */
+  double main_kernel_ms; /* HIP-event time of its dominant kernel (the
+                            dictionary GROUP BY with kernel repeats 1: of
+                            its two kernels together)                   */
   int64_t rows_scanned;  /* rows in row groups that survived zone pruning */
   int64_t rg_total;
   int64_t rg_scanned;

@@ -103,7 +103,7 @@ struct Args {
   i32 nspans;
   i32 tile_rows;
   i64 ntiles;
-  u64* part;          // vq_dense: partials [grid][G][NA+2]
+  u64* part;          // vq_dense: partials [G][NA+2][grid] (item-major)
   u64* bits;          // vq_filter_bits: pass words
   u32* tile_counts;   // vq_filter_bits: per-tile counts
   // join partitioning
@@ -593,7 +593,10 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
     }
   }
   constexpr int NI = NA + 2;
-  u64* out = a.part + (u64)blockIdx.x * G * NI;
+  // partials item-major ([G * NI][grid]): k_agg_final's block for one item
+  // then reads the waves' words contiguously
+  u64* out = a.part + blockIdx.x;
+  const u64 ostride = gridDim.x;
   if constexpr (G > 1) {
     // the partials: lane q folds item q (group q / NI, accumulator, count
     // or first row q % NI) over the 64 lanes' columns, starting at column q
@@ -605,7 +608,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
       if (item < NA) v = fold_columns(kAOP[item], &lds[(gi * NA + item) * kWave], q);
       else if (item == NA) v = fold_counts<false>(&cnt[gi * kWave], q);
       else v = fold_counts<true>(&first[gi * kWave], q);
-      out[q] = v;
+      out[q * ostride] = v;
     }
     return;
   }
@@ -625,7 +628,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
         op = 101;
       }
       v = wave_reduce(op, v);
-      if (lane == 0) out[gi * NI + item] = v;
+      if (lane == 0) out[(gi * NI + item) * ostride] = v;
     }
   }
 }

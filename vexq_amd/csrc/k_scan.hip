@@ -481,7 +481,10 @@ __global__ __launch_bounds__(kWave) void k_agg_dense(const QDesc d, uint64_t* __
 
   // Lane combine (fixed butterfly), lane 0 writes the wave's partials.
   const int NI = NA + 2;
-  uint64_t* out = part + (size_t)blockIdx.x * G * NI;
+  // partials item-major ([G * NI][grid]), as the run-time compiled kernel
+  // writes them
+  uint64_t* out = part + blockIdx.x;
+  const size_t ostride = gridDim.x;
   for (int gi = 0; gi < G; gi++) {
     for (int item = 0; item < NI; item++) {
       uint64_t v;
@@ -498,7 +501,7 @@ __global__ __launch_bounds__(kWave) void k_agg_dense(const QDesc d, uint64_t* __
         else v = first[gi * kWave + lane];
       }
       v = wave_reduce(item_op(d, item), v);
-      if (lane == 0) out[gi * NI + item] = v;
+      if (lane == 0) out[(gi * NI + item) * ostride] = v;
     }
   }
 }
@@ -533,11 +536,11 @@ __global__ __launch_bounds__(256) void k_agg_final(const QDesc d, int32_t nparts
   const int gi = blockIdx.x / NI, item = blockIdx.x % NI;
   const int op = item_op(d, item);
   const int tid = threadIdx.x;
-  const int G = d.ngroups;
   uint64_t v = op == ITEM_COUNT ? 0 : (op == ITEM_FIRST ? 0xFFFFFFFFu : acc_init(op));
   DD dd = {0.0, 0.0};
+  const uint64_t* col = part + (size_t)blockIdx.x * nparts;  // item gi*NI + item, every wave
   for (int w = tid; w < nparts; w += 256) {
-    const uint64_t x = part[((size_t)w * G + gi) * NI + item];
+    const uint64_t x = col[w];
     if (op == ACC_SUM_F) dd = dd_add(dd, {bits_to_f64((int64_t)x), 0.0});
     else v = combine_item(op, v, x);
   }

@@ -67,6 +67,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   Arena arena;
   void* pinned = nullptr;
+  void* pinned_dev = nullptr;  // the device's address of `pinned` (null: copy instead)
   size_t pinned_bytes = 0;
   // full-output GROUP BY and join records on their way to the host (grown on
   // demand; a pageable copy of megabytes costs milliseconds): one for a query
@@ -184,9 +185,17 @@ int pinned_get(Slot* S, size_t bytes, void** out) {
   if (S->pinned_bytes < bytes) {
     if (S->pinned) HIP_TRY(hipHostFree(S->pinned));
     S->pinned = nullptr;
+    S->pinned_dev = nullptr;
     size_t want = std::max(bytes, (size_t)1 << 16);
-    HIP_TRY(hipHostMalloc(&S->pinned, want, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&S->pinned, want, hipHostMallocMapped));
     S->pinned_bytes = want;
+    // kernels may write small results here directly (VQ_NO_ZERO_COPY=1:
+    // always through a copy)
+    const char* e = getenv("VQ_NO_ZERO_COPY");
+    if (!(e && *e == '1') && hipHostGetDevicePointer(&S->pinned_dev, S->pinned, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      S->pinned_dev = nullptr;
+    }
   }
   *out = S->pinned;
   return VQ_OK;
@@ -960,13 +969,14 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   // tiles: an even share of the kept rows for each wave the run-time
   // compiled kernel keeps on the device (a step of 64 lanes x U rows never
   // crosses a tile), at least 1024 rows
+  DenseJit dj;
+  std::string jerr;
+  const bool jit = jit_dense_prepare(d, &dj, &jerr) == 0;
   {
-    std::string derr;
     const int64_t kr = kept_rows(t, keep);
-    const int slots = jit_dense_slots(d, &derr);
-    if (slots > 0 && kr >= (int64_t)1 << 20) {
+    if (jit && kr >= (int64_t)1 << 20) {
       const int64_t chunk = dense_chunk_rows(d.nslots);
-      const int64_t share = (kr + slots - 1) / slots;
+      const int64_t share = (kr + dj.slots - 1) / dj.slots;
       d.tile_rows = (int32_t)std::max<int64_t>(1024, (share + chunk - 1) / chunk * chunk);
     } else {
       d.tile_rows = scan_tile_rows(kr);
@@ -978,10 +988,8 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   SlotLease S(c);
   hipSetDevice(c->device);
   const int NI = d.nacc + 2;
-  std::string jerr;
-  int32_t grid = d.ntiles > 0 ? jit_dense_grid(d, &jerr) : 0;
-  const bool jit = grid >= 0;
-  if (!jit) grid = dense_grid(d);
+  int32_t grid = 0;
+  if (d.ntiles > 0) grid = jit ? (int32_t)std::min<int64_t>(d.ntiles, dj.slots) : dense_grid(d);
   const size_t part_words = (size_t)std::max<int32_t>(grid, 1) * d.ngroups * NI;
   const size_t out_words = (size_t)d.ngroups * NI;
   void* ar;
@@ -993,16 +1001,28 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   if (int rc = pinned_get(S.get(), out_words * 8, &hp)) return rc;
   uint64_t* hfin = (uint64_t*)hp;
 
+  // events: around the whole query when stats are asked for, and between
+  // the two kernels only when the main kernel is timed on its own (repeats
+  // above 1): an event between two launches delays the second by microseconds
+  const bool timed = stats != nullptr;
+  const bool split = timed && c->kernel_repeats > 1;
   if (d.ntiles > 0) {
-    HIP_TRY(hipEventRecord(S->ev[0], S->stream));
+    if (timed) HIP_TRY(hipEventRecord(S->ev[0], S->stream));
     for (int32_t rep = 0; rep < c->kernel_repeats; rep++) {  // identical partials each time
-      if (jit) HIP_TRY(jit_launch_dense(d, grid, part, S->stream));
+      if (jit) HIP_TRY(jit_dense_launch(dj, d, grid, part, S->stream));
       else HIP_TRY(launch_agg_dense(d, grid, part, S->stream));
     }
-    HIP_TRY(hipEventRecord(S->ev[1], S->stream));
-    HIP_TRY(launch_agg_final(d, grid, part, fin, S->stream));
-    HIP_TRY(hipEventRecord(S->ev[2], S->stream));
-    HIP_TRY(hipMemcpyAsync(hfin, fin, out_words * 8, hipMemcpyDeviceToHost, S->stream));
+    if (split) HIP_TRY(hipEventRecord(S->ev[1], S->stream));
+    // the final words straight into the slot's pinned host buffer when the
+    // device can address it (no copy command), else through a copy
+    if (S->pinned_dev) {
+      HIP_TRY(launch_agg_final(d, grid, part, (uint64_t*)S->pinned_dev, S->stream));
+      if (timed) HIP_TRY(hipEventRecord(S->ev[2], S->stream));
+    } else {
+      HIP_TRY(launch_agg_final(d, grid, part, fin, S->stream));
+      if (timed) HIP_TRY(hipEventRecord(S->ev[2], S->stream));
+      HIP_TRY(hipMemcpyAsync(hfin, fin, out_words * 8, hipMemcpyDeviceToHost, S->stream));
+    }
     HIP_TRY(hipStreamSynchronize(S->stream));
   } else {
     for (int g = 0; g < d.ngroups; g++)
@@ -1010,9 +1030,12 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   }
   if (stats) {
     memset(stats, 0, sizeof *stats);
-    if (d.ntiles > 0) {
+    if (d.ntiles > 0 && split) {
       stats->main_kernel_ms = elapsed(S->ev[0], S->ev[1]) / c->kernel_repeats;
       stats->device_ms = stats->main_kernel_ms + elapsed(S->ev[1], S->ev[2]);
+    } else if (d.ntiles > 0) {  // one launch: the query's two kernels together
+      stats->device_ms = elapsed(S->ev[0], S->ev[2]);
+      stats->main_kernel_ms = stats->device_ms;
     }
     stats->rows_scanned = rows;
     stats->rg_total = t->nrg;

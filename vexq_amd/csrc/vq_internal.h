@@ -181,12 +181,18 @@ hipError_t launch_filter(const QDesc& d, bool jit, uint64_t* bits, uint32_t* til
                          hipStream_t s, hipEvent_t ev_main0, hipEvent_t ev_main1,
                          int32_t repeats);
 
-// Shape-specialized kernels compiled at run time (vq_jit.hip).  jit_dense_grid
-// returns the persistent grid, or -1 (with *err) when no kernel is available.
-int jit_dense_grid(const QDesc& d, std::string* err);
-// The waves the device holds at once for d's dense kernel (-1: no JIT).
-int jit_dense_slots(const QDesc& d, std::string* err);
-hipError_t jit_launch_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s);
+// Shape-specialized kernels compiled at run time (vq_jit.hip).
+// jit_dense_prepare resolves d's dense kernel once per query: its function,
+// its LDS and the waves the device holds at once (slots); -1 (with *err)
+// when no kernel is available.
+struct DenseJit {
+  hipFunction_t fn = nullptr;
+  size_t lds = 0;
+  int slots = 0;
+};
+int jit_dense_prepare(const QDesc& d, DenseJit* out, std::string* err);
+hipError_t jit_dense_launch(const DenseJit& h, const QDesc& d, int32_t grid, uint64_t* part,
+                            hipStream_t s);
 bool jit_available_filter(const QDesc& d, std::string* err);
 hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,
                                   hipStream_t s);

@@ -9,9 +9,12 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -26,6 +29,7 @@ namespace {
 struct Module {
   hipModule_t mod = nullptr;
   hipFunction_t dense = nullptr;
+  std::atomic<int> dense_per_cu{0};  // occupancy of `dense` (0: not asked yet)
   hipFunction_t filter_bits = nullptr;
   hipFunction_t part_count = nullptr;
   hipFunction_t part_scatter = nullptr;
@@ -33,7 +37,6 @@ struct Module {
 };
 
 std::mutex g_mu;
-std::unordered_map<std::string, Module*> g_cache;
 
 bool is_float_op(int op) { return op == ACC_SUM_F || op == ACC_MIN_F || op == ACC_MAX_F; }
 
@@ -152,60 +155,148 @@ int fail_jit(std::string* err, const std::string& msg) {
   return -1;
 }
 
-// Compiles (or finds) the module of this shape.
-Module* get_module(const std::string& pro, std::string* err) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  auto it = g_cache.find(pro);
-  if (it != g_cache.end()) return it->second;
+// Compiled modules by prologue text.  A module is compiled outside the lock:
+// a thread that needs a module another thread is compiling waits for that
+// one only (its shared future), and queries of shapes already compiled never
+// wait behind a compile.
+struct Entry {
+  std::shared_future<Module*> mod;
+  std::string err;  // a failed compile's message (set before the future)
+};
+std::unordered_map<std::string, std::shared_ptr<Entry>> g_cache;
+
+// FNV-1a, 64 bits: the on-disk cache's file name for one source text.
+uint64_t fnv64(const std::string& x) {
+  uint64_t h = 0xcbf29ce484222325ULL;
+  for (unsigned char ch : x) {
+    h ^= ch;
+    h *= 0x100000001b3ULL;
+  }
+  return h;
+}
+
+// VQ_JIT_CACHE_DIR=<dir>: code objects also kept on disk, by a hash of the
+// whole source text and options, so a new process (a restarted server) loads
+// a shape's kernel instead of compiling it again.
+std::string disk_path(const std::string& src, const char* const* opts, int nopts) {
+  const char* dir = getenv("VQ_JIT_CACHE_DIR");
+  if (!dir || !*dir) return "";
+  std::string key = src;
+  for (int i = 0; i < nopts; i++) key += std::string("\n") + opts[i];
+  char name[40];
+  snprintf(name, sizeof name, "/vq_%016llx.co", (unsigned long long)fnv64(key));
+  return std::string(dir) + name;
+}
+
+bool read_file(const std::string& path, std::string& out) {
+  FILE* fp = fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  fseek(fp, 0, SEEK_END);
+  const long n = ftell(fp);
+  fseek(fp, 0, SEEK_SET);
+  bool ok = n > 0;
+  if (ok) {
+    out.resize((size_t)n);
+    ok = fread(&out[0], 1, (size_t)n, fp) == (size_t)n;
+  }
+  fclose(fp);
+  return ok;
+}
+
+void write_file(const std::string& path, const std::string& data) {
+  const std::string tmp = path + ".tmp" + std::to_string((unsigned long long)(uintptr_t)&data);
+  FILE* fp = fopen(tmp.c_str(), "wb");
+  if (!fp) return;
+  const bool ok = fwrite(data.data(), 1, data.size(), fp) == data.size();
+  fclose(fp);
+  if (!ok || rename(tmp.c_str(), path.c_str()) != 0) remove(tmp.c_str());
+}
+
+// Compiles the module of one prologue (or reads its code from the disk
+// cache) and resolves its kernels.
+Module* compile_module(const std::string& pro, std::string* err) {
   const std::string src = pro + kFixChunksSrc + kJitTemplateSrc;
   // VQ_JIT_DUMP=<dir>: each module's prologue (its shape) written there, so
   // the same kernel can be compiled and inspected offline
   if (const char* dir = getenv("VQ_JIT_DUMP")) {
-    const std::string path = std::string(dir) + "/module" + std::to_string(g_cache.size()) + ".pro";
+    const std::string path = std::string(dir) + "/module" + std::to_string(fnv64(pro)) + ".pro";
     if (FILE* fp = fopen(path.c_str(), "w")) {
       fwrite(pro.data(), 1, pro.size(), fp);
       fclose(fp);
     }
   }
-  hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "vq_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
-    if (err) *err = "hiprtcCreateProgram failed";
-    return nullptr;
-  }
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
                         "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang",
                         "-packed-fp32-ops"};
-  hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
-  if (rc != HIPRTC_SUCCESS) {
-    size_t ls = 0;
-    hiprtcGetProgramLogSize(prog, &ls);
-    std::string log(ls, '\0');
-    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  const int nopts = (int)(sizeof(opts) / sizeof(opts[0]));
+  const std::string path = disk_path(src, opts, nopts);
+  std::string code;
+  if (path.empty() || !read_file(path, code)) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "vq_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+      if (err) *err = "hiprtcCreateProgram failed";
+      return nullptr;
+    }
+    hiprtcResult rc = hiprtcCompileProgram(prog, nopts, opts);
+    if (rc != HIPRTC_SUCCESS) {
+      size_t ls = 0;
+      hiprtcGetProgramLogSize(prog, &ls);
+      std::string log(ls, '\0');
+      if (ls) hiprtcGetProgramLog(prog, &log[0]);
+      hiprtcDestroyProgram(&prog);
+      if (err) *err = "hipRTC compile failed: " + log.substr(0, 400);
+      return nullptr;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.assign(cs, '\0');
+    hiprtcGetCode(prog, &code[0]);
     hiprtcDestroyProgram(&prog);
-    if (err) *err = "hipRTC compile failed: " + log.substr(0, 400);
-    return nullptr;
+    if (!path.empty()) write_file(path, code);
   }
-  size_t cs = 0;
-  hiprtcGetCodeSize(prog, &cs);
-  std::string code(cs, '\0');
-  hiprtcGetCode(prog, &code[0]);
-  hiprtcDestroyProgram(&prog);
   Module* m = new Module();
   bool ok = hipModuleLoadData(&m->mod, code.data()) == hipSuccess;
-  if (ok && pro.find("#define VQ_KERNELS 3") != std::string::npos)
+  if (ok && pro.find("#define VQ_KERNELS 3\n") != std::string::npos)
     ok = hipModuleGetFunction(&m->dense, m->mod, "vq_dense") == hipSuccess &&
          hipModuleGetFunction(&m->filter_bits, m->mod, "vq_filter_bits") == hipSuccess;
-  if (ok && pro.find("#define VQ_KERNELS 4") != std::string::npos)
+  if (ok && pro.find("#define VQ_KERNELS 4\n") != std::string::npos)
     ok = hipModuleGetFunction(&m->part_count, m->mod, "vq_part_count") == hipSuccess &&
          hipModuleGetFunction(&m->part_scatter, m->mod, "vq_part_scatter") == hipSuccess;
-  if (ok && pro.find("#define VQ_KERNELS 8") != std::string::npos)
+  if (ok && pro.find("#define VQ_KERNELS 8\n") != std::string::npos)
     ok = hipModuleGetFunction(&m->hash_local, m->mod, "vq_hash_local") == hipSuccess;
   if (!ok) {
     delete m;
     if (err) *err = "hipModuleLoadData failed";
     return nullptr;
   }
-  g_cache[pro] = m;
+  return m;
+}
+
+// Compiles (or finds) the module of this shape.
+Module* get_module(const std::string& pro, std::string* err) {
+  std::shared_ptr<Entry> e;
+  std::promise<Module*> prom;
+  bool mine = false;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_cache.find(pro);
+    if (it != g_cache.end()) {
+      e = it->second;
+    } else {
+      e = std::make_shared<Entry>();
+      e->mod = prom.get_future().share();
+      g_cache.emplace(pro, e);
+      mine = true;
+    }
+  }
+  if (mine) {
+    std::string cerr;
+    Module* m = compile_module(pro, &cerr);
+    e->err = cerr;
+    prom.set_value(m);
+  }
+  Module* m = e->mod.get();
+  if (!m && err) *err = e->err.empty() ? "run-time compilation failed" : e->err;
   return m;
 }
 
@@ -391,37 +482,37 @@ hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_
   return launch(ll.fn, (unsigned)ll.grid, ll.lds, s, a, 512);
 }
 
-int jit_dense_slots(const QDesc& d, std::string* err) {
+int jit_dense_prepare(const QDesc& d, DenseJit* out, std::string* err) {
   if (jit_disabled()) return fail_jit(err, "JIT disabled");
   const Flat f = flatten_terms(d);
   Module* m = get_module(prologue(d, f), err);
   if (!m) return -1;
   const size_t lds = dense_lds_bytes(d);
-  const int cus = device_cus();
-  int per_cu = 0;
-  hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->dense, 64, lds);
-  if (per_cu <= 0) per_cu = 1;
+  // the waves per CU the kernel's occupancy allows (once per module: its
+  // shape fixes its LDS)
+  int per_cu = m->dense_per_cu.load();
+  if (per_cu <= 0) {
+    hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->dense, 64, lds);
+    if (per_cu <= 0) per_cu = 1;
+    m->dense_per_cu.store(per_cu);
+  }
   if (const char* e = getenv("VQ_WAVES_PER_CU")) {
     const int v = atoi(e);
     if (v > 0) per_cu = v;
   }
-  return cus * per_cu;
+  out->fn = m->dense;
+  out->lds = lds;
+  out->slots = device_cus() * per_cu;
+  return 0;
 }
 
-int jit_dense_grid(const QDesc& d, std::string* err) {
-  const int g = jit_dense_slots(d, err);
-  if (g < 0) return g;
-  return (int)(d.ntiles < g ? d.ntiles : g);
-}
-
-hipError_t jit_launch_dense(const QDesc& d, int32_t grid, uint64_t* part, hipStream_t s) {
+hipError_t jit_dense_launch(const DenseJit& h, const QDesc& d, int32_t grid, uint64_t* part,
+                            hipStream_t s) {
   const Flat f = flatten_terms(d);
-  Module* m = get_module(prologue(d, f), nullptr);
-  if (!m) return hipErrorInvalidDeviceFunction;
   JitArgs a;
   fill_common(d, f, a);
   a.part = part;
-  return launch(m->dense, (unsigned)grid, dense_lds_bytes(d), s, a);
+  return launch(h.fn, (unsigned)grid, h.lds, s, a);
 }
 
 bool jit_available_filter(const QDesc& d, std::string* err) {

@@ -258,15 +258,22 @@ class Context:
         self.last_stats = st
         return int(n)
 
-    def agg_dict(self, dt: "DeviceTable", pred, key_cols, aggs, capacity: int | None = None) -> abi.AggOut:
-        if capacity is None:  # at most one group per code tuple, and per row
-            space = 1
-            for k in key_cols:
-                space *= max(1, dt.table.columns[k].dict_size)
-            capacity = max(1, min(space, dt.nrows))
+    def agg_dict(self, dt: "DeviceTable", pred, key_cols, aggs, capacity: int | None = None,
+                 out: abi.AggOut | None = None) -> abi.AggOut:
+        """The dictionary-code GROUP BY.  `aggs`: a list of aggregates or an
+        array abi.agg_array made of them (built once, for a query run many
+        times); `out`: a result buffer of an earlier call to fill again."""
+        if out is None:
+            if capacity is None:  # at most one group per code tuple, and per row
+                space = 1
+                for k in key_cols:
+                    space *= max(1, dt.table.columns[k].dict_size)
+                capacity = max(1, min(space, dt.nrows))
+            out = abi.AggOut(capacity, len(key_cols), len(aggs))
+        elif out.nkeys != len(key_cols) or out.naggs != len(aggs):
+            raise ValueError("agg_dict: result buffer shaped for another query")
         keys = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
-        arr = abi.agg_array(aggs)
-        out = abi.AggOut(capacity, len(key_cols), len(aggs))
+        arr = aggs if isinstance(aggs, C.Array) else abi.agg_array(aggs)
         st = abi.Stats()
         _check(lib().vq_agg_dict(self.h, dt.h, _pp(pred), len(key_cols), keys, len(aggs), arr,
                                  C.byref(out.c), C.byref(st)), out)
