@@ -36,6 +36,9 @@ extern "C" {
 #define VQ_E_BUDGET 3      /* query exceeds the context's device-memory budget  */
 #define VQ_E_UNSUPPORTED 4 /* shape the GPU operators do not cover: fall back   */
 #define VQ_E_NOMEM 5       /* host allocation failure                           */
+#define VQ_E_CORRUPT 6     /* a .vxq checksum or structure does not verify
+                              (storage.ErrChecksum / ErrCorruptFooter /
+                              ErrBadMagic, storage/errors.go)                   */
 
 /* Column physical types as they sit in HBM.  Mirrors storage type codes
  * (storage/format.go:20-26) after the decode in exec/scan.go:294-409:
@@ -238,6 +241,32 @@ int vq_table_destroy(vq_table* t);
 int64_t vq_table_rows(const vq_table* t);
 /* Copy the zone maps (nrg * ncols entries, row-group-major) into out. */
 int vq_table_zones(const vq_table* t, vq_zone* out, int64_t cap);
+
+/* The bulk upload straight from a .vxq file image (the bytes of the file as
+ * storage.Writer wrote it, storage/writer.go): the footer is parsed and its
+ * checksum verified on the host (storage/reader.go:295-408), each STRING
+ * column's per-row-group dictionaries are read and merged into one global
+ * dictionary in first-seen order (getOrBuildRemap, exec/agg_intkey.go:
+ * 118-134); then the row-group sections are copied to the device in one
+ * transfer, and the device checks every block's CRC-32 (ColumnReader.NextBlock,
+ * storage/rowgroup.go:101-151) and decodes it into the resident columns
+ * (TableScan's decode, exec/scan.go:254-409), STRING codes remapped to the
+ * global codes and narrowed to VQ_U8 (up to 256 strings) or VQ_U16 (up to
+ * 65,536).  cols: the file columns to keep, in table-column order.  Returns
+ * VQ_E_CORRUPT naming the row group, column and offset of a block whose
+ * checksum fails; VQ_E_UNSUPPORTED for a BOOL column, a column with NULLs or
+ * more than 65,536 strings.  stats (may be NULL): device_ms of the check and
+ * decode kernels, bytes_alg the block bytes read plus the column bytes
+ * written. */
+int vq_table_upload_vxq(vq_ctx* ctx, const void* file, uint64_t file_len, int32_t ncols,
+                        const int32_t* cols, vq_table** out, vq_stats* stats);
+/* A column's type and dictionary size (0: not a dictionary column). */
+int vq_table_column(const vq_table* t, int32_t col, int32_t* type, int32_t* dict_size);
+/* The string of a global dictionary code of a table ingested from a .vxq
+ * image (Dictionary.Get, storage/codec.go:160-194): *str points at len bytes
+ * owned by the table, valid until it is destroyed. */
+int vq_table_dict_entry(const vq_table* t, int32_t col, int32_t code, const char** str,
+                        int64_t* len);
 
 /* Zone-map pruning alone: writes 1/0 per row group into keep (nrg entries).
  * Restates TableScan's zonePred check (exec/scan.go:211-214) with the

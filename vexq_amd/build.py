@@ -17,7 +17,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvexqgpu.so")
-SOURCES = ["vq_api.hip", "k_scan.hip", "k_join.hip", "vq_jit.hip"]
+SOURCES = ["vq_api.hip", "k_scan.hip", "k_join.hip", "vq_jit.hip", "vq_ingest.hip"]
 JIT_TEMPLATE = "jit_template.hip"
 JIT_HEADER = "jit_template_src.h"  # generated: the template as a string literal
 

@@ -136,6 +136,9 @@ struct vq_table {
   std::vector<int64_t> rg_rows, rg_start;
   std::vector<vq_zone> zones;  // nrg * ncols
   uint64_t bytes;
+  // the strings of each dictionary column's global codes (tables ingested
+  // from a .vxq image; empty for uploads of decoded columns)
+  std::vector<std::vector<std::string>> dicts;
 };
 
 struct vq_selvec {
@@ -156,6 +159,40 @@ bool reserve(vq_ctx* c, uint64_t bytes) {
 }
 
 void release(vq_ctx* c, uint64_t bytes) { c->in_use.fetch_sub(bytes); }
+
+}  // namespace
+
+namespace vq {
+// Shared with the other translation units of the library (vq_internal.h).
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+bool ctx_reserve(vq_ctx* c, uint64_t bytes) { return reserve(c, bytes); }
+void ctx_release(vq_ctx* c, uint64_t bytes) { release(c, bytes); }
+int ctx_device(const vq_ctx* c) { return c->device; }
+
+int table_adopt(vq_ctx* c, TableParts&& p, vq_table** out) {
+  vq_table* t = new (std::nothrow) vq_table();
+  if (!t) return fail(VQ_E_NOMEM, "exec: gpu: table allocation");
+  t->ctx = c;
+  t->nrows = p.nrows;
+  t->ncols = (int32_t)p.dev.size();
+  t->nrg = (int32_t)p.rg_rows.size();
+  t->bytes = p.bytes;
+  for (size_t i = 0; i < p.dev.size(); i++) t->cols.push_back(ColDev{p.types[i], p.dict_sizes[i], p.dev[i]});
+  t->rg_rows = std::move(p.rg_rows);
+  t->rg_start.resize(t->nrg);
+  int64_t st = 0;
+  for (int32_t g = 0; g < t->nrg; g++) {
+    t->rg_start[g] = st;
+    st += t->rg_rows[g];
+  }
+  t->zones = std::move(p.zones);
+  t->dicts = std::move(p.dicts);
+  *out = t;
+  return VQ_OK;
+}
+}  // namespace vq
+
+namespace {
 
 int arena_get(vq_ctx* c, Slot* S, size_t bytes, void** out) {
   bytes = (bytes + 255) & ~(size_t)255;
@@ -697,6 +734,27 @@ extern "C" int vq_table_zones(const vq_table* t, vq_zone* out, int64_t cap) {
   if (!t || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
   if (cap < (int64_t)t->zones.size()) return fail(VQ_E_ARG, "exec: gpu: zone buffer too small");
   memcpy(out, t->zones.data(), t->zones.size() * sizeof(vq_zone));
+  return VQ_OK;
+}
+
+extern "C" int vq_table_column(const vq_table* t, int32_t col, int32_t* type, int32_t* dict_size) {
+  if (!t || !type || !dict_size) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (col < 0 || col >= t->ncols) return fail(VQ_E_ARG, "exec: gpu: column %d out of range", col);
+  *type = t->cols[col].type;
+  *dict_size = t->cols[col].dict_size;
+  return VQ_OK;
+}
+
+extern "C" int vq_table_dict_entry(const vq_table* t, int32_t col, int32_t code, const char** str,
+                                   int64_t* len) {
+  if (!t || !str || !len) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (col < 0 || col >= t->ncols) return fail(VQ_E_ARG, "exec: gpu: column %d out of range", col);
+  if ((size_t)col >= t->dicts.size() || t->dicts[col].empty())
+    return fail(VQ_E_ARG, "exec: gpu: column %d carries no dictionary strings", col);
+  if (code < 0 || (size_t)code >= t->dicts[col].size())
+    return fail(VQ_E_ARG, "exec: gpu: code %d outside column %d's dictionary", code, col);
+  *str = t->dicts[col][code].data();
+  *len = (int64_t)t->dicts[col][code].size();
   return VQ_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/vexq_gpu.h"
 #include "fixpoint.h"
@@ -161,6 +162,29 @@ __host__ __device__ inline int64_t f64_to_bits(double d) {
   return u.i;
 }
 
+}  // namespace vq
+
+// Shared by the library's translation units (defined in vq_api.hip).
+namespace vq {
+// Sets this thread's vq_last_error text; returns code.
+int set_error(int code, const char* msg);
+bool ctx_reserve(vq_ctx* c, uint64_t bytes);
+void ctx_release(vq_ctx* c, uint64_t bytes);
+int ctx_device(const vq_ctx* c);
+// A table whose columns are already on the device (hipMalloc'd, padded to a
+// 256-byte multiple plus 256 bytes, `bytes` reserved on the context).
+struct TableParts {
+  int64_t nrows = 0;
+  std::vector<int32_t> types, dict_sizes;
+  std::vector<void*> dev;
+  std::vector<int64_t> rg_rows;
+  std::vector<vq_zone> zones;  // nrg * ncols, row-group-major
+  uint64_t bytes = 0;
+  std::vector<std::vector<std::string>> dicts;
+};
+// The table takes the columns and the reservation; never fails after
+// allocating it.
+int table_adopt(vq_ctx* c, TableParts&& p, vq_table** out);
 }  // namespace vq
 
 // Launchers (defined in the kernel translation units).

@@ -49,6 +49,10 @@ def lib():
     L.vq_ctx_bytes_in_use.restype = C.c_uint64
     L.vq_table_upload.argtypes = [C.c_void_p, C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32,
                                   P(C.c_int64), P(abi.Zone), P(C.c_void_p)]
+    L.vq_table_upload_vxq.argtypes = [C.c_void_p, C.c_char_p, C.c_uint64, C.c_int32, P(C.c_int32),
+                                      P(C.c_void_p), P(abi.Stats)]
+    L.vq_table_column.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32)]
+    L.vq_table_dict_entry.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_char_p), P(C.c_int64)]
     L.vq_table_destroy.argtypes = [C.c_void_p]
     L.vq_table_rows.argtypes = [C.c_void_p]
     L.vq_table_rows.restype = C.c_int64
@@ -124,6 +128,11 @@ class Context:
 
     def upload(self, table, zones=None) -> "DeviceTable":
         return DeviceTable(self, table, zones)
+
+    def upload_vxq(self, image: bytes, cols) -> "VxqTable":
+        """A table straight from a .vxq file image (vq_table_upload_vxq):
+        the file columns `cols` checked and decoded on the device."""
+        return VxqTable(self, image, cols)
 
     def set_slots(self, nslots: int, serve_budget: int = 0) -> None:
         """nslots concurrent query slots; vq_submit admission bound in bytes."""
@@ -386,3 +395,55 @@ class DeviceTable:
         keep = np.zeros(max(self.nrg, 1), dtype=np.uint8)
         _check(lib().vq_zone_prune(self.h, _pp(pred), keep.ctypes.data_as(C.POINTER(C.c_uint8))))
         return keep[: self.nrg]
+
+
+class _ColInfo:
+    def __init__(self, name, typ, dict_size):
+        self.name, self.type, self.dict_size = name, typ, dict_size
+
+
+class _Schema:
+    """The column types and dictionary sizes of a device table (what the
+    query helpers read from a host table)."""
+
+    def __init__(self, cols, rg_rows):
+        self.columns = cols
+        self.rg_rows = rg_rows
+
+    def index(self, name):
+        for i, c in enumerate(self.columns):
+            if c.name == name:
+                return i
+        raise KeyError(name)
+
+
+class VxqTable(DeviceTable):
+    """vq_table ingested from a .vxq image on the device."""
+
+    def __init__(self, ctx: Context, image: bytes, cols, names=None):
+        self.ctx = ctx
+        cols = list(cols)
+        arr = (C.c_int32 * max(1, len(cols)))(*cols)
+        h = C.c_void_p()
+        st = abi.Stats()
+        _check(lib().vq_table_upload_vxq(ctx.h, image, len(image), len(cols), arr, C.byref(h),
+                                         C.byref(st)))
+        self.h = h
+        self.stats = st
+        self.nrows = int(lib().vq_table_rows(h))
+        self.nrg = int(st.rg_total)
+        info = []
+        for i in range(len(cols)):
+            t, d = C.c_int32(), C.c_int32()
+            _check(lib().vq_table_column(h, i, C.byref(t), C.byref(d)))
+            info.append(_ColInfo(names[i] if names else f"c{cols[i]}", t.value, d.value))
+        self.table = _Schema(info, None)
+
+    def dictionary(self, col: int):
+        """The strings of column col's global codes, in code order."""
+        out = []
+        for code in range(self.table.columns[col].dict_size):
+            p, n = C.c_char_p(), C.c_int64()
+            _check(lib().vq_table_dict_entry(self.h, col, code, C.byref(p), C.byref(n)))
+            out.append(C.string_at(p, n.value).decode())
+        return out
