@@ -12,9 +12,11 @@
 // Host: the footer and the dictionaries (a few bytes per row group and
 // column) are parsed and checked here; the per-row-group remap tables are
 // built from them.  Device: k_vxq_check computes every block's CRC-32 (one
-// lane per block, slice-by-8 tables in LDS) and counts its valid rows;
-// k_vxq_decode (one workgroup per block) writes the block's values into the
-// resident column, codes through their row group's remap table.
+// wave per block: each lane the CRC of its own stretch of the block with
+// slice-by-8 tables in LDS, the stretches joined by GF(2) shifts) and counts
+// its valid rows; k_vxq_decode (one workgroup per block) writes the block's
+// values into the resident column, codes through their row group's remap
+// table.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -70,6 +72,39 @@ uint32_t crc32_host(const uint8_t* p, size_t n) {
   for (size_t i = 0; i < n; i++) c = T[0][(c ^ p[i]) & 0xFF] ^ (c >> 8);
   return c ^ 0xFFFFFFFFu;
 }
+
+// CRC combination (the polynomial arithmetic behind joining CRCs of pieces;
+// bit 31 is x^0, as the reflected register holds it):
+//   multmodp(a, b) = a(x) b(x) mod P(x);  x8n(n) = x^(8n) mod P(x).
+// The register after bytes M from a start value r is f(0, M) ^ r x^(8|M|),
+// so a block's CRC is the XOR over its pieces M_i of f(0, M_i) shifted by
+// the bytes after M_i, plus 0xFFFFFFFF shifted by the whole length, then
+// 0xFFFFFFFF.
+__host__ __device__ inline uint32_t multmodp(uint32_t a, uint32_t b) {
+  uint32_t p = 0;
+  for (int i = 0; i < 32; i++) {
+    if (a & (0x80000000u >> i)) p ^= b;
+    b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+  }
+  return p;
+}
+
+uint32_t x8n(uint64_t n) {
+  static const auto pow2 = [] {
+    std::vector<uint32_t> t(32);
+    t[0] = 0x40000000u;  // x^1
+    for (int k = 1; k < 32; k++) t[k] = multmodp(t[k - 1], t[k - 1]);  // x^(2^k)
+    return t;
+  }();
+  uint32_t p = 0x80000000u;  // x^0
+  for (int k = 3; n; n >>= 1, k++)
+    if (n & 1) p = multmodp(pow2[k & 31], p);
+  return p;
+}
+
+// Bytes of one lane's stretch of an n-byte block: a multiple of 8, the
+// block spread over the 64 lanes of a wave.
+__host__ __device__ inline uint32_t stretch_bytes(uint32_t n) { return 8 * ((n + 511) / 512); }
 
 uint16_t rd16(const uint8_t* p) { return (uint16_t)(p[0] | p[1] << 8); }
 uint32_t rd32(const uint8_t* p) { return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24; }
@@ -177,6 +212,8 @@ struct VxqBlock {
   uint32_t col;       // table column
   uint32_t remap;     // STRING: first entry of its row group's remap table
   uint32_t ncodes;    // STRING: codes in its row group's dictionary
+  uint32_t kidx;      // its row of CRC shift constants (one per block length)
+  uint32_t pad_;
 };
 
 struct VxqCol {
@@ -193,6 +230,9 @@ struct VxqArgs {
   const VxqCol* cols;
   const uint32_t* remap;
   const uint32_t* crc;  // the slice-by-8 tables, [8][256]
+  // per block length n: [64] lane l's stretch shifted past the bytes after
+  // it, x^(8 (n - end_l)); [64] the start value's shift, x^(8 n)
+  const uint32_t* kconst;
   uint32_t* flags;      // [0] bad checksums, [1] first bad block, [2] NULL rows, [3] bad codes
   int64_t nblocks;
 };
@@ -203,39 +243,66 @@ __device__ __forceinline__ uint32_t ldu32(const uint8_t* p) {
   return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
 }
 
-// Block checks: lane = block.  CRC-32 of [bitmap][values] against the 4
-// bytes after them (VerifyTrailing, internal/encoding/crc.go:31-40), and the
-// valid rows of the bitmap (bit i of byte i/8, LSB first; codec.go:9-25).
-__global__ __launch_bounds__(64) void k_vxq_check(const VxqArgs a) {
+// Block checks, one wave per block: the CRC-32 of [bitmap][values] against
+// the 4 bytes after them (VerifyTrailing, internal/encoding/crc.go:31-40),
+// and the valid rows of the bitmap (bit i of byte i/8, LSB first;
+// storage/codec.go:9-25).  Lane l takes bytes [l s, l s + s) of the block
+// (s = stretch_bytes), 8 at a time through the slice-by-8 tables; the lanes'
+// registers are shifted past the bytes after their stretches and XORed.
+constexpr int kCheckThreads = 256;
+
+__device__ __forceinline__ uint32_t ld32_any(const uint8_t* p, bool a4) {
+  return a4 ? *(const uint32_t*)p : ldu32(p);
+}
+
+__global__ __launch_bounds__(kCheckThreads) void k_vxq_check(const VxqArgs a) {
   __shared__ uint32_t T[8][256];
-  for (int i = threadIdx.x; i < 8 * 256; i += 64) (&T[0][0])[i] = a.crc[i];
+  for (int i = threadIdx.x; i < 8 * 256; i += kCheckThreads) (&T[0][0])[i] = a.crc[i];
   __syncthreads();
-  const int64_t b = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (b >= a.nblocks) return;
-  const VxqBlock k = a.blocks[b];
-  const VxqCol c = a.cols[k.col];
-  const uint8_t* p = a.img + k.src;
-  const uint64_t n = (uint64_t)kNullBitmapBytes + (uint64_t)k.rows * c.vs;
-  uint32_t crc = 0xFFFFFFFFu;
-  uint64_t i = 0;
-  for (; i + 8 <= n; i += 8) {
-    const uint32_t lo = ldu32(p + i) ^ crc, hi = ldu32(p + i + 4);
-    crc = T[7][lo & 0xFF] ^ T[6][(lo >> 8) & 0xFF] ^ T[5][(lo >> 16) & 0xFF] ^ T[4][lo >> 24] ^
-          T[3][hi & 0xFF] ^ T[2][(hi >> 8) & 0xFF] ^ T[1][(hi >> 16) & 0xFF] ^ T[0][hi >> 24];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t waves = (int64_t)gridDim.x * (kCheckThreads / 64);
+  for (int64_t b = (int64_t)blockIdx.x * (kCheckThreads / 64) + wv; b < a.nblocks; b += waves) {
+    const VxqBlock k = a.blocks[b];
+    const VxqCol c = a.cols[k.col];
+    const uint8_t* p = a.img + k.src;
+    const uint32_t n = (uint32_t)kNullBitmapBytes + k.rows * (uint32_t)c.vs;
+    const uint32_t sb = stretch_bytes(n);
+    const uint32_t s0 = lane * sb < n ? lane * sb : n;
+    const uint32_t s1 = s0 + sb < n ? s0 + sb : n;
+    const bool a4 = (((uintptr_t)p) & 3) == 0;  // (s0 is a multiple of 8)
+    uint32_t crc = 0, i = s0;
+    for (; i + 8 <= s1; i += 8) {
+      const uint32_t lo = ld32_any(p + i, a4) ^ crc, hi = ld32_any(p + i + 4, a4);
+      crc = T[7][lo & 0xFF] ^ T[6][(lo >> 8) & 0xFF] ^ T[5][(lo >> 16) & 0xFF] ^ T[4][lo >> 24] ^
+            T[3][hi & 0xFF] ^ T[2][(hi >> 8) & 0xFF] ^ T[1][(hi >> 16) & 0xFF] ^ T[0][hi >> 24];
+    }
+    for (; i < s1; i++) crc = T[0][(crc ^ ld8(p + i)) & 0xFF] ^ (crc >> 8);
+    const uint32_t* K = a.kconst + (size_t)k.kidx * 65;
+    uint32_t v = crc ? multmodp(K[lane], crc) : 0u;
+    // valid rows: lanes 0..15 take 8 bitmap bytes each
+    uint32_t valid = 0;
+    if (lane < 16)
+      for (int j = 0; j < 8; j++) {
+        const uint32_t q = lane * 8 + j, r0 = q * 8;
+        if (r0 >= k.rows) break;
+        uint32_t byte = p[q];
+        if (k.rows - r0 < 8) byte &= (1u << (k.rows - r0)) - 1u;
+        valid += (uint32_t)__popc(byte);
+      }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      v ^= (uint32_t)__shfl_xor((int)v, off);
+      valid += (uint32_t)__shfl_xor((int)valid, off);
+    }
+    if (lane == 0) {
+      const uint32_t total = v ^ multmodp(K[64], 0xFFFFFFFFu) ^ 0xFFFFFFFFu;
+      if (total != ldu32(p + n)) {
+        atomicAdd(&a.flags[0], 1u);
+        atomicMin(&a.flags[1], (uint32_t)b);
+      }
+      if (valid != k.rows) atomicAdd(&a.flags[2], k.rows - valid);
+    }
   }
-  for (; i < n; i++) crc = T[0][(crc ^ ld8(p + i)) & 0xFF] ^ (crc >> 8);
-  crc ^= 0xFFFFFFFFu;
-  if (crc != ldu32(p + n)) {
-    atomicAdd(&a.flags[0], 1u);
-    atomicMin(&a.flags[1], (uint32_t)b);
-  }
-  uint32_t valid = 0;
-  for (uint32_t r = 0; r < k.rows; r += 8) {
-    uint32_t byte = p[r >> 3];
-    if (k.rows - r < 8) byte &= (1u << (k.rows - r)) - 1u;
-    valid += (uint32_t)__popc(byte);
-  }
-  if (valid != k.rows) atomicAdd(&a.flags[2], k.rows - valid);
 }
 
 // Decode: one 256-thread workgroup per block, rows strided over the threads.
@@ -243,20 +310,21 @@ __global__ __launch_bounds__(256) void k_vxq_decode(const VxqArgs a) {
   const VxqBlock k = a.blocks[blockIdx.x];
   const VxqCol c = a.cols[k.col];
   const uint8_t* v = a.img + k.src + kNullBitmapBytes;
+  const bool a4 = (((uintptr_t)v) & 3) == 0;
   for (uint32_t r = threadIdx.x; r < k.rows; r += 256) {
     const int64_t row = k.dst_row + r;
     if (c.is_string) {
-      const uint32_t code = ldu32(v + 4ull * r);
+      const uint32_t code = ld32_any(v + 4ull * r, a4);
       uint32_t g = 0;
       if (code < k.ncodes) g = a.remap[k.remap + code];
       else atomicAdd(&a.flags[3], 1u);
       if (c.type == VQ_U8) ((uint8_t*)c.dst)[row] = (uint8_t)g;
       else ((uint16_t*)c.dst)[row] = (uint16_t)g;
     } else if (c.vs == 8) {
-      const uint64_t x = (uint64_t)ldu32(v + 8ull * r) | (uint64_t)ldu32(v + 8ull * r + 4) << 32;
+      const uint64_t x = (uint64_t)ld32_any(v + 8ull * r, a4) | (uint64_t)ld32_any(v + 8ull * r + 4, a4) << 32;
       ((uint64_t*)c.dst)[row] = x;
     } else {
-      ((uint32_t*)c.dst)[row] = ldu32(v + 4ull * r);
+      ((uint32_t*)c.dst)[row] = ld32_any(v + 4ull * r, a4);
     }
   }
 }
@@ -283,6 +351,8 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   std::vector<VxqCol> vcols(ncols);
   std::vector<VxqBlock> blocks;
   std::vector<uint32_t> remap;
+  std::vector<uint32_t> kconst;                    // CRC shift constants, 65 per block length
+  std::unordered_map<uint32_t, uint32_t> kidx_of;  // block length -> row of kconst
   tp.dicts.resize(ncols);
   for (int32_t i = 0; i < ncols; i++) {
     const int32_t fc = cols[i];
@@ -346,7 +416,15 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
         const uint64_t blen = (uint64_t)kNullBitmapBytes + (uint64_t)br * vs + 4;
         if (pos + blen > sec_end)
           return vfail(VQ_E_CORRUPT, "exec: gpu: vxq: blocks of rg=%zu col=%s overrun their section", g, f.name.c_str());
-        blocks.push_back(VxqBlock{pos, row0 + b0, br, (uint32_t)i, remap0, ncodes});
+        const uint32_t n = (uint32_t)(blen - 4);
+        auto kit = kidx_of.find(n);
+        if (kit == kidx_of.end()) {
+          kit = kidx_of.emplace(n, (uint32_t)kconst.size() / 65).first;
+          const uint32_t sb = stretch_bytes(n);
+          for (uint32_t l = 0; l < 64; l++) kconst.push_back(x8n(n - std::min(n, (l + 1) * sb)));
+          kconst.push_back(x8n(n));
+        }
+        blocks.push_back(VxqBlock{pos, row0 + b0, br, (uint32_t)i, remap0, ncodes, kit->second, 0});
         pos += blen;
       }
       row0 += r.rows;
@@ -390,7 +468,7 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   for (int32_t i = 0; i < ncols; i++) col_bytes += (((uint64_t)nrows * width(vcols[i].type) + 255) & ~255ULL) + 256;
   const auto& tabs = crc_tables();
   const uint64_t aux = blocks.size() * sizeof(VxqBlock) + ncols * sizeof(VxqCol) + remap.size() * 4 +
-                       sizeof(tabs.t) + 64 + 4 * 256;
+                       kconst.size() * 4 + sizeof(tabs.t) + 64 + 6 * 256;
   const uint64_t stage = ((body_end + 255) & ~255ULL) + aux;
   if (!ctx_reserve(c, col_bytes + stage))
     return vfail(VQ_E_BUDGET, "exec: gpu: vxq: %llu bytes of columns and staging exceed the device budget",
@@ -422,6 +500,7 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
     a.cols = (const VxqCol*)carve(ncols * sizeof(VxqCol));
     a.remap = (const uint32_t*)carve(std::max<size_t>(remap.size(), 1) * 4);
     a.crc = (const uint32_t*)carve(sizeof(tabs.t));
+    a.kconst = (const uint32_t*)carve(std::max<size_t>(kconst.size(), 1) * 4);
     a.flags = (uint32_t*)carve(64);
     a.nblocks = (int64_t)blocks.size();
     // the one bulk upload: the file's row-group sections as they lie on disk
@@ -432,12 +511,15 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
     if (e == hipSuccess && !remap.empty())
       e = hipMemcpyAsync((void*)a.remap, remap.data(), remap.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync((void*)a.crc, &tabs.t[0][0], sizeof(tabs.t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !kconst.empty())
+      e = hipMemcpyAsync((void*)a.kconst, kconst.data(), kconst.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(a.flags, hflags, sizeof hflags, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipEventRecord(e0, s);
     if (e == hipSuccess && !blocks.empty()) {
-      hipLaunchKernelGGL(k_vxq_check, dim3((unsigned)((blocks.size() + 63) / 64)), dim3(64), 0, s, a);
+      const size_t wgs = std::min<size_t>((blocks.size() + 3) / 4, (size_t)device_cus() * 8);
+      hipLaunchKernelGGL(k_vxq_check, dim3((unsigned)wgs), dim3(kCheckThreads), 0, s, a);
       hipLaunchKernelGGL(k_vxq_decode, dim3((unsigned)blocks.size()), dim3(256), 0, s, a);
       e = hipGetLastError();
     }
