@@ -537,13 +537,14 @@ def main():
                # the block checks and decode on the device (the image already in
                # HBM); wall time adds the host's footer/dictionary parse and the
                # image's PCIe transfer
-               "roofline": roofline(bytes_alg / (dev_ms * 1e-3) / 1e9, "k_vxq_check + k_vxq_decode", dev_ms),
+               "roofline": roofline(bytes_alg / (dev_ms * 1e-3) / 1e9, "k_vxq_ingest", dev_ms),
                "pcie_inclusive_gbs": round(len(img) / wall_s / 1e9, 2)}
         if cpu_mod is not None:
             s0 = time.perf_counter()
             vxq.decode(img, cols)
             runs = [time.perf_counter() - s0]
-            rec["cpu_baseline"] = {"value": round(len(img) / statistics.median(runs) / 1e9, 3),
+            # the same algorithmic bytes (blocks read, columns written) per second
+            rec["cpu_baseline"] = {"value": round(bytes_alg / statistics.median(runs) / 1e9, 3),
                                    "unit": "GB/s", "cores": 1, "kind": "port",
                                    "sample": "the whole image decoded and checked once",
                                    "cpu": cpu_model(), "cpus_in_affinity": host_cores()[1],

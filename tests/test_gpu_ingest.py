@@ -148,6 +148,57 @@ def test_corruption_is_refused(gpu_ctx):
     assert ctx.bytes_in_use == before
 
 
+def _blocks(img, vs_of):
+    """(row group, column, offset, CRC'd length) of every block in the image,
+    from its footer: [128-byte bitmap][rows x vs][CRC] back to back from the
+    start of each column section (storage/rowgroup.go:101-151)."""
+    _, rgs = vxq.read_footer(img)
+    out = []
+    for g, (_, rows, cols) in enumerate(rgs):
+        for ci, m in enumerate(cols):
+            pos = m["sec"]
+            for b0 in range(0, rows, 1024):
+                n = 128 + min(1024, rows - b0) * vs_of[ci]
+                out.append((g, ci, pos, n))
+                pos += n + 4
+    return out
+
+
+def test_every_block_edge_is_under_the_checksum(gpu_ctx):
+    """A flipped bit at each edge of a block -- its first bitmap byte, its
+    first and last value bytes, each byte of its stored CRC -- is refused and
+    named, for blocks starting at every offset mod 16 and of full and partial
+    length; the untouched image loads."""
+    print("[case] ingest block edges", flush=True)
+    rng = np.random.default_rng(35)
+    sizes = [1024 * 3 + 5, 999, 2048, 1, 3001, 17, 1030, 2500]
+    n = sum(sizes)
+    names = ["N", "R", "A", "xyz", "q"]
+    strs = [names[i] for i in rng.integers(0, len(names), n)]
+    fields = [("s", vxq.T_STRING), ("v", vxq.T_INT64), ("d", vxq.T_DATE), ("f", vxq.T_FLOAT64)]
+    cols = [strs, rng.integers(-2**62, 2**62, n), rng.integers(0, 20000, n).astype(np.int32),
+            rng.standard_normal(n)]
+    img = vxq.write_vxq(fields, cols, sizes)
+    vt = gpu_ctx.upload_vxq(img, [0, 1, 2, 3])
+    vt.close()
+    fnames = [f[0] for f in fields]
+    blocks = _blocks(img, [4, 8, 4, 8])
+    picked, seen = [], set()
+    for b in blocks:  # one block per (offset mod 16, full or partial)
+        key = (b[2] % 16, b[3] == 128 + 1024 * (8 if b[1] in (1, 3) else 4))
+        if key not in seen:
+            seen.add(key)
+            picked.append(b)
+    assert len({b[2] % 16 for b in picked}) == 16
+    before = gpu_ctx.bytes_in_use
+    for g, ci, pos, ln in picked:
+        for at in (0, 127, 128, ln - 1, ln, ln + 1, ln + 2, ln + 3):
+            bad = bytearray(img)
+            bad[pos + at] ^= 1 << (at % 8)
+            _expect_corrupt(gpu_ctx, bytes(bad), [0, 1, 2, 3], f"crc block rg={g} col={fnames[ci]} off={pos}")
+    assert gpu_ctx.bytes_in_use == before
+
+
 def test_nulls_and_bool_are_unsupported(gpu_ctx):
     print("[case] ingest nulls", flush=True)
     n = 5000

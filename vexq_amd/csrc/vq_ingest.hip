@@ -11,12 +11,11 @@
 //
 // Host: the footer and the dictionaries (a few bytes per row group and
 // column) are parsed and checked here; the per-row-group remap tables are
-// built from them.  Device: k_vxq_check computes every block's CRC-32 (one
-// wave per block: each lane the CRC of its own stretch of the block with
-// slice-by-8 tables in LDS, the stretches joined by GF(2) shifts) and counts
-// its valid rows; k_vxq_decode (one workgroup per block) writes the block's
-// values into the resident column, codes through their row group's remap
-// table.
+// built from them.  Device: k_vxq_ingest reads every block once (one wave
+// per block, coalesced 16-byte loads), computes its CRC-32 with slice-by-8
+// tables in LDS (the lanes' lines joined by GF(2) shifts), counts its valid
+// rows and writes its values into the resident column, codes through their
+// row group's remap table.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -102,9 +101,23 @@ uint32_t x8n(uint64_t n) {
   return p;
 }
 
-// Bytes of one lane's stretch of an n-byte block: a multiple of 8, the
-// block spread over the 64 lanes of a wave.
-__host__ __device__ inline uint32_t stretch_bytes(uint32_t n) { return 8 * ((n + 511) / 512); }
+// The device's tables: the slice-by-8 tables, then x^8192 mod P times each
+// byte value at each of the register's four byte positions.
+struct DeviceCrcTables {
+  uint32_t t[12][256];
+  uint32_t lshift[64];
+  DeviceCrcTables() {
+    memcpy(t, crc_tables().t, sizeof crc_tables().t);
+    const uint32_t x1024 = x8n(1024);
+    for (int k = 0; k < 4; k++)
+      for (uint32_t b = 0; b < 256; b++) t[8 + k][b] = multmodp(x1024, b << (8 * k));
+    for (int l = 0; l < 64; l++) lshift[l] = x8n(1008 - 16 * l);
+  }
+};
+const DeviceCrcTables& device_crc_tables() {
+  static const DeviceCrcTables tabs;
+  return tabs;
+}
 
 uint16_t rd16(const uint8_t* p) { return (uint16_t)(p[0] | p[1] << 8); }
 uint32_t rd32(const uint8_t* p) { return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24; }
@@ -205,127 +218,213 @@ int parse_dict(const uint8_t* p, uint64_t n, std::vector<std::string>& out) {
 }
 
 // ---- device side --------------------------------------------------------------
-struct VxqBlock {
-  uint64_t src;       // byte offset of the block in the image
-  int64_t dst_row;    // its first row in the table
-  uint32_t rows;
-  uint32_t col;       // table column
-  uint32_t remap;     // STRING: first entry of its row group's remap table
-  uint32_t ncodes;    // STRING: codes in its row group's dictionary
-  uint32_t kidx;      // its row of CRC shift constants (one per block length)
-  uint32_t pad_;
-};
+// What a block's values become in the resident column
+enum : uint32_t { kCopy32 = 0, kCopy64 = 1, kCodeU8 = 2, kCodeU16 = 3 };
 
-struct VxqCol {
-  void* dst;
-  int32_t type;  // VQ_* of the resident column
-  int32_t vs;    // bytes per value in the file
-  int32_t is_string;
-  int32_t pad_;
+struct VxqBlock {
+  uint64_t src;     // byte offset of the block in the image
+  uint64_t dst;     // device address of its first value in the column
+  uint32_t rows;
+  uint32_t kind;    // kCopy32 .. kCodeU16
+  uint32_t remap;   // codes: first entry of its row group's remap table
+  uint32_t ncodes;  // codes: entries of its row group's dictionary
+  uint32_t shift;   // x^(8t): t zero bytes follow the block in its window
+  uint32_t initc;   // x^(8n) 0xFFFFFFFF ^ 0xFFFFFFFF: the start and end values' share
 };
 
 struct VxqArgs {
   const uint8_t* img;
   const VxqBlock* blocks;
-  const VxqCol* cols;
   const uint32_t* remap;
-  const uint32_t* crc;  // the slice-by-8 tables, [8][256]
-  // per block length n: [64] lane l's stretch shifted past the bytes after
-  // it, x^(8 (n - end_l)); [64] the start value's shift, x^(8 n)
-  const uint32_t* kconst;
-  uint32_t* flags;      // [0] bad checksums, [1] first bad block, [2] NULL rows, [3] bad codes
+  const uint32_t* crc;     // [12][256]: the slice-by-8 tables, then the 1024-byte shift's byte tables
+  const uint32_t* lshift;  // [64]: lane l's register moved to its window's end, x^(8 (1008 - 16 l))
+  uint32_t* flags;         // [0] bad checksums, [1] first bad block, [2] NULL rows, [3] bad codes
   int64_t nblocks;
 };
 
-__device__ __forceinline__ uint32_t ld8(const uint8_t* p) { return p[0]; }
-// 4 bytes at any byte address (sections after a dictionary are unaligned)
-__device__ __forceinline__ uint32_t ldu32(const uint8_t* p) {
-  return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+// One wave per block.  A block's window is the block rounded out to 16-byte
+// lines: [d masked bytes][bitmap][rows x vs][CRC], d = its offset mod 16, in
+// L = 5 or 9 lines of 1024 bytes (64 lanes x 16 bytes).  Lane l loads lines
+// l, l + 64, l + 128, ... (coalesced 16-byte loads, every byte of the image
+// read once) and the wave keeps the window in LDS.  The waves are a
+// software pipeline: a block's lines are loaded while the wave checks and
+// decodes the block before it.
+//
+// CRC-32 (VerifyTrailing, internal/encoding/crc.go:31-40) by linearity: lane
+// l's register R starts at 0 and, for each of its lines, moves 1024 bytes on
+// (its line and the other lanes' 63 lines; x^8192 mod P applied through four
+// byte tables) and takes the line's own register from zero (two slice-by-8
+// steps that do not depend on R, so a lane's lines overlap).  R is then
+// moved to the end of the window (x^(8 (1008 - 16 l))) and the lanes'
+// registers XORed: the register of the whole window, with the bytes outside
+// [bitmap, CRC) masked to zero, equals x^(8t) times the block's register from
+// zero, t = 1024 L - d - n for a block of n bytes under the CRC.  The block
+// passes when x^(8t) (stored ^ x^(8n) 0xFFFFFFFF ^ 0xFFFFFFFF) equals it.
+// The bitmap's valid rows are counted (bit i of byte i/8, LSB first;
+// storage/codec.go:9-25) and the values decoded from LDS into the resident
+// column, codes through their row group's remap table.
+constexpr int kIngestThreads = 256;
+constexpr int kMaxLines = 9;    // 16-byte lines per lane
+constexpr int kShortLines = 5;  // enough for a block of 4-byte values
+constexpr int kRowIters = kBlockRows / 64;
+static_assert(kMaxLines * 1024 >= 15 + kNullBitmapBytes + kBlockRows * 8 + 4, "a block's window fits");
+static_assert(kShortLines * 1024 >= 15 + kNullBitmapBytes + kBlockRows * 4, "4-byte blocks fit the short window");
+
+// lines of 1024 bytes in the window of a block of n bytes under the CRC at
+// offset d mod 16
+__host__ __device__ inline uint32_t window_lines(uint32_t d, uint32_t n) {
+  return d + n <= 1024u * kShortLines ? kShortLines : kMaxLines;
 }
 
-// Block checks, one wave per block: the CRC-32 of [bitmap][values] against
-// the 4 bytes after them (VerifyTrailing, internal/encoding/crc.go:31-40),
-// and the valid rows of the bitmap (bit i of byte i/8, LSB first;
-// storage/codec.go:9-25).  Lane l takes bytes [l s, l s + s) of the block
-// (s = stretch_bytes), 8 at a time through the slice-by-8 tables; the lanes'
-// registers are shifted past the bytes after their stretches and XORed.
-constexpr int kCheckThreads = 256;
-
-__device__ __forceinline__ uint32_t ld32_any(const uint8_t* p, bool a4) {
-  return a4 ? *(const uint32_t*)p : ldu32(p);
+// 4 bytes at any offset of a wave's LDS window (the dword after the read
+// must lie inside the window's array)
+__device__ __forceinline__ uint32_t win32(const uint32_t* w, uint32_t off) {
+  const uint32_t i = off >> 2;
+  return __builtin_amdgcn_alignbyte(w[i + 1], w[i], off & 3);
 }
 
-__global__ __launch_bounds__(kCheckThreads) void k_vxq_check(const VxqArgs a) {
-  __shared__ uint32_t T[8][256];
-  for (int i = threadIdx.x; i < 8 * 256; i += kCheckThreads) (&T[0][0])[i] = a.crc[i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t waves = (int64_t)gridDim.x * (kCheckThreads / 64);
-  for (int64_t b = (int64_t)blockIdx.x * (kCheckThreads / 64) + wv; b < a.nblocks; b += waves) {
-    const VxqBlock k = a.blocks[b];
-    const VxqCol c = a.cols[k.col];
-    const uint8_t* p = a.img + k.src;
-    const uint32_t n = (uint32_t)kNullBitmapBytes + k.rows * (uint32_t)c.vs;
-    const uint32_t sb = stretch_bytes(n);
-    const uint32_t s0 = lane * sb < n ? lane * sb : n;
-    const uint32_t s1 = s0 + sb < n ? s0 + sb : n;
-    const bool a4 = (((uintptr_t)p) & 3) == 0;  // (s0 is a multiple of 8)
-    uint32_t crc = 0, i = s0;
-    for (; i + 8 <= s1; i += 8) {
-      const uint32_t lo = ld32_any(p + i, a4) ^ crc, hi = ld32_any(p + i + 4, a4);
-      crc = T[7][lo & 0xFF] ^ T[6][(lo >> 8) & 0xFF] ^ T[5][(lo >> 16) & 0xFF] ^ T[4][lo >> 24] ^
-            T[3][hi & 0xFF] ^ T[2][(hi >> 8) & 0xFF] ^ T[1][(hi >> 16) & 0xFF] ^ T[0][hi >> 24];
+// v's bytes at window offsets [o, o + 4) kept where they lie in [lo, hi)
+__device__ __forceinline__ uint32_t keep_bytes(uint32_t v, int o, int lo, int hi) {
+  const int a = min(max(lo - o, 0), 4), b = min(max(hi - o, 0), 4);
+  return v & (uint32_t)(((1ull << (8 * b)) - 1) & ~((1ull << (8 * a)) - 1));
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void load_lines(const VxqArgs& a, const VxqBlock& k, uint32_t vs, int lane, uint4* v) {
+  const uint32_t d = (uint32_t)(k.src & 15);
+  const uint8_t* base = a.img + (k.src - d);
+  const uint32_t wlen = d + kNullBitmapBytes + k.rows * vs + 4;
+#pragma unroll
+  for (int j = 0; j < kMaxLines; j++) {
+    const uint32_t q = 16u * (lane + 64 * j);
+    v[j] = q < wlen ? *(const uint4*)(base + q) : make_uint4(0, 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ uint32_t slice8(const uint32_t (*T)[256], uint32_t lo, uint32_t hi) {
+  return T[7][lo & 0xFF] ^ T[6][(lo >> 8) & 0xFF] ^ T[5][(lo >> 16) & 0xFF] ^ T[4][lo >> 24] ^
+         T[3][hi & 0xFF] ^ T[2][(hi >> 8) & 0xFF] ^ T[1][(hi >> 16) & 0xFF] ^ T[0][hi >> 24];
+}
+
+// the lane's register over L lines of the window, moved to the window's end
+template <int L>
+__device__ __forceinline__ uint32_t lane_crc(const uint32_t (*T)[256], const uint4* win, int lane, uint32_t d,
+                                             uint32_t n, uint32_t wlen, uint32_t lshift) {
+  uint32_t raw[L];
+#pragma unroll
+  for (int j = 0; j < L; j++) {
+    const int q = 16 * (lane + 64 * j);
+    uint4 x = 1024u * j < wlen ? win[lane + 64 * j] : make_uint4(0, 0, 0, 0);
+    if ((j == 0 && d) || 1024u * (j + 1) > d + n) {  // the line holds an edge of the block
+      x.x = keep_bytes(x.x, q, d, d + n);
+      x.y = keep_bytes(x.y, q + 4, d, d + n);
+      x.z = keep_bytes(x.z, q + 8, d, d + n);
+      x.w = keep_bytes(x.w, q + 12, d, d + n);
     }
-    for (; i < s1; i++) crc = T[0][(crc ^ ld8(p + i)) & 0xFF] ^ (crc >> 8);
-    const uint32_t* K = a.kconst + (size_t)k.kidx * 65;
-    uint32_t v = crc ? multmodp(K[lane], crc) : 0u;
-    // valid rows: lanes 0..15 take 8 bitmap bytes each
-    uint32_t valid = 0;
-    if (lane < 16)
-      for (int j = 0; j < 8; j++) {
-        const uint32_t q = lane * 8 + j, r0 = q * 8;
-        if (r0 >= k.rows) break;
-        uint32_t byte = p[q];
-        if (k.rows - r0 < 8) byte &= (1u << (k.rows - r0)) - 1u;
-        valid += (uint32_t)__popc(byte);
+    raw[j] = slice8(T, x.z ^ slice8(T, x.x, x.y), x.w);
+  }
+  uint32_t R = 0;
+#pragma unroll
+  for (int j = 0; j < L; j++) R = T[8][R & 0xFF] ^ T[9][(R >> 8) & 0xFF] ^ T[10][(R >> 16) & 0xFF] ^ T[11][R >> 24] ^ raw[j];
+  return R ? multmodp(lshift, R) : 0u;
+}
+
+// blocks: the descriptors, read-only and wave-uniform (scalar loads)
+__global__ __launch_bounds__(kIngestThreads, 3) void k_vxq_ingest(const VxqArgs a, const VxqBlock* __restrict__ blocks) {
+  __shared__ uint32_t T[12][256];
+  __shared__ uint4 W[kIngestThreads / 64][kMaxLines * 64 + 1];
+  for (int i = threadIdx.x; i < 12 * 256; i += kIngestThreads) (&T[0][0])[i] = a.crc[i];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lshift = a.lshift[lane];
+  __syncthreads();
+  uint4* win = W[wv];
+  const uint32_t* w32 = (const uint32_t*)win;
+  const int64_t waves = (int64_t)gridDim.x * (kIngestThreads / 64);
+  int64_t b = (int64_t)blockIdx.x * (kIngestThreads / 64) + wv;
+  if (b >= a.nblocks) return;
+  VxqBlock k = blocks[b];
+  uint4 v[kMaxLines];
+  load_lines(a, k, k.kind == kCopy64 ? 8 : 4, lane, v);
+  for (;;) {
+    const int64_t nb = b + waves;
+    const bool more = nb < a.nblocks;
+    VxqBlock kn = k;
+    if (more) kn = blocks[nb];
+    const uint32_t vs = k.kind == kCopy64 ? 8 : 4;
+    const uint32_t d = (uint32_t)(k.src & 15), n = (uint32_t)kNullBitmapBytes + k.rows * vs, wlen = d + n + 4;
+    wave_sync();  // the previous block's reads of the window are done
+#pragma unroll
+    for (int j = 0; j < kMaxLines; j++)
+      if (1024u * j < wlen) win[lane + 64 * j] = v[j];
+    wave_sync();  // the window is in LDS
+    // decode: every lane reads its 16 rows (in the window's array whatever
+    // the block's length), stores those below the block's rows
+    const uint32_t v0 = d + (uint32_t)kNullBitmapBytes;
+    if (k.kind == kCopy64) {
+      uint64_t x[kRowIters];
+#pragma unroll
+      for (int i = 0; i < kRowIters; i++) {
+        const uint32_t o = v0 + 8u * (lane + 64 * i);
+        x[i] = (uint64_t)win32(w32, o) | (uint64_t)win32(w32, o + 4) << 32;
       }
 #pragma unroll
+      for (int i = 0; i < kRowIters; i++)
+        if (lane + 64u * i < k.rows) ((uint64_t*)k.dst)[lane + 64 * i] = x[i];
+    } else if (k.kind == kCopy32) {
+      uint32_t x[kRowIters];
+#pragma unroll
+      for (int i = 0; i < kRowIters; i++) x[i] = win32(w32, v0 + 4u * (lane + 64 * i));
+#pragma unroll
+      for (int i = 0; i < kRowIters; i++)
+        if (lane + 64u * i < k.rows) ((uint32_t*)k.dst)[lane + 64 * i] = x[i];
+    } else {
+      uint32_t g[kRowIters], bad = 0;
+#pragma unroll
+      for (int i = 0; i < kRowIters; i++) {
+        const uint32_t code = win32(w32, v0 + 4u * (lane + 64 * i));
+        const bool live = lane + 64u * i < k.rows;
+        bad += live && code >= k.ncodes;
+        g[i] = a.remap[k.remap + (code < k.ncodes ? code : 0u)];  // the table ends in a spare entry
+      }
+#pragma unroll
+      for (int i = 0; i < kRowIters; i++)
+        if (lane + 64u * i < k.rows) {
+          if (k.kind == kCodeU8) ((uint8_t*)k.dst)[lane + 64 * i] = (uint8_t)g[i];
+          else ((uint16_t*)k.dst)[lane + 64 * i] = (uint16_t)g[i];
+        }
+      if (bad) atomicAdd(&a.flags[3], bad);
+    }
+    // the next block's lines, in flight while this one is checked
+    if (more) load_lines(a, kn, kn.kind == kCopy64 ? 8 : 4, lane, v);
+    uint32_t valid = 0;
+    if (lane < kNullBitmapBytes / 4 && 32u * lane < k.rows) {
+      uint32_t bits = win32(w32, d + 4 * lane);
+      if (k.rows - 32u * lane < 32) bits &= (1u << (k.rows - 32u * lane)) - 1u;
+      valid = (uint32_t)__popc(bits);
+    }
+    uint32_t x = window_lines(d, n) == kShortLines ? lane_crc<kShortLines>(T, win, lane, d, n, wlen, lshift)
+                                                    : lane_crc<kMaxLines>(T, win, lane, d, n, wlen, lshift);
+#pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-      v ^= (uint32_t)__shfl_xor((int)v, off);
+      x ^= (uint32_t)__shfl_xor((int)x, off);
       valid += (uint32_t)__shfl_xor((int)valid, off);
     }
     if (lane == 0) {
-      const uint32_t total = v ^ multmodp(K[64], 0xFFFFFFFFu) ^ 0xFFFFFFFFu;
-      if (total != ldu32(p + n)) {
+      if (multmodp(k.shift, win32(w32, d + n) ^ k.initc) != x) {
         atomicAdd(&a.flags[0], 1u);
         atomicMin(&a.flags[1], (uint32_t)b);
       }
       if (valid != k.rows) atomicAdd(&a.flags[2], k.rows - valid);
     }
-  }
-}
-
-// Decode: one 256-thread workgroup per block, rows strided over the threads.
-__global__ __launch_bounds__(256) void k_vxq_decode(const VxqArgs a) {
-  const VxqBlock k = a.blocks[blockIdx.x];
-  const VxqCol c = a.cols[k.col];
-  const uint8_t* v = a.img + k.src + kNullBitmapBytes;
-  const bool a4 = (((uintptr_t)v) & 3) == 0;
-  for (uint32_t r = threadIdx.x; r < k.rows; r += 256) {
-    const int64_t row = k.dst_row + r;
-    if (c.is_string) {
-      const uint32_t code = ld32_any(v + 4ull * r, a4);
-      uint32_t g = 0;
-      if (code < k.ncodes) g = a.remap[k.remap + code];
-      else atomicAdd(&a.flags[3], 1u);
-      if (c.type == VQ_U8) ((uint8_t*)c.dst)[row] = (uint8_t)g;
-      else ((uint16_t*)c.dst)[row] = (uint16_t)g;
-    } else if (c.vs == 8) {
-      const uint64_t x = (uint64_t)ld32_any(v + 8ull * r, a4) | (uint64_t)ld32_any(v + 8ull * r + 4, a4) << 32;
-      ((uint64_t*)c.dst)[row] = x;
-    } else {
-      ((uint32_t*)c.dst)[row] = ld32_any(v + 4ull * r, a4);
-    }
+    if (!more) break;
+    k = kn;
+    b = nb;
   }
 }
 
@@ -348,11 +447,20 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   if (nrows > 0xFFFFFFFELL) return vfail(VQ_E_UNSUPPORTED, "exec: gpu: vxq: more than 2^32 rows");
   TableParts tp;
   tp.nrows = nrows;
-  std::vector<VxqCol> vcols(ncols);
+  struct OutCol {
+    int type = 0, vs = 0;
+    bool is_string = false;
+  };
+  struct BlockAt {
+    int32_t col;
+    int64_t row;
+  };
+  std::vector<OutCol> vcols(ncols);
   std::vector<VxqBlock> blocks;
+  std::vector<BlockAt> where;  // each block's column and first row
   std::vector<uint32_t> remap;
-  std::vector<uint32_t> kconst;                    // CRC shift constants, 65 per block length
-  std::unordered_map<uint32_t, uint32_t> kidx_of;  // block length -> row of kconst
+  std::vector<uint32_t> tshift(1024 * kMaxLines, 0);  // x^(8t) for the t zero bytes ending a window
+  std::unordered_map<uint32_t, uint32_t> initc_of;  // block length n -> x^(8n) 0xFFFFFFFF ^ 0xFFFFFFFF
   tp.dicts.resize(ncols);
   for (int32_t i = 0; i < ncols; i++) {
     const int32_t fc = cols[i];
@@ -416,15 +524,16 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
         const uint64_t blen = (uint64_t)kNullBitmapBytes + (uint64_t)br * vs + 4;
         if (pos + blen > sec_end)
           return vfail(VQ_E_CORRUPT, "exec: gpu: vxq: blocks of rg=%zu col=%s overrun their section", g, f.name.c_str());
-        const uint32_t n = (uint32_t)(blen - 4);
-        auto kit = kidx_of.find(n);
-        if (kit == kidx_of.end()) {
-          kit = kidx_of.emplace(n, (uint32_t)kconst.size() / 65).first;
-          const uint32_t sb = stretch_bytes(n);
-          for (uint32_t l = 0; l < 64; l++) kconst.push_back(x8n(n - std::min(n, (l + 1) * sb)));
-          kconst.push_back(x8n(n));
-        }
-        blocks.push_back(VxqBlock{pos, row0 + b0, br, (uint32_t)i, remap0, ncodes, kit->second, 0});
+        // the block's window on the device: d bytes before it (its offset
+        // mod 16), then lines of 1024 bytes per lane pass to its end
+        const uint32_t n = (uint32_t)(blen - 4), d = (uint32_t)(pos & 15);
+        const uint32_t t = 1024 * window_lines(d, n) - d - n;
+        if (!tshift[t]) tshift[t] = x8n(t);
+        auto kit = initc_of.find(n);
+        if (kit == initc_of.end()) kit = initc_of.emplace(n, multmodp(x8n(n), 0xFFFFFFFFu) ^ 0xFFFFFFFFu).first;
+        // dst and kind are set once the columns exist
+        blocks.push_back(VxqBlock{pos, 0, br, 0, remap0, ncodes, tshift[t], kit->second});
+        where.push_back(BlockAt{i, row0 + b0});
         pos += blen;
       }
       row0 += r.rows;
@@ -466,9 +575,10 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   auto width = [](int t) { return t == VQ_U8 ? 1 : t == VQ_U16 ? 2 : t == VQ_DATE ? 4 : 8; };
   uint64_t col_bytes = 0;
   for (int32_t i = 0; i < ncols; i++) col_bytes += (((uint64_t)nrows * width(vcols[i].type) + 255) & ~255ULL) + 256;
-  const auto& tabs = crc_tables();
-  const uint64_t aux = blocks.size() * sizeof(VxqBlock) + ncols * sizeof(VxqCol) + remap.size() * 4 +
-                       kconst.size() * 4 + sizeof(tabs.t) + 64 + 6 * 256;
+  remap.push_back(0);  // a spare entry: the kernel reads entry 0 of an empty dictionary
+  const auto& tabs = device_crc_tables();
+  const uint64_t aux = blocks.size() * sizeof(VxqBlock) + remap.size() * 4 +
+                       sizeof(tabs.t) + sizeof(tabs.lshift) + 64 + 6 * 256;
   const uint64_t stage = ((body_end + 255) & ~255ULL) + aux;
   if (!ctx_reserve(c, col_bytes + stage))
     return vfail(VQ_E_BUDGET, "exec: gpu: vxq: %llu bytes of columns and staging exceed the device budget",
@@ -480,8 +590,13 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   tp.dev.assign(ncols, nullptr);
   for (int32_t i = 0; i < ncols && e == hipSuccess; i++) {
     e = hipMalloc(&tp.dev[i], (((uint64_t)nrows * width(vcols[i].type) + 255) & ~255ULL) + 256);
-    vcols[i].dst = tp.dev[i];
   }
+  if (e == hipSuccess)
+    for (size_t j = 0; j < blocks.size(); j++) {
+      const OutCol& oc = vcols[where[j].col];
+      blocks[j].dst = (uint64_t)(uintptr_t)tp.dev[where[j].col] + (uint64_t)where[j].row * width(oc.type);
+      blocks[j].kind = !oc.is_string ? (oc.vs == 8 ? kCopy64 : kCopy32) : oc.type == VQ_U8 ? kCodeU8 : kCodeU16;
+    }
   // staging layout: image | blocks | columns | remap | CRC tables | flags
   uint64_t off = (body_end + 255) & ~255ULL;
   auto carve = [&](uint64_t bytes) {
@@ -497,30 +612,27 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   if (e == hipSuccess) {
     a.img = dimg;
     a.blocks = (const VxqBlock*)carve(blocks.size() * sizeof(VxqBlock));
-    a.cols = (const VxqCol*)carve(ncols * sizeof(VxqCol));
     a.remap = (const uint32_t*)carve(std::max<size_t>(remap.size(), 1) * 4);
     a.crc = (const uint32_t*)carve(sizeof(tabs.t));
-    a.kconst = (const uint32_t*)carve(std::max<size_t>(kconst.size(), 1) * 4);
+    a.lshift = (const uint32_t*)carve(sizeof(tabs.lshift));
     a.flags = (uint32_t*)carve(64);
     a.nblocks = (int64_t)blocks.size();
     // the one bulk upload: the file's row-group sections as they lie on disk
     e = hipMemcpyAsync(dimg, img, body_end, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && !blocks.empty())
       e = hipMemcpyAsync((void*)a.blocks, blocks.data(), blocks.size() * sizeof(VxqBlock), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync((void*)a.cols, vcols.data(), ncols * sizeof(VxqCol), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && !remap.empty())
+    if (e == hipSuccess)
       e = hipMemcpyAsync((void*)a.remap, remap.data(), remap.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync((void*)a.crc, &tabs.t[0][0], sizeof(tabs.t), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && !kconst.empty())
-      e = hipMemcpyAsync((void*)a.kconst, kconst.data(), kconst.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync((void*)a.lshift, tabs.lshift, sizeof(tabs.lshift), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(a.flags, hflags, sizeof hflags, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipEventRecord(e0, s);
     if (e == hipSuccess && !blocks.empty()) {
-      const size_t wgs = std::min<size_t>((blocks.size() + 3) / 4, (size_t)device_cus() * 8);
-      hipLaunchKernelGGL(k_vxq_check, dim3((unsigned)wgs), dim3(kCheckThreads), 0, s, a);
-      hipLaunchKernelGGL(k_vxq_decode, dim3((unsigned)blocks.size()), dim3(256), 0, s, a);
+      // persistent: three workgroups of four waves fit a CU's LDS
+      const size_t wgs = std::min<size_t>((blocks.size() + 3) / 4, (size_t)device_cus() * 3);
+      hipLaunchKernelGGL(k_vxq_ingest, dim3((unsigned)wgs), dim3(kIngestThreads), 0, s, a, a.blocks);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(e1, s);
@@ -543,10 +655,11 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
     return vfail(VQ_E_DEVICE, "exec: gpu: vxq upload: %s", hipGetErrorString(e));
   }
   if (hflags[0]) {
-    const VxqBlock& k = blocks[hflags[1] < blocks.size() ? hflags[1] : 0];
+    const size_t j = hflags[1] < blocks.size() ? hflags[1] : 0;
+    const VxqBlock& k = blocks[j];
     int64_t rg = 0, acc = 0;
-    while (rg + 1 < (int64_t)rgs.size() && acc + rgs[rg].rows <= k.dst_row) acc += rgs[rg++].rows;
-    const std::string name = fields[cols[k.col]].name;
+    while (rg + 1 < (int64_t)rgs.size() && acc + rgs[rg].rows <= where[j].row) acc += rgs[rg++].rows;
+    const std::string name = fields[cols[where[j].col]].name;
     drop();
     return vfail(VQ_E_CORRUPT, "exec: gpu: vxq: crc block rg=%lld col=%s off=%llu (%u bad blocks)", (long long)rg,
                  name.c_str(), (unsigned long long)k.src, hflags[0]);
@@ -565,8 +678,10 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
     stats->rg_total = stats->rg_scanned = (int64_t)rgs.size();
     // the blocks read once and the columns written once
     stats->bytes_alg = 0;
-    for (const VxqBlock& k : blocks)
-      stats->bytes_alg += kNullBitmapBytes + (int64_t)k.rows * (vcols[k.col].vs + width(vcols[k.col].type)) + 4;
+    for (size_t j = 0; j < blocks.size(); j++) {
+      const OutCol& oc = vcols[where[j].col];
+      stats->bytes_alg += kNullBitmapBytes + (int64_t)blocks[j].rows * (oc.vs + width(oc.type)) + 4;
+    }
   }
   std::vector<void*> owned = tp.dev;
   if (int rc = table_adopt(c, std::move(tp), out)) {
