@@ -218,15 +218,14 @@ int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out);
 int vq_ctx_destroy(vq_ctx* ctx);
 /* Thread-local text of the last error on this thread ("" if none). */
 const char* vq_last_error(void);
-/* Device bytes currently held by the context (tables + arenas). */
-/* Timing aid (no counterpart in the reference): each later vq_agg_dict and
- * vq_filter call launches its main kernel `repeats` times back to back
- * between the HIP events that time it, and reports the mean duration per
- * launch in vq_stats.main_kernel_ms, so the events' own overhead is
- * amortised.  Results are unchanged (the kernel's output is the same each
- * time).  1 (the default) to 1000. */
+/* Timing aid (no counterpart in the reference): each later vq_agg_dict,
+ * vq_filter and vq_table_upload_vxq call launches its main kernel `repeats`
+ * times back to back between the HIP events that time it, and reports the
+ * mean duration per launch in vq_stats.main_kernel_ms, so the events' own
+ * overhead is amortised.  Results are unchanged (the kernel's output is the
+ * same each time).  1 (the default) to 1000. */
 int vq_ctx_set_kernel_repeats(vq_ctx* ctx, int32_t repeats);
-
+/* Device bytes currently held by the context (tables + arenas). */
 uint64_t vq_ctx_bytes_in_use(const vq_ctx* ctx);
 
 /* ---- tables: the one bulk upload from storage.Reader ----------------------

@@ -168,6 +168,7 @@ int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 bool ctx_reserve(vq_ctx* c, uint64_t bytes) { return reserve(c, bytes); }
 void ctx_release(vq_ctx* c, uint64_t bytes) { release(c, bytes); }
 int ctx_device(const vq_ctx* c) { return c->device; }
+int ctx_kernel_repeats(const vq_ctx* c) { return c->kernel_repeats; }
 
 int table_adopt(vq_ctx* c, TableParts&& p, vq_table** out) {
   vq_table* t = new (std::nothrow) vq_table();

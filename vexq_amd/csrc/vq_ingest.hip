@@ -79,11 +79,23 @@ uint32_t crc32_host(const uint8_t* p, size_t n) {
 // so a block's CRC is the XOR over its pieces M_i of f(0, M_i) shifted by
 // the bytes after M_i, plus 0xFFFFFFFF shifted by the whole length, then
 // 0xFFFFFFFF.
-__host__ __device__ inline uint32_t multmodp(uint32_t a, uint32_t b) {
+inline uint32_t multmodp(uint32_t a, uint32_t b) {
   uint32_t p = 0;
   for (int i = 0; i < 32; i++) {
     if (a & (0x80000000u >> i)) p ^= b;
     b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+  }
+  return p;
+}
+
+// a(x) b(x) mod P on the device without branches: bit i of a (from the top)
+// selects b x^i
+__device__ __forceinline__ uint32_t gf_mul(uint32_t a, uint32_t b) {
+  uint32_t p = 0;
+#pragma unroll
+  for (int i = 0; i < 32; i++) {
+    p ^= b & (uint32_t)((int32_t)(a << i) >> 31);
+    b = (b >> 1) ^ (0xEDB88320u & (0u - (b & 1u)));
   }
   return p;
 }
@@ -101,17 +113,29 @@ uint32_t x8n(uint64_t n) {
   return p;
 }
 
+// x^(-8n) mod P: powers of x^-1 = (P(x) + 1) / x, which is 0xDB710641 with
+// bit 31 as x^0 (x times it is P(x) + 1 = 1 mod P)
+uint32_t xinv8n(uint64_t n) {
+  uint32_t p = 0x80000000u, b = 0xDB710641u;
+  for (uint64_t e = 8 * n; e; e >>= 1) {
+    if (e & 1) p = multmodp(b, p);
+    b = multmodp(b, b);
+  }
+  return p;
+}
+
 // The device's tables: the slice-by-8 tables, then x^8192 mod P times each
-// byte value at each of the register's four byte positions.
+// byte value at each of the register's four byte positions; and, on the host,
+// lane l's shift to the end of a window, x^(8 (1008 - 16 l)).
 struct DeviceCrcTables {
   uint32_t t[12][256];
-  uint32_t lshift[64];
+  uint32_t lane[64];
   DeviceCrcTables() {
     memcpy(t, crc_tables().t, sizeof crc_tables().t);
     const uint32_t x1024 = x8n(1024);
     for (int k = 0; k < 4; k++)
       for (uint32_t b = 0; b < 256; b++) t[8 + k][b] = multmodp(x1024, b << (8 * k));
-    for (int l = 0; l < 64; l++) lshift[l] = x8n(1008 - 16 * l);
+    for (int l = 0; l < 64; l++) lane[l] = x8n(1008 - 16 * l);
   }
 };
 const DeviceCrcTables& device_crc_tables() {
@@ -228,7 +252,7 @@ struct VxqBlock {
   uint32_t kind;    // kCopy32 .. kCodeU16
   uint32_t remap;   // codes: first entry of its row group's remap table
   uint32_t ncodes;  // codes: entries of its row group's dictionary
-  uint32_t shift;   // x^(8t): t zero bytes follow the block in its window
+  uint32_t krow;    // its row of lane constants (one row per t, the zero bytes after it in its window)
   uint32_t initc;   // x^(8n) 0xFFFFFFFF ^ 0xFFFFFFFF: the start and end values' share
 };
 
@@ -237,7 +261,8 @@ struct VxqArgs {
   const VxqBlock* blocks;
   const uint32_t* remap;
   const uint32_t* crc;     // [12][256]: the slice-by-8 tables, then the 1024-byte shift's byte tables
-  const uint32_t* lshift;  // [64]: lane l's register moved to its window's end, x^(8 (1008 - 16 l))
+  const uint32_t* kconst;  // [rows][64]: lane l's register moved to its window's end, then back
+                           // past the window's t zero bytes: x^(8 (1008 - 16 l - t))
   uint32_t* flags;         // [0] bad checksums, [1] first bad block, [2] NULL rows, [3] bad codes
   int64_t nblocks;
 };
@@ -255,11 +280,14 @@ struct VxqArgs {
 // (its line and the other lanes' 63 lines; x^8192 mod P applied through four
 // byte tables) and takes the line's own register from zero (two slice-by-8
 // steps that do not depend on R, so a lane's lines overlap).  R is then
-// moved to the end of the window (x^(8 (1008 - 16 l))) and the lanes'
-// registers XORed: the register of the whole window, with the bytes outside
-// [bitmap, CRC) masked to zero, equals x^(8t) times the block's register from
-// zero, t = 1024 L - d - n for a block of n bytes under the CRC.  The block
-// passes when x^(8t) (stored ^ x^(8n) 0xFFFFFFFF ^ 0xFFFFFFFF) equals it.
+// moved to the end of the window (x^(8 (1008 - 16 l))): the XOR of the lanes'
+// registers is the register of the whole window, with the bytes outside
+// [bitmap, CRC) masked to zero, which is x^(8t) times the block's register
+// from zero, t = 1024 L - d - n for a block of n bytes under the CRC.  So each
+// lane multiplies R by x^(8 (1008 - 16 l - t)) (a row of constants per t,
+// computed on the host) and the XOR of the lanes is the block's register
+// from zero; the block passes when it equals stored ^ x^(8n) 0xFFFFFFFF ^
+// 0xFFFFFFFF.
 // The bitmap's valid rows are counted (bit i of byte i/8, LSB first;
 // storage/codec.go:9-25) and the values decoded from LDS into the resident
 // column, codes through their row group's remap table.
@@ -314,7 +342,7 @@ __device__ __forceinline__ uint32_t slice8(const uint32_t (*T)[256], uint32_t lo
 // the lane's register over L lines of the window, moved to the window's end
 template <int L>
 __device__ __forceinline__ uint32_t lane_crc(const uint32_t (*T)[256], const uint4* win, int lane, uint32_t d,
-                                             uint32_t n, uint32_t wlen, uint32_t lshift) {
+                                             uint32_t n, uint32_t wlen, uint32_t kc) {
   uint32_t raw[L];
 #pragma unroll
   for (int j = 0; j < L; j++) {
@@ -331,7 +359,7 @@ __device__ __forceinline__ uint32_t lane_crc(const uint32_t (*T)[256], const uin
   uint32_t R = 0;
 #pragma unroll
   for (int j = 0; j < L; j++) R = T[8][R & 0xFF] ^ T[9][(R >> 8) & 0xFF] ^ T[10][(R >> 16) & 0xFF] ^ T[11][R >> 24] ^ raw[j];
-  return R ? multmodp(lshift, R) : 0u;
+  return gf_mul(kc, R);
 }
 
 // blocks: the descriptors, read-only and wave-uniform (scalar loads)
@@ -341,7 +369,6 @@ __global__ __launch_bounds__(kIngestThreads, 3) void k_vxq_ingest(const VxqArgs 
   for (int i = threadIdx.x; i < 12 * 256; i += kIngestThreads) (&T[0][0])[i] = a.crc[i];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t lshift = a.lshift[lane];
   __syncthreads();
   uint4* win = W[wv];
   const uint32_t* w32 = (const uint32_t*)win;
@@ -351,6 +378,7 @@ __global__ __launch_bounds__(kIngestThreads, 3) void k_vxq_ingest(const VxqArgs 
   VxqBlock k = blocks[b];
   uint4 v[kMaxLines];
   load_lines(a, k, k.kind == kCopy64 ? 8 : 4, lane, v);
+  uint32_t kc = a.kconst[(size_t)k.krow * 64 + lane];
   for (;;) {
     const int64_t nb = b + waves;
     const bool more = nb < a.nblocks;
@@ -401,22 +429,26 @@ __global__ __launch_bounds__(kIngestThreads, 3) void k_vxq_ingest(const VxqArgs 
       if (bad) atomicAdd(&a.flags[3], bad);
     }
     // the next block's lines, in flight while this one is checked
-    if (more) load_lines(a, kn, kn.kind == kCopy64 ? 8 : 4, lane, v);
+    const uint32_t kcur = kc;
+    if (more) {
+      load_lines(a, kn, kn.kind == kCopy64 ? 8 : 4, lane, v);
+      kc = a.kconst[(size_t)kn.krow * 64 + lane];
+    }
     uint32_t valid = 0;
     if (lane < kNullBitmapBytes / 4 && 32u * lane < k.rows) {
       uint32_t bits = win32(w32, d + 4 * lane);
       if (k.rows - 32u * lane < 32) bits &= (1u << (k.rows - 32u * lane)) - 1u;
       valid = (uint32_t)__popc(bits);
     }
-    uint32_t x = window_lines(d, n) == kShortLines ? lane_crc<kShortLines>(T, win, lane, d, n, wlen, lshift)
-                                                    : lane_crc<kMaxLines>(T, win, lane, d, n, wlen, lshift);
+    uint32_t x = window_lines(d, n) == kShortLines ? lane_crc<kShortLines>(T, win, lane, d, n, wlen, kcur)
+                                                    : lane_crc<kMaxLines>(T, win, lane, d, n, wlen, kcur);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
       x ^= (uint32_t)__shfl_xor((int)x, off);
       valid += (uint32_t)__shfl_xor((int)valid, off);
     }
     if (lane == 0) {
-      if (multmodp(k.shift, win32(w32, d + n) ^ k.initc) != x) {
+      if (x != (win32(w32, d + n) ^ k.initc)) {
         atomicAdd(&a.flags[0], 1u);
         atomicMin(&a.flags[1], (uint32_t)b);
       }
@@ -459,7 +491,9 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   std::vector<VxqBlock> blocks;
   std::vector<BlockAt> where;  // each block's column and first row
   std::vector<uint32_t> remap;
-  std::vector<uint32_t> tshift(1024 * kMaxLines, 0);  // x^(8t) for the t zero bytes ending a window
+  const auto& tabs = device_crc_tables();
+  std::vector<uint32_t> kconst;                      // [row][64] lane constants, one row per t
+  std::vector<int32_t> krow_of(1024 * kMaxLines, -1);  // t -> row
   std::unordered_map<uint32_t, uint32_t> initc_of;  // block length n -> x^(8n) 0xFFFFFFFF ^ 0xFFFFFFFF
   tp.dicts.resize(ncols);
   for (int32_t i = 0; i < ncols; i++) {
@@ -528,11 +562,15 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
         // mod 16), then lines of 1024 bytes per lane pass to its end
         const uint32_t n = (uint32_t)(blen - 4), d = (uint32_t)(pos & 15);
         const uint32_t t = 1024 * window_lines(d, n) - d - n;
-        if (!tshift[t]) tshift[t] = x8n(t);
+        if (krow_of[t] < 0) {
+          krow_of[t] = (int32_t)(kconst.size() / 64);
+          const uint32_t back = xinv8n(t);
+          for (int l = 0; l < 64; l++) kconst.push_back(multmodp(tabs.lane[l], back));
+        }
         auto kit = initc_of.find(n);
         if (kit == initc_of.end()) kit = initc_of.emplace(n, multmodp(x8n(n), 0xFFFFFFFFu) ^ 0xFFFFFFFFu).first;
         // dst and kind are set once the columns exist
-        blocks.push_back(VxqBlock{pos, 0, br, 0, remap0, ncodes, tshift[t], kit->second});
+        blocks.push_back(VxqBlock{pos, 0, br, 0, remap0, ncodes, (uint32_t)krow_of[t], kit->second});
         where.push_back(BlockAt{i, row0 + b0});
         pos += blen;
       }
@@ -576,9 +614,8 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   uint64_t col_bytes = 0;
   for (int32_t i = 0; i < ncols; i++) col_bytes += (((uint64_t)nrows * width(vcols[i].type) + 255) & ~255ULL) + 256;
   remap.push_back(0);  // a spare entry: the kernel reads entry 0 of an empty dictionary
-  const auto& tabs = device_crc_tables();
-  const uint64_t aux = blocks.size() * sizeof(VxqBlock) + remap.size() * 4 +
-                       sizeof(tabs.t) + sizeof(tabs.lshift) + 64 + 6 * 256;
+  const uint64_t aux = blocks.size() * sizeof(VxqBlock) + remap.size() * 4 + sizeof(tabs.t) +
+                       std::max<size_t>(kconst.size(), 1) * 4 + 64 + 6 * 256;
   const uint64_t stage = ((body_end + 255) & ~255ULL) + aux;
   if (!ctx_reserve(c, col_bytes + stage))
     return vfail(VQ_E_BUDGET, "exec: gpu: vxq: %llu bytes of columns and staging exceed the device budget",
@@ -607,6 +644,7 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
   VxqArgs a;
   memset(&a, 0, sizeof a);
   uint32_t hflags[4] = {0, 0xFFFFFFFFu, 0, 0};
+  const int reps = ctx_kernel_repeats(c);
   float ms = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (e == hipSuccess) {
@@ -614,7 +652,7 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
     a.blocks = (const VxqBlock*)carve(blocks.size() * sizeof(VxqBlock));
     a.remap = (const uint32_t*)carve(std::max<size_t>(remap.size(), 1) * 4);
     a.crc = (const uint32_t*)carve(sizeof(tabs.t));
-    a.lshift = (const uint32_t*)carve(sizeof(tabs.lshift));
+    a.kconst = (const uint32_t*)carve(std::max<size_t>(kconst.size(), 1) * 4);
     a.flags = (uint32_t*)carve(64);
     a.nblocks = (int64_t)blocks.size();
     // the one bulk upload: the file's row-group sections as they lie on disk
@@ -624,7 +662,8 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
     if (e == hipSuccess)
       e = hipMemcpyAsync((void*)a.remap, remap.data(), remap.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync((void*)a.crc, &tabs.t[0][0], sizeof(tabs.t), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync((void*)a.lshift, tabs.lshift, sizeof(tabs.lshift), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !kconst.empty())
+      e = hipMemcpyAsync((void*)a.kconst, kconst.data(), kconst.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(a.flags, hflags, sizeof hflags, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -632,13 +671,18 @@ extern "C" int vq_table_upload_vxq(vq_ctx* c, const void* file, uint64_t file_le
     if (e == hipSuccess && !blocks.empty()) {
       // persistent: three workgroups of four waves fit a CU's LDS
       const size_t wgs = std::min<size_t>((blocks.size() + 3) / 4, (size_t)device_cus() * 3);
-      hipLaunchKernelGGL(k_vxq_ingest, dim3((unsigned)wgs), dim3(kIngestThreads), 0, s, a, a.blocks);
+      for (int rep = 0; rep < reps; rep++)  // the same columns each time; the flags count each pass
+        hipLaunchKernelGGL(k_vxq_ingest, dim3((unsigned)wgs), dim3(kIngestThreads), 0, s, a, a.blocks);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(e1, s);
     if (e == hipSuccess) e = hipMemcpyAsync(hflags, a.flags, sizeof hflags, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)reps;
+    hflags[0] /= reps;
+    hflags[2] /= reps;
+    hflags[3] /= reps;
   }
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);

@@ -171,6 +171,7 @@ int set_error(int code, const char* msg);
 bool ctx_reserve(vq_ctx* c, uint64_t bytes);
 void ctx_release(vq_ctx* c, uint64_t bytes);
 int ctx_device(const vq_ctx* c);
+int ctx_kernel_repeats(const vq_ctx* c);
 // A table whose columns are already on the device (hipMalloc'd, padded to a
 // 256-byte multiple plus 256 bytes, `bytes` reserved on the context).
 struct TableParts {
