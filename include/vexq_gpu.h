@@ -134,6 +134,17 @@ typedef struct vq_pred {
 #define VQ_DIV 3
 
 #define VQ_MAX_FACTORS 4
+/* CASE WHEN <when> THEN <the fold> ELSE <literal> END: with nwhen > 0 the
+ * fold is guarded by a conjunction of nwhen predicate terms (vq_term: a
+ * compare with a literal, VQ_IN over codes, op | VQ_COLCMP) on any columns,
+ * and a row failing it takes the ELSE literal (else_f when is_float, else_i
+ * otherwise).  CaseExpr with one WHEN and a literal ELSE (exec/expr.go:
+ * 1386-1435), e.g. TPC-H Q12's SUM(CASE WHEN o_orderpriority IN ('1-URGENT',
+ * '2-HIGH') THEN 1 ELSE 0 END) with the string literals as global codes
+ * (StringEqExpr as a code compare, exec/expr.go:1342-1384).  A CASE with
+ * several WHENs, or with no ELSE (NULL), is not representable: the caller
+ * keeps the Go operator. */
+#define VQ_MAX_WHEN 2
 typedef struct vq_factor {
   int32_t kind;
   int32_t col;
@@ -147,6 +158,11 @@ typedef struct vq_expr {
   int32_t nfactors; /* 0 = no argument (COUNT(*)) */
   int32_t is_float;
   vq_factor f[VQ_MAX_FACTORS];
+  int32_t nwhen;    /* 0 = no CASE */
+  int32_t pad_;
+  vq_term when[VQ_MAX_WHEN];
+  double else_f;
+  int64_t else_i;
 } vq_expr;
 
 /* Aggregates: exec.AggExpr (exec/aggregate.go:25-37).  accum is the
@@ -254,9 +270,10 @@ int vq_table_zones(const vq_table* t, vq_zone* out, int64_t cap);
  * 65,536).  cols: the file columns to keep, in table-column order.  Returns
  * VQ_E_CORRUPT naming the row group, column and offset of a block whose
  * checksum fails; VQ_E_UNSUPPORTED for a BOOL column, a column with NULLs or
- * more than 65,536 strings.  stats (may be NULL): device_ms of the check and
- * decode kernels, bytes_alg the block bytes read plus the column bytes
- * written. */
+ * more than 65,536 strings.  stats (may be NULL): device_ms and
+ * main_kernel_ms of the one check-and-decode kernel (per launch, see
+ * vq_ctx_set_kernel_repeats), bytes_alg the block bytes read plus the column
+ * bytes written. */
 int vq_table_upload_vxq(vq_ctx* ctx, const void* file, uint64_t file_len, int32_t ncols,
                         const int32_t* cols, vq_table** out, vq_stats* stats);
 /* A column's type and dictionary size (0: not a dictionary column). */

@@ -228,27 +228,36 @@ static inline int cmp_f(int op, double a, double b) {
 }
 
 /* Filter over a conjunction (AndExpr, exec/expr.go:1030-1067). */
+/* One predicate term on row r. */
+static inline int term_pass(const vq_col_desc* cols, const vq_term* tm, int64_t r) {
+  const vq_col_desc* c = &cols[tm->col];
+  if (tm->op == VQ_IN) {
+    /* InExpr (exec/expr.go:1188-1238) over codes: the value is one of the
+     * set's members, given as a mask over 0..63 */
+    const int64_t v = col_i(c, r);
+    return v >= 0 && v < 64 && (((uint64_t)tm->ival >> v) & 1u);
+  }
+  if (tm->op & VQ_COLCMP) {
+    /* a BinOp compare of two columns (exec/expr.go:320-896) */
+    const vq_col_desc* c2 = &cols[tm->ival];
+    const int op = tm->op & ~VQ_COLCMP;
+    return c->type == VQ_F64 ? cmp_f(op, col_f(c, r), col_f(c2, r)) : cmp_i(op, col_i(c, r), col_i(c2, r));
+  }
+  return c->type == VQ_F64 ? cmp_f(tm->op, col_f(c, r), tm->fval) : cmp_i(tm->op, col_i(c, r), tm->ival);
+}
+
 static inline int pred_pass(const vq_col_desc* cols, const vq_pred* p, int64_t r) {
   if (!p) return 1;
-  for (int t = 0; t < p->nterms; t++) {
-    const vq_term* tm = &p->terms[t];
-    const vq_col_desc* c = &cols[tm->col];
-    int ok;
-    if (tm->op == VQ_IN) {
-      /* InExpr (exec/expr.go:1188-1238) over codes: the value is one of the
-       * set's members, given as a mask over 0..63 */
-      const int64_t v = col_i(c, r);
-      ok = v >= 0 && v < 64 && (((uint64_t)tm->ival >> v) & 1u);
-    } else if (tm->op & VQ_COLCMP) {
-      /* a BinOp compare of two columns (exec/expr.go:320-896) */
-      const vq_col_desc* c2 = &cols[tm->ival];
-      const int op = tm->op & ~VQ_COLCMP;
-      ok = c->type == VQ_F64 ? cmp_f(op, col_f(c, r), col_f(c2, r)) : cmp_i(op, col_i(c, r), col_i(c2, r));
-    } else {
-      ok = c->type == VQ_F64 ? cmp_f(tm->op, col_f(c, r), tm->fval) : cmp_i(tm->op, col_i(c, r), tm->ival);
-    }
-    if (!ok) return 0;
-  }
+  for (int t = 0; t < p->nterms; t++)
+    if (!term_pass(cols, &p->terms[t], r)) return 0;
+  return 1;
+}
+
+/* CaseExpr with one WHEN and a literal ELSE (exec/expr.go:1386-1435): the
+ * WHEN's conjunction on row r (no CASE: always). */
+static inline int when_pass(const vq_col_desc* cols, const vq_expr* e, int64_t r) {
+  for (int t = 0; t < e->nwhen; t++)
+    if (!term_pass(cols, &e->when[t], r)) return 0;
   return 1;
 }
 
@@ -284,6 +293,7 @@ static inline int64_t arith_i(int op, int64_t a, int64_t b) {
 }
 
 static inline double expr_f(const vq_col_desc* cols, const vq_expr* e, int64_t r) {
+  if (!when_pass(cols, e, r)) return e->else_f;
   double v = 0;
   for (int k = 0; k < e->nfactors; k++) {
     const vq_factor* f = &e->f[k];
@@ -300,6 +310,7 @@ static inline double expr_f(const vq_col_desc* cols, const vq_expr* e, int64_t r
 }
 
 static inline int64_t expr_i(const vq_col_desc* cols, const vq_expr* e, int64_t r) {
+  if (!when_pass(cols, e, r)) return e->else_i;
   int64_t v = 0;
   for (int k = 0; k < e->nfactors; k++) {
     const vq_factor* f = &e->f[k];

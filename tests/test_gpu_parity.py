@@ -153,6 +153,52 @@ def test_q12_shape_groupby_and_join(gpu_ctx, kernel_path):
     assert_agg_equal(top, O.join_agg(b, 0, bp, p, 0, pp, jaggs, topk=10, exact=True), jaggs)
 
 
+def test_case_when_aggregates(gpu_ctx, kernel_path):
+    """Aggregates over CASE WHEN <cond> THEN <fold> ELSE <literal> END (Q12's
+    indicator sums, guarded float sums, averages, MIN/MAX, guards comparing
+    two columns) under the dictionary GROUP BY (run-time compiled and
+    precompiled kernels), the hash GROUP BY by its partials and by its
+    partitions, and the join's probe side, against the oracle."""
+    from test_oracle import case_aggs, q12_table
+    _print_case("case_when", path=kernel_path)
+    X = abi.X
+    t = q12_table(300_000, seed=14)
+    aggs = case_aggs(t)
+    pred = abi.make_pred(t, [("receiptdate", ">=", 9000)])
+    dt = gpu_ctx.upload(t)
+    try:
+        key_sets = [[t.index("shipmode")], []]
+        if kernel_path == "jit":  # 21 groups x 10 words: past the dense LDS, the packed-key hash kernels
+            key_sets.append([t.index("flagA"), t.index("shipmode")])
+        for keys in key_sets:
+            got = gpu_ctx.agg_dict(dt, pred, keys, aggs)
+            assert_agg_equal(got, O.agg_dict(t, pred, keys, aggs, exact=True), aggs)
+        if kernel_path == "jit":
+            for key in ("shipmode", "quantity", "shipdate"):  # 7, 50 keys: partials; ~2.5K: partitions
+                got = gpu_ctx.agg_hash(dt, pred, t.index(key), aggs)
+                assert_agg_equal(got, O.agg_hash(t, pred, t.index(key), aggs, exact=True), aggs)
+    finally:
+        dt.close()
+    if kernel_path != "jit":
+        return
+    b, p = _join_tables(50_000, 200_000, seed=15)
+    bp, pp, _ = datagen.q_join(b, p)
+    jaggs = [abi.count_star(p),
+             abi.agg_sum(X.case([("quantity", ">", 25)], 1, 0), p),
+             abi.agg_sum(X.case([("discount", "<", 0.05), ("quantity", "<=", 40)],
+                                X.col("price") * (1 - X.col("discount")), 0.0), p),
+             abi.agg_max(X.case([("price", ">", X.col("discount"))], X.col("quantity"), -1), p)]
+    db, dp = gpu_ctx.upload(b), gpu_ctx.upload(p)
+    try:
+        got = gpu_ctx.join_agg(db, 0, bp, dp, 0, pp, jaggs)
+        top = gpu_ctx.join_agg(db, 0, bp, dp, 0, pp, jaggs, topk=10)
+    finally:
+        db.close()
+        dp.close()
+    assert_agg_equal(got, O.join_agg(b, 0, bp, p, 0, pp, jaggs, exact=True), jaggs)
+    assert_agg_equal(top, O.join_agg(b, 0, bp, p, 0, pp, jaggs, topk=10, exact=True), jaggs)
+
+
 def test_filter_pruned_sorted(gpu_ctx):
     t = datagen.lineitem_sorted(1_000_000, seed=3)
     dt = gpu_ctx.upload(t)

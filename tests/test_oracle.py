@@ -285,6 +285,90 @@ def test_filter_in_and_column_compares_match_numpy():
         np.testing.assert_array_equal(O.filter_rows(t, abi.make_pred(t, terms)), np.nonzero(mask)[0])
 
 
+def case_aggs(t):
+    """Aggregates over CASE WHEN <cond> THEN <fold> ELSE <literal> END:
+    Q12's indicator sums (IN, and the NOT IN written as two <> terms),
+    guarded float sums and averages, a guard comparing two columns, MIN/MAX
+    under a guard.  Seven columns, so a GROUP BY on one more stays within a
+    query's eight."""
+    X = abi.X
+    return [abi.count_star(t),
+            abi.agg_sum(X.case([("flagA", "in", [0, 1])], 1, 0), t),
+            abi.agg_sum(X.case([("flagA", "<>", 0), ("flagA", "<>", 1)], 1, 0), t),
+            abi.agg_sum(X.case([("quantity", "<", 25)], X.col("price"), 0.0), t),
+            abi.agg_avg(X.case([("commitdate", "<", X.col("receiptdate"))],
+                               X.col("price") * (1 - X.col("discount")), -1.5), t),
+            abi.agg_max(X.case([("shipdate", ">=", X.col("commitdate")), ("quantity", ">", 10)],
+                               X.col("quantity"), -7), t),
+            abi.agg_min(X.case([("discount", ">", 0.05)], X.col("discount") * 2, 1.0), t),
+            abi.agg_sum(X.case([("commitdate", ">", X.col("shipdate"))], X.col("quantity") * 3 - 1, 100), t)]
+
+
+def case_values_numpy(t):
+    """The same arguments per row, computed with numpy from the columns."""
+    d = {c.name: c.data for c in t.columns}
+    q = d["quantity"].astype(np.int64)
+    return [
+        None,
+        np.where(np.isin(d["flagA"], [0, 1]), 1, 0).astype(np.int64),
+        np.where((d["flagA"] != 0) & (d["flagA"] != 1), 1, 0).astype(np.int64),
+        np.where(q < 25, d["price"], 0.0),
+        np.where(d["commitdate"] < d["receiptdate"], d["price"] * (1 - d["discount"]), -1.5),
+        np.where((d["shipdate"] >= d["commitdate"]) & (q > 10), q, -7),
+        np.where(d["discount"] > 0.05, d["discount"] * 2, 1.0),
+        np.where(d["commitdate"] > d["shipdate"], q * 3 - 1, 100),
+    ]
+
+
+def test_case_when_aggregates_match_numpy():
+    """CaseExpr with one WHEN and a literal ELSE (exec/expr.go:1386-1435)
+    inside SUM/AVG/MIN/MAX, grouped by a dictionary column, against numpy:
+    integer results exact, float sums and averages equal to math.fsum of the
+    per-row values (correctly rounded, as the oracle's exact sums are)."""
+    t = q12_table(60_000, seed=12)
+    aggs = case_aggs(t)
+    vals = case_values_numpy(t)
+    key = t.index("shipmode")
+    r = O.agg_dict(t, None, [key], aggs, exact=True)
+    codes = t.columns[key].data
+    assert r.ngroups == 7
+    for g in range(r.ngroups):
+        m = codes == r.keys[g, 0]
+        assert r.counts[g] == int(m.sum())
+        assert int(r.ivals[g, 1]) == int(vals[1][m].sum())
+        assert int(r.ivals[g, 2]) == int(vals[2][m].sum())
+        assert r.fvals[g, 3] == math.fsum(vals[3][m])
+        assert r.fvals[g, 4] == math.fsum(vals[4][m]) / m.sum()
+        assert int(r.ivals[g, 5]) == int(vals[5][m].max())
+        assert r.fvals[g, 6] == vals[6][m].min()
+        assert int(r.ivals[g, 7]) == int(vals[7][m].sum())
+    # the indicator sums of Q12 partition the rows
+    np.testing.assert_array_equal(r.ivals[:7, 1] + r.ivals[:7, 2], r.counts[:7])
+
+
+def test_case_when_sum_equals_the_filtered_sum():
+    """SUM(CASE WHEN c THEN x ELSE 0 END) over all rows is SUM(x) over the
+    rows passing c -- the CASE path against the predicate path of the same
+    oracle, for the dictionary, hash and join aggregates."""
+    X = abi.X
+    t = q12_table(50_000, seed=13)
+    cond = [("shipmode", "in", [1, 3, 4]), ("commitdate", "<", X.col("receiptdate"))]
+    arg = X.col("price") * (1 - X.col("discount"))
+    guarded = [abi.agg_sum(X.case(cond, arg, 0.0), t), abi.agg_sum(X.case(cond, X.col("quantity"), 0), t)]
+    plain = [abi.agg_sum(arg, t), abi.agg_sum("quantity", t)]
+    key = t.index("flagA")
+    a = O.agg_dict(t, None, [key], guarded, exact=True)
+    b = O.agg_dict(t, abi.make_pred(t, cond), [key], plain, exact=True)
+    for g in range(a.ngroups):
+        j = list(b.keys[:, 0]).index(a.keys[g, 0])
+        assert a.fvals[g, 0] == b.fvals[j, 0] and a.ivals[g, 1] == b.ivals[j, 1]
+    ha = O.agg_hash(t, None, key, guarded, exact=True)
+    hb = O.agg_hash(t, abi.make_pred(t, cond), key, plain, exact=True)
+    for g in range(ha.ngroups):
+        j = list(hb.keys[:, 0]).index(ha.keys[g, 0])
+        assert ha.fvals[g, 0] == hb.fvals[j, 0] and ha.ivals[g, 1] == hb.ivals[j, 1]
+
+
 # ---- dictionary-code GROUP BY -----------------------------------------------
 
 def test_intkey_remap_kat():

@@ -68,6 +68,8 @@ def test_noalloc_matches_agg_dict_on_two_calls(gpu_ctx):
             ([t.index("flagA")], [abi.count_star(t), abi.agg_avg("tax", t)],
              abi.make_pred(t, [("quantity", ">", 99)])),
             ([], [abi.count_star(t), abi.agg_sum("price", t)], abi.make_pred(t, [("quantity", ">", 99)])),
+            ([t.index("flagA")], [abi.agg_sum(X.case([("quantity", "<", 25)], X.col("price"), 0.0), t),
+                                  abi.agg_sum(X.case([("flagB", "in", [0])], 1, 0), t)], None),
         ]
         for keys, aggs, pred in cases:
             _assert_same(_noalloc(gpu_ctx, dt, pred, keys, aggs), gpu_ctx.agg_dict(dt, pred, keys, aggs), aggs)

@@ -21,7 +21,7 @@ VQ_COLCMP = 16   # op | VQ_COLCMP: column OP column ival
 VQ_F_COL, VQ_F_LIT, VQ_F_LIT_OP_COL, VQ_F_COL_OP_LIT = range(4)
 VQ_ADD, VQ_SUB, VQ_MUL, VQ_DIV = range(4)
 VQ_AGG_COUNT, VQ_AGG_SUM, VQ_AGG_AVG, VQ_AGG_MIN, VQ_AGG_MAX = range(5)
-VQ_MAX_TERMS, VQ_MAX_FACTORS, VQ_MAX_AGGS = 8, 4, 8
+VQ_MAX_TERMS, VQ_MAX_FACTORS, VQ_MAX_AGGS, VQ_MAX_WHEN = 8, 4, 8, 2
 ROW_GROUP_ROWS = 65536
 
 STATUS_NAMES = {
@@ -57,7 +57,9 @@ class Factor(C.Structure):
 
 class Expr(C.Structure):
     _fields_ = [("nfactors", C.c_int32), ("is_float", C.c_int32),
-                ("f", Factor * VQ_MAX_FACTORS)]
+                ("f", Factor * VQ_MAX_FACTORS),
+                ("nwhen", C.c_int32), ("pad_", C.c_int32), ("when", Term * VQ_MAX_WHEN),
+                ("else_f", C.c_double), ("else_i", C.c_int64)]
 
 
 class Agg(C.Structure):
@@ -112,32 +114,35 @@ def make_pred(table, terms) -> Pred:
     p = Pred()
     p.nterms = len(terms)
     for i, (name, op, lit) in enumerate(terms):
-        ci = table.index(name)
-        t = p.terms[i]
-        t.col = ci
-        if op == "in":
-            mask = 0
-            for v in lit:
-                if not 0 <= int(v) < 64:
-                    raise ValueError("IN values must lie in [0, 64)")
-                mask |= 1 << int(v)
-            t.op = VQ_IN
-            t.ival = mask - (1 << 64) if mask >= 1 << 63 else mask
-            continue
-        if isinstance(lit, X):
-            if not lit._single():
-                raise TypeError("a column compare takes one column, X.col(name)")
-            t.op = _OPS[op] | VQ_COLCMP
-            t.ival = table.index(lit.factors[0][1].col)
-            continue
-        t.op = _OPS[op]
-        if table.columns[ci].type == VQ_F64:
-            t.fval = float(lit)
-        else:
-            if isinstance(lit, float):
-                raise TypeError(f"float literal against integer column {name}")
-            t.ival = int(lit)
+        _fill_term(table, p.terms[i], name, op, lit)
     return p
+
+
+def _fill_term(table, t, name, op, lit):
+    ci = table.index(name)
+    t.col = ci
+    if op == "in":
+        mask = 0
+        for v in lit:
+            if not 0 <= int(v) < 64:
+                raise ValueError("IN values must lie in [0, 64)")
+            mask |= 1 << int(v)
+        t.op = VQ_IN
+        t.ival = mask - (1 << 64) if mask >= 1 << 63 else mask
+        return
+    if isinstance(lit, X):
+        if not lit._single():
+            raise TypeError("a column compare takes one column, X.col(name)")
+        t.op = _OPS[op] | VQ_COLCMP
+        t.ival = table.index(lit.factors[0][1].col)
+        return
+    t.op = _OPS[op]
+    if table.columns[ci].type == VQ_F64:
+        t.fval = float(lit)
+    else:
+        if isinstance(lit, float):
+            raise TypeError(f"float literal against integer column {name}")
+        t.ival = int(lit)
 
 
 # ---- arithmetic builder: left fold of up to four factors -------------------
@@ -157,21 +162,39 @@ class X:
     """A left fold ((f0 c1 f1) c2 f2) ... built with Python operators, e.g.
     X.col("price") * (1 - X.col("discount")) * (1 + X.col("tax"))."""
 
-    def __init__(self, factors):
+    def __init__(self, factors, when=None, else_=0):
         self.factors = factors  # list of (combine, _F)
+        self.when = when        # CASE WHEN: terms as make_pred takes them
+        self.else_ = else_
 
     @staticmethod
     def col(name):
         return X([(VQ_ADD, _F(VQ_F_COL, col=name))])
 
     @staticmethod
+    def case(when, then, else_=0):
+        """CASE WHEN <when, a conjunction of up to VQ_MAX_WHEN predicate
+        terms> THEN <then, a left fold> ELSE <literal> END (CaseExpr with one
+        WHEN and a literal ELSE, exec/expr.go:1386-1435)."""
+        if isinstance(then, (int, float)):
+            then = X.lit(then)
+        if then.when is not None:
+            raise ValueError("nested CASE")
+        when = list(when)
+        if not 1 <= len(when) <= VQ_MAX_WHEN:
+            raise ValueError(f"a WHEN takes 1 to {VQ_MAX_WHEN} terms")
+        return X(list(then.factors), when, else_)
+
+    @staticmethod
     def lit(v):
         return X([(VQ_ADD, _F(VQ_F_LIT, lit=v))])
 
     def _single(self):
-        return len(self.factors) == 1 and self.factors[0][1].kind == VQ_F_COL
+        return self.when is None and len(self.factors) == 1 and self.factors[0][1].kind == VQ_F_COL
 
     def _fold(self, other, op):
+        if self.when is not None or getattr(other, "when", None) is not None:
+            raise ValueError("a CASE is an aggregate's whole argument")
         if isinstance(other, (int, float)):
             other = X.lit(other)
         if len(other.factors) != 1:
@@ -179,6 +202,8 @@ class X:
         return X(self.factors + [(op, other.factors[0][1])])
 
     def _lit_side(self, lit, op, lit_left):
+        if self.when is not None:
+            raise ValueError("a CASE is an aggregate's whole argument")
         # (lit op col) / (col op lit): only on a bare column
         if not self._single():
             if lit_left:
@@ -225,9 +250,19 @@ class X:
                 is_float = True
             if f.kind != VQ_F_COL and isinstance(f.lit, float):
                 is_float = True
+        if self.when is not None and isinstance(self.else_, float):
+            is_float = True
         e = Expr()
         e.nfactors = len(self.factors)
         e.is_float = int(is_float)
+        if self.when is not None:
+            e.nwhen = len(self.when)
+            for k, (name, op, lit) in enumerate(self.when):
+                _fill_term(table, e.when[k], name, op, lit)
+            if is_float:
+                e.else_f = float(self.else_)
+            else:
+                e.else_i = int(self.else_)
         for k, (comb, f) in enumerate(self.factors):
             cf = e.f[k]
             cf.kind = f.kind

@@ -33,9 +33,12 @@
 static constexpr int kKSL[8] = {2, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kKSH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
-static constexpr int kTS[8] = {6, 0, 0, 0, 0, 0, 0, 0};
-static constexpr int kTO[8] = {3, 0, 0, 0, 0, 0, 0, 0};
-static constexpr int kTS2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#define VQ_TERMS 24
+static constexpr int kTS[24] = {6, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kTO[24] = {3, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kTS2[24] = {0, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kAW0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kAWN[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kKS[8] = {2, 1, 0, 0, 0, 0, 0, 0};
 static constexpr int kAOP[8] = {1, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kAISF[8] = {0, 1, 1, 1, 1, 0, 0, 0};
@@ -93,10 +96,12 @@ using seq = typename MakeSeq<N>::type;
 
 struct Args {
   const void* col[8];
-  i64 ti[8];
-  double tf[8];
+  i64 ti[VQ_TERMS];   // the predicate's terms [0, VQ_NT), then CASE WHEN terms
+  double tf[VQ_TERMS];
   i64 fi[8][4];
   double ff[8][4];
+  i64 ei[8];          // ELSE literals of aggregates with a CASE WHEN
+  double ef[8];
   i64 sp_start[kMaxSpans];
   i64 sp_rows[kMaxSpans];
   i64 sp_tile0[kMaxSpans];
@@ -347,6 +352,28 @@ __device__ __forceinline__ i64 fold_i(const Args& a, const i64* rawrow, i64 v) {
   }
 }
 
+// CASE WHEN of aggregate J (CaseExpr with one WHEN and a literal ELSE,
+// exec/expr.go:1386-1435): its terms are kAW0[J] .. kAW0[J] + kAWN[J] - 1 of
+// the term arrays; a row failing them takes the ELSE literal.
+template <int J, int... T>
+__device__ __forceinline__ bool when_all(const Args& a, const i64* rawrow, Seq<T...>) {
+  return (true && ... && term<kAW0[J] + T>(a, rawrow));
+}
+
+template <int J>
+__device__ __forceinline__ double val_f(const Args& a, const i64* rawrow) {
+  const double v = fold_f<J, 0>(a, rawrow, 0.0);
+  if constexpr (kAWN[J] == 0) return v;
+  else return when_all<J>(a, rawrow, seq<kAWN[J]>{}) ? v : a.ef[J];
+}
+
+template <int J>
+__device__ __forceinline__ i64 val_i(const Args& a, const i64* rawrow) {
+  const i64 v = fold_i<J, 0>(a, rawrow, 0);
+  if constexpr (kAWN[J] == 0) return v;
+  else return when_all<J>(a, rawrow, seq<kAWN[J]>{}) ? v : a.ei[J];
+}
+
 // ---- accumulators ------------------------------------------------------------
 // ops: 0 SUM_F, 1 SUM_I, 2 MIN_F, 3 MAX_F, 4 MIN_I, 5 MAX_I
 __device__ __forceinline__ i64 f64_ord(double d) {
@@ -368,7 +395,7 @@ template <int J>
 __device__ __forceinline__ void fold_reg(u64& r, const Args& a, const i64* rawrow) {
   constexpr int OP = kAOP[J];
   if constexpr (kAISF[J]) {
-    const double v = fold_f<J, 0>(a, rawrow, 0.0);
+    const double v = val_f<J>(a, rawrow);
     if constexpr (OP == 0) {
       r = (u64)f64_bits(as_f64((i64)r) + v);
     } else if (v == v) {
@@ -377,7 +404,7 @@ __device__ __forceinline__ void fold_reg(u64& r, const Args& a, const i64* rawro
       else r = o > (i64)r ? (u64)o : r;
     }
   } else {
-    const i64 v = fold_i<J, 0>(a, rawrow, 0);
+    const i64 v = val_i<J>(a, rawrow);
     if constexpr (OP == 1) r = (u64)((i64)r + v);
     else if constexpr (OP == 4) r = v < (i64)r ? (u64)v : r;
     else r = v > (i64)r ? (u64)v : r;
@@ -389,7 +416,7 @@ template <int J>
 __device__ __forceinline__ void fold_lds(u64* w, const Args& a, const i64* rawrow) {
   constexpr int OP = kAOP[J];
   if constexpr (kAISF[J]) {
-    const double v = fold_f<J, 0>(a, rawrow, 0.0);
+    const double v = val_f<J>(a, rawrow);
     if constexpr (OP == 0) {
       atomicAdd((double*)w, v);
     } else if (v == v) {
@@ -397,7 +424,7 @@ __device__ __forceinline__ void fold_lds(u64* w, const Args& a, const i64* rawro
       else atomicMax((long long*)w, (long long)f64_ord(v));
     }
   } else {
-    const i64 v = fold_i<J, 0>(a, rawrow, 0);
+    const i64 v = val_i<J>(a, rawrow);
     if constexpr (OP == 1) atomicAdd((unsigned long long*)w, (unsigned long long)v);
     else if constexpr (OP == 4) atomicMin((long long*)w, (long long)v);
     else atomicMax((long long*)w, (long long)v);
@@ -795,8 +822,8 @@ __device__ __forceinline__ void ptile(const Args& a, i64 t, i64& row0, i64& rend
 // Value of aggregate argument J for one row, as 64 bits (float bits or int).
 template <int J>
 __device__ __forceinline__ u64 value_of(const Args& a, const i64* row) {
-  if constexpr (kAISF[J]) return (u64)f64_bits(fold_f<J, 0>(a, row, 0.0));
-  else return (u64)fold_i<J, 0>(a, row, 0);
+  if constexpr (kAISF[J]) return (u64)f64_bits(val_f<J>(a, row));
+  else return (u64)val_i<J>(a, row);
 }
 
 template <int... J>

@@ -181,8 +181,39 @@ __device__ __forceinline__ void eval_pred(const QSlot* slot, int nslots, int64_t
   }
 }
 
+// A CASE WHEN guard (CaseExpr with one WHEN, exec/expr.go:1386-1435): the
+// conjunction of its terms on each row.
+template <int U>
+__device__ __forceinline__ void eval_when(const QExpr& e, const QSlot* slot, const int64_t (&raw)[kMaxSlots][U],
+                                          bool (&ok)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) ok[u] = true;
+  for (int k = 0; k < e.nwhen; k++) {
+    const QWhen w = e.when[k];
+    const bool isf = slot[w.slot].type == VQ_F64;
+    int64_t x[U];
+    pick<U>(raw, w.slot, x);
+    if (w.op == VQ_IN) {
+#pragma unroll
+      for (int u = 0; u < U; u++) ok[u] = ok[u] && (uint64_t)x[u] < 64 && (((uint64_t)w.ival >> x[u]) & 1ull);
+    } else if (w.op & VQ_COLCMP) {
+      int64_t y[U];
+      pick<U>(raw, w.slot2, y);
+      const int op = w.op & ~VQ_COLCMP;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        ok[u] = ok[u] && (isf ? cmp_f(op, bits_to_f64(x[u]), bits_to_f64(y[u])) : cmp_i(op, x[u], y[u]));
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        ok[u] = ok[u] && (isf ? cmp_f(w.op, bits_to_f64(x[u]), w.fval) : cmp_i(w.op, x[u], w.ival));
+    }
+  }
+}
+
 // The left fold of exec/project.go's materialised BinOps, one rounding per
-// operation (built with -ffp-contract=off: no fused multiply-add).
+// operation (built with -ffp-contract=off: no fused multiply-add), under its
+// CASE WHEN guard when it has one.
 template <int U>
 __device__ __forceinline__ void eval_f(const QExpr& e, const QSlot* slot,
                                        const int64_t (&raw)[kMaxSlots][U], double (&v)[U]) {
@@ -224,10 +255,16 @@ __device__ __forceinline__ void eval_f(const QExpr& e, const QSlot* slot,
       for (int u = 0; u < U; u++) v[u] = arith_f(f.combine, v[u], x[u]);
     }
   }
+  if (e.nwhen) {
+    bool ok[U];
+    eval_when<U>(e, slot, raw, ok);
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = ok[u] ? v[u] : e.else_f;
+  }
 }
 
 template <int U>
-__device__ __forceinline__ void eval_i(const QExpr& e, const int64_t (&raw)[kMaxSlots][U],
+__device__ __forceinline__ void eval_i(const QExpr& e, const QSlot* slot, const int64_t (&raw)[kMaxSlots][U],
                                        int64_t (&v)[U]) {
   for (int k = 0; k < e.nfactors; k++) {
     const QFactor f = e.f[k];
@@ -256,6 +293,12 @@ __device__ __forceinline__ void eval_i(const QExpr& e, const int64_t (&raw)[kMax
 #pragma unroll
       for (int u = 0; u < U; u++) v[u] = arith_i(f.combine, v[u], x[u]);
     }
+  }
+  if (e.nwhen) {
+    bool ok[U];
+    eval_when<U>(e, slot, raw, ok);
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = ok[u] ? v[u] : e.else_i;
   }
 }
 
@@ -458,7 +501,7 @@ __global__ __launch_bounds__(kWave) void k_agg_dense(const QDesc d, uint64_t* __
         }
       } else {
         int64_t v[U];
-        eval_i<U>(a.e, raw, v);
+        eval_i<U>(a.e, d.slot, raw, v);
 #pragma unroll
         for (int u = 0; u < U; u++) {
           if (!pass[u]) continue;

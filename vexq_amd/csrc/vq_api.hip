@@ -539,6 +539,27 @@ int add_expr(const vq_table* t, const vq_expr& e, SlotMap& sm, QExpr& out) {
       q.slot = s;
     }
   }
+  // CASE WHEN: predicate terms on the row, the ELSE literal
+  if (e.nwhen < 0 || e.nwhen > VQ_MAX_WHEN) return fail(VQ_E_ARG, "exec: gpu: CASE with %d WHEN terms", e.nwhen);
+  if (e.nwhen > 0 && e.nfactors == 0) return fail(VQ_E_ARG, "exec: gpu: CASE without a THEN expression");
+  out.nwhen = e.nwhen;
+  for (int k = 0; k < e.nwhen; k++) {
+    const vq_term& tm = e.when[k];
+    if (int rc = check_col(t, tm.col)) return rc;
+    if (int rc = check_term(t, tm)) return rc;
+    QWhen& w = out.when[k];
+    w.slot = sm.get(tm.col);
+    w.slot2 = (tm.op & VQ_COLCMP) ? sm.get((int32_t)tm.ival) : 0;
+    if (w.slot < 0 || w.slot2 < 0)
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+    w.op = tm.op;
+    w.ival = tm.ival;
+    w.fval = tm.fval;
+  }
+  if (e.nwhen > 0) {
+    if (e.is_float) out.else_f = e.else_f;
+    else out.else_i = e.else_i;
+  }
   return VQ_OK;
 }
 

@@ -59,10 +59,25 @@ struct QFactor {
   int64_t ival;
 };
 
+// A CASE WHEN guard's term: as QTerm, on slot `slot`.
+struct QWhen {
+  int32_t slot;
+  int32_t op;
+  int32_t slot2;
+  int32_t pad_;
+  int64_t ival;
+  double fval;
+};
+
 struct QExpr {
   int32_t nfactors;
   int32_t is_float;
   QFactor f[VQ_MAX_FACTORS];
+  int32_t nwhen;  // CASE WHEN (conjunction of when[0, nwhen)) THEN fold ELSE else_*
+  int32_t pad_;
+  QWhen when[VQ_MAX_WHEN];
+  double else_f;
+  int64_t else_i;
 };
 
 struct QAcc {
@@ -317,12 +332,15 @@ int32_t topk_tree_nodes(int32_t P);
 
 // Arguments of every run-time compiled kernel; must match vqj::Args in
 // jit_template.hip field for field.
+constexpr int kJitTerms = 8 + 8 * VQ_MAX_WHEN;  // the predicate's, then each aggregate's CASE WHEN terms
 struct JitArgs {
   const void* col[8];
-  int64_t ti[8];
-  double tf[8];
+  int64_t ti[kJitTerms];
+  double tf[kJitTerms];
   int64_t fi[8][4];
   double ff[8][4];
+  int64_t ei[8];  // ELSE literals
+  double ef[8];
   int64_t sp_start[kMaxSpans];
   int64_t sp_rows[kMaxSpans];
   int64_t sp_tile0[kMaxSpans];

@@ -40,10 +40,10 @@ std::mutex g_mu;
 
 bool is_float_op(int op) { return op == ACC_SUM_F || op == ACC_MIN_F || op == ACC_MAX_F; }
 
-void arr(std::string& s, const char* name, const int* v, int n) {
+void arr(std::string& s, const char* name, const int* v, int n, int size = 8) {
   s += "static constexpr int ";
   s += name;
-  s += "[8] = {";
+  s += "[" + std::to_string(size) + "] = {";
   for (int i = 0; i < n; i++) s += (i ? "," : "") + std::to_string(v[i]);
   s += "};\n";
 }
@@ -60,12 +60,14 @@ void arr2(std::string& s, const char* name, const int (*v)[4]) {
   s += "};\n";
 }
 
-// Flattened predicate terms: (slot, op, ival, fval).
+// Flattened terms (slot, op, ival, fval): the predicate's [0, nt), then each
+// accumulator's CASE WHEN terms [aw0[j], aw0[j] + awn[j]).
 struct Flat {
-  int nt = 0;
-  int ts[8] = {0}, to[8] = {0}, ts2[8] = {0};
-  int64_t ti[8] = {0};
-  double tf[8] = {0};
+  int nt = 0, nall = 0;
+  int ts[kJitTerms] = {0}, to[kJitTerms] = {0}, ts2[kJitTerms] = {0};
+  int64_t ti[kJitTerms] = {0};
+  double tf[kJitTerms] = {0};
+  int aw0[8] = {0}, awn[8] = {0};
 };
 
 Flat flatten_terms(const QDesc& d) {
@@ -79,6 +81,20 @@ Flat flatten_terms(const QDesc& d) {
       f.tf[f.nt] = d.slot[c].terms[j].fval;
       f.nt++;
     }
+  f.nall = f.nt;
+  for (int j = 0; j < d.nacc && j < 8; j++) {
+    const QExpr& e = d.acc[j].e;
+    f.aw0[j] = f.nall;
+    f.awn[j] = e.nwhen;
+    for (int k = 0; k < e.nwhen; k++) {
+      f.ts[f.nall] = e.when[k].slot;
+      f.to[f.nall] = e.when[k].op;
+      f.ts2[f.nall] = e.when[k].slot2;
+      f.ti[f.nall] = e.when[k].ival;
+      f.tf[f.nall] = e.when[k].fval;
+      f.nall++;
+    }
+  }
   return f;
 }
 
@@ -131,9 +147,12 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   arr(s, "kKSL", ksl, 8);
   arr(s, "kKSH", ksh, 8);
   arr(s, "kST", st, 8);
-  arr(s, "kTS", f.ts, 8);
-  arr(s, "kTO", f.to, 8);
-  arr(s, "kTS2", f.ts2, 8);
+  s += "#define VQ_TERMS " + std::to_string(kJitTerms) + "\n";
+  arr(s, "kTS", f.ts, f.nall, kJitTerms);
+  arr(s, "kTO", f.to, f.nall, kJitTerms);
+  arr(s, "kTS2", f.ts2, f.nall, kJitTerms);
+  arr(s, "kAW0", f.aw0, 8);
+  arr(s, "kAWN", f.awn, 8);
   arr(s, "kKS", ks, 8);
   arr(s, "kAOP", aop, 8);
   arr(s, "kAISF", aisf, 8);
@@ -303,15 +322,18 @@ Module* get_module(const std::string& pro, std::string* err) {
 void fill_common(const QDesc& d, const Flat& f, JitArgs& a) {
   memset(&a, 0, sizeof a);
   for (int c = 0; c < d.nslots; c++) a.col[c] = d.slot[c].p;
-  for (int t = 0; t < f.nt; t++) {
+  for (int t = 0; t < f.nall; t++) {
     a.ti[t] = f.ti[t];
     a.tf[t] = f.tf[t];
   }
-  for (int j = 0; j < d.nacc; j++)
+  for (int j = 0; j < d.nacc; j++) {
     for (int k = 0; k < d.acc[j].e.nfactors; k++) {
       a.fi[j][k] = d.acc[j].e.f[k].ival;
       a.ff[j][k] = d.acc[j].e.f[k].fval;
     }
+    a.ei[j] = d.acc[j].e.else_i;
+    a.ef[j] = d.acc[j].e.else_f;
+  }
   for (int s = 0; s < d.nspans; s++) {
     a.sp_start[s] = d.span[s].start;
     a.sp_rows[s] = d.span[s].rows;
