@@ -352,6 +352,31 @@ int vq_join_agg(vq_ctx* ctx, const vq_table* build, int32_t build_key, const vq_
                 int32_t naggs, const vq_agg* aggs, int32_t radix_bits, int32_t topk,
                 vq_agg_result* out, vq_stats* stats);
 
+/* ---- join + GROUP BY columns of the joined rows (the Q12 shape) -----------
+ * ParallelHashJoinAggregate with its group keys and aggregates over the
+ * joined schema (exec/parallel_join.go:665-758,780-988; the Q12 benchmark,
+ * planner/parallel_join_bench_test.go:19-30): inner equi-join of
+ * build.build_key with probe.probe_key (integer columns, compared as int64),
+ * each side filtered by its own predicate (on its own columns), GROUP BY
+ * nkeys columns of the probe side (dictionary codes packed as the int-key
+ * path packs them, exec/agg_intkey.go:67-100, or one integer column),
+ * aggregates over expressions of either side: a build column is written
+ * col | VQ_BUILD_COL in a factor, a CASE WHEN term, or the other column of a
+ * VQ_COLCMP term.  Each join output row (a probe row times every build row
+ * with its key) is folded once.  Groups come out in the order of their first
+ * joined probe row (first_row); keys as vq_agg_dict's (nkeys codes) or
+ * vq_agg_hash's (one key).  Float sums are exact (fixed point, one rounding),
+ * so the result does not depend on the order matches are found in.  The
+ * build side goes into a device hash table (key -> its rows); the probe side
+ * runs through the few-key GROUP BY's per-workgroup partials, so the group
+ * keys are limited as there (8 passes of 256-1024 keys): more returns
+ * VQ_E_UNSUPPORTED.  A query touches at most 8 columns of the two tables. */
+#define VQ_BUILD_COL 0x10000
+int vq_join_group(vq_ctx* ctx, const vq_table* build, int32_t build_key, const vq_pred* build_pred,
+                  const vq_table* probe, int32_t probe_key, const vq_pred* probe_pred,
+                  int32_t nkeys, const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
+                  vq_agg_result* out, vq_stats* stats);
+
 /* ---- concurrent and asynchronous queries (serving) ------------------------
  * No counterpart in the reference's exec package: its operators run one
  * query per call on the caller's goroutine.  A served query stream

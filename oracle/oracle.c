@@ -1085,6 +1085,95 @@ int or_join_agg(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, int
   return rc;
 }
 
+static inline int64_t type_width(int32_t type) {
+  return type == VQ_U8 ? 1 : type == VQ_U16 ? 2 : type == VQ_DATE ? 4 : 8;
+}
+
+/* A column reference of a joined-row expression in the joined column space
+ * (probe columns, then build columns: col | VQ_BUILD_COL is column
+ * p_ncols + col); -1 when out of range. */
+static int32_t joined_col(int64_t col, int32_t p_ncols, int32_t b_ncols) {
+  if (col & VQ_BUILD_COL) {
+    const int64_t b = col & ~(int64_t)VQ_BUILD_COL;
+    return b >= 0 && b < b_ncols ? (int32_t)(p_ncols + b) : -1;
+  }
+  return col >= 0 && col < p_ncols ? (int32_t)col : -1;
+}
+
+/* HashJoin (exec/join.go:84-186) feeding HashAggregate with its group keys
+ * and aggregates over the joined schema, as ParallelHashJoinAggregate runs
+ * them per probe morsel (exec/parallel_join.go:780-988): build rows passing
+ * their predicate inserted in row order, probe rows visited in row order,
+ * each matching build row yielding one joined row; a joined row's columns
+ * are the probe row's and the build row's (the expression evaluated over a
+ * one-row view of the two). */
+int or_join_group(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, int32_t b_key,
+                  const vq_pred* b_pred, int64_t p_nrows, int32_t p_ncols,
+                  const vq_col_desc* p_cols, int32_t p_key, const vq_pred* p_pred, int32_t nkeys,
+                  const int32_t* key_cols, int32_t naggs, const vq_agg* aggs, int32_t exact,
+                  vq_agg_result* out) {
+  if (naggs < 0 || naggs > VQ_MAX_AGGS || nkeys < 1 || nkeys > 8) return VQ_E_ARG;
+  if (nkeys > 1 && !keys_fit(p_cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
+  /* the aggregates over the joined column space */
+  vq_agg ja[VQ_MAX_AGGS];
+  for (int j = 0; j < naggs; j++) {
+    ja[j] = aggs[j];
+    vq_expr* e = &ja[j].arg;
+    for (int k = 0; k < e->nfactors; k++) {
+      if (e->f[k].kind == VQ_F_LIT) continue;
+      if ((e->f[k].col = joined_col(e->f[k].col, p_ncols, b_ncols)) < 0) return VQ_E_ARG;
+    }
+    for (int k = 0; k < e->nwhen; k++) {
+      if ((e->when[k].col = joined_col(e->when[k].col, p_ncols, b_ncols)) < 0) return VQ_E_ARG;
+      if (e->when[k].op & VQ_COLCMP) {
+        const int32_t c2 = joined_col(e->when[k].ival, p_ncols, b_ncols);
+        if (c2 < 0) return VQ_E_ARG;
+        e->when[k].ival = c2;
+      }
+    }
+  }
+  int32_t* next = (int32_t*)malloc(sizeof(int32_t) * (b_nrows ? b_nrows : 1));
+  jtable t;
+  jt_init(&t, 8, next);
+  for (int64_t r = 0; r < b_nrows; r++)
+    if (pred_pass(b_cols, b_pred, r)) jt_insert(&t, col_i(&b_cols[b_key], r), (int32_t)r);
+  const int32_t nj = p_ncols + b_ncols;
+  vq_col_desc* view = (vq_col_desc*)malloc(sizeof(vq_col_desc) * (size_t)nj);
+  for (int32_t c = 0; c < nj; c++) view[c] = c < p_ncols ? p_cols[c] : b_cols[c - p_ncols];
+  kmap m;
+  kmap_init(&m, 1024);
+  glist l = {0, 0, 0};
+  if (t.keys > 0)
+    for (int64_t r = 0; r < p_nrows; r++) {
+      if (!pred_pass(p_cols, p_pred, r)) continue;
+      const int64_t key = col_i(&p_cols[p_key], r);
+      const int64_t gk = nkeys > 1 ? pack_key(p_cols, nkeys, key_cols, r) : col_i(&p_cols[key_cols[0]], r);
+      for (int32_t b = jt_lookup(&t, key, or_radix_hash(key)); b != -1; b = next[b]) {
+        for (int32_t c = 0; c < p_ncols; c++)
+          view[c].data = (const char*)p_cols[c].data + r * type_width(p_cols[c].type);
+        for (int32_t c = 0; c < b_ncols; c++)
+          view[p_ncols + c].data = (const char*)b_cols[c].data + (int64_t)b * type_width(b_cols[c].type);
+        group* g = kmap_group(&m, &l, gk, naggs, ja, exact);
+        if (g->first_row < 0) {
+          g->first_row = r;
+          if (nkeys > 1) unpack_key(gk, nkeys, g->key);
+          else g->key[0] = gk;
+        }
+        fold_row(g, view, naggs, ja, 0);
+      }
+    }
+  int rc = check_cap(out, l.n);
+  if (rc == VQ_OK)
+    for (int64_t i = 0; i < l.n; i++) emit_group(&l.g[i], nkeys, naggs, ja, out, i);
+  for (int64_t i = 0; i < l.n; i++) group_free(&l.g[i], naggs);
+  free(l.g);
+  kmap_free(&m);
+  free(view);
+  free(t.slots);
+  free(next);
+  return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* multithreaded CPU baseline                                                */
 /* ------------------------------------------------------------------------ */

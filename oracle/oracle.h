@@ -69,6 +69,17 @@ int or_join_agg(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, int
                 const vq_col_desc* p_cols, int32_t p_key, const vq_pred* p_pred, int32_t naggs,
                 const vq_agg* aggs, int32_t topk, int32_t exact, vq_agg_result* out);
 
+/* Serial HashJoin -> HashAggregate over the joined rows (the Q12 shape of
+ * ParallelHashJoinAggregate): GROUP BY nkeys probe columns (codes packed as
+ * the int-key path packs them when nkeys > 1), aggregates over columns of
+ * either side (col | VQ_BUILD_COL names a build column); groups in the order
+ * of their first joined probe row. */
+int or_join_group(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, int32_t b_key,
+                  const vq_pred* b_pred, int64_t p_nrows, int32_t p_ncols,
+                  const vq_col_desc* p_cols, int32_t p_key, const vq_pred* p_pred, int32_t nkeys,
+                  const int32_t* key_cols, int32_t naggs, const vq_agg* aggs, int32_t exact,
+                  vq_agg_result* out);
+
 /* Multithreaded morsel restatement (CPU baseline): ParallelHashAggregate and
  * ParallelHashJoinAggregate with 64K-row morsels on an atomic cursor,
  * thread-local partial aggregates merged serially, and the two-pass radix

@@ -72,6 +72,8 @@ def lib():
                  C.c_int32, P(abi.ColDesc), C.c_int32, P(abi.Pred), C.c_int32, P(abi.Agg),
                  C.c_int32]
     L.or_join_agg.argtypes = join_args + [C.c_int32, P(abi.AggResult)]
+    L.or_join_group.argtypes = join_args[:10] + [C.c_int32, P(C.c_int32), C.c_int32, P(abi.Agg), C.c_int32,
+                                                 P(abi.AggResult)]
     L.or_join_agg_parallel.argtypes = join_args + [C.c_int32, P(abi.AggResult)]
     _lib = L
     return L
@@ -189,6 +191,26 @@ def join_agg(build, build_key, build_pred, probe, probe_key, probe_pred, aggs, t
         rc = lib().or_join_agg(*args, int(exact), C.byref(out.c))
     if rc:
         raise RuntimeError(f"oracle join_agg failed: {rc}")
+    return out
+
+
+def join_group(build, build_key, build_pred, probe, probe_key, probe_pred, key_cols, aggs,
+               exact: bool = True, capacity: int | None = None) -> abi.AggOut:
+    """or_join_group: GROUP BY probe columns over the joined rows, aggregates
+    over either side (build columns flagged VQ_BUILD_COL; abi.JoinView builds
+    them by name)."""
+    bd = abi.col_descs(build)
+    pd = abi.col_descs(probe)
+    arr = abi.agg_array(aggs)
+    karr = (C.c_int32 * len(key_cols))(*key_cols)
+    if capacity is None:
+        capacity = max(1, probe.nrows)
+    out = abi.AggOut(capacity, len(key_cols), len(aggs))
+    rc = lib().or_join_group(build.nrows, len(build.columns), bd, build_key, _pp(build_pred), probe.nrows,
+                             len(probe.columns), pd, probe_key, _pp(probe_pred), len(key_cols), karr,
+                             len(aggs), arr, int(exact), C.byref(out.c))
+    if rc:
+        raise RuntimeError(f"oracle join_group failed: {rc}")
     return out
 
 

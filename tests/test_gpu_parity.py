@@ -199,6 +199,25 @@ def test_case_when_aggregates(gpu_ctx, kernel_path):
     assert_agg_equal(top, O.join_agg(b, 0, bp, p, 0, pp, jaggs, topk=10, exact=True), jaggs)
 
 
+def test_case_else_beyond_the_then_range(gpu_ctx):
+    """Exact (fixed-point) sums size their limbs from the argument's bounds:
+    an ELSE literal far outside the THEN expression's range must widen them
+    (hash GROUP BY by partials and partitions, and the join)."""
+    from test_oracle import q12_table
+    _print_case("case_else_range")
+    X = abi.X
+    t = q12_table(200_000, seed=19)
+    aggs = [abi.agg_sum(X.case([("quantity", "<", 10)], X.col("price"), -1.0e9), t),
+            abi.agg_avg(X.case([("discount", ">", 0.02)], X.col("discount"), 3.5e7), t)]
+    dt = gpu_ctx.upload(t)
+    try:
+        for key in ("shipmode", "shipdate"):
+            got = gpu_ctx.agg_hash(dt, None, t.index(key), aggs)
+            assert_agg_equal(got, O.agg_hash(t, None, t.index(key), aggs, exact=True), aggs)
+    finally:
+        dt.close()
+
+
 def test_filter_pruned_sorted(gpu_ctx):
     t = datagen.lineitem_sorted(1_000_000, seed=3)
     dt = gpu_ctx.upload(t)

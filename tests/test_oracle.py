@@ -369,6 +369,121 @@ def test_case_when_sum_equals_the_filtered_sum():
         assert ha.fvals[g, 0] == hb.fvals[j, 0] and ha.ivals[g, 1] == hb.ivals[j, 1]
 
 
+def q12_join_tables(nb, npr, seed=16):
+    """orders (build: key, orderdate, priority codes of 5 strings) and a
+    lineitem probe side with Q12's columns (ship mode codes, commit and
+    receipt dates)."""
+    b = datagen.orders(nb, seed)
+    p = datagen.lineitem_probe(b, npr, seed)
+    ship = p.col("shipdate").astype(np.int64)
+    n = p.nrows
+    commit = ship + datagen.rand_int(seed, 93, n, -30, 30)
+    receipt = ship + datagen.rand_int(seed, 94, n, 1, 30)
+    mode = datagen.rand_int(seed, 95, n, 0, 6).astype(np.uint8)
+    p = datagen.Table(list(p.columns) + [datagen.Column("commitdate", abi.VQ_DATE, commit.astype(np.int32)),
+                                         datagen.Column("receiptdate", abi.VQ_DATE, receipt.astype(np.int32)),
+                                         datagen.Column("shipmode", abi.VQ_U8, mode, 7)],
+                      datagen.row_groups(n))
+    return b, p
+
+
+def q12_query(b, p, lo=9000, hi=9365):
+    """TPC-H Q12's shape (planner/parallel_join_bench_test.go:19-30): GROUP BY
+    l_shipmode, SUM(CASE WHEN o_orderpriority IN ('1-URGENT', '2-HIGH') THEN 1
+    ELSE 0), SUM(CASE WHEN ... <> AND <> ... THEN 1 ELSE 0), plus a
+    build-side MAX and an average under a guard comparing a build column with
+    a probe column: eight columns, a query's limit."""
+    X = abi.X
+    v = abi.JoinView(p, b)
+    urgent = [b.columns[b.index("priority")].dictionary.index(s) for s in ("1-URGENT", "2-HIGH")]
+    pp = abi.make_pred(p, [("shipmode", "in", [1, 5]), ("commitdate", "<", X.col("receiptdate")),
+                           ("shipdate", "<", X.col("commitdate")), ("receiptdate", ">=", lo),
+                           ("receiptdate", "<", hi)])
+    aggs = [abi.count_star(v),
+            abi.agg_sum(X.case([("b.priority", "in", urgent)], 1, 0), v),
+            abi.agg_sum(X.case([("b.priority", "<>", urgent[0]), ("b.priority", "<>", urgent[1])], 1, 0), v),
+            abi.agg_max("b.orderdate", v),
+            abi.agg_avg(X.case([("b.orderdate", "<", X.col("commitdate"))], X.col("quantity"), 0), v)]
+    return pp, [p.index("shipmode")], aggs, urgent
+
+
+def test_join_group_q12_matches_numpy():
+    b, p = q12_join_tables(20_000, 80_000)
+    pp, keys, aggs, urgent = q12_query(b, p)
+    r = O.join_group(b, 0, None, p, 0, pp, keys, aggs, exact=True)
+    # numpy: the unique build key 1 + 4 i is build row i
+    d = {c.name: c.data for c in p.columns}
+    m = (np.isin(d["shipmode"], [1, 5]) & (d["commitdate"] < d["receiptdate"]) & (d["shipdate"] < d["commitdate"])
+         & (d["receiptdate"] >= 9000) & (d["receiptdate"] < 9365))
+    rows = np.nonzero(m)[0]
+    brow = (d["fk"][rows] - 1) // 4
+    prio = b.col("priority")[brow]
+    odate = b.col("orderdate")[brow].astype(np.int64)
+    mode = d["shipmode"][rows]
+    order = [int(x) for x in dict.fromkeys(mode.tolist())]  # first-occurrence order
+    assert r.ngroups == len(order) and list(r.keys[:, 0]) == order
+    for gi, g in enumerate(order):
+        s = mode == g
+        assert r.counts[gi] == int(s.sum())
+        assert r.first_row[gi] == int(rows[s][0])
+        hi = np.isin(prio[s], urgent)
+        assert int(r.ivals[gi, 1]) == int(hi.sum()) and int(r.ivals[gi, 2]) == int((~hi).sum())
+        assert int(r.ivals[gi, 3]) == int(odate[s].max())
+        qv = np.where(odate[s] < d["commitdate"][rows][s], d["quantity"][rows][s], 0)
+        assert r.fvals[gi, 4] == math.fsum(qv.astype(np.float64)) / s.sum()
+
+
+def test_join_group_duplicate_build_keys_pure_python():
+    """Build keys repeating (each probe row meets several build rows) and
+    INT64_MIN / 0 keys: a pure-Python nested-loop join."""
+    rng = np.random.default_rng(17)
+    nb, npr = 300, 900
+    bkey = rng.integers(0, 60, nb).astype(np.int64)
+    bkey[:5] = np.iinfo(np.int64).min
+    bval = rng.integers(-1000, 1000, nb).astype(np.int64)
+    pkey = rng.integers(0, 70, npr).astype(np.int64)
+    pkey[::97] = np.iinfo(np.int64).min
+    code = rng.integers(0, 4, npr).astype(np.uint8)
+    pval = rng.standard_normal(npr)
+    b = _table([("k", abi.VQ_I64, bkey, 0), ("v", abi.VQ_I64, bval, 0)], [nb])
+    p = _table([("k", abi.VQ_I64, pkey, 0), ("c", abi.VQ_U8, code, 4), ("x", abi.VQ_F64, pval, 0)], [npr])
+    v = abi.JoinView(p, b)
+    X = abi.X
+    aggs = [abi.count_star(v), abi.agg_sum("b.v", v), abi.agg_sum(X.col("x") * 2, v),
+            abi.agg_min(X.case([("b.v", ">", 0)], X.col("b.v"), 5000), v)]
+    bp = abi.make_pred(b, [("v", "!=", 7)])
+    r = O.join_group(b, 0, bp, p, 0, None, [1], aggs, exact=True)
+    groups = {}
+    for i in range(npr):
+        for j in range(nb):
+            if bkey[j] == pkey[i] and bval[j] != 7:
+                g = groups.setdefault(int(code[i]), [0, 0, [], 5000, i])
+                g[0] += 1
+                g[1] += int(bval[j])
+                g[2].append(pval[i] * 2)
+                g[3] = min(g[3], int(bval[j]) if bval[j] > 0 else 5000)
+    order = sorted(groups, key=lambda k: groups[k][4])
+    assert list(r.keys[:, 0]) == order
+    for gi, k in enumerate(order):
+        c, sv, xs, mn, first = groups[k]
+        assert (r.counts[gi], r.ivals[gi, 1], r.ivals[gi, 3], r.first_row[gi]) == (c, sv, mn, first)
+        assert r.fvals[gi, 2] == math.fsum(xs)
+
+
+def test_join_group_by_the_join_key_is_the_join_aggregate():
+    """GROUP BY the probe's join key over the joined rows is or_join_agg's
+    grouping: the two oracle paths agree bit for bit."""
+    b, p = q12_join_tables(5_000, 20_000, seed=18)
+    bp, pp, aggs = datagen.q_join(b, p)
+    aggs = aggs + [abi.count_star(p), abi.agg_max("quantity", p)]
+    a = O.join_group(b, 0, bp, p, 0, pp, [p.index("fk")], aggs, exact=True)
+    j = O.join_agg(b, 0, bp, p, 0, pp, aggs, exact=True)
+    assert a.ngroups == j.ngroups
+    for f in ("keys", "counts", "first_row", "ivals", "valid"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(j, f))
+    np.testing.assert_array_equal(a.fvals.view(np.int64), j.fvals.view(np.int64))
+
+
 # ---- dictionary-code GROUP BY -----------------------------------------------
 
 def test_intkey_remap_kat():

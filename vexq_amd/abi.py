@@ -18,6 +18,7 @@ VQ_U8, VQ_U16, VQ_DATE, VQ_I64, VQ_F64 = 1, 2, 3, 4, 5
 VQ_EQ, VQ_NE, VQ_LT, VQ_LE, VQ_GT, VQ_GE = range(6)
 VQ_IN = 6        # column IN (set of values in [0, 64)): ival is the mask
 VQ_COLCMP = 16   # op | VQ_COLCMP: column OP column ival
+VQ_BUILD_COL = 0x10000  # col | VQ_BUILD_COL: a build-side column (vq_join_group)
 VQ_F_COL, VQ_F_LIT, VQ_F_LIT_OP_COL, VQ_F_COL_OP_LIT = range(4)
 VQ_ADD, VQ_SUB, VQ_MUL, VQ_DIV = range(4)
 VQ_AGG_COUNT, VQ_AGG_SUM, VQ_AGG_AVG, VQ_AGG_MIN, VQ_AGG_MAX = range(5)
@@ -275,6 +276,30 @@ class X:
                 else:
                     cf.ival = int(f.lit)
         return e
+
+
+class JoinView:
+    """The column names of a GROUP BY over joined rows (vq_join_group):
+    probe columns by their names, build columns as "b.<name>", which resolve
+    to col | VQ_BUILD_COL.  Pass it as the table to X.to_c / agg_* /
+    make_pred-style builders."""
+
+    def __init__(self, probe, build):
+        self.probe, self.build = probe, build
+        self.columns = _JoinCols(probe, build)
+
+    def index(self, name: str) -> int:
+        if name.startswith("b."):
+            return VQ_BUILD_COL | self.build.index(name[2:])
+        return self.probe.index(name)
+
+
+class _JoinCols:
+    def __init__(self, probe, build):
+        self.probe, self.build = probe, build
+
+    def __getitem__(self, i):
+        return self.build.columns[i & ~VQ_BUILD_COL] if i & VQ_BUILD_COL else self.probe.columns[i]
 
 
 def _agg(kind, x, table) -> Agg:

@@ -30,6 +30,9 @@
 #define VQ_BLOOM 1
 #define VQ_LIVEBITS 1
 #define VQ_NKEY 1
+#define VQ_LOOKUP 0
+#define VQ_JKEY 0
+static constexpr int kSB[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kKSL[8] = {2, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kKSH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
@@ -147,6 +150,16 @@ struct Args {
   u64* livebits;
   int lk0;
   int lpad_;
+  // VQ_LOOKUP (vq_join_group): the build side's lookup table -- open
+  // addressing on radix_hash(key) >> 32, key INT64_MIN marks an empty slot;
+  // head[slot] starts the slot's chain of build rows, next[row] continues it
+  // (0xFFFFFFFF ends it); lk_min[0] starts the chain of key INT64_MIN
+  const i64* lk_keys;
+  const u32* lk_head;
+  const u32* lk_next;
+  const u32* lk_min;
+  u32 lk_mask;
+  u32 lk_pad;
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -1201,11 +1214,44 @@ namespace vqj {
 constexpr int kLT = 512;  // threads per workgroup
 constexpr u32 kLocalOverflow = 4u;
 
+// A probe-side slot's value of row r (VQ_LOOKUP: a build-side slot is
+// loaded for each matching build row instead).
+template <int C, int U>
+__device__ __forceinline__ void lload_slot(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u) {
+  if constexpr (kSB[C]) raw[C][u] = 0;
+  else raw[C][u] = ld<kST[C]>(a.col[C], r);
+}
+
 template <int U, int... C>
 __device__ __forceinline__ void lload_one(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u,
                                           Seq<C...>) {
-  ((raw[C][u] = ld<kST[C]>(a.col[C], r)), ...);
+  (lload_slot<C, U>(a, r, raw, u), ...);
 }
+
+#if VQ_LOOKUP
+// The first build row with this key (0xFFFFFFFF: none).
+__device__ __forceinline__ u32 lookup_head(const Args& a, i64 key) {
+  if (key == kI64Min) return *a.lk_min;
+  u32 i = (u32)(radix_hash((u64)key) >> 32) & a.lk_mask;
+  for (u32 pr = 0; pr <= a.lk_mask; pr++) {
+    const i64 k = a.lk_keys[i];
+    if (k == key) return a.lk_head[i];
+    if (k == kI64Min) break;
+    i = (i + 1) & a.lk_mask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+template <int C>
+__device__ __forceinline__ void build_slot(const Args& a, i64 b, i64* row) {
+  if constexpr (kSB[C]) row[C] = ld<kST[C]>(a.col[C], b);
+}
+
+template <int... C>
+__device__ __forceinline__ void build_load(const Args& a, i64 b, i64* row, Seq<C...>) {
+  (build_slot<C>(a, b, row), ...);
+}
+#endif
 
 // Rows base + u*kLT + tid, clamped into the tile.
 template <int U>
@@ -1385,13 +1431,21 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
         if (whole) atomicOr(&a.lflags[0], kLocalOverflow);
         continue;
       }
-      atomicAdd(&cnt[(u64)g * S + c], 1u);
-      atomicMin(&first[(u64)g * S + c], (u32)rr);
-#if VQ_NV > 0
-      u64 v[VQ_NV];
-      values_of(a, row, v, seq<VQ_NV>{});
-      local_folds(a, acc, S, g, c, v, seq<VQ_NAG>{});
+#if VQ_LOOKUP
+      // every build row with the probe row's join key: one joined row each
+      for (u32 b = lookup_head(a, row[VQ_JKEY]); b != 0xFFFFFFFFu; b = a.lk_next[b]) {
+        build_load(a, (i64)b, row, seq<VQ_NS>{});
+#else
+      {
 #endif
+        atomicAdd(&cnt[(u64)g * S + c], 1u);
+        atomicMin(&first[(u64)g * S + c], (u32)rr);
+#if VQ_NV > 0
+        u64 v[VQ_NV];
+        values_of(a, row, v, seq<VQ_NV>{});
+        local_folds(a, acc, S, g, c, v, seq<VQ_NAG>{});
+#endif
+      }
     }
   }
   __syncthreads();

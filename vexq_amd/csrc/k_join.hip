@@ -1310,8 +1310,9 @@ size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords) {
 }
 
 hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
-                        const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
-  if (ev) hipEventRecord(ev[0], s);
+                        const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev,
+                        bool ev0_done) {
+  if (ev && !ev0_done) hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_local_init, dim3((unsigned)std::max(1, plan.gcap / 256)), dim3(256), 0, s,
                      B.l.keys, plan.gcap, B.flags);
   LocalKeys k;

@@ -11,6 +11,7 @@
 // fixed function of its row index, lanes combine in a fixed butterfly, and the
 // cross-tile reduction runs in tile order with double-double compensation.
 // Reruns of one query on one table therefore give identical bits.
+#include <algorithm>
 #include <cstdlib>
 
 #include "vq_internal.h"
@@ -745,6 +746,77 @@ hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part
   const int NI = d.nacc + 2;
   hipLaunchKernelGGL(k_agg_final, dim3((unsigned)(d.ngroups * NI)), dim3(256), 0, s, d, nparts,
                      part, out);
+  return hipGetLastError();
+}
+
+// ---- the build side of a join + GROUP BY of the joined rows ---------------
+// A device hash table from key to its rows (the chained table of HashJoin's
+// build, exec/join_table.go:400-569): rows passing the build predicate are
+// chained under their key, head[slot] the last one inserted and next[row]
+// the one before.  Chain order is arrival order; the probe side folds every
+// match exactly (integer and fixed-point words), so it does not matter.
+__global__ __launch_bounds__(256) void k_lookup_init(const LookupBufs lk) {
+  const uint32_t n = lk.mask + 1;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    lk.keys[i] = INT64_MIN;
+    lk.head[i] = 0xFFFFFFFFu;
+    lk.cnt[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 4) lk.min_head[threadIdx.x] = threadIdx.x == 0 ? 0xFFFFFFFFu : 0u;
+}
+
+__device__ __forceinline__ void lookup_insert(const LookupBufs& lk, int64_t key, uint32_t r) {
+  if (key == INT64_MIN) {  // (the empty slots' mark: kept apart)
+    lk.next[r] = atomicExch(&lk.min_head[0], r);
+    atomicAdd(&lk.min_head[3], 1u);
+    return;
+  }
+  uint32_t i = (uint32_t)(radix_hash((uint64_t)key) >> 32) & lk.mask;
+  for (uint32_t pr = 0; pr <= lk.mask; pr++) {
+    const unsigned long long old = atomicCAS((unsigned long long*)&lk.keys[i],
+                                             (unsigned long long)INT64_MIN, (unsigned long long)key);
+    if (old == (unsigned long long)INT64_MIN || old == (unsigned long long)key) {
+      lk.next[r] = atomicExch(&lk.head[i], r);
+      atomicMax(&lk.min_head[2], atomicAdd(&lk.cnt[i], 1u) + 1u);  // the key's rows so far
+      return;
+    }
+    i = (i + 1) & lk.mask;
+  }
+  atomicOr(&lk.min_head[1], 1u);  // (the host sizes the table above the rows: not reached)
+}
+
+// One tile per wave (grid = ntiles), as k_filter_bits.
+template <int U>
+__global__ __launch_bounds__(kWave) void k_lookup_build(const QDesc d, int32_t key_slot, const LookupBufs lk) {
+  const int lane = threadIdx.x;
+  const int NS = d.nslots;
+  SlotLd ld[kMaxSlots];
+#pragma unroll
+  for (int c = 0; c < kMaxSlots; c++) ld[c] = slot_ld(d.slot[c < NS ? c : 0]);
+  TileIt it;
+  tile_begin(d, blockIdx.x, it);
+  const int64_t t0 = it.t;
+  while (it.have && it.t == t0) {
+    int64_t raw[kMaxSlots][U];
+    load_rows<U>(ld, NS, it.base, it.rend, lane, raw);
+    bool pass[U];
+    int g[U];
+    eval_pred<U>(d.slot, NS, it.base, it.rend, lane, raw, pass, g);
+    int64_t key[U];
+    pick<U>(raw, key_slot, key);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (pass[u]) lookup_insert(lk, key[u], (uint32_t)(it.base + u * kWave + lane));
+    it.base += kWave * U;
+    if (it.base >= it.rend) it.have = false;
+  }
+}
+
+hipError_t launch_lookup_build(const QDesc& d, int32_t key_slot, const LookupBufs& lk, hipStream_t s) {
+  const uint32_t blocks = std::min<uint32_t>(2048, (lk.mask + 256) / 256);
+  hipLaunchKernelGGL(k_lookup_init, dim3(blocks), dim3(256), 0, s, lk);
+  if (d.ntiles > 0)
+    hipLaunchKernelGGL((k_lookup_build<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, key_slot, lk);
   return hipGetLastError();
 }
 

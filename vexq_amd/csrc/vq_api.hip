@@ -1426,6 +1426,10 @@ Interval expr_interval(const vq_table* t, const vq_expr& e) {
     }
     v = k == 0 ? x : iv_op(f.combine, v, x);
   }
+  if (e.nwhen > 0) {  // CASE WHEN: the ELSE literal is a value too
+    const double el = e.is_float ? e.else_f : (double)e.else_i;
+    v = Interval{std::min(v.lo, el), std::max(v.hi, el), v.ok && std::isfinite(el)};
+  }
   return v;
 }
 
@@ -1454,7 +1458,8 @@ std::string build_est_key(const vq_table* t, int32_t key, const vq_pred* pred) {
 
 int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg* aggs,
               int32_t naggs, JSide& sd, JDesc* jd, SlotMap& sm, int32_t npack = 0,
-              const int32_t* pack_cols = nullptr, const int32_t* pack_shift = nullptr) {
+              const int32_t* pack_cols = nullptr, const int32_t* pack_shift = nullptr,
+              int32_t jkey = -1) {
   memset(&sd, 0, sizeof sd);
   if (int rc = check_col(t, key)) return rc;
   if (t->cols[key].type == VQ_F64) return fail(VQ_E_UNSUPPORTED, "exec: gpu: float join key");
@@ -1477,6 +1482,13 @@ int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg*
     for (int j = 0; j < naggs; j++)
       if (aggs[j].kind != VQ_AGG_COUNT)
         if (int rc = add_expr(t, aggs[j].arg, sm, jd->agg[j].e)) return rc;
+  }
+  if (jkey >= 0) {  // a join + GROUP BY of the joined rows: the probe's join key
+    if (int rc = check_col(t, jkey)) return rc;
+    const int s = sm.get(jkey);
+    if (s < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+    sd.lookup = 1;
+    sd.jkey_slot = s;
   }
   fill_slot_cols(t, sm, sd.slot);
   sd.nslots = sm.n;
@@ -1550,6 +1562,20 @@ struct JoinJob {
   size_t host_bytes = 0;   // pinned result staging
   uint32_t* hflags = nullptr;
   uint64_t* hout = nullptr;
+  // vq_join_group: the GROUP BY runs over `view` (the probe's columns, then
+  // the build's, over the probe's row groups) with the build side in a
+  // lookup table: lq its predicate's and key's slots over its kept spans
+  bool lookup = false;
+  vq_table view;
+  const vq_table* lprobe = nullptr;
+  const vq_table* lbuild = nullptr;
+  int32_t lbuild_key = 0, ljkey = 0, lsplit = 0, lkey_slot = 0;
+  vq_pred lbpred;
+  const vq_pred* lbp = nullptr;
+  QDesc lq;
+  uint32_t lcap = 0;
+  int64_t lrows = 0, lkept = 0, lrow_bytes = 0;
+  int64_t lmult = 1;  // the most build rows one key has, as planned
 };
 
 // Table fill: a GROUP BY's rows all find their keys, but fill the table by
@@ -1592,24 +1618,13 @@ int64_t local_max_keys(int32_t nwords) {
   return (int64_t)limit * local_passes_max(limit);
 }
 
-int join_setup(JoinJob& j) {
-  vq_ctx* c = j.c;
-  if (!c || (!j.build && !j.groupby) || !j.probe) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
-  if ((!j.groupby && j.build->ctx != c) || j.probe->ctx != c)
-    return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
-  if (j.naggs < 0 || j.naggs > VQ_MAX_AGGS) return fail(VQ_E_ARG, "exec: gpu: %d aggregates", j.naggs);
-  if (j.topk < 0) return fail(VQ_E_ARG, "exec: gpu: top-k of %d", j.topk);
-  if (j.radix_bits > 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d radix bits", j.radix_bits);
-  memset(&j.jd, 0, sizeof j.jd);
-  j.jd.naggs = j.naggs;
-  memset(&j.bs, 0, sizeof j.bs);
-  if (!j.groupby)
-    if (int rc = plan_side(j.build, j.build_key, j.bp, nullptr, 0, j.bs, nullptr, j.bsm)) return rc;
-  if (int rc = plan_side(j.probe, j.probe_key, j.pp, j.aggs, j.naggs, j.ps, &j.jd, j.psm, j.npack,
-                         j.pack_cols, j.pack_shift))
-    return rc;
-  // accumulator words and fixed-point scales
-  const int L = ilog2_ceil((uint64_t)j.probe->nrows + 1);
+// Accumulator words and fixed-point scales: a float sum's three limbs of w
+// bits leave 63 - w bits of headroom each, so w follows the additions a
+// group can take (the probe's rows).
+int join_scales(JoinJob& j) {
+  // (a join + GROUP BY of joined rows: each probe row may meet lmult build
+  // rows, each one addition)
+  const int L = ilog2_ceil((uint64_t)j.probe->nrows * (uint64_t)std::max<int64_t>(j.lmult, 1) + 1);
   int nw = 0, nvals = 0;
   for (int k = 0; k < j.naggs; k++) {
     if (int rc = check_agg(j.aggs[k])) return rc;
@@ -1648,6 +1663,30 @@ int join_setup(JoinJob& j) {
   j.jd.nwords = nw;
   j.nw = nw;
   j.nvals = nvals;
+  return VQ_OK;
+}
+
+int join_setup(JoinJob& j) {
+  vq_ctx* c = j.c;
+  if (!c || (!j.build && !j.groupby) || !j.probe) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if ((!j.groupby && j.build->ctx != c) || j.probe->ctx != c)
+    return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  if (j.naggs < 0 || j.naggs > VQ_MAX_AGGS) return fail(VQ_E_ARG, "exec: gpu: %d aggregates", j.naggs);
+  if (j.topk < 0) return fail(VQ_E_ARG, "exec: gpu: top-k of %d", j.topk);
+  if (j.radix_bits > 16) return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d radix bits", j.radix_bits);
+  memset(&j.jd, 0, sizeof j.jd);
+  j.jd.naggs = j.naggs;
+  memset(&j.bs, 0, sizeof j.bs);
+  if (!j.groupby)
+    if (int rc = plan_side(j.build, j.build_key, j.bp, nullptr, 0, j.bs, nullptr, j.bsm)) return rc;
+  if (int rc = plan_side(j.probe, j.probe_key, j.pp, j.aggs, j.naggs, j.ps, &j.jd, j.psm, j.npack,
+                         j.pack_cols, j.pack_shift, j.lookup ? j.ljkey : -1))
+    return rc;
+  if (j.lookup)  // the slots that hold build-side columns
+    for (int i = 0; i < j.psm.n; i++)
+      if (j.psm.col_of[i] >= j.lsplit) j.ps.build_slots |= 1 << i;
+  if (int rc = join_scales(j)) return rc;
+  const int nw = j.nw;
   // Partitioning plan.  Each partition's table lives in one workgroup's LDS:
   // cap slots of key and tag (multiplicity, then pool index), plus a pool of
   // accumulator entries (count, first row, multiplicity, accumulator words)
@@ -1660,8 +1699,9 @@ int join_setup(JoinJob& j) {
   // of a wave (a wave waits for its longest probe).  The pool starts at half the table, or at
   // twice the matched keys per partition of the last query of the same shape,
   // and doubles when it fills; a table that fills adds one bit.
-  j.est_key = j.groupby ? "G|" + build_est_key(j.probe, j.probe_key, j.pp)
+  j.est_key = j.groupby ? "G|" + build_est_key(j.lookup ? j.lprobe : j.probe, j.probe_key, j.pp)
                         : build_est_key(j.build, j.build_key, j.bp);
+  if (j.lookup) j.est_key += "|J" + build_est_key(j.lbuild, j.lbuild_key, j.lbp) + "/" + std::to_string(j.ljkey);
   for (int i = 0; j.npack > 1 && i < j.npack; i++) j.est_key += "#k" + std::to_string(j.pack_cols[i]);
   j.grp_key = j.groupby ? j.est_key : j.est_key + "#" + build_est_key(j.probe, j.probe_key, j.pp);
   j.est = std::max<int64_t>(j.groupby ? j.probe->nrows : j.build->nrows, 1);
@@ -1748,7 +1788,9 @@ int join_plan_local(JoinJob& j) {
   plan.kgrid = (int32_t)std::max<int64_t>(1, std::min<int64_t>(j.ps.ntiles, 2 * (int64_t)j.ll.grid));
   size_t bytes = 16 * 4 + (size_t)plan.gcap * 8 + (size_t)plan.limit * plan.passes * 8 +
                  (size_t)plan.lgrid * plan.limit * (j.nw + 2) * 8 + (size_t)j.rec_cap * j.RW * 8;
-  bytes += 8 * 256;  // alignment of the carved pieces
+  if (j.lookup)  // the build side's lookup table: keys, heads, counts, a next row per build row
+    bytes += (size_t)j.lcap * 16 + (size_t)std::max<int64_t>(j.lbuild->nrows, 1) * 4 + 16;
+  bytes += 12 * 256;  // alignment of the carved pieces
   j.bytes = (bytes + 255) & ~(size_t)255;
   j.host_bytes = 64 + (size_t)16 * j.RW * 8;
   return VQ_OK;
@@ -1763,6 +1805,8 @@ int join_plan(JoinJob& j) {
     if (rc != kRetry) return rc;
     memset(&plan, 0, sizeof plan);
   }
+  if (j.lookup)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: join GROUP BY with more group keys than the per-workgroup partials hold");
   plan.bits = j.bits;
   plan.P = 1 << j.bits;
   const double per = (double)j.est / plan.P;
@@ -1887,6 +1931,14 @@ void join_bind(JoinJob& j, char* dev, char* host) {
     B.l.gkey = cv.take<int64_t>(dev, (size_t)j.plan.limit * j.plan.passes);
     B.l.part = cv.take<uint64_t>(dev, (size_t)j.plan.lgrid * j.plan.limit * (j.nw + 2));
     B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
+    if (j.lookup) {
+      B.lk.keys = cv.take<int64_t>(dev, j.lcap);
+      B.lk.head = cv.take<uint32_t>(dev, j.lcap);
+      B.lk.next = cv.take<uint32_t>(dev, (size_t)std::max<int64_t>(j.lbuild->nrows, 1));
+      B.lk.cnt = cv.take<uint32_t>(dev, j.lcap);
+      B.lk.min_head = cv.take<uint32_t>(dev, 4);
+      B.lk.mask = j.lcap - 1;
+    }
     return;
   }
   B.b.perr = B.p.perr = B.flags + 5;
@@ -1937,10 +1989,17 @@ void join_bind(JoinJob& j, char* dev, char* host) {
 // Every launch and copy of one attempt, in stream order (capturable: no
 // memset nodes, no host synchronisation).
 hipError_t join_enqueue(JoinJob& j, hipStream_t s, hipEvent_t* ev) {
-  hipError_t e = j.plan.local ? launch_local(j.ll, j.ps, j.jd, j.plan, j.B, s, ev)
+  if (j.lookup) {  // the build side's table first, inside the timed span
+    if (ev) hipEventRecord(ev[0], s);
+    const hipError_t e = launch_lookup_build(j.lq, j.lkey_slot, j.B.lk, s);
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e = j.plan.local ? launch_local(j.ll, j.ps, j.jd, j.plan, j.B, s, ev, j.lookup)
                               : launch_join(j.bl, j.pl, j.jd, j.plan, j.B, s, ev);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(j.hflags, j.B.flags, 32, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && j.lookup)  // the lookup table's words after the flags
+    e = hipMemcpyAsync(j.hflags + 8, j.B.lk.min_head, 16, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess && j.dev_topk)
     e = hipMemcpyAsync(j.hout, j.B.out, (size_t)16 * j.RW * 8, hipMemcpyDeviceToHost, s);
   return e;
@@ -1960,6 +2019,17 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     return kRetry;
   }
   if (j.plan.local && hflags[0]) return fail(VQ_E_DEVICE, "exec: gpu: GROUP BY flags %#x", hflags[0]);
+  if (j.lookup) {
+    if (hflags[9]) return fail(VQ_E_DEVICE, "exec: gpu: join lookup table filled");
+    // a key with more build rows than planned: the exact sums' limbs need
+    // headroom for that many additions per probe row; plan again
+    const int64_t mult = std::max<int64_t>(hflags[10], hflags[11]);
+    if (mult > j.lmult) {
+      j.lmult = mult;
+      if (int rc = join_scales(j)) return rc;
+      return kRetry;
+    }
+  }
   if (hflags[5]) return fail(VQ_E_DEVICE, "exec: gpu: partition scatter wrote past a partition's end");
   if (hflags[0]) {
     // a partition's accumulator pool filled: a larger pool if the LDS has
@@ -2026,6 +2096,18 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     stats->rg_scanned = j.bkept + j.pkept;
     stats->bytes_alg = (j.groupby ? 0 : j.brows * slot_bytes_per_row(j.build, j.bsm)) +
                        j.prows * slot_bytes_per_row(j.probe, j.psm);
+    if (j.lookup) {
+      // probe rows read their own columns; each build row its predicate,
+      // key and the build-side columns the aggregates read
+      int64_t bw = 0;
+      for (int i = 0; i < j.psm.n; i++)
+        if (j.psm.col_of[i] >= j.lsplit) bw += width_of(j.probe->cols[j.psm.col_of[i]].type);
+      stats->bytes_alg = j.prows * (slot_bytes_per_row(j.probe, j.psm) - bw) + j.lrows * (j.lrow_bytes + bw);
+      stats->rows_scanned += j.lrows;
+      stats->rg_total += j.lbuild->nrg;
+      stats->rg_scanned += j.lkept;
+      stats->build_rows = j.lrows;
+    }
     stats->build_rows = j.groupby ? 0 : hflags[6];
     stats->build_keys = hflags[2];
     stats->partitions = j.plan.local ? 0 : j.plan.P;
@@ -2246,6 +2328,77 @@ int agg_packed(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
   return join_run(*j, out, stats);
 }
 
+// A whole column's zone over all row groups (has_minmax: any group has
+// rows), compared as col_interval reads the bits.
+vq_zone merged_zone(const vq_table* t, int32_t col) {
+  vq_zone m;
+  memset(&m, 0, sizeof m);
+  const int type = t->cols[col].type;
+  auto val = [&](uint64_t b) {
+    return type == VQ_F64 ? bits_to_f64((int64_t)b)
+                          : type == VQ_DATE ? (double)(int32_t)(uint32_t)b : (double)(int64_t)b;
+  };
+  for (int32_t g = 0; g < t->nrg; g++) {
+    const vq_zone& z = t->zones[(size_t)g * t->ncols + col];
+    m.null_count += z.null_count;
+    if (!z.has_minmax) continue;
+    if (!m.has_minmax || val(z.min_bits) < val(m.min_bits)) m.min_bits = z.min_bits;
+    if (!m.has_minmax || val(z.max_bits) > val(m.max_bits)) m.max_bits = z.max_bits;
+    m.has_minmax = 1;
+  }
+  return m;
+}
+
+// The column space of a GROUP BY over joined rows: the probe's columns, then
+// the build's (build column c is column probe->ncols + c), over the probe's
+// row groups.  A build column's zone map in every row group is its whole
+// table's (the bound of its values, for exact sums); no predicate on this
+// view touches a build column.
+void joined_view(const vq_table* probe, const vq_table* build, vq_table& v) {
+  v.ctx = probe->ctx;
+  v.nrows = probe->nrows;
+  v.ncols = probe->ncols + build->ncols;
+  v.cols = probe->cols;
+  v.cols.insert(v.cols.end(), build->cols.begin(), build->cols.end());
+  v.nrg = probe->nrg;
+  v.rg_rows = probe->rg_rows;
+  v.rg_start = probe->rg_start;
+  std::vector<vq_zone> bz(build->ncols);
+  for (int32_t c = 0; c < build->ncols; c++) bz[c] = merged_zone(build, c);
+  v.zones.resize((size_t)v.nrg * v.ncols);
+  for (int32_t g = 0; g < v.nrg; g++) {
+    for (int32_t c = 0; c < probe->ncols; c++)
+      v.zones[(size_t)g * v.ncols + c] = probe->zones[(size_t)g * probe->ncols + c];
+    for (int32_t c = 0; c < build->ncols; c++) v.zones[(size_t)g * v.ncols + probe->ncols + c] = bz[c];
+  }
+  v.bytes = 0;
+}
+
+// The build side of a join + GROUP BY: its predicate's and key's slots over
+// the row groups its predicate keeps, and the lookup table's size (a power
+// of two at least twice its rows).
+int lookup_setup(JoinJob& j) {
+  const vq_table* b = j.lbuild;
+  QDesc& d = j.lq;
+  memset(&d, 0, sizeof d);
+  SlotMap sm;
+  if (int rc = add_pred(b, j.lbp, sm, d.slot)) return rc;
+  j.lkey_slot = sm.get(j.lbuild_key);
+  if (j.lkey_slot < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: build side touches more than %d columns", kMaxSlots);
+  fill_slot_cols(b, sm, d.slot);
+  d.nslots = sm.n;
+  d.ngroups = 1;
+  std::vector<uint8_t> keep;
+  prune(b, j.lbp, keep);
+  d.tile_rows = scan_tile_rows(kept_rows(b, keep));
+  if (int rc = make_spans(b, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &j.lrows, &j.lkept)) return rc;
+  if (b->nrows >= (int64_t)INT32_MAX) return fail(VQ_E_UNSUPPORTED, "exec: gpu: build side above 2^31 rows");
+  j.lrow_bytes = slot_bytes_per_row(b, sm);
+  j.lcap = 16;
+  while ((int64_t)j.lcap < 2 * j.lrows) j.lcap *= 2;
+  return VQ_OK;
+}
+
 }  // namespace
 
 extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
@@ -2254,6 +2407,97 @@ extern "C" int vq_join_agg(vq_ctx* c, const vq_table* build, int32_t build_key,
                            int32_t radix_bits, int32_t topk, vq_agg_result* out, vq_stats* stats) {
   return join_impl(c, build, build_key, build_pred, probe, probe_key, probe_pred, naggs, aggs,
                    radix_bits, topk, out, stats, false);
+}
+
+extern "C" int vq_join_group(vq_ctx* c, const vq_table* build, int32_t build_key,
+                             const vq_pred* build_pred, const vq_table* probe, int32_t probe_key,
+                             const vq_pred* probe_pred, int32_t nkeys, const int32_t* key_cols,
+                             int32_t naggs, const vq_agg* aggs, vq_agg_result* out, vq_stats* stats) {
+  if (!c || !build || !probe || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  out->ngroups = 0;  // no result unless this call succeeds
+  if (build->ctx != c || probe->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  if (nkeys < 1 || nkeys > 8 || !key_cols) return fail(VQ_E_ARG, "exec: gpu: %d group-by columns", nkeys);
+  if (naggs < 0 || naggs > VQ_MAX_AGGS || (naggs > 0 && !aggs))
+    return fail(VQ_E_ARG, "exec: gpu: %d aggregates", naggs);
+  if (int rc = check_col(build, build_key)) return rc;
+  if (int rc = check_col(probe, probe_key)) return rc;
+  if (build->cols[build_key].type == VQ_F64 || probe->cols[probe_key].type == VQ_F64)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: float join key");
+  // group keys: probe columns (codes packed as the int-key path packs them)
+  const int w = nkeys <= 2 ? 32 : nkeys <= 4 ? 16 : 8;
+  for (int i = 0; i < nkeys; i++) {
+    if (int rc = check_col(probe, key_cols[i])) return rc;
+    const ColDev& cd = probe->cols[key_cols[i]];
+    if (cd.type == VQ_F64) return fail(VQ_E_UNSUPPORTED, "exec: gpu: float group-by column");
+    if (nkeys > 1 && ((cd.type != VQ_U8 && cd.type != VQ_U16) || cd.dict_size <= 0))
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: group-by column %d is not a dictionary code", key_cols[i]);
+    if (nkeys > 1 && w < 32 && cd.dict_size > (1 << w))
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d codes of column %d exceed the %d-bit packed key", cd.dict_size,
+                  key_cols[i], w);
+  }
+  // the probe predicate is on probe columns only
+  if (probe_pred)
+    for (int k = 0; k < probe_pred->nterms && k < VQ_MAX_TERMS; k++) {
+      const vq_term& tm = probe_pred->terms[k];
+      if (tm.col < 0 || tm.col >= probe->ncols ||
+          ((tm.op & VQ_COLCMP) && (tm.ival < 0 || tm.ival >= probe->ncols)))
+        return fail(VQ_E_ARG, "exec: gpu: probe predicate column out of range");
+    }
+  std::unique_ptr<JoinJob> j(new (std::nothrow) JoinJob());
+  if (!j) return fail(VQ_E_NOMEM, "exec: gpu: join job allocation");
+  joined_view(probe, build, j->view);
+  // aggregates' column references in the view's column space
+  vq_agg va[VQ_MAX_AGGS];
+  auto map_col = [&](int64_t col, int64_t* outc) -> int {
+    if (col & VQ_BUILD_COL) {
+      const int64_t b = col & ~(int64_t)VQ_BUILD_COL;
+      if (b < 0 || b >= build->ncols) return fail(VQ_E_ARG, "exec: gpu: build column %lld out of range", (long long)b);
+      *outc = probe->ncols + b;
+    } else {
+      if (col < 0 || col >= probe->ncols) return fail(VQ_E_ARG, "exec: gpu: column %lld out of range", (long long)col);
+      *outc = col;
+    }
+    return VQ_OK;
+  };
+  for (int k = 0; k < naggs; k++) {
+    va[k] = aggs[k];
+    vq_expr& e = va[k].arg;
+    for (int f = 0; f < e.nfactors && f < VQ_MAX_FACTORS; f++) {
+      if (e.f[f].kind == VQ_F_LIT) continue;
+      int64_t m;
+      if (int rc = map_col(e.f[f].col, &m)) return rc;
+      e.f[f].col = (int32_t)m;
+    }
+    for (int t = 0; t < e.nwhen && t < VQ_MAX_WHEN; t++) {
+      int64_t m;
+      if (int rc = map_col(e.when[t].col, &m)) return rc;
+      e.when[t].col = (int32_t)m;
+      if (e.when[t].op & VQ_COLCMP) {
+        if (int rc = map_col(e.when[t].ival, &m)) return rc;
+        e.when[t].ival = m;
+      }
+    }
+  }
+  join_job_init(*j, c, nullptr, 0, nullptr, &j->view, key_cols[0], probe_pred, naggs, va, -1, 0, true);
+  j->lookup = true;
+  j->lprobe = probe;
+  j->lbuild = build;
+  j->lbuild_key = build_key;
+  j->ljkey = probe_key;
+  j->lsplit = probe->ncols;
+  if (build_pred) {
+    j->lbpred = *build_pred;
+    j->lbp = &j->lbpred;
+  }
+  if (nkeys > 1) {
+    j->npack = nkeys;
+    for (int i = 0; i < nkeys; i++) {
+      j->pack_cols[i] = key_cols[i];
+      j->pack_shift[i] = w * i;
+    }
+  }
+  if (int rc = lookup_setup(*j)) return rc;
+  return join_run(*j, out, stats);
 }
 
 extern "C" int vq_agg_hash(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t key_col,

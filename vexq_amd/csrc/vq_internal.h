@@ -141,6 +141,13 @@ struct JSide {
   int32_t nkeys;
   int32_t key_slots[8];
   int32_t key_shift[8];
+  // a join + GROUP BY of the joined rows (vq_join_group): the probe side's
+  // join key slot, and the slots (bit mask) that hold build-side columns,
+  // loaded for each matching build row instead of the probe row
+  int32_t lookup;
+  int32_t jkey_slot;
+  int32_t build_slots;
+  int32_t pad_;
 };
 
 struct JDesc {
@@ -278,9 +285,25 @@ struct LocalBufs {
   uint64_t* part;       // [grid][limit][nwords + 2] one pass's partials
 };
 
+// The build side's lookup table (vq_join_group): open addressing on
+// radixHash(key) >> 32, linear probing; each slot's key (INT64_MIN: empty)
+// and the head of its chain of build rows (next[row], 0xFFFFFFFF ends it);
+// rows whose key is INT64_MIN are chained from *min_head.
+struct LookupBufs {
+  int64_t* keys;
+  uint32_t* head;
+  uint32_t* next;
+  uint32_t* cnt;       // rows per slot
+  uint32_t* min_head;  // [0] the INT64_MIN chain, [1] set when the table filled,
+                       // [2] the most rows one key has, [3] rows of key INT64_MIN
+  uint32_t mask;       // slots - 1
+  uint32_t pad_;
+};
+
 struct JoinBufs {
   PartBufs b, p;
   LocalBufs l;
+  LookupBufs lk;
   uint64_t* recs;       // one record per matched group (full-output mode)
   uint64_t* cand;       // [P][topk] per-partition top-k candidates (top-k mode)
   struct KItem* citems; // [P][topk] their sort keys (value, first row, index)
@@ -378,6 +401,13 @@ struct JitArgs {
   uint64_t* livebits;  // count pass -> scatter: one bit per live row
   int32_t lk0;         // vq_hash_local: the first key (rank) of this pass
   int32_t lpad_;
+  // vq_join_group: the build side's lookup table (LookupBufs)
+  const int64_t* lk_keys;
+  const uint32_t* lk_head;
+  const uint32_t* lk_next;
+  const uint32_t* lk_min;
+  uint32_t lk_mask;
+  uint32_t lk_pad;
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -430,7 +460,10 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
 hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s, int32_t k0);
 // LDS the per-workgroup path needs for `limit` keys of `nwords` words.
 size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords);
+// vq_join_group's build side: the lookup table of the rows of d's spans that
+// pass its predicate, keyed by slot key_slot (k_scan.hip).
+hipError_t launch_lookup_build(const QDesc& d, int32_t key_slot, const LookupBufs& lk, hipStream_t s);
 hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
                         const JoinPlan& plan, const JoinBufs& bufs, hipStream_t s,
-                        hipEvent_t* ev);
+                        hipEvent_t* ev, bool ev0_done = false);
 }  // namespace vq

@@ -144,6 +144,14 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
     }
   }
   s += "#define VQ_NKEY " + std::to_string(nkey) + "\n";
+  // a join + GROUP BY of the joined rows: the probe's join key slot and the
+  // build-side slots
+  int sb[8] = {0};
+  const bool lookup = keys && keys->lookup;
+  for (int c = 0; lookup && c < 8; c++) sb[c] = (keys->build_slots >> c) & 1;
+  s += "#define VQ_LOOKUP " + std::to_string(lookup ? 1 : 0) + "\n";
+  s += "#define VQ_JKEY " + std::to_string(lookup ? keys->jkey_slot : 0) + "\n";
+  arr(s, "kSB", sb, 8);
   arr(s, "kKSL", ksl, 8);
   arr(s, "kKSH", ksh, 8);
   arr(s, "kST", st, 8);
@@ -500,6 +508,11 @@ hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_
   a.lflags = B.flags;
   a.lgkey = B.l.gkey;
   a.part = B.l.part;
+  a.lk_keys = B.lk.keys;
+  a.lk_head = B.lk.head;
+  a.lk_next = B.lk.next;
+  a.lk_min = B.lk.min_head;
+  a.lk_mask = B.lk.mask;
   a.lds_words = ll.lds / 4;
   return launch(ll.fn, (unsigned)ll.grid, ll.lds, s, a, 512);
 }

@@ -72,6 +72,9 @@ def lib():
                               C.c_int32, P(abi.AggResult), P(abi.Stats)]
     L.vq_agg_hash.argtypes = [C.c_void_p, C.c_void_p, P(abi.Pred), C.c_int32, C.c_int32,
                               P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
+    L.vq_join_group.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_void_p,
+                                C.c_int32, P(abi.Pred), C.c_int32, P(C.c_int32), C.c_int32,
+                                P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
     L.vq_radix_bits_for.argtypes = [C.c_int64]
     L.vq_radix_bits_for.restype = C.c_int32
     L.vq_radix_hash.argtypes = [C.c_int64]
@@ -331,6 +334,24 @@ class Context:
         _check(lib().vq_join_agg(self.h, build.h, build_key, _pp(build_pred), probe.h, probe_key,
                                  _pp(probe_pred), len(aggs), arr, radix_bits, topk,
                                  C.byref(out.c), C.byref(st)), out)
+        out.stats = st
+        return out
+
+
+    def join_group(self, build: "DeviceTable", build_key: int, build_pred, probe: "DeviceTable",
+                   probe_key: int, probe_pred, key_cols, aggs, capacity: int | None = None,
+                   out: abi.AggOut | None = None) -> abi.AggOut:
+        """vq_join_group: GROUP BY probe columns over the joined rows;
+        aggregates over either side (build columns flagged VQ_BUILD_COL,
+        abi.JoinView resolves names)."""
+        arr = aggs if isinstance(aggs, C.Array) else abi.agg_array(aggs)
+        karr = (C.c_int32 * len(key_cols))(*key_cols)
+        if out is None:
+            out = abi.AggOut(capacity if capacity is not None else max(1, probe.nrows), len(key_cols), len(aggs))
+        st = abi.Stats()
+        _check(lib().vq_join_group(self.h, build.h, build_key, _pp(build_pred), probe.h, probe_key,
+                                   _pp(probe_pred), len(key_cols), karr, len(aggs), arr,
+                                   C.byref(out.c), C.byref(st)), out)
         out.stats = st
         return out
 
