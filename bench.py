@@ -9,7 +9,8 @@ the cross-tile reduction and the copy of the result to the host.  The same
 line carries every operator of the path under "operators": the 1M-row GROUP
 BY, the device-side ingestion of the 6M-row table from a .vxq image, the
 zone-pruned range scan + SUM(expr) over 6M sorted rows, the filter ->
-selection vector, the 1.5M x 6M radix join + aggregate + top-10, the
+selection vector, the 1.5M x 6M radix join + aggregate + top-10, TPC-H
+Q12's join + GROUP BY of the joined rows over 1.5M orders x 6M lineitem, the
 integer-key GROUP BY at 115K and 6 groups, 256 batched instances, and the
 served stream over a 60M-row fact table (whose 60M-row GROUP BY is also the
 cache-cold kernel figure, "roofline_cache_cold").
@@ -635,6 +636,32 @@ def main():
         db.close()
         dp.close()
         log(f"[bench] join: {rec['value']} GB/s")
+
+    # ---- TPC-H Q12's shape: join + GROUP BY of the joined rows ---------------
+    if want("join_group_q12"):
+        log("[bench] join_group_q12: generating 1.5M orders x 6M lineitem")
+        b, p = datagen.q12_tables(1_500_000, 6_000_000, seed=42)
+        pp, keys, aggs, _ = datagen.q_q12(b, p)
+        db, dp = ctx.upload(b), ctx.upload(p)
+        agg_arr = abi.agg_array(aggs)
+        ro = abi.AggOut(64, len(keys), len(aggs))
+        r, total, walls, mains, devs = time_gpu(
+            lambda: ctx.join_group(db, 0, None, dp, 0, pp, keys, agg_arr, out=ro), K, W)
+        cpu = None
+        if cpu_mod is not None:
+            runs = time_cpu(lambda: cpu_mod.join_group(b, 0, None, p, 0, pp, keys, aggs), max_runs=5)
+            cpu = cpu_record(runs, r.stats.bytes_alg, 1, f"the full query, serial restatement, {len(runs)} runs")
+        rec = operator_record("join_group_q12",
+                              "pipeline: k_lookup_init k_lookup_build k_local_init k_local_keys k_local_rank "
+                              "vq_hash_local k_local_merge",
+                              r, total, walls, mains, devs, K, traffic, cpu, b.nrows + p.nrows,
+                              whole_pipeline=True)
+        rec["groups"] = int(r.ngroups)
+        rec["joined_rows"] = int(ro.counts[:r.ngroups].sum())
+        ops["join_group_q12"] = rec
+        db.close()
+        dp.close()
+        log(f"[bench] join_group_q12: {rec['value']} GB/s, {rec['device_ms']} ms on the device")
 
     # ---- GROUP BY one int64 key of any cardinality (hash path) ---------------
     for name, groups in (("hash_groupby_6m_115k", 115_000), ("hash_groupby_6m_6", 6)):

@@ -441,6 +441,60 @@ bool ErrorsAreReported() {
   return true;
 }
 
+// TPC-H Q12's shape through the operator: GROUP BY a probe-side dictionary
+// column over the joined rows, a CASE WHEN indicator sum over a build-side
+// column, a duplicated build key fanning out (expected values by hand).
+bool JoinGroupedByProbeColumn() {
+  static const int64_t okey[] = {10, 20, 30, 40, 20};
+  static const uint8_t oprio[] = {0, 1, 2, 0, 2};
+  static const int64_t oval[] = {100, 200, 300, 400, 250};
+  static const int64_t lkey[] = {10, 20, 20, 30, 50, 40, 10};
+  static const uint8_t lmode[] = {0, 1, 0, 1, 0, 0, 1};
+  static const int64_t lqty[] = {1, 2, 3, 4, 5, 6, 7};
+  HostTable hb, hp;
+  hb.columns = {col("o_orderkey", DataType::Int64, VQ_I64, okey),
+                col("o_orderpriority", DataType::String, VQ_U8, oprio, dict({"1-URGENT", "2-HIGH", "3-MEDIUM"})),
+                col("o_val", DataType::Int64, VQ_I64, oval)};
+  hb.row_groups = {5};
+  hb.nrows = 5;
+  hp.columns = {col("l_orderkey", DataType::Int64, VQ_I64, lkey),
+                col("l_shipmode", DataType::String, VQ_U8, lmode, dict({"MAIL", "SHIP"})),
+                col("l_quantity", DataType::Int64, VQ_I64, lqty)};
+  hp.row_groups = {4, 3};
+  hp.nrows = 7;
+  std::unique_ptr<GpuTable> b, p;
+  CHECK_OK(GpuTable::Upload(g_dev.get(), hb, &b));
+  CHECK_OK(GpuTable::Upload(g_dev.get(), hp, &p));
+  // SUM(CASE WHEN o_orderpriority IN ('1-URGENT', '2-HIGH') THEN 1 ELSE 0 END)
+  AggExpr high = agg(AggKind::Sum, -1, DataType::Int64, "high_line_count");
+  high.expr.nfactors = 1;
+  high.expr.f[0].kind = VQ_F_LIT;
+  high.expr.f[0].ival = 1;
+  high.expr.nwhen = 1;
+  high.expr.when[0].col = 1 | VQ_BUILD_COL;
+  high.expr.when[0].op = VQ_IN;
+  high.expr.when[0].ival = 0x3;
+  AggExpr mx = agg(AggKind::Max, 2, DataType::Int64, "max_val");
+  mx.build_side = true;
+  JoinOptions opts;
+  opts.group_by = {1};
+  auto op = NewGpuParallelHashJoinAggregate(b.get(), 0, nullptr, p.get(), 0, nullptr,
+                                            {high, agg(AggKind::Count, -1, DataType::Int64, "c"),
+                                             agg(AggKind::Sum, 2, DataType::Int64, "qty"), mx},
+                                            opts);
+  CHECK(op->schema().fields.size() == 5 && op->schema().fields[0].name == "l_shipmode");
+  std::vector<Batch> bs;
+  CHECK_OK(Drain(op.get(), &bs));
+  CHECK(bs.size() == 1 && bs[0].length == 2);
+  const Batch& r = bs[0];
+  CHECK(r.vectors[0].Get(0) == "MAIL" && r.vectors[0].Get(1) == "SHIP");
+  CHECK(r.vectors[1].i64[0] == 3 && r.vectors[2].i64[0] == 4 && r.vectors[3].i64[0] == 13 &&
+        r.vectors[4].i64[0] == 400);
+  CHECK(r.vectors[1].i64[1] == 2 && r.vectors[2].i64[1] == 4 && r.vectors[3].i64[1] == 15 &&
+        r.vectors[4].i64[1] == 300);
+  return true;
+}
+
 }  // namespace
 
 int main() {
@@ -458,6 +512,7 @@ int main() {
   run("TestSerialHashJoinSemantics", SerialHashJoinSemantics);
   run("TestSharedProbeAndMultiplicities", SharedProbeAndMultiplicities);
   run("TestJoinTableSentinelAndExtremeKeys", SentinelAndExtremeKeys);
+  run("TestJoinGroupedByProbeColumn", JoinGroupedByProbeColumn);
   run("TestParallelAggregateClosedForm", ClosedFormSum);
   run("TestErrorsAreReported", ErrorsAreReported);
   g_dev.reset();

@@ -65,19 +65,30 @@ def test_packed_group_columns_and_build_payload(gpu_ctx):
         dp.close()
 
 
-def test_duplicate_and_extreme_build_keys(gpu_ctx):
-    """Build keys repeating up to dozens of times, INT64_MIN and 0 keys on
-    both sides, probe keys with no match: every probe row meets every build
-    row with its key."""
-    _case("duplicates")
+@pytest.mark.parametrize("keys", ["signed", "non_negative"])
+def test_duplicate_and_extreme_build_keys(gpu_ctx, keys):
+    """Build keys repeating up to dozens of times, probe keys with no match:
+    every probe row meets every build row with its key.  signed: INT64_MIN and
+    0 keys on both sides (two-array table); non_negative: keys below 2^32 - 1
+    (one-word slots, repeated keys chained by compare-and-swap), probe keys
+    at and past that bound."""
+    _case("duplicates", keys=keys)
     rng = np.random.default_rng(23)
     nb, npr = 60_000, 250_000
-    bkey = rng.integers(-500, 3000, nb).astype(np.int64)
-    bkey[:40] = np.iinfo(np.int64).min
-    bkey[40:90] = 0
-    pkey = rng.integers(-600, 3100, npr).astype(np.int64)
-    pkey[::131] = np.iinfo(np.int64).min
-    pkey[::257] = 0
+    if keys == "signed":
+        bkey = rng.integers(-500, 3000, nb).astype(np.int64)
+        bkey[:40] = np.iinfo(np.int64).min
+        bkey[40:90] = 0
+        pkey = rng.integers(-600, 3100, npr).astype(np.int64)
+        pkey[::131] = np.iinfo(np.int64).min
+        pkey[::257] = 0
+    else:
+        bkey = rng.integers(0, 3000, nb).astype(np.int64)
+        bkey[:30] = 0xFFFFFFFE
+        pkey = rng.integers(-100, 3100, npr).astype(np.int64)
+        pkey[::113] = 0xFFFFFFFE
+        pkey[::229] = 0xFFFFFFFF
+        pkey[::331] = 1 << 40
     b = datagen.Table([datagen.Column("k", abi.VQ_I64, bkey), datagen.Column("v", abi.VQ_I64,
                        rng.integers(-10**6, 10**6, nb).astype(np.int64)),
                        datagen.Column("w", abi.VQ_F64, rng.standard_normal(nb))], datagen.row_groups(nb))

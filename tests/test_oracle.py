@@ -370,41 +370,21 @@ def test_case_when_sum_equals_the_filtered_sum():
 
 
 def q12_join_tables(nb, npr, seed=16):
-    """orders (build: key, orderdate, priority codes of 5 strings) and a
-    lineitem probe side with Q12's columns (ship mode codes, commit and
-    receipt dates)."""
-    b = datagen.orders(nb, seed)
-    p = datagen.lineitem_probe(b, npr, seed)
-    ship = p.col("shipdate").astype(np.int64)
-    n = p.nrows
-    commit = ship + datagen.rand_int(seed, 93, n, -30, 30)
-    receipt = ship + datagen.rand_int(seed, 94, n, 1, 30)
-    mode = datagen.rand_int(seed, 95, n, 0, 6).astype(np.uint8)
-    p = datagen.Table(list(p.columns) + [datagen.Column("commitdate", abi.VQ_DATE, commit.astype(np.int32)),
-                                         datagen.Column("receiptdate", abi.VQ_DATE, receipt.astype(np.int32)),
-                                         datagen.Column("shipmode", abi.VQ_U8, mode, 7)],
-                      datagen.row_groups(n))
-    return b, p
+    return datagen.q12_tables(nb, npr, seed)
 
 
 def q12_query(b, p, lo=9000, hi=9365):
-    """TPC-H Q12's shape (planner/parallel_join_bench_test.go:19-30): GROUP BY
-    l_shipmode, SUM(CASE WHEN o_orderpriority IN ('1-URGENT', '2-HIGH') THEN 1
-    ELSE 0), SUM(CASE WHEN ... <> AND <> ... THEN 1 ELSE 0), plus a
-    build-side MAX and an average under a guard comparing a build column with
-    a probe column: eight columns, a query's limit."""
+    """TPC-H Q12's shape (datagen.q_q12: GROUP BY l_shipmode, the two CASE
+    WHEN sums over o_orderpriority) plus a build-side MAX and an average under
+    a guard comparing a build column with a probe column: eight columns, a
+    query's limit."""
     X = abi.X
     v = abi.JoinView(p, b)
-    urgent = [b.columns[b.index("priority")].dictionary.index(s) for s in ("1-URGENT", "2-HIGH")]
-    pp = abi.make_pred(p, [("shipmode", "in", [1, 5]), ("commitdate", "<", X.col("receiptdate")),
-                           ("shipdate", "<", X.col("commitdate")), ("receiptdate", ">=", lo),
-                           ("receiptdate", "<", hi)])
-    aggs = [abi.count_star(v),
-            abi.agg_sum(X.case([("b.priority", "in", urgent)], 1, 0), v),
-            abi.agg_sum(X.case([("b.priority", "<>", urgent[0]), ("b.priority", "<>", urgent[1])], 1, 0), v),
-            abi.agg_max("b.orderdate", v),
-            abi.agg_avg(X.case([("b.orderdate", "<", X.col("commitdate"))], X.col("quantity"), 0), v)]
-    return pp, [p.index("shipmode")], aggs, urgent
+    pp, keys, aggs, urgent = datagen.q_q12(b, p, lo, hi)
+    aggs = [abi.count_star(v)] + aggs + [
+        abi.agg_max("b.orderdate", v),
+        abi.agg_avg(X.case([("b.orderdate", "<", X.col("commitdate"))], X.col("quantity"), 0), v)]
+    return pp, keys, aggs, urgent
 
 
 def test_join_group_q12_matches_numpy():
@@ -413,7 +393,9 @@ def test_join_group_q12_matches_numpy():
     r = O.join_group(b, 0, None, p, 0, pp, keys, aggs, exact=True)
     # numpy: the unique build key 1 + 4 i is build row i
     d = {c.name: c.data for c in p.columns}
-    m = (np.isin(d["shipmode"], [1, 5]) & (d["commitdate"] < d["receiptdate"]) & (d["shipdate"] < d["commitdate"])
+    modes = p.columns[p.index("shipmode")].dictionary
+    m = (np.isin(d["shipmode"], [modes.index("MAIL"), modes.index("SHIP")]) & (d["commitdate"] < d["receiptdate"])
+         & (d["shipdate"] < d["commitdate"])
          & (d["receiptdate"] >= 9000) & (d["receiptdate"] < 9365))
     rows = np.nonzero(m)[0]
     brow = (d["fk"][rows] - 1) // 4

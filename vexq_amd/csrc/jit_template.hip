@@ -159,7 +159,7 @@ struct Args {
   const u32* lk_next;
   const u32* lk_min;
   u32 lk_mask;
-  u32 lk_pad;
+  u32 lk_packed;      // slots are words key << 32 | head (all ones: empty)
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -1231,6 +1231,18 @@ __device__ __forceinline__ void lload_one(const Args& a, i64 r, i64 (&raw)[VQ_NS
 #if VQ_LOOKUP
 // The first build row with this key (0xFFFFFFFF: none).
 __device__ __forceinline__ u32 lookup_head(const Args& a, i64 key) {
+  if (a.lk_packed) {
+    if (key < 0 || key >= 0xFFFFFFFFll) return 0xFFFFFFFFu;  // (no build key there)
+    const u64 kk = (u64)key << 32;
+    u32 i = (u32)(radix_hash((u64)key) >> 32) & a.lk_mask;
+    for (u32 pr = 0; pr <= a.lk_mask; pr++) {
+      const u64 w = ((const u64*)a.lk_keys)[i];
+      if ((w & 0xFFFFFFFF00000000ull) == kk) return (u32)w;
+      if (w == ~0ull) break;
+      i = (i + 1) & a.lk_mask;
+    }
+    return 0xFFFFFFFFu;
+  }
   if (key == kI64Min) return *a.lk_min;
   u32 i = (u32)(radix_hash((u64)key) >> 32) & a.lk_mask;
   for (u32 pr = 0; pr <= a.lk_mask; pr++) {

@@ -1079,6 +1079,10 @@ struct LocalKeys {
   int64_t sp_tile0[kMaxSpans];
   int64_t* keys;
   uint32_t* flags;
+  // dictionary-code keys: seed the table with every code combination
+  // (card[c] codes of column c) instead of reading the rows
+  int64_t seed;
+  int32_t card[8];
 };
 
 __global__ __launch_bounds__(256) void k_local_init(int64_t* __restrict__ keys, int32_t lcap,
@@ -1190,6 +1194,29 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_keys(const LocalKeys k)
   if (threadIdx.x == 0 && s_min && atomicCAS(&k.flags[10], 0u, 1u) == 0u &&
       atomicAdd(&k.flags[2], 1u) + 1 > (uint32_t)k.limit)
     atomicOr(&k.flags[0], kLocalOverflow);
+}
+
+// Every combination of the key columns' codes into the key table (a group
+// whose rows all fail the predicate, or meet no build row, has a count of
+// zero and leaves no record): for dictionary-code keys whose code space is
+// within the per-workgroup path's limit, no pass over the rows.
+__global__ __launch_bounds__(256) void k_local_seed(const LocalKeys k) {
+  const int cap = k.lcap, lg = __ffs(cap) - 1;
+  for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < k.seed; x += (int64_t)gridDim.x * 256) {
+    int64_t key = x;
+    if (k.nkeys > 1) {
+      uint64_t pk = 0;
+      int64_t rem = x;
+      for (int c = 0; c < k.nkeys; c++) {
+        pk |= (uint64_t)(rem % k.card[c]) << k.kshift[c];
+        rem /= k.card[c];
+      }
+      key = (int64_t)pk;
+    }
+    const int got = table_insert<__HIP_MEMORY_SCOPE_AGENT>((unsigned long long*)k.keys, cap, lg, key);
+    if (got > 0) atomicAdd(&k.flags[2], 1u);
+    else if (got < 0) atomicOr(&k.flags[0], kLocalOverflow);
+  }
 }
 
 // The key list: the table's regular keys in slot order (a key's rank is its
@@ -1346,8 +1373,14 @@ hipError_t launch_local(const LocalLaunch& ll, const JSide& ps, const JDesc& jd,
       allowed.store(bytes);
     }
   }
-  hipLaunchKernelGGL(k_local_keys, dim3((unsigned)plan.kgrid), dim3(kLocalThreads),
-                     (size_t)plan.gcap * 8, s, k);
+  k.seed = plan.seed_groups;
+  for (int c = 0; c < 8; c++) k.card[c] = plan.seed_card[c];
+  if (plan.seed_groups > 0)
+    hipLaunchKernelGGL(k_local_seed, dim3((unsigned)std::min<int64_t>(64, (plan.seed_groups + 255) / 256)),
+                       dim3(256), 0, s, k);
+  else
+    hipLaunchKernelGGL(k_local_keys, dim3((unsigned)plan.kgrid), dim3(kLocalThreads),
+                       (size_t)plan.gcap * 8, s, k);
   hipLaunchKernelGGL(k_local_rank, dim3(1), dim3(kRankThreads), 0, s, (const int64_t*)B.l.keys,
                      plan.gcap, B.flags, B.l.gkey);
   if (ev) hipEventRecord(ev[1], s);

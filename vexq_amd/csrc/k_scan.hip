@@ -758,14 +758,46 @@ hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part
 __global__ __launch_bounds__(256) void k_lookup_init(const LookupBufs lk) {
   const uint32_t n = lk.mask + 1;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    lk.keys[i] = INT64_MIN;
-    lk.head[i] = 0xFFFFFFFFu;
     lk.cnt[i] = 0;
+    if (lk.packed) {
+      lk.keys[i] = -1;  // all ones
+    } else {
+      lk.keys[i] = INT64_MIN;
+      lk.head[i] = 0xFFFFFFFFu;
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x < 4) lk.min_head[threadIdx.x] = threadIdx.x == 0 ? 0xFFFFFFFFu : 0u;
 }
 
 __device__ __forceinline__ void lookup_insert(const LookupBufs& lk, int64_t key, uint32_t r) {
+  if (lk.packed) {
+    if (key < 0 || key >= 0xFFFFFFFFll) {  // outside the bounds the zone maps gave
+      atomicOr(&lk.min_head[1], 2u);
+      return;
+    }
+    unsigned long long* slot = (unsigned long long*)lk.keys;
+    const unsigned long long kk = (unsigned long long)key << 32, empty = ~0ull;
+    uint32_t i = (uint32_t)(radix_hash((uint64_t)key) >> 32) & lk.mask;
+    lk.next[r] = 0xFFFFFFFFu;
+    for (uint32_t pr = 0; pr <= lk.mask; pr++) {
+      unsigned long long w = atomicCAS(&slot[i], empty, kk | r);
+      if (w == empty) return;  // a new key: r its only row
+      if ((w & 0xFFFFFFFF00000000ull) == kk) {
+        // a repeated key: r goes in front of its chain
+        for (;;) {
+          lk.next[r] = (uint32_t)w;
+          const unsigned long long prev = atomicCAS(&slot[i], w, kk | r);
+          if (prev == w) break;
+          w = prev;
+        }
+        atomicMax(&lk.min_head[2], atomicAdd(&lk.cnt[i], 1u) + 2u);
+        return;
+      }
+      i = (i + 1) & lk.mask;
+    }
+    atomicOr(&lk.min_head[1], 1u);  // (the host sizes the table above the rows: not reached)
+    return;
+  }
   if (key == INT64_MIN) {  // (the empty slots' mark: kept apart)
     lk.next[r] = atomicExch(&lk.min_head[0], r);
     atomicAdd(&lk.min_head[3], 1u);
@@ -777,7 +809,9 @@ __device__ __forceinline__ void lookup_insert(const LookupBufs& lk, int64_t key,
                                              (unsigned long long)INT64_MIN, (unsigned long long)key);
     if (old == (unsigned long long)INT64_MIN || old == (unsigned long long)key) {
       lk.next[r] = atomicExch(&lk.head[i], r);
-      atomicMax(&lk.min_head[2], atomicAdd(&lk.cnt[i], 1u) + 1u);  // the key's rows so far
+      // a repeated key: cnt counts its rows after the first, so this row makes
+      // cnt + 2 of them (unique keys take no atomics beyond these two)
+      if (old == (unsigned long long)key) atomicMax(&lk.min_head[2], atomicAdd(&lk.cnt[i], 1u) + 2u);
       return;
     }
     i = (i + 1) & lk.mask;

@@ -1576,6 +1576,7 @@ struct JoinJob {
   uint32_t lcap = 0;
   int64_t lrows = 0, lkept = 0, lrow_bytes = 0;
   int64_t lmult = 1;  // the most build rows one key has, as planned
+  bool lpacked = false;  // every build key in [0, 2^32 - 1): one-word slots
 };
 
 // Table fill: a GROUP BY's rows all find their keys, but fill the table by
@@ -1762,6 +1763,24 @@ int join_plan_local(JoinJob& j) {
   if (plan.limit > 0 && j.groups_seen >= 0)
     plan.passes = (int32_t)std::max<int64_t>(
         1, std::min<int64_t>(plan.passes, (j.est + plan.limit - 1) / plan.limit));
+  // dictionary-code keys whose every combination fits the passes: the key
+  // table is seeded with all of them (no pass over the rows to find them)
+  plan.seed_groups = 0;
+  if (plan.limit > 0) {
+    const int nk = j.npack > 1 ? j.npack : 1;
+    int64_t G = 1;
+    bool codes = true;
+    for (int i = 0; i < nk && codes; i++) {
+      const ColDev& cd = j.probe->cols[j.npack > 1 ? j.pack_cols[i] : j.probe_key];
+      codes = (cd.type == VQ_U8 || cd.type == VQ_U16) && cd.dict_size > 0;
+      plan.seed_card[i] = codes ? cd.dict_size : 0;
+      G = std::min<int64_t>(G * std::max(cd.dict_size, 1), (int64_t)1 << 40);
+    }
+    if (codes && G <= (int64_t)plan.limit * local_passes_max(plan.limit)) {
+      plan.seed_groups = G;
+      plan.passes = (int32_t)((G + plan.limit - 1) / plan.limit);
+    }
+  }
   plan.gcap = plan.lcap;
   while (plan.gcap < 2 * plan.limit * plan.passes) plan.gcap *= 2;
   plan.lds_max = j.lds_max;
@@ -1938,6 +1957,7 @@ void join_bind(JoinJob& j, char* dev, char* host) {
       B.lk.cnt = cv.take<uint32_t>(dev, j.lcap);
       B.lk.min_head = cv.take<uint32_t>(dev, 4);
       B.lk.mask = j.lcap - 1;
+      B.lk.packed = j.lpacked ? 1u : 0u;
     }
     return;
   }
@@ -2020,10 +2040,11 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
   }
   if (j.plan.local && hflags[0]) return fail(VQ_E_DEVICE, "exec: gpu: GROUP BY flags %#x", hflags[0]);
   if (j.lookup) {
+    if (hflags[9] & 2) return fail(VQ_E_CORRUPT, "exec: gpu: a build key outside its column's zone maps");
     if (hflags[9]) return fail(VQ_E_DEVICE, "exec: gpu: join lookup table filled");
     // a key with more build rows than planned: the exact sums' limbs need
     // headroom for that many additions per probe row; plan again
-    const int64_t mult = std::max<int64_t>(hflags[10], hflags[11]);
+    const int64_t mult = std::max<int64_t>(1, std::max<int64_t>(hflags[10], hflags[11]));
     if (mult > j.lmult) {
       j.lmult = mult;
       if (int rc = join_scales(j)) return rc;
@@ -2394,6 +2415,15 @@ int lookup_setup(JoinJob& j) {
   if (int rc = make_spans(b, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &j.lrows, &j.lkept)) return rc;
   if (b->nrows >= (int64_t)INT32_MAX) return fail(VQ_E_UNSUPPORTED, "exec: gpu: build side above 2^31 rows");
   j.lrow_bytes = slot_bytes_per_row(b, sm);
+  {
+    // every build key non-negative and below 2^32 - 1 (the zone maps'
+    // bounds): a slot is one word, key << 32 | head
+    const vq_zone z = merged_zone(b, j.lbuild_key);
+    const int type = b->cols[j.lbuild_key].type;
+    const int64_t lo = type == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.min_bits : (int64_t)z.min_bits;
+    const int64_t hi = type == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.max_bits : (int64_t)z.max_bits;
+    j.lpacked = !z.has_minmax || (lo >= 0 && hi < (int64_t)0xFFFFFFFFll);
+  }
   j.lcap = 16;
   while ((int64_t)j.lcap < 2 * j.lrows) j.lcap *= 2;
   return VQ_OK;

@@ -288,16 +288,18 @@ struct LocalBufs {
 // The build side's lookup table (vq_join_group): open addressing on
 // radixHash(key) >> 32, linear probing; each slot's key (INT64_MIN: empty)
 // and the head of its chain of build rows (next[row], 0xFFFFFFFF ends it);
-// rows whose key is INT64_MIN are chained from *min_head.
+// rows whose key is INT64_MIN are chained from *min_head.  packed: every
+// build key lies in [0, 2^32 - 1), and a slot is one word, key << 32 | head
+// (all ones: empty; head unused), so a new key takes one compare-and-swap.
 struct LookupBufs {
   int64_t* keys;
   uint32_t* head;
   uint32_t* next;
-  uint32_t* cnt;       // rows per slot
+  uint32_t* cnt;       // rows per slot after the first
   uint32_t* min_head;  // [0] the INT64_MIN chain, [1] set when the table filled,
                        // [2] the most rows one key has, [3] rows of key INT64_MIN
   uint32_t mask;       // slots - 1
-  uint32_t pad_;
+  uint32_t packed;
 };
 
 struct JoinBufs {
@@ -340,6 +342,8 @@ struct JoinPlan {
   int32_t lgrid;        // workgroups that aggregate (partials)
   int32_t kgrid;        // workgroups that collect the keys
   int32_t tnodes;       // top-k merge counters (1 with more than one partition)
+  int64_t seed_groups;  // > 0: the key table seeded with every code combination
+  int32_t seed_card[8]; // (codes of each key column)
   int32_t bloom_lpp;    // semi-join filter: log2 of its 64-byte lines per
                         // partition (-1: no filter)
   int32_t P1;           // coarse partitions of the partition kernels (P: one pass)
@@ -407,7 +411,7 @@ struct JitArgs {
   const uint32_t* lk_next;
   const uint32_t* lk_min;
   uint32_t lk_mask;
-  uint32_t lk_pad;
+  uint32_t lk_packed;
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):

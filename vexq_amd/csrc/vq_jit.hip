@@ -513,6 +513,7 @@ hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_
   a.lk_next = B.lk.next;
   a.lk_min = B.lk.min_head;
   a.lk_mask = B.lk.mask;
+  a.lk_packed = B.lk.packed;
   a.lds_words = ll.lds / 4;
   return launch(ll.fn, (unsigned)ll.grid, ll.lds, s, a, 512);
 }

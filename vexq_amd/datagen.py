@@ -224,6 +224,49 @@ def lineitem_probe(build: Table, nrows: int, seed: int = 42) -> Table:
     return Table(cols, row_groups(nrows))
 
 
+def q12_tables(nb: int, npr: int, seed: int = 16):
+    """orders (the build side: key, orderdate, priority codes of the five
+    TPC-H priority strings) and a lineitem probe side with Q12's columns:
+    ship mode codes (7 modes), commit and receipt dates around the ship
+    date."""
+    b = orders(nb, seed)
+    p = lineitem_probe(b, npr, seed)
+    ship = p.col("shipdate").astype(np.int64)
+    n = p.nrows
+    commit = ship + rand_int(seed, 93, n, -30, 30)
+    receipt = ship + rand_int(seed, 94, n, 1, 30)
+    mode = rand_int(seed, 95, n, 0, 6).astype(np.uint8)
+    p = Table(list(p.columns) + [_col("commitdate", abi.VQ_DATE, commit.astype(np.int32)),
+                                 _col("receiptdate", abi.VQ_DATE, receipt.astype(np.int32)),
+                                 _col("shipmode", abi.VQ_U8, mode,
+                                      ["REG AIR", "AIR", "RAIL", "SHIP", "TRUCK", "MAIL", "FOB"])],
+              row_groups(n))
+    return b, p
+
+
+def q_q12(b: Table, p: Table, lo: int = DATE_MIN + 731, hi: int = DATE_MIN + 1096):
+    """TPC-H Q12 (planner/parallel_join_bench_test.go:19-30) over q12_tables:
+    lineitem rows with shipmode IN ('MAIL', 'SHIP'), commitdate < receiptdate,
+    shipdate < commitdate, receiptdate in [lo, hi) (1994-01-01 to
+    1995-01-01 by default), joined to orders on the
+    order key, GROUP BY shipmode, SUM(CASE WHEN o_orderpriority IN
+    ('1-URGENT', '2-HIGH') THEN 1 ELSE 0 END), SUM(CASE WHEN o_orderpriority
+    <> '1-URGENT' AND o_orderpriority <> '2-HIGH' THEN 1 ELSE 0 END).
+    Returns (probe predicate, group columns, aggregates, the two codes)."""
+    X = abi.X
+    v = abi.JoinView(p, b)
+    pdict = b.columns[b.index("priority")].dictionary
+    urgent = [pdict.index(x) for x in ("1-URGENT", "2-HIGH")]
+    modes = p.columns[p.index("shipmode")].dictionary
+    pp = abi.make_pred(p, [("shipmode", "in", [modes.index("MAIL"), modes.index("SHIP")]),
+                           ("commitdate", "<", X.col("receiptdate")),
+                           ("shipdate", "<", X.col("commitdate")),
+                           ("receiptdate", ">=", lo), ("receiptdate", "<", hi)])
+    aggs = [abi.agg_sum(X.case([("b.priority", "in", urgent)], 1, 0), v),
+            abi.agg_sum(X.case([("b.priority", "<>", urgent[0]), ("b.priority", "<>", urgent[1])], 1, 0), v)]
+    return pp, [p.index("shipmode")], aggs, urgent
+
+
 def fact_serving(dim: Table, nrows: int, seed: int = 42) -> Table:
     """The served stream's fact table: lineitem's dictionary and measure
     columns plus the join probe side's foreign key (each dimension key 1..7

@@ -13,18 +13,19 @@ constexpr int kBlockRows = VQ_BLOCK_ROWS;  // storage.BlockRows (storage/format.
 
 int vq_type_of(const HostColumn& c) { return c.vq_type; }
 
-vq_expr expr_of(const HostTable& t, const AggExpr& a) {
+vq_expr expr_of(const HostTable& t, const AggExpr& a, const HostTable* build = nullptr) {
   if (a.expr.nfactors > 0 || a.col_idx < 0) return a.expr;
   vq_expr e;
   memset(&e, 0, sizeof e);
+  const HostTable& src = a.build_side && build ? *build : t;
   e.nfactors = 1;
-  e.is_float = t.columns[a.col_idx].vq_type == VQ_F64;
+  e.is_float = src.columns[a.col_idx].vq_type == VQ_F64;
   e.f[0].kind = VQ_F_COL;
-  e.f[0].col = a.col_idx;
+  e.f[0].col = a.build_side ? (a.col_idx | VQ_BUILD_COL) : a.col_idx;
   return e;
 }
 
-vq_agg to_vq(const HostTable& t, const AggExpr& a) {
+vq_agg to_vq(const HostTable& t, const AggExpr& a, const HostTable* build = nullptr) {
   vq_agg g;
   memset(&g, 0, sizeof g);
   switch (a.kind) {
@@ -35,19 +36,20 @@ vq_agg to_vq(const HostTable& t, const AggExpr& a) {
     case AggKind::Max: g.kind = VQ_AGG_MAX; break;
   }
   g.accum = a.accum_type == DataType::Float64 ? VQ_F64 : VQ_I64;
-  if (a.kind != AggKind::Count) g.arg = expr_of(t, a);
+  if (a.kind != AggKind::Count) g.arg = expr_of(t, a, build);
   return g;
 }
 
 // Output column type of an aggregate (buildOutputBatch, exec/aggregate.go:780-876):
 // MIN/MAX keep the source type, AVG is float, COUNT int, SUM its accumulator.
-DataType out_type(const HostTable& t, const AggExpr& a) {
+DataType out_type(const HostTable& t, const AggExpr& a, const HostTable* build = nullptr) {
   switch (a.kind) {
     case AggKind::Count: return DataType::Int64;
     case AggKind::Avg: return DataType::Float64;
     case AggKind::Sum: return a.accum_type;
     default:
-      if (a.expr.nfactors == 0 && a.col_idx >= 0) return t.columns[a.col_idx].type;
+      if (a.expr.nfactors == 0 && a.col_idx >= 0)
+        return (a.build_side && build ? *build : t).columns[a.col_idx].type;
       return a.accum_type;
   }
 }
@@ -341,12 +343,18 @@ GpuParallelHashJoinAggregate::GpuParallelHashJoinAggregate(GpuTable* build, int 
       aggs_(std::move(aggs)), opts_(opts) {
   if (build_pred) bpred_ = *build_pred;
   if (probe_pred) ppred_ = *probe_pred;
-  schema_.fields.push_back(build->host().schema().fields[build_key]);
+  if (opts_.group_by.empty()) {
+    schema_.fields.push_back(build->host().schema().fields[build_key]);
+  } else {
+    const Schema ps = probe->host().schema();
+    for (int c : opts_.group_by) schema_.fields.push_back(ps.fields[c]);
+  }
   for (const AggExpr& a : aggs_)
-    schema_.fields.push_back({a.out_name, out_type(probe->host(), a), a.kind != AggKind::Count});
+    schema_.fields.push_back({a.out_name, out_type(probe->host(), a, &build->host()), a.kind != AggKind::Count});
 }
 
 Status GpuParallelHashJoinAggregate::Setup() {
+  if (!opts_.group_by.empty()) return SetupGrouped();
   std::vector<vq_agg> va;
   for (const AggExpr& a : aggs_) va.push_back(to_vq(probe_->host(), a));
   const int64_t cap = opts_.topk > 0 ? opts_.topk
@@ -369,6 +377,43 @@ Status GpuParallelHashJoinAggregate::Setup() {
   const std::vector<DataType> kt = {DataType::Int64};
   for (int64_t lo = 0; lo < rb.r.ngroups; lo += kBlockRows)
     out_.push_back(agg_batch(schema_, rb.r, 1, (int)aggs_.size(), kd, kt, agg_types, lo,
+                             std::min<int64_t>(rb.r.ngroups, lo + kBlockRows)));
+  return Status::Ok();
+}
+
+// GROUP BY probe columns over the joined rows (vq_join_group), its result
+// sized by the two-call capacity query.
+Status GpuParallelHashJoinAggregate::SetupGrouped() {
+  const HostTable& hp = probe_->host();
+  std::vector<vq_agg> va;
+  for (const AggExpr& a : aggs_) va.push_back(to_vq(hp, a, &build_->host()));
+  const int nk = (int)opts_.group_by.size(), na = (int)aggs_.size();
+  ResultBufs rb;
+  vq_stats st;
+  int64_t cap = 1024;
+  int rc = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    rb.alloc(cap, nk, na);
+    rc = vq_join_group(build_->device()->ctx(), build_->handle(), build_key_, has_bpred_ ? &bpred_ : nullptr,
+                       probe_->handle(), probe_key_, has_ppred_ ? &ppred_ : nullptr, nk, opts_.group_by.data(),
+                       na, va.data(), &rb.r, &st);
+    if (rc != VQ_E_ARG || rb.r.ngroups <= cap) break;
+    cap = rb.r.ngroups;  // the groups the result needs
+  }
+  if (rc) return Status::FromLib(rc, "parallel join aggregate");
+  stats_.rows = st.build_rows;
+  stats_.keys = -1;
+  stats_.partitions = 0;
+  stats_.elapsed_ms = st.device_ms;
+  std::vector<std::shared_ptr<const std::vector<std::string>>> kd;
+  std::vector<DataType> kt, agg_types;
+  for (int c : opts_.group_by) {
+    kd.push_back(hp.columns[c].dict);
+    kt.push_back(hp.columns[c].type);
+  }
+  for (int j = 0; j < na; j++) agg_types.push_back(schema_.fields[nk + j].type);
+  for (int64_t lo = 0; lo < rb.r.ngroups; lo += kBlockRows)
+    out_.push_back(agg_batch(schema_, rb.r, nk, na, kd, kt, agg_types, lo,
                              std::min<int64_t>(rb.r.ngroups, lo + kBlockRows)));
   return Status::Ok();
 }

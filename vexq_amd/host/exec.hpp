@@ -90,6 +90,9 @@ struct AggExpr {
   std::string out_name;
   DataType accum_type = DataType::Int64;
   vq_expr expr{};
+  // a join grouped by its joined rows (JoinOptions::group_by): col_idx names
+  // a build-side column (in expr, a build column is col | VQ_BUILD_COL)
+  bool build_side = false;
 };
 
 // AccumTypeFor (exec/aggregate.go:51-72).
@@ -168,12 +171,18 @@ struct JoinBuildStats {
 struct JoinOptions {
   int radix_bits = -1;  // < 0: choose from the build row estimate
   int topk = 0;         // > 0: ORDER BY aggregate 0 DESC LIMIT topk
+  // non-empty: GROUP BY these probe columns over the joined rows (TPC-H Q12's
+  // shape), aggregates over either side (AggExpr::build_side); topk unused
+  std::vector<int> group_by;
 };
 
 // ParallelHashJoinAggregate (exec/parallel_join.go:718-758): inner join of
 // build[build_key] with probe[probe_key], each side filtered, GROUP BY the
 // join key, aggregates over probe columns.  Output: key column then one
-// column per aggregate.
+// column per aggregate.  With opts.group_by: GROUP BY those probe columns
+// over the joined rows, aggregates over columns of either side; output the
+// group-by columns then one column per aggregate, groups in the order of
+// their first joined probe row.
 class GpuParallelHashJoinAggregate : public Operator {
  public:
   GpuParallelHashJoinAggregate(GpuTable* build, int build_key, const vq_pred* build_pred,
@@ -186,6 +195,7 @@ class GpuParallelHashJoinAggregate : public Operator {
 
  private:
   Status Setup();
+  Status SetupGrouped();
   GpuTable* build_;
   GpuTable* probe_;
   int build_key_, probe_key_;
