@@ -96,12 +96,18 @@ def test_filter_in_and_column_compares(gpu_ctx, kernel_path):
          ("shipdate", "<", X.col("commitdate")), ("receiptdate", ">=", 9000)],
         [("discount", ">", X.col("tax")), ("quantity", "!=", X.col("quantity"))],
         [("flagA", "in", [])],
+        # StringEqExpr with a literal absent from the dictionary (code -1):
+        # no row for =, every row for <> (exec/expr.go:1342-1384)
+        [("flagA", "=", -1)],
+        [("flagA", "<>", -1), ("shipmode", "=", 3)],
     ]
     dt = gpu_ctx.upload(t)
     try:
         for terms in preds:
             p = abi.make_pred(t, terms)
             np.testing.assert_array_equal(gpu_ctx.filter(dt, p), O.filter_rows(t, p))
+        assert gpu_ctx.filter(dt, abi.make_pred(t, [("flagA", "=", -1)])).size == 0
+        assert gpu_ctx.filter(dt, abi.make_pred(t, [("flagA", "<>", -1)])).size == t.nrows
     finally:
         dt.close()
 

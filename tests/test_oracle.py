@@ -280,6 +280,10 @@ def test_filter_in_and_column_compares_match_numpy():
         ([("price", ">=", X.col("price"))], np.ones(t.nrows, bool)),
         ([("discount", ">", X.col("tax"))], d["discount"] > d["tax"]),
         ([("flagA", "in", [])], np.zeros(t.nrows, bool)),
+        # a string literal absent from the dictionary is code -1 (StringEqExpr,
+        # exec/expr.go:1342-1384: no row matches =, every row <>)
+        ([("flagA", "=", -1)], np.zeros(t.nrows, bool)),
+        ([("flagA", "<>", -1)], np.ones(t.nrows, bool)),
     ]
     for terms, mask in cases:
         np.testing.assert_array_equal(O.filter_rows(t, abi.make_pred(t, terms)), np.nonzero(mask)[0])
