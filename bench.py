@@ -10,7 +10,8 @@ line carries every operator of the path under "operators": the 1M-row GROUP
 BY, the device-side ingestion of the 6M-row table from a .vxq image, the
 zone-pruned range scan + SUM(expr) over 6M sorted rows, the filter ->
 selection vector, the 1.5M x 6M radix join + aggregate + top-10, TPC-H
-Q12's join + GROUP BY of the joined rows over 1.5M orders x 6M lineitem, the
+Q12's join + GROUP BY of the joined rows over 1.5M orders x 6M lineitem,
+TPC-H Q3's nested build over 150K customers x 1.5M orders x 6M lineitem, the
 integer-key GROUP BY at 115K and 6 groups, 256 batched instances, and the
 served stream over a 60M-row fact table (whose 60M-row GROUP BY is also the
 cache-cold kernel figure, "roofline_cache_cold").
@@ -662,6 +663,32 @@ def main():
         db.close()
         dp.close()
         log(f"[bench] join_group_q12: {rec['value']} GB/s, {rec['device_ms']} ms on the device")
+
+    # ---- TPC-H Q3's chain: a join whose build side is a join -----------------
+    if want("join_nested_q3"):
+        log("[bench] join_nested_q3: generating 150K customers x 1.5M orders x 6M lineitem")
+        tabs = datagen.q3_tables(150_000, 1_500_000, 6_000_000, seed=42)
+        ik, ip, bfk, bk, bp, pk, pp, pay, aggs, topk = datagen.q_q3(*tabs)
+        dc, do, dl = (ctx.upload(t) for t in tabs)
+        agg_arr = abi.agg_array(aggs)
+        r, total, walls, mains, devs = time_gpu(
+            lambda: ctx.join_agg_nested(dc, ik, ip, do, bfk, bk, bp, dl, pk, pp, pay, agg_arr, topk=topk)[0],
+            K, W)
+        cpu = None
+        if cpu_mod is not None:
+            runs = time_cpu(lambda: cpu_mod.join_agg_nested(*tabs[:1], ik, ip, tabs[1], bfk, bk, bp, tabs[2], pk,
+                                                            pp, pay, aggs, topk=topk), max_runs=5)
+            cpu = cpu_record(runs, r.stats.bytes_alg, 1, f"the full query, serial restatement, {len(runs)} runs")
+        rec = operator_record("join_nested_q3",
+                              "pipeline: k_lookup_init k_lookup_build k_semi_count k_filter_scan k_semi_gather "
+                              "+ the radix join (top-10) + k_keys_insert k_pay_gather",
+                              r, total, walls, mains, devs, K, traffic, cpu, sum(t.nrows for t in tabs),
+                              whole_pipeline=True)
+        rec["nested_build_rows"] = int(r.stats.build_rows)
+        ops["join_nested_q3"] = rec
+        for d in (dc, do, dl):
+            d.close()
+        log(f"[bench] join_nested_q3: {rec['value']} GB/s, {rec['device_ms']} ms on the device")
 
     # ---- GROUP BY one int64 key of any cardinality (hash path) ---------------
     for name, groups in (("hash_groupby_6m_115k", 115_000), ("hash_groupby_6m_6", 6)):

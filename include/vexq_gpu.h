@@ -377,6 +377,25 @@ int vq_join_group(vq_ctx* ctx, const vq_table* build, int32_t build_key, const v
                   int32_t nkeys, const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                   vq_agg_result* out, vq_stats* stats);
 
+/* ---- a join whose build side is itself a join (Q3's chain) ----------------
+ * The nested build of ParallelHashJoinAggregate (planner/parallel_join.go:
+ * 341-400; exec/parallel_join.go:389-552): the inner join
+ * inner.inner_key = build.build_fk (customer JOIN orders, each filtered by its
+ * own predicate: a build row meeting m inner rows is m rows of its output, in
+ * build-row order) is the build side of the outer join
+ * build.build_key = probe.probe_key (JOIN lineitem), GROUP BY the join key,
+ * aggregates over probe expressions, topk as vq_join_agg (TPC-H Q3's ORDER BY
+ * revenue DESC LIMIT 10).  npay (0..7) build columns that are functions of
+ * the key (Q3's o_orderdate, o_shippriority in its GROUP BY) come with each
+ * output group: pay_out[g * npay + c], int64 (a float64's raw bits); rows of
+ * one key that disagree on them return VQ_E_UNSUPPORTED (they would be
+ * separate groups).  Keys compare as int64; a float key is unsupported. */
+int vq_join_agg_nested(vq_ctx* ctx, const vq_table* inner, int32_t inner_key, const vq_pred* inner_pred,
+                       const vq_table* build, int32_t build_fk, int32_t build_key, const vq_pred* build_pred,
+                       const vq_table* probe, int32_t probe_key, const vq_pred* probe_pred, int32_t npay,
+                       const int32_t* pay_cols, int32_t naggs, const vq_agg* aggs, int32_t radix_bits,
+                       int32_t topk, vq_agg_result* out, int64_t* pay_out, vq_stats* stats);
+
 /* ---- concurrent and asynchronous queries (serving) ------------------------
  * No counterpart in the reference's exec package: its operators run one
  * query per call on the caller's goroutine.  A served query stream

@@ -1174,6 +1174,84 @@ int or_join_group(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, i
   return rc;
 }
 
+/* The nested build of ParallelHashJoinAggregate (planner/parallel_join.go:
+ * 341-400): the inner join's own build side hashed once (BuildSharedHashTable
+ * over the inner rows), its probe side (the build table's rows, in order)
+ * joined against it (HashJoinShared, exec/parallel_join.go:389-552), each
+ * output row once per inner match; those rows are the outer join's build
+ * side (HashJoin, exec/join.go:84-186, inserted in order), GROUP BY the outer
+ * join key with the build columns that ride with it. */
+int or_join_agg_nested(int64_t i_nrows, int32_t i_ncols, const vq_col_desc* i_cols, int32_t i_key,
+                       const vq_pred* i_pred, int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols,
+                       int32_t b_fk, int32_t b_key, const vq_pred* b_pred, int64_t p_nrows, int32_t p_ncols,
+                       const vq_col_desc* p_cols, int32_t p_key, const vq_pred* p_pred, int32_t npay,
+                       const int32_t* pay_cols, int32_t naggs, const vq_agg* aggs, int32_t topk, int32_t exact,
+                       vq_agg_result* out, int64_t* pay_out) {
+  (void)i_ncols;
+  (void)b_ncols;
+  (void)p_ncols;
+  if (naggs < 0 || naggs > VQ_MAX_AGGS || npay < 0 || npay > 7) return VQ_E_ARG;
+  int32_t* inext = (int32_t*)malloc(sizeof(int32_t) * (i_nrows ? i_nrows : 1));
+  jtable it;
+  jt_init(&it, 8, inext);
+  for (int64_t r = 0; r < i_nrows; r++)
+    if (pred_pass(i_cols, i_pred, r)) jt_insert(&it, col_i(&i_cols[i_key], r), (int32_t)r);
+  int64_t nn = 0, ncap = 1024;
+  int32_t* nested = (int32_t*)malloc(sizeof(int32_t) * ncap);
+  if (it.keys > 0)
+    for (int64_t b = 0; b < b_nrows; b++) {
+      if (!pred_pass(b_cols, b_pred, b)) continue;
+      const int64_t fk = col_i(&b_cols[b_fk], b);
+      for (int32_t x = jt_lookup(&it, fk, or_radix_hash(fk)); x != -1; x = inext[x]) {
+        if (nn == ncap) nested = (int32_t*)realloc(nested, sizeof(int32_t) * (ncap *= 2));
+        nested[nn++] = (int32_t)b;
+      }
+    }
+  int32_t* onext = (int32_t*)malloc(sizeof(int32_t) * (nn ? nn : 1));
+  jtable ot;
+  jt_init(&ot, 8, onext);
+  for (int64_t i = 0; i < nn; i++) jt_insert(&ot, col_i(&b_cols[b_key], nested[i]), (int32_t)i);
+  kmap m;
+  kmap_init(&m, 1024);
+  glist l = {0, 0, 0};
+  int rc = VQ_OK;
+  if (ot.keys > 0)
+    for (int64_t r = 0; r < p_nrows && rc == VQ_OK; r++) {
+      if (!pred_pass(p_cols, p_pred, r)) continue;
+      const int64_t key = col_i(&p_cols[p_key], r);
+      for (int32_t x = jt_lookup(&ot, key, or_radix_hash(key)); x != -1; x = onext[x]) {
+        group* g = kmap_group(&m, &l, key, naggs, aggs, exact);
+        const int64_t b = nested[x];
+        if (g->first_row < 0) {
+          g->first_row = r;
+          g->key[0] = key;
+          for (int c = 0; c < npay; c++) g->key[1 + c] = col_i(&b_cols[pay_cols[c]], b);
+        } else {
+          for (int c = 0; c < npay; c++)
+            if (g->key[1 + c] != col_i(&b_cols[pay_cols[c]], b)) rc = VQ_E_UNSUPPORTED;
+        }
+        fold_row(g, p_cols, naggs, aggs, r);
+      }
+    }
+  if (rc == VQ_OK) rc = emit_join_groups(&l, naggs, aggs, topk, out);
+  if (rc == VQ_OK)
+    for (int64_t i = 0; i < out->ngroups; i++) {
+      int found = 0;
+      const int64_t s = kmap_find(&m, out->keys[i], &found);
+      const group* g = &l.g[m.idx[s]];
+      for (int c = 0; c < npay; c++) pay_out[i * npay + c] = g->key[1 + c];
+    }
+  for (int64_t i = 0; i < l.n; i++) group_free(&l.g[i], naggs);
+  free(l.g);
+  kmap_free(&m);
+  free(ot.slots);
+  free(it.slots);
+  free(onext);
+  free(inext);
+  free(nested);
+  return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* multithreaded CPU baseline                                                */
 /* ------------------------------------------------------------------------ */

@@ -80,6 +80,20 @@ int or_join_group(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, i
                   const int32_t* key_cols, int32_t naggs, const vq_agg* aggs, int32_t exact,
                   vq_agg_result* out);
 
+/* A join whose build side is a join (the nested build, Q3's chain): inner
+ * rows passing i_pred hashed on i_key; build rows passing b_pred, each once
+ * per inner row its b_fk meets, in build-row order, are the outer build side
+ * keyed by b_key; probe rows join it as or_join_agg does, GROUP BY the join
+ * key [-> top-k]; npay build columns ride with each group into
+ * pay_out[g * npay + c] (int64: a date sign-extended, a float64's bits);
+ * VQ_E_UNSUPPORTED when rows of one key disagree on them. */
+int or_join_agg_nested(int64_t i_nrows, int32_t i_ncols, const vq_col_desc* i_cols, int32_t i_key,
+                       const vq_pred* i_pred, int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols,
+                       int32_t b_fk, int32_t b_key, const vq_pred* b_pred, int64_t p_nrows, int32_t p_ncols,
+                       const vq_col_desc* p_cols, int32_t p_key, const vq_pred* p_pred, int32_t npay,
+                       const int32_t* pay_cols, int32_t naggs, const vq_agg* aggs, int32_t topk, int32_t exact,
+                       vq_agg_result* out, int64_t* pay_out);
+
 /* Multithreaded morsel restatement (CPU baseline): ParallelHashAggregate and
  * ParallelHashJoinAggregate with 64K-row morsels on an atomic cursor,
  * thread-local partial aggregates merged serially, and the two-pass radix

@@ -72,6 +72,11 @@ def lib():
                  C.c_int32, P(abi.ColDesc), C.c_int32, P(abi.Pred), C.c_int32, P(abi.Agg),
                  C.c_int32]
     L.or_join_agg.argtypes = join_args + [C.c_int32, P(abi.AggResult)]
+    L.or_join_agg_nested.argtypes = ([C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32, P(abi.Pred),
+                                      C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32, C.c_int32, P(abi.Pred),
+                                      C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32, P(abi.Pred),
+                                      C.c_int32, P(C.c_int32), C.c_int32, P(abi.Agg), C.c_int32, C.c_int32,
+                                      P(abi.AggResult), P(C.c_int64)])
     L.or_join_group.argtypes = join_args[:10] + [C.c_int32, P(C.c_int32), C.c_int32, P(abi.Agg), C.c_int32,
                                                  P(abi.AggResult)]
     L.or_join_agg_parallel.argtypes = join_args + [C.c_int32, P(abi.AggResult)]
@@ -212,6 +217,27 @@ def join_group(build, build_key, build_pred, probe, probe_key, probe_pred, key_c
     if rc:
         raise RuntimeError(f"oracle join_group failed: {rc}")
     return out
+
+
+def join_agg_nested(inner, inner_key, inner_pred, build, build_fk, build_key, build_pred, probe, probe_key,
+                    probe_pred, pay_cols, aggs, topk=0, exact: bool = True, capacity: int | None = None):
+    """or_join_agg_nested: (result, payload [ngroups, npay] int64)."""
+    idesc, bd, pd = abi.col_descs(inner), abi.col_descs(build), abi.col_descs(probe)
+    arr = abi.agg_array(aggs)
+    parr = (C.c_int32 * max(1, len(pay_cols)))(*pay_cols)
+    if capacity is None:
+        capacity = topk if topk > 0 else max(1, probe.nrows)
+    out = abi.AggOut(capacity, 1, len(aggs))
+    pay = np.zeros(capacity * max(1, len(pay_cols)), dtype=np.int64)
+    rc = lib().or_join_agg_nested(inner.nrows, len(inner.columns), idesc, inner_key, _pp(inner_pred), build.nrows,
+                                  len(build.columns), bd, build_fk, build_key, _pp(build_pred), probe.nrows,
+                                  len(probe.columns), pd, probe_key, _pp(probe_pred), len(pay_cols), parr, len(aggs),
+                                  arr, topk, int(exact), C.byref(out.c),
+                                  pay.ctypes.data_as(C.POINTER(C.c_int64)))
+    if rc:
+        raise RuntimeError(f"oracle join_agg_nested failed: {rc}")
+    n = out.ngroups
+    return out, pay[: n * len(pay_cols)].reshape(n, len(pay_cols))
 
 
 # ---- the reference's own AVX2 filter kernel (built from its source) ---------

@@ -65,13 +65,15 @@ def test_packed_group_columns_and_build_payload(gpu_ctx):
         dp.close()
 
 
-@pytest.mark.parametrize("keys", ["signed", "non_negative"])
+@pytest.mark.parametrize("keys", ["signed", "non_negative", "dense"])
 def test_duplicate_and_extreme_build_keys(gpu_ctx, keys):
     """Build keys repeating up to dozens of times, probe keys with no match:
     every probe row meets every build row with its key.  signed: INT64_MIN and
-    0 keys on both sides (two-array table); non_negative: keys below 2^32 - 1
-    (one-word slots, repeated keys chained by compare-and-swap), probe keys
-    at and past that bound."""
+    0 keys on both sides (two-array hash table); non_negative: keys below
+    2^32 - 1 spread wide (one-word slots, repeated keys chained by
+    compare-and-swap), probe keys at and past that bound; dense: keys in a
+    range below four times the rows (a direct table, repeated keys chained in
+    by its third pass), probe keys on both sides of the range."""
     _case("duplicates", keys=keys)
     rng = np.random.default_rng(23)
     nb, npr = 60_000, 250_000
@@ -82,6 +84,9 @@ def test_duplicate_and_extreme_build_keys(gpu_ctx, keys):
         pkey = rng.integers(-600, 3100, npr).astype(np.int64)
         pkey[::131] = np.iinfo(np.int64).min
         pkey[::257] = 0
+    elif keys == "dense":
+        bkey = rng.integers(100, 3000, nb).astype(np.int64)
+        pkey = rng.integers(0, 3200, npr).astype(np.int64)
     else:
         bkey = rng.integers(0, 3000, nb).astype(np.int64)
         bkey[:30] = 0xFFFFFFFE

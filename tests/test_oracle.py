@@ -470,6 +470,63 @@ def test_join_group_by_the_join_key_is_the_join_aggregate():
     np.testing.assert_array_equal(a.fvals.view(np.int64), j.fvals.view(np.int64))
 
 
+def test_nested_build_pure_python():
+    """The nested build (planner/parallel_join.go:341-400) against a
+    pure-Python nested-loop evaluation: inner keys repeated (a build row meets
+    several inner rows and is that many rows of the outer build side), build
+    keys repeated with the same riding columns, unmatched keys on every side."""
+    rng = np.random.default_rng(41)
+    ni, nb, npr = 120, 400, 1500
+    ikey = rng.integers(0, 90, ni).astype(np.int64)
+    iflag = rng.integers(0, 3, ni).astype(np.uint8)
+    bfk = rng.integers(0, 100, nb).astype(np.int64)
+    bkey = rng.integers(0, 300, nb).astype(np.int64)
+    bdate = (9000 + (bkey * 7) % 50).astype(np.int32)  # a function of the key
+    bval = rng.integers(0, 1000, nb).astype(np.int64)
+    pkey = rng.integers(0, 320, npr).astype(np.int64)
+    px = rng.standard_normal(npr)
+    inner = _table([("k", abi.VQ_I64, ikey, 0), ("f", abi.VQ_U8, iflag, 3)], [ni])
+    build = _table([("fk", abi.VQ_I64, bfk, 0), ("k", abi.VQ_I64, bkey, 0), ("d", abi.VQ_DATE, bdate, 0),
+                    ("v", abi.VQ_I64, bval, 0)], [nb])
+    probe = _table([("k", abi.VQ_I64, pkey, 0), ("x", abi.VQ_F64, px, 0)], [npr])
+    ip = abi.make_pred(inner, [("f", "!=", 2)])
+    bp = abi.make_pred(build, [("v", "<", 800)])
+    aggs = [abi.count_star(probe), abi.agg_sum("x", probe), abi.agg_max("k", probe)]
+    r, pay = O.join_agg_nested(inner, 0, ip, build, 0, 1, bp, probe, 0, None, [2], aggs, exact=True)
+    nested = [b for b in range(nb) if bval[b] < 800 for i in range(ni) if ikey[i] == bfk[b] and iflag[i] != 2]
+    groups = {}
+    for p in range(npr):
+        for b in nested:
+            if bkey[b] == pkey[p]:
+                g = groups.setdefault(int(pkey[p]), [0, [], p, int(bdate[b])])
+                g[0] += 1
+                g[1].append(px[p])
+    order = sorted(groups, key=lambda k: groups[k][2])
+    assert r.ngroups == len(order) and list(r.keys[:, 0]) == order
+    for gi, k in enumerate(order):
+        cnt, xs, first, d = groups[k]
+        assert (r.counts[gi], r.ivals[gi, 0], r.first_row[gi], r.ivals[gi, 2], pay[gi, 0]) == (cnt, cnt, first, k, d)
+        assert r.fvals[gi, 1] == math.fsum(xs)
+    # the top 3 by SUM(x): the same groups, ranked
+    t3, p3 = O.join_agg_nested(inner, 0, ip, build, 0, 1, bp, probe, 0, None, [2], aggs[1:2], topk=3, exact=True)
+    want = sorted(order, key=lambda k: -math.fsum(groups[k][1]))[:3]
+    assert list(t3.keys[:, 0]) == want and list(p3[:, 0]) == [groups[k][3] for k in want]
+
+
+def test_nested_build_with_every_row_matched_once_is_the_join():
+    """An inner side that every build row meets exactly once (unique keys, no
+    predicate) leaves the build side as it is: or_join_agg's result."""
+    c, o, li = datagen.q3_tables(2_000, 20_000, 60_000, seed=42)
+    bp, pp, aggs = datagen.q_join(o, li)
+    aggs = aggs + [abi.count_star(li)]
+    r, _ = O.join_agg_nested(c, 0, None, o, o.index("custkey"), 0, bp, li, 0, pp, [], aggs, exact=True)
+    j = O.join_agg(o, 0, bp, li, 0, pp, aggs, exact=True)
+    assert r.ngroups == j.ngroups > 100
+    for f in ("keys", "counts", "first_row", "ivals"):
+        np.testing.assert_array_equal(getattr(r, f), getattr(j, f))
+    np.testing.assert_array_equal(r.fvals.view(np.int64), j.fvals.view(np.int64))
+
+
 # ---- dictionary-code GROUP BY -----------------------------------------------
 
 def test_intkey_remap_kat():

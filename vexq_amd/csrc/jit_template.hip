@@ -160,6 +160,9 @@ struct Args {
   const u32* lk_min;
   u32 lk_mask;
   u32 lk_packed;      // slots are words key << 32 | head (all ones: empty)
+  i64 lk_base;        // lk_range > 0: a direct table, head[key - lk_base]
+  u32 lk_range;
+  u32 lk_pad;
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -1231,6 +1234,10 @@ __device__ __forceinline__ void lload_one(const Args& a, i64 r, i64 (&raw)[VQ_NS
 #if VQ_LOOKUP
 // The first build row with this key (0xFFFFFFFF: none).
 __device__ __forceinline__ u32 lookup_head(const Args& a, i64 key) {
+  if (a.lk_range) {
+    const u64 i = (u64)key - (u64)a.lk_base;
+    return i < a.lk_range ? a.lk_head[i] : 0xFFFFFFFFu;
+  }
   if (a.lk_packed) {
     if (key < 0 || key >= 0xFFFFFFFFll) return 0xFFFFFFFFu;  // (no build key there)
     const u64 kk = (u64)key << 32;

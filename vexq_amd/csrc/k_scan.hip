@@ -756,6 +756,14 @@ hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part
 // the one before.  Chain order is arrival order; the probe side folds every
 // match exactly (integer and fixed-point words), so it does not matter.
 __global__ __launch_bounds__(256) void k_lookup_init(const LookupBufs lk) {
+  if (lk.range) {  // a direct table
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < lk.range; i += gridDim.x * 256) {
+      lk.head[i] = 0xFFFFFFFFu;
+      lk.cnt[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) lk.min_head[threadIdx.x] = threadIdx.x == 0 ? 0xFFFFFFFFu : 0u;
+    return;
+  }
   const uint32_t n = lk.mask + 1;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     lk.cnt[i] = 0;
@@ -819,9 +827,34 @@ __device__ __forceinline__ void lookup_insert(const LookupBufs& lk, int64_t key,
   atomicOr(&lk.min_head[1], 1u);  // (the host sizes the table above the rows: not reached)
 }
 
-// One tile per wave (grid = ntiles), as k_filter_bits.
+// A direct table's three passes over the rows it takes: (0) each row stores
+// its id at its key (one store survives); (1) a row whose store did not
+// marks itself; (2) the marked rows chain in by exchange (a repeated key)
+// and count its rows.  Keys outside [base, base + range) contradict the zone
+// maps the range came from: flagged.
+constexpr uint32_t kLostRow = 0xFFFFFFFEu;
+__device__ __forceinline__ void direct_pass(const LookupBufs& lk, int64_t key, uint32_t r, int pass) {
+  const uint64_t i = (uint64_t)key - (uint64_t)lk.base;
+  if (i >= lk.range) {
+    if (pass == 0) atomicOr(&lk.min_head[1], 2u);
+    return;
+  }
+  if (pass == 0) {
+    lk.head[i] = r;
+    lk.next[r] = 0xFFFFFFFFu;
+  } else if (pass == 1) {
+    if (lk.head[i] != r) lk.next[r] = kLostRow;
+  } else if (lk.next[r] == kLostRow) {
+    lk.next[r] = atomicExch(&lk.head[i], r);
+    atomicMax(&lk.min_head[2], atomicAdd(&lk.cnt[i], 1u) + 2u);
+  }
+}
+
+// One tile per wave (grid = ntiles), as k_filter_bits; `step` is
+// direct_pass's pass for a direct table (a hash table takes step 0 only).
 template <int U>
-__global__ __launch_bounds__(kWave) void k_lookup_build(const QDesc d, int32_t key_slot, const LookupBufs lk) {
+__global__ __launch_bounds__(kWave) void k_lookup_build(const QDesc d, int32_t key_slot, const LookupBufs lk,
+                                                        int32_t step) {
   const int lane = threadIdx.x;
   const int NS = d.nslots;
   SlotLd ld[kMaxSlots];
@@ -840,17 +873,226 @@ __global__ __launch_bounds__(kWave) void k_lookup_build(const QDesc d, int32_t k
     pick<U>(raw, key_slot, key);
 #pragma unroll
     for (int u = 0; u < U; u++)
-      if (pass[u]) lookup_insert(lk, key[u], (uint32_t)(it.base + u * kWave + lane));
+      if (pass[u]) {
+        const uint32_t r = (uint32_t)(it.base + u * kWave + lane);
+        if (lk.range) direct_pass(lk, key[u], r, step);
+        else lookup_insert(lk, key[u], r);
+      }
     it.base += kWave * U;
     if (it.base >= it.rend) it.have = false;
   }
 }
 
 hipError_t launch_lookup_build(const QDesc& d, int32_t key_slot, const LookupBufs& lk, hipStream_t s) {
-  const uint32_t blocks = std::min<uint32_t>(2048, (lk.mask + 256) / 256);
+  const uint32_t n = lk.range ? lk.range : lk.mask + 1;
+  const uint32_t blocks = std::min<uint32_t>(2048, (n + 255) / 256);
   hipLaunchKernelGGL(k_lookup_init, dim3(blocks), dim3(256), 0, s, lk);
+  for (int pass = 0; d.ntiles > 0 && pass < (lk.range ? 3 : 1); pass++)
+    hipLaunchKernelGGL((k_lookup_build<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, key_slot, lk, pass);
+  return hipGetLastError();
+}
+
+// ---- a join whose build side is itself a join (the nested build) ------------
+// The inner side's rows are in a lookup table (launch_lookup_build); a build
+// row that passes its predicate and meets m inner rows is m rows of the
+// nested join's output (HashJoinShared's rows, exec/parallel_join.go:
+// 389-552), which become the outer join's build side, kept in row order.
+
+__global__ __launch_bounds__(64) void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
+  for (int64_t i = threadIdx.x; i < n; i += 64) p[i] = 0;
+}
+
+__global__ __launch_bounds__(64) void k_zero_u32_1(uint32_t* __restrict__ p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+// The first row chained under key (0xFFFFFFFF: none).
+__device__ __forceinline__ uint32_t lookup_first(const LookupBufs& lk, int64_t key) {
+  if (lk.range) {
+    const uint64_t i = (uint64_t)key - (uint64_t)lk.base;
+    return i < lk.range ? lk.head[i] : 0xFFFFFFFFu;
+  }
+  if (lk.packed) {
+    if (key < 0 || key >= 0xFFFFFFFFll) return 0xFFFFFFFFu;
+    const uint64_t kk = (uint64_t)key << 32;
+    uint32_t i = (uint32_t)(radix_hash((uint64_t)key) >> 32) & lk.mask;
+    for (uint32_t pr = 0; pr <= lk.mask; pr++) {
+      const uint64_t w = ((const uint64_t*)lk.keys)[i];
+      if ((w & 0xFFFFFFFF00000000ull) == kk) return (uint32_t)w;
+      if (w == ~0ull) break;
+      i = (i + 1) & lk.mask;
+    }
+    return 0xFFFFFFFFu;
+  }
+  if (key == INT64_MIN) return lk.min_head[0];
+  uint32_t i = (uint32_t)(radix_hash((uint64_t)key) >> 32) & lk.mask;
+  for (uint32_t pr = 0; pr <= lk.mask; pr++) {
+    const int64_t k = lk.keys[i];
+    if (k == key) return lk.head[i];
+    if (k == INT64_MIN) break;
+    i = (i + 1) & lk.mask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t chain_len(const LookupBufs& lk, int64_t key) {
+  uint32_t m = 0;
+  for (uint32_t r = lookup_first(lk, key); r != 0xFFFFFFFFu; r = lk.next[r]) m++;
+  return m;
+}
+
+// Rows of each tile (one tile per wave, as k_filter_bits): the sum over its
+// passing rows of the inner rows each meets.
+template <int U>
+__global__ __launch_bounds__(kWave) void k_semi_count(const QDesc d, int32_t fk_slot, const LookupBufs lk,
+                                                      uint32_t* __restrict__ tile_counts) {
+  const int lane = threadIdx.x;
+  const int NS = d.nslots;
+  SlotLd ld[kMaxSlots];
+#pragma unroll
+  for (int c = 0; c < kMaxSlots; c++) ld[c] = slot_ld(d.slot[c < NS ? c : 0]);
+  TileIt it;
+  tile_begin(d, blockIdx.x, it);
+  const int64_t t0 = it.t;
+  uint32_t count = 0;
+  while (it.have && it.t == t0) {
+    int64_t raw[kMaxSlots][U];
+    load_rows<U>(ld, NS, it.base, it.rend, lane, raw);
+    bool pass[U];
+    int g[U];
+    eval_pred<U>(d.slot, NS, it.base, it.rend, lane, raw, pass, g);
+    int64_t fk[U];
+    pick<U>(raw, fk_slot, fk);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (pass[u]) count += chain_len(lk, fk[u]);
+    it.base += kWave * U;
+    if (it.base >= it.rend) it.have = false;
+  }
+  for (int off = 32; off >= 1; off >>= 1) count += (uint32_t)__shfl_xor((int)count, off);
+  if (lane == 0) tile_counts[t0] = count;
+}
+
+__device__ __forceinline__ void store_as(void* dst, int32_t type, int64_t i, int64_t v) {
+  switch (type) {
+    case VQ_U8: ((uint8_t*)dst)[i] = (uint8_t)v; break;
+    case VQ_U16: ((uint16_t*)dst)[i] = (uint16_t)v; break;
+    case VQ_DATE: ((int32_t*)dst)[i] = (int32_t)v; break;
+    default: ((int64_t*)dst)[i] = v; break;
+  }
+}
+
+// Each passing row's m copies at offsets[tile] + (the copies of the tile's
+// rows before it): the output in row order.
+template <int U>
+__global__ __launch_bounds__(kWave) void k_semi_gather(const QDesc d, int32_t fk_slot, const LookupBufs lk,
+                                                       const uint32_t* __restrict__ offsets, const SemiOut o) {
+  const int lane = threadIdx.x;
+  const int NS = d.nslots;
+  SlotLd ld[kMaxSlots];
+#pragma unroll
+  for (int c = 0; c < kMaxSlots; c++) ld[c] = slot_ld(d.slot[c < NS ? c : 0]);
+  TileIt it;
+  tile_begin(d, blockIdx.x, it);
+  const int64_t t0 = it.t;
+  int64_t pos = offsets[t0];
+  while (it.have && it.t == t0) {
+    int64_t raw[kMaxSlots][U];
+    load_rows<U>(ld, NS, it.base, it.rend, lane, raw);
+    bool pass[U];
+    int g[U];
+    eval_pred<U>(d.slot, NS, it.base, it.rend, lane, raw, pass, g);
+    int64_t fk[U];
+    pick<U>(raw, fk_slot, fk);
+#pragma unroll
+    for (int u = 0; u < U; u++) {  // rows base + u * 64 + lane: row order is (u, lane)
+      const uint32_t m = pass[u] ? chain_len(lk, fk[u]) : 0u;
+      uint32_t incl = m;  // inclusive scan over the wave's lanes
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= off) incl += y;
+      }
+      const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1);
+      const int64_t at = pos + incl - m;
+      for (uint32_t k = 0; k < m; k++)
+        for (int c = 0; c < o.ncols; c++) {
+          int64_t v[U];
+          pick<U>(raw, o.slot[c], v);
+          store_as(o.dst[c], o.type[c], at + k, v[u]);
+        }
+      pos += total;
+    }
+    it.base += kWave * U;
+    if (it.base >= it.rend) it.have = false;
+  }
+}
+
+// The output groups' build-side columns: keys[i] -> group i in a lookup
+// table, then every row of the nested output writes its columns to its
+// group's (pass 0), and checks them (pass 1: rows of one key that disagree
+// would be separate groups of GROUP BY key, columns -- flagged).
+__global__ __launch_bounds__(256) void k_keys_insert(const int64_t* __restrict__ keys, int64_t n,
+                                                     const LookupBufs lk, int32_t pass) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    if (lk.range) direct_pass(lk, keys[i], (uint32_t)i, pass);
+    else lookup_insert(lk, keys[i], (uint32_t)i);
+  }
+}
+
+__device__ __forceinline__ int64_t load_as(const void* p, int32_t type, int64_t i) {
+  switch (type) {
+    case VQ_U8: return ((const uint8_t*)p)[i];
+    case VQ_U16: return ((const uint16_t*)p)[i];
+    case VQ_DATE: return ((const int32_t*)p)[i];
+    default: return ((const int64_t*)p)[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pay_gather(const PayCols pc, int64_t n, const LookupBufs lk,
+                                                    int64_t* __restrict__ pay, uint32_t* __restrict__ flag,
+                                                    int32_t check) {
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) {
+    const uint32_t g = lookup_first(lk, load_as(pc.key, pc.key_type, r));
+    if (g == 0xFFFFFFFFu) continue;
+    for (int c = 0; c < pc.npay; c++) {
+      const int64_t v = load_as(pc.src[c], pc.type[c], r);
+      if (!check) pay[(int64_t)g * pc.npay + c] = v;
+      else if (pay[(int64_t)g * pc.npay + c] != v) atomicOr(flag, 1u);
+    }
+  }
+}
+
+hipError_t launch_semi(const QDesc& d, int32_t fk_slot, const LookupBufs& lk, uint32_t* tile_counts,
+                       uint32_t* tile_offsets, uint64_t* out_count, hipStream_t s) {
+  if (d.ntiles == 0) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out_count, (int64_t)1);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((k_semi_count<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, fk_slot, lk, tile_counts);
+  hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(1024), 0, s, tile_counts, d.ntiles, tile_offsets, out_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_semi_gather(const QDesc& d, int32_t fk_slot, const LookupBufs& lk,
+                              const uint32_t* tile_offsets, const SemiOut& o, hipStream_t s) {
   if (d.ntiles > 0)
-    hipLaunchKernelGGL((k_lookup_build<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, key_slot, lk);
+    hipLaunchKernelGGL((k_semi_gather<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, fk_slot, lk,
+                       tile_offsets, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_pay_gather(const int64_t* keys, int64_t nkeys, const LookupBufs& lk, const PayCols& pc,
+                             int64_t nrows, int64_t* pay, uint32_t* flag, hipStream_t s) {
+  const uint32_t kb = (uint32_t)std::min<int64_t>(1024, (nkeys + 255) / 256 + 1);
+  const uint32_t n = lk.range ? lk.range : lk.mask + 1;
+  const uint32_t lb = std::min<uint32_t>(2048, (n + 255) / 256);
+  hipLaunchKernelGGL(k_lookup_init, dim3(lb), dim3(256), 0, s, lk);
+  for (int pass = 0; pass < (lk.range ? 3 : 1); pass++)
+    hipLaunchKernelGGL(k_keys_insert, dim3(kb), dim3(256), 0, s, keys, nkeys, lk, pass);
+  const uint32_t rb = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(2048, (nrows + 255) / 256));
+  hipLaunchKernelGGL(k_zero_u32_1, dim3(1), dim3(64), 0, s, flag);
+  hipLaunchKernelGGL(k_pay_gather, dim3(rb), dim3(256), 0, s, pc, nrows, lk, pay, flag, 0);
+  hipLaunchKernelGGL(k_pay_gather, dim3(rb), dim3(256), 0, s, pc, nrows, lk, pay, flag, 1);
   return hipGetLastError();
 }
 

@@ -826,6 +826,10 @@ constexpr int kDenseGroupsMax = 128;
 // Tile rows for a scan of `rows` rows: a power of two between 256 and 16384
 // that cuts the scan into about 2048 tiles, so small (zone-pruned) scans still
 // spread over every CU and large ones keep per-tile overhead low.
+// Rows per tile (one wave each) of the kernels that insert into or look up
+// a device hash table row by row.
+constexpr int32_t kLookupTileRows = 256;
+
 int32_t scan_tile_rows(int64_t rows) {
   int64_t t = 256;
   while (t < 16384 && t * 2048 < rows) t *= 2;
@@ -1512,6 +1516,78 @@ struct JoinTune {
 
 constexpr int kRetry = -1;  // internal: repartition and run again
 
+// A whole column's zone over all row groups (has_minmax: any group has
+// rows), compared as col_interval reads the bits.
+vq_zone merged_zone(const vq_table* t, int32_t col) {
+  vq_zone m;
+  memset(&m, 0, sizeof m);
+  const int type = t->cols[col].type;
+  auto val = [&](uint64_t b) {
+    return type == VQ_F64 ? bits_to_f64((int64_t)b)
+                          : type == VQ_DATE ? (double)(int32_t)(uint32_t)b : (double)(int64_t)b;
+  };
+  for (int32_t g = 0; g < t->nrg; g++) {
+    const vq_zone& z = t->zones[(size_t)g * t->ncols + col];
+    m.null_count += z.null_count;
+    if (!z.has_minmax) continue;
+    if (!m.has_minmax || val(z.min_bits) < val(m.min_bits)) m.min_bits = z.min_bits;
+    if (!m.has_minmax || val(z.max_bits) > val(m.max_bits)) m.max_bits = z.max_bits;
+    m.has_minmax = 1;
+  }
+  return m;
+}
+
+// A device lookup table's shape for keys in [lo, hi] over `rows` rows: a
+// direct table (head[key - lo]) when the range is at most four times the
+// rows, else a hash table of a power of two of slots, at least twice the
+// rows, one-word slots when every key lies in [0, 2^32 - 1).
+struct LookupPlan {
+  uint32_t cap = 16;
+  uint32_t range = 0;
+  int64_t base = 0;
+  bool packed = false;
+};
+
+LookupPlan plan_lookup(int64_t rows, bool bounded, int64_t lo, int64_t hi) {
+  LookupPlan p;
+  if (bounded && hi >= lo && (uint64_t)(hi - lo) < (uint64_t)std::min<int64_t>(INT32_MAX, std::max<int64_t>(4 * rows, 4096))) {
+    p.range = (uint32_t)(hi - lo + 1);
+    p.base = lo;
+    return p;
+  }
+  while ((int64_t)p.cap < 2 * rows) p.cap *= 2;
+  p.packed = !bounded || (lo >= 0 && hi < (int64_t)0xFFFFFFFFll);
+  return p;
+}
+
+// The same from a key column's zone maps.
+LookupPlan plan_lookup_col(const vq_table* t, int32_t col, int64_t rows) {
+  const vq_zone z = merged_zone(t, col);
+  const int type = t->cols[col].type;
+  const int64_t lo = type == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.min_bits : (int64_t)z.min_bits;
+  const int64_t hi = type == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.max_bits : (int64_t)z.max_bits;
+  return plan_lookup(rows, z.has_minmax != 0, lo, hi);
+}
+
+size_t lookup_bytes(const LookupPlan& p, int64_t rows) {
+  const size_t slots = p.range ? p.range : p.cap;
+  return (p.range ? 0 : (size_t)p.cap * 8) + slots * 8 + (size_t)std::max<int64_t>(rows, 1) * 4 + 16 + 6 * 256;
+}
+
+void carve_lookup(Carve& cv, char* dev, const LookupPlan& p, int64_t rows, LookupBufs& lk) {
+  const size_t slots = p.range ? p.range : p.cap;
+  lk.keys = cv.take<int64_t>(dev, p.range ? 1 : p.cap);
+  lk.head = cv.take<uint32_t>(dev, slots);
+  lk.next = cv.take<uint32_t>(dev, (size_t)std::max<int64_t>(rows, 1));
+  lk.cnt = cv.take<uint32_t>(dev, slots);
+  lk.min_head = cv.take<uint32_t>(dev, 4);
+  lk.mask = p.cap - 1;
+  lk.packed = p.packed ? 1u : 0u;
+  lk.base = p.base;
+  lk.range = p.range;
+  lk.pad_ = 0;
+}
+
 // One join + aggregate (or, with groupby, one GROUP BY of the probe side's
 // key with no build side): planned on the host, bound to device and pinned
 // memory, enqueued on a stream (or captured into a graph), then turned into
@@ -1573,10 +1649,9 @@ struct JoinJob {
   vq_pred lbpred;
   const vq_pred* lbp = nullptr;
   QDesc lq;
-  uint32_t lcap = 0;
+  LookupPlan lplan;
   int64_t lrows = 0, lkept = 0, lrow_bytes = 0;
   int64_t lmult = 1;  // the most build rows one key has, as planned
-  bool lpacked = false;  // every build key in [0, 2^32 - 1): one-word slots
 };
 
 // Table fill: a GROUP BY's rows all find their keys, but fill the table by
@@ -1808,7 +1883,7 @@ int join_plan_local(JoinJob& j) {
   size_t bytes = 16 * 4 + (size_t)plan.gcap * 8 + (size_t)plan.limit * plan.passes * 8 +
                  (size_t)plan.lgrid * plan.limit * (j.nw + 2) * 8 + (size_t)j.rec_cap * j.RW * 8;
   if (j.lookup)  // the build side's lookup table: keys, heads, counts, a next row per build row
-    bytes += (size_t)j.lcap * 16 + (size_t)std::max<int64_t>(j.lbuild->nrows, 1) * 4 + 16;
+    bytes += lookup_bytes(j.lplan, j.lbuild->nrows);
   bytes += 12 * 256;  // alignment of the carved pieces
   j.bytes = (bytes + 255) & ~(size_t)255;
   j.host_bytes = 64 + (size_t)16 * j.RW * 8;
@@ -1950,15 +2025,7 @@ void join_bind(JoinJob& j, char* dev, char* host) {
     B.l.gkey = cv.take<int64_t>(dev, (size_t)j.plan.limit * j.plan.passes);
     B.l.part = cv.take<uint64_t>(dev, (size_t)j.plan.lgrid * j.plan.limit * (j.nw + 2));
     B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
-    if (j.lookup) {
-      B.lk.keys = cv.take<int64_t>(dev, j.lcap);
-      B.lk.head = cv.take<uint32_t>(dev, j.lcap);
-      B.lk.next = cv.take<uint32_t>(dev, (size_t)std::max<int64_t>(j.lbuild->nrows, 1));
-      B.lk.cnt = cv.take<uint32_t>(dev, j.lcap);
-      B.lk.min_head = cv.take<uint32_t>(dev, 4);
-      B.lk.mask = j.lcap - 1;
-      B.lk.packed = j.lpacked ? 1u : 0u;
-    }
+    if (j.lookup) carve_lookup(cv, dev, j.lplan, j.lbuild->nrows, B.lk);
     return;
   }
   B.b.perr = B.p.perr = B.flags + 5;
@@ -2349,27 +2416,6 @@ int agg_packed(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
   return join_run(*j, out, stats);
 }
 
-// A whole column's zone over all row groups (has_minmax: any group has
-// rows), compared as col_interval reads the bits.
-vq_zone merged_zone(const vq_table* t, int32_t col) {
-  vq_zone m;
-  memset(&m, 0, sizeof m);
-  const int type = t->cols[col].type;
-  auto val = [&](uint64_t b) {
-    return type == VQ_F64 ? bits_to_f64((int64_t)b)
-                          : type == VQ_DATE ? (double)(int32_t)(uint32_t)b : (double)(int64_t)b;
-  };
-  for (int32_t g = 0; g < t->nrg; g++) {
-    const vq_zone& z = t->zones[(size_t)g * t->ncols + col];
-    m.null_count += z.null_count;
-    if (!z.has_minmax) continue;
-    if (!m.has_minmax || val(z.min_bits) < val(m.min_bits)) m.min_bits = z.min_bits;
-    if (!m.has_minmax || val(z.max_bits) > val(m.max_bits)) m.max_bits = z.max_bits;
-    m.has_minmax = 1;
-  }
-  return m;
-}
-
 // The column space of a GROUP BY over joined rows: the probe's columns, then
 // the build's (build column c is column probe->ncols + c), over the probe's
 // row groups.  A build column's zone map in every row group is its whole
@@ -2411,21 +2457,13 @@ int lookup_setup(JoinJob& j) {
   d.ngroups = 1;
   std::vector<uint8_t> keep;
   prune(b, j.lbp, keep);
-  d.tile_rows = scan_tile_rows(kept_rows(b, keep));
+  // one tile per wave of 256 rows: each lane's inserts wait on their
+  // compare-and-swaps, so the pass wants many waves in flight
+  d.tile_rows = kLookupTileRows;
   if (int rc = make_spans(b, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &j.lrows, &j.lkept)) return rc;
   if (b->nrows >= (int64_t)INT32_MAX) return fail(VQ_E_UNSUPPORTED, "exec: gpu: build side above 2^31 rows");
   j.lrow_bytes = slot_bytes_per_row(b, sm);
-  {
-    // every build key non-negative and below 2^32 - 1 (the zone maps'
-    // bounds): a slot is one word, key << 32 | head
-    const vq_zone z = merged_zone(b, j.lbuild_key);
-    const int type = b->cols[j.lbuild_key].type;
-    const int64_t lo = type == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.min_bits : (int64_t)z.min_bits;
-    const int64_t hi = type == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.max_bits : (int64_t)z.max_bits;
-    j.lpacked = !z.has_minmax || (lo >= 0 && hi < (int64_t)0xFFFFFFFFll);
-  }
-  j.lcap = 16;
-  while ((int64_t)j.lcap < 2 * j.lrows) j.lcap *= 2;
+  j.lplan = plan_lookup_col(b, j.lbuild_key, j.lrows);
   return VQ_OK;
 }
 
@@ -2528,6 +2566,206 @@ extern "C" int vq_join_group(vq_ctx* c, const vq_table* build, int32_t build_key
   }
   if (int rc = lookup_setup(*j)) return rc;
   return join_run(*j, out, stats);
+}
+
+namespace {
+
+// One side of the nested build as a scan: its predicate's slots and the
+// given key columns' (slots returned in key_slots), over the row groups its
+// predicate keeps.
+int scan_desc(const vq_table* t, const vq_pred* pred, int nkeys, const int32_t* keys, QDesc& d, int* key_slots,
+              int64_t* rows, int64_t* kept) {
+  memset(&d, 0, sizeof d);
+  SlotMap sm;
+  if (int rc = add_pred(t, pred, sm, d.slot)) return rc;
+  for (int i = 0; i < nkeys; i++) {
+    if (int rc = check_col(t, keys[i])) return rc;
+    key_slots[i] = sm.get(keys[i]);
+    if (key_slots[i] < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+  }
+  fill_slot_cols(t, sm, d.slot);
+  d.nslots = sm.n;
+  d.ngroups = 1;
+  std::vector<uint8_t> keep;
+  prune(t, pred, keep);
+  d.tile_rows = kLookupTileRows;  // (lookups and inserts: latency-bound, many waves)
+  return make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, rows, kept);
+}
+
+}  // namespace
+
+extern "C" int vq_join_agg_nested(vq_ctx* c, const vq_table* inner, int32_t inner_key, const vq_pred* inner_pred,
+                                  const vq_table* build, int32_t build_fk, int32_t build_key,
+                                  const vq_pred* build_pred, const vq_table* probe, int32_t probe_key,
+                                  const vq_pred* probe_pred, int32_t npay, const int32_t* pay_cols, int32_t naggs,
+                                  const vq_agg* aggs, int32_t radix_bits, int32_t topk, vq_agg_result* out,
+                                  int64_t* pay_out, vq_stats* stats) {
+  if (!c || !inner || !build || !probe || !out) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  out->ngroups = 0;  // no result unless this call succeeds
+  if (inner->ctx != c || build->ctx != c || probe->ctx != c)
+    return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  if (npay < 0 || npay > 7 || (npay > 0 && (!pay_cols || !pay_out)))
+    return fail(VQ_E_ARG, "exec: gpu: %d columns with the groups", npay);
+  if (int rc = check_col(inner, inner_key)) return rc;
+  if (int rc = check_col(build, build_fk)) return rc;
+  if (int rc = check_col(build, build_key)) return rc;
+  if (inner->cols[inner_key].type == VQ_F64 || build->cols[build_fk].type == VQ_F64 ||
+      build->cols[build_key].type == VQ_F64)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: float join key");
+  // the inner side (into a lookup table) and the build side (scanned against it)
+  QDesc iq, bq;
+  int ikey = 0;
+  int64_t irows = 0, ikept = 0, brows = 0, bkept = 0;
+  if (int rc = scan_desc(inner, inner_pred, 1, &inner_key, iq, &ikey, &irows, &ikept)) return rc;
+  int32_t bkeys[9] = {build_fk, build_key};
+  for (int i = 0; i < npay; i++) bkeys[2 + i] = pay_cols[i];
+  int bslots[9] = {0};
+  if (int rc = scan_desc(build, build_pred, 2 + npay, bkeys, bq, bslots, &brows, &bkept)) return rc;
+  if (build->nrows >= (int64_t)INT32_MAX || inner->nrows >= (int64_t)INT32_MAX)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: join side above 2^31 rows");
+  const LookupPlan ilp = plan_lookup_col(inner, inner_key, irows);
+  float dev_ms = 0;
+  // phase 1: the nested join's output rows (key and the columns that ride
+  // with the groups) as a table of their own
+  vq_table* nested = nullptr;
+  int64_t total = 0;
+  {
+    SlotLease S(c);
+    hipSetDevice(c->device);
+    const size_t nt = (size_t)std::max<int64_t>(bq.ntiles, 1);
+    void* ar;
+    if (int rc = arena_get(c, S.get(), lookup_bytes(ilp, inner->nrows) + nt * 8 + 64 + 2048, &ar)) return rc;
+    Carve cv;
+    LookupBufs lk;
+    carve_lookup(cv, (char*)ar, ilp, inner->nrows, lk);
+    uint32_t* tcount = cv.take<uint32_t>((char*)ar, nt);
+    uint32_t* toff = cv.take<uint32_t>((char*)ar, nt);
+    uint64_t* dcount = cv.take<uint64_t>((char*)ar, 1);
+    void* hp;
+    if (int rc = pinned_get(S.get(), 64, &hp)) return rc;
+    uint64_t* hcount = (uint64_t*)hp;
+    uint32_t* hflag = (uint32_t*)((char*)hp + 16);
+    hipStream_t s = S->stream;
+    HIP_TRY(hipEventRecord(S->ev[0], s));
+    HIP_TRY(launch_lookup_build(iq, ikey, lk, s));
+    HIP_TRY(launch_semi(bq, bslots[0], lk, tcount, toff, dcount, s));
+    HIP_TRY(hipMemcpyAsync(hcount, dcount, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hflag, lk.min_head + 1, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(S->ev[1], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    dev_ms += elapsed(S->ev[0], S->ev[1]);
+    if (*hflag & 2) return fail(VQ_E_CORRUPT, "exec: gpu: an inner key outside its column's zone maps");
+    if (*hflag) return fail(VQ_E_DEVICE, "exec: gpu: inner lookup table filled");
+    total = (int64_t)*hcount;
+    if (total >= (int64_t)INT32_MAX) return fail(VQ_E_UNSUPPORTED, "exec: gpu: nested join above 2^31 rows");
+    TableParts tp;
+    tp.nrows = total;
+    const int ncol = 1 + npay;
+    SemiOut o;
+    memset(&o, 0, sizeof o);
+    o.ncols = ncol;
+    uint64_t col_bytes = 0;
+    for (int i = 0; i < ncol; i++) {
+      const ColDev& cd = build->cols[bkeys[1 + i]];
+      tp.types.push_back(cd.type);
+      tp.dict_sizes.push_back(cd.dict_size);
+      col_bytes += (((uint64_t)total * width_of(cd.type) + 255) & ~255ULL) + 256;
+      o.slot[i] = bslots[1 + i];
+      o.type[i] = cd.type;
+    }
+    if (!ctx_reserve(c, col_bytes))
+      return fail(VQ_E_BUDGET, "exec: gpu: the nested join's %lld rows exceed the device budget", (long long)total);
+    hipError_t e = hipSuccess;
+    tp.dev.assign(ncol, nullptr);
+    for (int i = 0; i < ncol && e == hipSuccess; i++) {
+      e = hipMalloc(&tp.dev[i], (((uint64_t)total * width_of(tp.types[i]) + 255) & ~255ULL) + 256);
+      o.dst[i] = tp.dev[i];
+    }
+    if (e == hipSuccess) e = hipEventRecord(S->ev[0], s);
+    if (e == hipSuccess) e = launch_semi_gather(bq, bslots[0], lk, toff, o, s);
+    if (e == hipSuccess) e = hipEventRecord(S->ev[1], s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      for (void* p : tp.dev)
+        if (p) hipFree(p);
+      ctx_release(c, col_bytes);
+      return fail(VQ_E_DEVICE, "exec: gpu: nested build: %s", hipGetErrorString(e));
+    }
+    dev_ms += elapsed(S->ev[0], S->ev[1]);
+    if (total > 0) tp.rg_rows.push_back(total);
+    tp.zones.assign(tp.rg_rows.size() * ncol, vq_zone{});  // (no bounds: nothing prunes it)
+    tp.bytes = col_bytes;
+    std::vector<void*> owned = tp.dev;
+    if (int rc = table_adopt(c, std::move(tp), &nested)) {
+      for (void* p : owned)
+        if (p) hipFree(p);
+      ctx_release(c, col_bytes);
+      return rc;
+    }
+  }
+  // phase 2: the outer join, grouped by its key, top-k
+  int rc = join_impl(c, nested, 0, nullptr, probe, probe_key, probe_pred, naggs, aggs, radix_bits, topk, out,
+                     stats, false);
+  // phase 3: the columns that ride with each output group
+  if (rc == VQ_OK && npay > 0 && out->ngroups > 0) {
+    const int64_t N = out->ngroups;
+    int64_t klo = out->keys[0], khi = out->keys[0];
+    for (int64_t i = 1; i < N; i++) {
+      klo = std::min(klo, out->keys[i]);
+      khi = std::max(khi, out->keys[i]);
+    }
+    const LookupPlan klp = plan_lookup(N, true, klo, khi);
+    SlotLease S(c);
+    hipSetDevice(c->device);
+    void* ar;
+    void* hp;
+    if ((rc = arena_get(c, S.get(), lookup_bytes(klp, N) + (size_t)N * 8 * (1 + npay) + 3 * 256, &ar)) == VQ_OK &&
+        (rc = pinned_get(S.get(), 64, &hp)) == VQ_OK) {
+      Carve cv;
+      LookupBufs lk;
+      carve_lookup(cv, (char*)ar, klp, N, lk);
+      int64_t* dkeys = cv.take<int64_t>((char*)ar, N);
+      int64_t* dpay = cv.take<int64_t>((char*)ar, (size_t)N * npay);
+      uint32_t* dflag = cv.take<uint32_t>((char*)ar, 1);
+      PayCols pc;
+      memset(&pc, 0, sizeof pc);
+      pc.npay = npay;
+      pc.key_type = nested->cols[0].type;
+      pc.key = nested->cols[0].d;
+      for (int i = 0; i < npay; i++) {
+        pc.src[i] = nested->cols[1 + i].d;
+        pc.type[i] = nested->cols[1 + i].type;
+      }
+      hipStream_t s = S->stream;
+      hipError_t e = hipMemcpyAsync(dkeys, out->keys, (size_t)N * 8, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) e = hipEventRecord(S->ev[0], s);
+      if (e == hipSuccess) e = launch_pay_gather(dkeys, N, lk, pc, total, dpay, dflag, s);
+      if (e == hipSuccess) e = hipEventRecord(S->ev[1], s);
+      if (e == hipSuccess) e = hipMemcpyAsync(pay_out, dpay, (size_t)N * npay * 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(hp, dflag, 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        rc = fail(VQ_E_DEVICE, "exec: gpu: nested join columns: %s", hipGetErrorString(e));
+      } else {
+        dev_ms += elapsed(S->ev[0], S->ev[1]);
+        if (*(uint32_t*)hp) rc = fail(VQ_E_UNSUPPORTED, "exec: gpu: rows of one join key disagree on a column grouped with it");
+      }
+    }
+    if (rc != VQ_OK) out->ngroups = 0;
+  }
+  vq_table_destroy(nested);
+  if (rc == VQ_OK && stats) {
+    stats->device_ms += dev_ms;
+    stats->rows_scanned += irows + brows;
+    stats->rg_total += inner->nrg + build->nrg;
+    stats->rg_scanned += ikept + bkept;
+    int64_t iw = 0, bw = 0;
+    for (int i = 0; i < iq.nslots; i++) iw += width_of(iq.slot[i].type);
+    for (int i = 0; i < bq.nslots; i++) bw += width_of(bq.slot[i].type);
+    stats->bytes_alg += irows * iw + brows * bw;
+    stats->build_rows = total;
+  }
+  return rc;
 }
 
 extern "C" int vq_agg_hash(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t key_col,

@@ -291,6 +291,11 @@ struct LocalBufs {
 // rows whose key is INT64_MIN are chained from *min_head.  packed: every
 // build key lies in [0, 2^32 - 1), and a slot is one word, key << 32 | head
 // (all ones: empty; head unused), so a new key takes one compare-and-swap.
+// direct: every key lies in [base, base + range) with range at most a few
+// times the rows (the zone maps say so): head[key - base] starts its chain,
+// and the table is built with plain stores (a row whose store another row's
+// overwrote chains itself in afterwards), no hashing and, for unique keys,
+// no atomics.
 struct LookupBufs {
   int64_t* keys;
   uint32_t* head;
@@ -300,6 +305,9 @@ struct LookupBufs {
                        // [2] the most rows one key has, [3] rows of key INT64_MIN
   uint32_t mask;       // slots - 1
   uint32_t packed;
+  int64_t base;        // direct: the smallest key
+  uint32_t range;      // direct: head's entries (0: a hash table)
+  uint32_t pad_;
 };
 
 struct JoinBufs {
@@ -412,6 +420,9 @@ struct JitArgs {
   const uint32_t* lk_min;
   uint32_t lk_mask;
   uint32_t lk_packed;
+  int64_t lk_base;     // direct tables (lk_range > 0): head[key - lk_base]
+  uint32_t lk_range;
+  uint32_t lk_pad;
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -464,6 +475,33 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
 hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s, int32_t k0);
 // LDS the per-workgroup path needs for `limit` keys of `nwords` words.
 size_t local_lds_for(int32_t lcap, int32_t limit, int32_t nwords);
+// The nested build (vq_join_agg_nested, k_scan.hip): the columns of the
+// nested join's output that are kept (slots of the build side's QDesc) and
+// where they go; the output groups' build-side columns.
+struct SemiOut {
+  int32_t ncols;
+  int32_t slot[8];
+  int32_t type[8];
+  void* dst[8];
+};
+struct PayCols {
+  int32_t npay;
+  int32_t key_type;
+  const void* key;
+  const void* src[8];
+  int32_t type[8];
+};
+// Per tile of d: the inner rows its passing rows meet (lookup table lk, key
+// slot fk_slot), scanned into tile_offsets and *out_count.
+hipError_t launch_semi(const QDesc& d, int32_t fk_slot, const LookupBufs& lk, uint32_t* tile_counts,
+                       uint32_t* tile_offsets, uint64_t* out_count, hipStream_t s);
+// Each passing row's copies (one per inner row it meets), in row order.
+hipError_t launch_semi_gather(const QDesc& d, int32_t fk_slot, const LookupBufs& lk,
+                              const uint32_t* tile_offsets, const SemiOut& o, hipStream_t s);
+// keys[0, nkeys) into lk (group i), then the pc columns of every row whose key
+// is one of them into pay[group][npay]; *flag set when rows of one key disagree.
+hipError_t launch_pay_gather(const int64_t* keys, int64_t nkeys, const LookupBufs& lk, const PayCols& pc,
+                             int64_t nrows, int64_t* pay, uint32_t* flag, hipStream_t s);
 // vq_join_group's build side: the lookup table of the rows of d's spans that
 // pass its predicate, keyed by slot key_slot (k_scan.hip).
 hipError_t launch_lookup_build(const QDesc& d, int32_t key_slot, const LookupBufs& lk, hipStream_t s);

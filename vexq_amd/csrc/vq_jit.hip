@@ -514,6 +514,8 @@ hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_
   a.lk_min = B.lk.min_head;
   a.lk_mask = B.lk.mask;
   a.lk_packed = B.lk.packed;
+  a.lk_base = B.lk.base;
+  a.lk_range = B.lk.range;
   a.lds_words = ll.lds / 4;
   return launch(ll.fn, (unsigned)ll.grid, ll.lds, s, a, 512);
 }

@@ -267,6 +267,41 @@ def q_q12(b: Table, p: Table, lo: int = DATE_MIN + 731, hi: int = DATE_MIN + 109
     return pp, [p.index("shipmode")], aggs, urgent
 
 
+SEGMENTS = ["AUTOMOBILE", "BUILDING", "FURNITURE", "HOUSEHOLD", "MACHINERY"]
+
+
+def q3_tables(nc: int, no: int, nl: int, seed: int = 31):
+    """TPC-H Q3's three tables: customer (custkey 1..nc, market segment codes
+    of the five segments), orders (the join build side's key, orderdate and
+    priority, plus the ordering customer and a ship priority) and a lineitem
+    probe side of its orders."""
+    c = Table([_col("custkey", abi.VQ_I64, 1 + np.arange(nc, dtype=np.int64)),
+               _col("mktsegment", abi.VQ_U8, rand_int(seed, 97, nc, 0, 4).astype(np.uint8), SEGMENTS)],
+              row_groups(nc))
+    o = orders(no, seed)
+    o = Table(list(o.columns) + [_col("custkey", abi.VQ_I64, rand_int(seed, 98, no, 1, nc)),
+                                 _col("shippriority", abi.VQ_I64, rand_int(seed, 99, no, 0, 1))],
+              o.rg_rows)
+    li = lineitem_probe(o, nl, seed)
+    return c, o, li
+
+
+def q_q3(c: Table, o: Table, li: Table, cutoff: int = DATE_MIN + 1169):
+    """TPC-H Q3 (planner/parallel_join_bench_test.go:31-41): customers in the
+    BUILDING segment, their orders dated before 1995-03-15, lineitems shipped
+    after it, GROUP BY l_orderkey (o_orderdate, o_shippriority riding with
+    it), SUM(l_extendedprice), ORDER BY revenue DESC LIMIT 10.  Returns the
+    arguments of join_agg_nested after the tables: (inner key, inner pred,
+    build fk, build key, build pred, probe key, probe pred, pay columns,
+    aggregates, top-k)."""
+    ip = abi.make_pred(c, [("mktsegment", "=", SEGMENTS.index("BUILDING"))])
+    bp = abi.make_pred(o, [("orderdate", "<", cutoff)])
+    pp = abi.make_pred(li, [("shipdate", ">", cutoff)])
+    aggs = [abi.agg_sum("price", li)]
+    return (c.index("custkey"), ip, o.index("custkey"), o.index("key"), bp, li.index("fk"), pp,
+            [o.index("orderdate"), o.index("shippriority")], aggs, 10)
+
+
 def fact_serving(dim: Table, nrows: int, seed: int = 42) -> Table:
     """The served stream's fact table: lineitem's dictionary and measure
     columns plus the join probe side's foreign key (each dimension key 1..7

@@ -72,6 +72,10 @@ def lib():
                               C.c_int32, P(abi.AggResult), P(abi.Stats)]
     L.vq_agg_hash.argtypes = [C.c_void_p, C.c_void_p, P(abi.Pred), C.c_int32, C.c_int32,
                               P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
+    L.vq_join_agg_nested.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_void_p, C.c_int32,
+                                     C.c_int32, P(abi.Pred), C.c_void_p, C.c_int32, P(abi.Pred), C.c_int32,
+                                     P(C.c_int32), C.c_int32, P(abi.Agg), C.c_int32, C.c_int32,
+                                     P(abi.AggResult), P(C.c_int64), P(abi.Stats)]
     L.vq_join_group.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_void_p,
                                 C.c_int32, P(abi.Pred), C.c_int32, P(C.c_int32), C.c_int32,
                                 P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
@@ -354,6 +358,28 @@ class Context:
                                    C.byref(out.c), C.byref(st)), out)
         out.stats = st
         return out
+
+
+    def join_agg_nested(self, inner: "DeviceTable", inner_key: int, inner_pred, build: "DeviceTable",
+                        build_fk: int, build_key: int, build_pred, probe: "DeviceTable", probe_key: int,
+                        probe_pred, pay_cols, aggs, radix_bits: int = -1, topk: int = 0,
+                        capacity: int | None = None):
+        """vq_join_agg_nested: (result, payload [ngroups, npay] int64)."""
+        import numpy as np
+        arr = aggs if isinstance(aggs, C.Array) else abi.agg_array(aggs)
+        parr = (C.c_int32 * max(1, len(pay_cols)))(*pay_cols)
+        if capacity is None:
+            capacity = topk if topk > 0 else max(1, min(build.nrows, probe.nrows))
+        out = abi.AggOut(capacity, 1, len(aggs))
+        pay = np.zeros(capacity * max(1, len(pay_cols)), dtype=np.int64)
+        st = abi.Stats()
+        _check(lib().vq_join_agg_nested(self.h, inner.h, inner_key, _pp(inner_pred), build.h, build_fk, build_key,
+                                        _pp(build_pred), probe.h, probe_key, _pp(probe_pred), len(pay_cols), parr,
+                                        len(aggs), arr, radix_bits, topk, C.byref(out.c),
+                                        pay.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(st)), out)
+        out.stats = st
+        n = out.ngroups
+        return out, pay[: n * len(pay_cols)].reshape(n, len(pay_cols))
 
 
 class Ticket:
