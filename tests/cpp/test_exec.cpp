@@ -495,6 +495,58 @@ bool JoinGroupedByProbeColumn() {
   return true;
 }
 
+// TPC-H Q3's chain through the operator: customers of one segment, their
+// orders, those orders' lineitems; GROUP BY the order key with its date and
+// ship priority riding along, top 2 by SUM(price) (expected values by hand).
+bool NestedBuildChain() {
+  static const int64_t ckey[] = {1, 2, 3};
+  static const uint8_t cseg[] = {0, 1, 0};
+  static const int64_t okey[] = {10, 20, 30, 40};
+  static const int64_t ocust[] = {1, 2, 3, 1};
+  static const int32_t odate[] = {100, 101, 102, 103};
+  static const int64_t oship[] = {0, 0, 1, 0};
+  static const int64_t lkey[] = {10, 20, 30, 40, 10, 30, 50};
+  static const double lprice[] = {1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0};
+  HostTable hc, ho, hl;
+  hc.columns = {col("c_custkey", DataType::Int64, VQ_I64, ckey),
+                col("c_mktsegment", DataType::String, VQ_U8, cseg, dict({"BUILDING", "MACHINERY"}))};
+  hc.row_groups = {3};
+  hc.nrows = 3;
+  ho.columns = {col("o_orderkey", DataType::Int64, VQ_I64, okey), col("o_custkey", DataType::Int64, VQ_I64, ocust),
+                col("o_orderdate", DataType::Date, VQ_DATE, odate),
+                col("o_shippriority", DataType::Int64, VQ_I64, oship)};
+  ho.row_groups = {4};
+  ho.nrows = 4;
+  hl.columns = {col("l_orderkey", DataType::Int64, VQ_I64, lkey),
+                col("l_extendedprice", DataType::Float64, VQ_F64, lprice)};
+  hl.row_groups = {4, 3};
+  hl.nrows = 7;
+  std::unique_ptr<GpuTable> c, o, l;
+  CHECK_OK(GpuTable::Upload(g_dev.get(), hc, &c));
+  CHECK_OK(GpuTable::Upload(g_dev.get(), ho, &o));
+  CHECK_OK(GpuTable::Upload(g_dev.get(), hl, &l));
+  const vq_pred seg = pred1(1, VQ_EQ, 0);
+  JoinOptions opts;
+  opts.inner = c.get();
+  opts.inner_key = 0;
+  opts.build_fk = 1;
+  opts.inner_pred = &seg;
+  opts.build_pay = {2, 3};
+  opts.topk = 2;
+  auto op = NewGpuParallelHashJoinAggregate(o.get(), 0, nullptr, l.get(), 0, nullptr,
+                                            {agg(AggKind::Sum, 1, DataType::Float64, "revenue")}, opts);
+  CHECK(op->schema().fields.size() == 4 && op->schema().fields[1].name == "o_orderdate");
+  std::vector<Batch> bs;
+  CHECK_OK(Drain(op.get(), &bs));
+  CHECK(bs.size() == 1 && bs[0].length == 2);
+  const Batch& r = bs[0];
+  CHECK(r.vectors[0].i64[0] == 30 && r.vectors[1].i64[0] == 102 && r.vectors[2].i64[0] == 1 &&
+        r.vectors[3].f64[0] == 9.0);
+  CHECK(r.vectors[0].i64[1] == 10 && r.vectors[1].i64[1] == 100 && r.vectors[2].i64[1] == 0 &&
+        r.vectors[3].f64[1] == 6.0);
+  return true;
+}
+
 }  // namespace
 
 int main() {
@@ -513,6 +565,7 @@ int main() {
   run("TestSharedProbeAndMultiplicities", SharedProbeAndMultiplicities);
   run("TestJoinTableSentinelAndExtremeKeys", SentinelAndExtremeKeys);
   run("TestJoinGroupedByProbeColumn", JoinGroupedByProbeColumn);
+  run("TestNestedBuildChain", NestedBuildChain);
   run("TestParallelAggregateClosedForm", ClosedFormSum);
   run("TestErrorsAreReported", ErrorsAreReported);
   g_dev.reset();

@@ -287,7 +287,7 @@ def q3_tables(nc: int, no: int, nl: int, seed: int = 31):
 
 
 def q_q3(c: Table, o: Table, li: Table, cutoff: int = DATE_MIN + 1169):
-    """TPC-H Q3 (planner/parallel_join_bench_test.go:31-41): customers in the
+    """TPC-H Q3 (planner/parallel_join_bench_test.go:32-44): customers in the
     BUILDING segment, their orders dated before 1995-03-15, lineitems shipped
     after it, GROUP BY l_orderkey (o_orderdate, o_shippriority riding with
     it), SUM(l_extendedprice), ORDER BY revenue DESC LIMIT 10.  Returns the

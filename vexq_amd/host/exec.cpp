@@ -66,8 +66,15 @@ Batch agg_batch(const Schema& schema, const vq_agg_result& r, int nkeys, int nag
     Vector v;
     v.type = key_types[k];
     for (int64_t i = lo; i < hi; i++) {
-      if (v.type == DataType::String) v.codes.push_back((uint32_t)r.keys[i * nkeys + k]);
-      else v.i64.push_back(r.keys[i * nkeys + k]);
+      if (v.type == DataType::String) {
+        v.codes.push_back((uint32_t)r.keys[i * nkeys + k]);
+      } else if (v.type == DataType::Float64) {  // (a float column riding with a key: its bits)
+        double d;
+        memcpy(&d, &r.keys[i * nkeys + k], 8);
+        v.f64.push_back(d);
+      } else {
+        v.i64.push_back(r.keys[i * nkeys + k]);
+      }
     }
     v.dict = key_dicts[k];
     b.vectors.push_back(std::move(v));
@@ -343,7 +350,11 @@ GpuParallelHashJoinAggregate::GpuParallelHashJoinAggregate(GpuTable* build, int 
       aggs_(std::move(aggs)), opts_(opts) {
   if (build_pred) bpred_ = *build_pred;
   if (probe_pred) ppred_ = *probe_pred;
-  if (opts_.group_by.empty()) {
+  if (opts_.inner) {  // the key, then the build columns that ride with it
+    const Schema bs = build->host().schema();
+    schema_.fields.push_back(bs.fields[build_key]);
+    for (int c : opts_.build_pay) schema_.fields.push_back(bs.fields[c]);
+  } else if (opts_.group_by.empty()) {
     schema_.fields.push_back(build->host().schema().fields[build_key]);
   } else {
     const Schema ps = probe->host().schema();
@@ -354,6 +365,7 @@ GpuParallelHashJoinAggregate::GpuParallelHashJoinAggregate(GpuTable* build, int 
 }
 
 Status GpuParallelHashJoinAggregate::Setup() {
+  if (opts_.inner) return SetupNested();
   if (!opts_.group_by.empty()) return SetupGrouped();
   std::vector<vq_agg> va;
   for (const AggExpr& a : aggs_) va.push_back(to_vq(probe_->host(), a));
@@ -378,6 +390,59 @@ Status GpuParallelHashJoinAggregate::Setup() {
   for (int64_t lo = 0; lo < rb.r.ngroups; lo += kBlockRows)
     out_.push_back(agg_batch(schema_, rb.r, 1, (int)aggs_.size(), kd, kt, agg_types, lo,
                              std::min<int64_t>(rb.r.ngroups, lo + kBlockRows)));
+  return Status::Ok();
+}
+
+// A nested build side (vq_join_agg_nested): the key and its riding build
+// columns, then the aggregates.
+Status GpuParallelHashJoinAggregate::SetupNested() {
+  std::vector<vq_agg> va;
+  for (const AggExpr& a : aggs_) va.push_back(to_vq(probe_->host(), a));
+  const int np = (int)opts_.build_pay.size(), na = (int)aggs_.size();
+  const int64_t cap = opts_.topk > 0 ? opts_.topk
+                                     : std::max<int64_t>(1, std::min(build_->host().nrows, probe_->host().nrows));
+  ResultBufs rb;
+  rb.alloc(cap, 1, na);
+  std::vector<int64_t> pay((size_t)cap * std::max(np, 1));
+  vq_pred ip{};
+  if (opts_.inner_pred) ip = *opts_.inner_pred;
+  vq_stats st;
+  int rc = vq_join_agg_nested(build_->device()->ctx(), opts_.inner->handle(), opts_.inner_key,
+                              opts_.inner_pred ? &ip : nullptr, build_->handle(), opts_.build_fk, build_key_,
+                              has_bpred_ ? &bpred_ : nullptr, probe_->handle(), probe_key_,
+                              has_ppred_ ? &ppred_ : nullptr, np, opts_.build_pay.data(), na, va.data(),
+                              opts_.radix_bits, opts_.topk, &rb.r, pay.data(), &st);
+  if (rc) return Status::FromLib(rc, "parallel join aggregate");
+  stats_.rows = st.build_rows;
+  stats_.keys = st.build_keys;
+  stats_.partitions = st.partitions;
+  stats_.elapsed_ms = st.device_ms;
+  // the batch: key and riding columns as group-by columns of the result
+  const HostTable& hb = build_->host();
+  std::vector<std::shared_ptr<const std::vector<std::string>>> kd = {nullptr};
+  std::vector<DataType> kt = {DataType::Int64}, agg_types;
+  for (int c : opts_.build_pay) {
+    kd.push_back(hb.columns[c].dict);
+    kt.push_back(hb.columns[c].type);
+  }
+  for (int j = 0; j < na; j++) agg_types.push_back(schema_.fields[1 + np + j].type);
+  const int64_t n = rb.r.ngroups;
+  ResultBufs kb;  // keys widened to 1 + np columns per group
+  kb.alloc(std::max<int64_t>(n, 1), 1 + np, na);
+  kb.r.ngroups = n;
+  for (int64_t g = 0; g < n; g++) {
+    kb.keys[g * (1 + np)] = rb.r.keys[g];
+    for (int c = 0; c < np; c++) kb.keys[g * (1 + np) + 1 + c] = pay[(size_t)g * np + c];
+    kb.counts[g] = rb.r.counts[g];
+    kb.first[g] = rb.r.first_row[g];
+    for (int j = 0; j < na; j++) {
+      kb.ivals[g * na + j] = rb.r.ivals[g * na + j];
+      kb.fvals[g * na + j] = rb.r.fvals[g * na + j];
+      kb.valid[g * na + j] = rb.r.valid[g * na + j];
+    }
+  }
+  for (int64_t lo = 0; lo < n; lo += kBlockRows)
+    out_.push_back(agg_batch(schema_, kb.r, 1 + np, na, kd, kt, agg_types, lo, std::min<int64_t>(n, lo + kBlockRows)));
   return Status::Ok();
 }
 

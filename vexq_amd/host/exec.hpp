@@ -174,6 +174,15 @@ struct JoinOptions {
   // non-empty: GROUP BY these probe columns over the joined rows (TPC-H Q12's
   // shape), aggregates over either side (AggExpr::build_side); topk unused
   std::vector<int> group_by;
+  // a build side that is itself a join (TPC-H Q3's chain, the nested build of
+  // planner/parallel_join.go:341-400): inner[inner_key] = build[build_fk],
+  // the inner rows filtered by inner_pred; build_pay: build columns that are
+  // functions of the key, output after it (Q3's o_orderdate, o_shippriority)
+  GpuTable* inner = nullptr;
+  int inner_key = 0;
+  int build_fk = 0;
+  const vq_pred* inner_pred = nullptr;
+  std::vector<int> build_pay;
 };
 
 // ParallelHashJoinAggregate (exec/parallel_join.go:718-758): inner join of
@@ -196,6 +205,7 @@ class GpuParallelHashJoinAggregate : public Operator {
  private:
   Status Setup();
   Status SetupGrouped();
+  Status SetupNested();
   GpuTable* build_;
   GpuTable* probe_;
   int build_key_, probe_key_;
