@@ -756,11 +756,9 @@ hipError_t launch_agg_final(const QDesc& d, int32_t nparts, const uint64_t* part
 // the one before.  Chain order is arrival order; the probe side folds every
 // match exactly (integer and fixed-point words), so it does not matter.
 __global__ __launch_bounds__(256) void k_lookup_init(const LookupBufs lk) {
-  if (lk.range) {  // a direct table
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < lk.range; i += gridDim.x * 256) {
+  if (lk.range) {  // a direct table (cnt is cleared where a key repeats: direct_pass)
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < lk.range; i += gridDim.x * 256)
       lk.head[i] = 0xFFFFFFFFu;
-      lk.cnt[i] = 0;
-    }
     if (blockIdx.x == 0 && threadIdx.x < 4) lk.min_head[threadIdx.x] = threadIdx.x == 0 ? 0xFFFFFFFFu : 0u;
     return;
   }
@@ -829,9 +827,11 @@ __device__ __forceinline__ void lookup_insert(const LookupBufs& lk, int64_t key,
 
 // A direct table's three passes over the rows it takes: (0) each row stores
 // its id at its key (one store survives); (1) a row whose store did not
-// marks itself; (2) the marked rows chain in by exchange (a repeated key)
-// and count its rows.  Keys outside [base, base + range) contradict the zone
-// maps the range came from: flagged.
+// marks itself, clears its key's count and raises min_head[3]; (2) the
+// marked rows chain in by exchange (a repeated key) and count its rows --
+// skipped by every wave when min_head[3] stayed 0 (unique keys).  Keys
+// outside [base, base + range) contradict the zone maps the range came
+// from: flagged.
 constexpr uint32_t kLostRow = 0xFFFFFFFEu;
 __device__ __forceinline__ void direct_pass(const LookupBufs& lk, int64_t key, uint32_t r, int pass) {
   const uint64_t i = (uint64_t)key - (uint64_t)lk.base;
@@ -843,7 +843,11 @@ __device__ __forceinline__ void direct_pass(const LookupBufs& lk, int64_t key, u
     lk.head[i] = r;
     lk.next[r] = 0xFFFFFFFFu;
   } else if (pass == 1) {
-    if (lk.head[i] != r) lk.next[r] = kLostRow;
+    if (lk.head[i] != r) {
+      lk.next[r] = kLostRow;
+      lk.cnt[i] = 0;
+      lk.min_head[3] = 1u;
+    }
   } else if (lk.next[r] == kLostRow) {
     lk.next[r] = atomicExch(&lk.head[i], r);
     atomicMax(&lk.min_head[2], atomicAdd(&lk.cnt[i], 1u) + 2u);
@@ -855,6 +859,7 @@ __device__ __forceinline__ void direct_pass(const LookupBufs& lk, int64_t key, u
 template <int U>
 __global__ __launch_bounds__(kWave) void k_lookup_build(const QDesc d, int32_t key_slot, const LookupBufs lk,
                                                         int32_t step) {
+  if (step == 2 && lk.min_head[3] == 0) return;  // no key repeated
   const int lane = threadIdx.x;
   const int NS = d.nslots;
   SlotLd ld[kMaxSlots];
@@ -1033,6 +1038,7 @@ __global__ __launch_bounds__(kWave) void k_semi_gather(const QDesc d, int32_t fk
 // would be separate groups of GROUP BY key, columns -- flagged).
 __global__ __launch_bounds__(256) void k_keys_insert(const int64_t* __restrict__ keys, int64_t n,
                                                      const LookupBufs lk, int32_t pass) {
+  if (lk.range && pass == 2 && lk.min_head[3] == 0) return;  // no key repeated
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     if (lk.range) direct_pass(lk, keys[i], (uint32_t)i, pass);
     else lookup_insert(lk, keys[i], (uint32_t)i);
