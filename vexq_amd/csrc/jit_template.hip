@@ -14,6 +14,7 @@
 //   vq_filter_bits pass bits + per-tile counts for the selection vector
 //   vq_part_count / vq_part_scatter   radix partitioning of a join side
 //   vq_hash_local  GROUP BY of few keys into per-workgroup partials
+//   vq_dj_probe    the direct join's probe and aggregate
 #ifndef VQ_JIT_PROLOGUE
 // Stand-alone build check (hipcc -c): the 8-aggregate GROUP BY shape.
 #define VQ_JIT_PROLOGUE 1
@@ -25,10 +26,11 @@
 #define VQ_KEYSLOT 2
 #define VQ_NV 5
 #define VQ_PROBE 1
-#define VQ_KERNELS 15
+#define VQ_KERNELS 31
 #define VQ_PB 8
 #define VQ_BLOOM 1
 #define VQ_LIVEBITS 1
+#define VQ_DJ_SLOTS 6
 #define VQ_NKEY 1
 #define VQ_LOOKUP 0
 #define VQ_JKEY 0
@@ -163,6 +165,21 @@ struct Args {
   i64 lk_base;        // lk_range > 0: a direct table, head[key - lk_base]
   u32 lk_range;
   u32 lk_pad;
+  // the direct join's probe (vq_dj_probe, vq_dj_match): build keys in
+  // [dj_base, dj_base + dj_range); bit o of dj_pres: a passing build row has
+  // key dj_base + o; dj_rec[o][8]: its matches so far, 1 + its latest match
+  // past kDjSlots, the entries of its first kDjSlots matches
+  const u32* dj_pres;
+  u32* dj_rec;
+  u32* dj_mrow;       // the matched probe rows of each workgroup, at its region
+  u32* dj_mcount;     // [grid] their count
+  u64* dj_ent;        // [e][dj_ew]: (1 + previous further match) << 32 | row, values
+  u32* dj_tlist;      // the keys each workgroup met first, at its region
+  u32* dj_tcount;     // [grid] their count
+  i64 dj_base;
+  u64 dj_range;
+  i32 dj_ew;
+  i32 dj_pad;
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -580,7 +597,8 @@ __device__ __forceinline__ void fold_ldss(u64* w, const Args& a, const i64* row,
 using namespace vqj;
 
 // VQ_KERNELS selects what a module holds: 1 dense aggregate, 2 filter bits,
-// 4 join partitioning.
+// 4 join partitioning, 8 GROUP BY through per-workgroup partials, 16 the
+// direct join's probe.
 #if VQ_KERNELS & 1
 // ---------------------------------------------------------------------------
 // Dense GROUP BY (see k_agg_dense in k_scan.hip for the contract).
@@ -712,7 +730,7 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
 }
 #endif
 
-#if VQ_KERNELS & 12
+#if VQ_KERNELS & 28
 // ---------------------------------------------------------------------------
 // Radix partitioning of one join side (exec/parallel_join.go:322-347,422-550):
 // rows that pass the side's predicate go to partition radixHash(key) & (P-1),
@@ -778,7 +796,8 @@ __device__ constexpr bool live_slot() {
     if (kTS[t] == C || (kTO[t] >= 16 && kTS2[t] == C)) return true;  // (either side of a compare)
   return false;
 }
-constexpr bool kLateVals = (VQ_BLOOM != 0) && (VQ_PROBE != 0);
+constexpr int kDjSlots = VQ_DJ_SLOTS;  // the direct join's slots per key (the host's kDjSlots)
+constexpr bool kLateVals = ((VQ_BLOOM != 0) || (VQ_KERNELS & 16) != 0) && (VQ_PROBE != 0);
 
 // The row's key: its key column, or (a GROUP BY of several dictionary-code
 // columns) their codes packed as the int-key path packs them
@@ -1495,5 +1514,126 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
     }
     out[x] = v;
   }
+}
+#endif
+
+#if VQ_KERNELS & 16
+// ---------------------------------------------------------------------------
+// Probe of the direct join (HashJoin's probe feeding ParallelHashJoinAggregate's
+// partial aggregate, exec/join.go:84-155, exec/parallel_join.go:780-988), for
+// build keys whose zone maps bound them to a dense range: the build side is
+// one bit per key of the range (k_dj_mark, k_dj_pack), so a probe row's
+// match is a test of one bit.  Two kernels over the same persistent grid of
+// even tile shares:
+//   vq_dj_probe  (1024 threads) streams the key and predicate columns (the
+//                next chunk's in flight), tests the predicate and the bit,
+//                and lists the matching rows in its region (tiles t0.. of
+//                tile_rows rows: at most one item per row);
+//   vq_dj_match  (256 threads) takes workgroup w's listed rows: loads their
+//                columns, evaluates the aggregate arguments, and makes each
+//                match entry e (its position in the region) -- probe row,
+//                argument values -- one of its key's slots by one atomic add
+//                on the key's count (past kDjSlots: chained by an exchange of
+//                the key's overflow head); the key's first match lists the
+//                key in the workgroup's list, so every met key is listed once.
+// Split so the probe's waves never wait on a match's loads or atomic: most
+// probe rows of a selective join do not match, and the matches, a few per
+// cent of the rows, are then handled all at once.  Global atomics are slow
+// next to stores (they are serviced past the caches), so one per match is
+// all this costs; the groups are summed exactly, entry by entry, when they are
+// collected (k_dj_collect).
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(1024, 8) void vq_dj_probe(const Args a) {
+  __shared__ u32 s_nm;
+  constexpr int U = VQ_U;
+  const i64 t0 = (i64)blockIdx.x * a.ntiles / gridDim.x;
+  const i64 t1 = (i64)(blockIdx.x + 1) * a.ntiles / gridDim.x;
+  const i64 region = t0 * (i64)a.tile_rows;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) s_nm = 0;
+  __syncthreads();
+  Chunks<U> ck{t0 - 1, t1, 0, 0};
+  ck.base = -(i64)kPB * U;  // next() steps onto tile t0
+  bool have = t0 < t1 && ck.next(a);
+  i64 raw[VQ_NS][U];
+  if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
+  while (have) {
+    const i64 base = ck.base, rend = ck.rend;
+    i64 cur[VQ_NS][U];
+#pragma unroll
+    for (int c = 0; c < VQ_NS; c++)
+#pragma unroll
+      for (int u = 0; u < U; u++) cur[c][u] = raw[c][u];
+    have = ck.next(a);
+    if (have) pload_rows<U>(a, ck.base, ck.rend, raw);
+    u64 off[U];
+    bool live[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      i64 row[VQ_NS];
+#pragma unroll
+      for (int c = 0; c < VQ_NS; c++) row[c] = cur[c][u];
+      const i64 r = base + u * kPB + threadIdx.x;
+      off[u] = (u64)row_key(row) - (u64)a.dj_base;
+      live[u] = r < rend && off[u] < a.dj_range && pass_all(a, row, seq<VQ_NT>{});
+    }
+    u32 pw[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) pw[u] = live[u] ? a.dj_pres[off[u] >> 5] : 0u;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool hit = (pw[u] >> (u32)(off[u] & 31)) & 1u;
+      const u64 m = __ballot(hit);
+      if (m == 0) continue;
+      const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      u32 at = 0;
+      if (lane == leader) at = atomicAdd(&s_nm, (u32)__popcll(m));
+      at = (u32)__builtin_amdgcn_readlane((int)at, leader) + below;
+      if (hit) a.dj_mrow[region + at] = (u32)(base + u * kPB + threadIdx.x);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.dj_mcount[blockIdx.x] = s_nm;
+}
+
+namespace vqj {
+template <int... C>
+__device__ __forceinline__ void dj_load(const Args& a, i64 r, i64* row, Seq<C...>) {
+  ((row[C] = ld<kST[C]>(a.col[C], r)), ...);
+}
+}  // namespace vqj
+
+extern "C" __global__ __launch_bounds__(256) void vq_dj_match(const Args a) {
+  __shared__ u32 s_nt;
+  const i64 t0 = (i64)blockIdx.x * a.ntiles / gridDim.x;
+  const i64 region = t0 * (i64)a.tile_rows;
+  const int ew = a.dj_ew;
+  if (threadIdx.x == 0) s_nt = 0;
+  __syncthreads();
+  const u32 n = a.dj_mcount[blockIdx.x];
+  for (u32 i = threadIdx.x; i < n; i += 256) {
+    const u32 e = (u32)region + i;
+    const u32 r = a.dj_mrow[e];
+    i64 row[VQ_NS];
+    dj_load(a, (i64)r, row, seq<VQ_NS>{});
+    const u64 off = (u64)row_key(row) - (u64)a.dj_base;
+    u32* rec = a.dj_rec + off * 8;
+    const u32 s = __hip_atomic_fetch_add(&rec[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    u64* ent = a.dj_ent + (u64)e * ew;
+#if VQ_NV > 0
+    u64 v[VQ_NV];
+    values_of(a, row, v, seq<VQ_NV>{});
+#pragma unroll
+    for (int j = 0; j < VQ_NV; j++) ent[1 + j] = v[j];
+#endif
+    u32 prev = 0;
+    if (s < (u32)kDjSlots) rec[2 + s] = e;
+    else prev = __hip_atomic_exchange(&rec[1], e + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ent[0] = (u64)prev << 32 | (u64)r;
+    if (s == 0) a.dj_tlist[region + atomicAdd(&s_nt, 1u)] = (u32)off;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.dj_tcount[blockIdx.x] = s_nt;
 }
 #endif

@@ -532,6 +532,74 @@ __device__ uint32_t prune_topk(KItem* a, uint32_t n, int k, uint32_t* s_cnt, KIt
 constexpr int kPoolPerThread = 2;  // pool <= kPoolPerThread * kJoinThreads
 constexpr int kRankMax = 512;      // top-k by counting ranks up to this many groups
 
+// The merge of P lists of k candidates (citems[p*k + r], rank order, and
+// their records cand), by one workgroup of T threads: the best k records
+// into out and their count into flags[3].  The lists were
+// written by other workgroups of the same kernel (read at the device's
+// coherence point).  The bound: each lane of wave 0 takes the best of its
+// share of the lists' first items, and the k-th best of those 64 is at or
+// ahead of the k-th best of all (k distinct items are at or ahead of it);
+// the items at or ahead of it are ranked by counting.  When more survive
+// than the ranking room holds, wave 0 selects over all items by rounds of
+// wave maxima.  ritems: kRankMax items of LDS; s_bound, s_nkeys: LDS.
+// (Plain pointers: a reference to the kernel's argument structs would make
+// the compiler copy them into scratch memory.)
+template <int T, int kIn = 4>
+__device__ __forceinline__ void topk_merge(const KItem* citems, const uint64_t* cand, uint64_t* out, int k,
+                                           int RW, uint32_t P, KItem* ritems, KItem* s_bound,
+                                           uint32_t* s_nkeys, uint32_t* flags) {
+  const uint32_t n = P * (uint32_t)k;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) *s_nkeys = 0;
+  // (kIn loads per lane in flight at a time: with thousands of partitions
+  // the lists hold tens of thousands of items)
+  if (wv == 0) {
+    KItem b = knone();
+    for (uint32_t l0 = lane; l0 < P; l0 += 64 * kIn) {
+      KItem xs[kIn];
+#pragma unroll
+      for (int u = 0; u < kIn; u++) {
+        const uint32_t l = l0 + u * 64;
+        xs[u] = l < P ? ld_item(&citems[(size_t)l * k]) : knone();
+      }
+#pragma unroll
+      for (int u = 0; u < kIn; u++) b = kmax(b, xs[u]);
+    }
+    wave_kth(b, k, s_bound);
+  }
+  __syncthreads();
+  const KItem t = *s_bound;
+  for (uint32_t i0 = 0; i0 < n; i0 += T * kIn) {
+    KItem xs[kIn];
+#pragma unroll
+    for (int u = 0; u < kIn; u++) {
+      const uint32_t i = i0 + u * T + threadIdx.x;
+      xs[u] = i < n ? ld_item(&citems[i]) : knone();
+    }
+#pragma unroll
+    for (int u = 0; u < kIn; u++) {
+      const bool keep = xs[u].idx != kNone && (t.idx == kNone || !kbetter(t, xs[u]));
+      const uint32_t at = wave_append(keep, s_nkeys);
+      if (keep && at < (uint32_t)kRankMax) ritems[at] = xs[u];
+    }
+  }
+  __syncthreads();
+  const uint32_t m = *s_nkeys;
+  if (m <= (uint32_t)kRankMax) {
+    for (uint32_t a = threadIdx.x; a < m; a += T) {
+      const KItem x = ritems[a];
+      const uint32_t r = rank_in(ritems, m, x);
+      if (r >= (uint32_t)k) continue;
+      for (int q = 0; q < RW; q++) out[(size_t)r * RW + q] = ld_dev(&cand[(size_t)x.idx * RW + q]);
+    }
+  } else if (wv == 0) {
+    const KItem x = wave_topk_dev(citems, n, k);
+    if (lane < k && x.idx != kNone)
+      for (int q = 0; q < RW; q++) out[(size_t)lane * RW + q] = ld_dev(&cand[(size_t)x.idx * RW + q]);
+  }
+  if (threadIdx.x == 0) flags[3] = m < (uint32_t)k ? m : (uint32_t)k;
+}
+
 // Eight waves per SIMD (at most 64 VGPRs): two workgroups per CU when the
 // tables leave room in LDS.
 __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, const JoinPlan plan,
@@ -957,56 +1025,325 @@ __global__ __launch_bounds__(kJoinThreads, 8) void k_join_agg(const JDesc jd, co
     __syncthreads();
     if (!s_last) return;
   }
-  const uint32_t P = gridDim.x, n = P * (uint32_t)k;
-  if (threadIdx.x == 0) s_nkeys = 0;
-  // (four loads per lane in flight at a time: with thousands of partitions
-  // the lists hold tens of thousands of items)
-  constexpr int kIn = 4;
-  if (wv == 0) {
-    KItem b = knone();
-    for (uint32_t l0 = lane; l0 < P; l0 += 64 * kIn) {
-      KItem xs[kIn];
-#pragma unroll
-      for (int u = 0; u < kIn; u++) {
-        const uint32_t l = l0 + u * 64;
-        xs[u] = l < P ? ld_item(&B.citems[(size_t)l * k]) : knone();
+  topk_merge<kJoinThreads>(B.citems, B.cand, B.out, k, RW, gridDim.x, ritems, &s_bound, &s_nkeys, flags);
+}
+
+// ---- the direct join's groups ----------------------------------------------
+// (the direct join, DirectBufs: k_dj_mark/k_dj_pack set the passing build
+// keys' bits, vq_dj_probe gives each match a slot of its key and lists each
+// met key once, in the list of the probe workgroup that met it first.)
+// k_dj_collect, workgroup w of the probe's grid: the keys of list w, one per
+// thread at a time.  A thread reads its key's matches (the entries of its
+// slots, loaded together, then any overflow chain) and sums them exactly in
+// its own LDS words (count, first probe row, fixed-point limbs,
+// integer sums, order-mapped MIN/MAX: the order of the chain changes no bit),
+// then
+//   all groups (plan.topk 0): writes the group's record (key, count, first
+//     row, the aggregates' words, as k_join_agg writes them) at the list's
+//     place in the output (the lists before it, summed: no atomics);
+//   top-k: offers it to the workgroup's running best k (value of aggregate 0
+//     DESC, first row ASC), ranked by counting, whose records stay in LDS;
+//     the best k become the workgroup's candidates and the last workgroup
+//     to finish merges every workgroup's (topk_merge).
+// Then the workgroup clears the counts of its keys and its share of the bits
+// (the region is zero for the next query) and the last workgroup reports
+// the flags into B.flags (bit 2 of flags[0] when a passing build key repeats:
+// the multiplicities are not counted here, the host runs such a join through
+// the partitions) and resets the region's.  Dynamic LDS: the threads' words,
+// kDjThreads * (nwords + 1).
+constexpr int kDjThreads = 256;
+constexpr int kDjRec = 3 + VQ_MAX_AGGS;  // the longest record
+
+// Folds match entry e (its probe row, its values: the first given, v0) into
+// the first row and the accumulator words w (k_join_agg's encoding).
+__device__ __forceinline__ void dj_fold(const JDesc& jd, const DirectBufs& D, uint32_t e, uint32_t row,
+                                        uint64_t v0, unsigned long long* w, uint32_t& first) {
+  first = row < first ? row : first;
+  for (int j = 0; j < jd.naggs; j++) {
+    const JAgg& a = jd.agg[j];
+    if (a.vidx < 0) continue;
+    const uint64_t v = a.vidx == 0 ? v0 : D.ent[(uint64_t)e * D.ew + 1 + a.vidx];
+    unsigned long long* x = w + a.word0;
+    switch (a.kind) {
+      case VQ_AGG_SUM:
+      case VQ_AGG_AVG:
+        if (a.e.is_float) {
+          int64_t c0, c1, c2;
+          if (fix_chunks(bits_to_f64((int64_t)v), a.fs, c0, c1, c2)) {
+            x[0] += (unsigned long long)c0;
+            x[1] += (unsigned long long)c1;
+            x[2] += (unsigned long long)c2;
+          }
+        } else {
+          x[0] += (unsigned long long)v;
+        }
+        break;
+      case VQ_AGG_MIN:
+      case VQ_AGG_MAX: {
+        const double dv = bits_to_f64((int64_t)v);
+        if (a.accum == VQ_F64 && dv != dv) break;  // NaN never wins a compare
+        const int64_t ov = a.accum == VQ_F64 ? f64_ord(dv) : (int64_t)v;
+        if (a.kind == VQ_AGG_MIN ? ov < (int64_t)x[0] : ov > (int64_t)x[0]) x[0] = (unsigned long long)ov;
+        break;
       }
-#pragma unroll
-      for (int u = 0; u < kIn; u++) b = kmax(b, xs[u]);
+      default:
+        break;
     }
-    wave_kth(b, k, &s_bound);
   }
-  __syncthreads();
-  const KItem t = s_bound;
-  for (uint32_t i0 = 0; i0 < n; i0 += kJoinThreads * kIn) {
-    KItem xs[kIn];
+}
+
+// A key's group: first row and the accumulator words in w[0, NW)
+// (k_join_agg's encoding) from its c matches: the entries of its slots (sl,
+// read with its record), each entry's row and first value in one load, then,
+// past kDjSlots matches, its overflow chain from ovh.
+__device__ __forceinline__ void dj_walk(const JDesc& jd, const DirectBufs& D, const uint8_t* wop, uint32_t c,
+                                        uint32_t ovh, const uint32_t (&sl)[kDjSlots], unsigned long long* w,
+                                        uint32_t& first) {
+  const int NW = jd.nwords;
+  for (int i = 0; i < NW; i++)
+    w[i] = wop[i] == 1 ? (unsigned long long)INT64_MAX : wop[i] == 2 ? (unsigned long long)INT64_MIN : 0ull;
+  first = 0xFFFFFFFFu;
+  const uint32_t ns = c < (uint32_t)kDjSlots ? c : (uint32_t)kDjSlots;
+  const bool two = D.ew == 2;  // entries of 16 bytes: row and value in one load
+  uint64_t lk[kDjSlots], va[kDjSlots];
 #pragma unroll
-    for (int u = 0; u < kIn; u++) {
-      const uint32_t i = i0 + u * kJoinThreads + threadIdx.x;
-      xs[u] = i < n ? ld_item(&B.citems[i]) : knone();
+  for (int q = 0; q < kDjSlots; q++) {
+    lk[q] = 0;
+    va[q] = 0;
+    if ((uint32_t)q >= ns) continue;
+    const uint64_t* ep = D.ent + (uint64_t)sl[q] * D.ew;
+    if (two) {
+      const ulonglong2 x = *(const ulonglong2*)ep;
+      lk[q] = x.x;
+      va[q] = x.y;
+    } else {
+      lk[q] = ep[0];
+      va[q] = D.nvals > 0 ? ep[1] : 0ull;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kDjSlots; q++)
+    if ((uint32_t)q < ns) dj_fold(jd, D, sl[q], (uint32_t)lk[q], va[q], w, first);
+  for (uint32_t e1 = c > (uint32_t)kDjSlots ? ovh : 0u; e1 != 0;) {
+    const uint32_t e = e1 - 1;
+    const uint64_t* ep = D.ent + (uint64_t)e * D.ew;
+    const uint64_t link = ep[0];
+    e1 = (uint32_t)(link >> 32);
+    dj_fold(jd, D, e, (uint32_t)link, D.nvals > 0 ? ep[1] : 0ull, w, first);
+  }
+}
+
+// The sum of a[0, n) by the workgroup (kDjThreads threads) into *acc (zero),
+// eight loads per thread in flight at a time.
+__device__ __forceinline__ uint32_t dj_sum(const uint32_t* a, int64_t n, uint32_t* acc) {
+  uint32_t x = 0;
+  for (int64_t i0 = threadIdx.x; i0 < n; i0 += 8 * kDjThreads) {
+    uint32_t y[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int64_t i = i0 + u * kDjThreads;
+      y[u] = i < n ? a[i] : 0u;
     }
 #pragma unroll
-    for (int u = 0; u < kIn; u++) {
-      const bool keep = xs[u].idx != kNone && (t.idx == kNone || !kbetter(t, xs[u]));
+    for (int u = 0; u < 8; u++) x += y[u];
+  }
+  for (int o = 32; o > 0; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+  if ((threadIdx.x & 63) == 0 && x) atomicAdd(acc, x);
+  __syncthreads();
+  return *acc;
+}
+
+__global__ __launch_bounds__(kDjThreads, 4) void k_dj_collect(const JDesc jd, const JoinPlan plan,
+                                                              const JoinBufs B) {
+  extern __shared__ unsigned long long s_w[];
+  __shared__ uint32_t s_last, s_nbest, s_nkeys, s_sum[3];
+  __shared__ KItem s_bound;
+  __shared__ KItem s_best[kMaxK];
+  __shared__ KItem s_cand[kDjThreads + kMaxK];
+  __shared__ KItem ritems[kRankMax];
+  __shared__ uint32_t s_key[kDjThreads], s_cnt[kDjThreads], s_first[kDjThreads];
+  __shared__ unsigned long long s_brec[2][kMaxK * kDjRec];  // the best's records (double-buffered)
+  __shared__ uint8_t s_wop[3 * VQ_MAX_AGGS];
+  const int k = plan.topk;
+  const int NW = jd.nwords, RW = 3 + jd.naggs, WS = NW + 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t base = B.d.base;
+  const uint32_t* tlist = B.d.tlist;
+  const int64_t region = (int64_t)blockIdx.x * B.d.ntiles / gridDim.x * B.d.tile_rows;
+  unsigned long long* w = s_w + (size_t)threadIdx.x * WS;
+  if (threadIdx.x == 0) {
+    s_nbest = 0;
+    s_sum[0] = s_sum[1] = s_sum[2] = 0;
+  }
+  if ((int)threadIdx.x < NW) {
+    uint8_t op = 0;
+    for (int j = 0; j < jd.naggs; j++) {
+      const JAgg& a = jd.agg[j];
+      if ((int)threadIdx.x >= a.word0 && (int)threadIdx.x < a.word0 + a.nwords)
+        op = a.kind == VQ_AGG_MIN ? 1 : a.kind == VQ_AGG_MAX ? 2 : 0;
+    }
+    s_wop[threadIdx.x] = op;
+  }
+  // (the list's first keys loaded with its length: the region is in bounds)
+  const uint32_t n = B.d.tcount[blockIdx.x];
+  const uint32_t o_first = tlist[region + threadIdx.x];
+  __syncthreads();
+  // workgroup 0 sums the counters while the others collect: the groups, the
+  // passing build rows and the distinct passing keys, for the last workgroup
+  if (blockIdx.x == 0) {
+    const uint32_t g = dj_sum(B.d.tcount, gridDim.x, &s_sum[0]);
+    const uint32_t b = dj_sum(B.d.bcnt, B.d.nbcnt, &s_sum[1]);
+    const uint32_t d = dj_sum(B.d.pcnt, B.d.npcnt, &s_sum[2]);
+    if (threadIdx.x == 0) {
+      st_dev((uint64_t*)&B.d.rflags[4], (uint64_t)g);
+      st_dev((uint64_t*)&B.d.rflags[6], (uint64_t)b | (uint64_t)d << 32);
+    }
+    if (threadIdx.x == 0) s_sum[0] = 0;
+  }
+  // all groups: this list's records follow the lists before it
+  const uint32_t out0 = k == 0 ? dj_sum(B.d.tcount, blockIdx.x, &s_sum[0]) : 0;
+  int cur = 0;
+  for (uint32_t i0 = 0; i0 < n; i0 += kDjThreads) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool mine = i < n;
+    uint32_t c = 0, first = 0, o = 0;
+    if (mine) {
+      o = i0 == 0 ? o_first : tlist[region + i];
+      // the key's record (count, overflow head, slots) in one load (slots
+      // past the count are not read as entries)
+      uint4* rp = (uint4*)(B.d.rec + (uint64_t)o * 8);
+      const uint4 r0 = rp[0], r1 = rp[1];
+      c = r0.x;
+      const uint32_t sl[kDjSlots] = {r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+      dj_walk(jd, B.d, s_wop, c, r0.y, sl, w, first);
+      // this key is done: its count and overflow head cleared now
+      *(uint64_t*)rp = 0;
+    }
+    if (k == 0) {
+      if (mine) emit_record(jd, B.recs + (size_t)(out0 + i) * RW, base + (int64_t)o, c, 1, first, w);
+      continue;
+    }
+    // top-k: this round's items (ritems[t], t a thread) and the running best
+    // (ritems[kDjThreads + r]); the k-th best of the first 64 bounds the best
+    // k from below (k valid items are at or ahead of it), the items at or
+    // ahead of it are ranked by counting.  idx names the item's record: a
+    // thread of this round (< kDjThreads), or kDjThreads + its rank in the
+    // running best.
+    KItem x = knone();
+    if (mine) {
+      x.v = jd.naggs > 0 ? (int64_t)agg_word_of(jd.agg[0], c, 1, w) : 0;
+      x.first = first;
+      x.idx = threadIdx.x;
+      s_key[threadIdx.x] = o;
+      s_cnt[threadIdx.x] = c;
+      s_first[threadIdx.x] = first;
+    }
+    const uint32_t nb = s_nbest;
+    ritems[threadIdx.x] = x;
+    if (threadIdx.x < nb) {
+      KItem y = s_best[threadIdx.x];
+      y.idx = kDjThreads + threadIdx.x;
+      ritems[kDjThreads + threadIdx.x] = y;
+    }
+    if (threadIdx.x == 0) s_nkeys = 0;
+    __syncthreads();
+    if (wv == 0) wave_kth(ritems[lane], k, &s_bound);
+    __syncthreads();
+    const KItem t = s_bound;
+    KItem* sv = s_cand;  // the survivors (at most kDjThreads + kMaxK)
+    {
+      const bool keep = x.idx != kNone && (t.idx == kNone || !kbetter(t, x));
       const uint32_t at = wave_append(keep, &s_nkeys);
-      if (keep && at < (uint32_t)kRankMax) ritems[at] = xs[u];
+      if (keep) sv[at] = x;
+      KItem y = threadIdx.x < nb ? ritems[kDjThreads + threadIdx.x] : knone();
+      const bool keepb = y.idx != kNone && (t.idx == kNone || !kbetter(t, y));
+      const uint32_t atb = wave_append(keepb, &s_nkeys);
+      if (keepb) sv[atb] = y;
     }
+    __syncthreads();
+    const uint32_t m = s_nkeys;
+    const unsigned long long* orec = s_brec[cur];
+    unsigned long long* nrec = s_brec[cur ^ 1];
+    for (uint32_t a = threadIdx.x; a < m; a += kDjThreads) {
+      const KItem y = sv[a];
+      const uint32_t r = rank_in(sv, m, y);
+      if (r >= (uint32_t)k) continue;
+      s_best[r] = y;
+      unsigned long long* d = nrec + r * kDjRec;
+      if (y.idx >= (uint32_t)kDjThreads) {
+        for (int q = 0; q < RW; q++) d[q] = orec[(y.idx - kDjThreads) * kDjRec + q];
+      } else {
+        const uint32_t tt = y.idx;
+        d[0] = (unsigned long long)(base + (int64_t)s_key[tt]);
+        d[1] = s_cnt[tt];
+        d[2] = s_first[tt];
+        for (int j = 0; j < jd.naggs; j++) d[3 + j] = agg_word_of(jd.agg[j], s_cnt[tt], 1, s_w + (size_t)tt * WS);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_nbest = m < (uint32_t)k ? m : (uint32_t)k;
+    cur ^= 1;
+    __syncthreads();
+  }
+  if (k > 0 && threadIdx.x < (uint32_t)k) {
+    // this workgroup's candidates: its best k and their records
+    const uint32_t cc = blockIdx.x * (uint32_t)k + threadIdx.x;
+    KItem ci = knone();
+    if (threadIdx.x < s_nbest) {
+      for (int q = 0; q < RW; q++) st_dev(&B.cand[(size_t)cc * RW + q], s_brec[cur][threadIdx.x * kDjRec + q]);
+      ci = s_best[threadIdx.x];
+      ci.idx = cc;
+    }
+    st_item(&B.citems[cc], ci);
+  }
+  // this workgroup's share of the bits, cleared (the region is zero for the
+  // next query)
+  {
+    const uint64_t w0 = (uint64_t)blockIdx.x * B.d.words / gridDim.x;
+    const uint64_t w1 = (uint64_t)(blockIdx.x + 1) * B.d.words / gridDim.x;
+    for (uint64_t q = w0 + threadIdx.x; q < w1; q += kDjThreads) B.d.pres[q] = 0;
+  }
+  // the last workgroup: the merge, the flags
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&B.d.rflags[2], 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  if (k > 0)
+    topk_merge<kDjThreads, 12>(B.citems, B.cand, B.out, k, RW, gridDim.x, ritems, &s_bound, &s_nkeys,
+                               B.flags);
+  if (threadIdx.x == 0) {
+    const uint32_t groups = (uint32_t)ld_dev((const uint64_t*)&B.d.rflags[4]);
+    const uint64_t bd = ld_dev((const uint64_t*)&B.d.rflags[6]);
+    const uint32_t built = (uint32_t)bd, distinct = (uint32_t)(bd >> 32);
+    B.flags[0] = __hip_atomic_load(&B.d.rflags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+                 (distinct != built ? 2u : 0u);
+    B.flags[1] = groups;
+    B.flags[2] = distinct;
+    if (k == 0) B.flags[3] = groups;
+    B.flags[4] = 0;
+    B.flags[5] = 0;
+    B.flags[6] = built;
+    B.flags[7] = 0;
   }
   __syncthreads();
-  const uint32_t m = s_nkeys;
-  if (m <= (uint32_t)kRankMax) {
-    for (uint32_t a = threadIdx.x; a < m; a += kJoinThreads) {
-      const KItem x = ritems[a];
-      const uint32_t r = rank_in(ritems, m, x);
-      if (r >= (uint32_t)k) continue;
-      for (int q = 0; q < RW; q++) B.out[(size_t)r * RW + q] = ld_dev(&B.cand[(size_t)x.idx * RW + q]);
-    }
-  } else if (wv == 0) {
-    const KItem x = wave_topk_dev(B.citems, n, k);
-    if (lane < k && x.idx != kNone)
-      for (int q = 0; q < RW; q++) B.out[(size_t)lane * RW + q] = ld_dev(&B.cand[(size_t)x.idx * RW + q]);
-  }
-  if (threadIdx.x == 0) flags[3] = m < (uint32_t)k ? m : (uint32_t)k;
+  if (threadIdx.x < 16) B.d.rflags[threadIdx.x] = 0;
+}
+
+hipError_t launch_direct(const QDesc& bq, int32_t bkey_slot, const DirectLaunch& dl, const JDesc& jd,
+                         const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev) {
+  if (ev) hipEventRecord(ev[0], s);
+  hipError_t e = launch_dj_build(bq, bkey_slot, B.d, s);
+  if (e != hipSuccess) return e;
+  if (ev) hipEventRecord(ev[1], s);
+  e = jit_direct_launch(dl, B.d, s);
+  if (e != hipSuccess) return e;
+  if (ev) hipEventRecord(ev[2], s);
+  // (one workgroup per probe workgroup: list w is workgroup w's)
+  const size_t lds = (size_t)kDjThreads * (jd.nwords + 1) * 8;
+  hipLaunchKernelGGL(k_dj_collect, dim3((unsigned)B.d.grid), dim3(kDjThreads), lds, s, jd, plan, B);
+  if (ev) hipEventRecord(ev[3], s);
+  return hipGetLastError();
 }
 
 size_t join_lds_for(int32_t cap, int32_t pool, int32_t nwords, int32_t reps, bool topk) {

@@ -897,6 +897,103 @@ hipError_t launch_lookup_build(const QDesc& d, int32_t key_slot, const LookupBuf
   return hipGetLastError();
 }
 
+// ---- the direct join's build side ------------------------------------------
+// Build keys bounded by their zone maps to [base, base + range).  k_dj_mark:
+// every row that passes the build predicate stores a byte at key - base (no
+// atomics: stores of one value to one byte agree) and writes its tile's
+// passing rows to bcnt.  k_dj_pack: thread w packs bytes [32w, 32w + 32) into
+// bit word w and clears them for the next query; a block writes its bits'
+// count (distinct keys: fewer than the passing rows means a key repeats) to
+// pcnt.
+template <int U>
+__global__ __launch_bounds__(kWave) void k_dj_mark(const QDesc d, int32_t key_slot, const DirectBufs D) {
+  const int lane = threadIdx.x;
+  const int NS = d.nslots;
+  SlotLd ld[kMaxSlots];
+#pragma unroll
+  for (int c = 0; c < kMaxSlots; c++) ld[c] = slot_ld(d.slot[c < NS ? c : 0]);
+  TileIt it;
+  tile_begin(d, blockIdx.x, it);
+  const int64_t t0 = it.t;
+  uint32_t npass = 0, err = 0;
+  while (it.have && it.t == t0) {
+    int64_t raw[kMaxSlots][U];
+    load_rows<U>(ld, NS, it.base, it.rend, lane, raw);
+    bool pass[U];
+    int g[U];
+    eval_pred<U>(d.slot, NS, it.base, it.rend, lane, raw, pass, g);
+    int64_t key[U];
+    pick<U>(raw, key_slot, key);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!pass[u]) continue;
+      const uint64_t off = (uint64_t)key[u] - (uint64_t)D.base;
+      if (off >= D.range) {
+        err = 1u;
+        continue;
+      }
+      npass++;
+      D.mark[off] = 1;
+    }
+    it.base += kWave * U;
+    if (it.base >= it.rend) it.have = false;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    npass += (uint32_t)__shfl_xor((int)npass, o, 64);
+    err |= (uint32_t)__shfl_xor((int)err, o, 64);
+  }
+  if (lane == 0) {
+    D.bcnt[blockIdx.x] = npass;
+    if (err) atomicOr(&D.rflags[0], err);  // (corrupt zone maps only)
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dj_pack(const DirectBufs D) {
+  __shared__ uint32_t s_n;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  uint32_t bits = 0;
+  if (w < D.words) {
+    uint4* m = (uint4*)(D.mark + w * 32);
+    const uint4 a = m[0], b = m[1];
+    const uint32_t x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint32_t v = x[i];
+      bits |= ((v & 1u) | ((v >> 7) & 2u) | ((v >> 14) & 4u) | ((v >> 21) & 8u)) << (4 * i);
+    }
+    D.pres[w] = bits;
+    if (bits) {
+      m[0] = make_uint4(0, 0, 0, 0);
+      m[1] = make_uint4(0, 0, 0, 0);
+    }
+  }
+  uint32_t n = (uint32_t)__popc(bits);
+  for (int o = 32; o > 0; o >>= 1) n += (uint32_t)__shfl_xor((int)n, o, 64);
+  if ((threadIdx.x & 63) == 0 && n) atomicAdd(&s_n, n);
+  __syncthreads();
+  if (threadIdx.x == 0) D.pcnt[blockIdx.x] = s_n;
+}
+
+__global__ __launch_bounds__(256) void k_zero_words(uint64_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = 0;
+}
+
+hipError_t launch_zero_words(uint64_t* p, uint64_t words, hipStream_t s) {
+  const uint64_t blocks = std::min<uint64_t>(4096, std::max<uint64_t>(1, (words + 255) / 256));
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, s, p, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_dj_build(const QDesc& d, int32_t key_slot, const DirectBufs& D, hipStream_t s) {
+  if (d.ntiles > 0)
+    hipLaunchKernelGGL((k_dj_mark<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, key_slot, D);
+  const uint64_t blocks = std::max<uint64_t>(1, (D.words + 255) / 256);
+  hipLaunchKernelGGL(k_dj_pack, dim3((unsigned)blocks), dim3(256), 0, s, D);
+  return hipGetLastError();
+}
+
 // ---- a join whose build side is itself a join (the nested build) ------------
 // The inner side's rows are in a lookup table (launch_lookup_build); a build
 // row that passes its predicate and meets m inner rows is m rows of the

@@ -89,6 +89,12 @@ struct Slot {
     hipGraphExec_t exec;
   };
   std::vector<CachedGraph> graphs;
+  // the direct join's region (all zero between queries; a query that may
+  // have left it otherwise marks it dirty, and the next one clears it first)
+  void* dreg = nullptr;
+  size_t dreg_bytes = 0;
+  uint64_t dreg_range = 0;  // the keys its layout holds (every query lays it out alike)
+  bool dreg_dirty = true;
 };
 
 struct Ticket;
@@ -267,6 +273,10 @@ void slot_destroy(vq_ctx* c, Slot* S) {
   if (S->arena.p) {
     hipFree(S->arena.p);
     release(c, S->arena.bytes);
+  }
+  if (S->dreg) {
+    hipFree(S->dreg);
+    release(c, S->dreg_bytes);
   }
   if (S->pinned) hipHostFree(S->pinned);
   for (auto& r : S->rec_pin)
@@ -1652,6 +1662,16 @@ struct JoinJob {
   LookupPlan lplan;
   int64_t lrows = 0, lkept = 0, lrow_bytes = 0;
   int64_t lmult = 1;  // the most build rows one key has, as planned
+  // the direct join (build keys in a dense range): the build side's scan,
+  // the compiled probe, the region's size and, once bound, its address
+  bool no_direct = false;  // batches (instances share a slot), VQ_NO_DIRECT
+  QDesc dq;
+  int32_t dkey_slot = 0;
+  DirectLaunch dl;
+  int64_t dbase = 0;
+  uint64_t drange = 0;
+  char* dreg = nullptr;
+  uint64_t dreg_range = 0;  // the region's layout (its key capacity, >= drange)
 };
 
 // Table fill: a GROUP BY's rows all find their keys, but fill the table by
@@ -1890,10 +1910,98 @@ int join_plan_local(JoinJob& j) {
   return VQ_OK;
 }
 
+int scan_desc(const vq_table* t, const vq_pred* pred, int nkeys, const int32_t* keys, QDesc& d, int* key_slots,
+              int64_t* rows, int64_t* kept);
+
+// The direct join: when the build key's zone maps bound it to a range of at
+// most kDirectDensity times the build rows (primary keys, surrogate keys)
+// and the range's accumulators fit kDirectMaxBytes, the build side is one
+// bit per key of the range and the groups are indexed by key (no
+// partitions).  Returns kRetry when the join does not qualify.
+constexpr uint64_t kDirectDensity = 16;
+constexpr uint64_t kDirectMaxBytes = (uint64_t)8 << 30;
+
+bool direct_disabled() {
+  const char* e = getenv("VQ_NO_DIRECT");
+  return e && *e && *e != '0';
+}
+
+// The region of a direct join over `range` keys: flags, the keys' 32-byte
+// records, bits, marks.
+uint64_t direct_region_bytes(uint64_t range) {
+  auto up = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+  const uint64_t words = (range + 31) / 32;
+  return 256 + up(range * 32) + up(words * 4) + up(words * 32);
+}
+
+int join_plan_direct(JoinJob& j) {
+  // (radix bits given by the caller ask for the partitioned join)
+  if (j.groupby || j.lookup || j.no_direct || j.radix_bits >= 0 || direct_disabled()) return kRetry;
+  const int bt = j.build->cols[j.build_key].type, pt = j.probe->cols[j.probe_key].type;
+  auto keytype = [](int t) { return t == VQ_I64 || t == VQ_DATE || t == VQ_U8 || t == VQ_U16; };
+  if (!keytype(bt) || !keytype(pt)) return kRetry;
+  const vq_zone z = merged_zone(j.build, j.build_key);
+  if (!z.has_minmax) return kRetry;
+  const int64_t lo = bt == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.min_bits : (int64_t)z.min_bits;
+  const int64_t hi = bt == VQ_DATE ? (int64_t)(int32_t)(uint32_t)z.max_bits : (int64_t)z.max_bits;
+  if (hi < lo) return kRetry;
+  const uint64_t span = (uint64_t)hi - (uint64_t)lo;  // range - 1
+  if (span >= (uint64_t)0xFFFFFFFEull ||
+      span >= kDirectDensity * (uint64_t)std::max<int64_t>(j.build->nrows, 4096))
+    return kRetry;
+  const uint64_t range = span + 1;
+  const uint64_t rbytes = direct_region_bytes(range);
+  if (rbytes > kDirectMaxBytes) return kRetry;
+  {
+    // a build side whose passing keys repeated last time: the partitions
+    // count multiplicities, this path does not
+    std::lock_guard<std::mutex> lk(j.c->mu);
+    if (j.c->build_est.count("D|" + build_est_key(j.build, j.build_key, j.bp))) return kRetry;
+  }
+  std::string jerr;
+  JoinPlan& plan = j.plan;
+  const int ts = std::max(0, std::min(4, j.tune.tile_shift));
+  std::vector<uint8_t> pkeep;
+  prune(j.probe, j.pp, pkeep);
+  j.ps.tile_rows = join_tile_rows(kept_rows(j.probe, pkeep), part_chunk_rows(j.ps.nslots), device_cus()) << ts;
+  j.brows = j.bkept = j.prows = j.pkept = 0;
+  if (int rc = make_spans(j.probe, pkeep, j.ps.tile_rows, j.ps.span, &j.ps.nspans, &j.ps.ntiles,
+                         &j.prows, &j.pkept))
+    return rc;
+  const int64_t cap = std::max<int64_t>(1, j.ps.ntiles * (int64_t)j.ps.tile_rows);
+  if (cap >= (int64_t)UINT32_MAX) return kRetry;
+  if (int rc = scan_desc(j.build, j.bp, 1, &j.build_key, j.dq, &j.dkey_slot, &j.brows, &j.bkept)) return rc;
+  if (jit_direct_prepare(j.ps, j.jd, &j.dl, &jerr)) return kRetry;
+  j.dbase = lo;
+  j.drange = range;
+  plan.direct = 1;
+  plan.bits = 0;
+  plan.topk = j.dev_topk ? j.topk : 0;
+  plan.nvals = j.nvals;
+  plan.P = j.dl.grid;  // the collect kernel's workgroups: one per probe workgroup
+  j.RW = 3 + j.naggs;
+  j.rec_cap = std::max<int64_t>(1, std::min<int64_t>(j.brows, j.prows));
+  memset(&j.B, 0, sizeof j.B);
+  size_t bytes = 64 * 4;
+  bytes += (size_t)cap * (4 + 4 + 8 * (1 + (size_t)j.nvals)) + (size_t)plan.P * 8;
+  bytes += (size_t)std::max<int64_t>(j.dq.ntiles, 1) * 4 + (size_t)((range + 31) / 32 + 255) / 256 * 4;
+  bytes += j.dev_topk ? (size_t)plan.P * j.topk * (j.RW * 8 + 16) + (size_t)16 * j.RW * 8
+                      : (size_t)j.rec_cap * j.RW * 8;
+  bytes += 14 * 256;
+  j.bytes = (bytes + 255) & ~(size_t)255;
+  j.host_bytes = 64 + (size_t)16 * j.RW * 8;
+  return VQ_OK;
+}
+
 int join_plan(JoinJob& j) {
   std::string jerr;
   JoinPlan& plan = j.plan;
   memset(&plan, 0, sizeof plan);
+  {
+    const int rc = join_plan_direct(j);
+    if (rc != kRetry) return rc;
+    memset(&plan, 0, sizeof plan);
+  }
   if (j.groupby && !j.no_local) {
     const int rc = join_plan_local(j);
     if (rc != kRetry) return rc;
@@ -2020,6 +2128,49 @@ void join_bind(JoinJob& j, char* dev, char* host) {
   B.tctr = B.flags + 16;
   j.hflags = (uint32_t*)host;
   j.hout = (uint64_t*)(host + 64);
+  if (j.plan.direct) {
+    // the region (laid out for its key capacity, the same for every query:
+    // what one query leaves uncleared, its slots, never lies where another's
+    // cleared arrays do) and this query's entries
+    DirectBufs& D = B.d;
+    auto up = [](uint64_t x) { return (size_t)((x + 255) & ~(uint64_t)255); };
+    const uint64_t words = (j.drange + 31) / 32, rc = j.dreg_range, rwords = (rc + 31) / 32;
+    char* r = j.dreg;
+    D.rflags = (uint32_t*)r;
+    r += 256;
+    D.rec = (uint32_t*)r;
+    r += up(rc * 32);
+    D.pres = (uint32_t*)r;
+    r += up(rwords * 4);
+    D.mark = (uint8_t*)r;
+    D.base = j.dbase;
+    D.range = j.drange;
+    D.words = words;
+    D.ntiles = j.ps.ntiles;
+    D.tile_rows = j.ps.tile_rows;
+    D.grid = j.dl.grid;
+    D.nvals = j.nvals;
+    const int64_t cap = std::max<int64_t>(1, j.ps.ntiles * (int64_t)j.ps.tile_rows);
+    D.cap = cap;
+    D.ew = 1 + j.nvals;
+    D.mrow = cv.take<uint32_t>(dev, (size_t)cap);
+    D.mcount = cv.take<uint32_t>(dev, (size_t)P);
+    D.ent = cv.take<uint64_t>(dev, (size_t)cap * D.ew);
+    D.tlist = cv.take<uint32_t>(dev, (size_t)cap);
+    D.tcount = cv.take<uint32_t>(dev, (size_t)P);
+    D.nbcnt = j.dq.ntiles;  // (written by the mark kernel, one per tile)
+    D.npcnt = (int64_t)std::max<uint64_t>(1, (words + 255) / 256);
+    D.bcnt = cv.take<uint32_t>(dev, (size_t)std::max<int64_t>(D.nbcnt, 1));
+    D.pcnt = cv.take<uint32_t>(dev, (size_t)D.npcnt);
+    if (j.dev_topk) {
+      B.cand = cv.take<uint64_t>(dev, (size_t)P * j.topk * j.RW);
+      B.citems = (KItem*)cv.take<uint64_t>(dev, (size_t)P * j.topk * 2);
+      B.out = cv.take<uint64_t>(dev, (size_t)16 * j.RW);
+    } else {
+      B.recs = cv.take<uint64_t>(dev, (size_t)j.rec_cap * j.RW);
+    }
+    return;
+  }
   if (j.plan.local) {
     B.l.keys = cv.take<int64_t>(dev, j.plan.gcap);
     B.l.gkey = cv.take<int64_t>(dev, (size_t)j.plan.limit * j.plan.passes);
@@ -2081,7 +2232,8 @@ hipError_t join_enqueue(JoinJob& j, hipStream_t s, hipEvent_t* ev) {
     const hipError_t e = launch_lookup_build(j.lq, j.lkey_slot, j.B.lk, s);
     if (e != hipSuccess) return e;
   }
-  hipError_t e = j.plan.local ? launch_local(j.ll, j.ps, j.jd, j.plan, j.B, s, ev, j.lookup)
+  hipError_t e = j.plan.direct ? launch_direct(j.dq, j.dkey_slot, j.dl, j.jd, j.plan, j.B, s, ev)
+                 : j.plan.local ? launch_local(j.ll, j.ps, j.jd, j.plan, j.B, s, ev, j.lookup)
                               : launch_join(j.bl, j.pl, j.jd, j.plan, j.B, s, ev);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(j.hflags, j.B.flags, 32, hipMemcpyDeviceToHost, s);
@@ -2106,6 +2258,17 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     return kRetry;
   }
   if (j.plan.local && hflags[0]) return fail(VQ_E_DEVICE, "exec: gpu: GROUP BY flags %#x", hflags[0]);
+  if (j.plan.direct) {
+    if (hflags[0] & 1) return fail(VQ_E_CORRUPT, "exec: gpu: a build key outside its column's zone maps");
+    if (hflags[0] & 2) {
+      // a passing build key repeats: through the partitions, which count
+      // multiplicities (now and for later joins over this build side)
+      std::lock_guard<std::mutex> lk(c->mu);
+      c->build_est["D|" + build_est_key(j.build, j.build_key, j.bp)] = 1;
+      j.no_direct = true;
+      return kRetry;
+    }
+  }
   if (j.lookup) {
     if (hflags[9] & 2) return fail(VQ_E_CORRUPT, "exec: gpu: a build key outside its column's zone maps");
     if (hflags[9]) return fail(VQ_E_DEVICE, "exec: gpu: join lookup table filled");
@@ -2119,7 +2282,7 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     }
   }
   if (hflags[5]) return fail(VQ_E_DEVICE, "exec: gpu: partition scatter wrote past a partition's end");
-  if (hflags[0]) {
+  if (hflags[0] && !j.plan.direct) {
     // a partition's accumulator pool filled: a larger pool if the LDS has
     // room, else (and when a table filled) one bit more of partitioning
     if (!(hflags[0] & 1) && j.plan.pool < j.pool_max) {
@@ -2198,8 +2361,8 @@ int join_result(JoinJob& j, vq_agg_result* out, vq_stats* stats, hipEvent_t* ev)
     }
     stats->build_rows = j.groupby ? 0 : hflags[6];
     stats->build_keys = hflags[2];
-    stats->partitions = j.plan.local ? 0 : j.plan.P;
-    stats->radix_bits = j.plan.local ? 0 : j.bits;
+    stats->partitions = j.plan.local || j.plan.direct ? 0 : j.plan.P;
+    stats->radix_bits = j.plan.local || j.plan.direct ? 0 : j.bits;
     stats->grid = j.plan.local ? j.plan.lgrid : j.plan.P;
     stats->tile_rows = j.ps.tile_rows;
   }
@@ -2337,6 +2500,43 @@ void join_job_init(JoinJob& j, vq_ctx* c, const vq_table* build, int32_t build_k
   j.groupby = groupby;
 }
 
+// The slot's direct-join region of at least `bytes` (grown on demand), clear:
+// a new or dirty region is cleared on the slot's stream first.
+int direct_region(vq_ctx* c, Slot* S, uint64_t range, char** out, uint64_t* cap_range) {
+  if (S->dreg_range < range) {
+    // grown by half again at least, so a stream of growing ranges
+    // reallocates rarely
+    range = std::max<uint64_t>(range, S->dreg_range + S->dreg_range / 2);
+    const size_t bytes = (size_t)direct_region_bytes(range);
+    if (S->dreg) {
+      HIP_TRY(hipStreamSynchronize(S->stream));
+      HIP_TRY(hipFree(S->dreg));
+      release(c, S->dreg_bytes);
+      S->dreg = nullptr;
+      S->dreg_bytes = 0;
+      S->dreg_range = 0;
+    }
+    if (!reserve(c, bytes))
+      return fail(VQ_E_BUDGET, "exec: gpu: direct join region of %zu bytes exceeds the device budget", bytes);
+    hipError_t e = hipMalloc(&S->dreg, bytes);
+    if (e != hipSuccess) {
+      release(c, bytes);
+      S->dreg = nullptr;
+      return fail(VQ_E_DEVICE, "exec: gpu: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    S->dreg_bytes = bytes;
+    S->dreg_range = range;
+    S->dreg_dirty = true;
+  }
+  if (S->dreg_dirty) {
+    HIP_TRY(launch_zero_words((uint64_t*)S->dreg, S->dreg_bytes / 8, S->stream));
+    S->dreg_dirty = false;
+  }
+  *out = (char*)S->dreg;
+  *cap_range = S->dreg_range;
+  return VQ_OK;
+}
+
 // One join (or hash GROUP BY) on one slot, start to finish.
 int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
   if (int rc = join_setup(j)) return rc;
@@ -2353,14 +2553,24 @@ int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
                   j.attempts, j.bits);
     if (int rc = join_plan(j)) return rc;
     j.attempts++;
+    if (j.plan.direct) {
+      const int rc = direct_region(c, S.get(), j.drange, &j.dreg, &j.dreg_range);
+      if (rc == VQ_E_BUDGET) {  // no room for the region: the partitioned join
+        j.no_direct = true;
+        continue;
+      }
+      if (rc) return rc;
+    }
     void* ar;
     void* hp;
     if (int rc = arena_get(c, S.get(), j.bytes, &ar)) return rc;
     if (int rc = pinned_get(S.get(), j.host_bytes, &hp)) return rc;
     join_bind(j, (char*)ar, (char*)hp);
     j.pin = &S->rec_pin[0];
+    if (j.plan.direct) S->dreg_dirty = true;  // until the query has run through
     HIP_TRY(join_enqueue(j, S->stream, S->ev));
     HIP_TRY(hipStreamSynchronize(S->stream));
+    if (j.plan.direct) S->dreg_dirty = false;
     const int rc = join_result(j, out, stats, S->ev);
     if (rc != kRetry) return rc;
   }
@@ -3085,6 +3295,7 @@ JoinTune autotune(const JoinJob& proto, vq_agg_result* out) {
                           proto.probe_key, proto.pp, proto.naggs, proto.aggs, proto.radix_bits,
                           proto.topk, proto.groupby);
             j->tune = t;
+            j->no_direct = true;
             const int rc = join_run(*j, out, &st);
             if (rc || j->attempts > 1) {
               ms = 1e30;
@@ -3131,6 +3342,7 @@ extern "C" int vq_batch_run(vq_ctx* c, int32_t n, const vq_query* qs, vq_agg_res
   // class not tuned yet is tuned here, by running it)
   auto plan_one = [&](int32_t i) -> int {
     JoinJob& j = *jobs[i];
+    j.no_direct = true;  // (the instances share the slot: no per-slot region)
     if (int rc = join_setup(j)) return rc;
     const std::string cls = tune_class(j);
     JoinTune t;

@@ -310,10 +310,58 @@ struct LookupBufs {
   uint32_t pad_;
 };
 
+// The direct join (build keys whose zone maps bound them to a dense range
+// [base, base + range)).  A persistent region per slot, zero between queries
+// where noted: rflags (zero); rec[o][8], key base + o's record -- [0] its
+// matches so far (zero), [1] 1 + its latest match past kDjSlots, chained
+// through the entries (zero), [2, 8) the entries of its first kDjSlots
+// matches (not cleared: read only below the count) -- one 32-byte sector per
+// key; pres, bit o: a passing build row has key base + o (zero: cleared by
+// k_dj_collect); mark, one byte per key, the build's marks (zero: packed into
+// pres and cleared by k_dj_pack).  Per query (the arena), in regions of the
+// probe workgroups (tile t0 * tile_rows onwards: at most one item per row):
+// the matched probe rows (mrow) and their count per workgroup (mcount), the
+// match entries ent[e][1 + nvals] -- word 0 (1 + the key's previous further
+// match) << 32 | probe row, then the argument values -- and the keys each
+// workgroup met first (tlist) with their count (tcount); the passing build
+// rows per build tile (bcnt) and the distinct passing keys per pack block
+// (pcnt): counters are summed from arrays, never added into one word by many
+// waves (atomics on one address serialize).  rflags: [0] error bits (1 a
+// build key outside the zone range), [2] collect workgroups finished, [4]
+// groups, [6] passing build rows and [7] distinct passing keys (summed by the
+// collect kernel's workgroup 0); reset by the collect kernel's last
+// workgroup.
+constexpr int kDjSlots = 6;
+struct DirectBufs {
+  uint32_t* rflags;
+  uint32_t* rec;
+  uint32_t* pres;
+  uint8_t* mark;
+  uint32_t* mrow;
+  uint32_t* mcount;
+  uint64_t* ent;
+  uint32_t* tlist;
+  uint32_t* tcount;
+  uint32_t* bcnt;
+  uint32_t* pcnt;
+  int64_t nbcnt;     // build tiles
+  int64_t npcnt;     // pack blocks
+  int64_t base;
+  uint64_t range;
+  uint64_t words;    // (range + 31) / 32
+  int64_t cap;       // items capacity: probe tiles * tile_rows
+  int64_t ntiles;    // the probe's tiles and their rows (the workgroup regions)
+  int32_t tile_rows;
+  int32_t grid;      // probe workgroups
+  int32_t nvals;
+  int32_t ew;        // entry words: 1 + nvals
+};
+
 struct JoinBufs {
   PartBufs b, p;
   LocalBufs l;
   LookupBufs lk;
+  DirectBufs d;
   uint64_t* recs;       // one record per matched group (full-output mode)
   uint64_t* cand;       // [P][topk] per-partition top-k candidates (top-k mode)
   struct KItem* citems; // [P][topk] their sort keys (value, first row, index)
@@ -356,7 +404,7 @@ struct JoinPlan {
                         // partition (-1: no filter)
   int32_t P1;           // coarse partitions of the partition kernels (P: one pass)
   int32_t bits1;        // log2(P1)
-  int32_t pad3_;
+  int32_t direct;       // the direct join (DirectBufs) instead of partitions
 };
 
 // LDS bytes of one partition's table and accumulator pool (with `reps`
@@ -423,6 +471,18 @@ struct JitArgs {
   int64_t lk_base;     // direct tables (lk_range > 0): head[key - lk_base]
   uint32_t lk_range;
   uint32_t lk_pad;
+  // the direct join's probe (vq_dj_probe, vq_dj_match): DirectBufs
+  const uint32_t* dj_pres;
+  uint32_t* dj_rec;
+  uint32_t* dj_mrow;
+  uint32_t* dj_mcount;
+  uint64_t* dj_ent;
+  uint32_t* dj_tlist;
+  uint32_t* dj_tcount;
+  int64_t dj_base;
+  uint64_t dj_range;
+  int32_t dj_ew;
+  int32_t dj_pad;
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):
@@ -469,6 +529,23 @@ struct LocalLaunch {
   uint32_t lds = 0;
   JitArgs args;
 };
+// The run-time compiled probe of the direct join (vq_dj_probe).
+struct DirectLaunch {
+  hipFunction_t fn = nullptr;     // vq_dj_probe
+  hipFunction_t match = nullptr;  // vq_dj_match
+  int32_t grid = 0;
+  JitArgs args;
+};
+int jit_direct_prepare(const JSide& sd, const JDesc& jd, DirectLaunch* out, std::string* err);
+hipError_t jit_direct_launch(const DirectLaunch& dl, const DirectBufs& D, hipStream_t s);
+// The direct join: build (k_dj_build), probe, collect (k_dj_collect: groups,
+// top-k, the region cleared for the next query); events ev[0..3] around it.
+hipError_t launch_direct(const QDesc& bq, int32_t bkey_slot, const DirectLaunch& dl, const JDesc& jd,
+                         const JoinPlan& plan, const JoinBufs& B, hipStream_t s, hipEvent_t* ev);
+// The direct join's build side (k_scan.hip): marks, then the packed bits.
+hipError_t launch_dj_build(const QDesc& d, int32_t key_slot, const DirectBufs& D, hipStream_t s);
+// Clears a direct region of `words` 64-bit words.
+hipError_t launch_zero_words(uint64_t* p, uint64_t words, hipStream_t s);
 constexpr int32_t kLocalLds = 60 * 1024;  // vq_hash_local's dynamic LDS
 int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunch* out,
                       std::string* err);

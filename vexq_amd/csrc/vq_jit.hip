@@ -34,6 +34,8 @@ struct Module {
   hipFunction_t part_count = nullptr;
   hipFunction_t part_scatter = nullptr;
   hipFunction_t hash_local = nullptr;
+  hipFunction_t dj_probe = nullptr;
+  hipFunction_t dj_match = nullptr;
 };
 
 std::mutex g_mu;
@@ -134,6 +136,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_PB " + std::to_string(pbuf) + "\n";
   s += "#define VQ_BLOOM " + std::to_string(bloom) + "\n";
   s += "#define VQ_LIVEBITS " + std::to_string(livebits) + "\n";
+  s += "#define VQ_DJ_SLOTS " + std::to_string(kDjSlots) + "\n";
   // the key: one column (key_slot), or several dictionary codes packed
   int nkey = 1, ksl[8] = {key_slot}, ksh[8] = {0};
   if (keys && keys->nkeys > 1) {
@@ -169,7 +172,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   arr2(s, "kFS", fs);
   arr2(s, "kFO", fo);
   arr2(s, "kFC", fc);
-  if (kernels & 8) {
+  if (kernels & 24) {
     s += "#define VQ_NAG " + std::to_string(nag) + "\n";
     arr(s, "kLOP", lop, nag);
     arr(s, "kLSRC", lsrc, nag);
@@ -291,6 +294,9 @@ Module* compile_module(const std::string& pro, std::string* err) {
          hipModuleGetFunction(&m->part_scatter, m->mod, "vq_part_scatter") == hipSuccess;
   if (ok && pro.find("#define VQ_KERNELS 8\n") != std::string::npos)
     ok = hipModuleGetFunction(&m->hash_local, m->mod, "vq_hash_local") == hipSuccess;
+  if (ok && pro.find("#define VQ_KERNELS 16\n") != std::string::npos)
+    ok = hipModuleGetFunction(&m->dj_probe, m->mod, "vq_dj_probe") == hipSuccess &&
+         hipModuleGetFunction(&m->dj_match, m->mod, "vq_dj_match") == hipSuccess;
   if (!ok) {
     delete m;
     if (err) *err = "hipModuleLoadData failed";
@@ -499,6 +505,72 @@ int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunc
   }
   a.nvop = v;
   return 0;
+}
+
+// The aggregates' folds and words as the run-time kernels take them (every
+// aggregate with an argument: its fold, first word, fixed-point scale and
+// value index); returns their count.
+int jit_value_ops(const JDesc& jd, int* lop, int* lsrc, JitArgs* a) {
+  int v = 0;
+  for (int j = 0; j < jd.naggs && v < 8; j++) {
+    const JAgg& g = jd.agg[j];
+    if (g.vidx < 0) continue;
+    int op;
+    if (g.kind == VQ_AGG_SUM || g.kind == VQ_AGG_AVG) op = g.e.is_float ? 0 : 1;
+    else if (g.kind == VQ_AGG_MIN) op = g.accum == VQ_F64 ? 2 : 4;
+    else op = g.accum == VQ_F64 ? 3 : 5;
+    lop[v] = op;
+    lsrc[v] = g.vidx;
+    if (a) {
+      a->vop[v] = op;
+      a->vword[v] = g.word0;
+      a->vlsb[v] = g.fs.lsb;
+      a->vw[v] = g.fs.w;
+      a->vsrc[v] = g.vidx;
+    }
+    v++;
+  }
+  if (a) a->nvop = v;
+  return v;
+}
+
+int jit_direct_prepare(const JSide& sd, const JDesc& jd, DirectLaunch* out, std::string* err) {
+  if (jit_disabled()) return fail_jit(err, "JIT disabled");
+  int nv = 0;
+  const QDesc d = side_desc(sd, &jd, &nv);
+  const Flat f = flatten_terms(d);
+  int lop[8] = {0}, lsrc[8] = {0};
+  const int nag = jit_value_ops(jd, lop, lsrc, nullptr);
+  Module* m = get_module(prologue(d, f, 16, sd.key_slot, nv, 1, 0, 0, lop, lsrc, nag, 0, &sd), err);
+  if (!m) return -1;
+  out->fn = m->dj_probe;
+  out->match = m->dj_match;
+  const int cus = device_cus();
+  int per_cu = 0;
+  hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->dj_probe, 1024, 0);
+  if (per_cu <= 0) per_cu = 1;
+  const int64_t g = (int64_t)cus * per_cu;
+  out->grid = (int32_t)std::max<int64_t>(1, d.ntiles < g ? d.ntiles : g);
+  fill_common(d, f, out->args);
+  jit_value_ops(jd, lop, lsrc, &out->args);
+  return 0;
+}
+
+hipError_t jit_direct_launch(const DirectLaunch& dl, const DirectBufs& D, hipStream_t s) {
+  JitArgs a = dl.args;
+  a.dj_pres = D.pres;
+  a.dj_rec = D.rec;
+  a.dj_mrow = D.mrow;
+  a.dj_mcount = D.mcount;
+  a.dj_ent = D.ent;
+  a.dj_tlist = D.tlist;
+  a.dj_tcount = D.tcount;
+  a.dj_base = D.base;
+  a.dj_range = D.range;
+  a.dj_ew = D.ew;
+  hipError_t e = launch(dl.fn, (unsigned)dl.grid, 0, s, a, 1024);
+  if (e != hipSuccess) return e;
+  return launch(dl.match, (unsigned)dl.grid, 0, s, a, 256);
 }
 
 hipError_t jit_local_launch(const LocalLaunch& ll, const JoinBufs& B, hipStream_t s, int32_t k0) {
