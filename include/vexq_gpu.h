@@ -319,6 +319,18 @@ int vq_agg_dict(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t nke
                 const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                 vq_agg_result* out, vq_stats* stats);
 
+/* Several global aggregates (no GROUP BY) over one table in one pass: n
+ * queries (at most 16) with the same aggregates and predicates of one shape
+ * (the same columns and comparisons; the literals may differ), preds[i] the
+ * i-th query's predicate (nterms 0: every row), outs[i] / stats[i] (stats may
+ * be NULL) its result, as vq_agg_dict with no group-by columns gives it.  Every
+ * row's columns are read once for all of them.  No counterpart in the
+ * reference (its operators run one query per call): this is the serving
+ * path's continuous batching of queued scans.  Queries not of one shape run
+ * one after another. */
+int vq_agg_dict_multi(vq_ctx* ctx, const vq_table* t, int32_t n, const vq_pred* preds,
+                      int32_t naggs, const vq_agg* aggs, vq_agg_result* outs, vq_stats* stats);
+
 /* The same dictionary-code GROUP BY with no device allocation.  No
  * counterpart in the reference: it is for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -14,7 +14,8 @@
 //   vq_filter_bits pass bits + per-tile counts for the selection vector
 //   vq_part_count / vq_part_scatter   radix partitioning of a join side
 //   vq_hash_local  GROUP BY of few keys into per-workgroup partials
-//   vq_dj_probe    the direct join's probe and aggregate
+//   vq_dj_probe / vq_dj_match  the direct join's probe and its matches
+//   vq_dense_multi several one-group aggregates of one shape, one pass
 #ifndef VQ_JIT_PROLOGUE
 // Stand-alone build check (hipcc -c): the 8-aggregate GROUP BY shape.
 #define VQ_JIT_PROLOGUE 1
@@ -26,11 +27,12 @@
 #define VQ_KEYSLOT 2
 #define VQ_NV 5
 #define VQ_PROBE 1
-#define VQ_KERNELS 31
+#define VQ_KERNELS 63
 #define VQ_PB 8
 #define VQ_BLOOM 1
 #define VQ_LIVEBITS 1
 #define VQ_DJ_SLOTS 6
+#define VQ_NQ 4
 #define VQ_NKEY 1
 #define VQ_LOOKUP 0
 #define VQ_JKEY 0
@@ -180,6 +182,9 @@ struct Args {
   u64 dj_range;
   i32 dj_ew;
   i32 dj_pad;
+  // vq_dense_multi: the predicate literals of each of the VQ_NQ queries,
+  // [q][VQ_NT] (floats as their bits)
+  const i64* mlit;
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -598,7 +603,8 @@ using namespace vqj;
 
 // VQ_KERNELS selects what a module holds: 1 dense aggregate, 2 filter bits,
 // 4 join partitioning, 8 GROUP BY through per-workgroup partials, 16 the
-// direct join's probe.
+// direct join's probe, 32 several one-group aggregates of one shape in one
+// pass.
 #if VQ_KERNELS & 1
 // ---------------------------------------------------------------------------
 // Dense GROUP BY (see k_agg_dense in k_scan.hip for the contract).
@@ -1635,5 +1641,111 @@ extern "C" __global__ __launch_bounds__(256) void vq_dj_match(const Args a) {
   }
   __syncthreads();
   if (threadIdx.x == 0) a.dj_tcount[blockIdx.x] = s_nt;
+}
+#endif
+
+#if VQ_KERNELS & 32
+// ---------------------------------------------------------------------------
+// Several aggregates of one shape in one pass (a served stream's scans that
+// differ only in their predicates' literals: continuous batching, no
+// counterpart in the reference, whose operators run one query per call).
+// The shape is vq_dense's with one group; VQ_NQ queries' literals come from
+// a.mlit.  Every row's columns are read once and every query's predicate
+// and folds run on it, into that query's own registers; each wave writes one
+// partial per query and item, [q][NA + 2][grid] (k_agg_final's layout with a
+// group per query), and k_agg_final reduces them in wave order, so each
+// query's words are those of the same query run by vq_dense up to the lanes'
+// summation order.
+// ---------------------------------------------------------------------------
+namespace vqj {
+template <int T>
+__device__ __forceinline__ bool termq(const Args& a, const i64* lit, const i64* rawrow) {
+  constexpr int S = kTS[T], OP = kTO[T];
+  if constexpr (OP == 6) {
+    const i64 v = rawrow[S];
+    return (u64)v < 64 && (((u64)lit[T] >> v) & 1ull);
+  } else if constexpr (OP >= 16) {
+    return term<T>(a, rawrow);  // (a compare of two columns has no literal)
+  } else if constexpr (kST[S] == VQ_F64) {
+    return cmp<OP>(as_f64(rawrow[S]), as_f64(lit[T]));
+  } else {
+    return cmp<OP>(rawrow[S], lit[T]);
+  }
+}
+
+template <int... T>
+__device__ __forceinline__ bool pass_q(const Args& a, const i64* lit, const i64* rawrow, Seq<T...>) {
+  return (true && ... && termq<T>(a, lit, rawrow));
+}
+}  // namespace vqj
+
+extern "C" __global__ __launch_bounds__(64) void vq_dense_multi(const Args a) {
+  constexpr int U = VQ_U, NA = VQ_NA, Q = VQ_NQ, NT = VQ_NT > 0 ? VQ_NT : 1;
+  const int lane = threadIdx.x;
+  i64 lit[Q][NT];
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) lit[q][t] = VQ_NT > 0 ? a.mlit[q * VQ_NT + t] : 0;
+  u64 racc[Q][NA > 0 ? NA : 1];
+  u32 rcnt[Q], rfirst[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    init_regs(racc[q], seq<NA>{});
+    rcnt[q] = 0;
+    rfirst[q] = 0xFFFFFFFFu;
+  }
+  TileIt it;
+  tile_begin<U>(a, blockIdx.x, it);
+  i64 nraw[VQ_NS][U];
+  if (it.have) load_rows_blocked<U>(a, it.base, it.rend, lane, nraw);
+  while (it.have) {
+    i64 raw[VQ_NS][U];
+#pragma unroll
+    for (int c = 0; c < VQ_NS; c++)
+#pragma unroll
+      for (int u = 0; u < U; u++) raw[c][u] = nraw[c][u];
+    const i64 base = it.base, lo = it.lo, rend = it.rend;
+    tile_advance<U>(a, it);
+    if (it.have) load_rows_blocked<U>(a, it.base, it.rend, lane, nraw);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      i64 row[VQ_NS];
+#pragma unroll
+      for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
+      const i64 r = base + (i64)lane * U + u;
+      if (!(r >= lo && r < rend)) continue;
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        if (!pass_q(a, lit[q], row, seq<VQ_NT>{})) continue;
+        rfirst[q] = rcnt[q] == 0 ? (u32)r : rfirst[q];
+        rcnt[q]++;
+        fold_regs(racc[q], a, row, seq<NA>{});
+      }
+    }
+  }
+  constexpr int NI = NA + 2;
+  u64* out = a.part + blockIdx.x;
+  const u64 ostride = gridDim.x;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+#pragma unroll
+    for (int item = 0; item < NI; item++) {
+      u64 v;
+      int op;
+      if (item < NA) {
+        v = racc[q][item];
+        op = kAOP[item];
+      } else if (item == NA) {
+        v = rcnt[q];
+        op = 100;
+      } else {
+        v = rfirst[q];
+        op = 101;
+      }
+      v = wave_reduce(op, v);
+      if (lane == 0) out[(q * NI + item) * ostride] = v;
+    }
+  }
 }
 #endif

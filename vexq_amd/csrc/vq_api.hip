@@ -1143,6 +1143,142 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
 }
 
 // ===========================================================================
+// several one-group aggregates of one shape in one pass (continuous batching)
+// ===========================================================================
+namespace {
+
+constexpr int32_t kMultiMax = 16;  // queries in one vq_agg_dict_multi pass
+
+// Same shape: the same columns, predicate terms (columns, comparisons) and
+// accumulators (expressions with their literals); only the predicate
+// literals may differ.
+bool same_shape(const QDesc& a, const QDesc& b) {
+  if (a.nslots != b.nslots || a.nacc != b.nacc || a.ngroups != 1 || b.ngroups != 1) return false;
+  for (int c = 0; c < a.nslots; c++) {
+    const QSlot &x = a.slot[c], &y = b.slot[c];
+    if (x.p != y.p || x.type != y.type || x.nterms != y.nterms || x.key_stride != y.key_stride) return false;
+    for (int t = 0; t < x.nterms; t++)
+      if (x.terms[t].op != y.terms[t].op || x.terms[t].slot2 != y.terms[t].slot2) return false;
+  }
+  return memcmp(a.acc, b.acc, sizeof(QAcc) * (size_t)a.nacc) == 0;
+}
+
+}  // namespace
+
+extern "C" int vq_agg_dict_multi(vq_ctx* c, const vq_table* t, int32_t n, const vq_pred* preds,
+                                 int32_t naggs, const vq_agg* aggs, vq_agg_result* outs,
+                                 vq_stats* stats) {
+  if (!c || !t || !outs || n < 1) return fail(VQ_E_ARG, "exec: gpu: NULL argument or %d queries", n);
+  for (int32_t q = 0; q < n; q++) outs[q].ngroups = 0;
+  if (n > kMultiMax) return fail(VQ_E_ARG, "exec: gpu: %d queries in one pass (at most %d)", n, kMultiMax);
+  if (t->ctx != c) return fail(VQ_E_ARG, "exec: gpu: table belongs to another context");
+  auto pred_of = [&](int32_t q) -> const vq_pred* { return preds && preds[q].nterms > 0 ? &preds[q] : nullptr; };
+  // each query planned as vq_agg_dict plans it; a batch that is not of one
+  // shape (or one the run-time kernels cannot take) runs query by query
+  std::vector<QDesc> dq((size_t)n);
+  std::vector<AccMap> am((size_t)n);
+  std::vector<SlotMap> smv((size_t)n);
+  int32_t card[4] = {1, 1, 1, 1};
+  bool one_shape = true;
+  for (int32_t q = 0; q < n; q++) {
+    if (int rc = plan_dense(t, pred_of(q), 0, nullptr, naggs, aggs, dq[q], am[q], smv[q], card)) return rc;
+    one_shape = one_shape && same_shape(dq[0], dq[q]);
+  }
+  int32_t nq = 1;
+  while (nq < n) nq *= 2;  // (one compiled kernel per power of two)
+  DenseJit dj;
+  std::string jerr;
+  if (!one_shape || jit_multi_prepare(dq[0], nq, &dj, &jerr)) {
+    for (int32_t q = 0; q < n; q++)
+      if (int rc = vq_agg_dict(c, t, pred_of(q), 0, nullptr, naggs, aggs, &outs[q], stats ? &stats[q] : nullptr))
+        return rc;
+    return VQ_OK;
+  }
+  // the rows any query keeps (pruning never changes a result)
+  std::vector<uint8_t> keep, kq;
+  prune(t, pred_of(0), keep);
+  for (int32_t q = 1; q < n; q++) {
+    prune(t, pred_of(q), kq);
+    for (size_t g = 0; g < keep.size(); g++) keep[g] |= kq[g];
+  }
+  // tiles and grid as vq_agg_dict sizes them for one query of this shape, so
+  // each wave folds the same rows in the same order as there: over the same
+  // kept row groups a query's words are the same bits alone and in the pass
+  QDesc d = dq[0];
+  DenseJit d1;
+  if (jit_dense_prepare(d, &d1, &jerr)) d1.slots = dj.slots;
+  {
+    const int64_t kr = kept_rows(t, keep);
+    if (kr >= (int64_t)1 << 20) {
+      const int64_t chunk = dense_chunk_rows(d.nslots);
+      const int64_t share = (kr + d1.slots - 1) / d1.slots;
+      d.tile_rows = (int32_t)std::max<int64_t>(1024, (share + chunk - 1) / chunk * chunk);
+    } else {
+      d.tile_rows = scan_tile_rows(kr);
+    }
+  }
+  int64_t rows = 0, kept = 0;
+  if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
+  // the literals, one row per query (the last repeated up to nq)
+  int64_t lit0[kJitTerms];
+  const int nt = jit_term_literals(d, lit0);
+  const int ntw = std::max(nt, 1);
+
+  SlotLease S(c);
+  hipSetDevice(c->device);
+  const int NI = d.nacc + 2;
+  const int32_t grid = d.ntiles > 0 ? (int32_t)std::min<int64_t>(d.ntiles, d1.slots) : 0;
+  const size_t part_words = (size_t)std::max<int32_t>(grid, 1) * nq * NI;
+  const size_t out_words = (size_t)nq * NI;
+  const size_t lit_words = (size_t)nq * ntw;
+  void* ar;
+  if (int rc = arena_get(c, S.get(), (part_words + out_words + lit_words) * 8 + 1024, &ar)) return rc;
+  Carve cv;
+  uint64_t* part = cv.take<uint64_t>((char*)ar, part_words);
+  uint64_t* fin = cv.take<uint64_t>((char*)ar, out_words);
+  int64_t* mlit = (int64_t*)cv.take<uint64_t>((char*)ar, lit_words);
+  void* hp;
+  if (int rc = pinned_get(S.get(), (out_words + lit_words) * 8, &hp)) return rc;
+  uint64_t* hfin = (uint64_t*)hp;
+  int64_t* hlit = (int64_t*)(hfin + out_words);
+  for (int32_t q = 0; q < nq; q++) {
+    int64_t l[kJitTerms];
+    jit_term_literals(dq[std::min(q, n - 1)], l);
+    for (int i = 0; i < ntw; i++) hlit[(size_t)q * ntw + i] = i < nt ? l[i] : 0;
+  }
+  // the queries as the groups of one reduction
+  QDesc dm = d;
+  dm.ngroups = nq;
+  if (d.ntiles > 0) {
+    HIP_TRY(hipMemcpyAsync(mlit, hlit, lit_words * 8, hipMemcpyHostToDevice, S->stream));
+    if (stats) HIP_TRY(hipEventRecord(S->ev[0], S->stream));
+    HIP_TRY(jit_multi_launch(dj, d, grid, part, mlit, S->stream));
+    HIP_TRY(launch_agg_final(dm, grid, part, fin, S->stream));
+    if (stats) HIP_TRY(hipEventRecord(S->ev[2], S->stream));
+    HIP_TRY(hipMemcpyAsync(hfin, fin, out_words * 8, hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(hipStreamSynchronize(S->stream));
+  } else {
+    for (int32_t q = 0; q < nq; q++)
+      for (int i = 0; i < NI; i++) hfin[q * NI + i] = (i == d.nacc + 1) ? 0xFFFFFFFFULL : 0;
+  }
+  for (int32_t q = 0; q < n; q++) {
+    if (stats) {
+      vq_stats& st = stats[q];
+      memset(&st, 0, sizeof st);
+      if (d.ntiles > 0) st.device_ms = st.main_kernel_ms = elapsed(S->ev[0], S->ev[2]);
+      st.rows_scanned = rows;
+      st.rg_total = t->nrg;
+      st.rg_scanned = kept;
+      st.bytes_alg = rows * slot_bytes_per_row(t, smv[q]);
+      st.grid = grid;
+      st.tile_rows = d.tile_rows;
+    }
+    if (int rc = dense_result(hfin + (size_t)q * NI, dq[q], am[q], card, 0, naggs, aggs, &outs[q])) return rc;
+  }
+  return VQ_OK;
+}
+
+// ===========================================================================
 // dictionary-code GROUP BY that allocates nothing
 // ===========================================================================
 // The partials and final words of vq_agg_dict_noalloc live in this file-scope
@@ -3050,10 +3186,62 @@ void run_ticket(Ticket* k) {
   k->cv.notify_all();
 }
 
+// A global aggregate (no GROUP BY) another can share a pass with: the same
+// table and aggregates, a predicate on the same columns with the same
+// comparisons (vq_agg_dict_multi checks the planned shapes again).
+bool shareable(const Ticket& a, const Ticket& b) {
+  const vq_query &x = a.q, &y = b.q;
+  if (x.kind != VQ_Q_AGG_DICT || y.kind != VQ_Q_AGG_DICT || x.nkeys != 0 || y.nkeys != 0) return false;
+  if (x.table != y.table || a.aggs.size() != b.aggs.size() || a.has_pred != b.has_pred) return false;
+  if (!a.aggs.empty() && memcmp(a.aggs.data(), b.aggs.data(), a.aggs.size() * sizeof(vq_agg)) != 0) return false;
+  if (!a.has_pred) return true;
+  if (a.pred.nterms != b.pred.nterms) return false;
+  for (int i = 0; i < a.pred.nterms; i++)
+    if (a.pred.terms[i].col != b.pred.terms[i].col || a.pred.terms[i].op != b.pred.terms[i].op) return false;
+  return true;
+}
+
+bool share_disabled() {
+  const char* e = getenv("VQ_SERVE_NO_SHARE");
+  return e && *e && *e != '0';
+}
+
+// Queued global aggregates of one shape in one pass (vq_agg_dict_multi).
+void run_shared(std::vector<Ticket*>& ks) {
+  const size_t n = ks.size();
+  const Ticket& k0 = *ks[0];
+  std::vector<vq_pred> preds(n);
+  std::vector<vq_agg_result> outs(n);
+  std::vector<vq_stats> st(n);
+  for (size_t i = 0; i < n; i++) {
+    memset(&preds[i], 0, sizeof preds[i]);
+    if (ks[i]->has_pred) preds[i] = ks[i]->pred;
+    outs[i] = *ks[i]->out;
+  }
+  const int rc = vq_agg_dict_multi(k0.c, k0.q.table, (int32_t)n, preds.data(), k0.q.naggs,
+                                   k0.aggs.empty() ? nullptr : k0.aggs.data(), outs.data(), st.data());
+  for (size_t i = 0; i < n; i++) {
+    Ticket* k = ks[i];
+    k->out->ngroups = outs[i].ngroups;
+    if (k->stats) *k->stats = st[i];
+    std::lock_guard<std::mutex> lk(k->mu);
+    k->status = rc;
+    if (rc) k->err = g_err;
+    k->done = true;
+    k->cv.notify_all();
+  }
+}
+
+// Continuous batching: a worker takes the oldest queued query that the
+// memory budget admits, and with a global aggregate also every queued one
+// that can share its pass (up to kMultiMax), wherever they wait in the queue.
 void worker_loop(vq_ctx* c) {
   hipSetDevice(c->device);
+  const bool share = !share_disabled();
+  std::vector<Ticket*> ks;
   for (;;) {
-    Ticket* k = nullptr;
+    ks.clear();
+    uint64_t est = 0;
     {
       std::unique_lock<std::mutex> lk(c->qmu);
       c->qcv.wait(lk, [&] {
@@ -3064,15 +3252,27 @@ void worker_loop(vq_ctx* c) {
       });
       if (c->stopping && c->queue.empty()) return;
       if (c->queue.empty()) continue;
-      k = c->queue.front();
+      Ticket* k = c->queue.front();
       c->queue.pop_front();
-      c->serve_reserved += k->est;
+      ks.push_back(k);
+      est = k->est;
+      if (share && k->q.kind == VQ_Q_AGG_DICT && k->q.nkeys == 0)
+        for (auto it = c->queue.begin(); it != c->queue.end() && (int32_t)ks.size() < kMultiMax;) {
+          if (shareable(*k, **it)) {
+            ks.push_back(*it);
+            it = c->queue.erase(it);
+          } else {
+            ++it;
+          }
+        }
+      c->serve_reserved += est;
       c->running++;
     }
-    run_ticket(k);
+    if (ks.size() > 1) run_shared(ks);
+    else run_ticket(ks[0]);
     {
       std::lock_guard<std::mutex> lk(c->qmu);
-      c->serve_reserved -= k->est;
+      c->serve_reserved -= est;
       c->running--;
     }
     c->qcv.notify_all();

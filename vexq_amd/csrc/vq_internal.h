@@ -241,6 +241,14 @@ int jit_dense_prepare(const QDesc& d, DenseJit* out, std::string* err);
 hipError_t jit_dense_launch(const DenseJit& h, const QDesc& d, int32_t grid, uint64_t* part,
                             hipStream_t s);
 bool jit_available_filter(const QDesc& d, std::string* err);
+// Several one-group aggregates of d's shape in one pass (vq_dense_multi):
+// nq queries (a compiled kernel per shape and nq); -1 (with *err) when none.
+int jit_multi_prepare(const QDesc& d, int32_t nq, DenseJit* out, std::string* err);
+hipError_t jit_multi_launch(const DenseJit& h, const QDesc& d, int32_t grid, uint64_t* part,
+                            const int64_t* mlit, hipStream_t s);
+// The literals of d's predicate terms in the run-time kernels' order (floats
+// as bits; 0 for a compare of two columns) into lit; returns their count.
+int jit_term_literals(const QDesc& d, int64_t* lit);
 hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,
                                   hipStream_t s);
 
@@ -483,6 +491,8 @@ struct JitArgs {
   uint64_t dj_range;
   int32_t dj_ew;
   int32_t dj_pad;
+  // vq_dense_multi: each query's predicate literals, [q][terms]
+  const int64_t* mlit;
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):

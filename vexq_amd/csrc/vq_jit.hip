@@ -36,6 +36,8 @@ struct Module {
   hipFunction_t hash_local = nullptr;
   hipFunction_t dj_probe = nullptr;
   hipFunction_t dj_match = nullptr;
+  hipFunction_t multi = nullptr;
+  std::atomic<int> multi_per_cu{0};
 };
 
 std::mutex g_mu;
@@ -294,6 +296,8 @@ Module* compile_module(const std::string& pro, std::string* err) {
          hipModuleGetFunction(&m->part_scatter, m->mod, "vq_part_scatter") == hipSuccess;
   if (ok && pro.find("#define VQ_KERNELS 8\n") != std::string::npos)
     ok = hipModuleGetFunction(&m->hash_local, m->mod, "vq_hash_local") == hipSuccess;
+  if (ok && pro.find("#define VQ_KERNELS 32\n") != std::string::npos)
+    ok = hipModuleGetFunction(&m->multi, m->mod, "vq_dense_multi") == hipSuccess;
   if (ok && pro.find("#define VQ_KERNELS 16\n") != std::string::npos)
     ok = hipModuleGetFunction(&m->dj_probe, m->mod, "vq_dj_probe") == hipSuccess &&
          hipModuleGetFunction(&m->dj_match, m->mod, "vq_dj_match") == hipSuccess;
@@ -629,6 +633,44 @@ bool jit_available_filter(const QDesc& d, std::string* err) {
   if (jit_disabled()) return false;
   const Flat f = flatten_terms(d);
   return get_module(prologue(d, f), err) != nullptr;
+}
+
+int jit_term_literals(const QDesc& d, int64_t* lit) {
+  const Flat f = flatten_terms(d);
+  for (int t = 0; t < f.nt; t++) {
+    const int op = f.to[t];
+    const bool fl = op < 16 && op != VQ_IN && d.slot[f.ts[t]].type == VQ_F64;
+    lit[t] = op >= 16 ? 0 : fl ? f64_to_bits(f.tf[t]) : f.ti[t];
+  }
+  return f.nt;
+}
+
+int jit_multi_prepare(const QDesc& d, int32_t nq, DenseJit* out, std::string* err) {
+  if (jit_disabled()) return fail_jit(err, "JIT disabled");
+  const Flat f = flatten_terms(d);
+  const std::string pro = prologue(d, f, 32) + "#define VQ_NQ " + std::to_string(nq) + "\n";
+  Module* m = get_module(pro, err);
+  if (!m) return -1;
+  int per_cu = m->multi_per_cu.load();
+  if (per_cu <= 0) {
+    hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->multi, 64, 0);
+    if (per_cu <= 0) per_cu = 1;
+    m->multi_per_cu.store(per_cu);
+  }
+  out->fn = m->multi;
+  out->lds = 0;
+  out->slots = device_cus() * per_cu;
+  return 0;
+}
+
+hipError_t jit_multi_launch(const DenseJit& h, const QDesc& d, int32_t grid, uint64_t* part,
+                            const int64_t* mlit, hipStream_t s) {
+  const Flat f = flatten_terms(d);
+  JitArgs a;
+  fill_common(d, f, a);
+  a.part = part;
+  a.mlit = mlit;
+  return launch(h.fn, (unsigned)grid, 0, s, a);
 }
 
 hipError_t jit_launch_filter_bits(const QDesc& d, uint64_t* bits, uint32_t* tile_counts,

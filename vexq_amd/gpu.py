@@ -42,6 +42,8 @@ def lib():
     L.vq_submit.argtypes = [C.c_void_p, P(abi.Query), P(abi.AggResult), P(abi.Stats),
                             P(C.c_void_p)]
     L.vq_ticket_done.argtypes = [C.c_void_p]
+    L.vq_agg_dict_multi.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_int32,
+                                    P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
     L.vq_batch_run.argtypes = [C.c_void_p, C.c_int32, P(abi.Query), P(abi.AggResult),
                                P(abi.Stats), C.c_int32]
     L.vq_wait.argtypes = [C.c_void_p]
@@ -295,6 +297,23 @@ class Context:
                                  C.byref(out.c), C.byref(st)), out)
         out.stats = st
         return out
+
+    def agg_dict_multi(self, dt: "DeviceTable", preds, aggs) -> list:
+        """vq_agg_dict_multi: global aggregates (no GROUP BY) of one shape,
+        one per predicate in `preds` (None: every row), in one pass over the
+        table.  Returns one AggOut per query."""
+        n = len(preds)
+        arr = aggs if isinstance(aggs, C.Array) else abi.agg_array(aggs)
+        outs = [abi.AggOut(1, 0, len(aggs)) for _ in range(n)]
+        parr = (abi.Pred * n)(*[p if p is not None else abi.Pred() for p in preds])
+        res = (abi.AggResult * n)(*[o.c for o in outs])
+        st = (abi.Stats * n)()
+        rc = lib().vq_agg_dict_multi(self.h, dt.h, n, parr, len(aggs), arr, res, st)
+        for i, o in enumerate(outs):
+            o.c.ngroups = res[i].ngroups
+            o.stats = st[i]
+        _check(rc)
+        return outs
 
     def agg_dict_noalloc(self, dt: "DeviceTable", pred, key_cols, aggs,
                          capacity: int = 4096) -> abi.AggOut:
