@@ -39,6 +39,9 @@ extern "C" {
 #define VQ_E_CORRUPT 6     /* a .vxq checksum or structure does not verify
                               (storage.ErrChecksum / ErrCorruptFooter /
                               ErrBadMagic, storage/errors.go)                   */
+#define VQ_E_CANCELED 7    /* the query was canceled (vq_ticket_cancel,
+                              vq_ctx_cancel): the reference's ctx.Err() check
+                              per batch (exec/scan.go:199-203)                   */
 
 /* Column physical types as they sit in HBM.  Mirrors storage type codes
  * (storage/format.go:20-26) after the decode in exec/scan.go:294-409:
@@ -461,6 +464,16 @@ int vq_ticket_done(const vq_ticket* ticket);
 /* Block until the query has finished; returns its status (its message is then
  * this thread's vq_last_error) and frees the ticket. */
 int vq_wait(vq_ticket* ticket);
+/* Cancel a query (the Go operator's ctx.Done(), as TableScan checks ctx.Err()
+ * per batch, exec/scan.go:199-203, and a failing worker cancels its siblings,
+ * exec/parallel_join.go:436-476): still queued, it is dropped and finishes
+ * with VQ_E_CANCELED at once; running, it stops at its next launch boundary
+ * (a query's kernels run for well under a millisecond each) and finishes
+ * with VQ_E_CANCELED, unless it finished first (then its result stands).
+ * The ticket stays valid until vq_wait. */
+int vq_ticket_cancel(vq_ticket* ticket);
+/* Cancel every query queued or running on the context, as vq_ticket_cancel. */
+int vq_ctx_cancel(vq_ctx* ctx);
 
 /* GROUP BY one integer column of any cardinality: HashAggregate's general
  * key path (exec/aggregate.go:361-559; the int-key path's output order,

@@ -98,3 +98,48 @@ def test_hash_groupbys_alone_and_in_the_mix(gpu_ctx, serving_tables):
         gpu_ctx.set_slots(1)
     for a, b in zip(alone, mixed):
         same(a, b)
+
+
+def _outcome(ticket, alone):
+    """A submitted query that was canceled, or that finished with the result
+    it gives alone."""
+    from vexq_amd import abi, gpu
+    try:
+        got = ticket.wait()
+    except gpu.GpuError as e:
+        assert e.code == abi.VQ_E_CANCELED, e
+        return "canceled"
+    same(got, alone)
+    return "ok"
+
+
+def test_cancel_queued_and_running(gpu_ctx, serving_tables):
+    """Queries queued behind a many-key hash GROUP BY on one slot are
+    canceled one by one (vq_ticket_cancel) and, in a second stream, all at
+    once (vq_ctx_cancel): each either finishes canceled or, when it had
+    already run, with its result alone; the queries not canceled all return
+    their results."""
+    from vexq_amd import abi
+    dim, fact, ddim, dfact = serving_tables
+    stream = datagen.serve_stream(fact, dim, 24, seed=31)
+    heavy_aggs = [abi.agg_sum("price", fact), abi.count_star(fact)]
+    gpu_ctx.set_slots(1)
+    alone = [run_sync(gpu_ctx, fact, dim, dfact, ddim, k, a) for k, a in stream]
+    heavy = gpu_ctx.agg_hash(dfact, None, fact.index("fk"), heavy_aggs)
+    outcomes = []
+    try:
+        blocker = gpu_ctx.submit_agg_hash(dfact, None, fact.index("fk"), heavy_aggs)
+        tickets = [submit(gpu_ctx, fact, dim, dfact, ddim, k, a) for k, a in stream]
+        for t in tickets[1::2]:
+            t.cancel()
+        same(blocker.wait(), heavy)
+        outcomes = [_outcome(t, a) for t, a in zip(tickets, alone)]
+        assert all(o == "ok" for o in outcomes[0::2])
+        blocker = gpu_ctx.submit_agg_hash(dfact, None, fact.index("fk"), heavy_aggs)
+        tickets = [submit(gpu_ctx, fact, dim, dfact, ddim, k, a) for k, a in stream]
+        gpu_ctx.cancel_all()
+        _outcome(blocker, heavy)
+        outcomes += [_outcome(t, a) for t, a in zip(tickets, alone)]
+    finally:
+        gpu_ctx.set_slots(1)
+    assert "canceled" in outcomes

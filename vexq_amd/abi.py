@@ -13,7 +13,7 @@ from __future__ import annotations
 import ctypes as C
 
 # ---- constants (include/vexq_gpu.h) ----------------------------------------
-VQ_OK, VQ_E_ARG, VQ_E_DEVICE, VQ_E_BUDGET, VQ_E_UNSUPPORTED, VQ_E_NOMEM, VQ_E_CORRUPT = range(7)
+VQ_OK, VQ_E_ARG, VQ_E_DEVICE, VQ_E_BUDGET, VQ_E_UNSUPPORTED, VQ_E_NOMEM, VQ_E_CORRUPT, VQ_E_CANCELED = range(8)
 VQ_U8, VQ_U16, VQ_DATE, VQ_I64, VQ_F64 = 1, 2, 3, 4, 5
 VQ_EQ, VQ_NE, VQ_LT, VQ_LE, VQ_GT, VQ_GE = range(6)
 VQ_IN = 6        # column IN (set of values in [0, 64)): ival is the mask
@@ -28,7 +28,7 @@ ROW_GROUP_ROWS = 65536
 STATUS_NAMES = {
     VQ_OK: "ok", VQ_E_ARG: "invalid argument", VQ_E_DEVICE: "device error",
     VQ_E_BUDGET: "device-memory budget exceeded", VQ_E_UNSUPPORTED: "unsupported shape",
-    VQ_E_NOMEM: "host allocation failure", VQ_E_CORRUPT: "corrupt file",
+    VQ_E_NOMEM: "host allocation failure", VQ_E_CORRUPT: "corrupt file", VQ_E_CANCELED: "canceled",
 }
 
 WIDTH = {VQ_U8: 1, VQ_U16: 2, VQ_DATE: 4, VQ_I64: 8, VQ_F64: 8}

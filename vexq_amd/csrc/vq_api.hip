@@ -125,6 +125,7 @@ struct vq_ctx {
   uint64_t serve_reserved = 0;
   int running = 0;
   bool stopping = false;
+  std::vector<Ticket*> active;  // the tickets the workers are running (under qmu)
 };
 
 struct ColDev {
@@ -608,6 +609,12 @@ int check_agg(const vq_agg& a) {
 // ===========================================================================
 // context
 // ===========================================================================
+// The cancel request of the query this thread runs for a serving worker
+// (null: a direct call, never canceled).
+thread_local const std::atomic<bool>* g_cancel = nullptr;
+bool canceled() { return g_cancel && g_cancel->load(std::memory_order_relaxed); }
+int fail_canceled() { return fail(VQ_E_CANCELED, "exec: gpu: query canceled"); }
+
 extern "C" const char* vq_last_error(void) { return g_err.c_str(); }
 
 extern "C" int vq_ctx_create(int device, uint64_t budget_bytes, vq_ctx** out) {
@@ -1080,6 +1087,7 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
 
   SlotLease S(c);
+  if (canceled()) return fail_canceled();
   hipSetDevice(c->device);
   const int NI = d.nacc + 2;
   int32_t grid = 0;
@@ -1225,6 +1233,7 @@ extern "C" int vq_agg_dict_multi(vq_ctx* c, const vq_table* t, int32_t n, const 
   const int ntw = std::max(nt, 1);
 
   SlotLease S(c);
+  if (canceled()) return fail_canceled();
   hipSetDevice(c->device);
   const int NI = d.nacc + 2;
   const int32_t grid = d.ntiles > 0 ? (int32_t)std::min<int64_t>(d.ntiles, d1.slots) : 0;
@@ -2703,10 +2712,12 @@ int join_run(JoinJob& j, vq_agg_result* out, vq_stats* stats) {
     if (int rc = pinned_get(S.get(), j.host_bytes, &hp)) return rc;
     join_bind(j, (char*)ar, (char*)hp);
     j.pin = &S->rec_pin[0];
+    if (canceled()) return fail_canceled();
     if (j.plan.direct) S->dreg_dirty = true;  // until the query has run through
     HIP_TRY(join_enqueue(j, S->stream, S->ev));
     HIP_TRY(hipStreamSynchronize(S->stream));
     if (j.plan.direct) S->dreg_dirty = false;
+    if (canceled()) return fail_canceled();
     const int rc = join_result(j, out, stats, S->ev);
     if (rc != kRetry) return rc;
   }
@@ -3137,6 +3148,7 @@ struct Ticket {
   bool done = false;
   int status = VQ_OK;
   std::string err;
+  std::atomic<bool> cancel{false};  // asked to stop (vq_ticket_cancel)
 };
 
 struct vq_ticket {
@@ -3167,7 +3179,9 @@ uint64_t estimate_bytes(const Ticket& k) {
 void run_ticket(Ticket* k) {
   const vq_query& q = k->q;
   int rc;
-  if (q.kind == VQ_Q_AGG_DICT)
+  g_cancel = &k->cancel;
+  if (k->cancel.load()) rc = fail_canceled();
+  else if (q.kind == VQ_Q_AGG_DICT)
     rc = vq_agg_dict(k->c, q.table, k->has_pred ? &k->pred : nullptr, q.nkeys,
                      k->key_cols.empty() ? nullptr : k->key_cols.data(), q.naggs,
                      k->aggs.empty() ? nullptr : k->aggs.data(), k->out, k->stats);
@@ -3179,6 +3193,7 @@ void run_ticket(Ticket* k) {
                      q.table, q.probe_key, k->has_pred ? &k->pred : nullptr, q.naggs,
                      k->aggs.empty() ? nullptr : k->aggs.data(), q.radix_bits, q.topk, k->out,
                      k->stats);
+  g_cancel = nullptr;
   std::lock_guard<std::mutex> lk(k->mu);
   k->status = rc;
   if (rc) k->err = g_err;
@@ -3218,15 +3233,23 @@ void run_shared(std::vector<Ticket*>& ks) {
     if (ks[i]->has_pred) preds[i] = ks[i]->pred;
     outs[i] = *ks[i]->out;
   }
-  const int rc = vq_agg_dict_multi(k0.c, k0.q.table, (int32_t)n, preds.data(), k0.q.naggs,
-                                   k0.aggs.empty() ? nullptr : k0.aggs.data(), outs.data(), st.data());
+  // (a canceled query's place in the pass is kept: the others' results do
+  // not depend on it; it finishes canceled)
+  bool all = true;
+  for (size_t i = 0; i < n; i++) all = all && ks[i]->cancel.load();
+  const int rc = all ? fail_canceled()
+                     : vq_agg_dict_multi(k0.c, k0.q.table, (int32_t)n, preds.data(), k0.q.naggs,
+                                         k0.aggs.empty() ? nullptr : k0.aggs.data(), outs.data(), st.data());
+  const std::string err = rc ? g_err : std::string();
   for (size_t i = 0; i < n; i++) {
     Ticket* k = ks[i];
-    k->out->ngroups = outs[i].ngroups;
+    const bool cx = k->cancel.load();
+    k->out->ngroups = cx ? 0 : outs[i].ngroups;
     if (k->stats) *k->stats = st[i];
     std::lock_guard<std::mutex> lk(k->mu);
-    k->status = rc;
-    if (rc) k->err = g_err;
+    k->status = cx ? VQ_E_CANCELED : rc;
+    if (cx) k->err = "exec: gpu: query canceled";
+    else if (rc) k->err = err;
     k->done = true;
     k->cv.notify_all();
   }
@@ -3267,6 +3290,7 @@ void worker_loop(vq_ctx* c) {
         }
       c->serve_reserved += est;
       c->running++;
+      c->active.insert(c->active.end(), ks.begin(), ks.end());
     }
     if (ks.size() > 1) run_shared(ks);
     else run_ticket(ks[0]);
@@ -3274,6 +3298,7 @@ void worker_loop(vq_ctx* c) {
       std::lock_guard<std::mutex> lk(c->qmu);
       c->serve_reserved -= est;
       c->running--;
+      for (Ticket* k : ks) c->active.erase(std::find(c->active.begin(), c->active.end(), k));
     }
     c->qcv.notify_all();
   }
@@ -3388,6 +3413,52 @@ extern "C" int vq_wait(vq_ticket* h) {
   delete k;
   delete h;
   return rc;
+}
+
+namespace {
+
+// A queued ticket dropped: finished as canceled.  (Under c->qmu.)
+void cancel_queued(Ticket* k) {
+  std::lock_guard<std::mutex> lk(k->mu);
+  k->out->ngroups = 0;
+  k->status = VQ_E_CANCELED;
+  k->err = "exec: gpu: query canceled";
+  k->done = true;
+  k->cv.notify_all();
+}
+
+}  // namespace
+
+extern "C" int vq_ticket_cancel(vq_ticket* h) {
+  if (!h || !h->t) return fail(VQ_E_ARG, "exec: gpu: NULL ticket");
+  Ticket* k = h->t;
+  vq_ctx* c = k->c;
+  {
+    std::lock_guard<std::mutex> lk(c->qmu);
+    k->cancel.store(true);
+    auto it = std::find(c->queue.begin(), c->queue.end(), k);
+    if (it != c->queue.end()) {
+      c->queue.erase(it);
+      cancel_queued(k);
+    }
+  }
+  c->qcv.notify_all();
+  return VQ_OK;
+}
+
+extern "C" int vq_ctx_cancel(vq_ctx* c) {
+  if (!c) return fail(VQ_E_ARG, "exec: gpu: NULL context");
+  {
+    std::lock_guard<std::mutex> lk(c->qmu);
+    for (Ticket* k : c->queue) {
+      k->cancel.store(true);
+      cancel_queued(k);
+    }
+    c->queue.clear();
+    for (Ticket* k : c->active) k->cancel.store(true);
+  }
+  c->qcv.notify_all();
+  return VQ_OK;
 }
 
 // ===========================================================================

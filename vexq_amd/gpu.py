@@ -42,6 +42,8 @@ def lib():
     L.vq_submit.argtypes = [C.c_void_p, P(abi.Query), P(abi.AggResult), P(abi.Stats),
                             P(C.c_void_p)]
     L.vq_ticket_done.argtypes = [C.c_void_p]
+    L.vq_ticket_cancel.argtypes = [C.c_void_p]
+    L.vq_ctx_cancel.argtypes = [C.c_void_p]
     L.vq_agg_dict_multi.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(abi.Pred), C.c_int32,
                                     P(abi.Agg), P(abi.AggResult), P(abi.Stats)]
     L.vq_batch_run.argtypes = [C.c_void_p, C.c_int32, P(abi.Query), P(abi.AggResult),
@@ -142,6 +144,10 @@ class Context:
         """A table straight from a .vxq file image (vq_table_upload_vxq):
         the file columns `cols` checked and decoded on the device."""
         return VxqTable(self, image, cols)
+
+    def cancel_all(self) -> None:
+        """vq_ctx_cancel: every queued or running submitted query."""
+        _check(lib().vq_ctx_cancel(self.h))
 
     def set_slots(self, nslots: int, serve_budget: int = 0) -> None:
         """nslots concurrent query slots; vq_submit admission bound in bytes."""
@@ -415,6 +421,12 @@ class Ticket:
             h, self.h = self.h, None
             _check(lib().vq_wait(h), self.out)
         return self.out
+
+    def cancel(self) -> None:
+        """vq_ticket_cancel: wait() then raises GpuError(VQ_E_CANCELED) unless
+        the query had finished."""
+        if self.h is not None:
+            _check(lib().vq_ticket_cancel(self.h))
 
     def __del__(self):
         # The library writes into this ticket's result arrays and stats until
