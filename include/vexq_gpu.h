@@ -74,6 +74,14 @@ typedef struct vq_col_desc {
   int32_t type;
   int32_t dict_size;
   const void* data;
+  /* Validity (exec.Vector's NullBitmap, exec/batch.go:113-121): bit r, LSB
+   * first in byte r / 8, set when row r holds a value and clear for NULL.
+   * NULL: every row holds a value.  The dictionary GROUP BY, its scan and
+   * the filter take columns with NULLs (a predicate term on a NULL is not
+   * TRUE, so the row is excluded; aggregates skip NULL inputs; a NULL group
+   * key is a group of its own); every other operator returns
+   * VQ_E_UNSUPPORTED for a query that reads one (fall back). */
+  const uint8_t* valid;
 } vq_col_desc;
 
 /* ---- predicates: a conjunction of `column OP literal` terms -------------

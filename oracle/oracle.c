@@ -197,6 +197,12 @@ static inline int64_t col_i(const vq_col_desc* c, int64_t r) {
   }
 }
 
+/* Validity (exec/batch.go:113-121, storage.IsNullBit): bit r of the LSB-first
+ * bitmap set when row r holds a value; no bitmap, every row does. */
+static inline int col_valid(const vq_col_desc* c, int64_t r) {
+  return !c->valid || ((c->valid[r >> 3] >> (r & 7)) & 1);
+}
+
 /* CastIntToFloatExpr (exec/expr.go:194-222) for integer columns. */
 static inline double col_f(const vq_col_desc* c, int64_t r) {
   if (c->type == VQ_F64) return ((const double*)c->data)[r];
@@ -228,9 +234,13 @@ static inline int cmp_f(int op, double a, double b) {
 }
 
 /* Filter over a conjunction (AndExpr, exec/expr.go:1030-1067). */
-/* One predicate term on row r. */
+/* One predicate term on row r.  A compare that reads a NULL is NULL
+ * (mergeNullBitmapsInto, exec/expr.go:290-299, 964-980), and the filter keeps
+ * only rows whose result is TRUE and valid, so the term fails. */
 static inline int term_pass(const vq_col_desc* cols, const vq_term* tm, int64_t r) {
   const vq_col_desc* c = &cols[tm->col];
+  if (!col_valid(c, r)) return 0;
+  if ((tm->op & VQ_COLCMP) && tm->op != VQ_IN && !col_valid(&cols[tm->ival], r)) return 0;
   if (tm->op == VQ_IN) {
     /* InExpr (exec/expr.go:1188-1238) over codes: the value is one of the
      * set's members, given as a mask over 0..63 */
@@ -254,10 +264,22 @@ static inline int pred_pass(const vq_col_desc* cols, const vq_pred* p, int64_t r
 }
 
 /* CaseExpr with one WHEN and a literal ELSE (exec/expr.go:1386-1435): the
- * WHEN's conjunction on row r (no CASE: always). */
+ * WHEN's conjunction on row r (no CASE: always).  mergeVectors takes the THEN
+ * value only where the condition is TRUE and valid, so a WHEN reading a NULL
+ * selects the ELSE literal. */
 static inline int when_pass(const vq_col_desc* cols, const vq_expr* e, int64_t r) {
   for (int t = 0; t < e->nwhen; t++)
     if (!term_pass(cols, &e->when[t], r)) return 0;
+  return 1;
+}
+
+/* Is the expression's value on row r non-NULL?  Arithmetic over a NULL is
+ * NULL (the operands' validity ANDed, exec/expr.go:290-299); literals always
+ * hold a value, and so does the ELSE literal of a CASE whose WHEN fails. */
+static inline int expr_valid(const vq_col_desc* cols, const vq_expr* e, int64_t r) {
+  if (!when_pass(cols, e, r)) return 1;
+  for (int k = 0; k < e->nfactors; k++)
+    if (e->f[k].kind != VQ_F_LIT && !col_valid(&cols[e->f[k].col], r)) return 0;
   return 1;
 }
 
@@ -330,7 +352,9 @@ static inline int64_t expr_i(const vq_col_desc* cols, const vq_expr* e, int64_t 
 /* zone maps and pruning                                                     */
 /* ------------------------------------------------------------------------ */
 
-/* storage/writer.go:205-348 (and 390-395 for dictionary columns). */
+/* storage/writer.go:205-348 (and 390-395 for dictionary columns): min and
+ * max over the rows that hold a value, the NULL count, and no min/max for a
+ * row group with no value at all. */
 void or_zone_maps(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
                   const int64_t* rg_rows, vq_zone* out) {
   (void)nrows;
@@ -341,49 +365,38 @@ void or_zone_maps(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t
       vq_zone z;
       memset(&z, 0, sizeof z);
       const vq_col_desc* cd = &cols[c];
-      if (n > 0) {
-        z.has_minmax = 1;
-        if (cd->type == VQ_U8 || cd->type == VQ_U16) {
-          if (cd->dict_size > 0) {
-            z.min_bits = 0;
-            z.max_bits = (uint64_t)(cd->dict_size - 1);
-          } else {
-            int64_t mn = col_i(cd, base), mx = mn;
-            for (int64_t r = base + 1; r < base + n; r++) {
-              int64_t v = col_i(cd, r);
-              if (v < mn) mn = v;
-              if (v > mx) mx = v;
-            }
-            z.min_bits = (uint64_t)mn;
-            z.max_bits = (uint64_t)mx;
-          }
-        } else if (cd->type == VQ_F64) {
-          const double* d = (const double*)cd->data;
-          double mn = d[base], mx = d[base];
-          for (int64_t r = base + 1; r < base + n; r++) {
-            if (d[r] < mn) mn = d[r];
-            if (d[r] > mx) mx = d[r];
-          }
-          memcpy(&z.min_bits, &mn, 8);
-          memcpy(&z.max_bits, &mx, 8);
-        } else if (cd->type == VQ_DATE) {
-          const int32_t* d = (const int32_t*)cd->data;
-          int32_t mn = d[base], mx = d[base];
-          for (int64_t r = base + 1; r < base + n; r++) {
-            if (d[r] < mn) mn = d[r];
-            if (d[r] > mx) mx = d[r];
-          }
-          z.min_bits = (uint64_t)(uint32_t)mn;
-          z.max_bits = (uint64_t)(uint32_t)mx;
+      int64_t seen = 0, imn = 0, imx = 0;
+      double fmn = 0, fmx = 0;
+      for (int64_t r = base; r < base + n; r++) {
+        if (!col_valid(cd, r)) {
+          z.null_count++;
+          continue;
+        }
+        if (cd->type == VQ_F64) {
+          const double v = ((const double*)cd->data)[r];
+          if (!seen || v < fmn) fmn = v;
+          if (!seen || v > fmx) fmx = v;
         } else {
-          const int64_t* d = (const int64_t*)cd->data;
-          int64_t mn = d[base], mx = d[base];
-          for (int64_t r = base + 1; r < base + n; r++) {
-            if (d[r] < mn) mn = d[r];
-            if (d[r] > mx) mx = d[r];
-          }
-          z.min_bits = (uint64_t)mn;
-          z.max_bits = (uint64_t)mx;
+          const int64_t v = col_i(cd, r);
+          if (!seen || v < imn) imn = v;
+          if (!seen || v > imx) imx = v;
+        }
+        seen++;
+      }
+      if (seen > 0) {
+        z.has_minmax = 1;
+        if ((cd->type == VQ_U8 || cd->type == VQ_U16) && cd->dict_size > 0) {
+          z.min_bits = 0;
+          z.max_bits = (uint64_t)(cd->dict_size - 1);
+        } else if (cd->type == VQ_F64) {
+          memcpy(&z.min_bits, &fmn, 8);
+          memcpy(&z.max_bits, &fmx, 8);
+        } else if (cd->type == VQ_DATE) {
+          z.min_bits = (uint64_t)(uint32_t)(int32_t)imn;
+          z.max_bits = (uint64_t)(uint32_t)(int32_t)imx;
+        } else {
+          z.min_bits = (uint64_t)imn;
+          z.max_bits = (uint64_t)imx;
         }
       }
       out[(int64_t)g * ncols + c] = z;
@@ -560,13 +573,15 @@ static inline int64_t f_bits(double d) {
   return b;
 }
 
-/* foldRow (exec/agg_intkey.go:303-361); mult folds the same row mult times
- * (a probe row meeting mult build rows yields mult join rows). */
+/* foldRow (exec/agg_intkey.go:303-361): COUNT(*) counts every row, COUNT(x)
+ * the rows where x holds a value; SUM/AVG/MIN/MAX skip a NULL argument and
+ * count the values they fold in nonNull (aggregate.go:458-546). */
 static void fold_row(group* g, const vq_col_desc* cols, int32_t naggs, const vq_agg* aggs,
                      int64_t r) {
   g->count++;
   for (int j = 0; j < naggs; j++) {
     const vq_agg* a = &aggs[j];
+    if (a->arg.nfactors > 0 && !expr_valid(cols, &a->arg, r)) continue;
     switch (a->kind) {
       case VQ_AGG_COUNT:
         g->acc[j]++;
@@ -776,12 +791,29 @@ static int check_cap(vq_agg_result* out, int64_t n) {
  * per column for <= 2 columns, 16 for <= 4, 8 for <= 8. */
 static inline int key_width(int32_t nkeys) { return nkeys <= 2 ? 32 : nkeys <= 4 ? 16 : 8; }
 
+/* A key column's code on row r; a NULL takes the code past the dictionary, a
+ * group of its own (the isNull sample of exec/aggregate.go:424-427). */
+static inline int64_t key_code(const vq_col_desc* c, int64_t r) {
+  return col_valid(c, r) ? col_i(c, r) : c->dict_size;
+}
+
 static inline int64_t pack_key(const vq_col_desc* cols, int32_t nkeys, const int32_t* key_cols,
                                int64_t r) {
   const int shift = key_width(nkeys);
   uint64_t k = 0;
-  for (int i = 0; i < nkeys; i++) k |= (uint64_t)col_i(&cols[key_cols[i]], r) << (shift * i);
+  for (int i = 0; i < nkeys; i++) k |= (uint64_t)key_code(&cols[key_cols[i]], r) << (shift * i);
   return (int64_t)k;
+}
+
+/* Result keys: a NULL group's code (past the dictionary) is reported as -1. */
+static void null_keys(const vq_col_desc* cols, int32_t nkeys, const int32_t* key_cols,
+                      vq_agg_result* out) {
+  for (int k = 0; k < nkeys; k++) {
+    const vq_col_desc* c = &cols[key_cols[k]];
+    if (!c->valid) continue;
+    for (int64_t i = 0; i < out->ngroups; i++)
+      if (out->keys[i * nkeys + k] == c->dict_size) out->keys[i * nkeys + k] = -1;
+  }
 }
 
 static void unpack_key(int64_t packed, int32_t nkeys, int64_t* key) {
@@ -797,7 +829,8 @@ static int keys_fit(const vq_col_desc* cols, int32_t nkeys, const int32_t* key_c
   const int w = key_width(nkeys);
   for (int i = 0; i < nkeys; i++) {
     const vq_col_desc* c = &cols[key_cols[i]];
-    if (c->dict_size <= 0 || (w < 32 && c->dict_size > (1 << w))) return 0;
+    const int64_t codes = (int64_t)c->dict_size + (c->valid ? 1 : 0);
+    if (c->dict_size <= 0 || (w < 32 && codes > (1 << w))) return 0;
   }
   return 1;
 }
@@ -843,8 +876,10 @@ int or_agg_dict(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t n
     if (rc == VQ_OK) emit_empty_global(naggs, aggs, out);
   } else {
     rc = check_cap(out, l.n);
-    if (rc == VQ_OK)
+    if (rc == VQ_OK) {
       for (int64_t i = 0; i < l.n; i++) emit_group(&l.g[i], nkeys, naggs, aggs, out, i);
+      null_keys(cols, nkeys, key_cols, out);
+    }
   }
   for (int64_t i = 0; i < l.n; i++) group_free(&l.g[i], naggs);
   free(l.g);
@@ -1382,8 +1417,10 @@ static int agg_parallel(int32_t ncols, const vq_col_desc* cols, int32_t nrg,
     if (rc == VQ_OK) emit_empty_global(naggs, aggs, out);
   } else {
     rc = check_cap(out, l.n);
-    if (rc == VQ_OK)
+    if (rc == VQ_OK) {
       for (int64_t i = 0; i < l.n; i++) emit_group(&l.g[ord[i].pos], nkeys, naggs, aggs, out, i);
+      if (hash_col < 0) null_keys(cols, nkeys, key_cols, out);
+    }
   }
   free(ord);
   free(l.g);

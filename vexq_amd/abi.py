@@ -40,7 +40,8 @@ class Zone(C.Structure):
 
 
 class ColDesc(C.Structure):
-    _fields_ = [("type", C.c_int32), ("dict_size", C.c_int32), ("data", C.c_void_p)]
+    _fields_ = [("type", C.c_int32), ("dict_size", C.c_int32), ("data", C.c_void_p),
+                ("valid", C.c_void_p)]
 
 
 class Term(C.Structure):
@@ -325,6 +326,12 @@ def count_star(table=None):
     return _agg(VQ_AGG_COUNT, None, table)
 
 
+def agg_count(x, table):
+    """COUNT(x): the rows where x holds a value (COUNT(*) where x reads no
+    column with NULLs)."""
+    return _agg(VQ_AGG_COUNT, x, table)
+
+
 def agg_sum(x, table):
     return _agg(VQ_AGG_SUM, x, table)
 
@@ -417,6 +424,8 @@ def col_descs(table):
         arr[i].type = c.type
         arr[i].dict_size = c.dict_size
         arr[i].data = c.data.ctypes.data
+        v = getattr(c, "valid", None)  # a LSB-first validity bitmap (uint8), or None
+        arr[i].valid = v.ctypes.data if v is not None else None
     return arr
 
 

@@ -33,10 +33,14 @@
 #define VQ_LIVEBITS 1
 #define VQ_DJ_SLOTS 6
 #define VQ_NQ 4
+#define VQ_NULLS 1
 #define VQ_NKEY 1
 #define VQ_LOOKUP 0
 #define VQ_JKEY 0
 static constexpr int kSB[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kSN[8] = {1, 0, 0, 1, 0, 0, 1, 0};
+static constexpr int kNC[8] = {3, 0, 0, 0, 0, 0, 0, 0};
+static constexpr int kACN[8] = {0, 0, 0, 0, 1, 0, 0, 0};
 static constexpr int kKSL[8] = {2, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kKSH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 static constexpr int kST[8] = {1, 1, 4, 5, 5, 5, 3, 0};
@@ -185,6 +189,9 @@ struct Args {
   // vq_dense_multi: the predicate literals of each of the VQ_NQ queries,
   // [q][VQ_NT] (floats as their bits)
   const i64* mlit;
+  // validity words of each slot's column (kSN: the slot can be NULL): bit r
+  // of word r / 64 set when row r holds a value
+  const u64* nul[8];
 };
 
 // ---- column reads: the element type is a compile-time constant -------------
@@ -342,6 +349,45 @@ __device__ __forceinline__ int group_of(const i64* rawrow, Seq<C...>) {
   return (0 + ... + (kKS[C] * (int)rawrow[C]));
 }
 
+// ---- NULLs (exec/batch.go:113-121: LSB-first validity, 1 = a value) ---------
+// vm: bit C set when slot C holds a value in the row (always, for a slot
+// whose column has no NULLs).  A predicate term on a NULL is not TRUE, so
+// the row fails the conjunction; an aggregate argument reading a NULL is NULL
+// and skipped (its count-valid accumulator, kACN, counts the others); a NULL
+// group key takes the code kNC past the dictionary.
+__host__ __device__ constexpr u32 null_slots() {
+  u32 m = 0;
+  for (int c = 0; c < VQ_NS; c++)
+    if (kSN[c]) m |= 1u << c;
+  return m;
+}
+__host__ __device__ constexpr u32 pred_mask() {
+  u32 m = 0;
+  for (int t = 0; t < VQ_NT; t++) {
+    if (kSN[kTS[t]]) m |= 1u << kTS[t];
+    if (kTO[t] >= 16 && kSN[kTS2[t]]) m |= 1u << kTS2[t];
+  }
+  return m;
+}
+template <int J>
+__host__ __device__ constexpr u32 acc_mask() {
+  u32 m = 0;
+  for (int k = 0; k < kANF[J]; k++)
+    if (kFK[J][k] != 1 && kSN[kFS[J][k]]) m |= 1u << kFS[J][k];
+  return m;
+}
+
+// Slot validity of row r (one word load per nullable slot).
+template <int... C>
+__device__ __forceinline__ u32 row_valid(const Args& a, i64 r, Seq<C...>) {
+  return (~null_slots() | ... | (kSN[C] ? (u32)((a.nul[C][r >> 6] >> (r & 63)) & 1ull) << C : 0u));
+}
+
+template <int... C>
+__device__ __forceinline__ int group_of_v(const i64* rawrow, u32 vm, Seq<C...>) {
+  return (0 + ... + (kKS[C] * (kSN[C] && !((vm >> C) & 1u) ? kNC[C] : (int)rawrow[C])));
+}
+
 // One factor of aggregate J (float fold).
 template <int J, int K>
 __device__ __forceinline__ double factor_f(const Args& a, const i64* rawrow) {
@@ -428,10 +474,18 @@ __device__ __forceinline__ u64 acc_init() {
   else return 0;
 }
 
-// Fold one row's value of aggregate J into register accumulator r.
+// Fold one row's value of aggregate J into register accumulator r (vm: the
+// row's slot validity; a NULL argument folds nothing; a count-valid
+// accumulator adds one).
 template <int J>
-__device__ __forceinline__ void fold_reg(u64& r, const Args& a, const i64* rawrow) {
+__device__ __forceinline__ void fold_reg(u64& r, const Args& a, const i64* rawrow, u32 vm = ~0u) {
   constexpr int OP = kAOP[J];
+  if constexpr (acc_mask<J>() != 0)
+    if ((vm & acc_mask<J>()) != acc_mask<J>()) return;
+  if constexpr (kACN[J]) {
+    r += 1;
+    return;
+  }
   if constexpr (kAISF[J]) {
     const double v = val_f<J>(a, rawrow);
     if constexpr (OP == 0) {
@@ -451,8 +505,14 @@ __device__ __forceinline__ void fold_reg(u64& r, const Args& a, const i64* rawro
 
 // Same, into this lane's private LDS word (no-return LDS atomic).
 template <int J>
-__device__ __forceinline__ void fold_lds(u64* w, const Args& a, const i64* rawrow) {
+__device__ __forceinline__ void fold_lds(u64* w, const Args& a, const i64* rawrow, u32 vm = ~0u) {
   constexpr int OP = kAOP[J];
+  if constexpr (acc_mask<J>() != 0)
+    if ((vm & acc_mask<J>()) != acc_mask<J>()) return;
+  if constexpr (kACN[J]) {
+    atomicAdd((unsigned long long*)w, 1ull);
+    return;
+  }
   if constexpr (kAISF[J]) {
     const double v = val_f<J>(a, rawrow);
     if constexpr (OP == 0) {
@@ -586,15 +646,15 @@ __device__ __forceinline__ void init_regs(u64* r, Seq<J...>) {
 }
 
 template <int... J>
-__device__ __forceinline__ void fold_regs(u64* r, const Args& a, const i64* row, Seq<J...>) {
-  (fold_reg<J>(r[J], a, row), ...);
+__device__ __forceinline__ void fold_regs(u64* r, const Args& a, const i64* row, Seq<J...>, u32 vm = ~0u) {
+  (fold_reg<J>(r[J], a, row, vm), ...);
 }
 
 // w points at this lane's word of accumulator 0 of the row's group; the
 // accumulator J word is kWave * J further.
 template <int... J>
-__device__ __forceinline__ void fold_ldss(u64* w, const Args& a, const i64* row, Seq<J...>) {
-  (fold_lds<J>(w + J * kWave, a, row), ...);
+__device__ __forceinline__ void fold_ldss(u64* w, const Args& a, const i64* row, Seq<J...>, u32 vm = ~0u) {
+  (fold_lds<J>(w + J * kWave, a, row, vm), ...);
 }
 
 }  // namespace vqj
@@ -647,15 +707,26 @@ extern "C" __global__ __launch_bounds__(64) void vq_dense(const Args a) {
       for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
       const i64 r = base + (i64)lane * U + u;
       if (!(r >= lo && r < rend && pass_all(a, row, seq<VQ_NT>{}))) continue;
+#if VQ_NULLS
+      // (the lane's U rows share a validity word: base is U-aligned)
+      const u32 vm = row_valid(a, r, seq<VQ_NS>{});
+      if ((vm & pred_mask()) != pred_mask()) continue;
+#else
+      const u32 vm = ~0u;
+#endif
       if constexpr (G == 1) {
         rfirst = rcnt == 0 ? (u32)r : rfirst;
         rcnt++;
-        fold_regs(racc, a, row, seq<NA>{});
+        fold_regs(racc, a, row, seq<NA>{}, vm);
       } else {
+#if VQ_NULLS
+        const int g = group_of_v(row, vm, seq<VQ_NS>{});
+#else
         const int g = group_of(row, seq<VQ_NS>{});
+#endif
         atomicAdd(&cnt[g * kWave + lane], 1u);
         atomicMin(&first[g * kWave + lane], (u32)r);
-        fold_ldss(&lds[g * NA * kWave + lane], a, row, seq<NA>{});
+        fold_ldss(&lds[g * NA * kWave + lane], a, row, seq<NA>{}, vm);
       }
     }
   }
@@ -725,7 +796,13 @@ extern "C" __global__ __launch_bounds__(64) void vq_filter_bits(const Args a) {
       i64 row[VQ_NS];
 #pragma unroll
       for (int c = 0; c < VQ_NS; c++) row[c] = raw[c][u];
-      const bool p = base + u * kWave + lane < it.rend && pass_all(a, row, seq<VQ_NT>{});
+      bool p = base + u * kWave + lane < it.rend && pass_all(a, row, seq<VQ_NT>{});
+#if VQ_NULLS
+      if (p) {
+        const u32 vm = row_valid(a, base + u * kWave + lane, seq<VQ_NS>{});
+        p = (vm & pred_mask()) == pred_mask();
+      }
+#endif
       const u64 m = __ballot(p);
       count += (u32)__popcll(m);
       if (lane == u) mine = m;

@@ -132,6 +132,7 @@ struct ColDev {
   int32_t type;
   int32_t dict_size;
   void* d;
+  uint64_t* nul = nullptr;  // validity words (bit r of word r / 64: row r holds a value); null: no NULLs
 };
 
 struct vq_table {
@@ -330,6 +331,9 @@ struct Carve {
 };
 
 // ---- zone maps: storage/writer.go:205-348 -----------------------------------
+// Zone maps as storage/writer.go:205-348 computes them: per row group and
+// column the minimum and maximum of the rows that hold a value (raw bits),
+// and the NULL count (no value at all: has_minmax 0).
 void compute_zones(const vq_col_desc* cols, int32_t ncols, int32_t nrg,
                    const std::vector<int64_t>& rg_start, const std::vector<int64_t>& rg_rows,
                    std::vector<vq_zone>& out) {
@@ -339,59 +343,51 @@ void compute_zones(const vq_col_desc* cols, int32_t ncols, int32_t nrg,
     for (int32_t c = 0; c < ncols; c++) {
       vq_zone& z = out[(size_t)g * ncols + c];
       if (n <= 0) continue;
-      z.has_minmax = 1;
       const vq_col_desc& cd = cols[c];
-      switch (cd.type) {
-        case VQ_U8:
-        case VQ_U16: {
-          if (cd.dict_size > 0) {
-            z.min_bits = 0;
-            z.max_bits = (uint64_t)(cd.dict_size - 1);
-          } else {
-            int64_t mn = INT64_MAX, mx = INT64_MIN;
-            for (int64_t r = b; r < b + n; r++) {
-              int64_t v = cd.type == VQ_U8 ? ((const uint8_t*)cd.data)[r] : ((const uint16_t*)cd.data)[r];
-              mn = std::min(mn, v);
-              mx = std::max(mx, v);
-            }
-            z.min_bits = (uint64_t)mn;
-            z.max_bits = (uint64_t)mx;
-          }
-          break;
-        }
-        case VQ_DATE: {
-          const int32_t* d = (const int32_t*)cd.data;
-          int32_t mn = d[b], mx = d[b];
-          for (int64_t r = b + 1; r < b + n; r++) {
-            if (d[r] < mn) mn = d[r];
-            if (d[r] > mx) mx = d[r];
-          }
-          z.min_bits = (uint64_t)(uint32_t)mn;
-          z.max_bits = (uint64_t)(uint32_t)mx;
-          break;
-        }
-        case VQ_I64: {
-          const int64_t* d = (const int64_t*)cd.data;
-          int64_t mn = d[b], mx = d[b];
-          for (int64_t r = b + 1; r < b + n; r++) {
-            if (d[r] < mn) mn = d[r];
-            if (d[r] > mx) mx = d[r];
-          }
-          z.min_bits = (uint64_t)mn;
-          z.max_bits = (uint64_t)mx;
-          break;
-        }
-        default: {
-          const double* d = (const double*)cd.data;
-          double mn = d[b], mx = d[b];
-          for (int64_t r = b + 1; r < b + n; r++) {
-            if (d[r] < mn) mn = d[r];
-            if (d[r] > mx) mx = d[r];
-          }
-          memcpy(&z.min_bits, &mn, 8);
-          memcpy(&z.max_bits, &mx, 8);
-        }
+      auto has = [&](int64_t r) { return !cd.valid || ((cd.valid[r >> 3] >> (r & 7)) & 1); };
+      int64_t nn = 0;
+      for (int64_t r = b; r < b + n; r++) nn += has(r) ? 0 : 1;
+      z.null_count = nn;
+      if (nn == n) continue;  // no value in the row group
+      z.has_minmax = 1;
+      if ((cd.type == VQ_U8 || cd.type == VQ_U16) && cd.dict_size > 0) {
+        z.min_bits = 0;
+        z.max_bits = (uint64_t)(cd.dict_size - 1);
+        continue;
       }
+      if (cd.type == VQ_F64) {
+        const double* d = (const double*)cd.data;
+        bool any = false;
+        double mn = 0, mx = 0;
+        for (int64_t r = b; r < b + n; r++) {
+          if (!has(r)) continue;
+          if (!any || d[r] < mn) mn = d[r];
+          if (!any || d[r] > mx) mx = d[r];
+          any = true;
+        }
+        memcpy(&z.min_bits, &mn, 8);
+        memcpy(&z.max_bits, &mx, 8);
+        continue;
+      }
+      auto val = [&](int64_t r) -> int64_t {
+        switch (cd.type) {
+          case VQ_U8: return ((const uint8_t*)cd.data)[r];
+          case VQ_U16: return ((const uint16_t*)cd.data)[r];
+          case VQ_DATE: return ((const int32_t*)cd.data)[r];
+          default: return ((const int64_t*)cd.data)[r];
+        }
+      };
+      bool any = false;
+      int64_t mn = 0, mx = 0;
+      for (int64_t r = b; r < b + n; r++) {
+        if (!has(r)) continue;
+        const int64_t v = val(r);
+        if (!any || v < mn) mn = v;
+        if (!any || v > mx) mx = v;
+        any = true;
+      }
+      z.min_bits = cd.type == VQ_DATE ? (uint64_t)(uint32_t)(int32_t)mn : (uint64_t)mn;
+      z.max_bits = cd.type == VQ_DATE ? (uint64_t)(uint32_t)(int32_t)mx : (uint64_t)mx;
     }
   }
 }
@@ -578,7 +574,24 @@ void fill_slot_cols(const vq_table* t, const SlotMap& sm, QSlot* slots) {
   for (int i = 0; i < sm.n; i++) {
     slots[i].p = t->cols[sm.col_of[i]].d;
     slots[i].type = t->cols[sm.col_of[i]].type;
+    slots[i].nul = t->cols[sm.col_of[i]].nul;
   }
+}
+
+// A query's columns, none with NULLs: the operators other than the
+// dictionary GROUP BY and the filter (the reference's eligibility check
+// falls back to the Go operators when ZoneMap.NullCount > 0).
+int refuse_nulls(const vq_table* t, const SlotMap& sm) {
+  for (int i = 0; i < sm.n; i++)
+    if (t->cols[sm.col_of[i]].nul)
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: column %d has NULLs (this operator takes none)", sm.col_of[i]);
+  return VQ_OK;
+}
+
+bool any_nulls(const vq_table* t, const SlotMap& sm) {
+  for (int i = 0; i < sm.n; i++)
+    if (t->cols[sm.col_of[i]].nul) return true;
+  return false;
 }
 
 int64_t slot_bytes_per_row(const vq_table* t, const SlotMap& sm) {
@@ -698,6 +711,16 @@ extern "C" int vq_table_upload(vq_ctx* c, int64_t nrows, int32_t ncols, const vq
       return fail(VQ_E_ARG, "exec: gpu: column %d: bad dictionary size %d", i, cols[i].dict_size);
     bytes += (((uint64_t)nrows * w + 255) & ~255ULL) + 256;
   }
+  // columns with NULLs: their validity words (padded: kernels read the word
+  // of any row of a chunk)
+  std::vector<uint8_t> nullable((size_t)ncols, 0);
+  const size_t vwords = (size_t)(nrows + 63) / 64 + 8;
+  for (int32_t i = 0; i < ncols; i++) {
+    if (!cols[i].valid) continue;
+    for (int64_t r = 0; r < nrows && !nullable[i]; r++)
+      if (!((cols[i].valid[r >> 3] >> (r & 7)) & 1)) nullable[i] = 1;
+    if (nullable[i]) bytes += vwords * 8;
+  }
   // dictionary codes must index their dictionary
   for (int32_t i = 0; i < ncols; i++) {
     if (cols[i].dict_size <= 0) continue;
@@ -734,9 +757,18 @@ extern "C" int vq_table_upload(vq_ctx* c, int64_t nrows, int32_t ncols, const vq
     // padded: kernels read the aligned 8-byte word holding an element
     hipError_t e = hipMalloc(&cd.d, ((nb + 255) & ~(size_t)255) + 256);
     if (e == hipSuccess && nb) e = hipMemcpy(cd.d, cols[i].data, nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nullable[i]) {
+      e = hipMalloc((void**)&cd.nul, vwords * 8);
+      if (e == hipSuccess) e = hipMemset(cd.nul, 0, vwords * 8);
+      if (e == hipSuccess) e = hipMemcpy(cd.nul, cols[i].valid, (size_t)(nrows + 7) / 8, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
       if (cd.d) hipFree(cd.d);
-      for (auto& x : t->cols) hipFree(x.d);
+      if (cd.nul) hipFree(cd.nul);
+      for (auto& x : t->cols) {
+        hipFree(x.d);
+        if (x.nul) hipFree(x.nul);
+      }
       release(c, bytes);
       delete t;
       return fail(VQ_E_DEVICE, "exec: gpu: upload of column %d: %s", i, hipGetErrorString(e));
@@ -754,7 +786,10 @@ extern "C" int vq_table_destroy(vq_table* t) {
   hipSetDevice(t->ctx->device);
   // the caller ends every query on the table first; finish any in flight
   for (Slot* S : t->ctx->slots) hipStreamSynchronize(S->stream);
-  for (auto& x : t->cols) hipFree(x.d);
+  for (auto& x : t->cols) {
+    hipFree(x.d);
+    if (x.nul) hipFree(x.nul);
+  }
   release(t->ctx, t->bytes);
   {
     std::lock_guard<std::mutex> lk(t->ctx->mu);
@@ -885,6 +920,11 @@ int64_t kept_rows(const vq_table* t, const std::vector<uint8_t>& keep) {
 
 struct AccMap {
   int32_t acc[VQ_MAX_AGGS];  // accumulator index per aggregate (-1 = COUNT)
+  // an aggregate whose argument can be NULL: the accumulator counting its
+  // rows that hold a value (-1: every row does)
+  int32_t nn[VQ_MAX_AGGS];
+  // a key column with NULLs: the code its NULL takes (-1: none)
+  int32_t null_code[4];
 };
 
 // *over is set when the query is well formed but its group slots exceed what
@@ -906,14 +946,19 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
   if (naggs == 0 && nkeys == 0)
     return fail(VQ_E_ARG, "exec: hash aggregate: no group-by columns or aggregate expressions");
   if (int rc = add_pred(t, pred, sm, d.slot)) return rc;
+  for (int j = 0; j < VQ_MAX_AGGS; j++) am.nn[j] = -1;
+  for (int i = 0; i < 4; i++) am.null_code[i] = -1;
   int64_t G = 1;
   for (int i = 0; i < nkeys; i++) {
     if (int rc = check_col(t, key_cols[i])) return rc;
     const ColDev& c = t->cols[key_cols[i]];
     if ((c.type != VQ_U8 && c.type != VQ_U16) || c.dict_size <= 0)
       return fail(VQ_E_UNSUPPORTED, "exec: gpu: group-by column %d is not a dictionary code", key_cols[i]);
-    card[i] = c.dict_size;
-    G = std::min<int64_t>(G * c.dict_size, (int64_t)1 << 40);  // (no overflow past 4 columns)
+    // a NULL key is a group of its own (exec/aggregate.go:424-427): the code
+    // past the dictionary
+    card[i] = c.dict_size + (c.nul ? 1 : 0);
+    if (c.nul) am.null_code[i] = c.dict_size;
+    G = std::min<int64_t>(G * card[i], (int64_t)1 << 40);  // (no overflow past 4 columns)
   }
   if (G > kDenseGroupsMax) {
     if (over) *over = true;
@@ -924,33 +969,66 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
     int s = sm.get(key_cols[i]);
     if (s < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
     d.slot[s].key_stride += (int32_t)stride;
+    d.slot[s].null_code = am.null_code[i] >= 0 ? am.null_code[i] : 0;
     stride *= card[i];
   }
+  // does an expression read a column with NULLs (its value can be NULL)?
+  auto nullable_expr = [&](const vq_expr& e) {
+    for (int k = 0; k < e.nfactors; k++)
+      if (e.f[k].kind != VQ_F_LIT && e.f[k].col >= 0 && e.f[k].col < t->ncols && t->cols[e.f[k].col].nul)
+        return true;
+    return false;
+  };
+  auto nullable_when = [&](const vq_expr& e) {
+    for (int k = 0; k < e.nwhen; k++) {
+      const vq_term& w = e.when[k];
+      if (w.col >= 0 && w.col < t->ncols && t->cols[w.col].nul) return true;
+      if ((w.op & VQ_COLCMP) && w.ival >= 0 && w.ival < t->ncols && t->cols[(int32_t)w.ival].nul) return true;
+    }
+    return false;
+  };
+  auto add_acc = [&](const QAcc& qa) -> int {
+    for (int k = 0; k < d.nacc; k++)
+      if (memcmp(&d.acc[k], &qa, sizeof qa) == 0) return k;
+    if (d.nacc == kMaxAcc) return -1;
+    d.acc[d.nacc] = qa;
+    return d.nacc++;
+  };
   d.ngroups = (int32_t)G;
-  // accumulators, de-duplicated (SUM(price) and AVG(price) share one)
+  // accumulators, de-duplicated (SUM(price) and AVG(price) share one); an
+  // argument that can be NULL also gets a count of the rows where it holds
+  // a value (the reference's nonNull count, exec/aggregate.go:458-546)
   for (int j = 0; j < naggs; j++) {
     if (int rc = check_agg(aggs[j])) return rc;
     am.acc[j] = -1;
-    if (aggs[j].kind == VQ_AGG_COUNT) continue;
+    const vq_expr& ex = aggs[j].arg;
+    const bool nullable = nullable_expr(ex);
+    // (a CASE whose WHEN fails takes its ELSE literal even where the THEN
+    // value is NULL: the kernels' per-argument validity mask does not model
+    // that, so CASE over columns with NULLs is left to the fallback)
+    if (nullable_when(ex) || (ex.nwhen > 0 && nullable))
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: CASE WHEN over a column with NULLs");
+    if (aggs[j].kind == VQ_AGG_COUNT && (ex.nfactors == 0 || !nullable)) continue;  // COUNT(*)
     QAcc qa;
     memset(&qa, 0, sizeof qa);
-    if (int rc = add_expr(t, aggs[j].arg, sm, qa.e)) return rc;
-    const bool f = aggs[j].arg.is_float;
+    if (int rc = add_expr(t, ex, sm, qa.e)) return rc;
+    if (nullable) {
+      QAcc qn = qa;
+      qn.op = ACC_SUM_I;
+      qn.count_valid = 1;
+      am.nn[j] = add_acc(qn);
+      if (am.nn[j] < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d accumulators", kMaxAcc);
+    }
+    if (aggs[j].kind == VQ_AGG_COUNT) continue;  // COUNT(x): the count of values alone
+    const bool f = ex.is_float;
     switch (aggs[j].kind) {
       case VQ_AGG_SUM:
       case VQ_AGG_AVG: qa.op = f ? ACC_SUM_F : ACC_SUM_I; break;
       case VQ_AGG_MIN: qa.op = f ? ACC_MIN_F : ACC_MIN_I; break;
       default: qa.op = f ? ACC_MAX_F : ACC_MAX_I; break;
     }
-    int found = -1;
-    for (int k = 0; k < d.nacc; k++)
-      if (memcmp(&d.acc[k], &qa, sizeof qa) == 0) found = k;
-    if (found < 0) {
-      if (d.nacc == kMaxAcc) return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d accumulators", kMaxAcc);
-      d.acc[d.nacc] = qa;
-      found = d.nacc++;
-    }
-    am.acc[j] = found;
+    am.acc[j] = add_acc(qa);
+    if (am.acc[j] < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d accumulators", kMaxAcc);
   }
   fill_slot_cols(t, sm, d.slot);
   d.nslots = sm.n;
@@ -1009,7 +1087,8 @@ int dense_result(const uint64_t* hfin, const QDesc& d, const AccMap& am, const i
     const int64_t cnt = (int64_t)r[d.nacc];
     int64_t rem = g;
     for (int k = nkeys - 1; k >= 0; k--) {
-      out->keys[i * nkeys + k] = rem % card[k];
+      const int64_t code = rem % card[k];
+      out->keys[i * nkeys + k] = am.null_code[k] >= 0 && code == am.null_code[k] ? -1 : code;  // -1: NULL
       rem /= card[k];
     }
     out->counts[i] = cnt;
@@ -1020,8 +1099,15 @@ int dense_result(const uint64_t* hfin, const QDesc& d, const AccMap& am, const i
       out->fvals[o] = 0;
       out->valid[o] = 1;
       const vq_agg& a = aggs[j];
+      // rows whose argument holds a value: SUM/AVG/MIN/MAX of none are NULL
+      // (exec/aggregate.go:780-876); COUNT(x) counts them
+      const int64_t vals = am.nn[j] >= 0 ? (int64_t)r[am.nn[j]] : cnt;
       if (a.kind == VQ_AGG_COUNT) {
-        out->ivals[o] = cnt;
+        out->ivals[o] = vals;
+        continue;
+      }
+      if (vals == 0) {
+        out->valid[o] = 0;
         continue;
       }
       uint64_t w = r[am.acc[j]];
@@ -1031,7 +1117,7 @@ int dense_result(const uint64_t* hfin, const QDesc& d, const AccMap& am, const i
       }
       if (a.kind == VQ_AGG_AVG) {
         const double s = a.arg.is_float ? bits_to_f64((int64_t)w) : (double)(int64_t)w;
-        out->fvals[o] = s / (double)cnt;
+        out->fvals[o] = s / (double)vals;
       } else if (a.accum == VQ_F64) {
         out->fvals[o] = bits_to_f64((int64_t)w);
       } else {
@@ -1073,6 +1159,8 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   DenseJit dj;
   std::string jerr;
   const bool jit = jit_dense_prepare(d, &dj, &jerr) == 0;
+  if (!jit && any_nulls(t, sm))  // (the generic kernels take no NULLs)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: NULLs without the run-time compiled kernel: %s", jerr.c_str());
   {
     const int64_t kr = kept_rows(t, keep);
     if (jit && kr >= (int64_t)1 << 20) {
@@ -1190,7 +1278,7 @@ extern "C" int vq_agg_dict_multi(vq_ctx* c, const vq_table* t, int32_t n, const 
   bool one_shape = true;
   for (int32_t q = 0; q < n; q++) {
     if (int rc = plan_dense(t, pred_of(q), 0, nullptr, naggs, aggs, dq[q], am[q], smv[q], card)) return rc;
-    one_shape = one_shape && same_shape(dq[0], dq[q]);
+    one_shape = one_shape && same_shape(dq[0], dq[q]) && !any_nulls(t, smv[q]);
   }
   int32_t nq = 1;
   while (nq < n) nq *= 2;  // (one compiled kernel per power of two)
@@ -1335,6 +1423,7 @@ extern "C" int vq_agg_dict_noalloc(vq_ctx* c, const vq_table* t, const vq_pred* 
   SlotMap sm;
   int32_t card[4] = {1, 1, 1, 1};
   if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card)) return rc;
+  if (int rc = refuse_nulls(t, sm)) return rc;  // (the generic kernels take no NULLs)
   const int NI = d.nacc + 2;
   // final words [G][NI] first (256-byte aligned), then partials [grid][G][NI]
   const size_t fin_words = (size_t)d.ngroups * NI;
@@ -1412,6 +1501,9 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
   d.tile_rows = scan_tile_rows(kept_rows(t, keep));
   int64_t rows = 0, kept = 0;
   if (int rc = make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, &rows, &kept)) return rc;
+  // (columns with NULLs: the run-time compiled kernel only)
+  if (any_nulls(t, sm) && !jit_available_filter(d, nullptr))
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: NULLs without the run-time compiled filter");
 
   SlotLease S(c);
   hipSetDevice(c->device);
@@ -1651,7 +1743,7 @@ int plan_side(const vq_table* t, int32_t key, const vq_pred* pred, const vq_agg*
   }
   fill_slot_cols(t, sm, sd.slot);
   sd.nslots = sm.n;
-  return VQ_OK;
+  return refuse_nulls(t, sm);
 }
 
 }  // namespace
@@ -2809,6 +2901,7 @@ int lookup_setup(JoinJob& j) {
   if (int rc = add_pred(b, j.lbp, sm, d.slot)) return rc;
   j.lkey_slot = sm.get(j.lbuild_key);
   if (j.lkey_slot < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: build side touches more than %d columns", kMaxSlots);
+  if (int rc = refuse_nulls(b, sm)) return rc;
   fill_slot_cols(b, sm, d.slot);
   d.nslots = sm.n;
   d.ngroups = 1;
@@ -2940,6 +3033,7 @@ int scan_desc(const vq_table* t, const vq_pred* pred, int nkeys, const int32_t* 
     key_slots[i] = sm.get(keys[i]);
     if (key_slots[i] < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
   }
+  if (int rc = refuse_nulls(t, sm)) return rc;
   fill_slot_cols(t, sm, d.slot);
   d.nslots = sm.n;
   d.ngroups = 1;

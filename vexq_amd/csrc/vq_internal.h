@@ -46,8 +46,9 @@ struct QSlot {
   int32_t type;
   int32_t nterms;
   int32_t key_stride;  // dense group slot contribution (0 = not a key)
-  int32_t pad_;
+  int32_t null_code;   // a key column with NULLs: the code a NULL key takes (its dictionary size)
   QTerm terms[kMaxSlotTerms];
+  const uint64_t* nul; // validity words (bit r of word r / 64 set: row r holds a value); null: no NULLs
 };
 
 struct QFactor {
@@ -82,7 +83,7 @@ struct QExpr {
 
 struct QAcc {
   int32_t op;
-  int32_t pad_;
+  int32_t count_valid;  // 1: counts the rows whose e holds a value (SUM_I of 1s), for a nullable e
   QExpr e;
 };
 
@@ -493,6 +494,8 @@ struct JitArgs {
   int32_t dj_pad;
   // vq_dense_multi: each query's predicate literals, [q][terms]
   const int64_t* mlit;
+  // validity words of each slot's column (null: no NULLs)
+  const uint64_t* nul[8];
 };
 
 // A join side's partition kernels, resolved once per query (vq_jit.hip):

@@ -109,12 +109,18 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
                      const int* lop = nullptr, const int* lsrc = nullptr, int nag = 0,
                      int livebits = 0, const JSide* keys = nullptr) {
   int st[8] = {0}, ks[8] = {0}, aop[8] = {0}, aisf[8] = {0}, anf[8] = {0};
+  int sn[8] = {0}, nc[8] = {0}, acn[8] = {0};
+  int nulls = 0;
   int fk[8][4] = {{0}}, fs[8][4] = {{0}}, fo[8][4] = {{0}}, fc[8][4] = {{0}};
   for (int c = 0; c < d.nslots; c++) {
     st[c] = d.slot[c].type;
     ks[c] = d.slot[c].key_stride;
+    sn[c] = d.slot[c].nul ? 1 : 0;
+    nc[c] = d.slot[c].null_code;
+    nulls |= sn[c];
   }
   for (int j = 0; j < d.nacc; j++) {
+    acn[j] = d.acc[j].count_valid;
     aop[j] = d.acc[j].op;
     aisf[j] = is_float_op(d.acc[j].op);
     anf[j] = d.acc[j].e.nfactors;
@@ -139,6 +145,7 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_BLOOM " + std::to_string(bloom) + "\n";
   s += "#define VQ_LIVEBITS " + std::to_string(livebits) + "\n";
   s += "#define VQ_DJ_SLOTS " + std::to_string(kDjSlots) + "\n";
+  s += "#define VQ_NULLS " + std::to_string(nulls) + "\n";
   // the key: one column (key_slot), or several dictionary codes packed
   int nkey = 1, ksl[8] = {key_slot}, ksh[8] = {0};
   if (keys && keys->nkeys > 1) {
@@ -157,6 +164,9 @@ std::string prologue(const QDesc& d, const Flat& f, int kernels = 3, int key_slo
   s += "#define VQ_LOOKUP " + std::to_string(lookup ? 1 : 0) + "\n";
   s += "#define VQ_JKEY " + std::to_string(lookup ? keys->jkey_slot : 0) + "\n";
   arr(s, "kSB", sb, 8);
+  arr(s, "kSN", sn, 8);
+  arr(s, "kNC", nc, 8);
+  arr(s, "kACN", acn, 8);
   arr(s, "kKSL", ksl, 8);
   arr(s, "kKSH", ksh, 8);
   arr(s, "kST", st, 8);
@@ -339,7 +349,10 @@ Module* get_module(const std::string& pro, std::string* err) {
 
 void fill_common(const QDesc& d, const Flat& f, JitArgs& a) {
   memset(&a, 0, sizeof a);
-  for (int c = 0; c < d.nslots; c++) a.col[c] = d.slot[c].p;
+  for (int c = 0; c < d.nslots; c++) {
+    a.col[c] = d.slot[c].p;
+    a.nul[c] = d.slot[c].nul;
+  }
   for (int t = 0; t < f.nall; t++) {
     a.ti[t] = f.ti[t];
     a.tf[t] = f.tf[t];

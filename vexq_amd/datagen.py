@@ -77,6 +77,25 @@ class Column:
     data: np.ndarray
     dict_size: int = 0
     dictionary: list = field(default_factory=list)
+    valid: np.ndarray | None = None  # LSB-first validity bitmap (uint8); None: no NULLs
+
+
+def validity_bitmap(mask: np.ndarray) -> np.ndarray:
+    """Row mask (True = holds a value) -> LSB-first bitmap bytes, as the
+    reference's NullBitmap (exec/batch.go:113-121), padded to whole words."""
+    m = np.asarray(mask, dtype=bool)
+    pad = (-m.size) % 64
+    b = np.packbits(np.concatenate([m, np.zeros(pad, dtype=bool)]), bitorder="little")
+    return np.ascontiguousarray(b)
+
+
+def with_nulls(col: Column, null_mask: np.ndarray) -> Column:
+    """The column with the rows of null_mask set to NULL (their values zeroed,
+    as the reference's vectors hold zero under a NULL bit)."""
+    data = col.data.copy()
+    data[np.asarray(null_mask, dtype=bool)] = 0
+    return Column(col.name, col.type, data, col.dict_size, col.dictionary,
+                  validity_bitmap(~np.asarray(null_mask, dtype=bool)))
 
 
 @dataclass
@@ -98,8 +117,14 @@ class Table:
         return self.columns[self.index(name)].data
 
     def take(self, order: np.ndarray) -> "Table":
+        def valid(c):
+            if c.valid is None:
+                return None
+            m = np.unpackbits(c.valid, bitorder="little")[: self.nrows].astype(bool)
+            return validity_bitmap(m[order])
+
         cols = [Column(c.name, c.type, np.ascontiguousarray(c.data[order]), c.dict_size,
-                       list(c.dictionary)) for c in self.columns]
+                       list(c.dictionary), valid(c)) for c in self.columns]
         return Table(cols, row_groups(len(order)))
 
 

@@ -290,8 +290,9 @@ class Context:
         if out is None:
             if capacity is None:  # at most one group per code tuple, and per row
                 space = 1
-                for k in key_cols:
-                    space *= max(1, dt.table.columns[k].dict_size)
+                for k in key_cols:  # (a key with NULLs: one group more, the NULL one)
+                    c = dt.table.columns[k]
+                    space *= max(1, c.dict_size + (getattr(c, "valid", None) is not None))
                 capacity = max(1, min(space, dt.nrows))
             out = abi.AggOut(capacity, len(key_cols), len(aggs))
         elif out.nkeys != len(key_cols) or out.naggs != len(aggs):
