@@ -116,6 +116,41 @@ bool IntKeyMultiRowGroupDictInstability() {
   return true;
 }
 
+// NULLs through the operator: a NULL group key is a group of its own
+// (exec/aggregate.go:424-427), emitted as a NULL row of the key column;
+// SUM skips a NULL argument and COUNT(col) counts the values
+// (aggregate.go:458-546), COUNT(*) every row.
+bool NullKeysAndArguments() {
+  static const uint8_t flag[] = {0, 1, 0, 0, 1, 0};     // R A R - A -
+  static const uint8_t flag_valid[8] = {0x17};          // rows 3 and 5 NULL
+  static const int64_t value[] = {10, 20, 30, 40, 50, 60};
+  static const uint8_t value_valid[8] = {0x3B};         // row 2 NULL
+  HostTable h;
+  h.columns = {col("flag", DataType::String, VQ_U8, flag, dict({"R", "A"})),
+               col("value", DataType::Int64, VQ_I64, value)};
+  h.columns[0].valid = flag_valid;
+  h.columns[1].valid = value_valid;
+  h.row_groups = {6};
+  h.nrows = 6;
+  std::unique_ptr<GpuTable> t;
+  CHECK_OK(GpuTable::Upload(g_dev.get(), h, &t));
+  auto op = NewGpuParallelHashAggregate(t.get(), nullptr, {0},
+                                        {agg(AggKind::Sum, 1, DataType::Int64, "s"),
+                                         agg(AggKind::Count, 1, DataType::Int64, "cv"),
+                                         agg(AggKind::Count, -1, DataType::Int64, "c")});
+  std::vector<Batch> bs;
+  CHECK_OK(Drain(op.get(), &bs));
+  CHECK(bs.size() == 1 && bs[0].length == 3);
+  const Batch& b = bs[0];
+  CHECK(!b.vectors[0].IsNull(0) && b.vectors[0].Get(0) == "R");
+  CHECK(!b.vectors[0].IsNull(1) && b.vectors[0].Get(1) == "A");
+  CHECK(b.vectors[0].IsNull(2));
+  CHECK(b.vectors[1].i64[0] == 10 && b.vectors[2].i64[0] == 1 && b.vectors[3].i64[0] == 2);
+  CHECK(b.vectors[1].i64[1] == 70 && b.vectors[2].i64[1] == 2 && b.vectors[3].i64[1] == 2);
+  CHECK(b.vectors[1].i64[2] == 100 && b.vectors[2].i64[2] == 2 && b.vectors[3].i64[2] == 2);
+  return true;
+}
+
 // TestAccumulateSelVecFourPaths (exec/aggregate_rows_test.go:249-322); the
 // selection vectors {0,2,4,6} / {7} become the predicate sel == 1.
 bool AccumulateSelVecFourPaths() {
@@ -556,6 +591,7 @@ int main() {
     return 1;
   }
   run("TestIntKeyMultiRowGroupDictInstability", IntKeyMultiRowGroupDictInstability);
+  run("TestNullKeysAndArguments", NullKeysAndArguments);
   run("TestAccumulateSelVecFourPaths", AccumulateSelVecFourPaths);
   run("TestMergePartialAggNonNullCounts", MergePartialAggNonNullCounts);
   run("TestEmptyGlobalAggregate", EmptyGlobalAggregate);

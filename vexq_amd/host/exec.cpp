@@ -36,7 +36,12 @@ vq_agg to_vq(const HostTable& t, const AggExpr& a, const HostTable* build = null
     case AggKind::Max: g.kind = VQ_AGG_MAX; break;
   }
   g.accum = a.accum_type == DataType::Float64 ? VQ_F64 : VQ_I64;
-  if (a.kind != AggKind::Count) g.arg = expr_of(t, a, build);
+  // COUNT(col) of a column with NULLs counts the rows holding a value
+  // (exec/aggregate.go:466-473); over any other column it is COUNT(*)
+  const HostTable& src = a.build_side && build ? *build : t;
+  const bool count_nullable = a.kind == AggKind::Count && a.expr.nfactors == 0 && a.col_idx >= 0 &&
+                              a.col_idx < (int)src.columns.size() && src.columns[a.col_idx].valid;
+  if (a.kind != AggKind::Count || count_nullable) g.arg = expr_of(t, a, build);
   return g;
 }
 
@@ -67,7 +72,11 @@ Batch agg_batch(const Schema& schema, const vq_agg_result& r, int nkeys, int nag
     v.type = key_types[k];
     for (int64_t i = lo; i < hi; i++) {
       if (v.type == DataType::String) {
-        v.codes.push_back((uint32_t)r.keys[i * nkeys + k]);
+        // a NULL key's group (key -1): a NULL row of the key column
+        const int64_t code = r.keys[i * nkeys + k];
+        if (code < 0 && v.valid.empty()) v.valid.assign((size_t)(i - lo), 1);
+        if (!v.valid.empty()) v.valid.push_back(code >= 0);
+        v.codes.push_back(code < 0 ? 0u : (uint32_t)code);
       } else if (v.type == DataType::Float64) {  // (a float column riding with a key: its bits)
         double d;
         memcpy(&d, &r.keys[i * nkeys + k], 8);
@@ -226,7 +235,9 @@ class GpuParallelHashAggregate : public Operator {
     if (!done_) {
       const HostTable& h = t_->host();
       int64_t cap = 1;
-      for (int c : group_by_) cap *= std::max<int64_t>(1, h.columns[c].dict ? (int64_t)h.columns[c].dict->size() : 1);
+      for (int c : group_by_)  // (a key with NULLs: one group more, the NULL one)
+        cap *= std::max<int64_t>(1, (h.columns[c].dict ? (int64_t)h.columns[c].dict->size() : 1) +
+                                        (h.columns[c].valid ? 1 : 0));
       std::vector<vq_agg> va;
       for (const AggExpr& a : aggs_) va.push_back(to_vq(h, a));
       ResultBufs rb;
@@ -309,10 +320,11 @@ Device::~Device() { vq_ctx_destroy(ctx_); }
 Status GpuTable::Upload(Device* dev, const HostTable& host, std::unique_ptr<GpuTable>* out) {
   std::vector<vq_col_desc> cols;
   for (const HostColumn& c : host.columns) {
-    vq_col_desc d;
+    vq_col_desc d{};
     d.type = vq_type_of(c);
     d.dict_size = c.dict ? (int32_t)c.dict->size() : 0;
     d.data = c.data;
+    d.valid = c.valid;
     cols.push_back(d);
   }
   std::unique_ptr<GpuTable> t(new GpuTable());

@@ -108,6 +108,9 @@ struct HostColumn {
   int vq_type = VQ_I64;  // physical type in HBM
   const void* data = nullptr;
   std::shared_ptr<const std::vector<std::string>> dict;
+  // validity (the NullBitmap of exec/batch.go:113-121): bit r LSB-first, set
+  // when row r holds a value; nullptr when every row does
+  const uint8_t* valid = nullptr;
 };
 
 struct HostTable {
