@@ -178,12 +178,23 @@ typedef struct vq_expr {
 
 /* Aggregates: exec.AggExpr (exec/aggregate.go:25-37).  accum is the
  * AccumType rule of AccumTypeFor (aggregate.go:51-72): VQ_F64 for AVG and for
- * SUM/MIN/MAX over a float argument, VQ_I64 otherwise. */
+ * SUM/MIN/MAX over a float argument, VQ_STR for MIN/MAX over a dictionary
+ * (STRING) column, VQ_I64 otherwise. */
 #define VQ_AGG_COUNT 0
 #define VQ_AGG_SUM 1
 #define VQ_AGG_AVG 2
 #define VQ_AGG_MIN 3
 #define VQ_AGG_MAX 4
+
+/* accum of MIN/MAX over a STRING column (AccumType TypeString, the strAccs
+ * of exec/aggregate.go:98-110, folded by minMaxString, aggregate.go:670-730):
+ * the argument is that one dictionary column; the result, in ivals, is the
+ * global code of the group's smallest / largest string as Go compares
+ * strings (bytewise), valid 0 for a group with no value.  The dictionary
+ * GROUP BY and its global aggregates take it once the column's byte order
+ * is known (vq_table_dict_order; a table from vq_table_upload_vxq knows its
+ * dictionaries); every other operator returns VQ_E_UNSUPPORTED. */
+#define VQ_STR 6
 
 #define VQ_MAX_AGGS 8
 typedef struct vq_agg {
@@ -294,6 +305,16 @@ int vq_table_column(const vq_table* t, int32_t col, int32_t* type, int32_t* dict
  * owned by the table, valid until it is destroyed. */
 int vq_table_dict_entry(const vq_table* t, int32_t col, int32_t code, const char** str,
                         int64_t* len);
+/* The byte order of a dictionary column's strings, for MIN/MAX with accum
+ * VQ_STR: rank[code] is the position of code's string among the column's
+ * dict_size strings sorted as Go compares strings (bytewise), a permutation
+ * of 0 .. dict_size - 1.  The library keeps a column of ranks on the device
+ * (2 bytes a row, under the context's budget) until the table is destroyed;
+ * calling again replaces the order.  Not needed for a table from
+ * vq_table_upload_vxq (its first VQ_STR query orders the dictionary it
+ * carries).  No counterpart in the reference, whose strAccs compare the
+ * strings themselves (exec/aggregate.go:670-730). */
+int vq_table_dict_order(vq_table* t, int32_t col, const uint32_t* rank);
 
 /* Zone-map pruning alone: writes 1/0 per row group into keep (nrg entries).
  * Restates TableScan's zonePred check (exec/scan.go:211-214) with the

@@ -573,6 +573,33 @@ static inline int64_t f_bits(double d) {
   return b;
 }
 
+/* ---- strings of dictionary columns (MIN/MAX over strings) --------------- */
+static int32_t g_ndicts;
+static const char* const* const* g_strs;
+static const int64_t* const* g_lens;
+
+void or_set_dicts(int32_t ncols, const char* const* const* strs, const int64_t* const* lens) {
+  g_ndicts = ncols;
+  g_strs = strs;
+  g_lens = lens;
+}
+
+/* Go's string compare (bytewise, a prefix first) of two codes' strings. */
+static int str_cmp(int32_t col, int64_t a, int64_t b) {
+  if (col < 0 || col >= g_ndicts || !g_strs[col]) return (a > b) - (a < b); /* (no strings: codes) */
+  const int64_t la = g_lens[col][a], lb = g_lens[col][b];
+  const int c = memcmp(g_strs[col][a], g_strs[col][b], (size_t)(la < lb ? la : lb));
+  if (c) return c;
+  return (la > lb) - (la < lb);
+}
+
+/* minMaxString (exec/aggregate.go:670-730): the group's first value, then
+ * the smaller (MIN) or larger (MAX) string; acc holds the winner's code. */
+static void str_minmax(group* g, int j, const vq_agg* a, int64_t code) {
+  const int c = g->nonnull[j] ? str_cmp(a->arg.f[0].col, code, g->acc[j]) : 0;
+  if (!g->nonnull[j] || (a->kind == VQ_AGG_MIN ? c < 0 : c > 0)) g->acc[j] = code;
+}
+
 /* foldRow (exec/agg_intkey.go:303-361): COUNT(*) counts every row, COUNT(x)
  * the rows where x holds a value; SUM/AVG/MIN/MAX skip a NULL argument and
  * count the values they fold in nonNull (aggregate.go:458-546). */
@@ -606,6 +633,11 @@ static void fold_row(group* g, const vq_col_desc* cols, int32_t naggs, const vq_
       }
       case VQ_AGG_MIN:
       case VQ_AGG_MAX: {
+        if (a->accum == VQ_STR) {
+          str_minmax(g, j, a, col_i(&cols[a->arg.f[0].col], r));
+          g->nonnull[j]++;
+          break;
+        }
         g->nonnull[j]++;
         int less = a->kind == VQ_AGG_MIN;
         if (a->accum == VQ_F64) {
@@ -638,6 +670,10 @@ static void merge_group(group* dst, const group* src, int32_t naggs, const vq_ag
         break;
       default: {
         int less = a->kind == VQ_AGG_MIN;
+        if (a->accum == VQ_STR) {
+          if (src->nonnull[j]) str_minmax(dst, j, a, src->acc[j]);
+          break;
+        }
         if (a->accum == VQ_F64) {
           double d = bits_f(dst->acc[j]), s = bits_f(src->acc[j]);
           if ((less && s < d) || (!less && s > d)) dst->acc[j] = src->acc[j];

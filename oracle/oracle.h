@@ -30,6 +30,12 @@ void or_xorshift_i64(int64_t n, uint64_t seed, int64_t* out);
 /* Correctly rounded sum of n doubles (exact superaccumulator). */
 double or_exact_sum(const double* x, int64_t n);
 
+/* The strings of the dictionary columns, for MIN/MAX over strings (accum
+ * VQ_STR): strs[col][code], lens[col][code] bytes; strs[col] NULL for a
+ * column without one.  Set before a call that folds such an aggregate and
+ * cleared with ncols 0 after it (one caller at a time). */
+void or_set_dicts(int32_t ncols, const char* const* const* strs, const int64_t* const* lens);
+
 /* storage/writer.go:205-348 zone statistics, nrg*ncols row-group-major. */
 void or_zone_maps(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t nrg,
                   const int64_t* rg_rows, vq_zone* out);

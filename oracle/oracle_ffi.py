@@ -45,6 +45,7 @@ def lib():
     L.or_xorshift_i64.argtypes = [C.c_int64, C.c_uint64, P(C.c_int64)]
     L.or_exact_sum.argtypes = [P(C.c_double), C.c_int64]
     L.or_exact_sum.restype = C.c_double
+    L.or_set_dicts.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
     L.or_zone_maps.argtypes = [C.c_int64, C.c_int32, P(abi.ColDesc), C.c_int32, P(C.c_int64),
                                P(abi.Zone)]
     L.or_zone_prune.argtypes = [C.c_int32, P(abi.ColDesc), C.c_int32, P(abi.Zone), P(abi.Pred),
@@ -138,6 +139,38 @@ def filter_rows(table, pred, zones=None, nthreads: int = 0) -> np.ndarray:
     return out[:n].copy()
 
 
+class _Dicts:
+    """The table's dictionary strings handed to the oracle for the length of
+    one call (MIN/MAX over strings compares them)."""
+
+    def __init__(self, table, aggs):
+        self.on = any(a.accum == abi.VQ_STR for a in aggs)
+        if not self.on:
+            return
+        n = len(table.columns)
+        self.strs = (C.c_void_p * n)()
+        self.lens = (C.c_void_p * n)()
+        self.keep = []
+        for i, c in enumerate(table.columns):
+            if not c.dictionary:
+                continue
+            bs = [abi._as_bytes(x) for x in c.dictionary]
+            sa = (C.c_char_p * len(bs))(*bs)
+            la = (C.c_int64 * len(bs))(*[len(b) for b in bs])
+            self.keep += [bs, sa, la]
+            self.strs[i] = C.cast(sa, C.c_void_p)
+            self.lens[i] = C.cast(la, C.c_void_p)
+
+    def __enter__(self):
+        if self.on:
+            lib().or_set_dicts(len(self.strs), C.cast(self.strs, C.c_void_p), C.cast(self.lens, C.c_void_p))
+        return self
+
+    def __exit__(self, *a):
+        if self.on:
+            lib().or_set_dicts(0, None, None)
+
+
 def agg_dict(table, pred, key_cols, aggs, exact: bool = True, capacity: int = 1 << 16,
              zones=None, nthreads: int = 0) -> abi.AggOut:
     """Serial restatement (nthreads == 0) or the morsel-parallel CPU baseline."""
@@ -147,14 +180,15 @@ def agg_dict(table, pred, key_cols, aggs, exact: bool = True, capacity: int = 1 
     keys = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
     arr = abi.agg_array(aggs)
     out = abi.AggOut(capacity, len(key_cols), len(aggs))
-    if nthreads:
-        rc = lib().or_agg_dict_parallel(table.nrows, len(table.columns), descs, len(rg), rgp, zones,
-                                        _pp(pred), len(key_cols), keys, len(aggs), arr, nthreads,
-                                        C.byref(out.c))
-    else:
-        rc = lib().or_agg_dict(table.nrows, len(table.columns), descs, len(rg), rgp, zones,
-                               _pp(pred), len(key_cols), keys, len(aggs), arr, int(exact),
-                               C.byref(out.c))
+    with _Dicts(table, aggs):
+        if nthreads:
+            rc = lib().or_agg_dict_parallel(table.nrows, len(table.columns), descs, len(rg), rgp, zones,
+                                            _pp(pred), len(key_cols), keys, len(aggs), arr, nthreads,
+                                            C.byref(out.c))
+        else:
+            rc = lib().or_agg_dict(table.nrows, len(table.columns), descs, len(rg), rgp, zones,
+                                   _pp(pred), len(key_cols), keys, len(aggs), arr, int(exact),
+                                   C.byref(out.c))
     if rc:
         raise RuntimeError(f"oracle agg_dict failed: {rc}")
     return out

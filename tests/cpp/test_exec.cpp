@@ -151,6 +151,36 @@ bool NullKeysAndArguments() {
   return true;
 }
 
+// MIN/MAX over a STRING column (minMaxString, exec/aggregate.go:670-730):
+// bytewise string order, not the codes' first-occurrence order; a NULL
+// string skipped.
+bool StringMinMax() {
+  static const uint8_t name[] = {0, 1, 2, 3, 0, 2, 1, 3};  // mango a Zebra ab mango Zebra a ab
+  static const uint8_t name_valid[8] = {0xF7};              // row 3 NULL
+  static const uint8_t grp[] = {0, 0, 0, 0, 1, 1, 2, 2};
+  HostTable h;
+  h.columns = {col("name", DataType::String, VQ_U8, name, dict({"mango", "a", "Zebra", "ab"})),
+               col("grp", DataType::String, VQ_U8, grp, dict({"x", "y", "z"}))};
+  h.columns[0].valid = name_valid;
+  h.row_groups = {8};
+  h.nrows = 8;
+  std::unique_ptr<GpuTable> t;
+  CHECK_OK(GpuTable::Upload(g_dev.get(), h, &t));
+  auto op = NewGpuParallelHashAggregate(t.get(), nullptr, {1},
+                                        {agg(AggKind::Min, 0, DataType::String, "mn"),
+                                         agg(AggKind::Max, 0, DataType::String, "mx")});
+  std::vector<Batch> bs;
+  CHECK_OK(Drain(op.get(), &bs));
+  CHECK(bs.size() == 1 && bs[0].length == 3);
+  const Batch& b = bs[0];
+  // x: mango a Zebra (ab NULL) -> Zebra / mango; y: mango Zebra; z: a ab
+  CHECK(b.vectors[0].Get(0) == "x" && b.vectors[1].Get(0) == "Zebra" && b.vectors[2].Get(0) == "mango");
+  CHECK(b.vectors[0].Get(1) == "y" && b.vectors[1].Get(1) == "Zebra" && b.vectors[2].Get(1) == "mango");
+  CHECK(b.vectors[0].Get(2) == "z" && b.vectors[1].Get(2) == "a" && b.vectors[2].Get(2) == "ab");
+  CHECK(!b.vectors[1].IsNull(0) && !b.vectors[2].IsNull(2));
+  return true;
+}
+
 // TestAccumulateSelVecFourPaths (exec/aggregate_rows_test.go:249-322); the
 // selection vectors {0,2,4,6} / {7} become the predicate sel == 1.
 bool AccumulateSelVecFourPaths() {
@@ -592,6 +622,7 @@ int main() {
   }
   run("TestIntKeyMultiRowGroupDictInstability", IntKeyMultiRowGroupDictInstability);
   run("TestNullKeysAndArguments", NullKeysAndArguments);
+  run("TestStringMinMax", StringMinMax);
   run("TestAccumulateSelVecFourPaths", AccumulateSelVecFourPaths);
   run("TestMergePartialAggNonNullCounts", MergePartialAggNonNullCounts);
   run("TestEmptyGlobalAggregate", EmptyGlobalAggregate);

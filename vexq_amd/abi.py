@@ -15,6 +15,7 @@ import ctypes as C
 # ---- constants (include/vexq_gpu.h) ----------------------------------------
 VQ_OK, VQ_E_ARG, VQ_E_DEVICE, VQ_E_BUDGET, VQ_E_UNSUPPORTED, VQ_E_NOMEM, VQ_E_CORRUPT, VQ_E_CANCELED = range(8)
 VQ_U8, VQ_U16, VQ_DATE, VQ_I64, VQ_F64 = 1, 2, 3, 4, 5
+VQ_STR = 6  # accumulator of MIN/MAX over a dictionary (string) column
 VQ_EQ, VQ_NE, VQ_LT, VQ_LE, VQ_GT, VQ_GE = range(6)
 VQ_IN = 6        # column IN (set of values in [0, 64)): ival is the mask
 VQ_COLCMP = 16   # op | VQ_COLCMP: column OP column ival
@@ -346,6 +347,36 @@ def agg_min(x, table):
 
 def agg_max(x, table):
     return _agg(VQ_AGG_MAX, x, table)
+
+
+def _agg_str(kind, name, table) -> Agg:
+    a = _agg(kind, name, table)
+    a.accum = VQ_STR
+    return a
+
+
+def agg_min_str(name, table):
+    """MIN over the strings of dictionary column `name` (Go's bytewise string
+    order); the result is the code of the smallest string."""
+    return _agg_str(VQ_AGG_MIN, name, table)
+
+
+def agg_max_str(name, table):
+    """MAX over the strings of dictionary column `name`."""
+    return _agg_str(VQ_AGG_MAX, name, table)
+
+
+def dict_rank(dictionary) -> "_np.ndarray":
+    """rank[code]: the position of code's string among the dictionary's
+    strings in bytewise order (what vq_table_dict_order takes)."""
+    order = sorted(range(len(dictionary)), key=lambda c: _as_bytes(dictionary[c]))
+    rank = _np.empty(len(dictionary), dtype=_np.uint32)
+    rank[order] = _np.arange(len(dictionary), dtype=_np.uint32)
+    return rank
+
+
+def _as_bytes(s):
+    return s if isinstance(s, (bytes, bytearray)) else str(s).encode()
 
 
 def agg_array(aggs):

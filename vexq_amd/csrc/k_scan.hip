@@ -986,6 +986,30 @@ hipError_t launch_zero_words(uint64_t* p, uint64_t words, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Ranks of a dictionary column's rows: out[r] = rank[code[r]], the position
+// of the row's string in byte order (MIN/MAX over strings fold ranks).  The
+// codes were checked against the dictionary at upload, so every code
+// indexes rank.
+template <typename C>
+__global__ __launch_bounds__(256) void k_rank_map(const C* __restrict__ codes, const uint16_t* __restrict__ rank,
+                                                   uint16_t* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    out[i] = rank[codes[i]];
+}
+
+hipError_t launch_rank_map(const void* codes, int32_t type, const uint16_t* rank, uint16_t* out, int64_t n,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>(4096, (n + 255) / 256);
+  if (type == VQ_U8)
+    hipLaunchKernelGGL(k_rank_map<uint8_t>, dim3((unsigned)blocks), dim3(256), 0, s, (const uint8_t*)codes, rank,
+                       out, n);
+  else
+    hipLaunchKernelGGL(k_rank_map<uint16_t>, dim3((unsigned)blocks), dim3(256), 0, s, (const uint16_t*)codes, rank,
+                       out, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_dj_build(const QDesc& d, int32_t key_slot, const DirectBufs& D, hipStream_t s) {
   if (d.ntiles > 0)
     hipLaunchKernelGGL((k_dj_mark<kU>), dim3((unsigned)d.ntiles), dim3(kWave), 0, s, d, key_slot, D);

@@ -147,6 +147,13 @@ struct vq_table {
   // the strings of each dictionary column's global codes (tables ingested
   // from a .vxq image; empty for uploads of decoded columns)
   std::vector<std::vector<std::string>> dicts;
+  // per dictionary column, its strings' byte order for MIN/MAX over strings:
+  // a device column of ranks (u16 a row; d null until the order is known)
+  // and the code of each rank
+  std::vector<ColDev> rank;
+  std::vector<std::vector<uint32_t>> code_of_rank;
+  uint64_t rank_bytes = 0;
+  std::mutex rank_mu;
 };
 
 struct vq_selvec {
@@ -196,6 +203,8 @@ int table_adopt(vq_ctx* c, TableParts&& p, vq_table** out) {
   }
   t->zones = std::move(p.zones);
   t->dicts = std::move(p.dicts);
+  t->rank.assign(t->ncols, ColDev{VQ_U16, 0, nullptr});
+  t->code_of_rank.resize(t->ncols);
   *out = t;
   return VQ_OK;
 }
@@ -570,11 +579,19 @@ int add_expr(const vq_table* t, const vq_expr& e, SlotMap& sm, QExpr& out) {
   return VQ_OK;
 }
 
+// A slot's column: a column of the table, or (kRankCol + c) the device
+// column of ranks of dictionary column c (MIN/MAX over strings).
+constexpr int32_t kRankCol = 1 << 24;
+const ColDev& col_dev(const vq_table* t, int32_t c) {
+  return c >= kRankCol ? t->rank[c - kRankCol] : t->cols[c];
+}
+
 void fill_slot_cols(const vq_table* t, const SlotMap& sm, QSlot* slots) {
   for (int i = 0; i < sm.n; i++) {
-    slots[i].p = t->cols[sm.col_of[i]].d;
-    slots[i].type = t->cols[sm.col_of[i]].type;
-    slots[i].nul = t->cols[sm.col_of[i]].nul;
+    const ColDev& cd = col_dev(t, sm.col_of[i]);
+    slots[i].p = cd.d;
+    slots[i].type = cd.type;
+    slots[i].nul = cd.nul;
   }
 }
 
@@ -583,20 +600,20 @@ void fill_slot_cols(const vq_table* t, const SlotMap& sm, QSlot* slots) {
 // falls back to the Go operators when ZoneMap.NullCount > 0).
 int refuse_nulls(const vq_table* t, const SlotMap& sm) {
   for (int i = 0; i < sm.n; i++)
-    if (t->cols[sm.col_of[i]].nul)
+    if (col_dev(t, sm.col_of[i]).nul)
       return fail(VQ_E_UNSUPPORTED, "exec: gpu: column %d has NULLs (this operator takes none)", sm.col_of[i]);
   return VQ_OK;
 }
 
 bool any_nulls(const vq_table* t, const SlotMap& sm) {
   for (int i = 0; i < sm.n; i++)
-    if (t->cols[sm.col_of[i]].nul) return true;
+    if (col_dev(t, sm.col_of[i]).nul) return true;
   return false;
 }
 
 int64_t slot_bytes_per_row(const vq_table* t, const SlotMap& sm) {
   int64_t w = 0;
-  for (int i = 0; i < sm.n; i++) w += width_of(t->cols[sm.col_of[i]].type);
+  for (int i = 0; i < sm.n; i++) w += width_of(col_dev(t, sm.col_of[i]).type);
   return w;
 }
 
@@ -607,10 +624,18 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // AccumTypeFor (exec/aggregate.go:51-72) agreement check.
-int check_agg(const vq_agg& a) {
+// strings: the operator folds MIN/MAX over strings (VQ_STR; its planner
+// checks the argument).
+int check_agg(const vq_agg& a, bool strings = false) {
   if (a.kind < VQ_AGG_COUNT || a.kind > VQ_AGG_MAX) return fail(VQ_E_ARG, "exec: gpu: bad aggregate kind %d", a.kind);
   if (a.kind == VQ_AGG_COUNT) return VQ_OK;
   if (a.arg.nfactors <= 0) return fail(VQ_E_ARG, "exec: gpu: aggregate %d needs an argument", a.kind);
+  if (a.accum == VQ_STR) {
+    if (a.kind != VQ_AGG_MIN && a.kind != VQ_AGG_MAX)
+      return fail(VQ_E_ARG, "exec: gpu: a string accumulator for aggregate kind %d", a.kind);
+    if (!strings) return fail(VQ_E_UNSUPPORTED, "exec: gpu: MIN/MAX over strings on this operator");
+    return VQ_OK;
+  }
   const int want = a.kind == VQ_AGG_AVG ? VQ_F64 : (a.arg.is_float ? VQ_F64 : VQ_I64);
   if (a.accum != want)
     return fail(VQ_E_ARG, "exec: gpu: accumulator type %d disagrees with AccumTypeFor (%d)", a.accum, want);
@@ -777,6 +802,8 @@ extern "C" int vq_table_upload(vq_ctx* c, int64_t nrows, int32_t ncols, const vq
   }
   if (zones) t->zones.assign(zones, zones + (size_t)nrg * ncols);
   else compute_zones(cols, ncols, nrg, t->rg_start, t->rg_rows, t->zones);
+  t->rank.assign(ncols, ColDev{VQ_U16, 0, nullptr});
+  t->code_of_rank.resize(ncols);
   *out = t;
   return VQ_OK;
 }
@@ -790,7 +817,8 @@ extern "C" int vq_table_destroy(vq_table* t) {
     hipFree(x.d);
     if (x.nul) hipFree(x.nul);
   }
-  release(t->ctx, t->bytes);
+  for (auto& x : t->rank) hipFree(x.d);  // (their validity is the column's)
+  release(t->ctx, t->bytes + t->rank_bytes);
   {
     std::lock_guard<std::mutex> lk(t->ctx->mu);
     char pre[32];
@@ -918,8 +946,77 @@ int64_t kept_rows(const vq_table* t, const std::vector<uint8_t>& keep) {
   return n;
 }
 
+// The device column of ranks of dictionary column `col` (its strings' byte
+// order), built from rank (rank[code], a permutation of the codes) on the
+// first call, or again when rank is given.  (Under the table's rank lock.)
+int build_rank_column(vq_table* t, int32_t col, const uint32_t* rank) {
+  const ColDev& c = t->cols[col];
+  const int32_t n = c.dict_size;
+  std::vector<uint32_t> inv((size_t)n, UINT32_MAX);
+  std::vector<uint16_t> r16((size_t)n);
+  for (int32_t k = 0; k < n; k++) {
+    if (rank[k] >= (uint32_t)n || inv[rank[k]] != UINT32_MAX)
+      return fail(VQ_E_ARG, "exec: gpu: column %d: the ranks are not a permutation of its %d codes", col, n);
+    inv[rank[k]] = (uint32_t)k;
+    r16[k] = (uint16_t)rank[k];
+  }
+  hipSetDevice(t->ctx->device);
+  ColDev& rc = t->rank[col];
+  if (!rc.d) {
+    const size_t bytes = (((size_t)t->nrows * 2 + 255) & ~(size_t)255) + 256;
+    if (!reserve(t->ctx, bytes))
+      return fail(VQ_E_BUDGET, "exec: gpu: the string order of column %d exceeds the device budget", col);
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      release(t->ctx, bytes);
+      return fail(VQ_E_DEVICE, "exec: gpu: allocation of the string order of column %d", col);
+    }
+    if (hipMemset(p, 0, bytes) != hipSuccess) {
+      hipFree(p);
+      release(t->ctx, bytes);
+      return fail(VQ_E_DEVICE, "exec: gpu: clearing the string order of column %d", col);
+    }
+    t->rank_bytes += bytes;
+    rc.d = p;
+  }
+  rc.type = VQ_U16;
+  rc.dict_size = 0;
+  rc.nul = c.nul;
+  uint16_t* dr = nullptr;
+  hipError_t e = hipMalloc((void**)&dr, std::max<size_t>(2, (size_t)n * 2));
+  if (e == hipSuccess) e = hipMemcpy(dr, r16.data(), (size_t)n * 2, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_rank_map(c.d, c.type, dr, (uint16_t*)rc.d, t->nrows, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (dr) hipFree(dr);
+  if (e != hipSuccess) return fail(VQ_E_DEVICE, "exec: gpu: string order of column %d: %s", col, hipGetErrorString(e));
+  t->code_of_rank[col] = std::move(inv);
+  return VQ_OK;
+}
+
+// Rank column of dictionary column `col` for MIN/MAX over strings: built
+// from the dictionary a .vxq table carries if not yet known; `build` false
+// (a caller that cannot allocate): only an order already set.
+int rank_column(const vq_table* tc, int32_t col, bool build) {
+  vq_table* t = const_cast<vq_table*>(tc);
+  std::lock_guard<std::mutex> lk(t->rank_mu);
+  if (t->rank[col].d && !t->code_of_rank[col].empty()) return VQ_OK;
+  const bool have_dict = (size_t)col < t->dicts.size() && (int64_t)t->dicts[col].size() == t->cols[col].dict_size;
+  if (!build || !have_dict)
+    return fail(VQ_E_UNSUPPORTED, "exec: gpu: MIN/MAX over the strings of column %d: their order is not set "
+                "(vq_table_dict_order)", col);
+  const std::vector<std::string>& dict = t->dicts[col];
+  std::vector<uint32_t> idx(dict.size()), rank(dict.size());
+  for (size_t k = 0; k < idx.size(); k++) idx[k] = (uint32_t)k;
+  // std::string's order compares bytes as unsigned, as Go's string compare
+  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return dict[a] < dict[b]; });
+  for (size_t k = 0; k < idx.size(); k++) rank[idx[k]] = (uint32_t)k;
+  return build_rank_column(t, col, rank.data());
+}
+
 struct AccMap {
   int32_t acc[VQ_MAX_AGGS];  // accumulator index per aggregate (-1 = COUNT)
+  // MIN/MAX over strings: the dictionary code of each rank (null: not one)
+  const uint32_t* str_code[VQ_MAX_AGGS];
   // an aggregate whose argument can be NULL: the accumulator counting its
   // rows that hold a value (-1: every row does)
   int32_t nn[VQ_MAX_AGGS];
@@ -932,7 +1029,7 @@ struct AccMap {
 // more than kDenseGroupsMax slots, or more LDS than kDenseLdsMax).
 int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int32_t* key_cols,
                int32_t naggs, const vq_agg* aggs, QDesc& d, AccMap& am, SlotMap& sm,
-               int32_t card[4], bool* over = nullptr) {
+               int32_t card[4], bool* over = nullptr, bool build_order = true) {
   memset(&d, 0, sizeof d);
   if (over) *over = false;
   if (nkeys < 0 || nkeys > 8 || (nkeys > 0 && !key_cols))
@@ -946,7 +1043,10 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
   if (naggs == 0 && nkeys == 0)
     return fail(VQ_E_ARG, "exec: hash aggregate: no group-by columns or aggregate expressions");
   if (int rc = add_pred(t, pred, sm, d.slot)) return rc;
-  for (int j = 0; j < VQ_MAX_AGGS; j++) am.nn[j] = -1;
+  for (int j = 0; j < VQ_MAX_AGGS; j++) {
+    am.nn[j] = -1;
+    am.str_code[j] = nullptr;
+  }
   for (int i = 0; i < 4; i++) am.null_code[i] = -1;
   int64_t G = 1;
   for (int i = 0; i < nkeys; i++) {
@@ -999,9 +1099,43 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
   // argument that can be NULL also gets a count of the rows where it holds
   // a value (the reference's nonNull count, exec/aggregate.go:458-546)
   for (int j = 0; j < naggs; j++) {
-    if (int rc = check_agg(aggs[j])) return rc;
+    if (int rc = check_agg(aggs[j], true)) return rc;
     am.acc[j] = -1;
     const vq_expr& ex = aggs[j].arg;
+    if (aggs[j].accum == VQ_STR) {
+      // MIN/MAX over strings (minMaxString, exec/aggregate.go:670-730): the
+      // integer MIN/MAX of the rows' ranks in byte order, mapped back to a
+      // code; a NULL string is skipped like any NULL argument
+      const vq_factor& f = ex.f[0];
+      if (ex.nfactors != 1 || f.kind != VQ_F_COL || ex.nwhen != 0)
+        return fail(VQ_E_ARG, "exec: gpu: MIN/MAX over strings takes one column");
+      if (int rc = check_col(t, f.col)) return rc;
+      const ColDev& sc = t->cols[f.col];
+      if ((sc.type != VQ_U8 && sc.type != VQ_U16) || sc.dict_size <= 0)
+        return fail(VQ_E_ARG, "exec: gpu: MIN/MAX over strings of column %d, not a dictionary column", f.col);
+      if (int rc = rank_column(t, f.col, build_order)) return rc;
+      QAcc qa;
+      memset(&qa, 0, sizeof qa);
+      qa.e.nfactors = 1;
+      qa.e.f[0].kind = VQ_F_COL;
+      qa.e.f[0].op = VQ_ADD;
+      qa.e.f[0].combine = VQ_ADD;
+      qa.e.f[0].slot = sm.get(kRankCol + f.col);
+      if (qa.e.f[0].slot < 0)
+        return fail(VQ_E_UNSUPPORTED, "exec: gpu: query touches more than %d columns", kMaxSlots);
+      if (sc.nul) {
+        QAcc qn = qa;
+        qn.op = ACC_SUM_I;
+        qn.count_valid = 1;
+        am.nn[j] = add_acc(qn);
+        if (am.nn[j] < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d accumulators", kMaxAcc);
+      }
+      qa.op = aggs[j].kind == VQ_AGG_MIN ? ACC_MIN_I : ACC_MAX_I;
+      am.acc[j] = add_acc(qa);
+      if (am.acc[j] < 0) return fail(VQ_E_UNSUPPORTED, "exec: gpu: more than %d accumulators", kMaxAcc);
+      am.str_code[j] = t->code_of_rank[f.col].data();
+      continue;
+    }
     const bool nullable = nullable_expr(ex);
     // (a CASE whose WHEN fails takes its ELSE literal even where the THEN
     // value is NULL: the kernels' per-argument validity mask does not model
@@ -1111,6 +1245,10 @@ int dense_result(const uint64_t* hfin, const QDesc& d, const AccMap& am, const i
         continue;
       }
       uint64_t w = r[am.acc[j]];
+      if (am.str_code[j]) {  // a rank in byte order -> its string's code
+        out->ivals[o] = (int64_t)am.str_code[j][w];
+        continue;
+      }
       if ((a.kind == VQ_AGG_MIN || a.kind == VQ_AGG_MAX) && a.accum == VQ_F64) {
         const int64_t o = (int64_t)w;  // order-preserving int64 -> IEEE bits
         w = (uint64_t)(o >= 0 ? o : (int64_t)((uint64_t)o ^ 0x7FFFFFFFFFFFFFFFULL));
@@ -1410,6 +1548,16 @@ int scratch_setup(int device) {
 
 }  // namespace
 
+extern "C" int vq_table_dict_order(vq_table* t, int32_t col, const uint32_t* rank) {
+  if (!t || !rank) return fail(VQ_E_ARG, "exec: gpu: NULL argument");
+  if (int rc = check_col(t, col)) return rc;
+  const ColDev& c = t->cols[col];
+  if ((c.type != VQ_U8 && c.type != VQ_U16) || c.dict_size <= 0)
+    return fail(VQ_E_ARG, "exec: gpu: column %d is not a dictionary column", col);
+  std::lock_guard<std::mutex> lk(t->rank_mu);
+  return build_rank_column(t, col, rank);
+}
+
 extern "C" int vq_agg_dict_noalloc(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
                                    const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                                    vq_agg_result* out, vq_stats* stats) {
@@ -1422,7 +1570,8 @@ extern "C" int vq_agg_dict_noalloc(vq_ctx* c, const vq_table* t, const vq_pred* 
   AccMap am;
   SlotMap sm;
   int32_t card[4] = {1, 1, 1, 1};
-  if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card)) return rc;
+  // (no allocation here: MIN/MAX over strings only with their order already set)
+  if (int rc = plan_dense(t, pred, nkeys, key_cols, naggs, aggs, d, am, sm, card, nullptr, false)) return rc;
   if (int rc = refuse_nulls(t, sm)) return rc;  // (the generic kernels take no NULLs)
   const int NI = d.nacc + 2;
   // final words [G][NI] first (256-byte aligned), then partials [grid][G][NI]

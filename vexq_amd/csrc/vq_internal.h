@@ -559,6 +559,8 @@ hipError_t launch_direct(const QDesc& bq, int32_t bkey_slot, const DirectLaunch&
 hipError_t launch_dj_build(const QDesc& d, int32_t key_slot, const DirectBufs& D, hipStream_t s);
 // Clears a direct region of `words` 64-bit words.
 hipError_t launch_zero_words(uint64_t* p, uint64_t words, hipStream_t s);
+hipError_t launch_rank_map(const void* codes, int32_t type, const uint16_t* rank, uint16_t* out, int64_t n,
+                           hipStream_t s);
 constexpr int32_t kLocalLds = 60 * 1024;  // vq_hash_local's dynamic LDS
 int jit_local_prepare(const JSide& sd, const JDesc& jd, int32_t lcap, LocalLaunch* out,
                       std::string* err);

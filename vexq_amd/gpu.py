@@ -57,6 +57,7 @@ def lib():
                                       P(C.c_void_p), P(abi.Stats)]
     L.vq_table_column.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32)]
     L.vq_table_dict_entry.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_char_p), P(C.c_int64)]
+    L.vq_table_dict_order.argtypes = [C.c_void_p, C.c_int32, P(C.c_uint32)]
     L.vq_table_destroy.argtypes = [C.c_void_p]
     L.vq_table_rows.argtypes = [C.c_void_p]
     L.vq_table_rows.restype = C.c_int64
@@ -463,6 +464,15 @@ class DeviceTable:
         if self.h:
             lib().vq_table_destroy(self.h)
             self.h = None
+
+    def dict_order(self, col: int, rank=None) -> None:
+        """vq_table_dict_order: the byte order of dictionary column col's
+        strings (by default from the host table's dictionary), for MIN/MAX
+        over strings."""
+        if rank is None:
+            rank = abi.dict_rank(self.table.columns[col].dictionary)
+        r = np.ascontiguousarray(rank, dtype=np.uint32)
+        _check(lib().vq_table_dict_order(self.h, col, r.ctypes.data_as(C.POINTER(C.c_uint32))))
 
     def zones(self):
         n = self.nrg * len(self.table.columns)

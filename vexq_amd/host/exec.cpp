@@ -35,7 +35,7 @@ vq_agg to_vq(const HostTable& t, const AggExpr& a, const HostTable* build = null
     case AggKind::Min: g.kind = VQ_AGG_MIN; break;
     case AggKind::Max: g.kind = VQ_AGG_MAX; break;
   }
-  g.accum = a.accum_type == DataType::Float64 ? VQ_F64 : VQ_I64;
+  g.accum = a.accum_type == DataType::Float64 ? VQ_F64 : a.accum_type == DataType::String ? VQ_STR : VQ_I64;
   // COUNT(col) of a column with NULLs counts the rows holding a value
   // (exec/aggregate.go:466-473); over any other column it is COUNT(*)
   const HostTable& src = a.build_side && build ? *build : t;
@@ -63,7 +63,8 @@ DataType out_type(const HostTable& t, const AggExpr& a, const HostTable* build =
 Batch agg_batch(const Schema& schema, const vq_agg_result& r, int nkeys, int naggs,
                 const std::vector<std::shared_ptr<const std::vector<std::string>>>& key_dicts,
                 const std::vector<DataType>& key_types, const std::vector<DataType>& agg_types,
-                int64_t lo, int64_t hi) {
+                int64_t lo, int64_t hi,
+                const std::vector<std::shared_ptr<const std::vector<std::string>>>* agg_dicts = nullptr) {
   Batch b;
   b.schema = schema;
   b.length = (int)(hi - lo);
@@ -94,9 +95,12 @@ Batch agg_batch(const Schema& schema, const vq_agg_result& r, int nkeys, int nag
     for (int64_t i = lo; i < hi; i++) {
       const int64_t o = i * naggs + j;
       if (v.type == DataType::Float64) v.f64.push_back(r.fvals[o]);
+      else if (v.type == DataType::String) v.codes.push_back(r.valid[o] ? (uint32_t)r.ivals[o] : 0u);
       else v.i64.push_back(r.ivals[o]);
       v.valid.push_back(r.valid[o]);
     }
+    // MIN/MAX over strings: codes of the source column's dictionary
+    if (v.type == DataType::String && agg_dicts) v.dict = (*agg_dicts)[j];
     b.vectors.push_back(std::move(v));
   }
   return b;
@@ -225,6 +229,7 @@ class GpuParallelHashAggregate : public Operator {
       const DataType ty = out_type(h, a);
       schema_.fields.push_back({a.out_name, ty, a.kind != AggKind::Count});
       agg_types_.push_back(ty);
+      agg_dicts_.push_back(ty == DataType::String && a.col_idx >= 0 ? h.columns[a.col_idx].dict : nullptr);
     }
   }
 
@@ -239,7 +244,13 @@ class GpuParallelHashAggregate : public Operator {
         cap *= std::max<int64_t>(1, (h.columns[c].dict ? (int64_t)h.columns[c].dict->size() : 1) +
                                         (h.columns[c].valid ? 1 : 0));
       std::vector<vq_agg> va;
-      for (const AggExpr& a : aggs_) va.push_back(to_vq(h, a));
+      for (const AggExpr& a : aggs_) {
+        va.push_back(to_vq(h, a));
+        if (va.back().accum == VQ_STR && a.col_idx >= 0) {
+          Status st = t_->EnsureDictOrder(a.col_idx);
+          if (!st.ok()) return st;
+        }
+      }
       ResultBufs rb;
       rb.alloc(cap, (int)group_by_.size(), (int)aggs_.size());
       int rc = vq_agg_dict(t_->device()->ctx(), t_->handle(), has_pred_ ? &pred_ : nullptr,
@@ -249,7 +260,7 @@ class GpuParallelHashAggregate : public Operator {
       for (int64_t lo = 0; lo < rb.r.ngroups; lo += kBlockRows)
         batches_.push_back(agg_batch(schema_, rb.r, (int)group_by_.size(), (int)aggs_.size(),
                                      key_dicts_, key_types_, agg_types_, lo,
-                                     std::min<int64_t>(rb.r.ngroups, lo + kBlockRows)));
+                                     std::min<int64_t>(rb.r.ngroups, lo + kBlockRows), &agg_dicts_));
       done_ = true;
     }
     if (pos_ < batches_.size()) *out = &batches_[pos_++];
@@ -265,7 +276,7 @@ class GpuParallelHashAggregate : public Operator {
   std::vector<int> group_by_;
   std::vector<AggExpr> aggs_;
   Schema schema_;
-  std::vector<std::shared_ptr<const std::vector<std::string>>> key_dicts_;
+  std::vector<std::shared_ptr<const std::vector<std::string>>> key_dicts_, agg_dicts_;
   std::vector<DataType> key_types_, agg_types_;
   bool done_ = false;
   std::vector<Batch> batches_;
@@ -338,6 +349,26 @@ Status GpuTable::Upload(Device* dev, const HostTable& host, std::unique_ptr<GpuT
 }
 
 GpuTable::~GpuTable() { vq_table_destroy(t_); }
+
+Status GpuTable::EnsureDictOrder(int col) {
+  if (ordered_.size() < host_.columns.size()) ordered_.resize(host_.columns.size(), false);
+  if (col < 0 || col >= (int)host_.columns.size() || !host_.columns[col].dict) {
+    Status s;
+    s.code = VQ_E_ARG;
+    s.msg = "exec: gpu: MIN/MAX over strings of a column without a dictionary";
+    return s;
+  }
+  if (ordered_[col]) return Status::Ok();
+  const std::vector<std::string>& d = *host_.columns[col].dict;
+  std::vector<uint32_t> idx(d.size()), rank(d.size());
+  for (size_t k = 0; k < d.size(); k++) idx[k] = (uint32_t)k;
+  // std::string compares bytes as unsigned, as Go compares strings
+  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return d[a] < d[b]; });
+  for (size_t k = 0; k < idx.size(); k++) rank[idx[k]] = (uint32_t)k;
+  if (int rc = vq_table_dict_order(t_, col, rank.data())) return Status::FromLib(rc, "string order");
+  ordered_[col] = true;
+  return Status::Ok();
+}
 
 std::unique_ptr<Operator> NewGpuFilterScan(GpuTable* table, std::vector<int> cols, const vq_pred* pred) {
   return std::unique_ptr<Operator>(new GpuFilterScan(table, std::move(cols), pred));

@@ -138,11 +138,15 @@ class GpuTable {
   const HostTable& host() const { return host_; }
   vq_table* handle() const { return t_; }
   Device* device() const { return dev_; }
+  // The byte order of dictionary column col's strings handed to the device
+  // once (MIN/MAX over strings fold ranks in that order).
+  Status EnsureDictOrder(int col);
 
  private:
   Device* dev_ = nullptr;
   HostTable host_;
   vq_table* t_ = nullptr;
+  std::vector<bool> ordered_;
 };
 
 // ---- operators --------------------------------------------------------------
