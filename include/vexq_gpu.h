@@ -346,7 +346,12 @@ int vq_selvec_destroy(vq_selvec* sv);
  * path packs them (32 bits per column for up to 2 columns, 16 for up to 4, 8
  * for up to 8; exec/agg_intkey.go:67-100) and runs through vq_agg_hash's
  * kernels, float sums then exact; codes wider than their packed width return
- * VQ_E_UNSUPPORTED (the reference's string-key path). */
+ * VQ_E_UNSUPPORTED (the reference's string-key path).  A key column may also
+ * be an integer column (VQ_U8/VQ_U16 without a dictionary, VQ_DATE,
+ * VQ_I64): a composite key (the string-key path, aggregate.go:409-557) packs
+ * its values as they are when its zone maps put them in [0, 2^width), its
+ * result keys being those values; one integer key alone has any values (the
+ * hash GROUP BY).  Composite keys with NULLs, or outside the width: VQ_E_UNSUPPORTED. */
 int vq_agg_dict(vq_ctx* ctx, const vq_table* t, const vq_pred* pred, int32_t nkeys,
                 const int32_t* key_cols, int32_t naggs, const vq_agg* aggs,
                 vq_agg_result* out, vq_stats* stats);

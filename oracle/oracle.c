@@ -860,13 +860,24 @@ static void unpack_key(int64_t packed, int32_t nkeys, int64_t* key) {
 
 /* Every code of a key column fits its packed width (the reference leaves the
  * int-key path for the string-key one otherwise, agg_intkey.go:80-100; this
- * restatement then refuses the query). */
-static int keys_fit(const vq_col_desc* cols, int32_t nkeys, const int32_t* key_cols) {
+ * restatement then refuses the query).  An integer key column (no dictionary:
+ * a composite key of the string-key path, aggregate.go:409-557) packs its
+ * values as they are: every one in [0, 2^width), and no NULL (whose code
+ * would collide with a value). */
+static int keys_fit(int64_t nrows, const vq_col_desc* cols, int32_t nkeys, const int32_t* key_cols) {
   const int w = key_width(nkeys);
   for (int i = 0; i < nkeys; i++) {
     const vq_col_desc* c = &cols[key_cols[i]];
-    const int64_t codes = (int64_t)c->dict_size + (c->valid ? 1 : 0);
-    if (c->dict_size <= 0 || (w < 32 && codes > (1 << w))) return 0;
+    if (c->dict_size > 0) {
+      const int64_t codes = (int64_t)c->dict_size + (c->valid ? 1 : 0);
+      if (w < 32 && codes > (1 << w)) return 0;
+      continue;
+    }
+    if (c->type == VQ_F64 || c->valid) return 0;
+    for (int64_t r = 0; r < nrows; r++) {
+      const int64_t v = col_i(c, r);
+      if (v < 0 || (uint64_t)v >= (1ULL << w)) return 0;
+    }
   }
   return 1;
 }
@@ -885,7 +896,7 @@ int or_agg_dict(int64_t nrows, int32_t ncols, const vq_col_desc* cols, int32_t n
                 int32_t exact, vq_agg_result* out) {
   (void)nrows;
   if (nkeys < 0 || nkeys > 8 || naggs < 0 || naggs > VQ_MAX_AGGS) return VQ_E_ARG;
-  if (!keys_fit(cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
+  if (!keys_fit(nrows, cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
   uint8_t* keep = (uint8_t*)malloc((size_t)nrg + 1);
   or_zone_prune(ncols, cols, nrg, zones, pred, keep);
   kmap m;
@@ -1184,7 +1195,7 @@ int or_join_group(int64_t b_nrows, int32_t b_ncols, const vq_col_desc* b_cols, i
                   const int32_t* key_cols, int32_t naggs, const vq_agg* aggs, int32_t exact,
                   vq_agg_result* out) {
   if (naggs < 0 || naggs > VQ_MAX_AGGS || nkeys < 1 || nkeys > 8) return VQ_E_ARG;
-  if (nkeys > 1 && !keys_fit(p_cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
+  if (nkeys > 1 && !keys_fit(p_nrows, p_cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
   /* the aggregates over the joined column space */
   vq_agg ja[VQ_MAX_AGGS];
   for (int j = 0; j < naggs; j++) {
@@ -1474,7 +1485,7 @@ int or_agg_dict_parallel(int64_t nrows, int32_t ncols, const vq_col_desc* cols, 
                          const vq_agg* aggs, int32_t nthreads, vq_agg_result* out) {
   (void)nrows;
   if (nkeys < 0 || nkeys > 8 || naggs < 0 || naggs > VQ_MAX_AGGS) return VQ_E_ARG;
-  if (!keys_fit(cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
+  if (!keys_fit(nrows, cols, nkeys, key_cols)) return VQ_E_UNSUPPORTED;
   return agg_parallel(ncols, cols, nrg, rg_rows, zones, pred, nkeys, key_cols, -1, naggs, aggs,
                       nthreads, out);
 }

@@ -181,6 +181,31 @@ bool StringMinMax() {
   return true;
 }
 
+// A composite key of a STRING and an INT64 column (the string-key path,
+// exec/aggregate.go:409-557): groups in first-occurrence order.
+bool CompositeKey() {
+  static const uint8_t flag[] = {0, 1, 0, 1, 1};
+  static const int64_t year[] = {2020, 2020, 2021, 2020, 2021};
+  static const int64_t value[] = {1, 2, 4, 8, 16};
+  HostTable h;
+  h.columns = {col("flag", DataType::String, VQ_U8, flag, dict({"R", "A"})),
+               col("year", DataType::Int64, VQ_I64, year), col("value", DataType::Int64, VQ_I64, value)};
+  h.row_groups = {5};
+  h.nrows = 5;
+  std::unique_ptr<GpuTable> t;
+  CHECK_OK(GpuTable::Upload(g_dev.get(), h, &t));
+  auto op = NewGpuParallelHashAggregate(t.get(), nullptr, {0, 1}, {agg(AggKind::Sum, 2, DataType::Int64, "s")});
+  std::vector<Batch> bs;
+  CHECK_OK(Drain(op.get(), &bs));
+  CHECK(bs.size() == 1 && bs[0].length == 4);
+  const Batch& b = bs[0];
+  CHECK(b.vectors[0].Get(0) == "R" && b.vectors[1].i64[0] == 2020 && b.vectors[2].i64[0] == 1);
+  CHECK(b.vectors[0].Get(1) == "A" && b.vectors[1].i64[1] == 2020 && b.vectors[2].i64[1] == 10);
+  CHECK(b.vectors[0].Get(2) == "R" && b.vectors[1].i64[2] == 2021 && b.vectors[2].i64[2] == 4);
+  CHECK(b.vectors[0].Get(3) == "A" && b.vectors[1].i64[3] == 2021 && b.vectors[2].i64[3] == 16);
+  return true;
+}
+
 // TestAccumulateSelVecFourPaths (exec/aggregate_rows_test.go:249-322); the
 // selection vectors {0,2,4,6} / {7} become the predicate sel == 1.
 bool AccumulateSelVecFourPaths() {
@@ -485,14 +510,15 @@ bool ClosedFormSum() {
 // Errors carry the exec: prefix and a library status; nothing runs.
 bool ErrorsAreReported() {
   static const int64_t x[] = {1};
+  static const double f[] = {1.5};
   HostTable h;
-  h.columns = {col("x", DataType::Int64, VQ_I64, x)};
+  h.columns = {col("x", DataType::Int64, VQ_I64, x), col("f", DataType::Float64, VQ_F64, f)};
   h.row_groups = {1};
   h.nrows = 1;
   std::unique_ptr<GpuTable> t;
   CHECK_OK(GpuTable::Upload(g_dev.get(), h, &t));
-  // group by a non-dictionary column: unsupported on the dense path
-  auto op = NewGpuParallelHashAggregate(t.get(), nullptr, {0}, {agg(AggKind::Count, -1, DataType::Int64, "c")});
+  // group by a float64 column: unsupported
+  auto op = NewGpuParallelHashAggregate(t.get(), nullptr, {1}, {agg(AggKind::Count, -1, DataType::Int64, "c")});
   const Batch* b = nullptr;
   Status s = op->Next(&b);
   CHECK(!s.ok() && s.code == VQ_E_UNSUPPORTED);
@@ -623,6 +649,7 @@ int main() {
   run("TestIntKeyMultiRowGroupDictInstability", IntKeyMultiRowGroupDictInstability);
   run("TestNullKeysAndArguments", NullKeysAndArguments);
   run("TestStringMinMax", StringMinMax);
+  run("TestCompositeKey", CompositeKey);
   run("TestAccumulateSelVecFourPaths", AccumulateSelVecFourPaths);
   run("TestMergePartialAggNonNullCounts", MergePartialAggNonNullCounts);
   run("TestEmptyGlobalAggregate", EmptyGlobalAggregate);

@@ -1052,8 +1052,12 @@ int plan_dense(const vq_table* t, const vq_pred* pred, int32_t nkeys, const int3
   for (int i = 0; i < nkeys; i++) {
     if (int rc = check_col(t, key_cols[i])) return rc;
     const ColDev& c = t->cols[key_cols[i]];
-    if ((c.type != VQ_U8 && c.type != VQ_U16) || c.dict_size <= 0)
+    if ((c.type != VQ_U8 && c.type != VQ_U16) || c.dict_size <= 0) {
+      // an integer key column (a composite key with codes, or one integer
+      // key): packed into one key by agg_packed
+      if (over && c.type != VQ_F64) *over = true;
       return fail(VQ_E_UNSUPPORTED, "exec: gpu: group-by column %d is not a dictionary code", key_cols[i]);
+    }
     // a NULL key is a group of its own (exec/aggregate.go:424-427): the code
     // past the dictionary
     card[i] = c.dict_size + (c.nul ? 1 : 0);
@@ -2997,11 +3001,24 @@ int agg_packed(vq_ctx* c, const vq_table* t, const vq_pred* pred, int32_t nkeys,
   for (int i = 0; i < nkeys; i++) {
     if (int rc = check_col(t, key_cols[i])) return rc;
     const ColDev& cd = t->cols[key_cols[i]];
-    if ((cd.type != VQ_U8 && cd.type != VQ_U16) || cd.dict_size <= 0)
-      return fail(VQ_E_UNSUPPORTED, "exec: gpu: group-by column %d is not a dictionary code", key_cols[i]);
-    if (w < 32 && cd.dict_size > (1 << w))
-      return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d codes of column %d exceed the %d-bit packed key", cd.dict_size,
-                  key_cols[i], w);
+    if (cd.type == VQ_F64) return fail(VQ_E_UNSUPPORTED, "exec: gpu: float64 group-by column %d", key_cols[i]);
+    if ((cd.type == VQ_U8 || cd.type == VQ_U16) && cd.dict_size > 0) {
+      if (w < 32 && cd.dict_size > (1 << w))
+        return fail(VQ_E_UNSUPPORTED, "exec: gpu: %d codes of column %d exceed the %d-bit packed key", cd.dict_size,
+                    key_cols[i], w);
+      continue;
+    }
+    if (nkeys == 1) continue;  // one integer key: the hash GROUP BY's own key
+    // an integer column of a composite key (the reference's string-key path,
+    // exec/aggregate.go:409-557) packs its values as they are: every value
+    // must lie in [0, 2^w) (its zone maps bound them)
+    const vq_zone z = merged_zone(t, key_cols[i]);
+    if (!z.has_minmax) continue;  // (no value: nothing to pack)
+    auto as_i64 = [&](uint64_t b) { return cd.type == VQ_DATE ? (int64_t)(int32_t)(uint32_t)b : (int64_t)b; };
+    const int64_t lo = as_i64(z.min_bits), hi = as_i64(z.max_bits);
+    if (lo < 0 || (uint64_t)hi >= (1ULL << w))
+      return fail(VQ_E_UNSUPPORTED, "exec: gpu: values of column %d in [%lld, %lld] outside the %d-bit packed key",
+                  key_cols[i], (long long)lo, (long long)hi, w);
   }
   std::unique_ptr<JoinJob> j(new (std::nothrow) JoinJob());
   if (!j) return fail(VQ_E_NOMEM, "exec: gpu: GROUP BY job allocation");

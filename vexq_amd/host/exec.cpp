@@ -239,10 +239,14 @@ class GpuParallelHashAggregate : public Operator {
     *out = nullptr;
     if (!done_) {
       const HostTable& h = t_->host();
+      // at most one group per code tuple (a key with NULLs: one group more,
+      // the NULL one), and per row (an integer key column: any values)
       int64_t cap = 1;
-      for (int c : group_by_)  // (a key with NULLs: one group more, the NULL one)
-        cap *= std::max<int64_t>(1, (h.columns[c].dict ? (int64_t)h.columns[c].dict->size() : 1) +
-                                        (h.columns[c].valid ? 1 : 0));
+      for (int c : group_by_)
+        cap = std::min<int64_t>(cap * std::max<int64_t>(1, h.columns[c].dict ? (int64_t)h.columns[c].dict->size() +
+                                                                                  (h.columns[c].valid ? 1 : 0)
+                                                                            : h.nrows),
+                                std::max<int64_t>(1, h.nrows));
       std::vector<vq_agg> va;
       for (const AggExpr& a : aggs_) {
         va.push_back(to_vq(h, a));
