@@ -628,7 +628,11 @@ def main():
             runs = time_cpu(lambda: cpu_mod.join_agg(b, 0, bp, p, 0, pp, jaggs, topk=10,
                                                      nthreads=cores))
             cpu = cpu_record(runs, r.stats.bytes_alg, cores, f"the full join query, {len(runs)} runs")
-        rec = operator_record("join_agg_top10", "join pipeline (partition, build+probe, top-k)", r,
+        # (build keys 1 + 4i: a dense range, so the direct join runs; its
+        # pipeline is k_dj_mark, k_dj_pack, vq_dj_probe, vq_dj_match, k_dj_collect)
+        path = ("direct join pipeline (mark, pack, probe, match, collect + top-k)" if r.stats.partitions == 0
+                else "join pipeline (partition, build+probe, top-k)")
+        rec = operator_record("join_agg_top10", path, r,
                               total, walls, mains, devs, K, traffic, cpu, b.nrows + p.nrows,
                               whole_pipeline=True)
         rec["radix_bits"] = int(r.stats.radix_bits)
