@@ -271,7 +271,7 @@ def serve_bench(ctx, scale, nq, slots=4, cpu_mod=None, cores=1):
     levels = []
     pool = cf.ThreadPoolExecutor(max_workers=64)
     rng = np.random.default_rng(11)
-    for mult in (0.5, 1.0, 1.5, 2.0, 3.0):
+    for mult in (0.15, 0.25, 0.35, 0.5, 1.0, 1.5, 2.0, 3.0):
         rate = mult * seq_qps
         gaps = rng.exponential(1.0 / rate, len(stream))
         arrivals, futs = [], []
@@ -460,6 +460,7 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="fraction of the served stream's 60M/15M-row tables")
     ap.add_argument("--serve-queries", type=int, default=240)
+    ap.add_argument("--serve-slots", type=int, default=4, help="concurrent query slots of the served stream")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-max-rows", type=int, default=4_000_000)
     args = ap.parse_args()
@@ -724,7 +725,7 @@ def main():
                                          cpu_mod=cpu_mod, cores=cores)
         log(f"[bench] batch: {ops['batch_mixed']}")
     if want("serve_mixed"):
-        ops["serve_mixed"] = serve_bench(ctx, args.serve_scale, args.serve_queries,
+        ops["serve_mixed"] = serve_bench(ctx, args.serve_scale, args.serve_queries, slots=args.serve_slots,
                                          cpu_mod=cpu_mod, cores=cores)
         if "dict_groupby_60m" in ops["serve_mixed"]:
             ops["dict_groupby_60m"] = ops["serve_mixed"].pop("dict_groupby_60m")
