@@ -1305,7 +1305,7 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: NULLs without the run-time compiled kernel: %s", jerr.c_str());
   {
     const int64_t kr = kept_rows(t, keep);
-    if (jit && kr >= (int64_t)1 << 20) {
+    if (jit && kr >= (int64_t)1 << 17) {
       const int64_t chunk = dense_chunk_rows(d.nslots);
       const int64_t share = (kr + dj.slots - 1) / dj.slots;
       d.tile_rows = (int32_t)std::max<int64_t>(1024, (share + chunk - 1) / chunk * chunk);

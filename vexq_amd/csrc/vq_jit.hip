@@ -621,6 +621,11 @@ int jit_dense_prepare(const QDesc& d, DenseJit* out, std::string* err) {
   if (per_cu <= 0) {
     hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->dense, 64, lds);
     if (per_cu <= 0) per_cu = 1;
+    // one wave per SIMD: the prefetching waves already keep each SIMD's
+    // loads in flight; more waves add per-wave set-up and partials (measured
+    // on the 6M- and 60M-row GROUP BYs: 4 waves a CU 36.3 / 376 us, 3 or 5
+    // 38.7 / 380+, 8 38.7 / 384)
+    per_cu = std::min(per_cu, 4);
     m->dense_per_cu.store(per_cu);
   }
   if (const char* e = getenv("VQ_WAVES_PER_CU")) {
