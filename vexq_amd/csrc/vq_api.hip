@@ -1305,7 +1305,7 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
     return fail(VQ_E_UNSUPPORTED, "exec: gpu: NULLs without the run-time compiled kernel: %s", jerr.c_str());
   {
     const int64_t kr = kept_rows(t, keep);
-    if (jit && kr >= (int64_t)1 << 17) {
+    if (jit && kr >= (int64_t)1 << 20) {
       const int64_t chunk = dense_chunk_rows(d.nslots);
       const int64_t share = (kr + dj.slots - 1) / dj.slots;
       d.tile_rows = (int32_t)std::max<int64_t>(1024, (share + chunk - 1) / chunk * chunk);
@@ -1321,7 +1321,7 @@ extern "C" int vq_agg_dict(vq_ctx* c, const vq_table* t, const vq_pred* pred, in
   hipSetDevice(c->device);
   const int NI = d.nacc + 2;
   int32_t grid = 0;
-  if (d.ntiles > 0) grid = jit ? (int32_t)std::min<int64_t>(d.ntiles, dj.slots) : dense_grid(d);
+  if (d.ntiles > 0) grid = jit ? (int32_t)std::min<int64_t>(d.ntiles, dj.cap(kept)) : dense_grid(d);
   const size_t part_words = (size_t)std::max<int32_t>(grid, 1) * d.ngroups * NI;
   const size_t out_words = (size_t)d.ngroups * NI;
   void* ar;
@@ -1444,7 +1444,10 @@ extern "C" int vq_agg_dict_multi(vq_ctx* c, const vq_table* t, int32_t n, const 
   // kept row groups a query's words are the same bits alone and in the pass
   QDesc d = dq[0];
   DenseJit d1;
-  if (jit_dense_prepare(d, &d1, &jerr)) d1.slots = dj.slots;
+  if (jit_dense_prepare(d, &d1, &jerr)) {
+    d1.slots = dj.slots;
+    d1.slots_max = dj.slots_max;
+  }
   {
     const int64_t kr = kept_rows(t, keep);
     if (kr >= (int64_t)1 << 20) {
@@ -1466,7 +1469,7 @@ extern "C" int vq_agg_dict_multi(vq_ctx* c, const vq_table* t, int32_t n, const 
   if (canceled()) return fail_canceled();
   hipSetDevice(c->device);
   const int NI = d.nacc + 2;
-  const int32_t grid = d.ntiles > 0 ? (int32_t)std::min<int64_t>(d.ntiles, d1.slots) : 0;
+  const int32_t grid = d.ntiles > 0 ? (int32_t)std::min<int64_t>(d.ntiles, d1.cap(kept)) : 0;
   const size_t part_words = (size_t)std::max<int32_t>(grid, 1) * nq * NI;
   const size_t out_words = (size_t)nq * NI;
   const size_t lit_words = (size_t)nq * ntw;

@@ -236,7 +236,11 @@ hipError_t launch_filter(const QDesc& d, bool jit, uint64_t* bits, uint32_t* til
 struct DenseJit {
   hipFunction_t fn = nullptr;
   size_t lds = 0;
-  int slots = 0;
+  int slots = 0;      // the waves a large query runs as (one per SIMD)
+  int slots_max = 0;  // the most the kernel's occupancy allows (small queries)
+  // the grid's cap for a query of kr kept rows: below 2^20 rows, short waves
+  // and as many as fit (their ramp and tail dominate)
+  int cap(int64_t kr) const { return kr >= ((int64_t)1 << 20) ? slots : slots_max; }
 };
 int jit_dense_prepare(const QDesc& d, DenseJit* out, std::string* err);
 hipError_t jit_dense_launch(const DenseJit& h, const QDesc& d, int32_t grid, uint64_t* part,

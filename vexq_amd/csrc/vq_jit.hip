@@ -617,17 +617,17 @@ int jit_dense_prepare(const QDesc& d, DenseJit* out, std::string* err) {
   const size_t lds = dense_lds_bytes(d);
   // the waves per CU the kernel's occupancy allows (once per module: its
   // shape fixes its LDS)
-  int per_cu = m->dense_per_cu.load();
-  if (per_cu <= 0) {
-    hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->dense, 64, lds);
-    if (per_cu <= 0) per_cu = 1;
-    // one wave per SIMD: the prefetching waves already keep each SIMD's
-    // loads in flight; more waves add per-wave set-up and partials (measured
-    // on the 6M- and 60M-row GROUP BYs: 4 waves a CU 36.3 / 376 us, 3 or 5
-    // 38.7 / 380+, 8 38.7 / 384)
-    per_cu = std::min(per_cu, 4);
-    m->dense_per_cu.store(per_cu);
+  int occ = m->dense_per_cu.load();
+  if (occ <= 0) {
+    hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, m->dense, 64, lds);
+    if (occ <= 0) occ = 1;
+    m->dense_per_cu.store(occ);
   }
+  // a large query: one wave per SIMD; the prefetching waves already keep
+  // each SIMD's loads in flight, more add per-wave set-up and partials
+  // (measured on the 6M- and 60M-row GROUP BYs: 4 waves a CU 36.3 / 376 us,
+  // 3 or 5 38.7 / 380+, 8 38.7 / 384)
+  int per_cu = std::min(occ, 4);
   if (const char* e = getenv("VQ_WAVES_PER_CU")) {
     const int v = atoi(e);
     if (v > 0) per_cu = v;
@@ -635,6 +635,7 @@ int jit_dense_prepare(const QDesc& d, DenseJit* out, std::string* err) {
   out->fn = m->dense;
   out->lds = lds;
   out->slots = device_cus() * per_cu;
+  out->slots_max = device_cus() * std::max(occ, per_cu);
   return 0;
 }
 
