@@ -2304,7 +2304,7 @@ int join_plan_local(JoinJob& j) {
 }
 
 int scan_desc(const vq_table* t, const vq_pred* pred, int nkeys, const int32_t* keys, QDesc& d, int* key_slots,
-              int64_t* rows, int64_t* kept);
+              int64_t* rows, int64_t* kept, int32_t tile_rows = kLookupTileRows);
 
 // The direct join: when the build key's zone maps bound it to a range of at
 // most kDirectDensity times the build rows (primary keys, surrogate keys)
@@ -2363,6 +2363,8 @@ int join_plan_direct(JoinJob& j) {
     return rc;
   const int64_t cap = std::max<int64_t>(1, j.ps.ntiles * (int64_t)j.ps.tile_rows);
   if (cap >= (int64_t)UINT32_MAX) return kRetry;
+  // (256-row tiles: one step of a wave each; 1024 took 18.1 us over the
+  // 1.5M-row build against 16.0, 4096 45.9)
   if (int rc = scan_desc(j.build, j.bp, 1, &j.build_key, j.dq, &j.dkey_slot, &j.brows, &j.bkept)) return rc;
   if (jit_direct_prepare(j.ps, j.jd, &j.dl, &jerr)) return kRetry;
   j.dbase = lo;
@@ -3193,7 +3195,7 @@ namespace {
 // given key columns' (slots returned in key_slots), over the row groups its
 // predicate keeps.
 int scan_desc(const vq_table* t, const vq_pred* pred, int nkeys, const int32_t* keys, QDesc& d, int* key_slots,
-              int64_t* rows, int64_t* kept) {
+              int64_t* rows, int64_t* kept, int32_t tile_rows) {
   memset(&d, 0, sizeof d);
   SlotMap sm;
   if (int rc = add_pred(t, pred, sm, d.slot)) return rc;
@@ -3208,7 +3210,7 @@ int scan_desc(const vq_table* t, const vq_pred* pred, int nkeys, const int32_t* 
   d.ngroups = 1;
   std::vector<uint8_t> keep;
   prune(t, pred, keep);
-  d.tile_rows = kLookupTileRows;  // (lookups and inserts: latency-bound, many waves)
+  d.tile_rows = tile_rows;  // (lookups and inserts: latency-bound, many waves)
   return make_spans(t, keep, d.tile_rows, d.span, &d.nspans, &d.ntiles, rows, kept);
 }
 
