@@ -1319,12 +1319,37 @@ namespace vqj {
 constexpr int kLT = 512;  // threads per workgroup
 constexpr u32 kLocalOverflow = 4u;
 
+// The slots the predicate reads.  A join + GROUP BY of the joined rows
+// (VQ_LOOKUP) loads only these for every probe row and the others (join
+// key, group columns, aggregate arguments) for the rows that pass: a join's
+// probe predicate keeps few rows (Q12's about 1 %), so most of those
+// columns' lines are never fetched.
+__host__ __device__ constexpr u32 lpred_slots() {
+  u32 m = 0;
+  for (int t = 0; t < VQ_NT; t++) {
+    m |= 1u << kTS[t];
+    if (kTO[t] >= 16) m |= 1u << kTS2[t];
+  }
+  return m;
+}
+constexpr bool kLateLocal = VQ_LOOKUP != 0 && VQ_NT > 0;
+template <int C>
+__host__ __device__ constexpr bool late_slot() {
+  return kLateLocal && !kSB[C] && ((lpred_slots() >> C) & 1u) == 0;
+}
+
 // A probe-side slot's value of row r (VQ_LOOKUP: a build-side slot is
-// loaded for each matching build row instead).
+// loaded for each matching build row instead; a late slot for a passing row).
 template <int C, int U>
 __device__ __forceinline__ void lload_slot(const Args& a, i64 r, i64 (&raw)[VQ_NS][U], int u) {
-  if constexpr (kSB[C]) raw[C][u] = 0;
+  if constexpr (kSB[C] || late_slot<C>()) raw[C][u] = 0;
   else raw[C][u] = ld<kST[C]>(a.col[C], r);
+}
+
+// The late slots of a passing row r.
+template <int... C>
+__device__ __forceinline__ void llate_load(const Args& a, i64 r, i64* row, Seq<C...>) {
+  ((late_slot<C>() ? (void)(row[C] = ld<kST[C]>(a.col[C], r)) : (void)0), ...);
 }
 
 template <int U, int... C>
@@ -1532,6 +1557,7 @@ extern "C" __global__ __launch_bounds__(512) void vq_hash_local(const Args a) {
       for (int q = 0; q < VQ_NS; q++) row[q] = cur[q][u];
       const i64 rr = cbase + u * kLT + tid;
       if (!(rr < cend && pass_all(a, row, seq<VQ_NT>{}))) continue;
+      if constexpr (kLateLocal) llate_load(a, rr, row, seq<VQ_NS>{});
       const i64 key = row_key(row);
       u32 g = 0xFFFFFFFFu;
       if (key == kI64Min) {
