@@ -1705,19 +1705,24 @@ extern "C" int vq_filter(vq_ctx* c, const vq_table* t, const vq_pred* pred, vq_s
     vq_selvec_destroy(sv);
     return rc;
   }
-  HIP_TRY(hipEventRecord(S->ev[0], S->stream));
+  // events around the filter when stats are asked for, and around the bits
+  // kernel only when it is timed on its own (repeats above 1): an event
+  // between two launches delays the second by microseconds
+  const bool timed = stats != nullptr, split = timed && d.ntiles > 0 && c->kernel_repeats > 1;
+  if (timed) HIP_TRY(hipEventRecord(S->ev[0], S->stream));
   const bool jit = d.ntiles > 0 && jit_available_filter(d, nullptr);
-  HIP_TRY(launch_filter(d, jit, bits, tcount, toff, sv->rows, dcount, S->stream, S->ev[1], S->ev[2],
-                        c->kernel_repeats));
-  HIP_TRY(hipEventRecord(S->ev[3], S->stream));
+  HIP_TRY(launch_filter(d, jit, bits, tcount, toff, sv->rows, dcount, S->stream, split ? S->ev[1] : nullptr,
+                        split ? S->ev[2] : nullptr, c->kernel_repeats));
+  if (timed) HIP_TRY(hipEventRecord(S->ev[3], S->stream));
   HIP_TRY(hipMemcpyAsync(hp, dcount, 8, hipMemcpyDeviceToHost, S->stream));
   HIP_TRY(hipStreamSynchronize(S->stream));
   sv->count = (int64_t)*(uint64_t*)hp;
   if (stats) {
     memset(stats, 0, sizeof *stats);
-    stats->main_kernel_ms = d.ntiles ? elapsed(S->ev[1], S->ev[2]) / c->kernel_repeats : 0;
-    stats->device_ms = elapsed(S->ev[0], S->ev[3]) -
-                       (d.ntiles ? (c->kernel_repeats - 1) * stats->main_kernel_ms : 0);
+    const double whole = elapsed(S->ev[0], S->ev[3]);
+    // (not timed on its own: the bits kernel's share is not separated)
+    stats->main_kernel_ms = split ? elapsed(S->ev[1], S->ev[2]) / c->kernel_repeats : d.ntiles ? whole : 0;
+    stats->device_ms = whole - (split ? (c->kernel_repeats - 1) * stats->main_kernel_ms : 0);
     stats->rows_scanned = rows;
     stats->rg_total = t->nrg;
     stats->rg_scanned = kept;
