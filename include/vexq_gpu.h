@@ -310,7 +310,8 @@ int vq_table_dict_entry(const vq_table* t, int32_t col, int32_t code, const char
  * dict_size strings sorted as Go compares strings (bytewise), a permutation
  * of 0 .. dict_size - 1.  The library keeps a column of ranks on the device
  * (2 bytes a row, under the context's budget) until the table is destroyed;
- * calling again replaces the order.  Not needed for a table from
+ * calling again replaces the order (not while a query on the table runs:
+ * the rank column is rewritten in place).  Not needed for a table from
  * vq_table_upload_vxq (its first VQ_STR query orders the dictionary it
  * carries).  No counterpart in the reference, whose strAccs compare the
  * strings themselves (exec/aggregate.go:670-730). */
