@@ -1604,11 +1604,13 @@ extern "C" int vq_agg_dict_noalloc(vq_ctx* c, const vq_table* t, const vq_pred* 
   uint64_t* part = fin + part_off;
   uint64_t* hfin = g_noalloc_host[c->device];
   if (d.ntiles > 0) {
-    HIP_TRY(hipEventRecord(S->ev[0], S->stream));
+    // (events only when stats are asked for: one between two launches
+    // delays the second by microseconds)
+    if (stats) HIP_TRY(hipEventRecord(S->ev[0], S->stream));
     HIP_TRY(launch_agg_dense(d, (int32_t)grid, part, S->stream));
-    HIP_TRY(hipEventRecord(S->ev[1], S->stream));
+    if (stats) HIP_TRY(hipEventRecord(S->ev[1], S->stream));
     HIP_TRY(launch_agg_final(d, (int32_t)grid, part, fin, S->stream));
-    HIP_TRY(hipEventRecord(S->ev[2], S->stream));
+    if (stats) HIP_TRY(hipEventRecord(S->ev[2], S->stream));
     HIP_TRY(hipMemcpyAsync(hfin, fin, fin_words * 8, hipMemcpyDeviceToHost, S->stream));
     HIP_TRY(hipStreamSynchronize(S->stream));
   } else {
